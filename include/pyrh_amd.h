@@ -1,0 +1,235 @@
+/* pyrh_amd C ABI: MI355X (gfx950) engine for pyrh's 1-D formal solution.
+ *
+ * The reference exposes one per-column entry point,
+ *     mySpectrum rhf1d(char *cwd, double mu, int Ndep, double *scale, ...)
+ * (rh/rhf1d/pyrh_compute1dray.c:112-389, declared in
+ *  rh/rhf1d/pyrh_compute1dray.h:26-36 and bound by pyrh.pyx:621-632), which
+ * re-parses every input file per call, mutates its arguments and calls
+ * exit() on errors.  This ABI splits that call into
+ *   - a model (rhg_model_t) that the host builds once per run directory and
+ *     wavelength grid from the same keyword/atom/molecule/line files, and
+ *   - batched column launches that take caller-owned buffers.
+ * All functions return 0 on success or a negative RHG_E* code; the message
+ * of the last failure on the calling thread is available from
+ * rhg_last_error().  Inputs are never modified.  A context is used by one
+ * host thread at a time and owns one HIP stream.
+ *
+ * Units of the per-column atmosphere follow pyrh.compute1d
+ * (pyrh.pyx:537-668): rows 0 scale (log tau500 | log cmass [g cm^-2] |
+ * height [km]), 1 T [K], 2 n_e [cm^-3], 3 v_LOS [km/s], 4 v_mic [km/s],
+ * 5 B [G], 6 gamma [rad], 7 chi [rad], 8 n_H,tot [cm^-3].
+ */
+#ifndef PYRH_AMD_H
+#define PYRH_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RHG_ABI_VERSION 1
+
+enum rhg_status {
+  RHG_OK = 0,
+  RHG_EINVAL = -1,      /* bad argument or inconsistent model */
+  RHG_EDEVICE = -2,     /* HIP runtime error */
+  RHG_ENOMEM = -3,      /* device allocation failed */
+  RHG_EUNSUPPORTED = -4 /* option outside what the engine implements */
+};
+
+/* Solver used for one wavelength, decided from the background flags
+ * exactly as rh/rhf1d/formal.c:78-103 does. */
+enum rhg_solver {
+  RHG_SOLVER_FEAUTRIER = 0, /* no line at this wavelength          */
+  RHG_SOLVER_BEZIER = 1,    /* unpolarized cubic Bezier (I only)    */
+  RHG_SOLVER_STOKES = 2     /* cubic DELO-Bezier, full Stokes       */
+};
+
+enum rhg_vdw { RHG_VDW_KURUCZ = 0, RHG_VDW_UNSOLD = 1, RHG_VDW_BARKLEM = 2 };
+
+/* Per-wavelength constants computed on the host (column independent). */
+enum rhg_lamconst {
+  RHG_LC_LAMBDA = 0,     /* nm (vacuum)                                   */
+  RHG_LC_HC_KLA,         /* (h c)/(k 1e-9 lambda)      [K]                */
+  RHG_LC_TWOHNU3_C2,     /* (2 h c)/(1e-9 lambda)^3                       */
+  RHG_LC_BF_HC_KLA,      /* ((h c)/(k 1e-9))/lambda    (bf routines)      */
+  RHG_LC_BF_TWOHNU3_C2,  /* ((2 h c)/1e-27)/lambda^3   (bf routines)      */
+  RHG_LC_HMIN_BF_ALPHA,  /* H- bf cross-section [m^2] (0 outside range)   */
+  RHG_LC_HMIN_BF_ON,     /* 1 if the H- bf term is present                */
+  RHG_LC_HMIN_FF_LIDX,   /* fractional lambda index, -1 = long-wave fit   */
+  RHG_LC_HFF_NU3,        /* (1e-9 lambda / c)^3                           */
+  RHG_LC_GFF_X3,         /* pow(x, 0.33333333) of the ff Gaunt factor     */
+  RHG_LC_GFF_CY,         /* (2 lambda 1e-9) k, ff Gaunt factor y = cy T/hc */
+  RHG_LC_H2P_LIDX,       /* H2+ ff fractional lambda index (-1 = off)     */
+  RHG_LC_H2M_LIDX,       /* H2- ff fractional lambda index (-1 = off)     */
+  RHG_LC_OH_EIDX,        /* OH bf fractional energy index (-1 = off)      */
+  RHG_LC_CH_EIDX,        /* CH bf fractional energy index (-1 = off)      */
+  RHG_LC_HFF_LONG0,      /* 5 coefficients of the long-wave H- ff fit ... */
+  RHG_LC_NCONST = RHG_LC_HFF_LONG0 + 5
+};
+
+/* Per-(column, depth) quantities produced by the equilibria stage and
+ * consumed by the wavelength stage (fields of the column state). */
+enum rhg_colfield {
+  RHG_CF_T = 0, RHG_CF_NE, RHG_CF_VLOS, RHG_CF_VTURB, RHG_CF_B,
+  RHG_CF_COSG, RHG_CF_COS2X, RHG_CF_SIN2X, RHG_CF_NHTOT,
+  RHG_CF_NHMIN,    /* H- number density                  */
+  RHG_CF_NH0,      /* H ground level (after chem. eq.)   */
+  RHG_CF_NP,       /* protons                            */
+  RHG_CF_NH2, RHG_CF_NOH, RHG_CF_NCH,
+  RHG_CF_HEIGHT,   /* [m], after convertScales           */
+  RHG_CF_NFIXED    /* continuum and line terms follow    */
+};
+
+typedef struct rhg_model {
+  int32_t abi_version;
+
+  /* --- switches (keyword.input, after pyrh's overrides) --------------- */
+  int32_t stokes_mode;       /* 0 NO_STOKES .. 3 FULL_STOKES          */
+  int32_t magneto_optical;   /* only 0 is implemented                 */
+  double vmacro_tresh;       /* [m/s]                                 */
+  double vmicro_char;        /* [m/s]                                 */
+  double wght_per_H;         /* amu per H nucleus                     */
+  double totalAbund;
+  double gravity;            /* [m s^-2]                              */
+
+  /* --- wavelength grid ------------------------------------------------ */
+  int32_t nspect;            /* wavelengths incl. the reference one   */
+  int32_t ref_index;         /* index of lambda_ref                   */
+  const double *lamconst;    /* [nspect][RHG_LC_NCONST]               */
+  const int32_t *solver;     /* [nspect] rhg_solver                   */
+  const int32_t *lam_line_ptr;  /* [nspect+1] CSR into lam_line_idx   */
+  const int32_t *lam_line_idx;  /* Kurucz line indices per wavelength */
+  const int32_t *lam_cont_ptr;  /* [nspect+1] CSR of bf continua      */
+  const int32_t *lam_cont_idx;  /* continuum indices per wavelength   */
+  const double *lam_cont_alpha; /* cross-section per CSR entry [m^2]  */
+
+  /* --- atoms for LTE populations (levels of all model atoms) ---------- */
+  int32_t natom;
+  const int32_t *atom_level_ptr; /* [natom+1] into level arrays        */
+  const double *level_E;         /* [nlevel] J                         */
+  const double *level_g;
+  const int32_t *level_stage;
+  const double *atom_abund;      /* [natom]                            */
+
+  /* --- bound-free continua (H first, then metals; in reference order) - */
+  int32_t ncont;
+  const int32_t *cont_atom;      /* [ncont] atom index (0 = H)          */
+  const int32_t *cont_i;         /* lower level (within atom)          */
+  const int32_t *cont_j;         /* upper level (within atom)          */
+
+  /* --- chemical equilibrium ------------------------------------------ */
+  int32_t nnuclei;               /* elements bound in molecules        */
+  const int32_t *nucl_atom;      /* [nnuclei] model atom or -1         */
+  const int32_t *nucl_elem;      /* [nnuclei] element index            */
+  const double *nucl_abund;      /* [nnuclei]                          */
+  int32_t nmol;
+  const int32_t *mol_fit;        /* [nmol]                             */
+  const int32_t *mol_charge;
+  const int32_t *mol_nnuclei;    /* total atoms in molecule            */
+  const double *mol_ediss;       /* J                                  */
+  const double *mol_tmin, *mol_tmax;
+  const int32_t *mol_neqc;       /* [nmol]                             */
+  const double *mol_eqc;         /* [nmol][8] coefficients (reversed)  */
+  const int32_t *mol_nel;        /* [nmol] constituent count           */
+  const int32_t *mol_el_nucl;    /* [nmol][4] nucleus index            */
+  const int32_t *mol_el_count;   /* [nmol][4]                          */
+  int32_t mol_h2, mol_oh, mol_ch;/* molecule indices or -1             */
+
+  /* --- element partition functions (elements used by lines) ---------- */
+  int32_t npf;
+  const double *Tpf;             /* [npf]                              */
+  int32_t nel;                   /* elements carried                   */
+  const int32_t *el_nstage;      /* [nel]                              */
+  const int32_t *el_pf_ptr;      /* [nel] offset of [nstage][npf] block */
+  const double *el_logpf;        /* natural log pf                     */
+  const double *el_ionpot;       /* [nel][nstage_max] J, at el_pf_ptr/npf */
+  const double *el_abund;        /* [nel]                              */
+  const double *el_weight;       /* [nel] amu                          */
+
+  /* --- Kurucz lines (sorted by wavelength) ---------------------------- */
+  int32_t nline;
+  const int32_t *line_el;        /* [nline] index into element arrays  */
+  const int32_t *line_stage;
+  const double *line_lambda0;    /* nm                                 */
+  const double *line_Ei;         /* J                                  */
+  const double *line_hc_la;      /* (h c)/(lambda0 1e-9)               */
+  const double *line_Bijhc4pi;   /* hc/4pi Bij x frac x gi             */
+  const double *line_twohnu3_c2; /* Aji/Bji                            */
+  const double *line_Grad, *line_GStark, *line_GvdW;
+  const int32_t *line_vdw;       /* rhg_vdw                            */
+  const double *line_cross, *line_alpha;
+  const double *line_larmor;     /* (LARMOR) lambda0                   */
+  const int32_t *line_polar;     /* polarizable flag                   */
+  const int32_t *line_comp_ptr;  /* [nline+1] Zeeman components        */
+  const int32_t *comp_q;
+  const double *comp_shift, *comp_strength;
+
+  /* --- static continuum tables ---------------------------------------- */
+  const double *hminff_theta;    /* [16]     */
+  const double *hminff_kappa;    /* [17*16]  */
+  const double *h2mff_theta;     /* [8]      */
+  const double *h2mff_kappa;     /* [19*8]   */
+  const double *h2pff_temp;      /* [10]     */
+  const double *h2pff_kappa;     /* [15*10]  */
+  const double *oh_T;            /* [15]     */
+  const double *oh_cross;        /* [130*15] */
+  const double *ch_T;            /* [15]     */
+  const double *ch_cross;        /* [105*15] */
+} rhg_model_t;
+
+/* Columns of one batch.  atmos is [ncol][9][ndep_max] in pyrh units;
+ * column c uses its first ndep[c] depth points. */
+typedef struct rhg_columns {
+  int32_t ncol;
+  int32_t ndep_max;
+  const int32_t *ndep;           /* [ncol]                             */
+  const double *atmos;           /* [ncol][9][ndep_max]                */
+  int32_t atm_scale;             /* 0 tau500, 1 column mass, 2 height  */
+  double mu;
+} rhg_columns_t;
+
+typedef struct rhg_context rhg_context_t;
+
+/* Returns the last error message on this thread ("" if none). */
+const char *rhg_last_error(void);
+
+/* Library version/ABI check. */
+int32_t rhg_abi_version(void);
+
+/* Create a context on HIP device `device`, uploading the model.  The model
+ * arrays are copied; the caller may free them afterwards.  Replaces the
+ * per-call file parsing at rh/rhf1d/pyrh_compute1dray.c:130-301. */
+int rhg_open(const rhg_model_t *model, int device, rhg_context_t **ctx);
+int rhg_close(rhg_context_t *ctx);
+
+/* LTE synthesis of a batch of columns: the work of one rhf1d() call per
+ * column with no active atom (pyrh_compute1dray.c:224-348, formal.c:44-346),
+ * emergent Stokes vectors at every wavelength of the model except the
+ * reference one.  stokes_out is [ncol][4][nspect-1] float64 (I,Q,U,V),
+ * host or device memory (device if `device_ptrs` != 0).  `precision` 0 =
+ * FP64 throughout, 1 = FP32 formal solution and line opacity with FP64
+ * equilibria.  Synchronous. */
+int rhg_synth_lte(rhg_context_t *ctx, const rhg_columns_t *cols,
+                  int32_t precision, int32_t device_ptrs, double *stokes_out);
+
+/* Same as rhg_synth_lte but also returns, per column, the depth scale and
+ * the background opacity at the reference wavelength (for testing):
+ * height_out/chiref_out are [ncol][ndep_max] float64 host buffers (NULL to
+ * skip). */
+int rhg_synth_lte_diag(rhg_context_t *ctx, const rhg_columns_t *cols,
+                       int32_t precision, double *stokes_out,
+                       double *height_out, double *chiref_out);
+
+/* Time the kernels of the last rhg_synth_lte call on the context's
+ * stream, in milliseconds (HIP events around each stage); out has
+ * RHG_NSTAGE entries: equilibria, geometry, formal solution. */
+#define RHG_NSTAGE 3
+int rhg_last_timings(rhg_context_t *ctx, double *ms_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYRH_AMD_H */

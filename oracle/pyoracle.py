@@ -1,0 +1,159 @@
+"""Python binding of the test oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke()
+as the checker, and by bench.py's CPU-baseline leg.  The product package
+never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from pyrh_amd.abi import ColumnsStruct, ModelStruct, RhgColumns, RhgModel
+
+HERE = Path(__file__).resolve().parent
+_LIB = None
+
+
+def build(quiet=True):
+    """Compile the oracle (and, where the reference sources exist, the
+    reference build under _ref/)."""
+    out = subprocess.run(["make", "-C", str(HERE), "oracle"], capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + out.stdout + out.stderr)
+    if os.path.isdir("/root/reference/rh"):
+        out = subprocess.run(["make", "-C", str(HERE), "-j8", "ref"], capture_output=True,
+                             text=True)
+        if out.returncode != 0:
+            raise RuntimeError("reference build failed:\n" + out.stdout + out.stderr)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = HERE / "liboracle.so"
+        if not path.exists():
+            build()
+        L = C.CDLL(str(path))
+        L.rho_synth_lte.argtypes = [C.POINTER(RhgModel), C.POINTER(RhgColumns), C.c_int,
+                                    C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double)]
+        L.rho_synth_lte.restype = C.c_int
+        L.rho_voigt_humlicek.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_double)]
+        L.rho_voigt_humlicek.restype = C.c_double
+        L.rho_voigt_armstrong.argtypes = [C.c_double, C.c_double]
+        L.rho_voigt_armstrong.restype = C.c_double
+        L.rho_bezier3_coeffs.argtypes = [C.c_double, C.POINTER(C.c_double)]
+        L.rho_w3.argtypes = [C.c_double, C.POINTER(C.c_double)]
+        L.rho_planck.argtypes = [C.c_double, C.c_double]
+        L.rho_planck.restype = C.c_double
+        P = C.POINTER(C.c_double)
+        L.rho_formal_stokes.argtypes = [C.c_int, P, C.c_double, P, P, P, P, C.c_int, P]
+        L.rho_formal_bezier.argtypes = [C.c_int, P, C.c_double, P, P, P, P]
+        L.rho_formal_feautrier.argtypes = [C.c_int, P, C.c_double, P, P, P]
+        L.rho_formal_feautrier.restype = C.c_double
+        L.rho_model_sizeof.restype = C.c_int
+        L.rho_columns_sizeof.restype = C.c_int
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def synth_lte(model, atmos, mu=1.0, atm_scale=0, ndep=None, inverse_fp32=False,
+              nthreads=0, diag=False):
+    """Emergent Stokes [ncol, 4, nlw] (+ height, chi_ref [ncol, ndep_max])."""
+    ms = ModelStruct(model)
+    cs = ColumnsStruct(atmos, ndep, atm_scale, mu)
+    ncol, ndmax = cs.struct.ncol, cs.struct.ndep_max
+    nout = model.scalars["nspect"] - 1
+    st = np.zeros((ncol, 4, nout))
+    h = np.zeros((ncol, ndmax)) if diag else None
+    cr = np.zeros((ncol, ndmax)) if diag else None
+    rc = lib().rho_synth_lte(ms.ptr(), cs.ptr(), int(inverse_fp32), int(nthreads),
+                             _p(st), _p(h), _p(cr))
+    if rc != 0:
+        raise RuntimeError(f"oracle failed with code {rc}")
+    return (st, h, cr) if diag else st
+
+
+def voigt_humlicek(a, v):
+    F = C.c_double()
+    H = lib().rho_voigt_humlicek(float(a), float(v), C.byref(F))
+    return H, F.value
+
+
+def voigt_armstrong(a, v):
+    return lib().rho_voigt_armstrong(float(a), float(v))
+
+
+def bezier3_coeffs(dt):
+    o = (C.c_double * 5)()
+    lib().rho_bezier3_coeffs(float(dt), o)
+    return list(o)
+
+
+def w3(dtau):
+    o = (C.c_double * 3)()
+    lib().rho_w3(float(dtau), o)
+    return list(o)
+
+
+def planck(T, lam_nm):
+    return lib().rho_planck(float(T), float(lam_nm))
+
+
+def formal_stokes(z, mu, chi, S4, chiQUV, B2=None, inverse_fp32=False):
+    n = len(z)
+    z, chi = np.ascontiguousarray(z, float), np.ascontiguousarray(chi, float)
+    S4, q = np.ascontiguousarray(S4, float), np.ascontiguousarray(chiQUV, float)
+    b = None if B2 is None else np.ascontiguousarray(B2, float)
+    I4 = np.zeros((4, n))
+    lib().rho_formal_stokes(n, _p(z), float(mu), _p(chi), _p(S4), _p(q), _p(b),
+                            int(inverse_fp32), _p(I4))
+    return I4
+
+
+def formal_bezier(z, mu, chi, S, B2=None):
+    n = len(z)
+    z, chi, S = (np.ascontiguousarray(x, float) for x in (z, chi, S))
+    b = None if B2 is None else np.ascontiguousarray(B2, float)
+    I = np.zeros(n)
+    lib().rho_formal_bezier(n, _p(z), float(mu), _p(chi), _p(S), _p(b), _p(I))
+    return I
+
+
+def formal_feautrier(z, mu, chi, S, B2=None):
+    n = len(z)
+    z, chi, S = (np.ascontiguousarray(x, float) for x in (z, chi, S))
+    b = None if B2 is None else np.ascontiguousarray(B2, float)
+    return lib().rho_formal_feautrier(n, _p(z), float(mu), _p(chi), _p(S), _p(b))
+
+
+def sizes():
+    return lib().rho_model_sizeof(), lib().rho_columns_sizeof()
+
+
+def background(model, atmos, ls, mu=1.0, atm_scale=0, ic=0):
+    """(chi_cont[N], eta_cont[N], chi_line[4,N], eta_line[4,N], pops[6,N])."""
+    ms = ModelStruct(model)
+    cs = ColumnsStruct(atmos, None, atm_scale, mu)
+    N = cs.struct.ndep_max
+    chi = np.zeros(8 * N)
+    eta = np.zeros(8 * N)
+    pop = np.zeros(6 * N)
+    L = lib()
+    L.rho_background.argtypes = [C.POINTER(RhgModel), C.POINTER(RhgColumns), C.c_int,
+                                 C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_double)]
+    rc = L.rho_background(ms.ptr(), cs.ptr(), ic, ls, _p(chi), _p(eta), _p(pop))
+    if rc:
+        raise RuntimeError("oracle background failed")
+    return chi[:N], eta[:N], chi[4 * N:].reshape(4, N), eta[4 * N:].reshape(4, N), \
+        pop.reshape(6, N)

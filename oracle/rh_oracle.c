@@ -1,0 +1,1400 @@
+/* Test oracle: CPU restatement of RH's LTE synthesis as pyrh runs it.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file restates, in plain C and FP64, the
+ * reference algorithm of rhf1d() with no active atom
+ * (rh/rhf1d/pyrh_compute1dray.c:112-389) for one column at a time, reading
+ * the host tables of rhg_model_t.  It is used as the checker of the GPU
+ * library and as the CPU baseline; the product never calls it.
+ *
+ * Floating-point operations are written in the order the reference
+ * evaluates them (operator precedence of its macros included), and libm is
+ * called for the same functions, so that on the same inputs results agree
+ * with the reference to round-off.  Every function cites the reference
+ * source it follows.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "rh_oracle.h"
+
+/* rh/constant.h:28-73, rh/rh.h:36-50 */
+#define CLIGHT      2.99792458E+08
+#define HPLANCK     6.6260755E-34
+#define KBOLTZMANN  1.380658E-23
+#define AMU         1.6605402E-27
+#define M_ELECTRON  9.1093897E-31
+#define Q_ELECTRON  1.60217733E-19
+#define EPSILON_0   8.854187817E-12
+#define EV          1.60217733E-19
+#define THETA0      5.03974756E+03
+#define NM_TO_M     1.0E-09
+#define CM_TO_M     1.0E-02
+#define KM_TO_M     1.0E+03
+#define G_TO_KG     1.0E-03
+#define ERG_TO_JOULE 1.0E-07
+#define PI          3.14159265358979
+#define SQRTPI      1.77245385090551
+#define LG10        2.30258509299404568402
+#define SQ(x)       ((x)*(x))
+#define CUBE(x)     ((x)*(x)*(x))
+#define POW10(x)    exp(LG10 * (x))
+#define MAXV(a, b)  (((a) > (b)) ? (a) : (b))
+
+#define N_MAX_CHEM_ITER 10        /* rh/background.h:18-19 */
+#define CHEM_ITER_LIMIT 1.0E-3
+#define MAX_EXPONENT    150.0     /* rh/planck.c:29 */
+#define MAX_GAUSS_DOPPLER 7.0     /* rh/kurucz.c:92 */
+
+#define NTHETA 16
+#define NFF 17
+#define NTHETA_H2 8
+#define NFF_H2 19
+#define NTEMP_H2P 10
+#define NFF_H2P 15
+#define NTOH 15
+#define NEOH 130
+#define NTCH 15
+#define NECH 105
+
+/* ------------------------------------------------------------------ */
+/* numerical utilities                                                 */
+
+/* rh/hunt.c:92-117 (ascending branch; all tables here ascend) */
+static int locate_asc(int n, const double *a, double v)
+{
+  int lo = 0, hi = n, idx;
+  if (a[n - 1] > a[0]) {
+    while (hi - lo > 1) {
+      idx = (hi + lo) >> 1;
+      if (v >= a[idx]) lo = idx; else hi = idx;
+    }
+  } else {
+    while (hi - lo > 1) {
+      idx = (hi + lo) >> 1;
+      if (v <= a[idx]) lo = idx; else hi = idx;
+    }
+  }
+  return lo;
+}
+
+/* rh/linear.c:22-54 for one abscissa */
+static double linear1(int n, const double *xt, const double *yt, double x)
+{
+  int asc = xt[1] > xt[0], j;
+  double xmin = asc ? xt[0] : xt[n - 1], xmax = asc ? xt[n - 1] : xt[0], fx;
+  if (x <= xmin) return asc ? yt[0] : yt[n - 1];
+  if (x >= xmax) return asc ? yt[n - 1] : yt[0];
+  j = locate_asc(n, xt, x);
+  fx = (xt[j + 1] - x) / (xt[j + 1] - xt[j]);
+  return fx * yt[j] + (1 - fx) * yt[j + 1];
+}
+
+/* rh/hydrogen.c:998-1020 */
+static double bilinear(int Ncol, int Nrow, const double *f, double x, double y)
+{
+  int i, j, i1, j1;
+  double fx, fy;
+  i = (int) x;  fx = x - i;
+  i1 = (i == Ncol - 1) ? i : i + 1;
+  j = (int) y;  fy = y - j;
+  j1 = (j == Nrow - 1) ? j : j + 1;
+  return (1.0 - fx) * (1.0 - fy) * f[j * Ncol + i] +
+         fx * (1.0 - fy) * f[j * Ncol + i1] +
+         (1.0 - fx) * fy * f[j1 * Ncol + i] +
+         fx * fy * f[j1 * Ncol + i1];
+}
+
+/* fractional index into an ascending table with end clamping, as the ff
+   routines form it (rh/hydrogen.c:509-523) */
+static double clamp_index(int n, const double *t, double v)
+{
+  int i;
+  if (v <= t[0]) return 0;
+  if (v >= t[n - 1]) return n - 1;
+  i = locate_asc(n, t, v);
+  return (double) i + (v - t[i]) / (t[i + 1] - t[i]);
+}
+
+/* ------------------------------------------------------------------ */
+/* Voigt functions                                                     */
+
+typedef struct { double r, i; } cplx;
+static cplx cmul(cplx a, cplx b) { cplx c = {a.r*b.r - a.i*b.i, a.r*b.i + a.i*b.r}; return c; }
+static cplx csc(double a, cplx b) { cplx c = {a * b.r, a * b.i}; return c; }
+static cplx cadr(cplx a, double b) { cplx c = {a.r + b, a.i}; return c; }
+static cplx csub(cplx a, cplx b) { cplx c = {a.r - b.r, a.i - b.i}; return c; }
+static cplx cdivi(cplx a, cplx b)   /* rh/complex.c:87-100 */
+{
+  double d = b.r * b.r + b.i * b.i;
+  cplx c = {(a.r * b.r + a.i * b.i) / d, (a.i * b.r - a.r * b.i) / d};
+  return c;
+}
+
+/* Humlicek (1982) rational approximations, regions I-IV, with the region
+   test of rh/voigt.c:396-413 and the coefficients of rh/humlicek.c:28-120 */
+double rho_voigt_humlicek(double a, double v, double *F)
+{
+  static const double a3[5] = {0.5642236, 3.778987, 11.96482, 20.20933, 16.4955};
+  static const double b3[5] = {6.699398, 21.69274, 39.27121, 38.82363, 16.4955};
+  static const double a4[7] = {0.56419, 1.320522, 35.7668, 219.031, 1540.787,
+                               3321.99, 36183.31};
+  static const double b4[7] = {1.841439, 61.57037, 364.2191, 2186.181, 9022.228,
+                               24322.84, 32066.6};
+  cplx z = {a, -v}, W, z1, z2, u;
+  double s = fabs(v) + a;
+  int n;
+  if (s >= 15.0) {
+    z1 = csc(0.5641896, z);
+    z2 = cadr(cmul(z, z), 0.5);
+    W = cdivi(z1, z2);
+  } else if (s >= 5.5) {
+    u = cmul(z, z);
+    z1 = csc(0.5641896, u);
+    z1 = cmul(z, cadr(z1, 1.410474));
+    z2 = cmul(u, cadr(u, 3.0));
+    z2 = cadr(z2, 0.75);
+    W = cdivi(z1, z2);
+  } else if (a >= 0.195 * fabs(v) - 0.176) {
+    z1.r = a3[0]; z1.i = 0.0;
+    z2 = cadr(z, b3[0]);
+    for (n = 1; n < 5; n++) {
+      z1 = cadr(cmul(z1, z), a3[n]);
+      z2 = cadr(cmul(z2, z), b3[n]);
+    }
+    W = cdivi(z1, z2);
+  } else {
+    cplx e;
+    u = cmul(z, csc(-1.0, z));
+    z1.r = a4[0]; z1.i = 0.0;
+    z2 = cadr(u, b4[0]);
+    for (n = 1; n < 7; n++) {
+      z1 = cadr(cmul(u, z1), a4[n]);
+      z2 = cadr(cmul(u, z2), b4[n]);
+    }
+    {
+      cplx mu = csc(-1.0, u), cs = {cos(mu.i), sin(mu.i)};
+      e = csc(exp(mu.r), cs);
+    }
+    W = csub(e, cdivi(cmul(z, z1), z2));
+  }
+  if (F) *F = W.i;
+  return W.r;
+}
+
+/* Armstrong (1967) Voigt function, rh/voigt.c:126-243 */
+static const double arm_t[10] = {0.2453407083, 0.7374737285, 1.2340762153,
+  1.7385377121, 2.2549740020, 2.7888060584, 3.3478545673, 3.9447640401,
+  4.6036824495, 5.3874808900};
+static const double arm_w[10] = {4.6224366960e-01, 2.8667550536e-01,
+  1.0901720602e-01, 2.4810520887e-02, 3.2437733422e-03, 2.2833863601e-04,
+  7.8025564785e-06, 1.0860693707e-07, 4.3993409922e-10, 2.2293936455e-13};
+
+static double voigt_k1(double a, double v)
+{
+  static const double c[34] = {0.1999999999972224, -0.1840000000029998,
+    0.1558399999965025, -0.1216640000043988, 0.0877081599940391,
+    -0.0585141248086907, 0.0362157301623914, -0.0208497654398036,
+    0.0111960116346270, -0.56231896167109e-02, 0.26487634172265e-02,
+    -0.11732670757704e-02, 0.4899519978088e-03, -0.1933630801528e-03,
+    0.722877446788e-04, -0.256555124979e-04, 0.86620736841e-05,
+    -0.27876379719e-05, 0.8566873627e-06, -0.2518433784e-06, 0.709360221e-07,
+    -0.191732257e-07, 0.49801256e-08, -0.12447734e-08, 0.2997777e-09,
+    -0.696450e-10, 0.156262e-10, -0.33897e-11, 0.7116e-12, -0.1447e-12,
+    0.285e-13, -0.55e-14, 0.10e-14, -0.2e-15};
+  const double TWOSQRTPI = 1.12837917, EXPMAX = 70.0, TINY = 1.0E-08;
+  double a2 = a * a, v2 = v * v, u1, dn01, dn02, dn, v2i, funct, an, q, g,
+         coef, bn01, bn02, bn = 0.0, v1;
+  int n;
+  if ((v2 - a2) > EXPMAX) u1 = 0.0;
+  else u1 = exp(a2 - v2) * cos(2.0 * v * a);
+  if (v > 5.0) {
+    v2i = 1.0 / v2;
+    dn01 = -v2i * (0.5 + v2i * (0.75 + v2i * (1.875 + v2i * (6.5625 +
+           v2i * (29.53125 + v2i * (1162.4218 + v2i * 1055.7421))))));
+    dn02 = (1.0 - dn01) / (2.0 * v);
+  } else {
+    bn01 = bn02 = 0.0;
+    v1 = v / 5.0;
+    coef = 4.0 * v1 * v1 - 2.0;
+    for (n = 33; n >= 0; n--) {
+      bn = coef * bn01 - bn02 + c[n];
+      bn02 = bn01;
+      bn01 = bn;
+    }
+    dn02 = (double) (v1 * (bn - bn02));
+    dn01 = 1.0 - 2.0 * v * dn02;
+  }
+  funct = a * dn01;
+  if (a > TINY) {
+    q = 1.0;
+    an = a;
+    for (n = 2; n <= 50; n++) {
+      dn = (v * dn01 + dn02) * (-2.0 / n);
+      dn02 = dn01;
+      dn01 = dn;
+      if (n % 2) {
+        q = -q;
+        an *= a2;
+        g = dn * an;
+        funct += q * g;
+        if (fabs(g / funct) <= TINY) return (u1 - TWOSQRTPI * funct);
+      }
+    }
+  }
+  return (u1 - TWOSQRTPI * funct);
+}
+
+static double voigt_k2(double a, double v)
+{
+  double g = 0.0, r, s, a2 = a * a;
+  int n;
+  for (n = 0; n < 10; n++) {
+    r = arm_t[n] - v;
+    s = arm_t[n] + v;
+    g += (4.0 * arm_t[n] * arm_t[n] - 2.0) * (r * atan(r / a) + s * atan(s / a) -
+         0.5 * a * (log(a2 + r * r) + log(a2 + s * s))) * arm_w[n];
+  }
+  return g / PI;
+}
+
+static double voigt_k3(double a, double v)
+{
+  double g = 0.0, a2 = a * a;
+  int n;
+  for (n = 0; n < 10; n++)
+    g += (1.0 / (SQ(v - arm_t[n]) + a2) + 1.0 / (SQ(v + arm_t[n]) + a2)) * arm_w[n];
+  return (a * g) / PI;
+}
+
+double rho_voigt_armstrong(double a, double v)
+{
+  if (v < 0.0) v = -v;
+  if ((a < 1.0 && v < 4.0) || (a < 1.8 / (v + 1.0))) return voigt_k1(a, v);
+  else if (a < 2.5 && v < 4.0) return voigt_k2(a, v);
+  else return voigt_k3(a, v);
+}
+
+/* ------------------------------------------------------------------ */
+/* formal-solution helpers                                            */
+
+double rho_planck(double T, double lambda)   /* rh/planck.c:38-63 */
+{
+  double hc_kla = (HPLANCK * CLIGHT) / (KBOLTZMANN * NM_TO_M * lambda);
+  double twohnu3_c2 = (2.0 * HPLANCK * CLIGHT) / CUBE(NM_TO_M * lambda);
+  double hc_Tkla = hc_kla / T;
+  return (hc_Tkla <= MAX_EXPONENT) ? twohnu3_c2 / (exp(hc_Tkla) - 1.0) : 0.0;
+}
+
+void rho_w3(double dtau, double *w)   /* rh/w3.c:44-63 */
+{
+  double expdt, delta;
+  if (dtau < 5.0E-4) {
+    w[0] = dtau * (1.0 - 0.5 * dtau);
+    delta = SQ(dtau);
+    w[1] = delta * (0.5 - dtau / 3.0);
+    delta *= dtau;
+    w[2] = delta * (1.0 / 3.0 - 0.25 * dtau);
+  } else if (dtau > 50.0) {
+    w[1] = w[0] = 1.0;
+    w[2] = 2.0;
+  } else {
+    expdt = exp(-dtau);
+    w[0] = 1.0 - expdt;
+    w[1] = w[0] - dtau * expdt;
+    w[2] = 2.0 * w[1] - SQ(dtau) * expdt;
+  }
+}
+
+/* Fritsch & Butland (1984) centred derivative, rh/bezier_aux.c:34-51 */
+static double cent_deriv(double dsup, double dsdn, double chiup, double chic,
+                         double chidn)
+{
+  double fim1 = (chic - chiup) / dsup, fi = (chidn - chic) / dsdn, alpha;
+  if (fim1 * fi > 0) {
+    alpha = 0.333333333333333333333333 * (1.0 + dsdn / (dsdn + dsup));
+    return (fim1 * fi) / ((1.0 - alpha) * fim1 + alpha * fi);
+  }
+  return 0.0;
+}
+
+/* Integration weights of the cubic Bezier interpolant (de la Cruz
+   Rodriguez & Piskunov 2013), rh/bezier_aux.c:333-359 */
+void rho_bezier3_coeffs(double dt, double *o)
+{
+  double dt2 = dt * dt, dt3 = dt2 * dt, dt4, eps;
+  if (dt >= 5.e-2) {
+    eps = exp(-dt);
+    o[0] = (-6.0 + 6.0 * dt - 3.0 * dt2 + dt3 + 6.0 * eps) / dt3;
+    dt3 = 1.0 / dt3;
+    o[1] = (6.0 + (-6.0 - dt * (6.0 + dt * (3.0 + dt))) * eps) * dt3;
+    o[2] = 3.0 * (6.0 + (-4.0 + dt) * dt - 2.0 * (3.0 + dt) * eps) * dt3;
+    o[3] = 3.0 * (eps * (6.0 + dt2 + 4.0 * dt) + 2.0 * dt - 6.0) * dt3;
+  } else {
+    dt4 = dt2 * dt2;
+    eps = 1.0 - dt + 0.5 * dt2 - dt3 / 6.0 + dt4 / 24.0;
+    o[0] = 0.25 * dt - 0.05 * dt2 + dt3 / 120.0 - dt4 / 840.0;
+    o[1] = 0.25 * dt - 0.20 * dt2 + dt3 / 12.0 - dt4 / 42.0;
+    o[2] = 0.25 * dt - 0.10 * dt2 + dt3 * 0.025 - dt4 / 210.0;
+    o[3] = 0.25 * dt - 0.15 * dt2 + dt3 * 0.05 - dt4 / 84.0;
+  }
+  o[4] = eps;
+}
+
+/* 4x4 inverse by cofactors (Cramer's rule), with the cofactor pairing of
+   rh/bezier_aux.c:235-328 (Intel AP-928 "Streaming SIMD Extensions -
+   Inverse of 4x4 Matrix", scalar form).  mat is row-major [4][4]. */
+#define CRAMER_BODY(T)                                                    \
+  T tmp[12], src[16], dst[16], det;                                       \
+  int i;                                                                  \
+  for (i = 0; i < 4; i++) {                                               \
+    src[i] = mat[i * 4]; src[i + 4] = mat[i * 4 + 1];                     \
+    src[i + 8] = mat[i * 4 + 2]; src[i + 12] = mat[i * 4 + 3];            \
+  }                                                                       \
+  tmp[0] = src[10] * src[15]; tmp[1] = src[11] * src[14];                 \
+  tmp[2] = src[9] * src[15];  tmp[3] = src[11] * src[13];                 \
+  tmp[4] = src[9] * src[14];  tmp[5] = src[10] * src[13];                 \
+  tmp[6] = src[8] * src[15];  tmp[7] = src[11] * src[12];                 \
+  tmp[8] = src[8] * src[14];  tmp[9] = src[10] * src[12];                 \
+  tmp[10] = src[8] * src[13]; tmp[11] = src[9] * src[12];                 \
+  dst[0] = tmp[0]*src[5] + tmp[3]*src[6] + tmp[4]*src[7];                 \
+  dst[0] -= tmp[1]*src[5] + tmp[2]*src[6] + tmp[5]*src[7];                \
+  dst[1] = tmp[1]*src[4] + tmp[6]*src[6] + tmp[9]*src[7];                 \
+  dst[1] -= tmp[0]*src[4] + tmp[7]*src[6] + tmp[8]*src[7];                \
+  dst[2] = tmp[2]*src[4] + tmp[7]*src[5] + tmp[10]*src[7];                \
+  dst[2] -= tmp[3]*src[4] + tmp[6]*src[5] + tmp[11]*src[7];               \
+  dst[3] = tmp[5]*src[4] + tmp[8]*src[5] + tmp[11]*src[6];                \
+  dst[3] -= tmp[4]*src[4] + tmp[9]*src[5] + tmp[10]*src[6];               \
+  dst[4] = tmp[1]*src[1] + tmp[2]*src[2] + tmp[5]*src[3];                 \
+  dst[4] -= tmp[0]*src[1] + tmp[3]*src[2] + tmp[4]*src[3];                \
+  dst[5] = tmp[0]*src[0] + tmp[7]*src[2] + tmp[8]*src[3];                 \
+  dst[5] -= tmp[1]*src[0] + tmp[6]*src[2] + tmp[9]*src[3];                \
+  dst[6] = tmp[3]*src[0] + tmp[6]*src[1] + tmp[11]*src[3];                \
+  dst[6] -= tmp[2]*src[0] + tmp[7]*src[1] + tmp[10]*src[3];               \
+  dst[7] = tmp[4]*src[0] + tmp[9]*src[1] + tmp[10]*src[2];                \
+  dst[7] -= tmp[5]*src[0] + tmp[8]*src[1] + tmp[11]*src[2];               \
+  tmp[0] = src[2]*src[7];  tmp[1] = src[3]*src[6];                        \
+  tmp[2] = src[1]*src[7];  tmp[3] = src[3]*src[5];                        \
+  tmp[4] = src[1]*src[6];  tmp[5] = src[2]*src[5];                        \
+  tmp[6] = src[0]*src[7];  tmp[7] = src[3]*src[4];                        \
+  tmp[8] = src[0]*src[6];  tmp[9] = src[2]*src[4];                        \
+  tmp[10] = src[0]*src[5]; tmp[11] = src[1]*src[4];                       \
+  dst[8] = tmp[0]*src[13] + tmp[3]*src[14] + tmp[4]*src[15];              \
+  dst[8] -= tmp[1]*src[13] + tmp[2]*src[14] + tmp[5]*src[15];             \
+  dst[9] = tmp[1]*src[12] + tmp[6]*src[14] + tmp[9]*src[15];              \
+  dst[9] -= tmp[0]*src[12] + tmp[7]*src[14] + tmp[8]*src[15];             \
+  dst[10] = tmp[2]*src[12] + tmp[7]*src[13] + tmp[10]*src[15];            \
+  dst[10] -= tmp[3]*src[12] + tmp[6]*src[13] + tmp[11]*src[15];           \
+  dst[11] = tmp[5]*src[12] + tmp[8]*src[13] + tmp[11]*src[14];            \
+  dst[11] -= tmp[4]*src[12] + tmp[9]*src[13] + tmp[10]*src[14];           \
+  dst[12] = tmp[2]*src[10] + tmp[5]*src[11] + tmp[1]*src[9];              \
+  dst[12] -= tmp[4]*src[11] + tmp[0]*src[9] + tmp[3]*src[10];             \
+  dst[13] = tmp[8]*src[11] + tmp[0]*src[8] + tmp[7]*src[10];              \
+  dst[13] -= tmp[6]*src[10] + tmp[9]*src[11] + tmp[1]*src[8];             \
+  dst[14] = tmp[6]*src[9] + tmp[11]*src[11] + tmp[3]*src[8];              \
+  dst[14] -= tmp[10]*src[11] + tmp[2]*src[8] + tmp[7]*src[9];             \
+  dst[15] = tmp[10]*src[10] + tmp[4]*src[8] + tmp[9]*src[9];              \
+  dst[15] -= tmp[8]*src[9] + tmp[11]*src[10] + tmp[5]*src[8];             \
+  det = src[0]*dst[0] + src[1]*dst[1] + src[2]*dst[2] + src[3]*dst[3];    \
+  det = 1.0 / det;                                                        \
+  for (i = 0; i < 16; i++) mat[i] = dst[i] * det;
+
+static void matinv_d(double *mat) { CRAMER_BODY(double) }
+static void matinv_f(float *mat) { CRAMER_BODY(float) }
+
+/* Md^-1 V0 with Md held in double or float (rh/bezier_aux.c:109-117) */
+static void solve4(double Md[4][4], const double *V0, double *V1, int fp32)
+{
+  int i, k;
+  if (fp32) {
+    float f[16];
+    for (i = 0; i < 16; i++) f[i] = (float) Md[i / 4][i % 4];
+    matinv_f(f);
+    for (i = 0; i < 4; i++) {
+      V1[i] = 0.0;
+      for (k = 0; k < 4; k++) V1[i] += ((double) f[i * 4 + k]) * V0[k];
+    }
+  } else {
+    double d[16];
+    for (i = 0; i < 16; i++) d[i] = Md[i / 4][i % 4];
+    matinv_d(d);
+    for (i = 0; i < 4; i++) {
+      V1[i] = 0.0;
+      for (k = 0; k < 4; k++) V1[i] += d[i * 4 + k] * V0[k];
+    }
+  }
+}
+
+/* reduced propagation matrix K' (rh/stokesopac.c:28-88), MO terms off;
+   q[3] are chi_Q,U,V at this depth */
+static void stokesK(const double *q, double chi_I, double K[4][4])
+{
+  int i, j;
+  for (j = 0; j < 4; j++) for (i = 0; i < 4; i++) K[j][i] = 0.0;
+  K[0][1] = q[0]; K[0][2] = q[1]; K[0][3] = q[2];
+  for (j = 0; j < 3; j++)
+    for (i = j + 1; i < 4; i++) {
+      K[j][i] /= chi_I;
+      K[i][j] = K[j][i];
+    }
+}
+
+/* m4m: c = b . a as rh/bezier_aux.c:94-103 accumulates it */
+static void m4m(double a[4][4], double b[4][4], double c[4][4])
+{
+  int i, j, k;
+  for (j = 0; j < 4; j++) for (i = 0; i < 4; i++) c[j][i] = 0.0;
+  for (j = 0; j < 4; j++)
+    for (i = 0; i < 4; i++)
+      for (k = 0; k < 4; k++) c[j][i] += a[k][i] * b[j][k];
+}
+
+/* Cubic DELO-Bezier polarized solver for the ray leaving the top
+   (to_obs), rh/rhf1d/bezier_1D.c:52-300.  S4/chiQUV/I4 are [4|3][n]. */
+void rho_formal_stokes(int Ndep, const double *z, double mu, const double *chi,
+                       const double *S4, const double *chiQUV,
+                       const double *B2, int fp32, double *I4)
+{
+  static const double ident[4][4] = {{1,0,0,0},{0,1,0,0},{0,0,1,0},{0,0,0,1}};
+  const int dk = -1, k_start = Ndep - 1, k_end = 0;
+  double imu = 1.0 / mu, dtau_uw, dtau_dw = 0.0, c1, c2, w[3], dsdn2, dchi_dn,
+         I_upw[4], dchi_up, dchi_c, dt03, dsup, dsdn, dt, co[5];
+  double Ku[4][4], K0[4][4], Kd[4][4], dKu[4][4], dK0[4][4];
+  double Su[4], S0[4], Sd[4], dSu[4], dS0[4];
+  double A[4][4], Ma[4][4], Mb[4][4], Mc[4][4], Md[4][4], V0[4], V1[4], q[3];
+  int k, n, m, i, j;
+#define IQ(s, kk) I4[(s) * Ndep + (kk)]
+#define SV(s, kk) S4[(s) * Ndep + (kk)]
+#define LOADK(kk, KK) do { q[0] = chiQUV[kk]; q[1] = chiQUV[Ndep + (kk)]; \
+    q[2] = chiQUV[2 * Ndep + (kk)]; stokesK(q, chi[kk], KK); } while (0)
+
+  dtau_uw = 0.5 * imu * (chi[k_start] + chi[k_start + dk]) *
+            fabs(z[k_start] - z[k_start + dk]);
+  if (B2) I_upw[0] = B2[1] - (B2[0] - B2[1]) / dtau_uw;
+  else I_upw[0] = 0.0;
+  for (n = 1; n < 4; n++) I_upw[n] = 0.0;
+  for (n = 0; n < 4; n++) IQ(n, k_start) = I_upw[n];
+
+  k = k_start + dk;
+  dsup = fabs(z[k] - z[k - dk]) * imu;
+  dsdn = fabs(z[k + dk] - z[k]) * imu;
+  dchi_up = (chi[k] - chi[k - dk]) / dsup;
+  dchi_c = cent_deriv(dsup, dsdn, chi[k - dk], chi[k], chi[k + dk]);
+  c2 = MAXV(chi[k] - (dsup / 3.0) * dchi_c, 0.0);
+  c1 = MAXV(chi[k - dk] + (dsup / 3.0) * dchi_up, 0.0);
+  dtau_uw = 0.25 * dsup * (chi[k] + chi[k - dk] + c1 + c2);
+
+  LOADK(k_start, Ku);
+  LOADK(k_start + dk, K0);
+  for (n = 0; n < 4; n++) { Su[n] = SV(n, k_start); S0[n] = SV(n, k_start + dk); }
+  for (n = 0; n < 4; n++) {
+    dSu[n] = (S0[n] - Su[n]) / dtau_uw;
+    for (m = 0; m < 4; m++) dKu[n][m] = (K0[n][m] - Ku[n][m]) / dtau_uw;
+  }
+
+  for (k = k_start + dk; k != k_end; k += dk) {
+    dsdn = fabs(z[k + dk] - z[k]) * imu;
+    if (fabs(k - k_end) > 1) {
+      dsdn2 = fabs(z[k + 2 * dk] - z[k + dk]) * imu;
+      dchi_dn = cent_deriv(dsdn, dsdn2, chi[k], chi[k + dk], chi[k + 2 * dk]);
+    } else
+      dchi_dn = (chi[k + dk] - chi[k]) / dsdn;
+    c2 = MAXV(chi[k] + (dsdn / 3.0) * dchi_c, 0.0);
+    c1 = MAXV(chi[k + dk] - (dsdn / 3.0) * dchi_dn, 0.0);
+    dtau_dw = 0.25 * dsdn * (chi[k] + chi[k + dk] + c1 + c2);
+    dt = dtau_uw; dt03 = dt / 3.0;
+    rho_bezier3_coeffs(dt, co);
+
+    LOADK(k + dk, Kd);
+    for (n = 0; n < 4; n++) Sd[n] = SV(n, k + dk);
+    for (j = 0; j < 4; j++)
+      for (i = 0; i < 4; i++)
+        dK0[j][i] = cent_deriv(dtau_uw, dtau_dw, Ku[j][i], K0[j][i], Kd[j][i]);
+    for (i = 0; i < 4; i++) dS0[i] = cent_deriv(dtau_uw, dtau_dw, Su[i], S0[i], Sd[i]);
+
+    m4m(Ku, Ku, Ma);
+    m4m(K0, K0, A);
+    for (j = 0; j < 4; j++)
+      for (i = 0; i < 4; i++) {
+        Md[j][i] = ident[j][i] + co[0] * K0[j][i] - co[2] *
+          -(dt03 * (A[j][i] + dK0[j][i] + K0[j][i]) + K0[j][i]);
+        Ma[j][i] = co[4] * ident[j][i] - co[1] * Ku[j][i] + co[3] *
+          (dt03 * (Ma[j][i] + dKu[j][i] + Ku[j][i]) - Ku[j][i]);
+        Mb[j][i] = co[1] * ident[j][i] + co[3] * (ident[j][i] - dt03 * Ku[j][i]);
+        Mc[j][i] = co[0] * ident[j][i] + co[2] * (ident[j][i] + dt03 * K0[j][i]);
+      }
+    for (i = 0; i < 4; i++) {
+      V0[i] = 0.0;
+      for (j = 0; j < 4; j++)
+        V0[i] += Ma[i][j] * IQ(j, k - dk) + Mb[i][j] * Su[j] + Mc[i][j] * S0[j];
+      V0[i] += dt03 * (co[2] * dS0[i] - co[3] * dSu[i]);
+    }
+    solve4(Md, V0, V1, fp32);
+    for (i = 0; i < 4; i++) IQ(i, k) = V1[i];
+
+    memcpy(Su, S0, sizeof(Su));
+    memcpy(S0, Sd, sizeof(S0));
+    memcpy(dSu, dS0, sizeof(dSu));
+    memcpy(Ku, K0, sizeof(Ku));
+    memcpy(K0, Kd, sizeof(K0));
+    memcpy(dKu, dK0, sizeof(dKu));
+    dtau_uw = dtau_dw;
+    dsup = dsdn;
+    dchi_up = dchi_c;
+    dchi_c = dchi_dn;
+  }
+
+  k = k_end;
+  dtau_uw = 0.5 * imu * (chi[k] + chi[k - dk]) * fabs(z[k] - z[k - dk]);
+  rho_w3(dtau_uw, w);
+  for (n = 0; n < 4; n++) V0[n] = w[0] * SV(n, k) + w[1] * -dSu[n];
+  for (n = 0; n < 4; n++) {
+    for (m = 0; m < 4; m++) {
+      A[n][m] = -w[1] / dtau_uw * Ku[n][m];
+      Md[n][m] = (w[0] - w[1] / dtau_uw) * K0[n][m];
+    }
+    A[n][n] = 1.0 - w[0];
+    Md[n][n] = 1.0;
+  }
+  for (n = 0; n < 4; n++)
+    for (m = 0; m < 4; m++) V0[n] += A[n][m] * IQ(m, k - dk);
+  solve4(Md, V0, V1, fp32);
+  for (n = 0; n < 4; n++) IQ(n, k) = V1[n];
+#undef IQ
+#undef SV
+#undef LOADK
+}
+
+/* Cubic Bezier solver for unpolarized light, ray leaving the top,
+   rh/rhf1d/bezier_1D.c:306-541 (without the log gf derivatives) */
+void rho_formal_bezier(int Ndep, const double *z, double mu, const double *chi,
+                       const double *S, const double *B2, double *I)
+{
+  const int dk = -1, k_start = Ndep - 1, k_end = 0;
+  double zmu = 1.0 / mu, dtau_uw, dtau_dw = 0.0, dS_uw, I_upw, c1, c2, w[3];
+  double dsup, dsdn, dt03, co[5], dS_up, dS_c, dchi_up, dchi_c, dchi_dn = 0.0, dsdn2;
+  int k;
+
+  dtau_uw = 0.5 * zmu * (chi[k_start] + chi[k_start + dk]) *
+            fabs(z[k_start] - z[k_start + dk]);
+  I_upw = B2 ? B2[1] - (B2[0] - B2[1]) / dtau_uw : 0.0;
+  I[k_start] = I_upw;
+
+  k = k_start + dk;
+  dsup = fabs(z[k] - z[k - dk]) * zmu;
+  dsdn = fabs(z[k + dk] - z[k]) * zmu;
+  dchi_up = (chi[k] - chi[k - dk]) / dsup;
+  dchi_c = cent_deriv(dsup, dsdn, chi[k - dk], chi[k], chi[k + dk]);
+  c1 = MAXV(chi[k] - (dsup / 3.0) * dchi_c, 0.0);
+  c2 = MAXV(chi[k - dk] + (dsup / 3.0) * dchi_up, 0.0);
+  dtau_uw = dsup * (chi[k] + chi[k - dk] + c1 + c2) * 0.25;
+  dS_up = (S[k] - S[k - dk]) / dtau_uw;
+
+  for (k = k_start + dk; k != k_end + dk; k += dk) {
+    if (k != k_end) {
+      dsdn = fabs(z[k + dk] - z[k]) * zmu;
+      if (fabs(k - k_end) > 1) {
+        dsdn2 = fabs(z[k + 2 * dk] - z[k + dk]) * zmu;
+        dchi_dn = cent_deriv(dsdn, dsdn2, chi[k], chi[k + dk], chi[k + 2 * dk]);
+      } else
+        dchi_dn = (chi[k + dk] - chi[k]) / dsdn;
+      c1 = MAXV(chi[k] + (dsdn / 3.0) * dchi_c, 0.0);
+      c2 = MAXV(chi[k + dk] - (dsdn / 3.0) * dchi_dn, 0.0);
+      dtau_dw = dsdn * (chi[k] + chi[k + dk] + c1 + c2) * 0.25;
+      dt03 = dtau_uw / 3.0;
+      rho_bezier3_coeffs(dtau_uw, co);
+      dS_c = cent_deriv(dtau_uw, dtau_dw, S[k - dk], S[k], S[k + dk]);
+      c1 = MAXV(S[k] - dt03 * dS_c, 0.0);
+      c2 = MAXV(S[k - dk] + dt03 * dS_up, 0.0);
+      I[k] = I_upw * co[4] + co[0] * S[k] + co[1] * S[k - dk] + co[2] * c1 + co[3] * c2;
+    } else {
+      dtau_uw = 0.5 * zmu * (chi[k] + chi[k - dk]) * fabs(z[k] - z[k - dk]);
+      dS_uw = -(S[k] - S[k - dk]) / dtau_uw;
+      rho_w3(dtau_uw, w);
+      I[k] = (1.0 - w[0]) * I_upw + w[0] * S[k] + w[1] * dS_uw;
+    }
+    I_upw = I[k];
+    dsup = dsdn;
+    dchi_up = dchi_c;
+    dchi_c = dchi_dn;
+    dtau_uw = dtau_dw;
+    dS_up = dS_c;
+  }
+  (void) dsup;
+  (void) dchi_up;
+}
+
+/* Feautrier solution, STANDARD order, ZERO upper boundary, thermalised
+   lower boundary; returns the emergent intensity.  rh/rhf1d/feautrier.c:56-202 */
+double rho_formal_feautrier(int Ndep, const double *z, double mu,
+                            const double *chi, const double *S, const double *B2)
+{
+  double r0 = 0.0, h0 = 0.0, rN = 0.0, hN, f0, fN, dtau_mid, zmu = 0.5 / mu;
+  double *dtau = malloc(8 * (size_t) Ndep), *abc = malloc(8 * (size_t) Ndep),
+         *A1 = malloc(8 * (size_t) Ndep), *C1 = malloc(8 * (size_t) Ndep),
+         *F = malloc(8 * (size_t) Ndep), *ztmp = malloc(8 * (size_t) Ndep),
+         *Stmp = malloc(8 * (size_t) Ndep), *P = malloc(8 * (size_t) Ndep), Iplus;
+  int k;
+  for (k = 0; k < Ndep - 1; k++)
+    dtau[k] = zmu * (chi[k] + chi[k + 1]) * (z[k] - z[k + 1]);
+  f0 = (1.0 - r0) / (1.0 + r0);
+  abc[0] = 1.0 + 2.0 * f0 / dtau[0];
+  C1[0] = 2.0 / SQ(dtau[0]);
+  Stmp[0] = S[0] + 2.0 * h0 / ((1.0 + r0) * dtau[0]);
+  hN = B2 ? B2[1] - (B2[0] - B2[1]) / dtau[Ndep - 2] : 0.0;
+  fN = (1.0 - rN) / (1.0 + rN);
+  abc[Ndep - 1] = 1.0 + 2.0 * fN / dtau[Ndep - 2];
+  A1[Ndep - 1] = 2.0 / SQ(dtau[Ndep - 2]);
+  Stmp[Ndep - 1] = S[Ndep - 1] + 2.0 * hN / ((1.0 + rN) * dtau[Ndep - 2]);
+  for (k = 1; k < Ndep - 1; k++) {
+    dtau_mid = 0.5 * (dtau[k] + dtau[k - 1]);
+    A1[k] = 1.0 / (dtau_mid * dtau[k - 1]);
+    C1[k] = 1.0 / (dtau_mid * dtau[k]);
+    abc[k] = 1.0;
+    Stmp[k] = S[k];
+  }
+  F[0] = abc[0] / C1[0];
+  ztmp[0] = Stmp[0] / (abc[0] + C1[0]);
+  for (k = 1; k < Ndep - 1; k++) {
+    F[k] = (abc[k] + A1[k] * F[k - 1] / (1.0 + F[k - 1])) / C1[k];
+    ztmp[k] = (Stmp[k] + A1[k] * ztmp[k - 1]) / (C1[k] * (1.0 + F[k]));
+  }
+  P[Ndep - 1] = (Stmp[Ndep - 1] + A1[Ndep - 1] * ztmp[Ndep - 2]) /
+    (abc[Ndep - 1] + A1[Ndep - 1] * (F[Ndep - 2] / (1.0 + F[Ndep - 2])));
+  for (k = Ndep - 2; k >= 0; k--) P[k] = P[k + 1] / (1.0 + F[k]) + ztmp[k];
+  Iplus = (1.0 + f0) * P[0] - h0 / (1.0 + r0);
+  free(dtau); free(abc); free(A1); free(C1); free(F); free(ztmp); free(Stmp); free(P);
+  return Iplus;
+}
+
+/* ------------------------------------------------------------------ */
+/* equilibria                                                          */
+
+/* Saha-Boltzmann populations of one model atom at one depth,
+   rh/ltepops.c:33-113 (Debye lowering disabled there) */
+static void ltepops(const rhg_model_t *m, int ia, double T, double ne,
+                    double ntotal, double *nstar)
+{
+  int i0 = m->atom_level_ptr[ia], nl = m->atom_level_ptr[ia + 1] - i0, i, mm, dZ;
+  const double *E = m->level_E + i0, *g = m->level_g + i0;
+  const int32_t *st = m->level_stage + i0;
+  double c1 = (HPLANCK / (2.0 * PI * M_ELECTRON)) * (HPLANCK / KBOLTZMANN);
+  double cNe_T = 0.5 * ne * pow(c1 / T, 1.5), sum = 1.0, dE, gi0, dE_kT;
+  for (i = 1; i < nl; i++) {
+    dE = E[i] - E[0];
+    gi0 = g[i] / g[0];
+    dZ = st[i] - st[0];
+    dE_kT = dE / (KBOLTZMANN * T);
+    nstar[i] = gi0 * exp(-dE_kT);
+    for (mm = 1; mm <= dZ; mm++) nstar[i] /= cNe_T;
+    sum += nstar[i];
+  }
+  nstar[0] = ntotal / sum;
+  for (i = 1; i < nl; i++) nstar[i] *= nstar[0];
+}
+
+/* equilibrium constant of a molecule, rh/chemequil.c:452-533 */
+static double equilconstant(const rhg_model_t *m, int im, double T)
+{
+  int i, mk, neqc = m->mol_neqc[im];
+  const double *c = m->mol_eqc + 8 * im;
+  double kT, t, eqc = 0.0, theta, cgs_to_SI = 1.0;
+  if (T < m->mol_tmin[im] || T > m->mol_tmax[im]) return 0.0;
+  switch (m->mol_fit[im]) {
+  case 0: /* KURUCZ_70 */
+  case 1: /* KURUCZ_85 */
+    t = (m->mol_fit[im] == 1) ? T * 1.0E-4 : T;
+    kT = KBOLTZMANN * T;
+    mk = m->mol_nnuclei[im] - 1 - m->mol_charge[im];
+    eqc = c[0];
+    for (i = 1; i < neqc; i++) eqc = eqc * t + c[i];
+    eqc = exp(m->mol_ediss[im] / kT + eqc - 1.5 * mk * log(T));
+    cgs_to_SI = pow(CUBE(CM_TO_M), mk);
+    break;
+  case 3: /* IRWIN_81 uses the SAUVAL_TATUM_84 form */
+  case 2:
+    theta = THETA0 / T;
+    t = log10(theta);
+    kT = KBOLTZMANN * T;
+    eqc = c[0];
+    for (i = 1; i < neqc; i++) eqc = eqc * t + c[i];
+    eqc = POW10((m->mol_ediss[im] / EV) * theta - eqc) * kT;
+    cgs_to_SI = 1.0;
+    break;
+  case 4: /* TSUJI_73 */
+    theta = THETA0 / T;
+    kT = KBOLTZMANN * T;
+    eqc = c[0];
+    for (i = 1; i < neqc; i++) eqc = eqc * theta + c[i];
+    eqc = SQ(kT) * POW10(-eqc);
+    cgs_to_SI = pow(CUBE(CM_TO_M) / ERG_TO_JOULE, m->mol_nnuclei[im] - 1);
+    break;
+  }
+  return eqc * cgs_to_SI;
+}
+
+/* Crout LU with implicit pivoting and one step of iterative improvement,
+   rh/ludcmp.c:36-177 (Press et al., Numerical Recipes) */
+#define NEQ_MAX 48
+static void lu_decomp(int N, double A[][NEQ_MAX], int *index)
+{
+  int i, j, k, imax = 0;
+  double big, dum, sum, temp, vv[NEQ_MAX];
+  for (i = 0; i < N; i++) {
+    big = 0.0;
+    for (j = 0; j < N; j++) if ((temp = fabs(A[i][j])) > big) big = temp;
+    vv[i] = 1.0 / big;
+  }
+  for (j = 0; j < N; j++) {
+    for (i = 0; i < j; i++) {
+      sum = A[i][j];
+      for (k = 0; k < i; k++) sum -= A[i][k] * A[k][j];
+      A[i][j] = sum;
+    }
+    big = 0.0;
+    for (i = j; i < N; i++) {
+      sum = A[i][j];
+      for (k = 0; k < j; k++) sum -= A[i][k] * A[k][j];
+      A[i][j] = sum;
+      if ((dum = vv[i] * fabs(sum)) >= big) { big = dum; imax = i; }
+    }
+    if (j != imax) {
+      for (k = 0; k < N; k++) { dum = A[imax][k]; A[imax][k] = A[j][k]; A[j][k] = dum; }
+      vv[imax] = vv[j];
+    }
+    index[j] = imax;
+    if (A[j][j] == 0.0) A[j][j] = 1.0e-20;
+    if (j != N) {
+      dum = 1.0 / A[j][j];
+      for (i = j + 1; i < N; i++) A[i][j] *= dum;
+    }
+  }
+}
+
+static void lu_backsubst(int N, double A[][NEQ_MAX], const int *index, double *b)
+{
+  int i, j, ii = -1, ip;
+  double sum;
+  for (i = 0; i < N; i++) {
+    ip = index[i];
+    sum = b[ip];
+    b[ip] = b[i];
+    if (ii >= 0) { for (j = ii; j < i; j++) sum -= A[i][j] * b[j]; }
+    else if (sum) ii = i;
+    b[i] = sum;
+  }
+  for (i = N - 1; i >= 0; i--) {
+    sum = b[i];
+    for (j = i + 1; j < N; j++) sum -= A[i][j] * b[j];
+    b[i] = sum / A[i][i];
+  }
+}
+
+static void solve_linear_improve(int N, double A[][NEQ_MAX], double *b)
+{
+  double Ac[NEQ_MAX][NEQ_MAX], bc[NEQ_MAX], res[NEQ_MAX];
+  int index[NEQ_MAX], i, j;
+  for (i = 0; i < N; i++) { bc[i] = b[i]; for (j = 0; j < N; j++) Ac[i][j] = A[i][j]; }
+  lu_decomp(N, A, index);
+  lu_backsubst(N, A, index, b);
+  for (i = 0; i < N; i++) {
+    res[i] = bc[i];
+    for (j = 0; j < N; j++) res[i] -= Ac[i][j] * b[j];
+  }
+  lu_backsubst(N, A, index, res);
+  for (i = 0; i < N; i++) b[i] += res[i];
+}
+
+/* Chemical equilibrium at one depth, rh/chemequil.c:107-393 (Ng order 0,
+   so Accelerate only stores the solution; MaxChange over the last two).
+   nstar_all holds all level populations of all model atoms (offsets
+   atom_level_ptr) and is rescaled in place for atoms bound in molecules;
+   ntotal[atom] likewise.  Returns nHmin; nmol_out gets molecule numbers. */
+static double chemeq(const rhg_model_t *m, double T, double ne, double nHtot,
+                     double *nstar_all, double *ntotal, double *nmol_out)
+{
+  const double CI = (HPLANCK / (2.0 * PI * M_ELECTRON)) * (HPLANCK / KBOLTZMANN);
+  int Nn = m->nnuclei, Nm = m->nmol, N = Nn + Nm, i, j, nu, niter, ia;
+  double f[NEQ_MAX], n[NEQ_MAX], a[NEQ_MAX], df[NEQ_MAX][NEQ_MAX], Phi[NEQ_MAX],
+         fn0[NEQ_MAX], prev[2][NEQ_MAX], PhiHmin, fHmin, saha, dnmax = 0.0, fraction;
+  int count;
+  for (i = 0; i < N; i++) a[i] = 0.0;     /* calloc'd in the reference */
+  for (i = 0; i < Nn; i++) {
+    ia = m->nucl_atom[i];
+    {
+      int l0 = m->atom_level_ptr[ia], nl = m->atom_level_ptr[ia + 1] - l0;
+      fn0[i] = 0.0;
+      for (j = 0; j < nl; j++) {
+        if (m->level_stage[l0 + j] > 0) break;
+        fn0[i] += nstar_all[l0 + j];
+      }
+      fn0[i] /= ntotal[ia];
+      a[i] = m->atom_abund[ia] * nHtot;
+    }
+  }
+  PhiHmin = 0.25 * pow(CI / T, 1.5) * exp(0.754 * EV / (KBOLTZMANN * T));
+  fHmin = ne * fn0[0] * PhiHmin;
+  for (i = 0; i < Nm; i++) Phi[i] = equilconstant(m, i, T);
+  for (i = 0; i < Nn; i++) n[i] = a[i];
+  for (i = 0; i < Nm; i++) n[Nn + i] = 0.0;
+  count = 1;
+  for (i = 0; i < N; i++) prev[0][i] = n[i];
+
+  niter = 1;
+  while (niter <= N_MAX_CHEM_ITER) {
+    for (i = 0; i < N; i++) {
+      f[i] = n[i] - a[i];
+      for (j = 0; j < N; j++) df[i][j] = 0.0;
+      df[i][i] = 1.0;
+    }
+    f[0] += fHmin * n[0];
+    df[0][0] += fHmin;
+    for (i = 0; i < Nm; i++) {
+      int nel = m->mol_nel[i];
+      saha = Phi[i];
+      for (j = 0; j < nel; j++) {
+        nu = m->mol_el_nucl[4 * i + j];
+        saha *= pow(fn0[nu] * n[nu], m->mol_el_count[4 * i + j]);
+        f[nu] += m->mol_el_count[4 * i + j] * n[Nn + i];
+      }
+      saha /= pow(ne, m->mol_charge[i]);
+      f[Nn + i] -= saha;
+      for (j = 0; j < nel; j++) {
+        nu = m->mol_el_nucl[4 * i + j];
+        df[nu][Nn + i] += m->mol_el_count[4 * i + j];
+        df[Nn + i][nu] = -saha * (m->mol_el_count[4 * i + j] / n[nu]);
+      }
+    }
+    solve_linear_improve(N, df, f);
+    for (i = 0; i < N; i++) n[i] -= f[i];
+    /* Accelerate with order 0: store, count++ */
+    for (i = 0; i < N; i++) prev[count % 2][i] = n[i];
+    count++;
+    {
+      const double *old = prev[(count - 2) % 2], *nw = prev[(count - 1) % 2];
+      dnmax = 0.0;
+      for (i = 0; i < N; i++)
+        if (nw[i]) dnmax = MAXV(dnmax, fabs((nw[i] - old[i]) / nw[i]));
+    }
+    if (dnmax <= CHEM_ITER_LIMIT) break;
+    niter++;
+  }
+  for (i = 0; i < Nn; i++) {
+    ia = m->nucl_atom[i];
+    {
+      int l0 = m->atom_level_ptr[ia], nl = m->atom_level_ptr[ia + 1] - l0;
+      fraction = n[i] / ntotal[ia];
+      for (j = 0; j < nl; j++) nstar_all[l0 + j] *= fraction;
+      ntotal[ia] = n[i];
+    }
+  }
+  for (i = 0; i < Nm; i++) nmol_out[i] = n[Nn + i];
+  return ne * (n[0] * PhiHmin);
+}
+
+/* Element stage populations from Kurucz partition functions,
+   rh/ltepops.c:116-164 (one depth); nst[Nstage] */
+static void ltepops_elem(const rhg_model_t *m, int ie, double T, double ne,
+                         double nHtot, double *nst)
+{
+  const double C1 = (HPLANCK / (2.0 * PI * M_ELECTRON)) * (HPLANCK / KBOLTZMANN);
+  int ns = m->el_nstage[ie], i, npf = m->npf;
+  const double *lp = m->el_logpf + m->el_pf_ptr[ie];
+  const double *ip = m->el_ionpot + m->el_pf_ptr[ie];
+  double CT_ne = 2.0 * pow(C1 / T, -1.5) / ne, sum = 1.0, Uk, Ukp1;
+  nst[0] = 1.0;
+  Uk = linear1(npf, m->Tpf, lp, T);
+  for (i = 1; i < ns; i++) {
+    Ukp1 = linear1(npf, m->Tpf, lp + i * npf, T);
+    nst[i] = nst[i - 1] * CT_ne * exp(Ukp1 - Uk - ip[i - 1] / (KBOLTZMANN * T));
+    sum += nst[i];
+    Uk = Ukp1;
+  }
+  nst[0] = m->el_abund[ie] * nHtot / sum;
+  for (i = 1; i < ns; i++) nst[i] *= nst[0];
+}
+
+/* ------------------------------------------------------------------ */
+/* one column                                                          */
+
+typedef struct {
+  int Ndep;
+  double *T, *ne, *vz, *vturb, *B, *nHtot, *cosg, *cos2x, *sin2x;
+  double *tau_ref, *cmass, *height;
+  double *nstar;        /* [Ndep][nlevel] */
+  double *ntotal;       /* [Ndep][natom]  */
+  double *nHmin, *nH2, *nOH, *nCH, *nH0, *np, *npstar;
+  double *nel;          /* [Ndep][nel][NSTAGE_MAX] */
+  int moving;
+} column_t;
+
+#define NSTAGE_MAX 8
+
+static void background_continuum(const rhg_model_t *m, const column_t *c, int ls,
+                                 double *chi_ai, double *eta_ai)
+{
+  /* rh/background.c:329-465 (scattering terms omitted: in LTE they enter
+     neither the opacity nor the source function, background.c:462-465) */
+  const double *lc = m->lamconst + (size_t) ls * RHG_LC_NCONST;
+  const double lambda = lc[RHG_LC_LAMBDA];
+  const int nlev = m->atom_level_ptr[m->natom];
+  double C0 = SQ(Q_ELECTRON) / (4.0 * PI * EPSILON_0) / sqrt(M_ELECTRON);
+  double sigma_ff = 4.0 / 3.0 * sqrt(2.0 * PI / (3.0 * KBOLTZMANN)) * CUBE(C0) /
+                    (HPLANCK * CLIGHT);
+  int k, cp0 = m->lam_cont_ptr[ls], cp1 = m->lam_cont_ptr[ls + 1], e;
+  int hasH = 0, hasM = 0;
+  for (e = cp0; e < cp1; e++) {
+    if (m->cont_atom[m->lam_cont_idx[e]] == 0) hasH = 1; else hasM = 1;
+  }
+  (void) hasM;
+  for (k = 0; k < c->Ndep; k++) {
+    double T = c->T[k], ne = c->ne[k], Bnu, chi, eta, stim, x;
+    const double *ns = c->nstar + (size_t) k * nlev;
+    chi_ai[k] = 0.0;
+    eta_ai[k] = 0.0;
+    {
+      double hc_Tkla = lc[RHG_LC_HC_KLA] / T;
+      Bnu = (hc_Tkla <= MAX_EXPONENT) ? lc[RHG_LC_TWOHNU3_C2] / (exp(hc_Tkla) - 1.0) : 0.0;
+    }
+    /* H- bound-free, rh/hydrogen.c:310-364 */
+    if (lc[RHG_LC_HMIN_BF_ON] != 0.0) {
+      stim = exp(-lc[RHG_LC_HC_KLA] / T);
+      chi_ai[k] += c->nHmin[k] * (1.0 - stim) * lc[RHG_LC_HMIN_BF_ALPHA];
+      eta_ai[k] += c->nHmin[k] * lc[RHG_LC_TWOHNU3_C2] * stim * lc[RHG_LC_HMIN_BF_ALPHA];
+    }
+    /* H- free-free, rh/hydrogen.c:370-548 */
+    if (lc[RHG_LC_HMIN_FF_LIDX] >= 0.0) {
+      double ti = clamp_index(NTHETA, m->hminff_theta, THETA0 / T);
+      double pe = ne * KBOLTZMANN * T;
+      double kappa = bilinear(NTHETA, NFF, m->hminff_kappa, ti, lc[RHG_LC_HMIN_FF_LIDX]);
+      chi = (c->nH0[k] * 1.0E-29) * pe * kappa;
+    } else {
+      double Ck = (KBOLTZMANN * THETA0 * 1.0E-32), theta_n = 1.0,
+             sqrt_theta = sqrt(THETA0 / T);
+      int n;
+      chi = 0.0;
+      for (n = 1; n < 6; n++) {
+        theta_n *= sqrt_theta;
+        chi += theta_n * lc[RHG_LC_HFF_LONG0 + n - 1];
+      }
+      chi *= c->nH0[k] * (ne * Ck);
+    }
+    chi_ai[k] += chi;
+    eta_ai[k] += chi * Bnu;
+    /* OH and CH bound-free, rh/ohchbf.c:33-394, 402-690 */
+    if (lc[RHG_LC_OH_EIDX] >= 0.0) {
+      if (T < m->oh_T[0] || T > m->oh_T[NTOH - 1]) { chi = 0.0; eta = 0.0; }
+      else {
+        /* at T equal to the last table temperature the reference reads
+           one element past the table, multiplied by (T - T[i2]) = 0 */
+        int i2 = locate_asc(NTOH, m->oh_T, T);
+        double ti = (i2 == NTOH - 1) ? (double) i2 :
+          (double) i2 + (T - m->oh_T[i2]) / (m->oh_T[i2 + 1] - m->oh_T[i2]);
+        double kappa = exp(LG10 * bilinear(NTOH, NEOH, m->oh_cross, ti,
+                                           lc[RHG_LC_OH_EIDX])) * SQ(CM_TO_M);
+        stim = exp(-lc[RHG_LC_HC_KLA] / T);
+        chi = c->nOH[k] * (1.0 - stim) * kappa;
+        eta = c->nOH[k] * lc[RHG_LC_TWOHNU3_C2] * stim * kappa;
+      }
+      chi_ai[k] += chi;
+      eta_ai[k] += eta;
+    }
+    if (lc[RHG_LC_CH_EIDX] >= 0.0) {
+      if (T < m->ch_T[0] || T > m->ch_T[NTCH - 1]) { chi = 0.0; eta = 0.0; }
+      else {
+        /* at T equal to the last table temperature the reference reads
+           one element past the table, multiplied by (T - T[i2]) = 0 */
+        int i2 = locate_asc(NTCH, m->ch_T, T);
+        double ti = (i2 == NTCH - 1) ? (double) i2 :
+          (double) i2 + (T - m->ch_T[i2]) / (m->ch_T[i2 + 1] - m->ch_T[i2]);
+        double kappa = exp(LG10 * bilinear(NTCH, NECH, m->ch_cross, ti,
+                                           lc[RHG_LC_CH_EIDX])) * SQ(CM_TO_M);
+        stim = exp(-lc[RHG_LC_HC_KLA] / T);
+        chi = c->nCH[k] * (1.0 - stim) * kappa;
+        eta = c->nCH[k] * lc[RHG_LC_TWOHNU3_C2] * stim * kappa;
+      }
+      chi_ai[k] += chi;
+      eta_ai[k] += eta;
+    }
+    /* H bound-free, rh/hydrogen.c:137-208 */
+    if (hasH) {
+      double expla;
+      chi = eta = 0.0;
+      for (e = cp0; e < cp1; e++) {
+        int ci = m->lam_cont_idx[e];
+        if (m->cont_atom[ci] != 0) continue;
+        {
+          double sigma = m->lam_cont_alpha[e], gijk;
+          int li = m->atom_level_ptr[0] + m->cont_i[ci];
+          expla = exp(-lc[RHG_LC_BF_HC_KLA] / T);
+          gijk = ns[li] / c->npstar[k] * expla;
+          chi += sigma * (1.0 - expla) * ns[li];
+          eta += lc[RHG_LC_BF_TWOHNU3_C2] * gijk * sigma * c->np[k];
+        }
+      }
+      chi_ai[k] += chi;
+      eta_ai[k] += eta;
+    }
+    /* H free-free, rh/hydrogen.c:213-243, Gaunt factor :264-281 */
+    {
+      double y = (lc[RHG_LC_GFF_CY] * T) / (HPLANCK * CLIGHT), x3 = lc[RHG_LC_GFF_X3], g_ff;
+      double gIII = 1.0 + 0.1728 * x3 * (1.0 + y) -
+                    0.0496 * SQ(x3) * (1.0 + (1.0 + y) * 0.33333333 * y);
+      g_ff = (gIII > 1.0) ? gIII : 1.0;
+      stim = 1.0 - exp(-lc[RHG_LC_HC_KLA] / T);
+      chi = sigma_ff / sqrt(T) * lc[RHG_LC_HFF_NU3] * ne * c->np[k] * stim * g_ff;
+      chi_ai[k] += chi;
+      eta_ai[k] += chi * Bnu;
+    }
+    /* H2+ free-free, rh/hydrogen.c:797-932 */
+    if (lc[RHG_LC_H2P_LIDX] >= 0.0) {
+      double ti = clamp_index(NTEMP_H2P, m->h2pff_temp, T);
+      double kappa = bilinear(NTEMP_H2P, NFF_H2P, m->h2pff_kappa, ti, lc[RHG_LC_H2P_LIDX]);
+      chi = (c->nH0[k] * 1.0E-29) * (c->np[k] * 1.0E-20) * kappa;
+      chi_ai[k] += chi;
+      eta_ai[k] += chi * Bnu;
+    }
+    /* H2- free-free, rh/hydrogen.c:620-793 */
+    if (lc[RHG_LC_H2M_LIDX] >= 0.0) {
+      if (c->nH2[k] > 0.0) {
+        double ti = clamp_index(NTHETA_H2, m->h2mff_theta, THETA0 / T);
+        double pe = ne * KBOLTZMANN * T;
+        double kappa = bilinear(NTHETA_H2, NFF_H2, m->h2mff_kappa, ti, lc[RHG_LC_H2M_LIDX]);
+        chi = (c->nH2[k] * 1.0E-29) * pe * kappa;
+      } else chi = 0.0;
+      chi_ai[k] += chi;
+      eta_ai[k] += chi * Bnu;
+    }
+    /* metal bound-free, rh/metal.c:69-172 */
+    {
+      double expla = 0.0;
+      int have = 0;
+      chi = eta = 0.0;
+      for (e = cp0; e < cp1; e++) {
+        int ci = m->lam_cont_idx[e], ia = m->cont_atom[ci];
+        int l0, li, lj;
+        double gijk, alpha_la;
+        if (ia == 0) continue;
+        if (!have) { expla = exp(-lc[RHG_LC_BF_HC_KLA] / T); have = 1; }
+        l0 = m->atom_level_ptr[ia];
+        li = l0 + m->cont_i[ci];
+        lj = l0 + m->cont_j[ci];
+        alpha_la = m->lam_cont_alpha[e];
+        gijk = ns[li] / ns[lj] * expla;
+        chi += alpha_la * (1.0 - expla) * ns[li];
+        eta += lc[RHG_LC_BF_TWOHNU3_C2] * gijk * alpha_la * ns[lj];
+      }
+      chi_ai[k] += chi * 1.0;
+      eta_ai[k] += eta * 1.0;
+    }
+    (void) x;
+  }
+}
+
+/* Kurucz-line opacity for the ray leaving the top (to_obs = 1),
+   rh/kurucz.c:511-827.  Writes chi/eta [4][Ndep] from zero; returns 0 if
+   the wavelength has no contributing line. */
+static int line_opacity(const rhg_model_t *m, const column_t *c, int ls,
+                        double mu, double *chi, double *eta)
+{
+  const double lambda = m->lamconst[(size_t) ls * RHG_LC_NCONST + RHG_LC_LAMBDA];
+  int p0 = m->lam_line_ptr[ls], p1 = m->lam_line_ptr[ls + 1], e, k, N = c->Ndep;
+  const int nlev = m->atom_level_ptr[m->natom];
+  if (p1 == p0) return 0;
+  for (k = 0; k < 4 * N; k++) { chi[k] = 0.0; eta[k] = 0.0; }
+  for (e = p0; e < p1; e++) {
+    int l = m->lam_line_idx[e], ie = m->line_el[l], st = m->line_stage[l];
+    double lambda0 = m->line_lambda0[l], weight = m->el_weight[ie];
+    double vtherm = 2.0 * KBOLTZMANN / (AMU * weight);
+    const double *lp = m->el_logpf + m->el_pf_ptr[ie] + (size_t) st * m->npf;
+    int c0 = m->line_comp_ptr[l], c1 = m->line_comp_ptr[l + 1];
+    for (k = 0; k < N; k++) {
+      double T = c->T[k], vbroad, v, sv, phi, phi_Q = 0, phi_U = 0, phi_V = 0;
+      double pf = linear1(m->npf, m->Tpf, lp, T);
+      vbroad = sqrt(vtherm * T + SQ(c->vturb[k]));
+      v = (lambda / lambda0 - 1.0) * CLIGHT / vbroad;
+      if (c->moving) v += (mu * c->vz[k]) / vbroad;
+      sv = 1.0 / (SQRTPI * vbroad);
+      if (m->line_Grad[l] != 0.0) {
+        double GvdW, adamp;
+        switch (m->line_vdw[l]) {
+        case RHG_VDW_UNSOLD: GvdW = m->line_cross[l] * pow(T, 0.3); break;
+        case RHG_VDW_BARKLEM:
+          GvdW = m->line_cross[l] * pow(T, (1.0 - m->line_alpha[l]) / 2.0); break;
+        default: GvdW = m->line_GvdW[l]; break;
+        }
+        adamp = (m->line_Grad[l] + m->line_GStark[l] * c->ne[k] +
+                 GvdW * (c->nHtot[k] - c->np[k])) *
+                (lambda0 * NM_TO_M) / (4.0 * PI * vbroad);
+        if (m->line_polar[l]) {
+          double sin2_gamma = 1.0 - SQ(c->cosg[k]);
+          double vB = m->line_larmor[l] * c->B[k] / vbroad, sign = 1.0;
+          double phi_sm = 0, phi_pi = 0, phi_sp = 0, phi_sigma, phi_delta, H, F;
+          int nz;
+          for (nz = c0; nz < c1; nz++) {
+            H = rho_voigt_humlicek(adamp, v - m->comp_shift[nz] * vB, &F);
+            switch (m->comp_q[nz]) {
+            case -1: phi_sm += m->comp_strength[nz] * H; break;
+            case 0: phi_pi += m->comp_strength[nz] * H; break;
+            case 1: phi_sp += m->comp_strength[nz] * H; break;
+            }
+          }
+          phi_sigma = phi_sp + phi_sm;
+          phi_delta = 0.5 * phi_pi - 0.25 * phi_sigma;
+          phi = (phi_delta * sin2_gamma + 0.5 * phi_sigma) * sv;
+          phi_Q = sign * phi_delta * sin2_gamma * c->cos2x[k] * sv;
+          phi_U = phi_delta * sin2_gamma * c->sin2x[k] * sv;
+          phi_V = sign * 0.5 * (phi_sp - phi_sm) * c->cosg[k] * sv;
+        } else {
+          phi = rho_voigt_armstrong(adamp, v) * sv;
+        }
+      } else {
+        phi = (fabs(v) <= MAX_GAUSS_DOPPLER) ? exp(-v * v) : 0.0;
+        phi = phi * sv;
+      }
+      if (phi) {
+        double kT = 1.0 / (KBOLTZMANN * T);
+        double ni_gi = c->nel[((size_t) k * m->nel + ie) * NSTAGE_MAX + st] *
+                       exp(-m->line_Ei[l] * kT - pf);
+        double nj_gj = ni_gi * exp(-m->line_hc_la[l] * kT);
+        double chi_l = m->line_Bijhc4pi[l] * (ni_gi - nj_gj);
+        double eta_l = m->line_Bijhc4pi[l] * m->line_twohnu3_c2[l] * nj_gj;
+        chi[k] += chi_l * phi;
+        eta[k] += eta_l * phi;
+        if (m->line_polar[l] && m->line_Grad[l]) {
+          chi[N + k] += chi_l * phi_Q;
+          chi[2 * N + k] += chi_l * phi_U;
+          chi[3 * N + k] += chi_l * phi_V;
+          eta[N + k] += eta_l * phi_Q;
+          eta[2 * N + k] += eta_l * phi_U;
+          eta[3 * N + k] += eta_l * phi_V;
+        }
+      }
+    }
+  }
+  (void) nlev;
+  return 1;
+}
+
+static int synth_column_ex(const rhg_model_t *m, const rhg_columns_t *cols, int ic,
+                           int fp32, double *stokes, double *height_out,
+                           double *chiref_out, int diag_ls, double *diag_chi,
+                           double *diag_eta, double *diag_pop)
+{
+  const int N = cols->ndep[ic], Nmax = cols->ndep_max, nspect = m->nspect;
+  const int nlev = m->atom_level_ptr[m->natom], natom = m->natom;
+  const double *atm = cols->atmos + (size_t) ic * 9 * Nmax;
+  const double mu = cols->mu;
+  column_t c;
+  double *mem, *chi_ai, *eta_ai, *chi_c, *eta_c, *lchi, *leta, *chiref, *S4, *I4,
+         *molbuf, *scr;
+  int k, ls, i, nout = 0;
+
+  memset(&c, 0, sizeof(c));
+  c.Ndep = N;
+  mem = calloc((size_t) N * (24 + nlev + natom + m->nel * NSTAGE_MAX + 30) +
+               (size_t) 4 * N * 8 + 64, sizeof(double));
+  if (!mem) return -1;
+  scr = mem;
+#define TAKE(p, cnt) do { p = scr; scr += (cnt); } while (0)
+  TAKE(c.T, N); TAKE(c.ne, N); TAKE(c.vz, N); TAKE(c.vturb, N); TAKE(c.B, N);
+  TAKE(c.nHtot, N); TAKE(c.cosg, N); TAKE(c.cos2x, N); TAKE(c.sin2x, N);
+  TAKE(c.tau_ref, N); TAKE(c.cmass, N); TAKE(c.height, N);
+  TAKE(c.nstar, (size_t) N * nlev); TAKE(c.ntotal, (size_t) N * natom);
+  TAKE(c.nHmin, N); TAKE(c.nH2, N); TAKE(c.nOH, N); TAKE(c.nCH, N); TAKE(c.nH0, N);
+  TAKE(c.np, N); TAKE(c.npstar, N);
+  TAKE(c.nel, (size_t) N * m->nel * NSTAGE_MAX);
+  TAKE(chi_ai, N); TAKE(eta_ai, N); TAKE(chi_c, 4 * N); TAKE(eta_c, 4 * N);
+  TAKE(lchi, 4 * N); TAKE(leta, 4 * N); TAKE(chiref, N); TAKE(S4, 4 * N);
+  TAKE(I4, 4 * N); TAKE(molbuf, 32);
+#undef TAKE
+
+  /* units and scales, rh/rhf1d/pyrh_compute1dray.c:230-278 */
+  for (k = 0; k < N; k++) {
+    double sc = atm[k];
+    c.T[k] = atm[Nmax + k];
+    c.ne[k] = atm[2 * Nmax + k] / CUBE(CM_TO_M);
+    c.vz[k] = atm[3 * Nmax + k] * KM_TO_M;
+    c.vturb[k] = atm[4 * Nmax + k] * KM_TO_M;
+    c.B[k] = atm[5 * Nmax + k] / 1e4;
+    c.nHtot[k] = atm[8 * Nmax + k] / CUBE(CM_TO_M);
+    if (cols->atm_scale == 0) c.tau_ref[k] = POW10(sc);
+    else if (cols->atm_scale == 1) c.cmass[k] = POW10(sc) * (G_TO_KG / SQ(CM_TO_M));
+    else c.height[k] = sc * KM_TO_M;
+    /* Bproject, rh/rhf1d/project.c:38-80 */
+    {
+      double gam = atm[6 * Nmax + k], chB = atm[7 * Nmax + k];
+      if (mu == 1.0) {
+        c.cosg[k] = cos(gam);
+        c.cos2x[k] = cos(2.0 * chB);
+        c.sin2x[k] = sin(2.0 * chB);
+      } else {
+        double mux = sqrt(1.0 - SQ(mu)), muy = 0.0, csc_theta = 1.0 / sqrt(1.0 - SQ(mu));
+        double sin_gamma = sin(gam), bx = sin_gamma * cos(chB), by = sin_gamma * sin(chB),
+               bz = cos(gam), b3 = mux * bx + muy * by + mu * bz,
+               b1 = csc_theta * (bz - mu * b3), b2 = csc_theta * (muy * bx - mux * by);
+        c.cosg[k] = b3;
+        c.cos2x[k] = (SQ(b1) - SQ(b2)) / (1.0 - SQ(b3));
+        c.sin2x[k] = 2.0 * b1 * b2 / (1.0 - SQ(b3));
+      }
+    }
+  }
+  c.moving = 0;
+  for (k = 0; k < N; k++)
+    if (fabs(c.vz[k]) >= m->vmacro_tresh) { c.moving = 1; break; }
+
+  /* LTE populations and chemical equilibrium, rh/background.c:164-169 */
+  for (k = 0; k < N; k++) {
+    double *ns = c.nstar + (size_t) k * nlev, *nt = c.ntotal + (size_t) k * natom;
+    int ia;
+    for (ia = 0; ia < natom; ia++) {
+      nt[ia] = m->atom_abund[ia] * c.nHtot[k];
+      ltepops(m, ia, c.T[k], c.ne[k], nt[ia], ns + m->atom_level_ptr[ia]);
+    }
+    c.nHmin[k] = chemeq(m, c.T[k], c.ne[k], c.nHtot[k], ns, nt, molbuf);
+    c.nH2[k] = m->mol_h2 >= 0 ? molbuf[m->mol_h2] : 0.0;
+    c.nOH[k] = m->mol_oh >= 0 ? molbuf[m->mol_oh] : 0.0;
+    c.nCH[k] = m->mol_ch >= 0 ? molbuf[m->mol_ch] : 0.0;
+    c.nH0[k] = ns[m->atom_level_ptr[0]];
+    c.np[k] = ns[m->atom_level_ptr[1] - 1];
+    c.npstar[k] = c.np[k];
+    for (i = 0; i < m->nel; i++)
+      ltepops_elem(m, i, c.T[k], c.ne[k], c.nHtot[k],
+                   c.nel + ((size_t) k * m->nel + i) * NSTAGE_MAX);
+  }
+
+  /* background at the reference wavelength, then convertScales
+     (rh/rhf1d/multiatmos.c:100-177) */
+  background_continuum(m, &c, m->ref_index, chiref, eta_ai);
+  if (line_opacity(m, &c, m->ref_index, mu, lchi, leta))
+    for (k = 0; k < N; k++) chiref[k] += lchi[k];
+  {
+    double *rho = S4, h_zero, unity = 1.0;   /* S4 as scratch */
+    for (k = 0; k < N; k++) rho[k] = (AMU * m->wght_per_H) * c.nHtot[k];
+    if (cols->atm_scale == 0) {
+      c.height[0] = 0.0;
+      c.cmass[0] = (c.tau_ref[0] / chiref[0]) * rho[0];
+      for (k = 1; k < N; k++) {
+        c.height[k] = c.height[k - 1] - 2.0 * (c.tau_ref[k] - c.tau_ref[k - 1]) /
+                      (chiref[k - 1] + chiref[k]);
+        c.cmass[k] = c.cmass[k - 1] + 0.5 * (rho[k - 1] + rho[k]) *
+                     (c.height[k - 1] - c.height[k]);
+      }
+    } else if (cols->atm_scale == 1) {
+      c.height[0] = 0.0;
+      c.tau_ref[0] = chiref[0] / rho[0] * c.cmass[0];
+      for (k = 1; k < N; k++) {
+        c.height[k] = c.height[k - 1] - 2.0 * (c.cmass[k] - c.cmass[k - 1]) /
+                      (rho[k - 1] + rho[k]);
+        c.tau_ref[k] = c.tau_ref[k - 1] + 0.5 * (chiref[k - 1] + chiref[k]) *
+                       (c.height[k - 1] - c.height[k]);
+      }
+    }
+    if (cols->atm_scale == 0 || cols->atm_scale == 1) {
+      h_zero = linear1(N, c.tau_ref, c.height, unity);
+      for (k = 0; k < N; k++) c.height[k] = c.height[k] - h_zero;
+    }
+  }
+  if (height_out) for (k = 0; k < N; k++) height_out[k] = c.height[k];
+  if (chiref_out) for (k = 0; k < N; k++) chiref_out[k] = chiref[k];
+  if (diag_pop) {
+    for (k = 0; k < N; k++) {
+      diag_pop[0 * N + k] = c.nHmin[k];
+      diag_pop[1 * N + k] = c.nH2[k];
+      diag_pop[2 * N + k] = c.nOH[k];
+      diag_pop[3 * N + k] = c.nCH[k];
+      diag_pop[4 * N + k] = c.nH0[k];
+      diag_pop[5 * N + k] = c.np[k];
+    }
+  }
+  if (diag_ls >= 0) {
+    background_continuum(m, &c, diag_ls, chi_ai, eta_ai);
+    for (k = 0; k < N; k++) { diag_chi[k] = chi_ai[k]; diag_eta[k] = eta_ai[k]; }
+    for (k = N; k < 4 * N; k++) { diag_chi[k] = 0.0; diag_eta[k] = 0.0; }
+    if (line_opacity(m, &c, diag_ls, mu, lchi, leta))
+      for (k = 0; k < 4 * N; k++) { diag_chi[4 * N + k] = lchi[k]; diag_eta[4 * N + k] = leta[k]; }
+    else
+      for (k = 0; k < 4 * N; k++) { diag_chi[4 * N + k] = 0.0; diag_eta[4 * N + k] = 0.0; }
+    free(mem);
+    return 0;
+  }
+
+  /* formal solution of the emergent ray per wavelength (formal.c:157-275) */
+  for (ls = 0; ls < nspect; ls++) {
+    int has, solver, n;
+    double B2[2], Iout[4] = {0, 0, 0, 0};
+    if (ls == m->ref_index) continue;
+    background_continuum(m, &c, ls, chi_ai, eta_ai);
+    has = line_opacity(m, &c, ls, mu, lchi, leta);
+    for (k = 0; k < N; k++) { chi_c[k] = chi_ai[k]; eta_c[k] = eta_ai[k]; }
+    for (k = N; k < 4 * N; k++) { chi_c[k] = 0.0; eta_c[k] = 0.0; }
+    if (has) for (k = 0; k < 4 * N; k++) { chi_c[k] += lchi[k]; eta_c[k] += leta[k]; }
+    solver = m->solver[ls];
+    if (solver == RHG_SOLVER_BEZIER && !c.moving) solver = RHG_SOLVER_FEAUTRIER;
+    B2[0] = rho_planck(c.T[N - 2], m->lamconst[(size_t) ls * RHG_LC_NCONST]);
+    B2[1] = rho_planck(c.T[N - 1], m->lamconst[(size_t) ls * RHG_LC_NCONST]);
+    if (solver == RHG_SOLVER_STOKES) {
+      for (n = 0; n < 4; n++)
+        for (k = 0; k < N; k++) S4[n * N + k] = (0.0 + eta_c[n * N + k]) / chi_c[k];
+      rho_formal_stokes(N, c.height, mu, chi_c, S4, chi_c + N, B2, fp32, I4);
+      for (n = 0; n < 4; n++) Iout[n] = I4[n * N];
+    } else if (solver == RHG_SOLVER_BEZIER) {
+      for (k = 0; k < N; k++) S4[k] = (0.0 + eta_c[k]) / chi_c[k];
+      rho_formal_bezier(N, c.height, mu, chi_c, S4, B2, I4);
+      Iout[0] = I4[0];
+    } else {
+      for (k = 0; k < N; k++) S4[k] = (0.0 + eta_c[k] + 0.0) / chi_c[k];
+      Iout[0] = rho_formal_feautrier(N, c.height, mu, chi_c, S4, B2);
+    }
+    for (n = 0; n < 4; n++) stokes[n * (nspect - 1) + nout] = Iout[n];
+    nout++;
+  }
+  free(mem);
+  return 0;
+}
+
+static int synth_column(const rhg_model_t *m, const rhg_columns_t *cols, int ic,
+                        int fp32, double *stokes, double *height_out,
+                        double *chiref_out)
+{
+  return synth_column_ex(m, cols, ic, fp32, stokes, height_out, chiref_out,
+                         -1, NULL, NULL, NULL);
+}
+
+/* Diagnostic: continuum (chi[0:N], eta[0:N]) and line (chi/eta[4N:8N])
+   opacity of column ic at wavelength ls for the emergent ray, and the
+   populations nHmin, nH2, nOH, nCH, nH(1), np ([6][N]). */
+int rho_background(const rhg_model_t *m, const rhg_columns_t *cols, int ic, int ls,
+                   double *chi8, double *eta8, double *pop6)
+{
+  return synth_column_ex(m, cols, ic, 0, NULL, NULL, NULL, ls, chi8, eta8, pop6);
+}
+
+int rho_synth_lte(const rhg_model_t *m, const rhg_columns_t *cols,
+                  int inverse_fp32, int nthreads, double *stokes_out,
+                  double *height_out, double *chiref_out)
+{
+  int ic, err = 0;
+  const int nout = m->nspect - 1;
+  if (m->nnuclei + m->nmol > NEQ_MAX) return RHG_EUNSUPPORTED;
+  for (ic = 0; ic < m->nel; ic++) if (m->el_nstage[ic] > NSTAGE_MAX) return RHG_EUNSUPPORTED;
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads) reduction(|:err)
+#endif
+  for (ic = 0; ic < cols->ncol; ic++) {
+    if (synth_column(m, cols, ic, inverse_fp32, stokes_out + (size_t) ic * 4 * nout,
+                     height_out ? height_out + (size_t) ic * cols->ndep_max : NULL,
+                     chiref_out ? chiref_out + (size_t) ic * cols->ndep_max : NULL))
+      err |= 1;
+  }
+  (void) nthreads;
+  return err ? RHG_ENOMEM : RHG_OK;
+}
+
+/* layout check for the host bindings */
+int rho_model_sizeof(void) { return (int) sizeof(rhg_model_t); }
+int rho_columns_sizeof(void) { return (int) sizeof(rhg_columns_t); }

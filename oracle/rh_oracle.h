@@ -1,0 +1,55 @@
+/* Test oracle: CPU restatement of the reference RH algorithm for pyrh's
+ * LTE synthesis path (rhf1d with no active atom).  TEST INFRASTRUCTURE
+ * ONLY: it is the checker for the GPU library and the CPU baseline of the
+ * benchmark; nothing in the product links or calls it.
+ *
+ * Each function names the reference file:line it follows.  It reads the
+ * same host tables the GPU library is given (rhg_model_t), so the model
+ * builder is exercised by the comparison with the reference itself.
+ */
+#ifndef RH_ORACLE_H
+#define RH_ORACLE_H
+
+#include "../include/pyrh_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Full LTE synthesis of a batch of columns (per column: equilibria,
+ * background, convertScales, formal solution of the emergent ray).
+ * stokes_out [ncol][4][nspect-1]; height_out/chiref_out [ncol][ndep_max]
+ * may be NULL.  inverse_fp32 != 0 inverts the 4x4 DELO matrix in float as
+ * the shipped reference does (rh/bezier_aux.c:235-328); 0 inverts in
+ * double.  nthreads <= 0 uses all OpenMP threads. */
+int rho_synth_lte(const rhg_model_t *m, const rhg_columns_t *cols,
+                  int inverse_fp32, int nthreads, double *stokes_out,
+                  double *height_out, double *chiref_out);
+
+/* Single-function entry points for closed-form and invariant tests. */
+double rho_voigt_humlicek(double a, double v, double *F);
+double rho_voigt_armstrong(double a, double v);
+void rho_bezier3_coeffs(double dt, double *out5); /* alpha,beta,gamma,theta,eps */
+void rho_w3(double dtau, double *w);
+double rho_planck(double T, double lambda_nm);
+
+/* Formal solvers on given opacity and source functions along the ray
+ * leaving the top (to_obs), height z [m] decreasing with index, with the
+ * thermalised lower boundary built from B2 = {B(T[n-2]), B(T[n-1])}
+ * (B2 == NULL: zero incoming intensity).  S4 is [4][n] (source functions
+ * of I,Q,U,V), chiQUV is [3][n] (chi_Q,U,V, not divided by chi);
+ * I4 receives I,Q,U,V at every depth, [4][n]; I receives [n]. */
+void rho_formal_stokes(int n, const double *z, double mu, const double *chi,
+                       const double *S4, const double *chiQUV,
+                       const double *B2, int inverse_fp32, double *I4);
+void rho_formal_bezier(int n, const double *z, double mu, const double *chi,
+                       const double *S, const double *B2, double *I);
+double rho_formal_feautrier(int n, const double *z, double mu,
+                            const double *chi, const double *S,
+                            const double *B2);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

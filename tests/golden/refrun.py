@@ -1,0 +1,105 @@
+"""Run the reference RH library built from source (oracle/_ref) on columns.
+
+Test infrastructure only (fixture generation in a container that holds the
+reference sources).  The binary layouts are those documented at the top of
+oracle/ref_harness.c.
+"""
+from __future__ import annotations
+
+import os
+import struct
+import subprocess
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+REPO = Path(__file__).resolve().parents[2]
+REF_DIR = REPO / "oracle" / "_ref"
+DATA_DIR = REPO / "tests" / "golden" / "rhdata"
+MAGIC = 0x4E494852
+
+
+def ref_available(variant: str = "o2") -> bool:
+    return (REF_DIR / f"rhref_run_{variant}").exists()
+
+
+def _write_input(path, atmos, wave, mu, atm_scale):
+    atmos = np.ascontiguousarray(atmos, dtype=np.float64)
+    if atmos.ndim == 2:
+        atmos = atmos[None]
+    ncol, nrow, ndep = atmos.shape
+    assert nrow == 9
+    wave = np.ascontiguousarray(wave, dtype=np.float64)
+    with open(path, "wb") as f:
+        f.write(struct.pack("<5i", MAGIC, ncol, ndep, wave.size, atm_scale))
+        f.write(struct.pack("<d", float(mu)))
+        f.write(wave.tobytes())
+        f.write(atmos.tobytes())
+    return ncol, ndep
+
+
+def read_capture(path):
+    """Return a list of (tag, col, nspect, mu, to_obs, data) records."""
+    out = []
+    raw = Path(path).read_bytes()
+    pos = 0
+    while pos < len(raw):
+        tag = raw[pos:pos + 16].split(b"\0", 1)[0].decode()
+        col, nspect, mu, to_obs, n = struct.unpack_from("<5i", raw, pos + 16)
+        pos += 36
+        data = np.frombuffer(raw, dtype=np.float64, count=n, offset=pos).copy()
+        pos += 8 * n
+        out.append((tag, col, nspect, mu, to_obs, data))
+    return out
+
+
+def run_synth(rundir, atmos, wave, mu=1.0, atm_scale=0, variant="o2",
+              capture=False, timeout=3600):
+    """Call rhf1d() once per column.  Returns (lam, I, Q, U, V[, capture])."""
+    exe = REF_DIR / f"rhref_run_{variant}"
+    env = dict(os.environ, PYRH_PATH=str(DATA_DIR))
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout, fcap = (os.path.join(td, n) for n in ("in", "out", "cap"))
+        ncol, ndep = _write_input(fin, atmos, wave, mu, atm_scale)
+        cmd = [str(exe), str(rundir), fin, fout] + ([fcap] if capture else [])
+        r = subprocess.run(cmd, env=env, capture_output=True, timeout=timeout)
+        if r.returncode != 0:
+            raise RuntimeError(f"reference run failed ({r.returncode}): "
+                               f"{r.stderr.decode()[-2000:]}")
+        raw = Path(fout).read_bytes()
+        pos = 0
+        res = []
+        for _ in range(ncol):
+            (nlw,) = struct.unpack_from("<i", raw, pos)
+            pos += 4
+            arr = np.frombuffer(raw, np.float64, 5 * nlw, pos).reshape(5, nlw)
+            pos += 40 * nlw
+            res.append(arr.copy())
+        res = np.stack(res)
+        lam = res[0, 0]
+        out = (lam, res[:, 1], res[:, 2], res[:, 3], res[:, 4])
+        if capture:
+            out = out + (read_capture(fcap),)
+        return out
+
+
+def run_hse(rundir, logtau, T, pg_top=0.1, variant="o2", timeout=600):
+    """pyrh.hse() on the tau500 scale; returns ne, nHtot, rho, pg [m^-3 ...]
+    exactly as the C entry point leaves them (units of pyrh_hse.c)."""
+    exe = REF_DIR / f"rhref_run_{variant}"
+    env = dict(os.environ, PYRH_PATH=str(DATA_DIR))
+    T = np.atleast_2d(np.asarray(T, np.float64))
+    ncol, ndep = T.shape
+    atmos = np.zeros((ncol, 9, ndep))
+    atmos[:, 0] = logtau
+    atmos[:, 1] = T
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in"), os.path.join(td, "out")
+        _write_input(fin, atmos, np.zeros(1), pg_top, 0)
+        r = subprocess.run([str(exe), "--hse", str(rundir), fin, fout],
+                           env=env, capture_output=True, timeout=timeout)
+        if r.returncode != 0:
+            raise RuntimeError(f"reference hse failed: {r.stderr.decode()[-2000:]}")
+        o = np.fromfile(fout, np.float64).reshape(ncol, 4, ndep)
+    return o
