@@ -224,9 +224,15 @@ int rhg_synth_lte_diag(rhg_context_t *ctx, const rhg_columns_t *cols,
 
 /* Time the kernels of the last rhg_synth_lte call on the context's
  * stream, in milliseconds (HIP events around each stage); out has
- * RHG_NSTAGE entries: equilibria, geometry, formal solution. */
-#define RHG_NSTAGE 3
+ * RHG_NSTAGE entries: column state (equilibria), geometry, continuum,
+ * formal solution. */
+#define RHG_NSTAGE 4
 int rhg_last_timings(rhg_context_t *ctx, double *ms_out);
+
+/* sizeof(rhg_model_t) and sizeof(rhg_columns_t) as compiled, for host
+ * bindings to check their struct layout. */
+int32_t rhg_model_sizeof(void);
+int32_t rhg_columns_sizeof(void);
 
 #ifdef __cplusplus
 }

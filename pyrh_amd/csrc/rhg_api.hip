@@ -1,0 +1,363 @@
+// C ABI of the engine (include/pyrh_amd.h): context, model upload and the
+// batched LTE synthesis launch.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rhg_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHECK(call)                                                             \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return fail(RHG_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t n) {
+    if (n <= bytes) return 0;
+    if (p) (void) hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (hipMalloc(&p, n) != hipSuccess) return -1;
+    bytes = n;
+    return 0;
+  }
+  ~DevBuf() {
+    if (p) (void) hipFree(p);
+  }
+};
+
+}  // namespace
+
+struct rhg_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  rhg::DevModel dm{};
+  std::vector<void *> allocs;   // model arrays
+  int nspect = 0, nout = 0, ncont = 0, nline = 0, natom = 0;
+  int32_t *d_out_ls = nullptr;
+  DevBuf atmos, ndep, state, moving, cchi, ceta, stokes, chiref;
+  hipEvent_t ev[RHG_NSTAGE + 2]{};
+  float last_ms[RHG_NSTAGE + 1]{};
+};
+
+namespace {
+
+template <typename T>
+int upload(rhg_context *c, const T *src, size_t n, const T **dst) {
+  if (n == 0) n = 1;
+  void *p = nullptr;
+  if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return fail(RHG_ENOMEM, "hipMalloc (model)");
+  c->allocs.push_back(p);
+  if (src) {
+    hipError_t e = hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(RHG_EDEVICE, std::string("hipMemcpy (model): ") + hipGetErrorString(e));
+  } else {
+    (void) hipMemset(p, 0, n * sizeof(T));
+  }
+  *dst = (const T *) p;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *rhg_last_error(void) { return g_err.c_str(); }
+
+int32_t rhg_abi_version(void) { return RHG_ABI_VERSION; }
+
+int32_t rhg_model_sizeof(void) { return (int32_t) sizeof(rhg_model_t); }
+int32_t rhg_columns_sizeof(void) { return (int32_t) sizeof(rhg_columns_t); }
+
+int rhg_close(rhg_context_t *ctx) {
+  if (!ctx) return 0;
+  (void) hipSetDevice(ctx->device);
+  if (ctx->stream) (void) hipStreamSynchronize(ctx->stream);
+  for (void *p : ctx->allocs) (void) hipFree(p);
+  if (ctx->d_out_ls) (void) hipFree(ctx->d_out_ls);
+  for (auto &e : ctx->ev)
+    if (e) (void) hipEventDestroy(e);
+  if (ctx->stream) (void) hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return 0;
+}
+
+int rhg_open(const rhg_model_t *m, int device, rhg_context_t **out) {
+  if (!m || !out) return fail(RHG_EINVAL, "null argument");
+  *out = nullptr;
+  if (m->abi_version != RHG_ABI_VERSION) return fail(RHG_EINVAL, "model ABI version mismatch");
+  if (m->nspect < 2 || m->ref_index < 0 || m->ref_index >= m->nspect)
+    return fail(RHG_EINVAL, "bad wavelength grid");
+  if (m->magneto_optical) return fail(RHG_EUNSUPPORTED, "magneto-optical terms are not implemented");
+  if (m->natom < 1 || m->natom > RHG_MAX_ATOM) return fail(RHG_EUNSUPPORTED, "too many model atoms");
+  if (m->nnuclei < 1 || m->nnuclei > rhg::NNUC_MAX) return fail(RHG_EUNSUPPORTED, "too many molecule nuclei");
+  if (m->nmol > rhg::NMOL_MAX) return fail(RHG_EUNSUPPORTED, "too many molecules");
+  for (int i = 0; i < m->nel; i++)
+    if (m->el_nstage[i] > rhg::NSTAGE_MAX) return fail(RHG_EUNSUPPORTED, "too many ionisation stages");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(RHG_EDEVICE, "no HIP device available");
+  if (device < 0 || device >= ndev) return fail(RHG_EINVAL, "bad device index");
+  HIPCHECK(hipSetDevice(device));
+
+  rhg_context *c = new rhg_context();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(RHG_EDEVICE, "hipStreamCreate");
+  }
+  for (auto &e : c->ev)
+    if (hipEventCreate(&e) != hipSuccess) { rhg_close(c); return fail(RHG_EDEVICE, "hipEventCreate"); }
+
+  rhg::DevModel &d = c->dm;
+  const int ns = m->nspect;
+  int rc = 0;
+#define UP(field, n) \
+  if (!rc) rc = upload(c, m->field, (size_t) (n), &d.field)
+  d.stokes_mode = m->stokes_mode;
+  d.magneto_optical = m->magneto_optical;
+  d.vmacro_tresh = m->vmacro_tresh;
+  d.vmicro_char = m->vmicro_char;
+  d.wght_per_H = m->wght_per_H;
+  d.totalAbund = m->totalAbund;
+  d.gravity = m->gravity;
+  d.nspect = ns;
+  d.ref_index = m->ref_index;
+  UP(lamconst, (size_t) ns * RHG_LC_NCONST);
+  UP(solver, ns);
+  UP(lam_line_ptr, ns + 1);
+  UP(lam_line_idx, m->lam_line_ptr[ns]);
+  UP(lam_cont_ptr, ns + 1);
+  UP(lam_cont_idx, m->lam_cont_ptr[ns]);
+  UP(lam_cont_alpha, m->lam_cont_ptr[ns]);
+  d.natom = m->natom;
+  UP(atom_level_ptr, m->natom + 1);
+  UP(level_E, m->atom_level_ptr[m->natom]);
+  UP(level_g, m->atom_level_ptr[m->natom]);
+  UP(level_stage, m->atom_level_ptr[m->natom]);
+  UP(atom_abund, m->natom);
+  d.ncont = m->ncont;
+  UP(cont_atom, m->ncont);
+  UP(cont_i, m->ncont);
+  UP(cont_j, m->ncont);
+  d.nnuclei = m->nnuclei;
+  UP(nucl_atom, m->nnuclei);
+  UP(nucl_elem, m->nnuclei);
+  UP(nucl_abund, m->nnuclei);
+  d.nmol = m->nmol;
+  UP(mol_fit, m->nmol);
+  UP(mol_charge, m->nmol);
+  UP(mol_nnuclei, m->nmol);
+  UP(mol_ediss, m->nmol);
+  UP(mol_tmin, m->nmol);
+  UP(mol_tmax, m->nmol);
+  UP(mol_neqc, m->nmol);
+  UP(mol_eqc, 8 * m->nmol);
+  UP(mol_nel, m->nmol);
+  UP(mol_el_nucl, 4 * m->nmol);
+  UP(mol_el_count, 4 * m->nmol);
+  d.mol_h2 = m->mol_h2;
+  d.mol_oh = m->mol_oh;
+  d.mol_ch = m->mol_ch;
+  d.npf = m->npf;
+  UP(Tpf, m->npf);
+  d.nel = m->nel;
+  size_t npfblk = 1;
+  if (m->nel > 0) npfblk = (size_t) m->el_pf_ptr[m->nel - 1] + (size_t) m->el_nstage[m->nel - 1] * m->npf;
+  UP(el_nstage, m->nel);
+  UP(el_pf_ptr, m->nel);
+  UP(el_logpf, npfblk);
+  UP(el_ionpot, npfblk);
+  UP(el_abund, m->nel);
+  UP(el_weight, m->nel);
+  d.nline = m->nline;
+  UP(line_el, m->nline);
+  UP(line_stage, m->nline);
+  UP(line_lambda0, m->nline);
+  UP(line_Ei, m->nline);
+  UP(line_hc_la, m->nline);
+  UP(line_Bijhc4pi, m->nline);
+  UP(line_twohnu3_c2, m->nline);
+  UP(line_Grad, m->nline);
+  UP(line_GStark, m->nline);
+  UP(line_GvdW, m->nline);
+  UP(line_vdw, m->nline);
+  UP(line_cross, m->nline);
+  UP(line_alpha, m->nline);
+  UP(line_larmor, m->nline);
+  UP(line_polar, m->nline);
+  UP(line_comp_ptr, m->nline + 1);
+  UP(comp_q, m->line_comp_ptr[m->nline]);
+  UP(comp_shift, m->line_comp_ptr[m->nline]);
+  UP(comp_strength, m->line_comp_ptr[m->nline]);
+  UP(hminff_theta, 16);
+  UP(hminff_kappa, 17 * 16);
+  UP(h2mff_theta, 8);
+  UP(h2mff_kappa, 19 * 8);
+  UP(h2pff_temp, 10);
+  UP(h2pff_kappa, 15 * 10);
+  UP(oh_T, 15);
+  UP(oh_cross, 130 * 15);
+  UP(ch_T, 15);
+  UP(ch_cross, 105 * 15);
+#undef UP
+  if (rc) { std::string e = g_err; rhg_close(c); return fail(rc, e); }
+
+  c->nspect = ns;
+  c->nout = ns - 1;
+  c->ncont = m->ncont;
+  c->nline = m->nline;
+  c->natom = m->natom;
+  std::vector<int32_t> ols;
+  for (int i = 0; i < ns; i++)
+    if (i != m->ref_index) ols.push_back(i);
+  if (hipMalloc(&c->d_out_ls, ols.size() * sizeof(int32_t)) != hipSuccess) {
+    rhg_close(c);
+    return fail(RHG_ENOMEM, "hipMalloc (wavelength index)");
+  }
+  if (hipMemcpy(c->d_out_ls, ols.data(), ols.size() * sizeof(int32_t), hipMemcpyHostToDevice) !=
+      hipSuccess) {
+    rhg_close(c);
+    return fail(RHG_EDEVICE, "hipMemcpy (wavelength index)");
+  }
+  *out = c;
+  g_err.clear();
+  return 0;
+}
+
+static int synth(rhg_context_t *c, const rhg_columns_t *cols, int32_t precision,
+                 int32_t device_ptrs, double *stokes_out, double *height_out,
+                 double *chiref_out) {
+  if (!c || !cols || !stokes_out) return fail(RHG_EINVAL, "null argument");
+  if (precision != 0) return fail(RHG_EUNSUPPORTED, "only precision 0 (FP64) is implemented");
+  const int ncol = cols->ncol, Nmax = cols->ndep_max, nout = c->nout;
+  if (ncol < 1 || Nmax < 4 || Nmax > RHG_MAX_DEPTH) return fail(RHG_EINVAL, "bad batch shape");
+  if (!(cols->mu > 0.0 && cols->mu <= 1.0)) return fail(RHG_EINVAL, "mu must be in (0, 1]");
+  if (cols->atm_scale < 0 || cols->atm_scale > 2) return fail(RHG_EINVAL, "bad atm_scale");
+  std::vector<int32_t> hndep(ncol);
+  if (device_ptrs) {
+    HIPCHECK(hipMemcpy(hndep.data(), cols->ndep, ncol * sizeof(int32_t), hipMemcpyDeviceToHost));
+  } else {
+    memcpy(hndep.data(), cols->ndep, ncol * sizeof(int32_t));
+  }
+  for (int i = 0; i < ncol; i++)
+    if (hndep[i] < 4 || hndep[i] > Nmax) return fail(RHG_EINVAL, "ndep out of range");
+  HIPCHECK(hipSetDevice(c->device));
+
+  const int nfield = RHG_CF_NFIXED + 3 * c->ncont + 4 * c->nline + 1;
+  const size_t atm_b = (size_t) ncol * 9 * Nmax * sizeof(double);
+  const size_t st_b = (size_t) ncol * nfield * Nmax * sizeof(double);
+  const size_t cc_b = (size_t) ncol * Nmax * nout * sizeof(double);
+  const size_t out_b = (size_t) ncol * 4 * nout * sizeof(double);
+  if (c->state.ensure(st_b) || c->moving.ensure(ncol * sizeof(int32_t)) ||
+      c->cchi.ensure(cc_b) || c->ceta.ensure(cc_b) ||
+      (!device_ptrs && (c->atmos.ensure(atm_b) || c->ndep.ensure(ncol * sizeof(int32_t)) ||
+                        c->stokes.ensure(out_b))) ||
+      (chiref_out && c->chiref.ensure((size_t) ncol * Nmax * sizeof(double))))
+    return fail(RHG_ENOMEM, "device allocation for the batch failed");
+  const double *d_atm = (const double *) cols->atmos;
+  const int32_t *d_ndep = cols->ndep;
+  double *d_out = stokes_out;
+  if (!device_ptrs) {
+    HIPCHECK(hipMemcpyAsync(c->atmos.p, cols->atmos, atm_b, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(c->ndep.p, cols->ndep, ncol * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    d_atm = (const double *) c->atmos.p;
+    d_ndep = (const int32_t *) c->ndep.p;
+    d_out = (double *) c->stokes.p;
+  }
+  double *state = (double *) c->state.p;
+  int32_t *moving = (int32_t *) c->moving.p;
+  double *cchi = (double *) c->cchi.p, *ceta = (double *) c->ceta.p;
+  double *chiref = chiref_out ? (double *) c->chiref.p : nullptr;
+
+  HIPCHECK(hipEventRecord(c->ev[0], c->stream));
+  {
+    const int64_t nthr = (int64_t) ncol * Nmax;
+    dim3 blk(64), grd((unsigned) ((nthr + 63) / 64));
+    hipLaunchKernelGGL(rhg::k_state, grd, blk, 0, c->stream, c->dm, d_atm, d_ndep, ncol, Nmax,
+                       cols->mu, cols->atm_scale, state, nfield);
+    HIPCHECK(hipGetLastError());
+  }
+  HIPCHECK(hipEventRecord(c->ev[1], c->stream));
+  {
+    dim3 blk(64), grd((unsigned) ((ncol + 63) / 64));
+    hipLaunchKernelGGL(rhg::k_geometry, grd, blk, 0, c->stream, c->dm, d_ndep, ncol, Nmax,
+                       cols->atm_scale, cols->mu, state, nfield, moving, chiref);
+    HIPCHECK(hipGetLastError());
+  }
+  HIPCHECK(hipEventRecord(c->ev[2], c->stream));
+  const int CH = 65535;
+  for (int c0 = 0; c0 < ncol; c0 += CH) {
+    const int nc = (ncol - c0 < CH) ? ncol - c0 : CH;
+    dim3 blk(64), grd((unsigned) ((nout + 63) / 64), (unsigned) Nmax, (unsigned) nc);
+    hipLaunchKernelGGL(rhg::k_continuum, grd, blk, 0, c->stream, c->dm, d_ndep + c0, nc, Nmax,
+                       state + (size_t) c0 * nfield * Nmax, nfield, c->d_out_ls, nout,
+                       cchi + (size_t) c0 * Nmax * nout, ceta + (size_t) c0 * Nmax * nout);
+    HIPCHECK(hipGetLastError());
+  }
+  HIPCHECK(hipEventRecord(c->ev[3], c->stream));
+  for (int c0 = 0; c0 < ncol; c0 += CH) {
+    const int nc = (ncol - c0 < CH) ? ncol - c0 : CH;
+    dim3 blk(64), grd((unsigned) ((nout + 63) / 64), (unsigned) nc);
+    hipLaunchKernelGGL(rhg::k_formal, grd, blk, 0, c->stream, c->dm, d_ndep + c0, moving + c0, nc,
+                       Nmax, state + (size_t) c0 * nfield * Nmax, nfield, c->d_out_ls, nout,
+                       cchi + (size_t) c0 * Nmax * nout, ceta + (size_t) c0 * Nmax * nout,
+                       cols->mu, d_out + (size_t) c0 * 4 * nout);
+    HIPCHECK(hipGetLastError());
+  }
+  HIPCHECK(hipEventRecord(c->ev[4], c->stream));
+  if (!device_ptrs)
+    HIPCHECK(hipMemcpyAsync(stokes_out, d_out, out_b, hipMemcpyDeviceToHost, c->stream));
+  if (height_out) {
+    for (int ic = 0; ic < ncol; ic++)
+      HIPCHECK(hipMemcpyAsync(height_out + (size_t) ic * Nmax,
+                              state + ((size_t) ic * nfield + RHG_CF_HEIGHT) * Nmax,
+                              Nmax * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  }
+  if (chiref_out)
+    HIPCHECK(hipMemcpyAsync(chiref_out, chiref, (size_t) ncol * Nmax * sizeof(double),
+                            hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  // events of the stages: state, geometry, continuum, formal
+  float ms;
+  for (int s = 0; s < 4; s++) {
+    HIPCHECK(hipEventElapsedTime(&ms, c->ev[s], c->ev[s + 1]));
+    c->last_ms[s] = ms;
+  }
+  g_err.clear();
+  return 0;
+}
+
+int rhg_synth_lte(rhg_context_t *ctx, const rhg_columns_t *cols, int32_t precision,
+                  int32_t device_ptrs, double *stokes_out) {
+  return synth(ctx, cols, precision, device_ptrs, stokes_out, nullptr, nullptr);
+}
+
+int rhg_synth_lte_diag(rhg_context_t *ctx, const rhg_columns_t *cols, int32_t precision,
+                       double *stokes_out, double *height_out, double *chiref_out) {
+  return synth(ctx, cols, precision, 0, stokes_out, height_out, chiref_out);
+}
+
+int rhg_last_timings(rhg_context_t *ctx, double *ms_out) {
+  if (!ctx || !ms_out) return fail(RHG_EINVAL, "null argument");
+  for (int s = 0; s < 4; s++) ms_out[s] = ctx->last_ms[s];
+  return 0;
+}
+
+}  // extern "C"

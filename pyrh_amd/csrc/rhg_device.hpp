@@ -1,0 +1,291 @@
+// Device-side numerics shared by the gfx950 kernels.
+//
+// The FP64 ("plain") path evaluates every expression in the order of the
+// reference routine it restates (cited per function) and is compiled with
+// FP contraction disabled, so that the only differences from the CPU are
+// the last-bit differences of the GPU's exp/log/pow.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/pyrh_amd.h"
+
+namespace rhg {
+
+// rh/constant.h:28-73, rh/rh.h:36-50
+constexpr double CLIGHT = 2.99792458E+08;
+constexpr double HPLANCK = 6.6260755E-34;
+constexpr double KBOLTZMANN = 1.380658E-23;
+constexpr double AMU = 1.6605402E-27;
+constexpr double M_ELECTRON = 9.1093897E-31;
+constexpr double Q_ELECTRON = 1.60217733E-19;
+constexpr double EPSILON_0 = 8.854187817E-12;
+constexpr double EV = 1.60217733E-19;
+constexpr double THETA0 = 5.03974756E+03;
+constexpr double NM_TO_M = 1.0E-09;
+constexpr double CM_TO_M = 1.0E-02;
+constexpr double KM_TO_M = 1.0E+03;
+constexpr double G_TO_KG = 1.0E-03;
+constexpr double ERG_TO_JOULE = 1.0E-07;
+constexpr double PI = 3.14159265358979;
+constexpr double SQRTPI = 1.77245385090551;
+constexpr double LG10 = 2.30258509299404568402;
+constexpr double MAX_EXPONENT = 150.0;
+constexpr double MAX_GAUSS_DOPPLER = 7.0;
+
+constexpr int N_MAX_CHEM_ITER = 10;
+constexpr double CHEM_ITER_LIMIT = 1.0E-3;
+constexpr int NSTAGE_MAX = 8;
+constexpr int NNUC_MAX = 8;     // nuclei bound in molecules
+constexpr int NMOL_MAX = 24;
+
+constexpr int NTHETA = 16, NFF = 17, NTHETA_H2 = 8, NFF_H2 = 19;
+constexpr int NTEMP_H2P = 10, NFF_H2P = 15, NTOH = 15, NEOH = 130, NTCH = 15, NECH = 105;
+
+__device__ __forceinline__ double pow10d(double x) { return exp(LG10 * x); }
+
+// rh/hunt.c:92-117, ascending table
+__device__ __forceinline__ int locate_asc(int n, const double *a, double v) {
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    int idx = (hi + lo) >> 1;
+    if (v >= a[idx]) lo = idx; else hi = idx;
+  }
+  return lo;
+}
+
+// rh/linear.c:22-54 for one abscissa (ascending table)
+__device__ __forceinline__ double linear1(int n, const double *xt, const double *yt, double x) {
+  if (x <= xt[0]) return yt[0];
+  if (x >= xt[n - 1]) return yt[n - 1];
+  int j = locate_asc(n, xt, x);
+  double fx = (xt[j + 1] - x) / (xt[j + 1] - xt[j]);
+  return fx * yt[j] + (1 - fx) * yt[j + 1];
+}
+
+// rh/hydrogen.c:998-1020
+__device__ __forceinline__ double bilinear(int Ncol, int Nrow, const double *f, double x, double y) {
+  int i = (int) x;
+  double fx = x - i;
+  int i1 = (i == Ncol - 1) ? i : i + 1;
+  int j = (int) y;
+  double fy = y - j;
+  int j1 = (j == Nrow - 1) ? j : j + 1;
+  return (1.0 - fx) * (1.0 - fy) * f[j * Ncol + i] + fx * (1.0 - fy) * f[j * Ncol + i1] +
+         (1.0 - fx) * fy * f[j1 * Ncol + i] + fx * fy * f[j1 * Ncol + i1];
+}
+
+// clamped fractional index (rh/hydrogen.c:509-523)
+__device__ __forceinline__ double clamp_index(int n, const double *t, double v) {
+  if (v <= t[0]) return 0;
+  if (v >= t[n - 1]) return n - 1;
+  int i = locate_asc(n, t, v);
+  return (double) i + (v - t[i]) / (t[i + 1] - t[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Voigt and Faraday-Voigt functions: Humlicek (1982) regions I-IV with the
+// region test of rh/voigt.c:396-413 and the rational forms of
+// rh/humlicek.c:28-120, in complex arithmetic written out in real pairs
+// (products and the quotient as rh/complex.c:45-100 form them).
+template <typename T>
+struct cpx { T r, i; };
+
+template <typename T>
+__device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
+  return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r};
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> cdiv(cpx<T> a, cpx<T> b) {
+  T d = b.r * b.r + b.i * b.i;
+  return {(a.r * b.r + a.i * b.i) / d, (a.i * b.r - a.r * b.i) / d};
+}
+
+__device__ __forceinline__ double voigt_humlicek(double a, double v, double *F) {
+  cpx<double> z = {a, -v}, W, z1, z2, u;
+  double s = fabs(v) + a;
+  if (s >= 15.0) {
+    z1 = {0.5641896 * z.r, 0.5641896 * z.i};
+    z2 = cmul(z, z);
+    z2.r = z2.r + 0.5;
+    W = cdiv(z1, z2);
+  } else if (s >= 5.5) {
+    u = cmul(z, z);
+    z1 = {0.5641896 * u.r, 0.5641896 * u.i};
+    z1.r = z1.r + 1.410474;
+    z1 = cmul(z, z1);
+    cpx<double> u3 = {u.r + 3.0, u.i};
+    z2 = cmul(u, u3);
+    z2.r = z2.r + 0.75;
+    W = cdiv(z1, z2);
+  } else if (a >= 0.195 * fabs(v) - 0.176) {
+    const double a3[5] = {0.5642236, 3.778987, 11.96482, 20.20933, 16.4955};
+    const double b3[5] = {6.699398, 21.69274, 39.27121, 38.82363, 16.4955};
+    z1 = {a3[0], 0.0};
+    z2 = {z.r + b3[0], z.i};
+#pragma unroll
+    for (int n = 1; n < 5; n++) {
+      z1 = cmul(z1, z); z1.r = z1.r + a3[n];
+      z2 = cmul(z2, z); z2.r = z2.r + b3[n];
+    }
+    W = cdiv(z1, z2);
+  } else {
+    const double a4[7] = {0.56419, 1.320522, 35.7668, 219.031, 1540.787, 3321.99, 36183.31};
+    const double b4[7] = {1.841439, 61.57037, 364.2191, 2186.181, 9022.228, 24322.84, 32066.6};
+    cpx<double> mz = {-1.0 * z.r, -1.0 * z.i};
+    u = cmul(z, mz);
+    z1 = {a4[0], 0.0};
+    z2 = {u.r + b4[0], u.i};
+#pragma unroll
+    for (int n = 1; n < 7; n++) {
+      z1 = cmul(u, z1); z1.r = z1.r + a4[n];
+      z2 = cmul(u, z2); z2.r = z2.r + b4[n];
+    }
+    cpx<double> mu = {-1.0 * u.r, -1.0 * u.i};
+    double ex = exp(mu.r);
+    cpx<double> e = {ex * cos(mu.i), ex * sin(mu.i)};
+    cpx<double> q = cdiv(cmul(z, z1), z2);
+    W = {e.r - q.r, e.i - q.i};
+  }
+  *F = W.i;
+  return W.r;
+}
+
+// ---------------------------------------------------------------------------
+// Armstrong Voigt function, rh/voigt.c:126-243
+__device__ constexpr double c_arm_t[10] = {0.2453407083, 0.7374737285, 1.2340762153, 1.7385377121,
+                                   2.2549740020, 2.7888060584, 3.3478545673, 3.9447640401,
+                                   4.6036824495, 5.3874808900};
+__device__ constexpr double c_arm_w[10] = {4.6224366960e-01, 2.8667550536e-01, 1.0901720602e-01,
+                                   2.4810520887e-02, 3.2437733422e-03, 2.2833863601e-04,
+                                   7.8025564785e-06, 1.0860693707e-07, 4.3993409922e-10,
+                                   2.2293936455e-13};
+__device__ constexpr double c_k1[34] = {
+    0.1999999999972224, -0.1840000000029998, 0.1558399999965025, -0.1216640000043988,
+    0.0877081599940391, -0.0585141248086907, 0.0362157301623914, -0.0208497654398036,
+    0.0111960116346270, -0.56231896167109e-02, 0.26487634172265e-02, -0.11732670757704e-02,
+    0.4899519978088e-03, -0.1933630801528e-03, 0.722877446788e-04, -0.256555124979e-04,
+    0.86620736841e-05, -0.27876379719e-05, 0.8566873627e-06, -0.2518433784e-06,
+    0.709360221e-07, -0.191732257e-07, 0.49801256e-08, -0.12447734e-08, 0.2997777e-09,
+    -0.696450e-10, 0.156262e-10, -0.33897e-11, 0.7116e-12, -0.1447e-12, 0.285e-13,
+    -0.55e-14, 0.10e-14, -0.2e-15};
+
+__device__ inline double voigt_armstrong(double a, double v) {
+  if (v < 0.0) v = -v;
+  if ((a < 1.0 && v < 4.0) || (a < 1.8 / (v + 1.0))) {
+    const double TWOSQRTPI = 1.12837917, EXPMAX = 70.0, TINY = 1.0E-08;
+    double a2 = a * a, v2 = v * v, u1, dn01, dn02, dn, funct, bn = 0.0;
+    if ((v2 - a2) > EXPMAX) u1 = 0.0;
+    else u1 = exp(a2 - v2) * cos(2.0 * v * a);
+    if (v > 5.0) {
+      double v2i = 1.0 / v2;
+      dn01 = -v2i * (0.5 + v2i * (0.75 + v2i * (1.875 + v2i * (6.5625 +
+             v2i * (29.53125 + v2i * (1162.4218 + v2i * 1055.7421))))));
+      dn02 = (1.0 - dn01) / (2.0 * v);
+    } else {
+      double bn01 = 0.0, bn02 = 0.0, v1 = v / 5.0, coef = 4.0 * v1 * v1 - 2.0;
+      for (int n = 33; n >= 0; n--) {
+        bn = coef * bn01 - bn02 + c_k1[n];
+        bn02 = bn01;
+        bn01 = bn;
+      }
+      dn02 = v1 * (bn - bn02);
+      dn01 = 1.0 - 2.0 * v * dn02;
+    }
+    funct = a * dn01;
+    if (a > TINY) {
+      double q = 1.0, an = a;
+      for (int n = 2; n <= 50; n++) {
+        dn = (v * dn01 + dn02) * (-2.0 / n);
+        dn02 = dn01;
+        dn01 = dn;
+        if (n % 2) {
+          q = -q;
+          an *= a2;
+          double g = dn * an;
+          funct += q * g;
+          if (fabs(g / funct) <= TINY) return (u1 - TWOSQRTPI * funct);
+        }
+      }
+    }
+    return (u1 - TWOSQRTPI * funct);
+  } else if (a < 2.5 && v < 4.0) {
+    double g = 0.0, a2 = a * a;
+    for (int n = 0; n < 10; n++) {
+      double r = c_arm_t[n] - v, s = c_arm_t[n] + v;
+      g += (4.0 * c_arm_t[n] * c_arm_t[n] - 2.0) *
+           (r * atan(r / a) + s * atan(s / a) - 0.5 * a * (log(a2 + r * r) + log(a2 + s * s))) *
+           c_arm_w[n];
+    }
+    return g / PI;
+  } else {
+    double g = 0.0, a2 = a * a;
+    for (int n = 0; n < 10; n++) {
+      double d1 = v - c_arm_t[n], d2 = v + c_arm_t[n];
+      g += (1.0 / (d1 * d1 + a2) + 1.0 / (d2 * d2 + a2)) * c_arm_w[n];
+    }
+    return (a * g) / PI;
+  }
+}
+
+
+// ---------------------------------------------------------------------------
+// rh/planck.c:38-63
+__device__ __forceinline__ double planck(double T, double hc_kla, double twohnu3_c2) {
+  double hc_Tkla = hc_kla / T;
+  return (hc_Tkla <= MAX_EXPONENT) ? twohnu3_c2 / (exp(hc_Tkla) - 1.0) : 0.0;
+}
+
+// rh/w3.c:44-63
+__device__ __forceinline__ void w3(double dtau, double *w) {
+  if (dtau < 5.0E-4) {
+    w[0] = dtau * (1.0 - 0.5 * dtau);
+    double delta = dtau * dtau;
+    w[1] = delta * (0.5 - dtau / 3.0);
+    delta *= dtau;
+    w[2] = delta * (1.0 / 3.0 - 0.25 * dtau);
+  } else if (dtau > 50.0) {
+    w[1] = w[0] = 1.0;
+    w[2] = 2.0;
+  } else {
+    double expdt = exp(-dtau);
+    w[0] = 1.0 - expdt;
+    w[1] = w[0] - dtau * expdt;
+    w[2] = 2.0 * w[1] - dtau * dtau * expdt;
+  }
+}
+
+// Fritsch & Butland (1984), rh/bezier_aux.c:34-51
+__device__ __forceinline__ double cent_deriv(double dsup, double dsdn, double up, double c,
+                                             double dn) {
+  double fim1 = (c - up) / dsup, fi = (dn - c) / dsdn;
+  if (fim1 * fi > 0) {
+    double alpha = 0.333333333333333333333333 * (1.0 + dsdn / (dsdn + dsup));
+    return (fim1 * fi) / ((1.0 - alpha) * fim1 + alpha * fi);
+  }
+  return 0.0;
+}
+
+// cubic Bezier integration weights, rh/bezier_aux.c:333-359
+__device__ __forceinline__ void bezier3_coeffs(double dt, double &alpha, double &beta,
+                                               double &gamma, double &theta, double &eps) {
+  double dt2 = dt * dt, dt3 = dt2 * dt;
+  if (dt >= 5.e-2) {
+    eps = exp(-dt);
+    alpha = (-6.0 + 6.0 * dt - 3.0 * dt2 + dt3 + 6.0 * eps) / dt3;
+    dt3 = 1.0 / dt3;
+    beta = (6.0 + (-6.0 - dt * (6.0 + dt * (3.0 + dt))) * eps) * dt3;
+    gamma = 3.0 * (6.0 + (-4.0 + dt) * dt - 2.0 * (3.0 + dt) * eps) * dt3;
+    theta = 3.0 * (eps * (6.0 + dt2 + 4.0 * dt) + 2.0 * dt - 6.0) * dt3;
+  } else {
+    double dt4 = dt2 * dt2;
+    eps = 1.0 - dt + 0.5 * dt2 - dt3 / 6.0 + dt4 / 24.0;
+    alpha = 0.25 * dt - 0.05 * dt2 + dt3 / 120.0 - dt4 / 840.0;
+    beta = 0.25 * dt - 0.20 * dt2 + dt3 / 12.0 - dt4 / 42.0;
+    gamma = 0.25 * dt - 0.10 * dt2 + dt3 * 0.025 - dt4 / 210.0;
+    theta = 0.25 * dt - 0.15 * dt2 + dt3 * 0.05 - dt4 / 84.0;
+  }
+}
+
+}  // namespace rhg

@@ -1,0 +1,115 @@
+"""ctypes binding of the gfx950 library (include/pyrh_amd.h).
+
+There is no CPU fallback: if the library is missing or no GPU is present,
+calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from .abi import ColumnsStruct, ModelStruct, RhgColumns, RhgModel
+
+LIB_PATH = Path(__file__).resolve().parent / "libpyrh_amd.so"
+_LIB = None
+
+RHG_NSTAGE = 4
+STAGES = ("state", "geometry", "continuum", "formal")
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libpyrh_amd.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise EngineError(f"{LIB_PATH} is not built; run pyrh_amd.build.build_library() "
+                              f"(or __graft_entry__.build())")
+        L = C.CDLL(str(LIB_PATH))
+        L.rhg_last_error.restype = C.c_char_p
+        L.rhg_abi_version.restype = C.c_int32
+        L.rhg_model_sizeof.restype = C.c_int32
+        L.rhg_columns_sizeof.restype = C.c_int32
+        L.rhg_open.argtypes = [C.POINTER(RhgModel), C.c_int, C.POINTER(C.c_void_p)]
+        L.rhg_close.argtypes = [C.c_void_p]
+        PD = C.POINTER(C.c_double)
+        L.rhg_synth_lte.argtypes = [C.c_void_p, C.POINTER(RhgColumns), C.c_int32, C.c_int32, PD]
+        L.rhg_synth_lte_diag.argtypes = [C.c_void_p, C.POINTER(RhgColumns), C.c_int32, PD, PD, PD]
+        L.rhg_last_timings.argtypes = [C.c_void_p, PD]
+        if L.rhg_model_sizeof() != C.sizeof(RhgModel) or \
+                L.rhg_columns_sizeof() != C.sizeof(RhgColumns):
+            raise EngineError("rhg_model_t / rhg_columns_t layout mismatch between the "
+                              "library and pyrh_amd.abi")
+        _LIB = L
+    return _LIB
+
+
+def _check(rc):
+    if rc != 0:
+        raise EngineError(f"pyrh_amd error {rc}: {lib().rhg_last_error().decode()}")
+
+
+def _pd(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Engine:
+    """One device context holding one model (run directory + wavelength
+    grid)."""
+
+    def __init__(self, model, device: int = 0):
+        self.model = model
+        self._ms = ModelStruct(model)
+        h = C.c_void_p()
+        _check(lib().rhg_open(self._ms.ptr(), int(device), C.byref(h)))
+        self._h = h
+        self.nout = model.scalars["nspect"] - 1
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().rhg_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synth_lte(self, atmos, mu=1.0, atm_scale=0, ndep=None, precision=0, diag=False):
+        """Emergent Stokes [ncol, 4, nout] float64 for host atmospheres
+        [ncol, 9, ndep_max] in pyrh units."""
+        cs = ColumnsStruct(atmos, ndep, atm_scale, mu)
+        ncol, nd = cs.struct.ncol, cs.struct.ndep_max
+        out = np.empty((ncol, 4, self.nout))
+        if diag:
+            h = np.zeros((ncol, nd))
+            cr = np.zeros((ncol, nd))
+            _check(lib().rhg_synth_lte_diag(self._h, cs.ptr(), int(precision), _pd(out),
+                                            _pd(h), _pd(cr)))
+            return out, h, cr
+        _check(lib().rhg_synth_lte(self._h, cs.ptr(), int(precision), 0, _pd(out)))
+        return out
+
+    def synth_lte_device(self, d_atmos, d_ndep, ncol, ndep_max, d_out, mu=1.0, atm_scale=0,
+                         precision=0):
+        """Same, with device pointers (ints) for atmosphere, depth counts and
+        output (e.g. from torch tensors); no host copies."""
+        s = RhgColumns()
+        s.ncol, s.ndep_max = int(ncol), int(ndep_max)
+        s.ndep = C.cast(C.c_void_p(d_ndep), C.POINTER(C.c_int32))
+        s.atmos = C.cast(C.c_void_p(d_atmos), C.POINTER(C.c_double))
+        s.atm_scale, s.mu = int(atm_scale), float(mu)
+        _check(lib().rhg_synth_lte(self._h, C.byref(s), int(precision), 1,
+                                   C.cast(C.c_void_p(d_out), C.POINTER(C.c_double))))
+
+    def timings(self):
+        """Per-stage kernel milliseconds of the last call."""
+        t = np.zeros(RHG_NSTAGE)
+        _check(lib().rhg_last_timings(self._h, _pd(t)))
+        return dict(zip(STAGES, t))

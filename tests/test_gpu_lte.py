@@ -1,0 +1,119 @@
+"""GPU parity: the gfx950 LTE Stokes path against the oracle and the
+reference's golden vectors (FP64 path: Stokes I within 1e-10 relative,
+Q/U/V within 1e-10 of I_c; the target is 1e-10 / 1e-5)."""
+import numpy as np
+import pytest
+
+import pyoracle
+from conftest import fixture_cases
+from pyrh_amd import model, workloads
+
+pytestmark = pytest.mark.gpu
+
+TOL_I = 1e-10
+TOL_QUV = 1e-10
+
+
+@pytest.fixture(scope="module")
+def engines(setups):
+    from pyrh_amd.engine import Engine
+    cache = {}
+
+    def get(run, wave):
+        key = (run, np.asarray(wave).tobytes())
+        if key not in cache:
+            cache[key] = Engine(model.build_model(setups[run], wave))
+        return cache[key]
+    return get
+
+
+def _err(gpu, ref):
+    Ic = ref[:, 0, :1]
+    eI = np.max(np.abs(gpu[:, 0] - ref[:, 0]) / np.abs(ref[:, 0]))
+    eP = np.max(np.abs(gpu[:, 1:] - ref[:, 1:]) / Ic[:, None])
+    return eI, eP
+
+
+def test_gpu_matches_golden_vectors(fixtures, engines):
+    F = fixtures
+    for name in fixture_cases(F):
+        eng = engines(str(F[name + "__run"]), F[name + "__wave"])
+        gpu = eng.synth_lte(F[name + "__atmos"], mu=float(F[name + "__mu"]),
+                            atm_scale=int(F[name + "__scale"]))
+        eI, eP = _err(gpu, F[name + "__stokes_o2"])
+        print(f"{name}: I {eI:.2e} QUV/Ic {eP:.2e}", flush=True)
+        assert eI <= TOL_I and eP <= TOL_QUV, name
+
+
+def test_gpu_depth_scale(fixtures, engines):
+    F = fixtures
+    eng = engines("lte", F["single__wave"])
+    _, h, cr = eng.synth_lte(F["single__atmos"], diag=True)
+    np.testing.assert_allclose(cr[0], F["single__chiref"], rtol=1e-13)
+    np.testing.assert_allclose(h[0], F["single__height"], rtol=1e-12, atol=1e-5)
+
+
+@pytest.mark.parametrize("seed,ncol,mu", [(11, 64, 1.0), (12, 32, 0.45)])
+def test_gpu_matches_oracle_on_stokes_map(engines, seed, ncol, mu):
+    wave = workloads.wavelengths_630()
+    eng = engines("lte", wave)
+    atm = workloads.stokes_map(ncol, seed=seed)
+    gpu = eng.synth_lte(atm, mu=mu)
+    cpu = pyoracle.synth_lte(eng.model, atm, mu=mu)
+    eI, eP = _err(gpu, cpu)
+    print(f"seed {seed}: I {eI:.2e} QUV/Ic {eP:.2e}", flush=True)
+    assert eI <= TOL_I and eP <= TOL_QUV
+
+
+def test_gpu_ragged_depths(engines):
+    """Columns with fewer depth points than the batch maximum."""
+    wave = workloads.wavelengths_630()
+    eng = engines("lte", wave)
+    full = workloads.stokes_map(6, seed=21)
+    ndep = np.array([82, 70, 82, 50, 64, 82], np.int32)
+    atm = full.copy()
+    for i, n in enumerate(ndep):          # each column uses its first n points
+        atm[i, :, n:] = 0.0
+    gpu = eng.synth_lte(atm, ndep=ndep)
+    for i, n in enumerate(ndep):
+        cpu = pyoracle.synth_lte(eng.model, np.ascontiguousarray(full[i:i + 1, :, :n]))
+        eI, eP = _err(gpu[i:i + 1], cpu)
+        assert eI <= TOL_I and eP <= TOL_QUV, (i, n)
+
+
+def test_gpu_polarization_invariants(engines):
+    """B = 0 gives Q = U = V = 0; gamma -> pi - gamma flips V and keeps
+    I, Q, U; rotating chi by d rotates (Q, U) by 2d."""
+    wave = workloads.wavelengths_630()
+    eng = engines("lte", wave)
+    a = workloads.stokes_map(3, seed=31)
+    a[0, 5] = 0.0
+    b = a.copy()
+    b[:, 6] = np.pi - a[:, 6]
+    c = a.copy()
+    c[:, 7] = a[:, 7] + 0.3
+    sa, sb, sc = eng.synth_lte(a), eng.synth_lte(b), eng.synth_lte(c)
+    Ic = sa[:, 0, :1]
+    assert np.max(np.abs(sa[0, 1:])) <= 1e-25
+    np.testing.assert_allclose(sb[:, 0], sa[:, 0], rtol=1e-13)
+    np.testing.assert_allclose(sb[:, 3], -sa[:, 3], atol=1e-13 * Ic.max())
+    cq = np.cos(0.6) * sa[:, 1] - np.sin(0.6) * sa[:, 2]
+    cu = np.sin(0.6) * sa[:, 1] + np.cos(0.6) * sa[:, 2]
+    assert np.max(np.abs(sc[:, 1] - cq) / Ic) < 1e-11
+    assert np.max(np.abs(sc[:, 2] - cu) / Ic) < 1e-11
+
+
+def test_gpu_full_map_properties(engines):
+    """The 16384-column workload: finite, positive intensities, line cores
+    darker than the continuum, |Q|,|U|,|V| < I, and one column equal to the
+    same column synthesised alone."""
+    wave = workloads.wavelengths_630()
+    eng = engines("lte", wave)
+    atm = workloads.stokes_map(16384, seed=workloads.SEED_BASE + 1)
+    st = eng.synth_lte(atm)
+    assert np.all(np.isfinite(st))
+    assert np.all(st[:, 0] > 0)
+    assert np.all(np.abs(st[:, 1:]) < st[:, 0:1])
+    assert np.all(st[:, 0].min(axis=1) < st[:, 0, 0])
+    one = eng.synth_lte(atm[1234:1235])
+    np.testing.assert_array_equal(one[0], st[1234])

@@ -1,0 +1,129 @@
+"""Host logic: input readers, model builder and the C ABI library (no GPU)."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from pyrh_amd import abi, constants as K, model, workloads
+from pyrh_amd.cscan import sscanf
+from pyrh_amd.rhdata import RHInputError, read_keywords
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def test_sscanf_semantics():
+    assert sscanf("630.1500-10.123 26.00", "%lf %lf %s") == (3, [630.15, -10.123, "26.00"])
+    assert sscanf("26.01", "%d.%d") == (2, [26, 1])
+    assert sscanf("   \n", "%lf") == (-1, [])
+    assert sscanf(" 1503 1835", "%5d%5d") == (2, [1503, 1835])
+    assert sscanf("5D", "%d%1s") == (2, [5, "D"])
+
+
+def test_keyword_defaults_and_required(tmp_path):
+    kw = read_keywords(ROOT / "tests/golden/run_lte/keyword.input")
+    assert kw.str("STOKES_MODE") == "FULL_STOKES"
+    assert kw.float("LAMBDA_REF") == 500.0        # default
+    assert not kw.bool("MAGNETO_OPTICAL")
+    p = tmp_path / "keyword.input"
+    p.write_text("  NRAYS = 1\n")
+    with pytest.raises(RHInputError):
+        read_keywords(p)
+
+
+def test_air_to_vacuum_known_value():
+    # 630.1500 nm (air) -> 630.3245 nm (vacuum) for the Fe I line
+    assert abs(K.air_to_vacuum(630.15) - 630.32427) < 1e-4
+    # the two directions are separate fits (Edlen 1966 and its inverse
+    # series), consistent to about 1e-6 relative
+    lam = np.array([500.0, 630.0, 1000.0])
+    back = np.array(K.vacuum_to_air(K.air_to_vacuum(lam)))
+    np.testing.assert_allclose(back, lam, rtol=2e-6)
+    assert K.air_to_vacuum(150.0) == 150.0        # below 200 nm nothing changes
+
+
+def test_fe_line_zeeman_patterns(setups):
+    """Fe I 630.15 (z5P2-e5D2: 13 components, g_eff 1.667) and 630.25
+    (z5P1-e5D0: 3 components, g_eff 2.5); strengths sum to 1 per q."""
+    rlk = setups["lte"].rlk
+    assert [len(r.comps) for r in rlk] == [13, 3]
+    for r, geff in zip(rlk, (5.0 / 3.0, 2.5)):
+        for q in (-1, 0, 1):
+            s = sum(c[2] for c in r.comps if c[0] == q)
+            if any(c[0] == q and c[2] > 0 for c in r.comps):
+                assert abs(s - 1) < 1e-14
+        g = sum(c[1] * c[2] for c in r.comps if c[0] == 1)
+        assert abs(g - geff) < 1e-12
+        assert r.vdwaals == 2              # ABO (Anstee-Barklem-O'Mara) broadening
+        assert r.polarizable
+
+
+def test_unpolarizable_labels(setups):
+    assert not any(r.polarizable for r in setups["unpol"].rlk)
+
+
+def test_model_solvers_and_tables(setups):
+    m = model.build_model(setups["lte"], workloads.wavelengths_630())
+    assert m.scalars["nspect"] == 113 and m.lam[m.ref_index] == 500.0
+    assert np.all(m.arrays["solver"][m.out_index] == 2)
+    assert m.arrays["solver"][m.ref_index] == 0
+    # every output wavelength sees both lines (window 20 vmicro_char)
+    ptr = m.arrays["lam_line_ptr"]
+    assert np.all(np.diff(ptr)[m.out_index] == 2)
+    # continua active at 500 nm and 630 nm exist and have positive cross-sections
+    assert np.all(m.arrays["lam_cont_alpha"] > 0)
+
+
+def test_model_rejects_unsupported(setups, tmp_path):
+    import shutil
+    d = tmp_path / "run"
+    shutil.copytree(ROOT / "tests/golden/run_lte", d)
+    kw = (d / "keyword.input").read_text().replace("MAGNETO_OPTICAL = FALSE",
+                                                   "MAGNETO_OPTICAL = TRUE")
+    (d / "keyword.input").write_text(kw)
+    s = model.open_run(str(d))
+    with pytest.raises(NotImplementedError):
+        model.build_model(s, workloads.wavelengths_630())
+
+
+def test_workloads_are_seeded():
+    a = workloads.stokes_map(8, seed=3)
+    b = workloads.stokes_map(8, seed=3)
+    np.testing.assert_array_equal(a, b)
+    assert a.shape == (8, 9, 82)
+    assert np.all(a[:, 5] >= 0) and np.all(a[:, 5] <= 3000)
+    assert np.all(np.abs(np.cos(a[:, 6])) <= 1)
+
+
+def test_abi_struct_layout_matches_oracle():
+    import pyoracle
+    ms, cs = pyoracle.sizes()
+    assert ms == C.sizeof(abi.RhgModel) and cs == C.sizeof(abi.RhgColumns)
+
+
+def _header_functions():
+    txt = (ROOT / "include" / "pyrh_amd.h").read_text()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(rhg_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_header_function():
+    """libpyrh_amd.so loads on the host and exports what the header declares."""
+    from pyrh_amd import engine
+    lib = engine.lib()
+    names = _header_functions()
+    assert "rhg_synth_lte" in names and "rhg_open" in names
+    for n in names:
+        assert hasattr(lib, n), n
+    assert lib.rhg_abi_version() == abi.ABI_VERSION
+    assert lib.rhg_model_sizeof() == C.sizeof(abi.RhgModel)
+
+
+def test_library_reports_errors_without_device():
+    """A null model is rejected with a status code and a message."""
+    from pyrh_amd import engine
+    lib = engine.lib()
+    h = C.c_void_p()
+    rc = lib.rhg_open(None, 0, C.byref(h))
+    assert rc == -1 and b"null" in lib.rhg_last_error()
