@@ -229,6 +229,29 @@ int rhg_synth_lte_diag(rhg_context_t *ctx, const rhg_columns_t *cols,
 #define RHG_NSTAGE 4
 int rhg_last_timings(rhg_context_t *ctx, double *ms_out);
 
+/* Algorithmic work census of the last rhg_synth_lte call on this context
+ * (its columns, depth counts and column state must still be alive): sums
+ * over emergent rays (column, output wavelength) and depth indices
+ * 1..N-1 of the counts below, written to counts_out[RHG_NWORK].  The
+ * Humlicek branch of every Zeeman component is replayed from the same
+ * damping and Doppler offsets the synthesis used.  It is the input of the
+ * roofline's FLOP count; it has no counterpart in the reference. */
+enum rhg_work {
+  RHG_WORK_STOKES = 0,    /* depth points on polarized DELO-Bezier rays */
+  RHG_WORK_BEZIER = 1,    /* ... on scalar Bezier rays */
+  RHG_WORK_FEAUTRIER = 2, /* ... on Feautrier rays */
+  RHG_WORK_LINE = 3,      /* (line in window, depth point) pairs */
+  RHG_WORK_VOIGT_I = 4,   /* Zeeman-component Voigt evaluations, region I */
+  RHG_WORK_VOIGT_II = 5,
+  RHG_WORK_VOIGT_III = 6,
+  RHG_WORK_VOIGT_IV = 7,
+  RHG_WORK_ARMSTRONG = 8, /* unpolarized-line Voigt evaluations */
+  RHG_WORK_GAUSS = 9,     /* Doppler-profile evaluations (no damping) */
+  RHG_WORK_COLDEPTH = 10, /* (column, depth) points of the column state */
+  RHG_NWORK = 12
+};
+int rhg_count_work(rhg_context_t *ctx, int64_t *counts_out);
+
 /* sizeof(rhg_model_t) and sizeof(rhg_columns_t) as compiled, for host
  * bindings to check their struct layout. */
 int32_t rhg_model_sizeof(void);

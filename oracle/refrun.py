@@ -1,7 +1,7 @@
 """Run the reference RH library built from source (oracle/_ref) on columns.
 
-Test infrastructure only (fixture generation in a container that holds the
-reference sources).  The binary layouts are those documented at the top of
+Test infrastructure only: fixture generation and the CPU baseline of the
+benchmark.  The binary layouts are those documented at the top of
 oracle/ref_harness.c.
 """
 from __future__ import annotations
@@ -14,7 +14,7 @@ from pathlib import Path
 
 import numpy as np
 
-REPO = Path(__file__).resolve().parents[2]
+REPO = Path(__file__).resolve().parents[1]
 REF_DIR = REPO / "oracle" / "_ref"
 DATA_DIR = REPO / "tests" / "golden" / "rhdata"
 MAGIC = 0x4E494852

@@ -51,9 +51,14 @@ struct rhg_context {
   std::vector<void *> allocs;   // model arrays
   int nspect = 0, nout = 0, ncont = 0, nline = 0, natom = 0;
   int32_t *d_out_ls = nullptr;
-  DevBuf atmos, ndep, state, moving, cchi, ceta, stokes, chiref;
+  DevBuf atmos, ndep, state, moving, cchi, ceta, stokes, chiref, census, census_sum;
   hipEvent_t ev[RHG_NSTAGE + 2]{};
   float last_ms[RHG_NSTAGE + 1]{};
+  // the last batch, for rhg_count_work
+  int last_ncol = 0, last_Nmax = 0, last_nfield = 0;
+  double last_mu = 1.0;
+  const int32_t *last_ndep = nullptr;
+  std::vector<int32_t> last_hndep;
 };
 
 namespace {
@@ -334,6 +339,12 @@ static int synth(rhg_context_t *c, const rhg_columns_t *cols, int32_t precision,
     HIPCHECK(hipMemcpyAsync(chiref_out, chiref, (size_t) ncol * Nmax * sizeof(double),
                             hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
+  c->last_ncol = ncol;
+  c->last_Nmax = Nmax;
+  c->last_nfield = nfield;
+  c->last_mu = cols->mu;
+  c->last_ndep = d_ndep;
+  c->last_hndep.swap(hndep);
   // events of the stages: state, geometry, continuum, formal
   float ms;
   for (int s = 0; s < 4; s++) {
@@ -352,6 +363,46 @@ int rhg_synth_lte(rhg_context_t *ctx, const rhg_columns_t *cols, int32_t precisi
 int rhg_synth_lte_diag(rhg_context_t *ctx, const rhg_columns_t *cols, int32_t precision,
                        double *stokes_out, double *height_out, double *chiref_out) {
   return synth(ctx, cols, precision, 0, stokes_out, height_out, chiref_out);
+}
+
+int rhg_count_work(rhg_context_t *c, int64_t *counts_out) {
+  if (!c || !counts_out) return fail(RHG_EINVAL, "null argument");
+  if (c->last_ncol < 1) return fail(RHG_EINVAL, "no batch has been synthesised on this context");
+  HIPCHECK(hipSetDevice(c->device));
+  const int ncol = c->last_ncol, Nmax = c->last_Nmax, nout = c->nout, nfield = c->last_nfield;
+  if (c->census.ensure((size_t) ncol * nout * RHG_NWORK * sizeof(int32_t)) ||
+      c->census_sum.ensure((size_t) ncol * RHG_NWORK * sizeof(int64_t)))
+    return fail(RHG_ENOMEM, "device allocation for the work census failed");
+  int32_t *census = (int32_t *) c->census.p;
+  const double *state = (const double *) c->state.p;
+  const int CH = 65535;
+  for (int c0 = 0; c0 < ncol; c0 += CH) {
+    const int nc = (ncol - c0 < CH) ? ncol - c0 : CH;
+    dim3 blk(64), grd((unsigned) ((nout + 63) / 64), (unsigned) nc);
+    hipLaunchKernelGGL(rhg::k_census, grd, blk, 0, c->stream, c->dm, c->last_ndep + c0,
+                       (const int32_t *) c->moving.p + c0, nc, Nmax,
+                       state + (size_t) c0 * nfield * Nmax, nfield, c->d_out_ls, nout, c->last_mu,
+                       census + (size_t) c0 * nout * RHG_NWORK);
+    HIPCHECK(hipGetLastError());
+  }
+  {
+    dim3 blk(64), grd((unsigned) ((ncol + 63) / 64));
+    hipLaunchKernelGGL(rhg::k_census_sum, grd, blk, 0, c->stream, census, ncol, nout,
+                       (int64_t *) c->census_sum.p);
+    HIPCHECK(hipGetLastError());
+  }
+  std::vector<int64_t> cs((size_t) ncol * RHG_NWORK);
+  HIPCHECK(hipMemcpyAsync(cs.data(), c->census_sum.p, cs.size() * sizeof(int64_t),
+                          hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < RHG_NWORK; i++) counts_out[i] = 0;
+  for (int ic = 0; ic < ncol; ic++)
+    for (int i = 0; i < RHG_NWORK; i++) counts_out[i] += cs[(size_t) ic * RHG_NWORK + i];
+  int64_t cd = 0;
+  for (int ic = 0; ic < ncol; ic++) cd += c->last_hndep[ic];
+  counts_out[RHG_WORK_COLDEPTH] = cd;
+  g_err.clear();
+  return 0;
 }
 
 int rhg_last_timings(rhg_context_t *ctx, double *ms_out) {

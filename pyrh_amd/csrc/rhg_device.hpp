@@ -102,15 +102,25 @@ __device__ __forceinline__ cpx<T> cdiv(cpx<T> a, cpx<T> b) {
   return {(a.r * b.r + a.i * b.i) / d, (a.i * b.r - a.r * b.i) / d};
 }
 
+// Humlicek's region (0..3 for I..IV) of the point z = a - i v, the test of
+// rh/voigt.c:396-413; voigt_humlicek() branches on exactly this test.
+__device__ __forceinline__ int humlicek_region(double a, double v) {
+  const double s = fabs(v) + a;
+  if (s >= 15.0) return 0;
+  if (s >= 5.5) return 1;
+  if (a >= 0.195 * fabs(v) - 0.176) return 2;
+  return 3;
+}
+
 __device__ __forceinline__ double voigt_humlicek(double a, double v, double *F) {
   cpx<double> z = {a, -v}, W, z1, z2, u;
-  double s = fabs(v) + a;
-  if (s >= 15.0) {
+  const int region = humlicek_region(a, v);
+  if (region == 0) {
     z1 = {0.5641896 * z.r, 0.5641896 * z.i};
     z2 = cmul(z, z);
     z2.r = z2.r + 0.5;
     W = cdiv(z1, z2);
-  } else if (s >= 5.5) {
+  } else if (region == 1) {
     u = cmul(z, z);
     z1 = {0.5641896 * u.r, 0.5641896 * u.i};
     z1.r = z1.r + 1.410474;
@@ -119,7 +129,7 @@ __device__ __forceinline__ double voigt_humlicek(double a, double v, double *F) 
     z2 = cmul(u, u3);
     z2.r = z2.r + 0.75;
     W = cdiv(z1, z2);
-  } else if (a >= 0.195 * fabs(v) - 0.176) {
+  } else if (region == 2) {
     const double a3[5] = {0.5642236, 3.778987, 11.96482, 20.20933, 16.4955};
     const double b3[5] = {6.699398, 21.69274, 39.27121, 38.82363, 16.4955};
     z1 = {a3[0], 0.0};

@@ -358,4 +358,42 @@ k_formal(DevModel M, const int32_t *__restrict__ ndep, const int32_t *__restrict
 #undef POINT
 }
 
+// Work census: one thread per (column, output wavelength), counts per ray in
+// census[(ic * nout + io) * RHG_NWORK + slot].
+__global__ void __launch_bounds__(64)
+k_census(DevModel M, const int32_t *__restrict__ ndep, const int32_t *__restrict__ moving_arr,
+         int ncol, int Nmax, const double *__restrict__ state, int nfield,
+         const int32_t *__restrict__ out_ls, int nout, double mu, int32_t *__restrict__ census) {
+  const int io = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ic = blockIdx.y;
+  if (io >= nout || ic >= ncol) return;
+  const int N = ndep[ic], ls = out_ls[io], moving = moving_arr[ic];
+  const double *st = state + (int64_t) ic * nfield * Nmax;
+  int32_t cnt[RHG_NWORK];
+  for (int i = 0; i < RHG_NWORK; i++) cnt[i] = 0;
+  int solver = M.solver[ls];
+  if (solver == RHG_SOLVER_BEZIER && !moving) solver = RHG_SOLVER_FEAUTRIER;
+  const int slot = solver == RHG_SOLVER_STOKES ? RHG_WORK_STOKES
+                 : solver == RHG_SOLVER_BEZIER ? RHG_WORK_BEZIER : RHG_WORK_FEAUTRIER;
+  const int has_line = M.lam_line_ptr[ls + 1] > M.lam_line_ptr[ls];
+  for (int k = 1; k < N; k++) {
+    cnt[slot]++;
+    if (has_line) line_census(M, ls, st + k, Nmax, mu, moving, cnt);
+  }
+  int32_t *o = census + ((int64_t) ic * nout + io) * RHG_NWORK;
+  for (int i = 0; i < RHG_NWORK; i++) o[i] = cnt[i];
+}
+
+// Per-column sums of the census: thread per column.
+__global__ void k_census_sum(const int32_t *__restrict__ census, int ncol, int nout,
+                             int64_t *__restrict__ col_sum) {
+  const int ic = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ic >= ncol) return;
+  int64_t acc[RHG_NWORK];
+  for (int i = 0; i < RHG_NWORK; i++) acc[i] = 0;
+  for (int io = 0; io < nout; io++)
+    for (int i = 0; i < RHG_NWORK; i++) acc[i] += census[((int64_t) ic * nout + io) * RHG_NWORK + i];
+  for (int i = 0; i < RHG_NWORK; i++) col_sum[(int64_t) ic * RHG_NWORK + i] = acc[i];
+}
+
 }  // namespace rhg

@@ -123,6 +123,35 @@ __device__ __forceinline__ void line_point(const DevModel &M, int ls, const doub
   }
 }
 
+// Work census of one emergent ray (column, wavelength) over depth indices
+// 1..N-1: the same line window, Doppler offsets and Voigt branches as
+// line_point(), with counts in place of arithmetic (slots: RHG_WORK_*).
+__device__ __forceinline__ void line_census(const DevModel &M, int ls, const double *st, int64_t fs,
+                                            double mu, int moving, int32_t *cnt) {
+  const double lambda = M.lamconst[(int64_t) ls * RHG_LC_NCONST + RHG_LC_LAMBDA];
+  const int p0 = M.lam_line_ptr[ls], p1 = M.lam_line_ptr[ls + 1];
+  const int lb = line_field_base(M);
+  const double vz = st[RHG_CF_VLOS * fs];
+  for (int e = p0; e < p1; e++) {
+    const int l = M.lam_line_idx[e];
+    const double adamp = st[(lb + 4 * l + 2) * fs], vbroad = st[(lb + 4 * l + 3) * fs];
+    double v = (lambda / M.line_lambda0[l] - 1.0) * CLIGHT / vbroad;
+    if (moving) v += (mu * vz) / vbroad;
+    cnt[RHG_WORK_LINE]++;
+    if (M.line_Grad[l] != 0.0) {
+      if (M.line_polar[l] != 0) {
+        const double vB = M.line_larmor[l] * st[RHG_CF_B * fs] / vbroad;
+        for (int nz = M.line_comp_ptr[l]; nz < M.line_comp_ptr[l + 1]; nz++)
+          cnt[RHG_WORK_VOIGT_I + humlicek_region(adamp, v - M.comp_shift[nz] * vB)]++;
+      } else {
+        cnt[RHG_WORK_ARMSTRONG]++;
+      }
+    } else {
+      cnt[RHG_WORK_GAUSS]++;
+    }
+  }
+}
+
 __global__ void k_state(DevModel M, const double *atmos, const int32_t *ndep, int ncol, int Nmax,
                         double mu, int atm_scale, double *state, int nfield);
 __global__ void k_geometry(DevModel M, const int32_t *ndep, int ncol, int Nmax, int atm_scale,
@@ -135,5 +164,10 @@ __global__ void k_formal(DevModel M, const int32_t *ndep, const int32_t *moving,
                          int Nmax, const double *state, int nfield, const int32_t *out_ls,
                          int nout, const double *cchi, const double *ceta, double mu,
                          double *stokes);
+
+__global__ void k_census(DevModel M, const int32_t *ndep, const int32_t *moving, int ncol,
+                         int Nmax, const double *state, int nfield, const int32_t *out_ls,
+                         int nout, double mu, int32_t *census);
+__global__ void k_census_sum(const int32_t *census, int ncol, int nout, int64_t *col_sum);
 
 }  // namespace rhg

@@ -16,6 +16,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libpyrh_amd.so"
 _LIB = None
 
 RHG_NSTAGE = 4
+RHG_NWORK = 12
 STAGES = ("state", "geometry", "continuum", "formal")
 
 
@@ -41,6 +42,7 @@ def lib():
         L.rhg_synth_lte.argtypes = [C.c_void_p, C.POINTER(RhgColumns), C.c_int32, C.c_int32, PD]
         L.rhg_synth_lte_diag.argtypes = [C.c_void_p, C.POINTER(RhgColumns), C.c_int32, PD, PD, PD]
         L.rhg_last_timings.argtypes = [C.c_void_p, PD]
+        L.rhg_count_work.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
         if L.rhg_model_sizeof() != C.sizeof(RhgModel) or \
                 L.rhg_columns_sizeof() != C.sizeof(RhgColumns):
             raise EngineError("rhg_model_t / rhg_columns_t layout mismatch between the "
@@ -107,6 +109,13 @@ class Engine:
         s.atm_scale, s.mu = int(atm_scale), float(mu)
         _check(lib().rhg_synth_lte(self._h, C.byref(s), int(precision), 1,
                                    C.cast(C.c_void_p(d_out), C.POINTER(C.c_double))))
+
+    def count_work(self):
+        """Work census of the last synthesis (slots RHG_WORK_*, see
+        pyrh_amd.roofline.WORK_SLOTS)."""
+        cnt = np.zeros(RHG_NWORK, np.int64)
+        _check(lib().rhg_count_work(self._h, cnt.ctypes.data_as(C.POINTER(C.c_int64))))
+        return cnt
 
     def timings(self):
         """Per-stage kernel milliseconds of the last call."""

@@ -20,6 +20,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 sys.path.insert(0, str(HERE))
 sys.path.insert(0, str(HERE.parents[1]))
+sys.path.insert(0, str(HERE.parents[1] / "oracle"))
 
 import refrun  # noqa: E402
 from pyrh_amd import workloads as W  # noqa: E402

@@ -94,13 +94,18 @@ def test_gpu_polarization_invariants(engines):
     c[:, 7] = a[:, 7] + 0.3
     sa, sb, sc = eng.synth_lte(a), eng.synth_lte(b), eng.synth_lte(c)
     Ic = sa[:, 0, :1]
-    assert np.max(np.abs(sa[0, 1:])) <= 1e-25
-    np.testing.assert_allclose(sb[:, 0], sa[:, 0], rtol=1e-13)
-    np.testing.assert_allclose(sb[:, 3], -sa[:, 3], atol=1e-13 * Ic.max())
+    # with B = 0 the Zeeman components coincide; what is left is the rounding
+    # of the per-q strength sums
+    assert np.max(np.abs(sa[0, 1:])) <= 1e-13 * Ic[0, 0]
+    # cos(pi - g) and -cos(g) differ in the last bit, and the depth
+    # recursion amplifies input rounding by ~1e5: the invariants hold to
+    # the parity bar, not bit for bit
+    np.testing.assert_allclose(sb[:, 0], sa[:, 0], rtol=TOL_I)
+    np.testing.assert_allclose(sb[:, 3], -sa[:, 3], atol=TOL_QUV * Ic.max())
     cq = np.cos(0.6) * sa[:, 1] - np.sin(0.6) * sa[:, 2]
     cu = np.sin(0.6) * sa[:, 1] + np.cos(0.6) * sa[:, 2]
-    assert np.max(np.abs(sc[:, 1] - cq) / Ic) < 1e-11
-    assert np.max(np.abs(sc[:, 2] - cu) / Ic) < 1e-11
+    assert np.max(np.abs(sc[:, 1] - cq) / Ic) < TOL_QUV
+    assert np.max(np.abs(sc[:, 2] - cu) / Ic) < TOL_QUV
 
 
 def test_gpu_full_map_properties(engines):
@@ -117,3 +122,26 @@ def test_gpu_full_map_properties(engines):
     assert np.all(st[:, 0].min(axis=1) < st[:, 0, 0])
     one = eng.synth_lte(atm[1234:1235])
     np.testing.assert_array_equal(one[0], st[1234])
+
+
+def test_gpu_work_census(engines):
+    """The census behind the roofline's FLOP count: every output ray of the
+    map is a polarized DELO-Bezier ray over N-1 depth steps, sees both
+    lines, and evaluates 13 + 3 Zeeman components per step."""
+    from pyrh_amd import roofline
+    wave = workloads.wavelengths_630()
+    eng = engines("lte", wave)
+    atm = workloads.stokes_map(40, seed=41)
+    ndep = np.full(40, 82, np.int32)
+    ndep[::3] = 60
+    eng.synth_lte(atm, ndep=ndep)
+    c = roofline.census_dict(eng.count_work())
+    steps = int(np.sum(ndep - 1)) * eng.nout
+    assert c["stokes"] == steps and c["bezier"] == 0 and c["feautrier"] == 0
+    assert c["line"] == 2 * steps
+    assert sum(c[r] for r in ("voigt_I", "voigt_II", "voigt_III", "voigt_IV")) == 16 * steps
+    assert c["armstrong"] == 0 and c["gauss"] == 0
+    assert c["coldepth"] == int(ndep.sum())
+    # the line window spans several Doppler widths: every region occurs
+    assert min(c["voigt_I"], c["voigt_II"], c["voigt_III"], c["voigt_IV"]) > 0
+    print(c, flush=True)
