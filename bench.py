@@ -6,10 +6,11 @@ Zeeman splitting, 112 wavelengths from 630.089 nm air at 2.15 pm, mu = 1),
 synthetic atmospheres from pyrh_amd.workloads.  One step = one synthesis of
 the whole map with the atmosphere already resident in HBM.
 
-Prints one JSON line: columns/s, ms per step, the roofline of the dominant
-kernel (k_formal: algorithmic FLOPs from the library's work census over its
-HIP-event launch time on the library's stream), and a CPU baseline timed on
-this host on a bounded sample of the same columns.
+Prints one JSON line: columns/s of the FP32 form (the plain FP64 form beside
+it), ms per step, the roofline of the dominant kernel (k_formal_stokes:
+algorithmic FLOPs from the library's work census over its HIP-event launch
+time on the library's stream), and a CPU baseline timed on this host on a
+bounded sample of the same columns.
 
     python bench.py [--steps K] [--warmup W] [--ncol N] [--no-cpu]
 """
@@ -83,7 +84,7 @@ def main():
     ap.add_argument("--ncol", type=int, default=16384)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-step", action="store_true",
-                    help="one synthesis and one 1 GiB device copy, for rocprofv3 --pmc")
+                    help="one FP32-form synthesis and one 1 GiB device copy, for rocprofv3 --pmc")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); columns are "
@@ -109,40 +110,45 @@ def main():
     d_out = torch.empty((a.ncol, 4, nout), dtype=torch.float64, device=dev)
     eng = Engine(mdl, device=0)
 
-    def step():
+    def step(precision=0):
         eng.synth_lte_device(d_atm.data_ptr(), d_ndep.data_ptr(), a.ncol, atm.shape[2],
-                             d_out.data_ptr())
+                             d_out.data_ptr(), precision=precision)
 
     if a.profile_step:
-        step()
+        step(1)
         x = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
         y = torch.empty_like(x)
         y.copy_(x)
         torch.cuda.synchronize()
-        print("profile step done: k_formal on 16384 columns; 1 GiB copy", flush=True)
+        print(f"profile step done: {a.ncol} columns; 1 GiB copy", flush=True)
         return
 
-    for _ in range(a.warmup):
-        step()
-    torch.cuda.synchronize()
-    stage_ms = []
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        step()
-        stage_ms.append(eng.timings())
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    def timed(precision):
+        for _ in range(a.warmup):
+            step(precision)
+        torch.cuda.synchronize()
+        stage_ms = []
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            step(precision)
+            stage_ms.append(eng.timings())
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        stages = {k: round(float(np.mean([s[k] for s in stage_ms])), 3) for k in stage_ms[0]}
+        return dt, stages
+
+    dt64, stages64 = timed(0)
+    fl = R.flops(eng.count_work(), mdl.scalars["nline"])
+    dt, stages = timed(1)
     ms_step = dt / a.steps * 1e3
     value = a.ncol * a.steps / dt
-
-    fl = R.flops(eng.count_work(), mdl.scalars["nline"])
-    formal_ms = float(np.mean([s["formal"] for s in stage_ms]))
+    formal_ms = stages["formal"]
     achieved = fl["formal_kernel"] / (formal_ms * 1e-3) / 1e12
-    stages = {k: round(float(np.mean([s[k] for s in stage_ms])), 3) for k in stage_ms[0]}
+    ach64 = fl["formal_kernel"] / (stages64["formal"] * 1e-3) / 1e12
 
     # host-buffer rate (PCIe both ways), reported beside the resident rate
     t1 = time.perf_counter()
-    eng.synth_lte(atm, ndep=ndep)
+    eng.synth_lte(atm, ndep=ndep, precision=1)
     host_rate = a.ncol / (time.perf_counter() - t1)
 
     cpu = None
@@ -161,18 +167,27 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (seeded LTE Stokes map, no input files beyond the line/atom data)",
+        "dtype": "f32",
+        "data": "synthetic (seeded LTE Stokes map; line and atom data from the run directory)",
         "config": {"workload": WORKLOAD if a.ncol == 16384 else f"lte_stokes_map_{a.ncol}",
                    "ncol": a.ncol, "ndep": int(atm.shape[2]), "nlambda": nout, "mu": 1.0,
-                   "lines": "Fe I 630.15 + 630.25 nm, Zeeman", "form": "plain FP64 Stokes"},
-        "roofline": {"bound": "valu", "kernel": "k_formal", "achieved": round(achieved, 3),
-                     "peak": R.PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / R.PEAK_FP64_TFLOPS, 4),
+                   "lines": "Fe I 630.15 + 630.25 nm, Zeeman",
+                   "form": "FP32 form: float line opacity and DELO-Bezier recursion; FP64 "
+                           "equilibria, heights, path lengths and Bezier weights",
+                   "response_functions": "not built"},
+        "roofline": {"bound": "valu", "kernel": "k_formal_stokes<float>",
+                     "achieved": round(achieved, 3), "peak": R.PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / R.PEAK_FP32_TFLOPS, 4),
                      "traffic": _traffic_from_profile(a.ncol),
                      "flops_per_launch": fl["formal_kernel"], "launch_ms": round(formal_ms, 3)},
         "cpu_baseline": cpu,
         "stage_ms": stages,
+        "fp64_plain": {"value": round(a.ncol * a.steps / dt64, 1), "unit": "columns/s",
+                       "ms_per_step": round(dt64 / a.steps * 1e3, 3), "stage_ms": stages64,
+                       "roofline": {"kernel": "k_formal_stokes<double>",
+                                    "achieved": round(ach64, 3), "peak": R.PEAK_FP64_TFLOPS,
+                                    "unit": "TFLOP/s",
+                                    "frac": round(ach64 / R.PEAK_FP64_TFLOPS, 4)}},
         "pipeline_flops": fl["total"],
         "host_buffers_columns_per_s": round(host_rate, 1),
     }

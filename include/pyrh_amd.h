@@ -204,13 +204,19 @@ int32_t rhg_abi_version(void);
 int rhg_open(const rhg_model_t *model, int device, rhg_context_t **ctx);
 int rhg_close(rhg_context_t *ctx);
 
+/* Arithmetic of rhg_synth_lte.  FP64: every stage in double.  FP32: the
+ * line opacity (Voigt and Faraday-Voigt profiles), the continuum sums'
+ * storage and the DELO-Bezier recursion in float; equilibria, heights,
+ * path lengths, the Bezier/w3 weights (they cancel at small dtau) and
+ * the Feautrier recursion stay double. */
+enum rhg_precision { RHG_PREC_FP64 = 0, RHG_PREC_FP32 = 1 };
+
 /* LTE synthesis of a batch of columns: the work of one rhf1d() call per
  * column with no active atom (pyrh_compute1dray.c:224-348, formal.c:44-346),
  * emergent Stokes vectors at every wavelength of the model except the
  * reference one.  stokes_out is [ncol][4][nspect-1] float64 (I,Q,U,V),
- * host or device memory (device if `device_ptrs` != 0).  `precision` 0 =
- * FP64 throughout, 1 = FP32 formal solution and line opacity with FP64
- * equilibria.  Synchronous. */
+ * host or device memory (device if `device_ptrs` != 0).  `precision` is an
+ * rhg_precision.  Synchronous. */
 int rhg_synth_lte(rhg_context_t *ctx, const rhg_columns_t *cols,
                   int32_t precision, int32_t device_ptrs, double *stokes_out);
 

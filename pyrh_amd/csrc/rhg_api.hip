@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -47,10 +48,15 @@ struct DevBuf {
 struct rhg_context {
   int device = 0;
   hipStream_t stream = nullptr;
-  rhg::DevModel dm{};
+  rhg::DevModel dm{};                 // host copy (device pointers inside)
+  const rhg::DevModel *d_model = nullptr;  // the same struct in device memory
   std::vector<void *> allocs;   // model arrays
   int nspect = 0, nout = 0, ncont = 0, nline = 0, natom = 0;
   int32_t *d_out_ls = nullptr;
+  // output-wavelength positions by solver class: polarized (STOKES) and
+  // scalar (BEZIER / FEAUTRIER)
+  int n_pol = 0, n_scal = 0;
+  const int32_t *d_io_pol = nullptr, *d_io_scal = nullptr;
   DevBuf atmos, ndep, state, moving, cchi, ceta, stokes, chiref, census, census_sum;
   hipEvent_t ev[RHG_NSTAGE + 2]{};
   float last_ms[RHG_NSTAGE + 1]{};
@@ -224,14 +230,80 @@ int rhg_open(const rhg_model_t *m, int device, rhg_context_t **out) {
 #undef UP
   if (rc) { std::string e = g_err; rhg_close(c); return fail(rc, e); }
 
-  c->nspect = ns;
-  c->nout = ns - 1;
-  c->ncont = m->ncont;
-  c->nline = m->nline;
-  c->natom = m->natom;
+  // derived tables: Doppler offsets per (wavelength, line) entry and the
+  // bound-free cross-sections tiled over output wavelengths
   std::vector<int32_t> ols;
   for (int i = 0; i < ns; i++)
     if (i != m->ref_index) ols.push_back(i);
+  const int nout = (int) ols.size();
+  if (nout > 65535) { rhg_close(c); return fail(RHG_EUNSUPPORTED, "more than 65535 output wavelengths"); }
+  {
+    const int nent = m->lam_line_ptr[ns];
+    std::vector<double> dv((size_t) (nent > 0 ? nent : 1));
+    for (int ls = 0; ls < ns; ls++) {
+      const double lambda = m->lamconst[(size_t) ls * RHG_LC_NCONST + RHG_LC_LAMBDA];
+      for (int e = m->lam_line_ptr[ls]; e < m->lam_line_ptr[ls + 1]; e++)
+        dv[e] = (lambda / m->line_lambda0[m->lam_line_idx[e]] - 1.0) * rhg::CLIGHT;
+    }
+    rc = upload(c, dv.data(), dv.size(), &d.lam_line_dv);
+  }
+  if (!rc) {
+    const int T = rhg::BF_TILE;
+    const int ntile = (nout + T - 1) / T;
+    std::vector<int32_t> tptr(1, 0), tcont;
+    std::vector<double> talpha;
+    std::vector<int> slot(m->ncont > 0 ? m->ncont : 1);
+    for (int t = 0; t < ntile; t++) {
+      std::fill(slot.begin(), slot.end(), -1);
+      const int base = (int) tcont.size();
+      for (int j = 0; j < T && t * T + j < nout; j++) {
+        const int ls = ols[t * T + j];
+        for (int e = m->lam_cont_ptr[ls]; e < m->lam_cont_ptr[ls + 1]; e++) {
+          const int ci = m->lam_cont_idx[e];
+          if (slot[ci] < 0) {
+            slot[ci] = (int) tcont.size() - base;
+            tcont.push_back(ci);
+            talpha.resize(talpha.size() + T, 0.0);
+          }
+          talpha[(size_t) (base + slot[ci]) * T + j] = m->lam_cont_alpha[e];
+        }
+      }
+      tptr.push_back((int32_t) tcont.size());
+    }
+    d.ntile = ntile;
+    rc = upload(c, tptr.data(), tptr.size(), &d.tile_ptr);
+    if (!rc) rc = upload(c, tcont.empty() ? nullptr : tcont.data(), tcont.size(), &d.tile_cont);
+    if (!rc) rc = upload(c, talpha.empty() ? nullptr : talpha.data(), talpha.size(), &d.tile_alpha);
+  }
+  if (!rc) {
+    std::vector<double> cnt((size_t) (m->nmol > 0 ? m->nmol : 1) * rhg::NNUC_MAX, 0.0);
+    for (int im = 0; im < m->nmol; im++)
+      for (int j = 0; j < m->mol_nel[im]; j++) {
+        const int nu = m->mol_el_nucl[4 * im + j];
+        if (nu < 0 || nu >= m->nnuclei) { rc = fail(RHG_EINVAL, "molecule nucleus index out of range"); break; }
+        cnt[(size_t) im * rhg::NNUC_MAX + nu] += m->mol_el_count[4 * im + j];
+      }
+    if (!rc) rc = upload(c, cnt.data(), cnt.size(), &d.mol_cnt);
+  }
+  if (rc) { std::string e = g_err; rhg_close(c); return fail(rc, e); }
+
+  {
+    std::vector<int32_t> pol, scal;
+    for (int io = 0; io < nout; io++)
+      (m->solver[ols[io]] == RHG_SOLVER_STOKES ? pol : scal).push_back(io);
+    c->n_pol = (int) pol.size();
+    c->n_scal = (int) scal.size();
+    rc = upload(c, pol.empty() ? nullptr : pol.data(), pol.size(), &c->d_io_pol);
+    if (!rc) rc = upload(c, scal.empty() ? nullptr : scal.data(), scal.size(), &c->d_io_scal);
+    if (rc) { std::string e = g_err; rhg_close(c); return fail(rc, e); }
+  }
+  if (upload(c, &c->dm, 1, &c->d_model)) { std::string e = g_err; rhg_close(c); return fail(RHG_ENOMEM, e); }
+
+  c->nspect = ns;
+  c->nout = nout;
+  c->ncont = m->ncont;
+  c->nline = m->nline;
+  c->natom = m->natom;
   if (hipMalloc(&c->d_out_ls, ols.size() * sizeof(int32_t)) != hipSuccess) {
     rhg_close(c);
     return fail(RHG_ENOMEM, "hipMalloc (wavelength index)");
@@ -250,9 +322,11 @@ static int synth(rhg_context_t *c, const rhg_columns_t *cols, int32_t precision,
                  int32_t device_ptrs, double *stokes_out, double *height_out,
                  double *chiref_out) {
   if (!c || !cols || !stokes_out) return fail(RHG_EINVAL, "null argument");
-  if (precision != 0) return fail(RHG_EUNSUPPORTED, "only precision 0 (FP64) is implemented");
+  if (precision != RHG_PREC_FP64 && precision != RHG_PREC_FP32)
+    return fail(RHG_EINVAL, "precision must be RHG_PREC_FP64 or RHG_PREC_FP32");
   const int ncol = cols->ncol, Nmax = cols->ndep_max, nout = c->nout;
   if (ncol < 1 || Nmax < 4 || Nmax > RHG_MAX_DEPTH) return fail(RHG_EINVAL, "bad batch shape");
+  if ((int64_t) ncol * Nmax > (int64_t) INT32_MAX) return fail(RHG_EINVAL, "batch too large");
   if (!(cols->mu > 0.0 && cols->mu <= 1.0)) return fail(RHG_EINVAL, "mu must be in (0, 1]");
   if (cols->atm_scale < 0 || cols->atm_scale > 2) return fail(RHG_EINVAL, "bad atm_scale");
   std::vector<int32_t> hndep(ncol);
@@ -265,10 +339,11 @@ static int synth(rhg_context_t *c, const rhg_columns_t *cols, int32_t precision,
     if (hndep[i] < 4 || hndep[i] > Nmax) return fail(RHG_EINVAL, "ndep out of range");
   HIPCHECK(hipSetDevice(c->device));
 
-  const int nfield = RHG_CF_NFIXED + 3 * c->ncont + 4 * c->nline + 1;
+  const int nfield = rhg::state_nfield(c->ncont, c->nline);
+  const size_t rsz = precision == RHG_PREC_FP32 ? sizeof(float) : sizeof(double);
   const size_t atm_b = (size_t) ncol * 9 * Nmax * sizeof(double);
   const size_t st_b = (size_t) ncol * nfield * Nmax * sizeof(double);
-  const size_t cc_b = (size_t) ncol * Nmax * nout * sizeof(double);
+  const size_t cc_b = (size_t) ncol * Nmax * nout * rsz;
   const size_t out_b = (size_t) ncol * 4 * nout * sizeof(double);
   if (c->state.ensure(st_b) || c->moving.ensure(ncol * sizeof(int32_t)) ||
       c->cchi.ensure(cc_b) || c->ceta.ensure(cc_b) ||
@@ -288,57 +363,83 @@ static int synth(rhg_context_t *c, const rhg_columns_t *cols, int32_t precision,
   }
   double *state = (double *) c->state.p;
   int32_t *moving = (int32_t *) c->moving.p;
-  double *cchi = (double *) c->cchi.p, *ceta = (double *) c->ceta.p;
   double *chiref = chiref_out ? (double *) c->chiref.p : nullptr;
+  const int64_t npts = (int64_t) ncol * Nmax;
 
   HIPCHECK(hipEventRecord(c->ev[0], c->stream));
   {
-    const int64_t nthr = (int64_t) ncol * Nmax;
-    dim3 blk(64), grd((unsigned) ((nthr + 63) / 64));
-    hipLaunchKernelGGL(rhg::k_state, grd, blk, 0, c->stream, c->dm, d_atm, d_ndep, ncol, Nmax,
-                       cols->mu, cols->atm_scale, state, nfield);
+    dim3 blk(64), grd((unsigned) ((npts + 63) / 64));
+    const size_t lds = (size_t) 3 * c->dm.nmol * 64 * sizeof(double);
+    switch (c->dm.nnuclei) {
+#define RHG_LAUNCH_STATE(NN)                                                                   \
+  case NN:                                                                                     \
+    hipLaunchKernelGGL(rhg::k_state<NN>, grd, blk, lds, c->stream, c->d_model, d_atm, d_ndep, ncol, \
+                       Nmax, cols->mu, cols->atm_scale, state, nfield);                        \
+    break;
+      RHG_LAUNCH_STATE(1) RHG_LAUNCH_STATE(2) RHG_LAUNCH_STATE(3) RHG_LAUNCH_STATE(4)
+      RHG_LAUNCH_STATE(5) RHG_LAUNCH_STATE(6) RHG_LAUNCH_STATE(7) RHG_LAUNCH_STATE(8)
+#undef RHG_LAUNCH_STATE
+      default: return fail(RHG_EUNSUPPORTED, "number of molecule nuclei out of range");
+    }
     HIPCHECK(hipGetLastError());
   }
   HIPCHECK(hipEventRecord(c->ev[1], c->stream));
   {
     dim3 blk(64), grd((unsigned) ((ncol + 63) / 64));
-    hipLaunchKernelGGL(rhg::k_geometry, grd, blk, 0, c->stream, c->dm, d_ndep, ncol, Nmax,
+    hipLaunchKernelGGL(rhg::k_geometry, grd, blk, 0, c->stream, c->d_model, d_ndep, ncol, Nmax,
                        cols->atm_scale, cols->mu, state, nfield, moving, chiref);
     HIPCHECK(hipGetLastError());
   }
   HIPCHECK(hipEventRecord(c->ev[2], c->stream));
-  const int CH = 65535;
-  for (int c0 = 0; c0 < ncol; c0 += CH) {
-    const int nc = (ncol - c0 < CH) ? ncol - c0 : CH;
-    dim3 blk(64), grd((unsigned) ((nout + 63) / 64), (unsigned) Nmax, (unsigned) nc);
-    hipLaunchKernelGGL(rhg::k_continuum, grd, blk, 0, c->stream, c->dm, d_ndep + c0, nc, Nmax,
-                       state + (size_t) c0 * nfield * Nmax, nfield, c->d_out_ls, nout,
-                       cchi + (size_t) c0 * Nmax * nout, ceta + (size_t) c0 * Nmax * nout);
+  {
+    dim3 blk(rhg::CONT_BLOCK), grd((unsigned) ((npts + rhg::CONT_BLOCK - 1) / rhg::CONT_BLOCK));
+    if (precision == RHG_PREC_FP32)
+      hipLaunchKernelGGL(rhg::k_continuum<float>, grd, blk, 0, c->stream, c->d_model, d_ndep, ncol, Nmax,
+                         state, nfield, c->d_out_ls, nout, (float *) c->cchi.p, (float *) c->ceta.p);
+    else
+      hipLaunchKernelGGL(rhg::k_continuum<double>, grd, blk, 0, c->stream, c->d_model, d_ndep, ncol, Nmax,
+                         state, nfield, c->d_out_ls, nout, (double *) c->cchi.p, (double *) c->ceta.p);
     HIPCHECK(hipGetLastError());
   }
   HIPCHECK(hipEventRecord(c->ev[3], c->stream));
-  for (int c0 = 0; c0 < ncol; c0 += CH) {
-    const int nc = (ncol - c0 < CH) ? ncol - c0 : CH;
-    dim3 blk(64), grd((unsigned) ((nout + 63) / 64), (unsigned) nc);
-    hipLaunchKernelGGL(rhg::k_formal, grd, blk, 0, c->stream, c->dm, d_ndep + c0, moving + c0, nc,
-                       Nmax, state + (size_t) c0 * nfield * Nmax, nfield, c->d_out_ls, nout,
-                       cchi + (size_t) c0 * Nmax * nout, ceta + (size_t) c0 * Nmax * nout,
-                       cols->mu, d_out + (size_t) c0 * 4 * nout);
-    HIPCHECK(hipGetLastError());
+  {
+    const int ncb = (ncol + 63) / 64;
+    dim3 blk(64);
+#define RHG_LAUNCH_FORMAL(KNAME, LIST, NLIST)                                                    \
+  if (NLIST > 0) {                                                                               \
+    dim3 grd((unsigned) rhg::xcd_grid(ncb, NLIST));                                              \
+    if (precision == RHG_PREC_FP32)                                                              \
+      hipLaunchKernelGGL(rhg::KNAME<float>, grd, blk, 0, c->stream, c->d_model, d_ndep, moving,  \
+                         ncol, Nmax, state, c->d_out_ls, LIST, NLIST, nout,                     \
+                         (const float *) c->cchi.p, (const float *) c->ceta.p, cols->mu, d_out); \
+    else                                                                                         \
+      hipLaunchKernelGGL(rhg::KNAME<double>, grd, blk, 0, c->stream, c->d_model, d_ndep, moving, \
+                         ncol, Nmax, state, c->d_out_ls, LIST, NLIST, nout,                     \
+                         (const double *) c->cchi.p, (const double *) c->ceta.p, cols->mu,      \
+                         d_out);                                                                 \
+    HIPCHECK(hipGetLastError());                                                                 \
+  }
+    RHG_LAUNCH_FORMAL(k_formal_stokes, c->d_io_pol, c->n_pol)
+    RHG_LAUNCH_FORMAL(k_formal_scalar, c->d_io_scal, c->n_scal)
+#undef RHG_LAUNCH_FORMAL
   }
   HIPCHECK(hipEventRecord(c->ev[4], c->stream));
   if (!device_ptrs)
     HIPCHECK(hipMemcpyAsync(stokes_out, d_out, out_b, hipMemcpyDeviceToHost, c->stream));
+  std::vector<double> hfield;
   if (height_out) {
-    for (int ic = 0; ic < ncol; ic++)
-      HIPCHECK(hipMemcpyAsync(height_out + (size_t) ic * Nmax,
-                              state + ((size_t) ic * nfield + RHG_CF_HEIGHT) * Nmax,
-                              Nmax * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    hfield.resize((size_t) Nmax * ncol);
+    HIPCHECK(hipMemcpyAsync(hfield.data(), state + (size_t) RHG_CF_HEIGHT * Nmax * ncol,
+                            hfield.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   }
   if (chiref_out)
     HIPCHECK(hipMemcpyAsync(chiref_out, chiref, (size_t) ncol * Nmax * sizeof(double),
                             hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
+  if (height_out)
+    for (int ic = 0; ic < ncol; ic++)
+      for (int k = 0; k < Nmax; k++)
+        height_out[(size_t) ic * Nmax + k] = k < hndep[ic] ? hfield[(size_t) k * ncol + ic] : 0.0;
   c->last_ncol = ncol;
   c->last_Nmax = Nmax;
   c->last_nfield = nfield;
@@ -375,14 +476,11 @@ int rhg_count_work(rhg_context_t *c, int64_t *counts_out) {
     return fail(RHG_ENOMEM, "device allocation for the work census failed");
   int32_t *census = (int32_t *) c->census.p;
   const double *state = (const double *) c->state.p;
-  const int CH = 65535;
-  for (int c0 = 0; c0 < ncol; c0 += CH) {
-    const int nc = (ncol - c0 < CH) ? ncol - c0 : CH;
-    dim3 blk(64), grd((unsigned) ((nout + 63) / 64), (unsigned) nc);
-    hipLaunchKernelGGL(rhg::k_census, grd, blk, 0, c->stream, c->dm, c->last_ndep + c0,
-                       (const int32_t *) c->moving.p + c0, nc, Nmax,
-                       state + (size_t) c0 * nfield * Nmax, nfield, c->d_out_ls, nout, c->last_mu,
-                       census + (size_t) c0 * nout * RHG_NWORK);
+  {
+    dim3 blk(64), grd((unsigned) ((ncol + 63) / 64), (unsigned) nout);
+    hipLaunchKernelGGL(rhg::k_census, grd, blk, 0, c->stream, c->d_model, c->last_ndep,
+                       (const int32_t *) c->moving.p, ncol, Nmax, state, nfield, c->d_out_ls, nout,
+                       c->last_mu, census);
     HIPCHECK(hipGetLastError());
   }
   {

@@ -30,6 +30,7 @@ constexpr double G_TO_KG = 1.0E-03;
 constexpr double ERG_TO_JOULE = 1.0E-07;
 constexpr double PI = 3.14159265358979;
 constexpr double SQRTPI = 1.77245385090551;
+constexpr double INV_SQRTPI = 1.0 / SQRTPI;
 constexpr double LG10 = 2.30258509299404568402;
 constexpr double MAX_EXPONENT = 150.0;
 constexpr double MAX_GAUSS_DOPPLER = 7.0;
@@ -87,8 +88,9 @@ __device__ __forceinline__ double clamp_index(int n, const double *t, double v) 
 // ---------------------------------------------------------------------------
 // Voigt and Faraday-Voigt functions: Humlicek (1982) regions I-IV with the
 // region test of rh/voigt.c:396-413 and the rational forms of
-// rh/humlicek.c:28-120, in complex arithmetic written out in real pairs
-// (products and the quotient as rh/complex.c:45-100 form them).
+// rh/humlicek.c:28-120, in complex arithmetic written out in real pairs.
+// R = double follows the products and the quotient of rh/complex.c:45-100;
+// R = float is the FP32 form.
 template <typename T>
 struct cpx { T r, i; };
 
@@ -96,68 +98,88 @@ template <typename T>
 __device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
   return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r};
 }
+
+// a / b with one reciprocal of |b|^2
 template <typename T>
 __device__ __forceinline__ cpx<T> cdiv(cpx<T> a, cpx<T> b) {
-  T d = b.r * b.r + b.i * b.i;
-  return {(a.r * b.r + a.i * b.i) / d, (a.i * b.r - a.r * b.i) / d};
+  const T id = T(1) / (b.r * b.r + b.i * b.i);
+  return {(a.r * b.r + a.i * b.i) * id, (a.i * b.r - a.r * b.i) * id};
+}
+
+__device__ __forceinline__ double rexp(double x) { return exp(x); }
+__device__ __forceinline__ float rexp(float x) { return __expf(x); }
+__device__ __forceinline__ void rsincos(double x, double *s, double *c) { sincos(x, s, c); }
+// FP32 form: the hardware sine/cosine (the argument here is 2 a v with
+// a v small inside Humlicek region IV)
+__device__ __forceinline__ void rsincos(float x, float *s, float *c) {
+  *s = __sinf(x);
+  *c = __cosf(x);
 }
 
 // Humlicek's region (0..3 for I..IV) of the point z = a - i v, the test of
 // rh/voigt.c:396-413; voigt_humlicek() branches on exactly this test.
-__device__ __forceinline__ int humlicek_region(double a, double v) {
-  const double s = fabs(v) + a;
-  if (s >= 15.0) return 0;
-  if (s >= 5.5) return 1;
-  if (a >= 0.195 * fabs(v) - 0.176) return 2;
+template <typename R>
+__device__ __forceinline__ int humlicek_region(R a, R v) {
+  const R s = fabs(v) + a;
+  if (s >= R(15.0)) return 0;
+  if (s >= R(5.5)) return 1;
+  if (a >= R(0.195) * fabs(v) - R(0.176)) return 2;
   return 3;
 }
 
-__device__ __forceinline__ double voigt_humlicek(double a, double v, double *F) {
-  cpx<double> z = {a, -v}, W, z1, z2, u;
+// Each region ends in one complex quotient num/den (region IV: z num/den,
+// subtracted from exp(-u)); the branches form num and den only and the one
+// quotient is taken after they rejoin, so a wavefront whose lanes sit in
+// several regions pays for the quotient once.  The operations and their
+// order per lane are those of the per-region formulas.
+template <typename R>
+__device__ __forceinline__ R voigt_humlicek(R a, R v, R *F) {
+  cpx<R> z = {a, -v}, z1, z2, u, e = {R(0), R(0)};
   const int region = humlicek_region(a, v);
   if (region == 0) {
-    z1 = {0.5641896 * z.r, 0.5641896 * z.i};
+    z1 = {R(0.5641896) * z.r, R(0.5641896) * z.i};
     z2 = cmul(z, z);
-    z2.r = z2.r + 0.5;
-    W = cdiv(z1, z2);
+    z2.r = z2.r + R(0.5);
   } else if (region == 1) {
     u = cmul(z, z);
-    z1 = {0.5641896 * u.r, 0.5641896 * u.i};
-    z1.r = z1.r + 1.410474;
+    z1 = {R(0.5641896) * u.r, R(0.5641896) * u.i};
+    z1.r = z1.r + R(1.410474);
     z1 = cmul(z, z1);
-    cpx<double> u3 = {u.r + 3.0, u.i};
+    cpx<R> u3 = {u.r + R(3.0), u.i};
     z2 = cmul(u, u3);
-    z2.r = z2.r + 0.75;
-    W = cdiv(z1, z2);
+    z2.r = z2.r + R(0.75);
   } else if (region == 2) {
-    const double a3[5] = {0.5642236, 3.778987, 11.96482, 20.20933, 16.4955};
-    const double b3[5] = {6.699398, 21.69274, 39.27121, 38.82363, 16.4955};
-    z1 = {a3[0], 0.0};
+    const R a3[5] = {R(0.5642236), R(3.778987), R(11.96482), R(20.20933), R(16.4955)};
+    const R b3[5] = {R(6.699398), R(21.69274), R(39.27121), R(38.82363), R(16.4955)};
+    z1 = {a3[0], R(0.0)};
     z2 = {z.r + b3[0], z.i};
 #pragma unroll
     for (int n = 1; n < 5; n++) {
       z1 = cmul(z1, z); z1.r = z1.r + a3[n];
       z2 = cmul(z2, z); z2.r = z2.r + b3[n];
     }
-    W = cdiv(z1, z2);
   } else {
-    const double a4[7] = {0.56419, 1.320522, 35.7668, 219.031, 1540.787, 3321.99, 36183.31};
-    const double b4[7] = {1.841439, 61.57037, 364.2191, 2186.181, 9022.228, 24322.84, 32066.6};
-    cpx<double> mz = {-1.0 * z.r, -1.0 * z.i};
+    const R a4[7] = {R(0.56419), R(1.320522), R(35.7668), R(219.031), R(1540.787), R(3321.99),
+                     R(36183.31)};
+    const R b4[7] = {R(1.841439), R(61.57037), R(364.2191), R(2186.181), R(9022.228), R(24322.84),
+                     R(32066.6)};
+    cpx<R> mz = {R(-1.0) * z.r, R(-1.0) * z.i};
     u = cmul(z, mz);
-    z1 = {a4[0], 0.0};
+    z1 = {a4[0], R(0.0)};
     z2 = {u.r + b4[0], u.i};
 #pragma unroll
     for (int n = 1; n < 7; n++) {
       z1 = cmul(u, z1); z1.r = z1.r + a4[n];
       z2 = cmul(u, z2); z2.r = z2.r + b4[n];
     }
-    cpx<double> mu = {-1.0 * u.r, -1.0 * u.i};
-    double ex = exp(mu.r);
-    cpx<double> e = {ex * cos(mu.i), ex * sin(mu.i)};
-    cpx<double> q = cdiv(cmul(z, z1), z2);
-    W = {e.r - q.r, e.i - q.i};
+    cpx<R> mu = {R(-1.0) * u.r, R(-1.0) * u.i};
+    R ex = rexp(mu.r), sn, cs;
+    rsincos(mu.i, &sn, &cs);
+    e = {ex * cs, ex * sn};
+    z1 = cmul(z, z1);
   }
+  cpx<R> W = cdiv(z1, z2);
+  if (region == 3) W = {e.r - W.r, e.i - W.i};
   *F = W.i;
   return W.r;
 }
@@ -181,7 +203,7 @@ __device__ constexpr double c_k1[34] = {
     -0.696450e-10, 0.156262e-10, -0.33897e-11, 0.7116e-12, -0.1447e-12, 0.285e-13,
     -0.55e-14, 0.10e-14, -0.2e-15};
 
-__device__ inline double voigt_armstrong(double a, double v) {
+__device__ __noinline__ double voigt_armstrong(double a, double v) {
   if (v < 0.0) v = -v;
   if ((a < 1.0 && v < 4.0) || (a < 1.8 / (v + 1.0))) {
     const double TWOSQRTPI = 1.12837917, EXPMAX = 70.0, TINY = 1.0E-08;
@@ -266,15 +288,25 @@ __device__ __forceinline__ void w3(double dtau, double *w) {
   }
 }
 
-// Fritsch & Butland (1984), rh/bezier_aux.c:34-51
-__device__ __forceinline__ double cent_deriv(double dsup, double dsdn, double up, double c,
-                                             double dn) {
-  double fim1 = (c - up) / dsup, fi = (dn - c) / dsdn;
-  if (fim1 * fi > 0) {
-    double alpha = 0.333333333333333333333333 * (1.0 + dsdn / (dsdn + dsup));
-    return (fim1 * fi) / ((1.0 - alpha) * fim1 + alpha * fi);
+// Fritsch & Butland (1984) monotone central derivative, rh/bezier_aux.c:34-51.
+// CentDeriv holds what depends only on the two intervals, so that the
+// derivatives of several quantities on the same intervals share it.
+template <typename R>
+struct CentDeriv {
+  R idup, idn, alpha;
+  __device__ __forceinline__ CentDeriv(R dsup, R dsdn)
+      : idup(R(1) / dsup), idn(R(1) / dsdn),
+        alpha(R(0.333333333333333333333333) * (R(1.0) + dsdn / (dsdn + dsup))) {}
+  __device__ __forceinline__ R operator()(R up, R c, R dn) const {
+    const R fim1 = (c - up) * idup, fi = (dn - c) * idn;
+    if (fim1 * fi > R(0)) return (fim1 * fi) / ((R(1.0) - alpha) * fim1 + alpha * fi);
+    return R(0.0);
   }
-  return 0.0;
+};
+
+template <typename R>
+__device__ __forceinline__ R cent_deriv(R dsup, R dsdn, R up, R c, R dn) {
+  return CentDeriv<R>(dsup, dsdn)(up, c, dn);
 }
 
 // cubic Bezier integration weights, rh/bezier_aux.c:333-359

@@ -1,8 +1,16 @@
-// Stage 4: emergent-ray formal solution, one thread per (column,
-// wavelength).  Opacity and source functions are formed depth by depth,
-// two points ahead of the recursion, from the stage-3 continuum and the
-// per-depth line constants of the column state; nothing per depth is kept
-// except the rolling window the recursion needs.
+// Stage 4: emergent-ray formal solution.  One thread per (column,
+// wavelength); a wavefront is 64 columns at one wavelength, so the
+// wavelength's line list, Zeeman pattern and solver are wave-uniform and the
+// Humlicek branch taken by a lane follows its own column's Doppler offset
+// (columns at one wavelength mostly share a branch, where wavelengths
+// across a line do not).  Opacity and source functions are formed depth by
+// depth, two points ahead of the recursion, from the stage-3 continuum and
+// the per-depth line constants of the column state; nothing per depth is
+// kept except the rolling window the recursion needs.
+//
+// Precision R: double is the plain form; float is the FP32 form, in which the
+// Bezier and w3 weights (cancellation at small dtau), the geometric path
+// lengths (differences of heights) and the Feautrier recursion stay FP64.
 #include "rhg_kernels.hpp"
 
 namespace rhg {
@@ -10,365 +18,381 @@ namespace rhg {
 namespace {
 
 // Opacity, source functions and reduced propagation-matrix parameters of
-// the emergent ray at depth k (rh/background.c:519-546 sums, formal.c:178-
-// 208 source functions, stokesopac.c:28-88 K').
+// the emergent ray at one depth (rh/background.c:519-546 sums, formal.c:178-
+// 208 source functions, stokesopac.c:28-88 K' with magneto-optical terms
+// off: K' = [[0, q^T], [q, 0]]).
+template <typename R>
 struct Point {
-  double chi;     // chi_I
-  double S[4];    // eta_n / chi_I
-  double q[3];    // chi_Q,U,V / chi_I  (K'[0][1..3])
+  R chi;     // chi_I
+  R S[4];    // eta_n / chi_I
+  R q[3];    // chi_Q,U,V / chi_I
 };
 
+template <typename R>
 __device__ __forceinline__ void form_point(const DevModel &M, int ls, int has_line,
-                                           const double *st, int64_t fs, double mu, int moving,
-                                           double cchi, double ceta, Point &p) {
-  double chi4[4], eta4[4];
+                                           const double *st, int64_t fs, R mu, int moving,
+                                           R cchi, R ceta, Point<R> &p) {
+  R chi4[4], eta4[4];
   chi4[0] = cchi; eta4[0] = ceta;
-  chi4[1] = chi4[2] = chi4[3] = 0.0;
-  eta4[1] = eta4[2] = eta4[3] = 0.0;
+  chi4[1] = chi4[2] = chi4[3] = R(0);
+  eta4[1] = eta4[2] = eta4[3] = R(0);
   if (has_line) {
-    double lI, lE, lq[3], le[3];
-    line_point(M, ls, st, fs, mu, lI, lq, lE, le, nullptr, moving);
+    R lI, lE, lq[3], le[3];
+    line_point<R>(M, ls, st, fs, mu, lI, lq, lE, le, moving);
     chi4[0] += lI; eta4[0] += lE;
+#pragma unroll
     for (int n = 0; n < 3; n++) { chi4[n + 1] += lq[n]; eta4[n + 1] += le[n]; }
   }
   p.chi = chi4[0];
-  for (int n = 0; n < 4; n++) p.S[n] = (0.0 + eta4[n]) / chi4[0];
-  for (int n = 0; n < 3; n++) p.q[n] = chi4[n + 1] / p.chi;
+  const R ichi = R(1) / chi4[0];
+#pragma unroll
+  for (int n = 0; n < 4; n++) p.S[n] = eta4[n] * ichi;
+#pragma unroll
+  for (int n = 0; n < 3; n++) p.q[n] = chi4[n + 1] * ichi;
 }
 
-__device__ __forceinline__ void make_K(const Point &p, double K[4][4]) {
-#pragma unroll
-  for (int j = 0; j < 4; j++)
-#pragma unroll
-    for (int i = 0; i < 4; i++) K[j][i] = 0.0;
-  K[0][1] = p.q[0]; K[0][2] = p.q[1]; K[0][3] = p.q[2];
-  K[1][0] = K[0][1]; K[2][0] = K[0][2]; K[3][0] = K[0][3];
+// y = K(q) x for K(q) = [[0, q^T], [q, 0]]
+template <typename R>
+__device__ __forceinline__ void kmul(const R *q, const R *x, R *y) {
+  y[0] = q[0] * x[1] + q[1] * x[2] + q[2] * x[3];
+  y[1] = q[0] * x[0];
+  y[2] = q[1] * x[0];
+  y[3] = q[2] * x[0];
 }
 
-// 4x4 inverse by cofactors with the pairing of rh/bezier_aux.c:235-328
-// (FP64); mat is row-major and replaced by its inverse.
-__device__ __forceinline__ void matinv4(double *mat) {
-  double tmp[12], src[16], dst[16], det;
+// Solve (I + c K(q) + g (K(q)^2 + K(dq))) x = V.  The matrix is
+// [[a, b^T], [b, I + g q q^T]] with a = 1 + g |q|^2 and b = c q + g dq; the
+// lower block is inverted by Sherman-Morrison and the first component
+// follows from its Schur complement.
+template <typename R>
+__device__ __forceinline__ void solve_md(R c, R g, const R *q, const R *dq, const R *V, R *x) {
+  const R qq = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+  const R a = R(1) + g * qq;
+  R b[3];
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    src[i] = mat[i * 4]; src[i + 4] = mat[i * 4 + 1];
-    src[i + 8] = mat[i * 4 + 2]; src[i + 12] = mat[i * 4 + 3];
-  }
-  tmp[0] = src[10] * src[15]; tmp[1] = src[11] * src[14];
-  tmp[2] = src[9] * src[15];  tmp[3] = src[11] * src[13];
-  tmp[4] = src[9] * src[14];  tmp[5] = src[10] * src[13];
-  tmp[6] = src[8] * src[15];  tmp[7] = src[11] * src[12];
-  tmp[8] = src[8] * src[14];  tmp[9] = src[10] * src[12];
-  tmp[10] = src[8] * src[13]; tmp[11] = src[9] * src[12];
-  dst[0] = tmp[0]*src[5] + tmp[3]*src[6] + tmp[4]*src[7];
-  dst[0] -= tmp[1]*src[5] + tmp[2]*src[6] + tmp[5]*src[7];
-  dst[1] = tmp[1]*src[4] + tmp[6]*src[6] + tmp[9]*src[7];
-  dst[1] -= tmp[0]*src[4] + tmp[7]*src[6] + tmp[8]*src[7];
-  dst[2] = tmp[2]*src[4] + tmp[7]*src[5] + tmp[10]*src[7];
-  dst[2] -= tmp[3]*src[4] + tmp[6]*src[5] + tmp[11]*src[7];
-  dst[3] = tmp[5]*src[4] + tmp[8]*src[5] + tmp[11]*src[6];
-  dst[3] -= tmp[4]*src[4] + tmp[9]*src[5] + tmp[10]*src[6];
-  dst[4] = tmp[1]*src[1] + tmp[2]*src[2] + tmp[5]*src[3];
-  dst[4] -= tmp[0]*src[1] + tmp[3]*src[2] + tmp[4]*src[3];
-  dst[5] = tmp[0]*src[0] + tmp[7]*src[2] + tmp[8]*src[3];
-  dst[5] -= tmp[1]*src[0] + tmp[6]*src[2] + tmp[9]*src[3];
-  dst[6] = tmp[3]*src[0] + tmp[6]*src[1] + tmp[11]*src[3];
-  dst[6] -= tmp[2]*src[0] + tmp[7]*src[1] + tmp[10]*src[3];
-  dst[7] = tmp[4]*src[0] + tmp[9]*src[1] + tmp[10]*src[2];
-  dst[7] -= tmp[5]*src[0] + tmp[8]*src[1] + tmp[11]*src[2];
-  tmp[0] = src[2]*src[7];  tmp[1] = src[3]*src[6];
-  tmp[2] = src[1]*src[7];  tmp[3] = src[3]*src[5];
-  tmp[4] = src[1]*src[6];  tmp[5] = src[2]*src[5];
-  tmp[6] = src[0]*src[7];  tmp[7] = src[3]*src[4];
-  tmp[8] = src[0]*src[6];  tmp[9] = src[2]*src[4];
-  tmp[10] = src[0]*src[5]; tmp[11] = src[1]*src[4];
-  dst[8] = tmp[0]*src[13] + tmp[3]*src[14] + tmp[4]*src[15];
-  dst[8] -= tmp[1]*src[13] + tmp[2]*src[14] + tmp[5]*src[15];
-  dst[9] = tmp[1]*src[12] + tmp[6]*src[14] + tmp[9]*src[15];
-  dst[9] -= tmp[0]*src[12] + tmp[7]*src[14] + tmp[8]*src[15];
-  dst[10] = tmp[2]*src[12] + tmp[7]*src[13] + tmp[10]*src[15];
-  dst[10] -= tmp[3]*src[12] + tmp[6]*src[13] + tmp[11]*src[15];
-  dst[11] = tmp[5]*src[12] + tmp[8]*src[13] + tmp[11]*src[14];
-  dst[11] -= tmp[4]*src[12] + tmp[9]*src[13] + tmp[10]*src[14];
-  dst[12] = tmp[2]*src[10] + tmp[5]*src[11] + tmp[1]*src[9];
-  dst[12] -= tmp[4]*src[11] + tmp[0]*src[9] + tmp[3]*src[10];
-  dst[13] = tmp[8]*src[11] + tmp[0]*src[8] + tmp[7]*src[10];
-  dst[13] -= tmp[6]*src[10] + tmp[9]*src[11] + tmp[1]*src[8];
-  dst[14] = tmp[6]*src[9] + tmp[11]*src[11] + tmp[3]*src[8];
-  dst[14] -= tmp[10]*src[11] + tmp[2]*src[8] + tmp[7]*src[9];
-  dst[15] = tmp[10]*src[10] + tmp[4]*src[8] + tmp[9]*src[9];
-  dst[15] -= tmp[8]*src[9] + tmp[11]*src[10] + tmp[5]*src[8];
-  det = src[0]*dst[0] + src[1]*dst[1] + src[2]*dst[2] + src[3]*dst[3];
-  det = 1.0 / det;
+  for (int i = 0; i < 3; i++) b[i] = c * q[i] + g * dq[i];
+  const R ga = g / a;
+  const R qb = q[0] * b[0] + q[1] * b[1] + q[2] * b[2];
+  const R bb = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+  const R qV = q[0] * V[1] + q[1] * V[2] + q[2] * V[3];
+  const R bV = b[0] * V[1] + b[1] * V[2] + b[2] * V[3];
+  const R x0 = (V[0] - bV + ga * qV * qb) / (a - bb + ga * qb * qb);
+  R y[3];
 #pragma unroll
-  for (int i = 0; i < 16; i++) mat[i] = dst[i] * det;
+  for (int i = 0; i < 3; i++) y[i] = V[i + 1] - b[i] * x0;
+  const R qy = ga * (q[0] * y[0] + q[1] * y[1] + q[2] * y[2]);
+  x[0] = x0;
+#pragma unroll
+  for (int i = 0; i < 3; i++) x[i + 1] = y[i] - qy * q[i];
 }
+
+template <typename R>
+__device__ __forceinline__ R rmax0(R x) { return x > R(0) ? x : R(0); }
 
 }  // namespace
 
+// Common set-up of one emergent ray: lane -> column, workgroup -> (column
+// block, entry t of the wavelength list io_list).
+#define RAY_SETUP()                                                                      \
+  const DevModel &M = *Mp;                                                               \
+  int cb, t;                                                                             \
+  xcd_tile(blockIdx.x, nlist, cb, t);                                                    \
+  const int ic = cb * 64 + threadIdx.x;                                                  \
+  if (ic >= ncol) return;                                                                \
+  const int io = io_list[t];                                                             \
+  const int N = ndep[ic];                                                                \
+  const int ls = out_ls[io];                                                             \
+  const int moving = moving_arr[ic];                                                     \
+  const int64_t fs = (int64_t) Nmax * ncol;                                              \
+  const double *st0 = state + ic;                                                        \
+  const double *zc = st0 + (int64_t) RHG_CF_HEIGHT * fs;                                 \
+  const double *Tc = st0 + (int64_t) RHG_CF_T * fs;                                      \
+  const R *cc = cchi + (int64_t) io * Nmax * ncol + ic;                                  \
+  const R *ce = ceta + (int64_t) io * Nmax * ncol + ic;                                  \
+  const int has_line = M.lam_line_ptr[ls + 1] > M.lam_line_ptr[ls];                      \
+  const double *lc = M.lamconst + (int64_t) ls * RHG_LC_NCONST;                          \
+  const R mu = (R) mu_d;                                                                 \
+  double B2[2];                                                                          \
+  B2[0] = planck(Tc[(int64_t) (N - 2) * ncol], lc[RHG_LC_HC_KLA], lc[RHG_LC_TWOHNU3_C2]); \
+  B2[1] = planck(Tc[(int64_t) (N - 1) * ncol], lc[RHG_LC_HC_KLA], lc[RHG_LC_TWOHNU3_C2]); \
+  double *out = stokes + (int64_t) ic * 4 * nout + io;
+
+#define Z(kk) zc[(int64_t) (kk) * ncol]
+#define POINT(kk, P) form_point<R>(M, ls, has_line, st0 + (int64_t) (kk) * ncol, fs, mu, moving, \
+                                   cc[(int64_t) (kk) * ncol], ce[(int64_t) (kk) * ncol], P)
+
+// Polarized cubic DELO-Bezier, rh/rhf1d/bezier_1D.c:52-300 (to_obs: depth
+// from N-1 to 0), for the wavelengths of io_list (solver STOKES).  The loop
+// forms one new point per iteration (index j, the only place the opacity is
+// evaluated); the first three iterations fill the window, each later one
+// advances the recursion at depth k = j + 2, and j = -1 takes the step at
+// k = 1 whose downwind derivative is one-sided.
+template <typename R>
 __global__ void __launch_bounds__(64)
-k_formal(DevModel M, const int32_t *__restrict__ ndep, const int32_t *__restrict__ moving_arr,
-         int ncol, int Nmax, const double *__restrict__ state, int nfield,
-         const int32_t *__restrict__ out_ls, int nout, const double *__restrict__ cchi,
-         const double *__restrict__ ceta, double mu, double *__restrict__ stokes) {
-  const int io = blockIdx.x * blockDim.x + threadIdx.x;
-  const int ic = blockIdx.y;
-  if (io >= nout) return;
-  const int N = ndep[ic];
-  const int ls = out_ls[io];
-  const int moving = moving_arr[ic];
-  const double *st = state + (int64_t) ic * nfield * Nmax;
-  const double *z = st + (int64_t) RHG_CF_HEIGHT * Nmax;
-  const double *T = st + (int64_t) RHG_CF_T * Nmax;
-  const double *cc = cchi + (int64_t) ic * Nmax * nout + io;
-  const double *ce = ceta + (int64_t) ic * Nmax * nout + io;
-  const int has_line = M.lam_line_ptr[ls + 1] > M.lam_line_ptr[ls];
+k_formal_stokes(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep,
+                const int32_t *__restrict__ moving_arr, int ncol, int Nmax,
+                const double *__restrict__ state, const int32_t *__restrict__ out_ls,
+                const int32_t *__restrict__ io_list, int nlist, int nout,
+                const R *__restrict__ cchi, const R *__restrict__ ceta, double mu_d,
+                double *__restrict__ stokes) {
+  RAY_SETUP();
+  const double imu = 1.0 / mu_d;
+  Point<R> Pu, P0, Pd, Pn;
+  R dSu[4], dqu[3], Iv[4];
+  R dtau_uw = R(0), dsup = R(0), dchi_up = R(0), dchi_c = R(0);
+  for (int j = N - 1; j >= -1; j--) {
+    if (j >= 0) POINT(j, Pn);
+    if (j >= N - 3) {
+      if (j == N - 1) {
+        Pu = Pn;
+      } else if (j == N - 2) {
+        P0 = Pn;
+      } else {
+        Pd = Pn;
+        const int k = N - 2;
+        const double dtau0 = 0.5 * imu * (double) (Pu.chi + P0.chi) * fabs(Z(N - 1) - Z(N - 2));
+        Iv[0] = (R) (B2[1] - (B2[0] - B2[1]) / dtau0);
+        Iv[1] = Iv[2] = Iv[3] = R(0);
+        dsup = (R) (fabs(Z(k) - Z(k + 1)) * imu);
+        const R dsdn = (R) (fabs(Z(k - 1) - Z(k)) * imu);
+        dchi_up = (P0.chi - Pu.chi) / dsup;
+        dchi_c = cent_deriv<R>(dsup, dsdn, Pu.chi, P0.chi, Pd.chi);
+        const R c2 = rmax0(P0.chi - (dsup * R(1.0 / 3.0)) * dchi_c);
+        const R c1 = rmax0(Pu.chi + (dsup * R(1.0 / 3.0)) * dchi_up);
+        dtau_uw = R(0.25) * dsup * (P0.chi + Pu.chi + c1 + c2);
+        const R idt = R(1) / dtau_uw;
+#pragma unroll
+        for (int n = 0; n < 4; n++) dSu[n] = (P0.S[n] - Pu.S[n]) * idt;
+#pragma unroll
+        for (int n = 0; n < 3; n++) dqu[n] = (P0.q[n] - Pu.q[n]) * idt;
+      }
+      continue;
+    }
+    const int k = j + 2;
+    const R dsdn = (R) (fabs(Z(k - 1) - Z(k)) * imu);
+    R dchi_dn;
+    if (j >= 0) {
+      const R dsdn2 = (R) (fabs(Z(k - 2) - Z(k - 1)) * imu);
+      dchi_dn = cent_deriv<R>(dsdn, dsdn2, P0.chi, Pd.chi, Pn.chi);
+    } else {
+      dchi_dn = (Pd.chi - P0.chi) / dsdn;
+    }
+    const R c2 = rmax0(P0.chi + (dsdn * R(1.0 / 3.0)) * dchi_c);
+    const R c1 = rmax0(Pd.chi - (dsdn * R(1.0 / 3.0)) * dchi_dn);
+    const R dtau_dw = R(0.25) * dsdn * (P0.chi + Pd.chi + c1 + c2);
+    double al_d, be_d, ga_d, th_d, ep_d;
+    bezier3_coeffs((double) dtau_uw, al_d, be_d, ga_d, th_d, ep_d);
+    const R dt03 = dtau_uw * R(1.0 / 3.0);
+    const R alpha = (R) al_d, beta = (R) be_d, gamma = (R) ga_d, theta = (R) th_d,
+            eps = (R) ep_d;
+    R dS0[4], dq0[3];
+    const CentDeriv<R> cd(dtau_uw, dtau_dw);
+#pragma unroll
+    for (int i = 0; i < 4; i++) dS0[i] = cd(Pu.S[i], P0.S[i], Pd.S[i]);
+#pragma unroll
+    for (int i = 0; i < 3; i++) dq0[i] = cd(Pu.q[i], P0.q[i], Pd.q[i]);
+    // V0 = Ma Iv + Mb Su + Mc S0 + dt03 (gamma dS0 - theta dSu) with
+    // Ma = eps I - beta Ku + theta (dt03 (Ku^2 + dKu + Ku) - Ku),
+    // Mb = beta I + theta (I - dt03 Ku), Mc = alpha I + gamma (I + dt03 K0)
+    R KI[4], KKI[4], dKI[4], KSu[4], KS0[4], V0[4];
+    kmul(Pu.q, Iv, KI);
+    kmul(Pu.q, KI, KKI);
+    kmul(dqu, Iv, dKI);
+    kmul(Pu.q, Pu.S, KSu);
+    kmul(P0.q, P0.S, KS0);
+    const R ca = theta * (dt03 - R(1)) - beta, gu = theta * dt03;
+    const R cb0 = beta + theta, cc0 = alpha + gamma, g0 = gamma * dt03;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+      V0[i] = eps * Iv[i] + ca * KI[i] + gu * (KKI[i] + dKI[i]) + cb0 * Pu.S[i] - gu * KSu[i] +
+              cc0 * P0.S[i] + g0 * KS0[i] + dt03 * (gamma * dS0[i] - theta * dSu[i]);
+    // Md = I + alpha K0 + gamma (dt03 (K0^2 + dK0 + K0) + K0)
+    solve_md<R>(alpha + gamma * (R(1) + dt03), g0, P0.q, dq0, V0, Iv);
+    Pu = P0;
+    P0 = Pd;
+    if (j >= 0) Pd = Pn;
+#pragma unroll
+    for (int n = 0; n < 4; n++) dSu[n] = dS0[n];
+#pragma unroll
+    for (int n = 0; n < 3; n++) dqu[n] = dq0[n];
+    dtau_uw = dtau_dw;
+    dsup = dsdn;
+    dchi_up = dchi_c;
+    dchi_c = dchi_dn;
+  }
+  // last interval: linear, rh/rhf1d/bezier_1D.c:264-299 (P0 = depth 0)
+  double w[3];
+  const double dtl = 0.5 * imu * (double) (P0.chi + Pu.chi) * fabs(Z(0) - Z(1));
+  w3(dtl, w);
+  const R w0 = (R) w[0], w1 = (R) w[1], w1t = (R) (w[1] / dtl);
+  // V0 = w0 S0 - w1 dSu + ((1 - w0) I - w1/dtau Ku) Iv;  Md = I + (w0 - w1/dtau) K0
+  R KI[4], V0[4], x[4];
+  kmul(Pu.q, Iv, KI);
+#pragma unroll
+  for (int n = 0; n < 4; n++) V0[n] = w0 * P0.S[n] - w1 * dSu[n] + (R(1) - w0) * Iv[n] - w1t * KI[n];
+  const R e = w0 - w1t;
+  const R q0V = P0.q[0] * V0[1] + P0.q[1] * V0[2] + P0.q[2] * V0[3];
+  const R qq = P0.q[0] * P0.q[0] + P0.q[1] * P0.q[1] + P0.q[2] * P0.q[2];
+  x[0] = (V0[0] - e * q0V) / (R(1) - e * e * qq);
+#pragma unroll
+  for (int n = 0; n < 3; n++) x[n + 1] = V0[n + 1] - e * P0.q[n] * x[0];
+#pragma unroll
+  for (int i = 0; i < 4; i++) out[(int64_t) i * nout] = (double) x[i];
+}
+
+// Unpolarized rays (solver BEZIER or FEAUTRIER) for the wavelengths of
+// io_list: the scalar cubic Bezier of rh/rhf1d/bezier_1D.c:306-541 for a
+// moving column, else Feautrier, rh/rhf1d/feautrier.c:56-202 (STANDARD order,
+// zero incident intensity at the top, thermalised bottom, FP64).  A BEZIER
+// wavelength is solved with Feautrier in a static column, as
+// rh/rhf1d/formal.c:228-241 decides.  Q = U = V = 0.
+template <typename R>
+__global__ void __launch_bounds__(64)
+k_formal_scalar(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep,
+                const int32_t *__restrict__ moving_arr, int ncol, int Nmax,
+                const double *__restrict__ state, const int32_t *__restrict__ out_ls,
+                const int32_t *__restrict__ io_list, int nlist, int nout,
+                const R *__restrict__ cchi, const R *__restrict__ ceta, double mu_d,
+                double *__restrict__ stokes) {
+  RAY_SETUP();
   int solver = M.solver[ls];
   if (solver == RHG_SOLVER_BEZIER && !moving) solver = RHG_SOLVER_FEAUTRIER;
-  const double *lc = M.lamconst + (int64_t) ls * RHG_LC_NCONST;
-  double B2[2];
-  B2[0] = planck(T[N - 2], lc[RHG_LC_HC_KLA], lc[RHG_LC_TWOHNU3_C2]);
-  B2[1] = planck(T[N - 1], lc[RHG_LC_HC_KLA], lc[RHG_LC_TWOHNU3_C2]);
-  double *out = stokes + (int64_t) ic * 4 * nout;
-
-#define POINT(kk, P) form_point(M, ls, has_line, st + (kk), Nmax, mu, moving, \
-                                cc[(int64_t) (kk) * nout], ce[(int64_t) (kk) * nout], P)
-
-  if (solver == RHG_SOLVER_STOKES) {
-    // cubic DELO-Bezier, rh/rhf1d/bezier_1D.c:52-300 (to_obs: k from N-1 to 0)
-    const double imu = 1.0 / mu;
-    const double ident[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
-    Point Pu, P0, Pd, Pdd;
-    double Ku[4][4], K0[4][4], Kd[4][4], dKu[4][4], dK0[4][4];
-    double dSu[4], dS0[4], Iv[4], A[4][4], Ma[4][4], Mb[4][4], Mc[4][4], Md[16], V0[4];
-    POINT(N - 1, Pu);
-    POINT(N - 2, P0);
-    POINT(N - 3, Pd);
-    double dtau_uw = 0.5 * imu * (Pu.chi + P0.chi) * fabs(z[N - 1] - z[N - 2]);
-    Iv[0] = B2[1] - (B2[0] - B2[1]) / dtau_uw;
-    Iv[1] = Iv[2] = Iv[3] = 0.0;
-    int k = N - 2;
-    double dsup = fabs(z[k] - z[k + 1]) * imu;
-    double dsdn = fabs(z[k - 1] - z[k]) * imu;
-    double dchi_up = (P0.chi - Pu.chi) / dsup;
-    double dchi_c = cent_deriv(dsup, dsdn, Pu.chi, P0.chi, Pd.chi);
-    double c2 = fmax(P0.chi - (dsup / 3.0) * dchi_c, 0.0);
-    double c1 = fmax(Pu.chi + (dsup / 3.0) * dchi_up, 0.0);
-    dtau_uw = 0.25 * dsup * (P0.chi + Pu.chi + c1 + c2);
-    make_K(Pu, Ku);
-    make_K(P0, K0);
-    for (int n = 0; n < 4; n++) {
-      dSu[n] = (P0.S[n] - Pu.S[n]) / dtau_uw;
-      for (int m = 0; m < 4; m++) dKu[n][m] = (K0[n][m] - Ku[n][m]) / dtau_uw;
-    }
-    for (k = N - 2; k != 0; k--) {
-      double dchi_dn;
-      dsdn = fabs(z[k - 1] - z[k]) * imu;
-      if (k > 1) {
-        POINT(k - 2, Pdd);
-        double dsdn2 = fabs(z[k - 2] - z[k - 1]) * imu;
-        dchi_dn = cent_deriv(dsdn, dsdn2, P0.chi, Pd.chi, Pdd.chi);
+  Point<R> Pn;
+  if (solver == RHG_SOLVER_BEZIER) {
+    const double zmu = 1.0 / mu_d;
+    Point<R> Pu, P0, Pd;
+    R I_upw = R(0), dtau_uw = R(0), dsup = R(0), dchi_up = R(0), dchi_c = R(0), dS_up = R(0);
+    for (int j = N - 1; j >= -1; j--) {
+      if (j >= 0) POINT(j, Pn);
+      if (j >= N - 3) {
+        if (j == N - 1) {
+          Pu = Pn;
+        } else if (j == N - 2) {
+          P0 = Pn;
+        } else {
+          Pd = Pn;
+          const int k = N - 2;
+          const double dtau0 = 0.5 * zmu * (double) (Pu.chi + P0.chi) * fabs(Z(N - 1) - Z(N - 2));
+          I_upw = (R) (B2[1] - (B2[0] - B2[1]) / dtau0);
+          dsup = (R) (fabs(Z(k) - Z(k + 1)) * zmu);
+          const R dsdn = (R) (fabs(Z(k - 1) - Z(k)) * zmu);
+          dchi_up = (P0.chi - Pu.chi) / dsup;
+          dchi_c = cent_deriv<R>(dsup, dsdn, Pu.chi, P0.chi, Pd.chi);
+          const R c1 = rmax0(P0.chi - (dsup * R(1.0 / 3.0)) * dchi_c);
+          const R c2 = rmax0(Pu.chi + (dsup * R(1.0 / 3.0)) * dchi_up);
+          dtau_uw = dsup * (P0.chi + Pu.chi + c1 + c2) * R(0.25);
+          dS_up = (P0.S[0] - Pu.S[0]) / dtau_uw;
+        }
+        continue;
+      }
+      const int k = j + 2;
+      const R dsdn = (R) (fabs(Z(k - 1) - Z(k)) * zmu);
+      R dchi_dn;
+      if (j >= 0) {
+        const R dsdn2 = (R) (fabs(Z(k - 2) - Z(k - 1)) * zmu);
+        dchi_dn = cent_deriv<R>(dsdn, dsdn2, P0.chi, Pd.chi, Pn.chi);
       } else {
         dchi_dn = (Pd.chi - P0.chi) / dsdn;
       }
-      c2 = fmax(P0.chi + (dsdn / 3.0) * dchi_c, 0.0);
-      c1 = fmax(Pd.chi - (dsdn / 3.0) * dchi_dn, 0.0);
-      double dtau_dw = 0.25 * dsdn * (P0.chi + Pd.chi + c1 + c2);
-      double dt = dtau_uw, dt03 = dt / 3.0, alpha, beta, gamma, theta, eps;
-      bezier3_coeffs(dt, alpha, beta, gamma, theta, eps);
-      make_K(Pd, Kd);
-#pragma unroll
-      for (int j = 0; j < 4; j++)
-#pragma unroll
-        for (int i = 0; i < 4; i++) dK0[j][i] = cent_deriv(dtau_uw, dtau_dw, Ku[j][i], K0[j][i], Kd[j][i]);
-#pragma unroll
-      for (int i = 0; i < 4; i++) dS0[i] = cent_deriv(dtau_uw, dtau_dw, Pu.S[i], P0.S[i], Pd.S[i]);
-      // Ma = Ku.Ku, A = K0.K0 (rh/bezier_aux.c:94-103 summation order)
-#pragma unroll
-      for (int j = 0; j < 4; j++)
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-          double sa = 0.0, sb = 0.0;
-#pragma unroll
-          for (int kk = 0; kk < 4; kk++) {
-            sa += Ku[kk][i] * Ku[j][kk];
-            sb += K0[kk][i] * K0[j][kk];
-          }
-          Ma[j][i] = sa;
-          A[j][i] = sb;
-        }
-#pragma unroll
-      for (int j = 0; j < 4; j++)
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-          Md[j * 4 + i] = ident[j][i] + alpha * K0[j][i] - gamma *
-                          -(dt03 * (A[j][i] + dK0[j][i] + K0[j][i]) + K0[j][i]);
-          Ma[j][i] = eps * ident[j][i] - beta * Ku[j][i] + theta *
-                     (dt03 * (Ma[j][i] + dKu[j][i] + Ku[j][i]) - Ku[j][i]);
-          Mb[j][i] = beta * ident[j][i] + theta * (ident[j][i] - dt03 * Ku[j][i]);
-          Mc[j][i] = alpha * ident[j][i] + gamma * (ident[j][i] + dt03 * K0[j][i]);
-        }
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        double v = 0.0;
-#pragma unroll
-        for (int j = 0; j < 4; j++) v += Ma[i][j] * Iv[j] + Mb[i][j] * Pu.S[j] + Mc[i][j] * P0.S[j];
-        v += dt03 * (gamma * dS0[i] - theta * dSu[i]);
-        V0[i] = v;
-      }
-      matinv4(Md);
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        double v = 0.0;
-#pragma unroll
-        for (int kk = 0; kk < 4; kk++) v += Md[i * 4 + kk] * V0[kk];
-        Iv[i] = v;
-      }
-      Pu = P0; P0 = Pd; if (k > 1) Pd = Pdd;
-#pragma unroll
-      for (int n = 0; n < 4; n++) dSu[n] = dS0[n];
-#pragma unroll
-      for (int j = 0; j < 4; j++)
-#pragma unroll
-        for (int i = 0; i < 4; i++) { Ku[j][i] = K0[j][i]; K0[j][i] = Kd[j][i]; dKu[j][i] = dK0[j][i]; }
-      dtau_uw = dtau_dw;
-      dsup = dsdn;
-      dchi_up = dchi_c;
-      dchi_c = dchi_dn;
-    }
-    // last interval: linear, rh/rhf1d/bezier_1D.c:264-299 (P0 = depth 0)
-    {
-      double w[3];
-      dtau_uw = 0.5 * imu * (P0.chi + Pu.chi) * fabs(z[0] - z[1]);
-      w3(dtau_uw, w);
-      for (int n = 0; n < 4; n++) V0[n] = w[0] * P0.S[n] + w[1] * -dSu[n];
-      for (int n = 0; n < 4; n++) {
-        for (int m = 0; m < 4; m++) {
-          A[n][m] = -w[1] / dtau_uw * Ku[n][m];
-          Md[n * 4 + m] = (w[0] - w[1] / dtau_uw) * K0[n][m];
-        }
-        A[n][n] = 1.0 - w[0];
-        Md[n * 4 + n] = 1.0;
-      }
-      for (int n = 0; n < 4; n++)
-        for (int m = 0; m < 4; m++) V0[n] += A[n][m] * Iv[m];
-      matinv4(Md);
-      for (int i = 0; i < 4; i++) {
-        double v = 0.0;
-        for (int kk = 0; kk < 4; kk++) v += Md[i * 4 + kk] * V0[kk];
-        out[i * nout + io] = v;
-      }
-    }
-  } else if (solver == RHG_SOLVER_BEZIER) {
-    // unpolarized cubic Bezier, rh/rhf1d/bezier_1D.c:306-541
-    const double zmu = 1.0 / mu;
-    Point Pu, P0, Pd, Pdd;
-    POINT(N - 1, Pu);
-    POINT(N - 2, P0);
-    POINT(N - 3, Pd);
-    double dtau_uw = 0.5 * zmu * (Pu.chi + P0.chi) * fabs(z[N - 1] - z[N - 2]);
-    double I_upw = B2[1] - (B2[0] - B2[1]) / dtau_uw;
-    int k = N - 2;
-    double dsup = fabs(z[k] - z[k + 1]) * zmu;
-    double dsdn = fabs(z[k - 1] - z[k]) * zmu;
-    double dchi_up = (P0.chi - Pu.chi) / dsup;
-    double dchi_c = cent_deriv(dsup, dsdn, Pu.chi, P0.chi, Pd.chi);
-    double c1 = fmax(P0.chi - (dsup / 3.0) * dchi_c, 0.0);
-    double c2 = fmax(Pu.chi + (dsup / 3.0) * dchi_up, 0.0);
-    dtau_uw = dsup * (P0.chi + Pu.chi + c1 + c2) * 0.25;
-    double dS_up = (P0.S[0] - Pu.S[0]) / dtau_uw;
-    double I = I_upw;
-    for (k = N - 2; k != 0; k--) {
-      double dchi_dn;
-      dsdn = fabs(z[k - 1] - z[k]) * zmu;
-      if (k > 1) {
-        POINT(k - 2, Pdd);
-        double dsdn2 = fabs(z[k - 2] - z[k - 1]) * zmu;
-        dchi_dn = cent_deriv(dsdn, dsdn2, P0.chi, Pd.chi, Pdd.chi);
-      } else dchi_dn = (Pd.chi - P0.chi) / dsdn;
-      c1 = fmax(P0.chi + (dsdn / 3.0) * dchi_c, 0.0);
-      c2 = fmax(Pd.chi - (dsdn / 3.0) * dchi_dn, 0.0);
-      double dtau_dw = dsdn * (P0.chi + Pd.chi + c1 + c2) * 0.25;
-      double dt03 = dtau_uw / 3.0, alpha, beta, gamma, theta, eps;
-      bezier3_coeffs(dtau_uw, alpha, beta, gamma, theta, eps);
-      double dS_c = cent_deriv(dtau_uw, dtau_dw, Pu.S[0], P0.S[0], Pd.S[0]);
-      c1 = fmax(P0.S[0] - dt03 * dS_c, 0.0);
-      c2 = fmax(Pu.S[0] + dt03 * dS_up, 0.0);
-      I = I_upw * eps + alpha * P0.S[0] + beta * Pu.S[0] + gamma * c1 + theta * c2;
-      I_upw = I;
-      Pu = P0; P0 = Pd; if (k > 1) Pd = Pdd;
+      R c1 = rmax0(P0.chi + (dsdn * R(1.0 / 3.0)) * dchi_c);
+      R c2 = rmax0(Pd.chi - (dsdn * R(1.0 / 3.0)) * dchi_dn);
+      const R dtau_dw = dsdn * (P0.chi + Pd.chi + c1 + c2) * R(0.25);
+      double al_d, be_d, ga_d, th_d, ep_d;
+      bezier3_coeffs((double) dtau_uw, al_d, be_d, ga_d, th_d, ep_d);
+      const R dt03 = dtau_uw * R(1.0 / 3.0);
+      const R dS_c = cent_deriv<R>(dtau_uw, dtau_dw, Pu.S[0], P0.S[0], Pd.S[0]);
+      c1 = rmax0(P0.S[0] - dt03 * dS_c);
+      c2 = rmax0(Pu.S[0] + dt03 * dS_up);
+      I_upw = I_upw * (R) ep_d + (R) al_d * P0.S[0] + (R) be_d * Pu.S[0] + (R) ga_d * c1 +
+              (R) th_d * c2;
+      Pu = P0;
+      P0 = Pd;
+      if (j >= 0) Pd = Pn;
       dsup = dsdn;
       dchi_up = dchi_c;
       dchi_c = dchi_dn;
       dtau_uw = dtau_dw;
       dS_up = dS_c;
     }
-    {
-      double w[3];
-      dtau_uw = 0.5 * zmu * (P0.chi + Pu.chi) * fabs(z[0] - z[1]);
-      double dS_uw = -(P0.S[0] - Pu.S[0]) / dtau_uw;
-      w3(dtau_uw, w);
-      I = (1.0 - w[0]) * I_upw + w[0] * P0.S[0] + w[1] * dS_uw;
-    }
-    out[io] = I;
-    out[nout + io] = 0.0;
-    out[2 * nout + io] = 0.0;
-    out[3 * nout + io] = 0.0;
+    double w[3];
+    const double dtl = 0.5 * zmu * (double) (P0.chi + Pu.chi) * fabs(Z(0) - Z(1));
+    const double dS_uw = -(double) (P0.S[0] - Pu.S[0]) / dtl;
+    w3(dtl, w);
+    out[0] = (1.0 - w[0]) * (double) I_upw + w[0] * (double) P0.S[0] + w[1] * dS_uw;
   } else {
-    // Feautrier, rh/rhf1d/feautrier.c:56-202 (STANDARD order, zero
-    // incident intensity at the top, thermalised bottom).  The back
-    // substitution for P[0] is accumulated in the forward sweep:
+    // The back substitution for P[0] is accumulated in the forward sweep:
     // P[0] = sum_k ztmp[k] prod_{j<k} 1/(1+F[j]) + P[N-1] prod_{j<N-1} 1/(1+F[j]).
-    const double zmu = 0.5 / mu;
-    Point Pk, Pn;
-    POINT(0, Pk);
-    POINT(1, Pn);
-    double dtau_prev = zmu * (Pk.chi + Pn.chi) * (z[0] - z[1]);
-    double abc0 = 1.0 + 2.0 * 1.0 / dtau_prev;
-    double C10 = 2.0 / (dtau_prev * dtau_prev);
-    double Stmp = Pk.S[0] + 2.0 * 0.0 / ((1.0 + 0.0) * dtau_prev);
-    double F = abc0 / C10;
-    double ztmp = Stmp / (abc0 + C10);
-    double P0acc = ztmp, prod = 1.0 / (1.0 + F);
-    for (int k = 1; k < N - 1; k++) {
-      Pk = Pn;
-      POINT(k + 1, Pn);
-      double dtau_k = zmu * (Pk.chi + Pn.chi) * (z[k] - z[k + 1]);
-      double dtau_mid = 0.5 * (dtau_k + dtau_prev);
-      double A1 = 1.0 / (dtau_mid * dtau_prev);
-      double C1 = 1.0 / (dtau_mid * dtau_k);
-      double Fn = (1.0 + A1 * F / (1.0 + F)) / C1;
-      ztmp = (Pk.S[0] + A1 * ztmp) / (C1 * (1.0 + Fn));
-      F = Fn;
-      P0acc += ztmp * prod;
-      prod *= 1.0 / (1.0 + F);
-      dtau_prev = dtau_k;
+    const double zmu = 0.5 / mu_d;
+    double chi_prev = 0.0, S_prev = 0.0, dtau_prev = 0.0, F = 0.0, ztmp = 0.0, P0acc = 0.0,
+           prod = 1.0;
+    for (int j = 0; j < N; j++) {
+      POINT(j, Pn);
+      const double chi_j = (double) Pn.chi, S_j = (double) Pn.S[0];
+      if (j == 1) {
+        dtau_prev = zmu * (chi_prev + chi_j) * (Z(0) - Z(1));
+        const double abc0 = 1.0 + 2.0 * 1.0 / dtau_prev;
+        const double C10 = 2.0 / (dtau_prev * dtau_prev);
+        const double Stmp = S_prev + 2.0 * 0.0 / ((1.0 + 0.0) * dtau_prev);
+        F = abc0 / C10;
+        ztmp = Stmp / (abc0 + C10);
+        P0acc = ztmp;
+        prod = 1.0 / (1.0 + F);
+      } else if (j >= 2) {
+        // row k = j - 1 with its downwind point j
+        const int k = j - 1;
+        const double dtau_k = zmu * (chi_prev + chi_j) * (Z(k) - Z(k + 1));
+        const double dtau_mid = 0.5 * (dtau_k + dtau_prev);
+        const double A1 = 1.0 / (dtau_mid * dtau_prev);
+        const double C1 = 1.0 / (dtau_mid * dtau_k);
+        const double Fn = (1.0 + A1 * F / (1.0 + F)) / C1;
+        ztmp = (S_prev + A1 * ztmp) / (C1 * (1.0 + Fn));
+        F = Fn;
+        P0acc += ztmp * prod;
+        prod *= 1.0 / (1.0 + F);
+        dtau_prev = dtau_k;
+      }
+      chi_prev = chi_j;
+      S_prev = S_j;
     }
-    double hN = B2[1] - (B2[0] - B2[1]) / dtau_prev;
-    double abcN = 1.0 + 2.0 * 1.0 / dtau_prev;
-    double A1N = 2.0 / (dtau_prev * dtau_prev);
-    double StmpN = Pn.S[0] + 2.0 * hN / ((1.0 + 0.0) * dtau_prev);
-    double PN = (StmpN + A1N * ztmp) / (abcN + A1N * (F / (1.0 + F)));
-    double P0 = P0acc + PN * prod;
-    out[io] = (1.0 + 1.0) * P0 - 0.0 / (1.0 + 0.0);
-    out[nout + io] = 0.0;
-    out[2 * nout + io] = 0.0;
-    out[3 * nout + io] = 0.0;
+    const double hN = B2[1] - (B2[0] - B2[1]) / dtau_prev;
+    const double abcN = 1.0 + 2.0 * 1.0 / dtau_prev;
+    const double A1N = 2.0 / (dtau_prev * dtau_prev);
+    const double StmpN = S_prev + 2.0 * hN / ((1.0 + 0.0) * dtau_prev);
+    const double PN = (StmpN + A1N * ztmp) / (abcN + A1N * (F / (1.0 + F)));
+    const double P0 = P0acc + PN * prod;
+    out[0] = (1.0 + 1.0) * P0 - 0.0 / (1.0 + 0.0);
   }
-#undef POINT
+  out[nout] = 0.0;
+  out[2 * (int64_t) nout] = 0.0;
+  out[3 * (int64_t) nout] = 0.0;
 }
+#undef POINT
+#undef Z
+#undef RAY_SETUP
 
-// Work census: one thread per (column, output wavelength), counts per ray in
-// census[(ic * nout + io) * RHG_NWORK + slot].
+#define RHG_KFORMAL(KNAME, R)                                                                 \
+  template __global__ void KNAME<R>(const DevModel *, const int32_t *, const int32_t *, int, int, \
+                                    const double *, const int32_t *, const int32_t *, int, int,  \
+                                    const R *, const R *, double, double *);
+RHG_KFORMAL(k_formal_stokes, double)
+RHG_KFORMAL(k_formal_stokes, float)
+RHG_KFORMAL(k_formal_scalar, double)
+RHG_KFORMAL(k_formal_scalar, float)
+#undef RHG_KFORMAL
+
+// Work census: one thread per (column, output wavelength), the mapping of
+// k_formal; counts per ray in census[(ic * nout + io) * RHG_NWORK + slot].
 __global__ void __launch_bounds__(64)
-k_census(DevModel M, const int32_t *__restrict__ ndep, const int32_t *__restrict__ moving_arr,
+k_census(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep, const int32_t *__restrict__ moving_arr,
          int ncol, int Nmax, const double *__restrict__ state, int nfield,
          const int32_t *__restrict__ out_ls, int nout, double mu, int32_t *__restrict__ census) {
-  const int io = blockIdx.x * blockDim.x + threadIdx.x;
-  const int ic = blockIdx.y;
-  if (io >= nout || ic >= ncol) return;
+  const DevModel &M = *Mp;
+  const int ic = blockIdx.x * blockDim.x + threadIdx.x;
+  const int io = blockIdx.y;
+  if (ic >= ncol || io >= nout) return;
   const int N = ndep[ic], ls = out_ls[io], moving = moving_arr[ic];
-  const double *st = state + (int64_t) ic * nfield * Nmax;
+  const int64_t fs = (int64_t) Nmax * ncol;
+  const double *st0 = state + ic;
   int32_t cnt[RHG_NWORK];
   for (int i = 0; i < RHG_NWORK; i++) cnt[i] = 0;
   int solver = M.solver[ls];
@@ -378,7 +402,7 @@ k_census(DevModel M, const int32_t *__restrict__ ndep, const int32_t *__restrict
   const int has_line = M.lam_line_ptr[ls + 1] > M.lam_line_ptr[ls];
   for (int k = 1; k < N; k++) {
     cnt[slot]++;
-    if (has_line) line_census(M, ls, st + k, Nmax, mu, moving, cnt);
+    if (has_line) line_census(M, ls, st0 + (int64_t) k * ncol, fs, mu, moving, cnt);
   }
   int32_t *o = census + ((int64_t) ic * nout + io) * RHG_NWORK;
   for (int i = 0; i < RHG_NWORK; i++) o[i] = cnt[i];

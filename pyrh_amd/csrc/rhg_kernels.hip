@@ -1,4 +1,7 @@
-// gfx950 kernels for batched LTE Stokes synthesis (FP64 path).
+// gfx950 kernels for batched LTE Stokes synthesis: column state, geometry
+// and continuum.  Every kernel maps consecutive lanes to consecutive
+// columns, so loads and stores of the column state
+// (state[(field * Nmax + k) * ncol + column]) are coalesced.
 //
 // Stage 1  k_state      one thread per (column, depth): unit conversion,
 //                       B projection, Saha-Boltzmann populations of every
@@ -8,12 +11,11 @@
 //                       wavelength.  Writes the column state.
 // Stage 2  k_geometry   one thread per column: tau500 -> height
 //                       (convertScales, a serial scan of Ndep).
-// Stage 3  k_continuum  one thread per (column, depth, wavelength):
-//                       continuum opacity and emissivity.
-// Stage 4  k_formal     one thread per (column, wavelength): line opacity
-//                       for the emergent ray computed depth by depth while
-//                       the DELO-Bezier / Bezier / Feautrier recursion runs
-//                       in registers.
+// Stage 3  k_continuum  one thread per (column, depth), all output
+//                       wavelengths: continuum opacity and emissivity, the
+//                       bound-free sums register-tiled over BF_TILE
+//                       wavelengths.
+// Stage 4  k_formal     (rhg_formal.hip) one thread per (column, wavelength).
 #include "rhg_kernels.hpp"
 
 namespace rhg {
@@ -32,6 +34,14 @@ __device__ __forceinline__ double level_ratio(const DevModel &M, int l0, int i, 
   double r = gi0 * exp(-dE_kT);
   for (int m = 1; m <= dZ; m++) r /= cNe_T;
   return r;
+}
+
+// 1 + sum of the level ratios of an atom (the Saha-Boltzmann normalisation)
+__device__ __forceinline__ double atom_sum(const DevModel &M, int l0, int nl, double T,
+                                           double cNe_T) {
+  double sum = 1.0;
+  for (int i = 1; i < nl; i++) sum += level_ratio(M, l0, i, T, cNe_T);
+  return sum;
 }
 
 __device__ double equilconstant(const DevModel &M, int im, double T) {
@@ -73,126 +83,166 @@ __device__ double equilconstant(const DevModel &M, int im, double T) {
   return eqc * cgs_to_SI;
 }
 
-// Solve the 4x4 (Nn x Nn) system by Gaussian elimination with partial
-// pivoting; a[Nn][NNUC_MAX], b[Nn] overwritten with the solution.
-__device__ void small_solve(int n, double a[NNUC_MAX][NNUC_MAX], double *b) {
-  for (int j = 0; j < n; j++) {
+// x^n for a small integer n (the stoichiometric counts and charges of the
+// molecules), by multiplication
+__device__ __forceinline__ double ipow(double x, int n) {
+  if (n == 0) return 1.0;
+  double r = x;
+  for (int i = 1; i < (n < 0 ? -n : n); i++) r *= x;
+  return n < 0 ? 1.0 / r : r;
+}
+
+// Solve the NN x NN system a x = b by Gaussian elimination with partial
+// pivoting; b is overwritten with the solution.  NN is a compile-time size,
+// so the arrays live in registers.
+template <int NN>
+__device__ __forceinline__ void small_solve(double (&a)[NN][NN], double (&b)[NN]) {
+#pragma unroll
+  for (int j = 0; j < NN; j++) {
     int p = j;
     double big = fabs(a[j][j]);
-    for (int i = j + 1; i < n; i++)
+#pragma unroll
+    for (int i = j + 1; i < NN; i++)
       if (fabs(a[i][j]) > big) { big = fabs(a[i][j]); p = i; }
-    if (p != j) {
-      for (int k = 0; k < n; k++) { double t = a[j][k]; a[j][k] = a[p][k]; a[p][k] = t; }
-      double t = b[j]; b[j] = b[p]; b[p] = t;
+    // swap rows j and p (p >= j), with compile-time indices
+#pragma unroll
+    for (int i = j + 1; i < NN; i++) {
+      if (i == p) {
+#pragma unroll
+        for (int k = 0; k < NN; k++) { double t = a[j][k]; a[j][k] = a[i][k]; a[i][k] = t; }
+        double t = b[j]; b[j] = b[i]; b[i] = t;
+      }
     }
     double inv = 1.0 / a[j][j];
-    for (int i = j + 1; i < n; i++) {
+#pragma unroll
+    for (int i = j + 1; i < NN; i++) {
       double f = a[i][j] * inv;
-      for (int k = j; k < n; k++) a[i][k] -= f * a[j][k];
+#pragma unroll
+      for (int k = j; k < NN; k++) a[i][k] -= f * a[j][k];
       b[i] -= f * b[j];
     }
   }
-  for (int i = n - 1; i >= 0; i--) {
+#pragma unroll
+  for (int i = NN - 1; i >= 0; i--) {
     double s = b[i];
-    for (int k = i + 1; k < n; k++) s -= a[i][k] * b[k];
+#pragma unroll
+    for (int k = i + 1; k < NN; k++) s -= a[i][k] * b[k];
     b[i] = s / a[i][i];
   }
 }
 
-// Chemical equilibrium at one depth (rh/chemequil.c:107-393).  The Newton
-// system [[D, B], [C, I]] (nuclei, molecules) is reduced to the nuclei
-// block by the Schur complement S = D - B C, solved, and corrected once
-// with the residual of the full system, as the reference refines its LU
-// solution once (rh/ludcmp.c:55-84).
-__device__ void chem_eq(const DevModel &M, double T, double ne, double nHtot, const double *fn0,
-                        double *nnuc, double *nmol, double &PhiHmin_out) {
-  const int Nn = M.nnuclei, Nm = M.nmol;
+// Chemical equilibrium at one depth (rh/chemequil.c:107-393) for NN nuclei.
+// The Newton system [[D, B], [C, I]] (nuclei, molecules) is reduced to the
+// nuclei block by the Schur complement S = D - B C, solved, and corrected
+// once with the residual of the reduced system, as the reference refines
+// its LU solution once (rh/ludcmp.c:55-84).  Molecule stoichiometry is the
+// dense table M.mol_cnt[m][nucleus]; the per-molecule equilibrium constant,
+// density and Saha product live in LDS (lds_phi, lds_nmol, lds_saha, stride
+// blockDim.x), everything per nucleus in registers.
+template <int NN>
+__device__ void chem_eq(const DevModel &M, double T, double ne, double nHtot,
+                        const double (&fn0)[NN], double (&nnuc)[NN], double *lds_phi,
+                        double *lds_nmol, double *lds_saha, int ls_stride, double &PhiHmin_out) {
+  const int Nm = M.nmol;
   const double CI = (HPLANCK / (2.0 * PI * M_ELECTRON)) * (HPLANCK / KBOLTZMANN);
-  double a[NNUC_MAX], Phi[NMOL_MAX], prev_n[NNUC_MAX + NMOL_MAX];
-  for (int i = 0; i < Nn; i++) a[i] = M.nucl_abund[i] * nHtot;
+  double a[NN];
+#pragma unroll
+  for (int i = 0; i < NN; i++) a[i] = M.nucl_abund[i] * nHtot;
   double PhiHmin = 0.25 * pow(CI / T, 1.5) * exp(0.754 * EV / (KBOLTZMANN * T));
   double fHmin = ne * fn0[0] * PhiHmin;
-  for (int i = 0; i < Nm; i++) Phi[i] = equilconstant(M, i, T);
-  for (int i = 0; i < Nn; i++) nnuc[i] = a[i];
-  for (int i = 0; i < Nm; i++) nmol[i] = 0.0;
-  for (int i = 0; i < Nn; i++) prev_n[i] = nnuc[i];
-  for (int i = 0; i < Nm; i++) prev_n[Nn + i] = 0.0;
+  for (int m = 0; m < Nm; m++) {
+    lds_phi[m * ls_stride] = equilconstant(M, m, T);
+    lds_nmol[m * ls_stride] = 0.0;
+  }
+#pragma unroll
+  for (int i = 0; i < NN; i++) nnuc[i] = a[i];
 
   for (int niter = 1; niter <= N_MAX_CHEM_ITER; niter++) {
-    double f1[NNUC_MAX], f2[NMOL_MAX], saha[NMOL_MAX];
-    for (int i = 0; i < Nn; i++) f1[i] = nnuc[i] - a[i];
-    for (int i = 0; i < Nm; i++) f2[i] = nmol[i] - 0.0;
+    double f1[NN], S[NN][NN], r[NN], x1[NN];
+#pragma unroll
+    for (int i = 0; i < NN; i++) f1[i] = nnuc[i] - a[i];
     f1[0] += fHmin * nnuc[0];
-    for (int i = 0; i < Nm; i++) {
-      double s = Phi[i];
-      int nel = M.mol_nel[i];
-      for (int j = 0; j < nel; j++) {
-        int nu = M.mol_el_nucl[4 * i + j];
-        s *= pow(fn0[nu] * nnuc[nu], (double) M.mol_el_count[4 * i + j]);
-        f1[nu] += M.mol_el_count[4 * i + j] * nmol[i];
-      }
-      s /= pow(ne, (double) M.mol_charge[i]);
-      f2[i] -= s;
-      saha[i] = s;
+    double fn[NN];
+#pragma unroll
+    for (int i = 0; i < NN; i++) fn[i] = fn0[i] * nnuc[i];
+    for (int m = 0; m < Nm; m++) {
+      const double *cnt = M.mol_cnt + m * NNUC_MAX;
+      const double nm = lds_nmol[m * ls_stride];
+      double sv = lds_phi[m * ls_stride];
+#pragma unroll
+      for (int i = 0; i < NN; i++)
+        if (cnt[i] != 0.0) {
+          sv *= ipow(fn[i], (int) cnt[i]);
+          f1[i] += cnt[i] * nm;
+        }
+      sv /= ipow(ne, M.mol_charge[m]);
+      lds_saha[m * ls_stride] = sv;
     }
-    // S = D - B C, rhs = f1 - B f2  (C[m][nu] = -saha_m count / n_nu)
-    double S[NNUC_MAX][NNUC_MAX], S0[NNUC_MAX][NNUC_MAX], r[NNUC_MAX], x1[NNUC_MAX];
-    for (int i = 0; i < Nn; i++) {
-      for (int j = 0; j < Nn; j++) S[i][j] = (i == j) ? 1.0 : 0.0;
+    // S = D - B C, rhs = f1 - B f2  (C[m][nu] = -saha_m count_nu / n_nu,
+    // f2[m] = n_m - saha_m)
+#pragma unroll
+    for (int i = 0; i < NN; i++) {
+#pragma unroll
+      for (int j = 0; j < NN; j++) S[i][j] = (i == j) ? 1.0 : 0.0;
       r[i] = f1[i];
     }
     S[0][0] += fHmin;
     for (int m = 0; m < Nm; m++) {
-      int nel = M.mol_nel[m];
-      for (int j = 0; j < nel; j++) {
-        int nu = M.mol_el_nucl[4 * m + j];
-        double cnt = M.mol_el_count[4 * m + j];
-        r[nu] -= cnt * f2[m];
-        for (int jj = 0; jj < nel; jj++) {
-          int nu2 = M.mol_el_nucl[4 * m + jj];
-          double C = -saha[m] * (M.mol_el_count[4 * m + jj] / nnuc[nu2]);
-          S[nu][nu2] -= cnt * C;
+      const double *cnt = M.mol_cnt + m * NNUC_MAX;
+      const double sv = lds_saha[m * ls_stride];
+      const double f2 = lds_nmol[m * ls_stride] - sv;
+      double C[NN];
+#pragma unroll
+      for (int j = 0; j < NN; j++) C[j] = -sv * (cnt[j] / nnuc[j]);
+#pragma unroll
+      for (int i = 0; i < NN; i++)
+        if (cnt[i] != 0.0) {
+          r[i] -= cnt[i] * f2;
+#pragma unroll
+          for (int j = 0; j < NN; j++)
+            if (cnt[j] != 0.0) S[i][j] -= cnt[i] * C[j];
         }
-      }
     }
-    for (int i = 0; i < Nn; i++) {
+    double S0[NN][NN], S1[NN][NN];
+#pragma unroll
+    for (int i = 0; i < NN; i++) {
       x1[i] = r[i];
-      for (int j = 0; j < Nn; j++) S0[i][j] = S[i][j];
+#pragma unroll
+      for (int j = 0; j < NN; j++) { S0[i][j] = S[i][j]; S1[i][j] = S[i][j]; }
     }
-    small_solve(Nn, S, x1);
-    // one refinement step on the reduced system
+    small_solve<NN>(S, x1);
     {
-      double res[NNUC_MAX];
-      for (int i = 0; i < Nn; i++) {
+      double res[NN];
+#pragma unroll
+      for (int i = 0; i < NN; i++) {
         res[i] = r[i];
-        for (int j = 0; j < Nn; j++) res[i] -= S0[i][j] * x1[j];
+#pragma unroll
+        for (int j = 0; j < NN; j++) res[i] -= S0[i][j] * x1[j];
       }
-      double S1[NNUC_MAX][NNUC_MAX];
-      for (int i = 0; i < Nn; i++)
-        for (int j = 0; j < Nn; j++) S1[i][j] = S0[i][j];
-      small_solve(Nn, S1, res);
-      for (int i = 0; i < Nn; i++) x1[i] += res[i];
+      small_solve<NN>(S1, res);
+#pragma unroll
+      for (int i = 0; i < NN; i++) x1[i] += res[i];
     }
+    // back substitution for the molecules and MaxChange over the last two
+    // solutions (rh/maxchange.c:32-50)
     double dnmax = 0.0;
     for (int m = 0; m < Nm; m++) {
-      double x2 = f2[m];
-      int nel = M.mol_nel[m];
-      for (int j = 0; j < nel; j++) {
-        int nu = M.mol_el_nucl[4 * m + j];
-        double C = -saha[m] * (M.mol_el_count[4 * m + j] / nnuc[nu]);
-        x2 -= C * x1[nu];
-      }
-      nmol[m] -= x2;
+      const double *cnt = M.mol_cnt + m * NNUC_MAX;
+      const double sv = lds_saha[m * ls_stride];
+      const double nold = lds_nmol[m * ls_stride];
+      double x2 = nold - sv;
+#pragma unroll
+      for (int j = 0; j < NN; j++)
+        if (cnt[j] != 0.0) x2 -= (-sv * (cnt[j] / nnuc[j])) * x1[j];
+      const double nnew = nold - x2;
+      lds_nmol[m * ls_stride] = nnew;
+      if (nnew) dnmax = fmax(dnmax, fabs((nnew - nold) / nnew));
     }
-    for (int i = 0; i < Nn; i++) nnuc[i] -= x1[i];
-    // MaxChange over the last two solutions (rh/maxchange.c:32-50)
-    for (int i = 0; i < Nn; i++) {
-      if (nnuc[i]) dnmax = fmax(dnmax, fabs((nnuc[i] - prev_n[i]) / nnuc[i]));
-      prev_n[i] = nnuc[i];
-    }
-    for (int i = 0; i < Nm; i++) {
-      if (nmol[i]) dnmax = fmax(dnmax, fabs((nmol[i] - prev_n[Nn + i]) / nmol[i]));
-      prev_n[Nn + i] = nmol[i];
+#pragma unroll
+    for (int i = 0; i < NN; i++) {
+      const double nold = nnuc[i];
+      nnuc[i] -= x1[i];
+      if (nnuc[i]) dnmax = fmax(dnmax, fabs((nnuc[i] - nold) / nnuc[i]));
     }
     if (dnmax <= CHEM_ITER_LIMIT) break;
   }
@@ -278,10 +328,9 @@ __device__ __forceinline__ void continuum_point(const DevModel &M, int ls, const
       if (M.cont_atom[ci] != 0) continue;
       any = true;
       double sigma = M.lam_cont_alpha[e];
-      double ni = st[(RHG_CF_NFIXED + 3 * ci) * fs], r = st[(RHG_CF_NFIXED + 3 * ci + 1) * fs];
-      double gijk = r * expla;
+      double ni = st[cont_field(ci) * fs];
       chi += sigma * (1.0 - expla) * ni;
-      eta += lc[RHG_LC_BF_TWOHNU3_C2] * gijk * sigma * np;
+      eta += lc[RHG_LC_BF_TWOHNU3_C2] * expla * sigma * ni;
     }
     if (any) { chi_ai += chi; eta_ai += eta; }
   }
@@ -317,26 +366,26 @@ __device__ __forceinline__ void continuum_point(const DevModel &M, int ls, const
     int ci = M.lam_cont_idx[e];
     if (M.cont_atom[ci] == 0) continue;
     double alpha_la = M.lam_cont_alpha[e];
-    double ni = st[(RHG_CF_NFIXED + 3 * ci) * fs], r = st[(RHG_CF_NFIXED + 3 * ci + 1) * fs],
-           nj = st[(RHG_CF_NFIXED + 3 * ci + 2) * fs];
-    double gijk = r * expla;
+    double ni = st[cont_field(ci) * fs];
     chi += alpha_la * (1.0 - expla) * ni;
-    eta += lc[RHG_LC_BF_TWOHNU3_C2] * gijk * alpha_la * nj;
+    eta += lc[RHG_LC_BF_TWOHNU3_C2] * expla * alpha_la * ni;
   }
   chi_ai += chi * 1.0;
   eta_ai += eta * 1.0;
 }
 
-__global__ void k_state(DevModel M, const double *__restrict__ atmos, const int32_t *__restrict__ ndep,
-                        int ncol, int Nmax, double mu, int atm_scale, double *__restrict__ state,
-                        int nfield) {
+template <int NN>
+__global__ void __launch_bounds__(64)
+k_state(const DevModel *__restrict__ Mp, const double *__restrict__ atmos, const int32_t *__restrict__ ndep, int ncol,
+        int Nmax, double mu, int atm_scale, double *__restrict__ state, int nfield) {
+  const DevModel &M = *Mp;
   int64_t gid = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= (int64_t) ncol * Nmax) return;
-  const int ic = (int) (gid / Nmax), k = (int) (gid % Nmax);
+  const int k = (int) (gid / ncol), ic = (int) (gid % ncol);
   if (k >= ndep[ic]) return;
   const double *atm = atmos + (int64_t) ic * 9 * Nmax;
-  double *st = state + (int64_t) ic * nfield * Nmax + k;
-  const int64_t fs = Nmax;
+  double *st = state + (int64_t) k * ncol + ic;
+  const int64_t fs = (int64_t) Nmax * ncol;
 
   // units, rh/rhf1d/pyrh_compute1dray.c:230-270
   const double CUBE_CM = CM_TO_M * CM_TO_M * CM_TO_M;
@@ -374,55 +423,75 @@ __global__ void k_state(DevModel M, const double *__restrict__ atmos, const int3
   // Saha-Boltzmann normalisation per atom, rh/ltepops.c:33-113
   const double c1 = (HPLANCK / (2.0 * PI * M_ELECTRON)) * (HPLANCK / KBOLTZMANN);
   const double cNe_T = 0.5 * ne * pow(c1 / T, 1.5);
-  double nstar0[RHG_MAX_ATOM], fn0[NNUC_MAX], frac[RHG_MAX_ATOM];
-  for (int ia = 0; ia < M.natom; ia++) {
-    int l0 = M.atom_level_ptr[ia], nl = M.atom_level_ptr[ia + 1] - l0;
-    double sum = 1.0;
-    for (int i = 1; i < nl; i++) sum += level_ratio(M, l0, i, T, cNe_T);
-    nstar0[ia] = (M.atom_abund[ia] * nHtot) / sum;
-    frac[ia] = 1.0;
-  }
-  // neutral fractions of the nuclei (rh/chemequil.c:222-236)
-  for (int i = 0; i < M.nnuclei; i++) {
-    int ia = M.nucl_atom[i], l0 = M.atom_level_ptr[ia], nl = M.atom_level_ptr[ia + 1] - l0;
+  // the atoms that are molecule nuclei: total LTE normalisation and the
+  // neutral fraction (rh/chemequil.c:222-236)
+  double nstar0n[NN], fn0[NN];
+#pragma unroll
+  for (int i = 0; i < NN; i++) {
+    const int ia = M.nucl_atom[i], l0 = M.atom_level_ptr[ia], nl = M.atom_level_ptr[ia + 1] - l0;
+    nstar0n[i] = (M.atom_abund[ia] * nHtot) / atom_sum(M, l0, nl, T, cNe_T);
     double ntot = M.atom_abund[ia] * nHtot, f = 0.0;
     for (int j = 0; j < nl; j++) {
       if (M.level_stage[l0 + j] > 0) break;
-      f += (j == 0) ? nstar0[ia] : level_ratio(M, l0, j, T, cNe_T) * nstar0[ia];
+      f += (j == 0) ? nstar0n[i] : level_ratio(M, l0, j, T, cNe_T) * nstar0n[i];
     }
     fn0[i] = f / ntot;
   }
-  double nnuc[NNUC_MAX], nmol[NMOL_MAX], PhiHmin;
-  chem_eq(M, T, ne, nHtot, fn0, nnuc, nmol, PhiHmin);
-  for (int i = 0; i < M.nnuclei; i++) {
-    int ia = M.nucl_atom[i];
-    frac[ia] = nnuc[i] / (M.atom_abund[ia] * nHtot);
-  }
+  double nnuc[NN], PhiHmin;
+  extern __shared__ double lds_mol[];
+  const int lstride = blockDim.x;
+  double *lds_phi = lds_mol + threadIdx.x;
+  double *lds_nmol = lds_phi + M.nmol * lstride;
+  double *lds_saha = lds_nmol + M.nmol * lstride;
+  chem_eq<NN>(M, T, ne, nHtot, fn0, nnuc, lds_phi, lds_nmol, lds_saha, lstride, PhiHmin);
+  double fracn[NN];
+#pragma unroll
+  for (int i = 0; i < NN; i++) fracn[i] = nnuc[i] / (M.atom_abund[M.nucl_atom[i]] * nHtot);
   const double nHmin = ne * (nnuc[0] * PhiHmin);
   st[RHG_CF_NHMIN * fs] = nHmin;
-  st[RHG_CF_NH2 * fs] = M.mol_h2 >= 0 ? nmol[M.mol_h2] : 0.0;
-  st[RHG_CF_NOH * fs] = M.mol_oh >= 0 ? nmol[M.mol_oh] : 0.0;
-  st[RHG_CF_NCH * fs] = M.mol_ch >= 0 ? nmol[M.mol_ch] : 0.0;
+  st[RHG_CF_NH2 * fs] = M.mol_h2 >= 0 ? lds_nmol[M.mol_h2 * lstride] : 0.0;
+  st[RHG_CF_NOH * fs] = M.mol_oh >= 0 ? lds_nmol[M.mol_oh * lstride] : 0.0;
+  st[RHG_CF_NCH * fs] = M.mol_ch >= 0 ? lds_nmol[M.mol_ch * lstride] : 0.0;
+  // hydrogen is atom 0 and nucleus 0
   const int lH = M.atom_level_ptr[0], nlH = M.atom_level_ptr[1] - lH;
-  const double nH0 = nstar0[0] * frac[0];
-  const double np = (level_ratio(M, lH, nlH - 1, T, cNe_T) * nstar0[0]) * frac[0];
+  const double nH0 = nstar0n[0] * fracn[0];
+  const double np = (level_ratio(M, lH, nlH - 1, T, cNe_T) * nstar0n[0]) * fracn[0];
   st[RHG_CF_NH0 * fs] = nH0;
   st[RHG_CF_NP * fs] = np;
 
-  // bound-free continuum level terms: n_i, n_i*/n_j*, n_j
-  for (int ci = 0; ci < M.ncont; ci++) {
-    int ia = M.cont_atom[ci], l0 = M.atom_level_ptr[ia];
-    int i = M.cont_i[ci], j = M.cont_j[ci];
-    double ni = ((i == 0) ? nstar0[ia] : level_ratio(M, l0, i, T, cNe_T) * nstar0[ia]) * frac[ia];
-    double nj = ((j == 0) ? nstar0[ia] : level_ratio(M, l0, j, T, cNe_T) * nstar0[ia]) * frac[ia];
-    st[(RHG_CF_NFIXED + 3 * ci) * fs] = ni;
-    st[(RHG_CF_NFIXED + 3 * ci + 1) * fs] = ni / nj;
-    st[(RHG_CF_NFIXED + 3 * ci + 2) * fs] = nj;
+  // bound-free continua: lower-level population n_i.  In LTE n = n*, so the
+  // emissivity factor n_j n_i*/n_j* of rh/metal.c:120-140 (n_p n_i*/n_p* for
+  // hydrogen, rh/hydrogen.c:300-330, whose continuum level is the proton
+  // level) is n_i as well.  The continua come grouped by atom; each atom's
+  // normalisation is formed once.
+  {
+    int cur = -1;
+    double nst0 = 0.0, fr = 1.0;
+    for (int ci = 0; ci < M.ncont; ci++) {
+      const int ia = M.cont_atom[ci], l0 = M.atom_level_ptr[ia];
+      if (ia != cur) {
+        cur = ia;
+        int inuc = -1;
+#pragma unroll
+        for (int i = 0; i < NN; i++)
+          if (M.nucl_atom[i] == ia) inuc = i;
+        if (inuc >= 0) {
+#pragma unroll
+          for (int i = 0; i < NN; i++)
+            if (i == inuc) { nst0 = nstar0n[i]; fr = fracn[i]; }
+        } else {
+          nst0 = (M.atom_abund[ia] * nHtot) / atom_sum(M, l0, M.atom_level_ptr[ia + 1] - l0, T, cNe_T);
+          fr = 1.0;
+        }
+      }
+      const int i = M.cont_i[ci];
+      st[cont_field(ci) * fs] = ((i == 0) ? nst0 : level_ratio(M, l0, i, T, cNe_T) * nst0) * fr;
+    }
   }
 
   // Kurucz lines: lambda-independent per-depth quantities,
   // rh/kurucz.c:662-677 and rh/kurucz.c:743-775
-  const int lbase = RHG_CF_NFIXED + 3 * M.ncont;
+  const int lbase = line_field_base(M.ncont);
   for (int l = 0; l < M.nline; l++) {
     int ie = M.line_el[l], stg = M.line_stage[l];
     // stage populations of the element, rh/ltepops.c:116-164
@@ -463,7 +532,7 @@ __global__ void k_state(DevModel M, const double *__restrict__ atmos, const int3
     st[(lbase + 4 * l) * fs] = chi_l;
     st[(lbase + 4 * l + 1) * fs] = eta_l;
     st[(lbase + 4 * l + 2) * fs] = adamp;
-    st[(lbase + 4 * l + 3) * fs] = vbroad;
+    st[(lbase + 4 * l + 3) * fs] = 1.0 / vbroad;
   }
 
   // background continuum at the reference wavelength (for convertScales;
@@ -485,98 +554,254 @@ __global__ void k_state(DevModel M, const double *__restrict__ atmos, const int3
 
 // ---------------------------------------------------------------------------
 // Stage 2: convertScales, rh/rhf1d/multiatmos.c:100-177
-__global__ void k_geometry(DevModel M, const int32_t *__restrict__ ndep, int ncol, int Nmax,
+__global__ void k_geometry(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep, int ncol, int Nmax,
                            int atm_scale, double mu, double *__restrict__ state, int nfield,
                            int32_t *__restrict__ moving_out, double *__restrict__ chiref_out) {
-  int ic = blockIdx.x * blockDim.x + threadIdx.x;
+  const DevModel &M = *Mp;
+  const int ic = blockIdx.x * blockDim.x + threadIdx.x;
   if (ic >= ncol) return;
   const int N = ndep[ic];
-  double *st = state + (int64_t) ic * nfield * Nmax;
-  double *h = st + (int64_t) RHG_CF_HEIGHT * Nmax;      // holds chi_ref on entry
-  const double *scale = st + (int64_t) (nfield - 1) * Nmax;
-  const double *nH = st + (int64_t) RHG_CF_NHTOT * Nmax;
+  const int64_t fs = (int64_t) Nmax * ncol;
+  double *st0 = state + ic;
+#define H(k) st0[(int64_t) RHG_CF_HEIGHT * fs + (int64_t) (k) * ncol]   // chi_ref on entry
+#define SCALE(k) st0[(int64_t) (nfield - 1) * fs + (int64_t) (k) * ncol]
+#define NH(k) st0[(int64_t) RHG_CF_NHTOT * fs + (int64_t) (k) * ncol]
   const double rho_f = AMU * M.wght_per_H;
   // moving atmosphere flag, rh/rhf1d/pyrh_compute1dray.c:272-278
   int moving = 0;
   for (int k = 0; k < N; k++)
-    if (fabs(st[(int64_t) RHG_CF_VLOS * Nmax + k]) >= M.vmacro_tresh) { moving = 1; break; }
+    if (fabs(st0[(int64_t) RHG_CF_VLOS * fs + (int64_t) k * ncol]) >= M.vmacro_tresh) {
+      moving = 1;
+      break;
+    }
   moving_out[ic] = moving;
   if (M.lam_line_ptr[M.ref_index + 1] > M.lam_line_ptr[M.ref_index]) {
     for (int k = 0; k < N; k++) {
       double lchi, leta, q[3], e[3];
-      line_point(M, M.ref_index, st + k, Nmax, mu, lchi, q, leta, e, nullptr, moving);
-      h[k] += lchi;
+      line_point<double>(M, M.ref_index, st0 + (int64_t) k * ncol, fs, mu, lchi, q, leta, e, moving);
+      H(k) += lchi;
     }
   }
   if (chiref_out)
-    for (int k = 0; k < N; k++) chiref_out[(int64_t) ic * Nmax + k] = h[k];
+    for (int k = 0; k < N; k++) chiref_out[(int64_t) ic * Nmax + k] = H(k);
   if (atm_scale == 0) {
-    double chi_prev = h[0], tau_prev = scale[0], hp = 0.0;
-    h[0] = 0.0;
+    double chi_prev = H(0), tau_prev = SCALE(0), hp = 0.0;
+    H(0) = 0.0;
     for (int k = 1; k < N; k++) {
-      double chik = h[k];
-      double hk = hp - 2.0 * (scale[k] - tau_prev) / (chi_prev + chik);
-      h[k] = hk;
+      double chik = H(k);
+      double hk = hp - 2.0 * (SCALE(k) - tau_prev) / (chi_prev + chik);
+      H(k) = hk;
       hp = hk;
       chi_prev = chik;
-      tau_prev = scale[k];
+      tau_prev = SCALE(k);
     }
     // h_zero = Linear(tau, height, 1.0) (bisection, rh/linear.c)
     double hz;
-    if (1.0 <= scale[0]) hz = h[0];
-    else if (1.0 >= scale[N - 1]) hz = h[N - 1];
+    if (1.0 <= SCALE(0)) hz = H(0);
+    else if (1.0 >= SCALE(N - 1)) hz = H(N - 1);
     else {
-      int j = locate_asc(N, scale, 1.0);
-      double fx = (scale[j + 1] - 1.0) / (scale[j + 1] - scale[j]);
-      hz = fx * h[j] + (1 - fx) * h[j + 1];
+      int lo = 0, hi = N;
+      while (hi - lo > 1) {
+        int idx = (hi + lo) >> 1;
+        if (1.0 >= SCALE(idx)) lo = idx; else hi = idx;
+      }
+      const int j = lo;
+      double fx = (SCALE(j + 1) - 1.0) / (SCALE(j + 1) - SCALE(j));
+      hz = fx * H(j) + (1 - fx) * H(j + 1);
     }
-    for (int k = 0; k < N; k++) h[k] = h[k] - hz;
+    for (int k = 0; k < N; k++) H(k) = H(k) - hz;
   } else if (atm_scale == 1) {
-    double chi_prev = h[0], hp = 0.0;
-    double taup = chi_prev / (rho_f * nH[0]) * scale[0];
+    double chi_prev = H(0), hp = 0.0;
+    double taup = chi_prev / (rho_f * NH(0)) * SCALE(0);
     double tauv[RHG_MAX_DEPTH];
     tauv[0] = taup;
-    h[0] = 0.0;
+    H(0) = 0.0;
     for (int k = 1; k < N; k++) {
-      double chik = h[k];
-      double hk = hp - 2.0 * (scale[k] - scale[k - 1]) / (rho_f * nH[k - 1] + rho_f * nH[k]);
+      double chik = H(k);
+      double hk = hp - 2.0 * (SCALE(k) - SCALE(k - 1)) / (rho_f * NH(k - 1) + rho_f * NH(k));
       taup = taup + 0.5 * (chi_prev + chik) * (hp - hk);
       tauv[k] = taup;
-      h[k] = hk;
+      H(k) = hk;
       hp = hk;
       chi_prev = chik;
     }
     double hz;
-    if (1.0 <= tauv[0]) hz = h[0];
-    else if (1.0 >= tauv[N - 1]) hz = h[N - 1];
+    if (1.0 <= tauv[0]) hz = H(0);
+    else if (1.0 >= tauv[N - 1]) hz = H(N - 1);
     else {
       int j = locate_asc(N, tauv, 1.0);
       double fx = (tauv[j + 1] - 1.0) / (tauv[j + 1] - tauv[j]);
-      hz = fx * h[j] + (1 - fx) * h[j + 1];
+      hz = fx * H(j) + (1 - fx) * H(j + 1);
     }
-    for (int k = 0; k < N; k++) h[k] = h[k] - hz;
+    for (int k = 0; k < N; k++) H(k) = H(k) - hz;
   } else {
-    for (int k = 0; k < N; k++) h[k] = scale[k];
+    for (int k = 0; k < N; k++) H(k) = SCALE(k);
   }
+#undef H
+#undef SCALE
+#undef NH
 }
 
 // ---------------------------------------------------------------------------
-// Stage 3: continuum per (column, depth, output wavelength)
-__global__ void k_continuum(DevModel M, const int32_t *__restrict__ ndep, int ncol, int Nmax,
-                            const double *__restrict__ state, int nfield,
-                            const int32_t *__restrict__ out_ls, int nout,
-                            double *__restrict__ cchi, double *__restrict__ ceta) {
-  // block: x = wavelength tile, y = depth; grid.z = column
-  const int io = blockIdx.x * blockDim.x + threadIdx.x;
-  const int k = blockIdx.y;
-  const int ic = blockIdx.z;
-  if (io >= nout || k >= ndep[ic]) return;
-  const double *st = state + (int64_t) ic * nfield * Nmax + k;
-  double chi, eta;
-  continuum_point(M, out_ls[io], st, Nmax, chi, eta);
-  int64_t o = ((int64_t) ic * Nmax + k) * nout + io;
-  cchi[o] = chi;
-  ceta[o] = eta;
+// Stage 3: continuum opacity and emissivity for every output wavelength at
+// one (column, depth), the terms of continuum_point() with what depends on
+// T and ne alone evaluated once.  The bound-free sums
+// sum_c alpha_c(lambda) n_i,c are accumulated BF_TILE wavelengths at a time
+// in registers (one load of n_i,c per tile; the tile's cross-sections are
+// wave-uniform) and handed to the per-wavelength pass through LDS.  One
+// exponential per wavelength: exp(-hc/(k lambda T)) is the reciprocal of the
+// Planck function's exp(hc/(k lambda T)).
+// Output layout: cchi[(io * Nmax + k) * ncol + column].
+template <typename R>
+__global__ void __launch_bounds__(CONT_BLOCK)
+k_continuum(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep, int ncol, int Nmax,
+            const double *__restrict__ state, int nfield, const int32_t *__restrict__ out_ls,
+            int nout, R *__restrict__ cchi, R *__restrict__ ceta) {
+  const DevModel &M = *Mp;
+  __shared__ double bf_lds[BF_TILE][CONT_BLOCK];
+  const int64_t gid = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (int64_t) ncol * Nmax) return;
+  const int k = (int) (gid / ncol), ic = (int) (gid % ncol);
+  if (k >= ndep[ic]) return;
+  const int64_t fs = (int64_t) Nmax * ncol;
+  const double *st = state + (int64_t) k * ncol + ic;
+  const double T = st[RHG_CF_T * fs], ne = st[RHG_CF_NE * fs];
+  const double nHmin = st[RHG_CF_NHMIN * fs], nH0 = st[RHG_CF_NH0 * fs], np = st[RHG_CF_NP * fs];
+  const double nH2 = st[RHG_CF_NH2 * fs], nOH = st[RHG_CF_NOH * fs], nCH = st[RHG_CF_NCH * fs];
+  const double C0 = Q_ELECTRON * Q_ELECTRON / (4.0 * PI * EPSILON_0) / sqrt(M_ELECTRON);
+  const double sigma_ff = 4.0 / 3.0 * sqrt(2.0 * PI / (3.0 * KBOLTZMANN)) * (C0 * C0 * C0) /
+                          (HPLANCK * CLIGHT);
+  // T- and ne-only factors (rh/hydrogen.c:509-523 table indices)
+  const double iT = 1.0 / T;
+  const double pe = ne * KBOLTZMANN * T;
+  const double ti_hmin = clamp_index(NTHETA, M.hminff_theta, THETA0 * iT);
+  const double ti_h2m = clamp_index(NTHETA_H2, M.h2mff_theta, THETA0 * iT);
+  const double ti_h2p = clamp_index(NTEMP_H2P, M.h2pff_temp, T);
+  const double sqrt_theta = sqrt(THETA0 * iT), sff_T = sigma_ff / sqrt(T);
+  const double hff_fac = ne * np;
+  const double y_fac = T * (1.0 / (HPLANCK * CLIGHT));
+  const bool oh_in = !(T < M.oh_T[0] || T > M.oh_T[NTOH - 1]);
+  const bool ch_in = !(T < M.ch_T[0] || T > M.ch_T[NTCH - 1]);
+  double ti_oh = 0.0, ti_ch = 0.0;
+  if (oh_in) {
+    int i2 = locate_asc(NTOH, M.oh_T, T);
+    ti_oh = (i2 == NTOH - 1) ? (double) i2 : (double) i2 + (T - M.oh_T[i2]) / (M.oh_T[i2 + 1] - M.oh_T[i2]);
+  }
+  if (ch_in) {
+    int i2 = locate_asc(NTCH, M.ch_T, T);
+    ti_ch = (i2 == NTCH - 1) ? (double) i2 : (double) i2 + (T - M.ch_T[i2]) / (M.ch_T[i2 + 1] - M.ch_T[i2]);
+  }
+
+  for (int t = 0; t < M.ntile; t++) {
+    const int io0 = t * BF_TILE;
+    {
+      double bf[BF_TILE];
+#pragma unroll
+      for (int j = 0; j < BF_TILE; j++) bf[j] = 0.0;
+      for (int e = M.tile_ptr[t]; e < M.tile_ptr[t + 1]; e++) {
+        const double ni = st[cont_field(M.tile_cont[e]) * fs];
+        const double *al = M.tile_alpha + (int64_t) e * BF_TILE;
+#pragma unroll
+        for (int j = 0; j < BF_TILE; j++) bf[j] += al[j] * ni;
+      }
+#pragma unroll
+      for (int j = 0; j < BF_TILE; j++) bf_lds[j][threadIdx.x] = bf[j];
+    }
+    const int nj = (nout - io0 < BF_TILE) ? nout - io0 : BF_TILE;
+#pragma unroll 1
+    for (int j = 0; j < nj; j++) {
+      const int io = io0 + j;
+      const double bfj = bf_lds[j][threadIdx.x];
+      const double *lc = M.lamconst + (int64_t) out_ls[io] * RHG_LC_NCONST;
+      const double hc_kla = lc[RHG_LC_HC_KLA], twohnu3 = lc[RHG_LC_TWOHNU3_C2];
+      const double x = hc_kla * iT;
+      // Planck function, rh/planck.c:38-63, and the stimulated-emission
+      // factor exp(-x); past MAX_EXPONENT the Planck function is 0 and
+      // exp(-x) < 1e-65 is taken as 0
+      const bool in_range = x <= MAX_EXPONENT;
+      const double ex = exp(in_range ? x : MAX_EXPONENT);
+      const double Bnu = in_range ? twohnu3 / (ex - 1.0) : 0.0;
+      const double stim = in_range ? 1.0 / ex : 0.0;
+      double chi_ai = 0.0, eta_ai = 0.0, chi;
+      if (lc[RHG_LC_HMIN_BF_ON] != 0.0) {
+        chi_ai += nHmin * (1.0 - stim) * lc[RHG_LC_HMIN_BF_ALPHA];
+        eta_ai += nHmin * twohnu3 * stim * lc[RHG_LC_HMIN_BF_ALPHA];
+      }
+      if (lc[RHG_LC_HMIN_FF_LIDX] >= 0.0) {
+        double kappa = bilinear(NTHETA, NFF, M.hminff_kappa, ti_hmin, lc[RHG_LC_HMIN_FF_LIDX]);
+        chi = (nH0 * 1.0E-29) * pe * kappa;
+      } else {
+        double Ck = (KBOLTZMANN * THETA0 * 1.0E-32), theta_n = 1.0;
+        chi = 0.0;
+        for (int n = 1; n < 6; n++) {
+          theta_n *= sqrt_theta;
+          chi += theta_n * lc[RHG_LC_HFF_LONG0 + n - 1];
+        }
+        chi *= nH0 * (ne * Ck);
+      }
+      chi_ai += chi;
+      eta_ai += chi * Bnu;
+      if (lc[RHG_LC_OH_EIDX] >= 0.0 && oh_in) {
+        double kappa = exp(LG10 * bilinear(NTOH, NEOH, M.oh_cross, ti_oh, lc[RHG_LC_OH_EIDX])) *
+                       (CM_TO_M * CM_TO_M);
+        chi_ai += nOH * (1.0 - stim) * kappa;
+        eta_ai += nOH * twohnu3 * stim * kappa;
+      }
+      if (lc[RHG_LC_CH_EIDX] >= 0.0 && ch_in) {
+        double kappa = exp(LG10 * bilinear(NTCH, NECH, M.ch_cross, ti_ch, lc[RHG_LC_CH_EIDX])) *
+                       (CM_TO_M * CM_TO_M);
+        chi_ai += nCH * (1.0 - stim) * kappa;
+        eta_ai += nCH * twohnu3 * stim * kappa;
+      }
+      // bound-free (hydrogen and metals), rh/hydrogen.c:300-330, rh/metal.c:120-140
+      {
+        const double expla = (lc[RHG_LC_BF_HC_KLA] == hc_kla) ? stim : exp(-lc[RHG_LC_BF_HC_KLA] * iT);
+        chi_ai += (1.0 - expla) * bfj;
+        eta_ai += lc[RHG_LC_BF_TWOHNU3_C2] * expla * bfj;
+      }
+      {
+        double y = lc[RHG_LC_GFF_CY] * y_fac, x3 = lc[RHG_LC_GFF_X3];
+        double gIII = 1.0 + 0.1728 * x3 * (1.0 + y) - 0.0496 * (x3 * x3) * (1.0 + (1.0 + y) * 0.33333333 * y);
+        double g_ff = (gIII > 1.0) ? gIII : 1.0;
+        chi = sff_T * lc[RHG_LC_HFF_NU3] * hff_fac * (1.0 - stim) * g_ff;
+        chi_ai += chi;
+        eta_ai += chi * Bnu;
+      }
+      if (lc[RHG_LC_H2P_LIDX] >= 0.0) {
+        double kappa = bilinear(NTEMP_H2P, NFF_H2P, M.h2pff_kappa, ti_h2p, lc[RHG_LC_H2P_LIDX]);
+        chi = (nH0 * 1.0E-29) * (np * 1.0E-20) * kappa;
+        chi_ai += chi;
+        eta_ai += chi * Bnu;
+      }
+      if (lc[RHG_LC_H2M_LIDX] >= 0.0 && nH2 > 0.0) {
+        double kappa = bilinear(NTHETA_H2, NFF_H2, M.h2mff_kappa, ti_h2m, lc[RHG_LC_H2M_LIDX]);
+        chi = (nH2 * 1.0E-29) * pe * kappa;
+        chi_ai += chi;
+        eta_ai += chi * Bnu;
+      }
+      const int64_t o = ((int64_t) io * Nmax + k) * ncol + ic;
+      cchi[o] = (R) chi_ai;
+      ceta[o] = (R) eta_ai;
+    }
+  }
 }
+
+#define RHG_KSTATE(NN)                                                                       \
+  template __global__ void k_state<NN>(const DevModel *, const double *, const int32_t *, int, int,  \
+                                       double, int, double *, int);
+RHG_KSTATE(1)
+RHG_KSTATE(2)
+RHG_KSTATE(3)
+RHG_KSTATE(4)
+RHG_KSTATE(5)
+RHG_KSTATE(6)
+RHG_KSTATE(7)
+RHG_KSTATE(8)
+#undef RHG_KSTATE
+
+template __global__ void k_continuum<double>(const DevModel *, const int32_t *, int, int, const double *,
+                                             int, const int32_t *, int, double *, double *);
+template __global__ void k_continuum<float>(const DevModel *, const int32_t *, int, int, const double *,
+                                            int, const int32_t *, int, float *, float *);
 
 }  // namespace rhg

@@ -50,65 +50,109 @@ struct DevModel {
   const double *comp_shift, *comp_strength;
   const double *hminff_theta, *hminff_kappa, *h2mff_theta, *h2mff_kappa, *h2pff_temp,
       *h2pff_kappa, *oh_T, *oh_cross, *ch_T, *ch_cross;
+  // derived on the device copy (rhg_open):
+  // (lambda/lambda0 - 1) c for every (wavelength, line) entry of lam_line_idx
+  const double *lam_line_dv;
+  // bound-free cross-sections by tiles of BF_TILE output wavelengths: tile t
+  // holds the continua active anywhere in it, entries tile_ptr[t]..
+  // tile_ptr[t+1]-1, with cross-sections tile_alpha[entry][BF_TILE] (zero
+  // where a continuum is inactive)
+  int ntile;
+  const int32_t *tile_ptr, *tile_cont;
+  const double *tile_alpha;
+  // molecule stoichiometry as a dense table [nmol][NNUC_MAX] of counts
+  const double *mol_cnt;
 };
 
+// output wavelengths per bound-free tile of k_continuum, and its block size
+constexpr int BF_TILE = 16;
+constexpr int CONT_BLOCK = 256;
+
+// Workgroup -> (column block of 64, output wavelength) for the per-ray
+// kernels.  Workgroups are dispatched round-robin over the 8 XCDs (linear
+// id mod 8), each XCD with its own L2; the mapping sends all nout
+// wavelengths of a column block to one XCD, consecutively, so that the
+// block's column state is fetched into that L2 once and reused by every
+// wavelength.  Launch xcd_grid(ncb, nout) workgroups; ids that map past the
+// last column block return cb >= ncb.
+constexpr int NXCD = 8;
+__host__ __device__ __forceinline__ int64_t xcd_grid(int ncb, int nout) {
+  return (int64_t) ((ncb + NXCD - 1) / NXCD) * NXCD * nout;
+}
+__device__ __forceinline__ void xcd_tile(int64_t b, int nout, int &cb, int &io) {
+  const int64_t per_group = (int64_t) NXCD * nout;
+  const int64_t g = b / per_group, r = b % per_group;
+  cb = (int) (g * NXCD + r % NXCD);
+  io = (int) (r / NXCD);
+}
+
+// Column state layout: state[(field * Nmax + k) * ncol + column]; fields
+// RHG_CF_* (rhg_colfield), then per continuum its lower-level population
+// n_i, then per line chi_l, eta_l, adamp, 1/vbroad, then the depth-scale
+// input.  A kernel addresses one (column, depth) point with
+// st = state + k * ncol + column and field stride fs = Nmax * ncol.
+__host__ __device__ constexpr int cont_field(int ci) { return RHG_CF_NFIXED + ci; }
+
 // index of the first per-line field in the column state
-__device__ __forceinline__ int line_field_base(const DevModel &M) {
-  return RHG_CF_NFIXED + 3 * M.ncont;
+__host__ __device__ __forceinline__ int line_field_base(int ncont) {
+  return RHG_CF_NFIXED + ncont;
+}
+__host__ __device__ __forceinline__ int state_nfield(int ncont, int nline) {
+  return RHG_CF_NFIXED + ncont + 4 * nline + 1;
 }
 
 // Kurucz-line opacity and emissivity of the emergent ray (to_obs = 1) at
 // wavelength ls and the depth that st points at (field stride fs),
-// rh/kurucz.c:600-715 and RLKProfile rh/kurucz.c:729-827.
-// chiI/etaI accumulate from zero; chiQ[3]/etaQ[3] likewise.
+// rh/kurucz.c:600-715 and RLKProfile rh/kurucz.c:729-827, in precision R
+// (the Doppler offset (lambda/lambda0 - 1) c is FP64, from the model).
+// chiI/etaI and chiQ[3]/etaQ[3] accumulate from zero.
+template <typename R>
 __device__ __forceinline__ void line_point(const DevModel &M, int ls, const double *st, int64_t fs,
-                                           double mu, double &chiI, double *chiQ, double &etaI,
-                                           double *etaQ, double * /*unused*/, int moving) {
-  const double lambda = M.lamconst[(int64_t) ls * RHG_LC_NCONST + RHG_LC_LAMBDA];
+                                           R mu, R &chiI, R *chiQ, R &etaI, R *etaQ, int moving) {
   const int p0 = M.lam_line_ptr[ls], p1 = M.lam_line_ptr[ls + 1];
-  const int lb = line_field_base(M);
-  chiI = 0.0; etaI = 0.0;
-  chiQ[0] = chiQ[1] = chiQ[2] = 0.0;
-  etaQ[0] = etaQ[1] = etaQ[2] = 0.0;
-  const double vz = st[RHG_CF_VLOS * fs];
+  const int lb = line_field_base(M.ncont);
+  chiI = R(0); etaI = R(0);
+  chiQ[0] = chiQ[1] = chiQ[2] = R(0);
+  etaQ[0] = etaQ[1] = etaQ[2] = R(0);
+  const R vz = (R) st[RHG_CF_VLOS * fs];
   for (int e = p0; e < p1; e++) {
     const int l = M.lam_line_idx[e];
-    const double chi_l = st[(lb + 4 * l) * fs], eta_l = st[(lb + 4 * l + 1) * fs];
-    const double adamp = st[(lb + 4 * l + 2) * fs], vbroad = st[(lb + 4 * l + 3) * fs];
-    double v = (lambda / M.line_lambda0[l] - 1.0) * CLIGHT / vbroad;
-    if (moving) v += (mu * vz) / vbroad;
-    const double sv = 1.0 / (SQRTPI * vbroad);
-    double phi, phi_Q = 0.0, phi_U = 0.0, phi_V = 0.0;
+    const R chi_l = (R) st[(lb + 4 * l) * fs], eta_l = (R) st[(lb + 4 * l + 1) * fs];
+    const R adamp = (R) st[(lb + 4 * l + 2) * fs], ivb = (R) st[(lb + 4 * l + 3) * fs];
+    R v = (R) M.lam_line_dv[e] * ivb;
+    if (moving) v += (mu * vz) * ivb;
+    const R sv = ivb * R(INV_SQRTPI);
+    R phi, phi_Q = R(0), phi_U = R(0), phi_V = R(0);
     const bool polar = M.line_polar[l] != 0;
     if (M.line_Grad[l] != 0.0) {
       if (polar) {
-        const double cosg = st[RHG_CF_COSG * fs];
-        const double sin2_gamma = 1.0 - cosg * cosg;
-        const double vB = M.line_larmor[l] * st[RHG_CF_B * fs] / vbroad;
-        const double sign = 1.0;
-        double phi_sm = 0.0, phi_pi = 0.0, phi_sp = 0.0, H, F;
+        const R cosg = (R) st[RHG_CF_COSG * fs];
+        const R sin2_gamma = R(1.0) - cosg * cosg;
+        const R vB = (R) M.line_larmor[l] * (R) st[RHG_CF_B * fs] * ivb;
+        R phi_sm = R(0), phi_pi = R(0), phi_sp = R(0), H, F;
         const int c0 = M.line_comp_ptr[l], c1 = M.line_comp_ptr[l + 1];
         for (int nz = c0; nz < c1; nz++) {
-          H = voigt_humlicek(adamp, v - M.comp_shift[nz] * vB, &F);
+          H = voigt_humlicek<R>(adamp, v - (R) M.comp_shift[nz] * vB, &F);
           const int q = M.comp_q[nz];
-          if (q == -1) phi_sm += M.comp_strength[nz] * H;
-          else if (q == 0) phi_pi += M.comp_strength[nz] * H;
-          else phi_sp += M.comp_strength[nz] * H;
+          const R sH = (R) M.comp_strength[nz] * H;
+          if (q == -1) phi_sm += sH;
+          else if (q == 0) phi_pi += sH;
+          else phi_sp += sH;
         }
-        const double phi_sigma = phi_sp + phi_sm;
-        const double phi_delta = 0.5 * phi_pi - 0.25 * phi_sigma;
-        phi = (phi_delta * sin2_gamma + 0.5 * phi_sigma) * sv;
-        phi_Q = sign * phi_delta * sin2_gamma * st[RHG_CF_COS2X * fs] * sv;
-        phi_U = phi_delta * sin2_gamma * st[RHG_CF_SIN2X * fs] * sv;
-        phi_V = sign * 0.5 * (phi_sp - phi_sm) * cosg * sv;
+        const R phi_sigma = phi_sp + phi_sm;
+        const R phi_delta = R(0.5) * phi_pi - R(0.25) * phi_sigma;
+        phi = (phi_delta * sin2_gamma + R(0.5) * phi_sigma) * sv;
+        phi_Q = phi_delta * sin2_gamma * (R) st[RHG_CF_COS2X * fs] * sv;
+        phi_U = phi_delta * sin2_gamma * (R) st[RHG_CF_SIN2X * fs] * sv;
+        phi_V = R(0.5) * (phi_sp - phi_sm) * cosg * sv;
       } else {
-        phi = voigt_armstrong(adamp, v) * sv;
+        phi = (R) voigt_armstrong((double) adamp, (double) v) * sv;
       }
     } else {
-      phi = (fabs(v) <= MAX_GAUSS_DOPPLER) ? exp(-v * v) : 0.0;
+      phi = (fabs(v) <= R(MAX_GAUSS_DOPPLER)) ? rexp(-v * v) : R(0);
       phi = phi * sv;
     }
-    if (phi != 0.0) {
+    if (phi != R(0)) {
       chiI += chi_l * phi;
       etaI += eta_l * phi;
       if (polar && M.line_Grad[l] != 0.0) {
@@ -128,21 +172,20 @@ __device__ __forceinline__ void line_point(const DevModel &M, int ls, const doub
 // line_point(), with counts in place of arithmetic (slots: RHG_WORK_*).
 __device__ __forceinline__ void line_census(const DevModel &M, int ls, const double *st, int64_t fs,
                                             double mu, int moving, int32_t *cnt) {
-  const double lambda = M.lamconst[(int64_t) ls * RHG_LC_NCONST + RHG_LC_LAMBDA];
   const int p0 = M.lam_line_ptr[ls], p1 = M.lam_line_ptr[ls + 1];
-  const int lb = line_field_base(M);
+  const int lb = line_field_base(M.ncont);
   const double vz = st[RHG_CF_VLOS * fs];
   for (int e = p0; e < p1; e++) {
     const int l = M.lam_line_idx[e];
-    const double adamp = st[(lb + 4 * l + 2) * fs], vbroad = st[(lb + 4 * l + 3) * fs];
-    double v = (lambda / M.line_lambda0[l] - 1.0) * CLIGHT / vbroad;
-    if (moving) v += (mu * vz) / vbroad;
+    const double adamp = st[(lb + 4 * l + 2) * fs], ivb = st[(lb + 4 * l + 3) * fs];
+    double v = M.lam_line_dv[e] * ivb;
+    if (moving) v += (mu * vz) * ivb;
     cnt[RHG_WORK_LINE]++;
     if (M.line_Grad[l] != 0.0) {
       if (M.line_polar[l] != 0) {
-        const double vB = M.line_larmor[l] * st[RHG_CF_B * fs] / vbroad;
+        const double vB = M.line_larmor[l] * st[RHG_CF_B * fs] * ivb;
         for (int nz = M.line_comp_ptr[l]; nz < M.line_comp_ptr[l + 1]; nz++)
-          cnt[RHG_WORK_VOIGT_I + humlicek_region(adamp, v - M.comp_shift[nz] * vB)]++;
+          cnt[RHG_WORK_VOIGT_I + humlicek_region<double>(adamp, v - M.comp_shift[nz] * vB)]++;
       } else {
         cnt[RHG_WORK_ARMSTRONG]++;
       }
@@ -152,20 +195,27 @@ __device__ __forceinline__ void line_census(const DevModel &M, int ls, const dou
   }
 }
 
-__global__ void k_state(DevModel M, const double *atmos, const int32_t *ndep, int ncol, int Nmax,
+template <int NN>
+__global__ void k_state(const DevModel *M, const double *atmos, const int32_t *ndep, int ncol, int Nmax,
                         double mu, int atm_scale, double *state, int nfield);
-__global__ void k_geometry(DevModel M, const int32_t *ndep, int ncol, int Nmax, int atm_scale,
+__global__ void k_geometry(const DevModel *M, const int32_t *ndep, int ncol, int Nmax, int atm_scale,
                            double mu, double *state, int nfield, int32_t *moving_out,
                            double *chiref_out);
-__global__ void k_continuum(DevModel M, const int32_t *ndep, int ncol, int Nmax,
+template <typename R>
+__global__ void k_continuum(const DevModel *M, const int32_t *ndep, int ncol, int Nmax,
                             const double *state, int nfield, const int32_t *out_ls, int nout,
-                            double *cchi, double *ceta);
-__global__ void k_formal(DevModel M, const int32_t *ndep, const int32_t *moving, int ncol,
-                         int Nmax, const double *state, int nfield, const int32_t *out_ls,
-                         int nout, const double *cchi, const double *ceta, double mu,
-                         double *stokes);
-
-__global__ void k_census(DevModel M, const int32_t *ndep, const int32_t *moving, int ncol,
+                            R *cchi, R *ceta);
+template <typename R>
+__global__ void k_formal_stokes(const DevModel *M, const int32_t *ndep, const int32_t *moving,
+                                int ncol, int Nmax, const double *state, const int32_t *out_ls,
+                                const int32_t *io_list, int nlist, int nout, const R *cchi,
+                                const R *ceta, double mu, double *stokes);
+template <typename R>
+__global__ void k_formal_scalar(const DevModel *M, const int32_t *ndep, const int32_t *moving,
+                                int ncol, int Nmax, const double *state, const int32_t *out_ls,
+                                const int32_t *io_list, int nlist, int nout, const R *cchi,
+                                const R *ceta, double mu, double *stokes);
+__global__ void k_census(const DevModel *M, const int32_t *ndep, const int32_t *moving, int ncol,
                          int Nmax, const double *state, int nfield, const int32_t *out_ls,
                          int nout, double mu, int32_t *census);
 __global__ void k_census_sum(const int32_t *census, int ncol, int nout, int64_t *col_sum);

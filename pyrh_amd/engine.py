@@ -6,13 +6,16 @@ calls raise.
 from __future__ import annotations
 
 import ctypes as C
+import os
+import sys
 from pathlib import Path
 
 import numpy as np
 
 from .abi import ColumnsStruct, ModelStruct, RhgColumns, RhgModel
 
-LIB_PATH = Path(__file__).resolve().parent / "libpyrh_amd.so"
+# the in-tree build; PYRH_AMD_LIB names another build of the same library
+LIB_PATH = Path(os.environ.get("PYRH_AMD_LIB", Path(__file__).resolve().parent / "libpyrh_amd.so"))
 _LIB = None
 
 RHG_NSTAGE = 4
@@ -24,10 +27,24 @@ class EngineError(RuntimeError):
     pass
 
 
+def _share_hip_runtime():
+    """PyTorch-ROCm bundles its own libamdhip64.so.7 with the same soname as
+    the system one this library links to; the first one loaded serves the
+    whole process, and torch cannot initialise on another build of it.
+    Loading torch first (when installed) makes the library use torch's HIP
+    runtime, so both work in one process and share device pointers."""
+    if "torch" not in sys.modules:
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+
+
 def lib():
     """Load libpyrh_amd.so (raises if it has not been built)."""
     global _LIB
     if _LIB is None:
+        _share_hip_runtime()
         if not LIB_PATH.exists():
             raise EngineError(f"{LIB_PATH} is not built; run pyrh_amd.build.build_library() "
                               f"(or __graft_entry__.build())")

@@ -145,3 +145,36 @@ def test_gpu_work_census(engines):
     # the line window spans several Doppler widths: every region occurs
     assert min(c["voigt_I"], c["voigt_II"], c["voigt_III"], c["voigt_IV"]) > 0
     print(c, flush=True)
+
+
+def test_gpu_fp32_form_accuracy(engines):
+    """FP32 form (float line opacity and DELO-Bezier recursion, FP64
+    equilibria, heights and Bezier weights) against the FP64 path:
+    I within 1e-4 of I_c, Q/U/V within 1e-5 I_c."""
+    wave = workloads.wavelengths_630()
+    eng = engines("lte", wave)
+    atm = workloads.stokes_map(512, seed=51)
+    s64 = eng.synth_lte(atm)
+    s32 = eng.synth_lte(atm, precision=1)
+    Ic = s64[:, 0, :1]
+    eI = np.max(np.abs(s32[:, 0] - s64[:, 0]) / Ic)
+    eP = np.max(np.abs(s32[:, 1:] - s64[:, 1:]) / Ic[:, None])
+    print(f"FP32 form: |dI|/Ic {eI:.2e}  |dQUV|/Ic {eP:.2e}", flush=True)
+    assert eI <= 1e-4 and eP <= 1e-5
+
+
+def test_gpu_device_pointer_path(engines):
+    """Atmospheres and outputs in torch device tensors (no host copies)
+    give the same Stokes vectors as the host-buffer call."""
+    import torch
+    wave = workloads.wavelengths_630()
+    eng = engines("lte", wave)
+    atm = workloads.stokes_map(300, seed=61)
+    ndep = np.full(300, 82, np.int32)
+    ndep[7] = 64
+    ref = eng.synth_lte(atm, ndep=ndep, mu=0.8)
+    d_atm = torch.from_numpy(atm).cuda()
+    d_nd = torch.from_numpy(ndep).cuda()
+    d_out = torch.empty((300, 4, eng.nout), dtype=torch.float64, device="cuda")
+    eng.synth_lte_device(d_atm.data_ptr(), d_nd.data_ptr(), 300, 82, d_out.data_ptr(), mu=0.8)
+    np.testing.assert_array_equal(d_out.cpu().numpy(), ref)
