@@ -12,7 +12,7 @@ os.environ.setdefault("PYRH_PATH", str(ROOT / "tests" / "golden" / "rhdata"))
 
 GOLDEN = ROOT / "tests" / "golden"
 RUNS = {"lte": GOLDEN / "run_lte", "nostokes": GOLDEN / "run_lte_nostokes",
-        "unpol": GOLDEN / "run_lte_unpol"}
+        "unpol": GOLDEN / "run_lte_unpol", "falc": GOLDEN / "run_falc"}
 
 
 def pytest_configure(config):

@@ -25,8 +25,10 @@ sys.path.insert(0, str(HERE.parents[1] / "oracle"))
 import refrun  # noqa: E402
 from pyrh_amd import workloads as W  # noqa: E402
 
+import falc  # noqa: E402
+
 RUN = {"lte": HERE / "run_lte", "nostokes": HERE / "run_lte_nostokes",
-       "unpol": HERE / "run_lte_unpol"}
+       "unpol": HERE / "run_lte_unpol", "falc": HERE / "run_falc"}
 
 
 def cases():
@@ -54,6 +56,13 @@ def cases():
     mv = mp[2:4].copy()
     mv[:, 3] = np.where(np.abs(mv[:, 3]) < 0.2, 0.5, mv[:, 3])
     yield "unpol_moving", "unpol", mv, wave, 1.0, 0
+    # the reference's benchmark case: FALC, depth-dependent velocities, B = 0,
+    # line records without orbital quantum numbers (Unsold broadening)
+    fa = falc.atmosphere()[None]
+    yield "falc", "falc", fa, falc.wavelengths(), 1.0, 0
+    fb = fa.copy()
+    fb[0, 5] = 1500.0
+    yield "falc_mu05", "falc", fb, falc.wavelengths(), 0.5, 0
 
 
 def main():
