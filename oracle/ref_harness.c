@@ -246,6 +246,48 @@ void Piecewise_Bezier3_1D(int nspect, int mu, bool_t to_obs, double *chi,
 }
 
 /* ------------------------------------------------------------------- */
+/* Optional per-run overrides of pyrh.compute1d's keyword arguments, from
+   the text file named by RHREF_OVERRIDES, one per line:
+       loggf <line index> <log gf>
+       lam   <line index> <delta lambda [nm]>
+       abun  <atomic number> <abundance>
+       rf    <output path>      (also request get_atomic_rfs; the response
+                                 functions [nlw][Nloggf+Nlam] go there)   */
+typedef struct {
+  int nloggf, nlam, nabun;
+  int loggf_ids[64], lam_ids[64], abun_ids[64];
+  double loggf_val[64], lam_val[64], abun_val[64];
+  FILE *rf_out;
+} Overrides;
+
+static int read_overrides(const char *path, Overrides *ov) {
+  char key[16], line[512], rfpath[400];
+  memset(ov, 0, sizeof(*ov));
+  if (!path) return 0;
+  FILE *f = fopen(path, "r");
+  if (!f) { perror("overrides"); return -1; }
+  while (fgets(line, sizeof(line), f)) {
+    int id;
+    double v;
+    if (sscanf(line, "%15s", key) != 1) continue;
+    if (strcmp(key, "rf") == 0) {
+      if (sscanf(line, "%*s %399s", rfpath) == 1) ov->rf_out = fopen(rfpath, "wb");
+      if (!ov->rf_out) { perror("rf output"); fclose(f); return -1; }
+      continue;
+    }
+    if (sscanf(line, "%*s %d %lf", &id, &v) != 2) continue;
+    if (strcmp(key, "loggf") == 0 && ov->nloggf < 64) {
+      ov->loggf_ids[ov->nloggf] = id; ov->loggf_val[ov->nloggf++] = v;
+    } else if (strcmp(key, "lam") == 0 && ov->nlam < 64) {
+      ov->lam_ids[ov->nlam] = id; ov->lam_val[ov->nlam++] = v;
+    } else if (strcmp(key, "abun") == 0 && ov->nabun < 64) {
+      ov->abun_ids[ov->nabun] = id; ov->abun_val[ov->nabun++] = v;
+    }
+  }
+  fclose(f);
+  return 0;
+}
+
 int main(int argc, char **argv)
 {
   int hdr[5], Ncol, Ndep, Nwave, atm_scale, c, n;
@@ -253,6 +295,7 @@ int main(int argc, char **argv)
   FILE *fi, *fo;
 
   int hse_mode = 0;
+  Overrides ov;
   if (argc > 1 && strcmp(argv[1], "--hse") == 0) { hse_mode = 1; argv++; argc--; }
   if (argc < 4) {
     fprintf(stderr, "usage: %s rundir in.bin out.bin [capture.bin]\n", argv[0]);
@@ -275,6 +318,7 @@ int main(int argc, char **argv)
   if (!fo) { perror("out"); return 2; }
   if (argc > 4) cap = fopen(argv[4], "wb");
   if (getenv("RHREF_CAPTURE_VOIGT")) cap_voigt = 1;
+  if (read_overrides(getenv("RHREF_OVERRIDES"), &ov) != 0) return 2;
 
   for (c = 0; c < Ncol; c++) {
     mySpectrum spec;
@@ -296,8 +340,14 @@ int main(int argc, char **argv)
     spec = rhf1d(".", mu, Ndep, atm, atm + Ndep, atm + 2 * Ndep,
                  atm + 3 * Ndep, atm + 4 * Ndep, atm + 5 * Ndep,
                  atm + 6 * Ndep, atm + 7 * Ndep, atm + 8 * Ndep, atm_scale,
-                 Nwave, w, 0, NULL, NULL, 0, NULL, NULL, 0, NULL, NULL,
-                 0, NULL, NULL, 0, 0, 0, ".");
+                 Nwave, w, 0, NULL, NULL, ov.nloggf, ov.loggf_ids, ov.loggf_val,
+                 ov.nlam, ov.lam_ids, ov.lam_val, ov.nabun, ov.abun_ids, ov.abun_val,
+                 ov.rf_out != NULL, 0, 0, ".");
+    if (ov.rf_out && spec.rfs) {
+      int np = ov.nloggf + ov.nlam;
+      for (int i = 0; i < spec.nlw; i++) fwrite(spec.rfs[i], sizeof(double), np, ov.rf_out);
+      fflush(ov.rf_out);
+    }
     zero = calloc(spec.nlw, sizeof(double));
     fwrite(&spec.nlw, sizeof(int), 1, fo);
     fwrite(spec.lam, sizeof(double), spec.nlw, fo);
@@ -313,5 +363,6 @@ int main(int argc, char **argv)
   fclose(fo);
   fclose(fi);
   if (cap) fclose(cap);
+  if (ov.rf_out) fclose(ov.rf_out);
   return 0;
 }

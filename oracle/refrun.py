@@ -55,12 +55,30 @@ def read_capture(path):
 
 
 def run_synth(rundir, atmos, wave, mu=1.0, atm_scale=0, variant="o2",
-              capture=False, timeout=3600):
-    """Call rhf1d() once per column.  Returns (lam, I, Q, U, V[, capture])."""
+              capture=False, timeout=3600, loggf=(), lam=(), abun=(), rf=False):
+    """Call rhf1d() once per column.  Returns (lam, I, Q, U, V[, capture]
+    [, rf]).  loggf / lam / abun are sequences of (index, value) pairs passed
+    as pyrh.compute1d's loggf_ids/values, lam_ids/values and
+    atomic_number/abundance; rf requests get_atomic_rfs and returns the
+    response functions [ncol][nlw][len(loggf) + len(lam)]."""
     exe = REF_DIR / f"rhref_run_{variant}"
     env = dict(os.environ, PYRH_PATH=str(DATA_DIR))
     with tempfile.TemporaryDirectory() as td:
         fin, fout, fcap = (os.path.join(td, n) for n in ("in", "out", "cap"))
+        frf = os.path.join(td, "rf")
+        npar = len(loggf) + len(lam)
+        if loggf or lam or abun or rf:
+            fov = os.path.join(td, "overrides")
+            with open(fov, "w") as f:
+                for i, v in loggf:
+                    f.write(f"loggf {int(i)} {float(v)!r}\n")
+                for i, v in lam:
+                    f.write(f"lam {int(i)} {float(v)!r}\n")
+                for i, v in abun:
+                    f.write(f"abun {int(i)} {float(v)!r}\n")
+                if rf:
+                    f.write(f"rf {frf}\n")
+            env["RHREF_OVERRIDES"] = fov
         ncol, ndep = _write_input(fin, atmos, wave, mu, atm_scale)
         cmd = [str(exe), str(rundir), fin, fout] + ([fcap] if capture else [])
         r = subprocess.run(cmd, env=env, capture_output=True, timeout=timeout)
@@ -81,6 +99,8 @@ def run_synth(rundir, atmos, wave, mu=1.0, atm_scale=0, variant="o2",
         out = (lam, res[:, 1], res[:, 2], res[:, 3], res[:, 4])
         if capture:
             out = out + (read_capture(fcap),)
+        if rf:
+            out = out + (np.fromfile(frf, np.float64).reshape(ncol, -1, npar),)
         return out
 
 

@@ -65,6 +65,26 @@ def cases():
     yield "falc_mu05", "falc", fb, falc.wavelengths(), 0.5, 0
 
 
+# pyrh.compute1d keyword overrides (pyrh.pyx:537-551): log gf and wavelength
+# shifts of Kurucz lines by index, element abundances by atomic number
+# (the reference carries d I / d log gf only through the scalar Bezier
+# solver, bezier_1D.c:416-536: the NO_STOKES case with moving columns has
+# them, the FULL_STOKES one returns zeros)
+OVERRIDES = {
+    "overrides": dict(loggf=[(0, -0.5), (1, -1.2)], lam=[(1, 0.002)], abun=[(26, 7.30)]),
+    "rf_nostokes": dict(loggf=[(0, -0.71), (1, -0.969)], lam=[(0, 0.0)]),
+}
+
+
+def override_cases():
+    wave = W.wavelengths_630()
+    mp = W.stokes_map(16, seed=W.SEED_BASE + 101)
+    yield "overrides", "lte", mp[:3], wave, 1.0, 0
+    mv = mp[4:7].copy()
+    mv[:, 3] = np.where(np.abs(mv[:, 3]) < 0.2, 0.5, mv[:, 3])
+    yield "rf_nostokes", "nostokes", mv, wave, 1.0, 0
+
+
 def main():
     if not refrun.ref_available("o2"):
         raise SystemExit("reference build missing: make -C oracle ref")
@@ -90,6 +110,25 @@ def main():
             ref = [c for c in cap if c[0] == "bg_chi" and c[2] == 0 and c[4] == 1][0][5]
             out[f"{name}__chiref"] = ref
         print(name, atm.shape, lam.size)
+    for name, run, atm, wave, mu, scale in override_cases():
+        ov = OVERRIDES[name]
+        lam, I, Q, U, V, rf = refrun.run_synth(RUN[run], atm, wave, mu=mu, atm_scale=scale,
+                                               variant="o2", rf=True, **ov)
+        out[f"{name}__run"] = np.array(run)
+        out[f"{name}__atmos"] = atm
+        out[f"{name}__wave"] = wave
+        out[f"{name}__mu"] = np.array(mu)
+        out[f"{name}__scale"] = np.array(scale)
+        out[f"{name}__lam"] = lam
+        out[f"{name}__stokes_o2"] = np.stack([I, Q, U, V], axis=1)
+        for key in ("loggf", "lam", "abun"):
+            pairs = np.array(ov.get(key, []), dtype=np.float64).reshape(-1, 2)
+            out[f"{name}__ov_{key}"] = pairs
+        # the reference's analytic d I / d log gf (kurucz.c:224-260), per
+        # column; its d I / d lambda0 columns are allocated but never filled
+        # (uninitialised memory), so only the log gf part is kept
+        out[f"{name}__rf_loggf"] = rf[:, :, :len(ov.get("loggf", []))]
+        print(name, atm.shape, lam.size, rf.shape)
     np.savez_compressed(HERE / "lte_fixtures.npz", **out)
 
 

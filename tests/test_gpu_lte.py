@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import pyoracle
-from conftest import fixture_cases
+from conftest import RUNS, fixture_cases, golden_model, override_kwargs
 from pyrh_amd import model, workloads
 
 pytestmark = pytest.mark.gpu
@@ -27,6 +27,18 @@ def engines(setups):
     return get
 
 
+@pytest.fixture(scope="module")
+def golden_engine(fixtures, setups):
+    from pyrh_amd.engine import Engine
+    cache = {}
+
+    def get(name):
+        if name not in cache:
+            cache[name] = Engine(golden_model(fixtures, setups, name))
+        return cache[name]
+    return get
+
+
 def _err(gpu, ref):
     Ic = ref[:, 0, :1]
     eI = np.max(np.abs(gpu[:, 0] - ref[:, 0]) / np.abs(ref[:, 0]))
@@ -34,10 +46,10 @@ def _err(gpu, ref):
     return eI, eP
 
 
-def test_gpu_matches_golden_vectors(fixtures, engines):
+def test_gpu_matches_golden_vectors(fixtures, golden_engine):
     F = fixtures
     for name in fixture_cases(F):
-        eng = engines(str(F[name + "__run"]), F[name + "__wave"])
+        eng = golden_engine(name)
         gpu = eng.synth_lte(F[name + "__atmos"], mu=float(F[name + "__mu"]),
                             atm_scale=int(F[name + "__scale"]))
         eI, eP = _err(gpu, F[name + "__stokes_o2"])
@@ -178,3 +190,20 @@ def test_gpu_device_pointer_path(engines):
     d_out = torch.empty((300, 4, eng.nout), dtype=torch.float64, device="cuda")
     eng.synth_lte_device(d_atm.data_ptr(), d_nd.data_ptr(), 300, 82, d_out.data_ptr(), mu=0.8)
     np.testing.assert_array_equal(d_out.cpu().numpy(), ref)
+
+
+def test_gpu_compute1d_api(fixtures):
+    """pyrh_amd.api.compute1d with pyrh's arguments, including log gf,
+    wavelength-shift and abundance overrides, on one golden column."""
+    from pyrh_amd import api
+    F = fixtures
+    for name in ("single", "overrides"):
+        kw = override_kwargs(F, name)
+        atm = F[name + "__atmos"][0]
+        I, Q, U, V, lam = api.compute1d(str(RUNS[str(F[name + "__run"])]), float(F[name + "__mu"]),
+                                        int(F[name + "__scale"]), atm, F[name + "__wave"], **kw)
+        ref = F[name + "__stokes_o2"][0]
+        np.testing.assert_array_equal(lam, F[name + "__lam"])
+        assert np.max(np.abs(I - ref[0]) / ref[0]) <= TOL_I
+        for n, X in ((1, Q), (2, U), (3, V)):
+            assert np.max(np.abs(X - ref[n])) <= TOL_QUV * ref[0, 0]

@@ -8,12 +8,11 @@ import numpy as np
 import pytest
 
 import pyoracle
-from conftest import fixture_cases
+from conftest import fixture_cases, golden_model
 
 
 def _run(F, setups, name, fp32=False):
-    from pyrh_amd import model
-    m = model.build_model(setups[str(F[name + "__run"])], F[name + "__wave"])
+    m = golden_model(F, setups, name)
     st = pyoracle.synth_lte(m, F[name + "__atmos"], mu=float(F[name + "__mu"]),
                             atm_scale=int(F[name + "__scale"]), inverse_fp32=fp32)
     return m, st
@@ -21,7 +20,7 @@ def _run(F, setups, name, fp32=False):
 
 @pytest.mark.parametrize("name", ["single", "map16", "mu07", "cmass", "feautrier",
                                   "nostokes", "unpol_static", "unpol_moving", "falc",
-                                  "falc_mu05"])
+                                  "falc_mu05", "overrides", "rf_nostokes"])
 def test_oracle_matches_reference(fixtures, setups, name):
     F = fixtures
     m, st = _run(F, setups, name)
@@ -37,7 +36,7 @@ def test_fixture_cases_cover_every_solver(fixtures, setups):
     from pyrh_amd import model
     seen = set()
     for name in fixture_cases(fixtures):
-        m = model.build_model(setups[str(fixtures[name + "__run"])], fixtures[name + "__wave"])
+        m = golden_model(fixtures, setups, name)
         seen |= set(np.unique(m.arrays["solver"][m.out_index]).tolist())
     assert seen == {0, 1, 2}
 
@@ -82,3 +81,14 @@ def test_reference_benchmark_spectrum_continuum(fixtures, setups):
     # B = 0: Q, U, V vanish to round-off in both
     assert np.max(np.abs(st[0, 1:])) < 1e-16 * I.max()
     assert np.max(np.abs(ref[1:4])) < 1e-16 * ref[0].max()
+
+
+def test_overrides_change_the_spectrum(fixtures, setups):
+    """The overrides case differs from the same columns without overrides
+    (so the log gf / wavelength / abundance plumbing is exercised)."""
+    F = fixtures
+    from pyrh_amd import model
+    m0 = model.build_model(setups["lte"], F["overrides__wave"])
+    st0 = pyoracle.synth_lte(m0, F["overrides__atmos"])
+    ref = F["overrides__stokes_o2"]
+    assert np.max(np.abs(st0[:, 0] - ref[:, 0]) / ref[:, 0]) > 1e-3
