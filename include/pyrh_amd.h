@@ -79,6 +79,7 @@ enum rhg_colfield {
   RHG_CF_NP,       /* protons                            */
   RHG_CF_NH2, RHG_CF_NOH, RHG_CF_NCH,
   RHG_CF_HEIGHT,   /* [m], after convertScales           */
+  RHG_CF_DZ,       /* |z[k-1] - z[k]| [m], k >= 1        */
   RHG_CF_NFIXED    /* continuum and line terms follow    */
 };
 

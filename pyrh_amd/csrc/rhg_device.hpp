@@ -99,10 +99,18 @@ __device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
   return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r};
 }
 
+// Reciprocal and quotient: IEEE-exact in FP64; in the FP32 form the
+// hardware reciprocal (v_rcp_f32, 1 ulp), which the form's 1e-4 tolerance
+// allows.
+__device__ __forceinline__ double rrcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float rrcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float rdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
 // a / b with one reciprocal of |b|^2
 template <typename T>
 __device__ __forceinline__ cpx<T> cdiv(cpx<T> a, cpx<T> b) {
-  const T id = T(1) / (b.r * b.r + b.i * b.i);
+  const T id = rrcp(b.r * b.r + b.i * b.i);
   return {(a.r * b.r + a.i * b.i) * id, (a.i * b.r - a.r * b.i) * id};
 }
 
@@ -295,11 +303,11 @@ template <typename R>
 struct CentDeriv {
   R idup, idn, alpha;
   __device__ __forceinline__ CentDeriv(R dsup, R dsdn)
-      : idup(R(1) / dsup), idn(R(1) / dsdn),
-        alpha(R(0.333333333333333333333333) * (R(1.0) + dsdn / (dsdn + dsup))) {}
+      : idup(rrcp(dsup)), idn(rrcp(dsdn)),
+        alpha(R(0.333333333333333333333333) * (R(1.0) + rdiv(dsdn, dsdn + dsup))) {}
   __device__ __forceinline__ R operator()(R up, R c, R dn) const {
     const R fim1 = (c - up) * idup, fi = (dn - c) * idn;
-    if (fim1 * fi > R(0)) return (fim1 * fi) / ((R(1.0) - alpha) * fim1 + alpha * fi);
+    if (fim1 * fi > R(0)) return rdiv(fim1 * fi, (R(1.0) - alpha) * fim1 + alpha * fi);
     return R(0.0);
   }
 };
@@ -328,6 +336,60 @@ __device__ __forceinline__ void bezier3_coeffs(double dt, double &alpha, double 
     gamma = 0.25 * dt - 0.10 * dt2 + dt3 * 0.025 - dt4 / 210.0;
     theta = 0.25 * dt - 0.15 * dt2 + dt3 * 0.05 - dt4 / 84.0;
   }
+}
+
+// Cubic Bezier weights (alpha, beta, gamma, theta, eps = exp(-dt)) in the
+// precision of the recursion.  FP64: the reference's branches,
+// bezier3_coeffs().  FP32: the closed forms cancel (at dt = 0.05 the
+// numerator of alpha is 1.5e-6 of its terms), so below dt = 1 the weights
+// are their Taylor series about 0, whose coefficients follow from
+// exp(-t) = sum e_k t^k with e_k = (-1)^k / k!:
+//   alpha_m = 6 e_{m+3},  beta_m = -(6 e_{m+3} + 6 e_{m+2} + 3 e_{m+1} + e_m),
+//   gamma_m = -3 (6 e_{m+3} + 2 e_{m+2}),  theta_m = 3 (6 e_{m+3} + 4 e_{m+2} + e_{m+1})
+// (m = 1..10; the first four terms are the reference's 4th-order branch),
+// and from dt = 1 on the closed forms, whose cancellation is then < 30x.
+template <typename R>
+struct Bezier3 {
+  R alpha, beta, gamma, theta, eps;
+};
+
+constexpr double e_k(int k) {
+  double f = 1.0;
+  for (int i = 2; i <= k; i++) f *= i;
+  return ((k & 1) ? -1.0 : 1.0) / f;
+}
+constexpr int BEZ_NSERIES = 10;
+
+__device__ __forceinline__ Bezier3<double> bezier3(double dt) {
+  Bezier3<double> b;
+  bezier3_coeffs(dt, b.alpha, b.beta, b.gamma, b.theta, b.eps);
+  return b;
+}
+
+__device__ __forceinline__ Bezier3<float> bezier3(float dt) {
+  Bezier3<float> b;
+  b.eps = __expf(-dt);
+  if (dt < 1.0f) {
+    float a = 0.0f, be = 0.0f, g = 0.0f, th = 0.0f;
+#pragma unroll
+    for (int m = BEZ_NSERIES; m >= 1; m--) {
+      a = a * dt + (float) (6.0 * e_k(m + 3));
+      be = be * dt + (float) (-(6.0 * e_k(m + 3) + 6.0 * e_k(m + 2) + 3.0 * e_k(m + 1) + e_k(m)));
+      g = g * dt + (float) (-3.0 * (6.0 * e_k(m + 3) + 2.0 * e_k(m + 2)));
+      th = th * dt + (float) (3.0 * (6.0 * e_k(m + 3) + 4.0 * e_k(m + 2) + e_k(m + 1)));
+    }
+    b.alpha = a * dt;
+    b.beta = be * dt;
+    b.gamma = g * dt;
+    b.theta = th * dt;
+  } else {
+    const float dt2 = dt * dt, idt3 = rrcp(dt2 * dt), e = b.eps;
+    b.alpha = (-6.0f + 6.0f * dt - 3.0f * dt2 + dt2 * dt + 6.0f * e) * idt3;
+    b.beta = (6.0f + (-6.0f - dt * (6.0f + dt * (3.0f + dt))) * e) * idt3;
+    b.gamma = 3.0f * (6.0f + (-4.0f + dt) * dt - 2.0f * (3.0f + dt) * e) * idt3;
+    b.theta = 3.0f * (e * (6.0f + dt2 + 4.0f * dt) + 2.0f * dt - 6.0f) * idt3;
+  }
+  return b;
 }
 
 }  // namespace rhg

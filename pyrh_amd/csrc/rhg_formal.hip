@@ -44,7 +44,7 @@ __device__ __forceinline__ void form_point(const DevModel &M, int ls, int has_li
     for (int n = 0; n < 3; n++) { chi4[n + 1] += lq[n]; eta4[n + 1] += le[n]; }
   }
   p.chi = chi4[0];
-  const R ichi = R(1) / chi4[0];
+  const R ichi = rrcp(chi4[0]);
 #pragma unroll
   for (int n = 0; n < 4; n++) p.S[n] = eta4[n] * ichi;
 #pragma unroll
@@ -71,12 +71,12 @@ __device__ __forceinline__ void solve_md(R c, R g, const R *q, const R *dq, cons
   R b[3];
 #pragma unroll
   for (int i = 0; i < 3; i++) b[i] = c * q[i] + g * dq[i];
-  const R ga = g / a;
+  const R ga = rdiv(g, a);
   const R qb = q[0] * b[0] + q[1] * b[1] + q[2] * b[2];
   const R bb = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
   const R qV = q[0] * V[1] + q[1] * V[2] + q[2] * V[3];
   const R bV = b[0] * V[1] + b[1] * V[2] + b[2] * V[3];
-  const R x0 = (V[0] - bV + ga * qV * qb) / (a - bb + ga * qb * qb);
+  const R x0 = rdiv(V[0] - bV + ga * qV * qb, a - bb + ga * qb * qb);
   R y[3];
 #pragma unroll
   for (int i = 0; i < 3; i++) y[i] = V[i + 1] - b[i] * x0;
@@ -106,6 +106,7 @@ __device__ __forceinline__ R rmax0(R x) { return x > R(0) ? x : R(0); }
   const int64_t fs = (int64_t) Nmax * ncol;                                              \
   const double *st0 = state + ic;                                                        \
   const double *zc = st0 + (int64_t) RHG_CF_HEIGHT * fs;                                 \
+  const double *dzc = st0 + (int64_t) RHG_CF_DZ * fs;                                    \
   const double *Tc = st0 + (int64_t) RHG_CF_T * fs;                                      \
   const R *cc = cchi + (int64_t) io * Nmax * ncol + ic;                                  \
   const R *ce = ceta + (int64_t) io * Nmax * ncol + ic;                                  \
@@ -118,6 +119,8 @@ __device__ __forceinline__ R rmax0(R x) { return x > R(0) ? x : R(0); }
   double *out = stokes + (int64_t) ic * 4 * nout + io;
 
 #define Z(kk) zc[(int64_t) (kk) * ncol]
+// |z[kk-1] - z[kk]| (kk >= 1), from the column state
+#define DZ(kk) dzc[(int64_t) (kk) * ncol]
 #define POINT(kk, P) form_point<R>(M, ls, has_line, st0 + (int64_t) (kk) * ncol, fs, mu, moving, \
                                    cc[(int64_t) (kk) * ncol], ce[(int64_t) (kk) * ncol], P)
 
@@ -150,17 +153,17 @@ k_formal_stokes(const DevModel *__restrict__ Mp, const int32_t *__restrict__ nde
       } else {
         Pd = Pn;
         const int k = N - 2;
-        const double dtau0 = 0.5 * imu * (double) (Pu.chi + P0.chi) * fabs(Z(N - 1) - Z(N - 2));
+        const double dtau0 = 0.5 * imu * (double) (Pu.chi + P0.chi) * DZ(N - 1);
         Iv[0] = (R) (B2[1] - (B2[0] - B2[1]) / dtau0);
         Iv[1] = Iv[2] = Iv[3] = R(0);
-        dsup = (R) (fabs(Z(k) - Z(k + 1)) * imu);
-        const R dsdn = (R) (fabs(Z(k - 1) - Z(k)) * imu);
-        dchi_up = (P0.chi - Pu.chi) / dsup;
+        dsup = (R) (DZ(k + 1) * imu);
+        const R dsdn = (R) (DZ(k) * imu);
+        dchi_up = rdiv(P0.chi - Pu.chi, dsup);
         dchi_c = cent_deriv<R>(dsup, dsdn, Pu.chi, P0.chi, Pd.chi);
         const R c2 = rmax0(P0.chi - (dsup * R(1.0 / 3.0)) * dchi_c);
         const R c1 = rmax0(Pu.chi + (dsup * R(1.0 / 3.0)) * dchi_up);
         dtau_uw = R(0.25) * dsup * (P0.chi + Pu.chi + c1 + c2);
-        const R idt = R(1) / dtau_uw;
+        const R idt = rrcp(dtau_uw);
 #pragma unroll
         for (int n = 0; n < 4; n++) dSu[n] = (P0.S[n] - Pu.S[n]) * idt;
 #pragma unroll
@@ -169,22 +172,20 @@ k_formal_stokes(const DevModel *__restrict__ Mp, const int32_t *__restrict__ nde
       continue;
     }
     const int k = j + 2;
-    const R dsdn = (R) (fabs(Z(k - 1) - Z(k)) * imu);
+    const R dsdn = (R) (DZ(k) * imu);
     R dchi_dn;
     if (j >= 0) {
-      const R dsdn2 = (R) (fabs(Z(k - 2) - Z(k - 1)) * imu);
+      const R dsdn2 = (R) (DZ(k - 1) * imu);
       dchi_dn = cent_deriv<R>(dsdn, dsdn2, P0.chi, Pd.chi, Pn.chi);
     } else {
-      dchi_dn = (Pd.chi - P0.chi) / dsdn;
+      dchi_dn = rdiv(Pd.chi - P0.chi, dsdn);
     }
     const R c2 = rmax0(P0.chi + (dsdn * R(1.0 / 3.0)) * dchi_c);
     const R c1 = rmax0(Pd.chi - (dsdn * R(1.0 / 3.0)) * dchi_dn);
     const R dtau_dw = R(0.25) * dsdn * (P0.chi + Pd.chi + c1 + c2);
-    double al_d, be_d, ga_d, th_d, ep_d;
-    bezier3_coeffs((double) dtau_uw, al_d, be_d, ga_d, th_d, ep_d);
+    const Bezier3<R> bz = bezier3(dtau_uw);
     const R dt03 = dtau_uw * R(1.0 / 3.0);
-    const R alpha = (R) al_d, beta = (R) be_d, gamma = (R) ga_d, theta = (R) th_d,
-            eps = (R) ep_d;
+    const R alpha = bz.alpha, beta = bz.beta, gamma = bz.gamma, theta = bz.theta, eps = bz.eps;
     R dS0[4], dq0[3];
     const CentDeriv<R> cd(dtau_uw, dtau_dw);
 #pragma unroll
@@ -222,7 +223,7 @@ k_formal_stokes(const DevModel *__restrict__ Mp, const int32_t *__restrict__ nde
   }
   // last interval: linear, rh/rhf1d/bezier_1D.c:264-299 (P0 = depth 0)
   double w[3];
-  const double dtl = 0.5 * imu * (double) (P0.chi + Pu.chi) * fabs(Z(0) - Z(1));
+  const double dtl = 0.5 * imu * (double) (P0.chi + Pu.chi) * DZ(1);
   w3(dtl, w);
   const R w0 = (R) w[0], w1 = (R) w[1], w1t = (R) (w[1] / dtl);
   // V0 = w0 S0 - w1 dSu + ((1 - w0) I - w1/dtau Ku) Iv;  Md = I + (w0 - w1/dtau) K0
@@ -233,7 +234,7 @@ k_formal_stokes(const DevModel *__restrict__ Mp, const int32_t *__restrict__ nde
   const R e = w0 - w1t;
   const R q0V = P0.q[0] * V0[1] + P0.q[1] * V0[2] + P0.q[2] * V0[3];
   const R qq = P0.q[0] * P0.q[0] + P0.q[1] * P0.q[1] + P0.q[2] * P0.q[2];
-  x[0] = (V0[0] - e * q0V) / (R(1) - e * e * qq);
+  x[0] = rdiv(V0[0] - e * q0V, R(1) - e * e * qq);
 #pragma unroll
   for (int n = 0; n < 3; n++) x[n + 1] = V0[n + 1] - e * P0.q[n] * x[0];
 #pragma unroll
@@ -272,39 +273,38 @@ k_formal_scalar(const DevModel *__restrict__ Mp, const int32_t *__restrict__ nde
         } else {
           Pd = Pn;
           const int k = N - 2;
-          const double dtau0 = 0.5 * zmu * (double) (Pu.chi + P0.chi) * fabs(Z(N - 1) - Z(N - 2));
+          const double dtau0 = 0.5 * zmu * (double) (Pu.chi + P0.chi) * DZ(N - 1);
           I_upw = (R) (B2[1] - (B2[0] - B2[1]) / dtau0);
-          dsup = (R) (fabs(Z(k) - Z(k + 1)) * zmu);
-          const R dsdn = (R) (fabs(Z(k - 1) - Z(k)) * zmu);
-          dchi_up = (P0.chi - Pu.chi) / dsup;
+          dsup = (R) (DZ(k + 1) * zmu);
+          const R dsdn = (R) (DZ(k) * zmu);
+          dchi_up = rdiv(P0.chi - Pu.chi, dsup);
           dchi_c = cent_deriv<R>(dsup, dsdn, Pu.chi, P0.chi, Pd.chi);
           const R c1 = rmax0(P0.chi - (dsup * R(1.0 / 3.0)) * dchi_c);
           const R c2 = rmax0(Pu.chi + (dsup * R(1.0 / 3.0)) * dchi_up);
           dtau_uw = dsup * (P0.chi + Pu.chi + c1 + c2) * R(0.25);
-          dS_up = (P0.S[0] - Pu.S[0]) / dtau_uw;
+          dS_up = rdiv(P0.S[0] - Pu.S[0], dtau_uw);
         }
         continue;
       }
       const int k = j + 2;
-      const R dsdn = (R) (fabs(Z(k - 1) - Z(k)) * zmu);
+      const R dsdn = (R) (DZ(k) * zmu);
       R dchi_dn;
       if (j >= 0) {
-        const R dsdn2 = (R) (fabs(Z(k - 2) - Z(k - 1)) * zmu);
+        const R dsdn2 = (R) (DZ(k - 1) * zmu);
         dchi_dn = cent_deriv<R>(dsdn, dsdn2, P0.chi, Pd.chi, Pn.chi);
       } else {
-        dchi_dn = (Pd.chi - P0.chi) / dsdn;
+        dchi_dn = rdiv(Pd.chi - P0.chi, dsdn);
       }
       R c1 = rmax0(P0.chi + (dsdn * R(1.0 / 3.0)) * dchi_c);
       R c2 = rmax0(Pd.chi - (dsdn * R(1.0 / 3.0)) * dchi_dn);
       const R dtau_dw = dsdn * (P0.chi + Pd.chi + c1 + c2) * R(0.25);
-      double al_d, be_d, ga_d, th_d, ep_d;
-      bezier3_coeffs((double) dtau_uw, al_d, be_d, ga_d, th_d, ep_d);
+      const Bezier3<R> bz = bezier3(dtau_uw);
       const R dt03 = dtau_uw * R(1.0 / 3.0);
       const R dS_c = cent_deriv<R>(dtau_uw, dtau_dw, Pu.S[0], P0.S[0], Pd.S[0]);
       c1 = rmax0(P0.S[0] - dt03 * dS_c);
       c2 = rmax0(Pu.S[0] + dt03 * dS_up);
-      I_upw = I_upw * (R) ep_d + (R) al_d * P0.S[0] + (R) be_d * Pu.S[0] + (R) ga_d * c1 +
-              (R) th_d * c2;
+      I_upw = I_upw * bz.eps + bz.alpha * P0.S[0] + bz.beta * Pu.S[0] + bz.gamma * c1 +
+              bz.theta * c2;
       Pu = P0;
       P0 = Pd;
       if (j >= 0) Pd = Pn;
@@ -315,7 +315,7 @@ k_formal_scalar(const DevModel *__restrict__ Mp, const int32_t *__restrict__ nde
       dS_up = dS_c;
     }
     double w[3];
-    const double dtl = 0.5 * zmu * (double) (P0.chi + Pu.chi) * fabs(Z(0) - Z(1));
+    const double dtl = 0.5 * zmu * (double) (P0.chi + Pu.chi) * DZ(1);
     const double dS_uw = -(double) (P0.S[0] - Pu.S[0]) / dtl;
     w3(dtl, w);
     out[0] = (1.0 - w[0]) * (double) I_upw + w[0] * (double) P0.S[0] + w[1] * dS_uw;
@@ -368,6 +368,7 @@ k_formal_scalar(const DevModel *__restrict__ Mp, const int32_t *__restrict__ nde
 }
 #undef POINT
 #undef Z
+#undef DZ
 #undef RAY_SETUP
 
 #define RHG_KFORMAL(KNAME, R)                                                                 \

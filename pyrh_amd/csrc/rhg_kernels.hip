@@ -637,6 +637,10 @@ __global__ void k_geometry(const DevModel *__restrict__ Mp, const int32_t *__res
   } else {
     for (int k = 0; k < N; k++) H(k) = SCALE(k);
   }
+  // geometric path lengths of the depth intervals
+  double *dz = st0 + (int64_t) RHG_CF_DZ * fs;
+  dz[0] = 0.0;
+  for (int k = 1; k < N; k++) dz[(int64_t) k * ncol] = fabs(H(k - 1) - H(k));
 #undef H
 #undef SCALE
 #undef NH
