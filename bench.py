@@ -10,9 +10,12 @@ Prints one JSON line: columns/s of the FP32 form (the plain FP64 form beside
 it), ms per step, the roofline of the dominant kernel (k_formal_stokes:
 algorithmic FLOPs from the library's work census over its HIP-event launch
 time on the library's stream), and a CPU baseline timed on this host on a
-bounded sample of the same columns.
+bounded sample of the same columns.  The response-function leg (object
+response_functions) times the adjoint of a 1024-column batch in the FP32
+form and in FP64 against central differences of the plain FP64 synthesis
+(2 x 6 x 82 perturbed syntheses per column) and reports the errors.
 
-    python bench.py [--steps K] [--warmup W] [--ncol N] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--ncol N] [--no-cpu] [--no-rf]
 """
 from __future__ import annotations
 
@@ -34,6 +37,10 @@ RUN_DIR = ROOT / "tests" / "golden" / "run_lte"
 PROFILES = ROOT / "profiles"
 METRIC = ("columns/s Stokes+RF, FP32 adjoint vs plain FP64 GPU at ≤1e-4 rel err; CPU ref")
 WORKLOAD = "lte_stokes_map_16384"
+RF_WORKLOAD = "lte_stokes_rf_1024"
+# central-difference steps of the finite-difference response functions, in
+# the atmosphere's units (K, km/s, G, rad, rad, km/s)
+RF_STEP = {"T": 1.0, "vlos": 0.01, "B": 1.0, "gamma": 0.01, "chi": 0.01, "vmic": 0.01}
 
 
 def _traffic_from_profile(ncol):
@@ -76,6 +83,112 @@ def _cpu_baseline(setup, wave, atm, nworkers):
                       f"{dt:.1f} s wall"}
 
 
+def _err_stats(x, ref, thr):
+    """Error of x against ref per parameter, relative to each (parameter,
+    depth, column)'s largest |ref| over Stokes and wavelength; arrays in the
+    library's layout [param, depth, 4, nout, ncol].  Median and 99th
+    percentile (from a strided sample) and the exact share above thr."""
+    import torch
+    from pyrh_amd.engine import RF_PARAMS
+    out = {}
+    for ip, name in enumerate(RF_PARAMS):
+        r = ref[ip]
+        scale = r.abs().amax(dim=(1, 2), keepdim=True)
+        e = (x[ip].double() - r).abs() / scale.clamp_min(1e-300)
+        e = e.masked_select((scale > 0).expand_as(e))
+        smp = e[:: max(1, e.numel() // 8_000_000)]
+        q = torch.quantile(smp, torch.tensor([0.5, 0.99], dtype=smp.dtype, device=smp.device))
+        out[name] = {"p50": float(f"{q[0].item():.3g}"), "p99": float(f"{q[1].item():.3g}"),
+                     f"share_gt_{thr:g}": float(f"{(e > thr).double().mean().item():.3g}")}
+    return out
+
+
+def _rf_leg(eng, W, torch, dev, reps):
+    """Response functions of the 1024-column batch: the adjoint (FP32 form
+    and FP64) against central differences of the plain FP64 synthesis on the
+    GPU, 2 x 6 x ndep perturbed syntheses per column, batched 16 depths x
+    1024 columns per launch."""
+    from pyrh_amd.engine import RF_PARAMS, RF_ROWS, RF_NPARAM
+    atm = W.rf_batch()
+    ncol, nd = atm.shape[0], atm.shape[2]
+    nout = eng.nout
+    d_atm = torch.from_numpy(atm).to(dev)
+    d_ndep = torch.full((ncol,), nd, dtype=torch.int32, device=dev)
+    d_st = {p: torch.empty((ncol, 4, nout), dtype=torch.float64, device=dev) for p in (0, 1)}
+    d_rf = {p: torch.empty((RF_NPARAM, nd, 4, nout, ncol),
+                           dtype=torch.float32 if p else torch.float64, device=dev) for p in (0, 1)}
+    torch.cuda.synchronize()
+
+    def adjoint(p):
+        eng.rf_lte_device(d_atm.data_ptr(), d_ndep.data_ptr(), ncol, nd, d_st[p].data_ptr(),
+                          d_rf[p].data_ptr(), precision=p)
+
+    t_adj, stages = {}, {}
+    for p in (1, 0):
+        adjoint(p)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            adjoint(p)
+        t_adj[p] = (time.perf_counter() - t0) / reps
+        stages[p] = {k: round(float(v), 3) for k, v in eng.rf_timings().items()}
+
+    blk = 16
+    d_pert = torch.empty((blk * ncol, 9, nd), dtype=torch.float64, device=dev)
+    d_pndep = torch.full((blk * ncol,), nd, dtype=torch.int32, device=dev)
+    d_po = [torch.empty((blk * ncol, 4, nout), dtype=torch.float64, device=dev) for _ in (0, 1)]
+    fd = torch.empty((RF_NPARAM, nd, 4, nout, ncol), dtype=torch.float64, device=dev)
+    nsynth = 0
+
+    def finite_differences():
+        nonlocal nsynth
+        nsynth = 0
+        for ip, name in enumerate(RF_PARAMS):
+            h, row = RF_STEP[name], RF_ROWS[ip]
+            for k0 in range(0, nd, blk):
+                ks = range(k0, min(nd, k0 + blk))
+                nb = len(ks) * ncol
+                for sg in (0, 1):
+                    v = d_pert[:nb].view(len(ks), ncol, 9, nd)
+                    v.copy_(d_atm.expand(len(ks), ncol, 9, nd))
+                    for j, k in enumerate(ks):
+                        v[j, :, row, k] += h if sg == 0 else -h
+                    torch.cuda.current_stream().synchronize()
+                    eng.synth_lte_device(d_pert.data_ptr(), d_pndep.data_ptr(), nb, nd,
+                                         d_po[sg].data_ptr(), precision=0)
+                    nsynth += len(ks)
+                q = (d_po[0][:nb] - d_po[1][:nb]) / (2.0 * h)
+                fd[ip, k0:k0 + len(ks)] = q.view(len(ks), ncol, 4, nout).permute(0, 2, 3, 1)
+        torch.cuda.synchronize()
+
+    t0 = time.perf_counter()
+    finite_differences()
+    t_fd = time.perf_counter() - t0
+
+    I64 = d_st[0][:, 0]
+    i_err = float(((d_st[1][:, 0] - I64).abs() / I64).max().item())
+    stokes_err = float(((d_st[1] - d_st[0]).abs() / I64[:, :1, None]).max().item())
+    res = {
+        "workload": RF_WORKLOAD, "ncol": ncol, "ndep": nd, "nlambda": nout,
+        "params": list(RF_PARAMS), "mu": 1.0,
+        "adjoint_fp32": {"ms": round(t_adj[1] * 1e3, 2),
+                         "columns_per_s": round(ncol / t_adj[1], 1), "stage_ms": stages[1]},
+        "adjoint_fp64": {"ms": round(t_adj[0] * 1e3, 2),
+                         "columns_per_s": round(ncol / t_adj[0], 1), "stage_ms": stages[0]},
+        "fd_fp64": {"ms": round(t_fd * 1e3, 2), "columns_per_s": round(ncol / t_fd, 1),
+                    "syntheses_per_column": nsynth, "steps": RF_STEP},
+        "speedup_fp32_adjoint_vs_fd_fp64": round(t_fd / t_adj[1], 2),
+        "speedup_fp64_adjoint_vs_fd_fp64": round(t_fd / t_adj[0], 2),
+        "fp32_I_rel_err_max": float(f"{i_err:.3g}"),
+        "fp32_stokes_err_over_Ic_max": float(f"{stokes_err:.3g}"),
+        "rf_err_fp32_vs_fp64_adjoint": _err_stats(d_rf[1], d_rf[0], 1e-4),
+        "rf_err_fp64_adjoint_vs_fd": _err_stats(d_rf[0], fd, 1e-4),
+        "rf_err_fp32_adjoint_vs_fd": _err_stats(d_rf[1], fd, 1e-4),
+    }
+    del d_rf, fd, d_pert, d_po
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +196,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--ncol", type=int, default=16384)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-rf", action="store_true", help="skip the response-function leg")
     ap.add_argument("--profile-step", action="store_true",
                     help="one FP32-form synthesis and one 1 GiB device copy, for rocprofv3 --pmc")
     a = ap.parse_args()
@@ -146,6 +260,8 @@ def main():
     achieved = fl["formal_kernel"] / (formal_ms * 1e-3) / 1e12
     ach64 = fl["formal_kernel"] / (stages64["formal"] * 1e-3) / 1e12
 
+    rf = None if a.no_rf else _rf_leg(eng, W, torch, dev, reps=3)
+
     # host-buffer rate (PCIe both ways), reported beside the resident rate
     t1 = time.perf_counter()
     eng.synth_lte(atm, ndep=ndep, precision=1)
@@ -174,7 +290,7 @@ def main():
                    "lines": "Fe I 630.15 + 630.25 nm, Zeeman",
                    "form": "FP32 form: float line opacity and DELO-Bezier recursion; FP64 "
                            "equilibria, heights, path lengths and Bezier weights",
-                   "response_functions": "not built"},
+                   "response_functions": RF_WORKLOAD + " (object response_functions)"},
         "roofline": {"bound": "valu", "kernel": "k_formal_stokes<float>",
                      "achieved": round(achieved, 3), "peak": R.PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / R.PEAK_FP32_TFLOPS, 4),
@@ -188,6 +304,7 @@ def main():
                                     "achieved": round(ach64, 3), "peak": R.PEAK_FP64_TFLOPS,
                                     "unit": "TFLOP/s",
                                     "frac": round(ach64 / R.PEAK_FP64_TFLOPS, 4)}},
+        "response_functions": rf,
         "pipeline_flops": fl["total"],
         "host_buffers_columns_per_s": round(host_rate, 1),
     }

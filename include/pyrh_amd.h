@@ -221,6 +221,29 @@ enum rhg_precision { RHG_PREC_FP64 = 0, RHG_PREC_FP32 = 1 };
 int rhg_synth_lte(rhg_context_t *ctx, const rhg_columns_t *cols,
                   int32_t precision, int32_t device_ptrs, double *stokes_out);
 
+/* Response functions of the emergent Stokes vector: dS(lambda)/dp(k) for
+ * the atmosphere parameters below at every depth k, for the wavelengths
+ * solved with the polarized DELO-Bezier solver (zero at the others).  Units
+ * are those of the atmosphere rows (K, km/s, G, rad, rad, km/s).  Same
+ * columns and Stokes output as rhg_synth_lte; rf_out holds
+ *   rf[((((p * ndep_max + k) * 4 + s) * (nspect-1) + l) * ncol + c)]
+ * in float for RHG_PREC_FP32 and double for RHG_PREC_FP64 (device memory if
+ * device_ptrs).  At most RHG_RF_MAX_COLUMNS columns per call.  The reference
+ * computes only d I / d log gf (rh/kurucz.c:224-260); these have no
+ * counterpart there. */
+enum rhg_rf_param {
+  RHG_RF_T = 0,    /* temperature, atmosphere row 1          */
+  RHG_RF_VLOS,     /* line-of-sight velocity, row 3          */
+  RHG_RF_B,        /* field strength, row 5                  */
+  RHG_RF_GAMMA,    /* field inclination, row 6               */
+  RHG_RF_CHI,      /* field azimuth, row 7                   */
+  RHG_RF_VMIC,     /* microturbulence, row 4                 */
+  RHG_RF_NPARAM
+};
+#define RHG_RF_MAX_COLUMNS 4096
+int rhg_rf_lte(rhg_context_t *ctx, const rhg_columns_t *cols, int32_t precision,
+               int32_t device_ptrs, double *stokes_out, void *rf_out);
+
 /* Same as rhg_synth_lte but also returns, per column, the depth scale and
  * the background opacity at the reference wavelength (for testing):
  * height_out/chiref_out are [ncol][ndep_max] float64 host buffers (NULL to

@@ -58,6 +58,9 @@ struct rhg_context {
   int n_pol = 0, n_scal = 0;
   const int32_t *d_io_pol = nullptr, *d_io_scal = nullptr;
   DevBuf atmos, ndep, state, moving, cchi, ceta, stokes, chiref, census, census_sum;
+  // response functions: perturbed inputs/states, tangents, tape, adjoints
+  DevBuf rf_atm[2], rf_state[2], rf_cont[4], rf_chiref[2], rf_moving, rf_dstate, rf_dcont[2],
+      rf_dchiref, rf_pts, rf_car, rf_aP, rf_aZ, rf_aB, rf_out;
   hipEvent_t ev[RHG_NSTAGE + 2]{};
   float last_ms[RHG_NSTAGE + 1]{};
   // the last batch, for rhg_count_work
@@ -499,6 +502,235 @@ int rhg_count_work(rhg_context_t *c, int64_t *counts_out) {
   int64_t cd = 0;
   for (int ic = 0; ic < ncol; ic++) cd += c->last_hndep[ic];
   counts_out[RHG_WORK_COLDEPTH] = cd;
+  g_err.clear();
+  return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+// k_state + k_geometry + k_continuum<R> on one batch (no events, no copies)
+template <typename R>
+int launch_base(rhg_context_t *c, const double *d_atm, const int32_t *d_ndep, int ncol, int Nmax,
+                double mu, int atm_scale, double *state, int nfield, int32_t *moving,
+                double *chiref, R *cchi, R *ceta) {
+  const int64_t npts = (int64_t) ncol * Nmax;
+  {
+    dim3 blk(64), grd((unsigned) ((npts + 63) / 64));
+    const size_t lds = (size_t) 3 * c->dm.nmol * 64 * sizeof(double);
+    switch (c->dm.nnuclei) {
+#define RHG_LAUNCH_STATE(NN)                                                                   \
+  case NN:                                                                                     \
+    hipLaunchKernelGGL(rhg::k_state<NN>, grd, blk, lds, c->stream, c->d_model, d_atm, d_ndep, ncol, \
+                       Nmax, mu, atm_scale, state, nfield);                                    \
+    break;
+      RHG_LAUNCH_STATE(1) RHG_LAUNCH_STATE(2) RHG_LAUNCH_STATE(3) RHG_LAUNCH_STATE(4)
+      RHG_LAUNCH_STATE(5) RHG_LAUNCH_STATE(6) RHG_LAUNCH_STATE(7) RHG_LAUNCH_STATE(8)
+#undef RHG_LAUNCH_STATE
+      default: return fail(RHG_EUNSUPPORTED, "number of molecule nuclei out of range");
+    }
+    HIPCHECK(hipGetLastError());
+  }
+  {
+    dim3 blk(64), grd((unsigned) ((ncol + 63) / 64));
+    hipLaunchKernelGGL(rhg::k_geometry, grd, blk, 0, c->stream, c->d_model, d_ndep, ncol, Nmax,
+                       atm_scale, mu, state, nfield, moving, chiref);
+    HIPCHECK(hipGetLastError());
+  }
+  if (cchi) {
+    dim3 blk(rhg::CONT_BLOCK), grd((unsigned) ((npts + rhg::CONT_BLOCK - 1) / rhg::CONT_BLOCK));
+    hipLaunchKernelGGL(rhg::k_continuum<R>, grd, blk, 0, c->stream, c->d_model, d_ndep, ncol, Nmax,
+                       state, nfield, c->d_out_ls, c->nout, cchi, ceta);
+    HIPCHECK(hipGetLastError());
+  }
+  return 0;
+}
+
+// the state tangent steps (atmosphere row, step in the row's units)
+struct RfParam { int row; double h; };
+const RfParam kRfParam[RHG_RF_NPARAM] = {
+    {1, 1.0e-2},   // T [K]
+    {3, 1.0e-4},   // v_LOS [km/s]
+    {5, 1.0e-1},   // B [G]
+    {6, 1.0e-6},   // gamma [rad]
+    {7, 1.0e-6},   // chi [rad]
+    {4, 1.0e-4},   // v_mic [km/s]
+};
+
+template <typename R>
+int rf_run(rhg_context_t *c, const rhg_columns_t *cols, int32_t device_ptrs, double *stokes_out,
+           void *rf_out) {
+  const int ncol = cols->ncol, Nmax = cols->ndep_max, nout = c->nout;
+  const int nfield = rhg::state_nfield(c->ncont, c->nline);
+  const int64_t npts = (int64_t) ncol * Nmax;
+  const size_t atm_b = (size_t) ncol * 9 * Nmax * sizeof(double);
+  const size_t st_b = (size_t) npts * nfield * sizeof(double);
+  const size_t cc_b = (size_t) npts * nout * sizeof(R);
+  const size_t cc64_b = (size_t) npts * nout * sizeof(double);
+  const size_t out_b = (size_t) ncol * 4 * nout * sizeof(double);
+  const size_t rf_b = (size_t) RHG_RF_NPARAM * Nmax * 4 * nout * ncol * sizeof(R);
+  const int npol = c->n_pol;
+  const size_t pts_b = (size_t) (npol > 0 ? npol : 1) * Nmax * 8 * ncol * sizeof(R);
+  const size_t car_b = (size_t) (npol > 0 ? npol : 1) * Nmax * rhg::NCARRY * ncol * sizeof(R);
+  const size_t aP_b = (size_t) (npol > 0 ? npol : 1) * 4 * Nmax * 8 * ncol * sizeof(R);
+  const size_t aZ_b = (size_t) (npol > 0 ? npol : 1) * 4 * Nmax * ncol * sizeof(R);
+  const size_t aB_b = (size_t) (npol > 0 ? npol : 1) * 4 * 2 * ncol * sizeof(R);
+  if (c->state.ensure(st_b) || c->moving.ensure(ncol * sizeof(int32_t)) || c->cchi.ensure(cc_b) ||
+      c->ceta.ensure(cc_b) || c->chiref.ensure((size_t) npts * sizeof(double)) ||
+      (!device_ptrs && (c->atmos.ensure(atm_b) || c->ndep.ensure(ncol * sizeof(int32_t)) ||
+                        c->stokes.ensure(out_b) || c->rf_out.ensure(rf_b))) ||
+      c->rf_atm[0].ensure(atm_b) || c->rf_atm[1].ensure(atm_b) || c->rf_state[0].ensure(st_b) ||
+      c->rf_state[1].ensure(st_b) || c->rf_cont[0].ensure(cc64_b) || c->rf_cont[1].ensure(cc64_b) ||
+      c->rf_cont[2].ensure(cc64_b) || c->rf_cont[3].ensure(cc64_b) ||
+      c->rf_chiref[0].ensure((size_t) npts * sizeof(double)) ||
+      c->rf_chiref[1].ensure((size_t) npts * sizeof(double)) ||
+      c->rf_moving.ensure(ncol * sizeof(int32_t)) ||
+      c->rf_dstate.ensure(st_b * RHG_RF_NPARAM) || c->rf_dcont[0].ensure(cc_b) ||
+      c->rf_dcont[1].ensure(cc_b) || c->rf_dchiref.ensure((size_t) npts * sizeof(double)) ||
+      c->rf_pts.ensure(pts_b) || c->rf_car.ensure(car_b) || c->rf_aP.ensure(aP_b) ||
+      c->rf_aZ.ensure(aZ_b) || c->rf_aB.ensure(aB_b))
+    return fail(RHG_ENOMEM, "device allocation for the response functions failed");
+  const double *d_atm = (const double *) cols->atmos;
+  const int32_t *d_ndep = cols->ndep;
+  double *d_out = stokes_out;
+  R *d_rf = (R *) rf_out;
+  if (!device_ptrs) {
+    HIPCHECK(hipMemcpyAsync(c->atmos.p, cols->atmos, atm_b, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(c->ndep.p, cols->ndep, ncol * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    d_atm = (const double *) c->atmos.p;
+    d_ndep = (const int32_t *) c->ndep.p;
+    d_out = (double *) c->stokes.p;
+    d_rf = (R *) c->rf_out.p;
+  }
+  double *state = (double *) c->state.p;
+  int32_t *moving = (int32_t *) c->moving.p;
+  R *cchi = (R *) c->cchi.p, *ceta = (R *) c->ceta.p;
+  double *chiref = (double *) c->chiref.p;
+  const double mu = cols->mu;
+  const int scale = cols->atm_scale;
+  HIPCHECK(hipEventRecord(c->ev[0], c->stream));
+  int rc = launch_base<R>(c, d_atm, d_ndep, ncol, Nmax, mu, scale, state, nfield, moving, chiref,
+                          cchi, ceta);
+  if (rc) return rc;
+  HIPCHECK(hipEventRecord(c->ev[1], c->stream));
+
+  // 1. state tangents by central differences (every depth at once)
+  const int64_t natm = (int64_t) ncol * 9 * Nmax;
+  double *dstate = (double *) c->rf_dstate.p;
+  for (int p = 0; p < RHG_RF_NPARAM; p++) {
+    const RfParam &P = kRfParam[p];
+    for (int sgn = 0; sgn < 2; sgn++) {
+      dim3 blk(256), grd((unsigned) ((natm + 255) / 256));
+      hipLaunchKernelGGL(rhg::k_rf_perturb, grd, blk, 0, c->stream, d_atm, (double *) c->rf_atm[sgn].p,
+                         natm, Nmax, P.row, sgn == 0 ? P.h : -P.h);
+      HIPCHECK(hipGetLastError());
+      const bool pT = p == RHG_RF_T;
+      rc = launch_base<double>(c, (const double *) c->rf_atm[sgn].p, d_ndep, ncol, Nmax, mu, scale,
+                               (double *) c->rf_state[sgn].p, nfield, (int32_t *) c->rf_moving.p,
+                               pT ? (double *) c->rf_chiref[sgn].p : nullptr,
+                               pT ? (double *) c->rf_cont[2 * sgn].p : nullptr,
+                               pT ? (double *) c->rf_cont[2 * sgn + 1].p : nullptr);
+      if (rc) return rc;
+    }
+    const double inv2h = 0.5 / P.h;
+    const int64_t nst = npts * nfield;
+    {
+      dim3 blk(256), grd((unsigned) ((nst + 255) / 256));
+      hipLaunchKernelGGL(rhg::k_rf_diff<double>, grd, blk, 0, c->stream,
+                         (const double *) c->rf_state[0].p, (const double *) c->rf_state[1].p, nst,
+                         inv2h, dstate + p * nst);
+      HIPCHECK(hipGetLastError());
+    }
+    if (p == RHG_RF_T) {
+      const int64_t ncc = npts * nout;
+      dim3 blk(256), grd((unsigned) ((ncc + 255) / 256));
+      for (int q = 0; q < 2; q++) {
+        hipLaunchKernelGGL(rhg::k_rf_diff<R>, grd, blk, 0, c->stream, (const double *) c->rf_cont[q].p,
+                           (const double *) c->rf_cont[2 + q].p, ncc, inv2h, (R *) c->rf_dcont[q].p);
+        HIPCHECK(hipGetLastError());
+      }
+      dim3 grd2((unsigned) ((npts + 255) / 256));
+      hipLaunchKernelGGL(rhg::k_rf_diff<double>, grd2, blk, 0, c->stream,
+                         (const double *) c->rf_chiref[0].p, (const double *) c->rf_chiref[1].p, npts,
+                         inv2h, (double *) c->rf_dchiref.p);
+      HIPCHECK(hipGetLastError());
+    }
+  }
+  HIPCHECK(hipEventRecord(c->ev[2], c->stream));
+
+  // 2-4. tape, reverse sweep, chain (polarized wavelengths); the others by
+  // the scalar solver with zero response functions
+  HIPCHECK(hipMemsetAsync(d_rf, 0, rf_b, c->stream));
+  const int ncb = (ncol + 63) / 64;
+  dim3 blk(64);
+  if (npol > 0) {
+    dim3 grd((unsigned) rhg::xcd_grid(ncb, npol));
+    R *pts = (R *) c->rf_pts.p, *car = (R *) c->rf_car.p;
+    R *aP = (R *) c->rf_aP.p, *aZ = (R *) c->rf_aZ.p, *aB = (R *) c->rf_aB.p;
+    hipLaunchKernelGGL(rhg::k_rf_forward<R>, grd, blk, 0, c->stream, c->d_model, d_ndep, moving, ncol,
+                       Nmax, state, c->d_out_ls, c->d_io_pol, npol, nout, cchi, ceta, mu, d_out, pts,
+                       car);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipEventRecord(c->ev[3], c->stream));
+    hipLaunchKernelGGL(rhg::k_rf_reverse<R>, grd, blk, 0, c->stream, c->d_model, d_ndep, moving, ncol,
+                       Nmax, state, c->d_out_ls, c->d_io_pol, npol, mu, pts, car, aP, aZ, aB);
+    HIPCHECK(hipGetLastError());
+    hipLaunchKernelGGL(rhg::k_rf_chain<R>, grd, blk, 0, c->stream, c->d_model, d_ndep, moving, ncol,
+                       Nmax, state, dstate, chiref, (const double *) c->rf_dchiref.p, cchi, ceta,
+                       (const R *) c->rf_dcont[0].p, (const R *) c->rf_dcont[1].p, c->d_out_ls,
+                       c->d_io_pol, npol, nout, mu, scale, aP, aZ, aB, d_rf);
+    HIPCHECK(hipGetLastError());
+  } else {
+    HIPCHECK(hipEventRecord(c->ev[3], c->stream));
+  }
+  if (c->n_scal > 0) {
+    dim3 grd((unsigned) rhg::xcd_grid(ncb, c->n_scal));
+    hipLaunchKernelGGL(rhg::k_formal_scalar<R>, grd, blk, 0, c->stream, c->d_model, d_ndep, moving,
+                       ncol, Nmax, state, c->d_out_ls, c->d_io_scal, c->n_scal, nout,
+                       (const R *) cchi, (const R *) ceta, mu, d_out);
+    HIPCHECK(hipGetLastError());
+  }
+  HIPCHECK(hipEventRecord(c->ev[4], c->stream));
+  if (!device_ptrs) {
+    HIPCHECK(hipMemcpyAsync(stokes_out, d_out, out_b, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipMemcpyAsync(rf_out, d_rf, rf_b, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  float ms;
+  for (int st = 0; st < 4; st++) {
+    HIPCHECK(hipEventElapsedTime(&ms, c->ev[st], c->ev[st + 1]));
+    c->last_ms[st] = ms;
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rhg_rf_lte(rhg_context_t *c, const rhg_columns_t *cols, int32_t precision, int32_t device_ptrs,
+               double *stokes_out, void *rf_out) {
+  if (!c || !cols || !stokes_out || !rf_out) return fail(RHG_EINVAL, "null argument");
+  if (precision != RHG_PREC_FP64 && precision != RHG_PREC_FP32)
+    return fail(RHG_EINVAL, "precision must be RHG_PREC_FP64 or RHG_PREC_FP32");
+  const int ncol = cols->ncol, Nmax = cols->ndep_max;
+  if (ncol < 1 || ncol > RHG_RF_MAX_COLUMNS || Nmax < 4 || Nmax > RHG_MAX_DEPTH)
+    return fail(RHG_EINVAL, "bad batch shape (at most RHG_RF_MAX_COLUMNS columns per call)");
+  if (!(cols->mu > 0.0 && cols->mu <= 1.0)) return fail(RHG_EINVAL, "mu must be in (0, 1]");
+  if (cols->atm_scale < 0 || cols->atm_scale > 2) return fail(RHG_EINVAL, "bad atm_scale");
+  std::vector<int32_t> hndep(ncol);
+  if (device_ptrs) {
+    HIPCHECK(hipMemcpy(hndep.data(), cols->ndep, ncol * sizeof(int32_t), hipMemcpyDeviceToHost));
+  } else {
+    memcpy(hndep.data(), cols->ndep, ncol * sizeof(int32_t));
+  }
+  for (int i = 0; i < ncol; i++)
+    if (hndep[i] < 4 || hndep[i] > Nmax) return fail(RHG_EINVAL, "ndep out of range");
+  HIPCHECK(hipSetDevice(c->device));
+  const int rc = precision == RHG_PREC_FP32 ? rf_run<float>(c, cols, device_ptrs, stokes_out, rf_out)
+                                            : rf_run<double>(c, cols, device_ptrs, stokes_out, rf_out);
+  if (rc) return rc;
   g_err.clear();
   return 0;
 }

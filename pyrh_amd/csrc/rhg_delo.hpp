@@ -1,0 +1,217 @@
+// The polarized cubic DELO-Bezier recursion of one emergent ray
+// (rh/rhf1d/bezier_1D.c:52-300, to_obs: depth N-1 up to 0) split into its
+// prologue, one depth step and the last (linear) interval, on an explicit
+// state, for any number type T (double, float, or a dual number of either).
+// The synthesis kernels run it on plain numbers; the response-function
+// kernels run the same code on dual numbers to differentiate each step.
+#pragma once
+
+#include "rhg_kernels.hpp"
+
+namespace rhg {
+
+// Opacity, source functions and reduced propagation-matrix parameters of
+// the emergent ray at one depth (rh/background.c:519-546 sums, formal.c:178-
+// 208 source functions, stokesopac.c:28-88 K' with magneto-optical terms
+// off: K' = [[0, q^T], [q, 0]]).
+template <typename T>
+struct Point {
+  T chi;     // chi_I
+  T S[4];    // eta_n / chi_I
+  T q[3];    // chi_Q,U,V / chi_I
+};
+
+// Point from the continuum (cchi, ceta) and the line opacity at one depth;
+// F(field) reads the column state there.
+template <typename T, typename Field>
+__device__ __forceinline__ void form_point_f(const DevModel &M, int ls, int has_line,
+                                             const Field &F, T mu, int moving, T cchi, T ceta,
+                                             Point<T> &p) {
+  T chi4[4], eta4[4];
+  chi4[0] = cchi; eta4[0] = ceta;
+  chi4[1] = chi4[2] = chi4[3] = T(0);
+  eta4[1] = eta4[2] = eta4[3] = T(0);
+  if (has_line) {
+    T lI, lE, lq[3], le[3];
+    line_point_f<T>(M, ls, F, mu, lI, lq, lE, le, moving);
+    chi4[0] += lI; eta4[0] += lE;
+#pragma unroll
+    for (int n = 0; n < 3; n++) { chi4[n + 1] += lq[n]; eta4[n + 1] += le[n]; }
+  }
+  p.chi = chi4[0];
+  const T ichi = rrcp(chi4[0]);
+#pragma unroll
+  for (int n = 0; n < 4; n++) p.S[n] = eta4[n] * ichi;
+#pragma unroll
+  for (int n = 0; n < 3; n++) p.q[n] = chi4[n + 1] * ichi;
+}
+
+// y = K(q) x for K(q) = [[0, q^T], [q, 0]]
+template <typename T>
+__device__ __forceinline__ void kmul(const T *q, const T *x, T *y) {
+  y[0] = q[0] * x[1] + q[1] * x[2] + q[2] * x[3];
+  y[1] = q[0] * x[0];
+  y[2] = q[1] * x[0];
+  y[3] = q[2] * x[0];
+}
+
+// Solve (I + c K(q) + g (K(q)^2 + K(dq))) x = V.  The matrix is
+// [[a, b^T], [b, I + g q q^T]] with a = 1 + g |q|^2 and b = c q + g dq; the
+// lower block is inverted by Sherman-Morrison and the first component
+// follows from its Schur complement.
+template <typename T>
+__device__ __forceinline__ void solve_md(T c, T g, const T *q, const T *dq, const T *V, T *x) {
+  const T qq = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+  const T a = T(1) + g * qq;
+  T b[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) b[i] = c * q[i] + g * dq[i];
+  const T ga = rdiv(g, a);
+  const T qb = q[0] * b[0] + q[1] * b[1] + q[2] * b[2];
+  const T bb = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+  const T qV = q[0] * V[1] + q[1] * V[2] + q[2] * V[3];
+  const T bV = b[0] * V[1] + b[1] * V[2] + b[2] * V[3];
+  const T x0 = rdiv(V[0] - bV + ga * qV * qb, a - bb + ga * qb * qb);
+  T y[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) y[i] = V[i + 1] - b[i] * x0;
+  const T qy = ga * (q[0] * y[0] + q[1] * y[1] + q[2] * y[2]);
+  x[0] = x0;
+#pragma unroll
+  for (int i = 0; i < 3; i++) x[i + 1] = y[i] - qy * q[i];
+}
+
+template <typename T>
+__device__ __forceinline__ T rmax0(T x) { return x > T(0) ? x : T(0); }
+
+// State carried from one depth step to the next (the step at depth k reads
+// the upwind point k+1, the point k and the downwind point k-1).
+template <typename T>
+struct DeloState {
+  T Iv[4];                  // Stokes vector at the upwind depth
+  Point<T> Pu, P0, Pd;      // points at k+1, k, k-1
+  T dSu[4], dqu[3];         // derivatives of S and q at the upwind point
+  T dtau_uw, dsup, dchi_up, dchi_c;
+};
+
+// Set-up from the three deepest points and the thermalised lower boundary
+// I = B[N-1] - (B[N-2] - B[N-1]) / dtau (bezier_1D.c:93-101).  On entry
+// s.Pu and s.P0 hold the points at N-1 and N-2; PNm3 is the point at N-3.
+// dz1 = |z[N-2] - z[N-1]|, dz2 = |z[N-3] - z[N-2]|; B2 = (B[N-2], B[N-1]).
+template <typename T>
+__device__ __forceinline__ void delo_prologue(DeloState<T> &s, const Point<T> &PNm3, T dz1,
+                                              T dz2, T imu, T B2a, T B2b) {
+  s.Pd = PNm3;
+  const T dtau0 = T(0.5) * imu * (s.Pu.chi + s.P0.chi) * dz1;
+  s.Iv[0] = B2b - rdiv(B2a - B2b, dtau0);
+  s.Iv[1] = s.Iv[2] = s.Iv[3] = T(0);
+  s.dsup = dz1 * imu;
+  const T dsdn = dz2 * imu;
+  s.dchi_up = rdiv(s.P0.chi - s.Pu.chi, s.dsup);
+  s.dchi_c = CentDeriv<T>(s.dsup, dsdn)(s.Pu.chi, s.P0.chi, s.Pd.chi);
+  const T c2 = rmax0(s.P0.chi - (s.dsup * T(1.0 / 3.0)) * s.dchi_c);
+  const T c1 = rmax0(s.Pu.chi + (s.dsup * T(1.0 / 3.0)) * s.dchi_up);
+  s.dtau_uw = T(0.25) * s.dsup * (s.P0.chi + s.Pu.chi + c1 + c2);
+  const T idt = rrcp(s.dtau_uw);
+#pragma unroll
+  for (int n = 0; n < 4; n++) s.dSu[n] = (s.P0.S[n] - s.Pu.S[n]) * idt;
+#pragma unroll
+  for (int n = 0; n < 3; n++) s.dqu[n] = (s.P0.q[n] - s.Pu.q[n]) * idt;
+}
+
+// One step at depth k: Iv(k) from Iv(k+1).  Pn is the point at k-2 (has_pn;
+// at k = 1 there is none and the downwind derivative of chi is one-sided).
+// dzk = |z[k-1] - z[k]|, dzk1 = |z[k-2] - z[k-1]|.
+template <typename T>
+__device__ __forceinline__ void delo_step(DeloState<T> &s, const Point<T> &Pn, bool has_pn, T dzk,
+                                          T dzk1, T imu) {
+  const T dsdn = dzk * imu;
+  T dchi_dn;
+  if (has_pn) {
+    const T dsdn2 = dzk1 * imu;
+    dchi_dn = CentDeriv<T>(dsdn, dsdn2)(s.P0.chi, s.Pd.chi, Pn.chi);
+  } else {
+    dchi_dn = rdiv(s.Pd.chi - s.P0.chi, dsdn);
+  }
+  const T c2 = rmax0(s.P0.chi + (dsdn * T(1.0 / 3.0)) * s.dchi_c);
+  const T c1 = rmax0(s.Pd.chi - (dsdn * T(1.0 / 3.0)) * dchi_dn);
+  const T dtau_dw = T(0.25) * dsdn * (s.P0.chi + s.Pd.chi + c1 + c2);
+  const Bezier3<T> bz = bezier3(s.dtau_uw);
+  const T dt03 = s.dtau_uw * T(1.0 / 3.0);
+  const T alpha = bz.alpha, beta = bz.beta, gamma = bz.gamma, theta = bz.theta, eps = bz.eps;
+  T dS0[4], dq0[3];
+  const CentDeriv<T> cd(s.dtau_uw, dtau_dw);
+#pragma unroll
+  for (int i = 0; i < 4; i++) dS0[i] = cd(s.Pu.S[i], s.P0.S[i], s.Pd.S[i]);
+#pragma unroll
+  for (int i = 0; i < 3; i++) dq0[i] = cd(s.Pu.q[i], s.P0.q[i], s.Pd.q[i]);
+  // V0 = Ma Iv + Mb Su + Mc S0 + dt03 (gamma dS0 - theta dSu) with
+  // Ma = eps I - beta Ku + theta (dt03 (Ku^2 + dKu + Ku) - Ku),
+  // Mb = beta I + theta (I - dt03 Ku), Mc = alpha I + gamma (I + dt03 K0)
+  T KI[4], KKI[4], dKI[4], KSu[4], KS0[4], V0[4];
+  kmul(s.Pu.q, s.Iv, KI);
+  kmul(s.Pu.q, KI, KKI);
+  kmul(s.dqu, s.Iv, dKI);
+  kmul(s.Pu.q, s.Pu.S, KSu);
+  kmul(s.P0.q, s.P0.S, KS0);
+  const T ca = theta * (dt03 - T(1)) - beta, gu = theta * dt03;
+  const T cb0 = beta + theta, cc0 = alpha + gamma, g0 = gamma * dt03;
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+    V0[i] = eps * s.Iv[i] + ca * KI[i] + gu * (KKI[i] + dKI[i]) + cb0 * s.Pu.S[i] - gu * KSu[i] +
+            cc0 * s.P0.S[i] + g0 * KS0[i] + dt03 * (gamma * dS0[i] - theta * s.dSu[i]);
+  // Md = I + alpha K0 + gamma (dt03 (K0^2 + dK0 + K0) + K0)
+  solve_md<T>(alpha + gamma * (T(1) + dt03), g0, s.P0.q, dq0, V0, s.Iv);
+  s.Pu = s.P0;
+  s.P0 = s.Pd;
+  if (has_pn) s.Pd = Pn;
+#pragma unroll
+  for (int n = 0; n < 4; n++) s.dSu[n] = dS0[n];
+#pragma unroll
+  for (int n = 0; n < 3; n++) s.dqu[n] = dq0[n];
+  s.dtau_uw = dtau_dw;
+  s.dsup = dsdn;
+  s.dchi_up = s.dchi_c;
+  s.dchi_c = dchi_dn;
+}
+
+// w3 weights of the last interval in double with their derivative in dtau
+// carried by dual numbers
+__device__ __forceinline__ void w3_t(double dt, double *w) { w3(dt, w); }
+__device__ __forceinline__ void w3_t(float dt, float *w) {
+  double wd[3];
+  w3((double) dt, wd);
+  w[0] = (float) wd[0]; w[1] = (float) wd[1]; w[2] = (float) wd[2];
+}
+template <typename R>
+__device__ __forceinline__ void w3_t(Dual<R> dt, Dual<R> *w) {
+  double wd[3], dw[2];
+  w3d((double) dt.v, wd, dw);
+  w[0] = {(R) wd[0], (R) dw[0] * dt.d};
+  w[1] = {(R) wd[1], (R) dw[1] * dt.d};
+  w[2] = {(R) wd[2], R(0)};
+}
+
+// The last (linear) interval, bezier_1D.c:264-299, from the state after the
+// step at k = 1 (P0 = depth 0, Pu = depth 1); dz = |z[0] - z[1]|.
+template <typename T>
+__device__ __forceinline__ void delo_final(const DeloState<T> &s, T dz, T imu, T *x) {
+  T w[3];
+  const T dtl = T(0.5) * imu * (s.P0.chi + s.Pu.chi) * dz;
+  w3_t(dtl, w);
+  const T w0 = w[0], w1 = w[1], w1t = rdiv(w[1], dtl);
+  // V0 = w0 S0 - w1 dSu + ((1 - w0) I - w1/dtau Ku) Iv;  Md = I + (w0 - w1/dtau) K0
+  T KI[4], V0[4];
+  kmul(s.Pu.q, s.Iv, KI);
+#pragma unroll
+  for (int n = 0; n < 4; n++)
+    V0[n] = w0 * s.P0.S[n] - w1 * s.dSu[n] + (T(1) - w0) * s.Iv[n] - w1t * KI[n];
+  const T e = w0 - w1t;
+  const T q0V = s.P0.q[0] * V0[1] + s.P0.q[1] * V0[2] + s.P0.q[2] * V0[3];
+  const T qq = s.P0.q[0] * s.P0.q[0] + s.P0.q[1] * s.P0.q[1] + s.P0.q[2] * s.P0.q[2];
+  x[0] = rdiv(V0[0] - e * q0V, T(1) - e * e * qq);
+#pragma unroll
+  for (int n = 0; n < 3; n++) x[n + 1] = V0[n + 1] - e * s.P0.q[n] * x[0];
+}
+
+}  // namespace rhg

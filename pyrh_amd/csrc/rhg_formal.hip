@@ -11,85 +11,18 @@
 // Precision R: double is the plain form; float is the FP32 form, in which the
 // Bezier and w3 weights (cancellation at small dtau), the geometric path
 // lengths (differences of heights) and the Feautrier recursion stay FP64.
-#include "rhg_kernels.hpp"
+#include "rhg_delo.hpp"
 
 namespace rhg {
 
-namespace {
-
-// Opacity, source functions and reduced propagation-matrix parameters of
-// the emergent ray at one depth (rh/background.c:519-546 sums, formal.c:178-
-// 208 source functions, stokesopac.c:28-88 K' with magneto-optical terms
-// off: K' = [[0, q^T], [q, 0]]).
-template <typename R>
-struct Point {
-  R chi;     // chi_I
-  R S[4];    // eta_n / chi_I
-  R q[3];    // chi_Q,U,V / chi_I
-};
-
+// form_point over the column state at st (stride fs)
 template <typename R>
 __device__ __forceinline__ void form_point(const DevModel &M, int ls, int has_line,
                                            const double *st, int64_t fs, R mu, int moving,
                                            R cchi, R ceta, Point<R> &p) {
-  R chi4[4], eta4[4];
-  chi4[0] = cchi; eta4[0] = ceta;
-  chi4[1] = chi4[2] = chi4[3] = R(0);
-  eta4[1] = eta4[2] = eta4[3] = R(0);
-  if (has_line) {
-    R lI, lE, lq[3], le[3];
-    line_point<R>(M, ls, st, fs, mu, lI, lq, lE, le, moving);
-    chi4[0] += lI; eta4[0] += lE;
-#pragma unroll
-    for (int n = 0; n < 3; n++) { chi4[n + 1] += lq[n]; eta4[n + 1] += le[n]; }
-  }
-  p.chi = chi4[0];
-  const R ichi = rrcp(chi4[0]);
-#pragma unroll
-  for (int n = 0; n < 4; n++) p.S[n] = eta4[n] * ichi;
-#pragma unroll
-  for (int n = 0; n < 3; n++) p.q[n] = chi4[n + 1] * ichi;
+  auto F = [st, fs](int f) { return (R) st[f * fs]; };
+  form_point_f<R>(M, ls, has_line, F, mu, moving, cchi, ceta, p);
 }
-
-// y = K(q) x for K(q) = [[0, q^T], [q, 0]]
-template <typename R>
-__device__ __forceinline__ void kmul(const R *q, const R *x, R *y) {
-  y[0] = q[0] * x[1] + q[1] * x[2] + q[2] * x[3];
-  y[1] = q[0] * x[0];
-  y[2] = q[1] * x[0];
-  y[3] = q[2] * x[0];
-}
-
-// Solve (I + c K(q) + g (K(q)^2 + K(dq))) x = V.  The matrix is
-// [[a, b^T], [b, I + g q q^T]] with a = 1 + g |q|^2 and b = c q + g dq; the
-// lower block is inverted by Sherman-Morrison and the first component
-// follows from its Schur complement.
-template <typename R>
-__device__ __forceinline__ void solve_md(R c, R g, const R *q, const R *dq, const R *V, R *x) {
-  const R qq = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
-  const R a = R(1) + g * qq;
-  R b[3];
-#pragma unroll
-  for (int i = 0; i < 3; i++) b[i] = c * q[i] + g * dq[i];
-  const R ga = rdiv(g, a);
-  const R qb = q[0] * b[0] + q[1] * b[1] + q[2] * b[2];
-  const R bb = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
-  const R qV = q[0] * V[1] + q[1] * V[2] + q[2] * V[3];
-  const R bV = b[0] * V[1] + b[1] * V[2] + b[2] * V[3];
-  const R x0 = rdiv(V[0] - bV + ga * qV * qb, a - bb + ga * qb * qb);
-  R y[3];
-#pragma unroll
-  for (int i = 0; i < 3; i++) y[i] = V[i + 1] - b[i] * x0;
-  const R qy = ga * (q[0] * y[0] + q[1] * y[1] + q[2] * y[2]);
-  x[0] = x0;
-#pragma unroll
-  for (int i = 0; i < 3; i++) x[i + 1] = y[i] - qy * q[i];
-}
-
-template <typename R>
-__device__ __forceinline__ R rmax0(R x) { return x > R(0) ? x : R(0); }
-
-}  // namespace
 
 // Common set-up of one emergent ray: lane -> column, workgroup -> (column
 // block, entry t of the wavelength list io_list).
@@ -139,104 +72,24 @@ k_formal_stokes(const DevModel *__restrict__ Mp, const int32_t *__restrict__ nde
                 const R *__restrict__ cchi, const R *__restrict__ ceta, double mu_d,
                 double *__restrict__ stokes) {
   RAY_SETUP();
-  const double imu = 1.0 / mu_d;
-  Point<R> Pu, P0, Pd, Pn;
-  R dSu[4], dqu[3], Iv[4];
-  R dtau_uw = R(0), dsup = R(0), dchi_up = R(0), dchi_c = R(0);
+  const R imu = (R) (1.0 / mu_d);
+  DeloState<R> st;
+  Point<R> Pn;
   for (int j = N - 1; j >= -1; j--) {
     if (j >= 0) POINT(j, Pn);
-    if (j >= N - 3) {
-      if (j == N - 1) {
-        Pu = Pn;
-      } else if (j == N - 2) {
-        P0 = Pn;
-      } else {
-        Pd = Pn;
-        const int k = N - 2;
-        const double dtau0 = 0.5 * imu * (double) (Pu.chi + P0.chi) * DZ(N - 1);
-        Iv[0] = (R) (B2[1] - (B2[0] - B2[1]) / dtau0);
-        Iv[1] = Iv[2] = Iv[3] = R(0);
-        dsup = (R) (DZ(k + 1) * imu);
-        const R dsdn = (R) (DZ(k) * imu);
-        dchi_up = rdiv(P0.chi - Pu.chi, dsup);
-        dchi_c = cent_deriv<R>(dsup, dsdn, Pu.chi, P0.chi, Pd.chi);
-        const R c2 = rmax0(P0.chi - (dsup * R(1.0 / 3.0)) * dchi_c);
-        const R c1 = rmax0(Pu.chi + (dsup * R(1.0 / 3.0)) * dchi_up);
-        dtau_uw = R(0.25) * dsup * (P0.chi + Pu.chi + c1 + c2);
-        const R idt = rrcp(dtau_uw);
-#pragma unroll
-        for (int n = 0; n < 4; n++) dSu[n] = (P0.S[n] - Pu.S[n]) * idt;
-#pragma unroll
-        for (int n = 0; n < 3; n++) dqu[n] = (P0.q[n] - Pu.q[n]) * idt;
-      }
-      continue;
-    }
-    const int k = j + 2;
-    const R dsdn = (R) (DZ(k) * imu);
-    R dchi_dn;
-    if (j >= 0) {
-      const R dsdn2 = (R) (DZ(k - 1) * imu);
-      dchi_dn = cent_deriv<R>(dsdn, dsdn2, P0.chi, Pd.chi, Pn.chi);
+    if (j == N - 1) {
+      st.Pu = Pn;
+    } else if (j == N - 2) {
+      st.P0 = Pn;
+    } else if (j == N - 3) {
+      delo_prologue<R>(st, Pn, (R) DZ(N - 1), (R) DZ(N - 2), imu, (R) B2[0], (R) B2[1]);
     } else {
-      dchi_dn = rdiv(Pd.chi - P0.chi, dsdn);
+      const int k = j + 2;
+      delo_step<R>(st, Pn, j >= 0, (R) DZ(k), (R) (j >= 0 ? DZ(k - 1) : 0.0), imu);
     }
-    const R c2 = rmax0(P0.chi + (dsdn * R(1.0 / 3.0)) * dchi_c);
-    const R c1 = rmax0(Pd.chi - (dsdn * R(1.0 / 3.0)) * dchi_dn);
-    const R dtau_dw = R(0.25) * dsdn * (P0.chi + Pd.chi + c1 + c2);
-    const Bezier3<R> bz = bezier3(dtau_uw);
-    const R dt03 = dtau_uw * R(1.0 / 3.0);
-    const R alpha = bz.alpha, beta = bz.beta, gamma = bz.gamma, theta = bz.theta, eps = bz.eps;
-    R dS0[4], dq0[3];
-    const CentDeriv<R> cd(dtau_uw, dtau_dw);
-#pragma unroll
-    for (int i = 0; i < 4; i++) dS0[i] = cd(Pu.S[i], P0.S[i], Pd.S[i]);
-#pragma unroll
-    for (int i = 0; i < 3; i++) dq0[i] = cd(Pu.q[i], P0.q[i], Pd.q[i]);
-    // V0 = Ma Iv + Mb Su + Mc S0 + dt03 (gamma dS0 - theta dSu) with
-    // Ma = eps I - beta Ku + theta (dt03 (Ku^2 + dKu + Ku) - Ku),
-    // Mb = beta I + theta (I - dt03 Ku), Mc = alpha I + gamma (I + dt03 K0)
-    R KI[4], KKI[4], dKI[4], KSu[4], KS0[4], V0[4];
-    kmul(Pu.q, Iv, KI);
-    kmul(Pu.q, KI, KKI);
-    kmul(dqu, Iv, dKI);
-    kmul(Pu.q, Pu.S, KSu);
-    kmul(P0.q, P0.S, KS0);
-    const R ca = theta * (dt03 - R(1)) - beta, gu = theta * dt03;
-    const R cb0 = beta + theta, cc0 = alpha + gamma, g0 = gamma * dt03;
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-      V0[i] = eps * Iv[i] + ca * KI[i] + gu * (KKI[i] + dKI[i]) + cb0 * Pu.S[i] - gu * KSu[i] +
-              cc0 * P0.S[i] + g0 * KS0[i] + dt03 * (gamma * dS0[i] - theta * dSu[i]);
-    // Md = I + alpha K0 + gamma (dt03 (K0^2 + dK0 + K0) + K0)
-    solve_md<R>(alpha + gamma * (R(1) + dt03), g0, P0.q, dq0, V0, Iv);
-    Pu = P0;
-    P0 = Pd;
-    if (j >= 0) Pd = Pn;
-#pragma unroll
-    for (int n = 0; n < 4; n++) dSu[n] = dS0[n];
-#pragma unroll
-    for (int n = 0; n < 3; n++) dqu[n] = dq0[n];
-    dtau_uw = dtau_dw;
-    dsup = dsdn;
-    dchi_up = dchi_c;
-    dchi_c = dchi_dn;
   }
-  // last interval: linear, rh/rhf1d/bezier_1D.c:264-299 (P0 = depth 0)
-  double w[3];
-  const double dtl = 0.5 * imu * (double) (P0.chi + Pu.chi) * DZ(1);
-  w3(dtl, w);
-  const R w0 = (R) w[0], w1 = (R) w[1], w1t = (R) (w[1] / dtl);
-  // V0 = w0 S0 - w1 dSu + ((1 - w0) I - w1/dtau Ku) Iv;  Md = I + (w0 - w1/dtau) K0
-  R KI[4], V0[4], x[4];
-  kmul(Pu.q, Iv, KI);
-#pragma unroll
-  for (int n = 0; n < 4; n++) V0[n] = w0 * P0.S[n] - w1 * dSu[n] + (R(1) - w0) * Iv[n] - w1t * KI[n];
-  const R e = w0 - w1t;
-  const R q0V = P0.q[0] * V0[1] + P0.q[1] * V0[2] + P0.q[2] * V0[3];
-  const R qq = P0.q[0] * P0.q[0] + P0.q[1] * P0.q[1] + P0.q[2] * P0.q[2];
-  x[0] = rdiv(V0[0] - e * q0V, R(1) - e * e * qq);
-#pragma unroll
-  for (int n = 0; n < 3; n++) x[n + 1] = V0[n + 1] - e * P0.q[n] * x[0];
+  R x[4];
+  delo_final<R>(st, (R) DZ(1), imu, x);
 #pragma unroll
   for (int i = 0; i < 4; i++) out[(int64_t) i * nout] = (double) x[i];
 }

@@ -3,6 +3,7 @@
 #pragma once
 
 #include "rhg_device.hpp"
+#include "rhg_dual.hpp"
 
 #define RHG_MAX_ATOM 32
 #define RHG_MAX_DEPTH 512
@@ -102,57 +103,58 @@ __host__ __device__ __forceinline__ int state_nfield(int ncont, int nline) {
 }
 
 // Kurucz-line opacity and emissivity of the emergent ray (to_obs = 1) at
-// wavelength ls and the depth that st points at (field stride fs),
-// rh/kurucz.c:600-715 and RLKProfile rh/kurucz.c:729-827, in precision R
-// (the Doppler offset (lambda/lambda0 - 1) c is FP64, from the model).
+// wavelength ls and one depth, rh/kurucz.c:600-715 and RLKProfile
+// rh/kurucz.c:729-827, in the number type T (double, float, or a dual number
+// of either).  F(field) returns a column-state field at that depth as a T.
+// The Doppler offset (lambda/lambda0 - 1) c comes from the model in FP64.
 // chiI/etaI and chiQ[3]/etaQ[3] accumulate from zero.
-template <typename R>
-__device__ __forceinline__ void line_point(const DevModel &M, int ls, const double *st, int64_t fs,
-                                           R mu, R &chiI, R *chiQ, R &etaI, R *etaQ, int moving) {
+template <typename T, typename Field>
+__device__ __forceinline__ void line_point_f(const DevModel &M, int ls, const Field &F, T mu,
+                                             T &chiI, T *chiQ, T &etaI, T *etaQ, int moving) {
   const int p0 = M.lam_line_ptr[ls], p1 = M.lam_line_ptr[ls + 1];
   const int lb = line_field_base(M.ncont);
-  chiI = R(0); etaI = R(0);
-  chiQ[0] = chiQ[1] = chiQ[2] = R(0);
-  etaQ[0] = etaQ[1] = etaQ[2] = R(0);
-  const R vz = (R) st[RHG_CF_VLOS * fs];
+  chiI = T(0); etaI = T(0);
+  chiQ[0] = chiQ[1] = chiQ[2] = T(0);
+  etaQ[0] = etaQ[1] = etaQ[2] = T(0);
+  const T vz = F(RHG_CF_VLOS);
   for (int e = p0; e < p1; e++) {
     const int l = M.lam_line_idx[e];
-    const R chi_l = (R) st[(lb + 4 * l) * fs], eta_l = (R) st[(lb + 4 * l + 1) * fs];
-    const R adamp = (R) st[(lb + 4 * l + 2) * fs], ivb = (R) st[(lb + 4 * l + 3) * fs];
-    R v = (R) M.lam_line_dv[e] * ivb;
+    const T chi_l = F(lb + 4 * l), eta_l = F(lb + 4 * l + 1);
+    const T adamp = F(lb + 4 * l + 2), ivb = F(lb + 4 * l + 3);
+    T v = T(M.lam_line_dv[e]) * ivb;
     if (moving) v += (mu * vz) * ivb;
-    const R sv = ivb * R(INV_SQRTPI);
-    R phi, phi_Q = R(0), phi_U = R(0), phi_V = R(0);
+    const T sv = ivb * T(INV_SQRTPI);
+    T phi, phi_Q = T(0), phi_U = T(0), phi_V = T(0);
     const bool polar = M.line_polar[l] != 0;
     if (M.line_Grad[l] != 0.0) {
       if (polar) {
-        const R cosg = (R) st[RHG_CF_COSG * fs];
-        const R sin2_gamma = R(1.0) - cosg * cosg;
-        const R vB = (R) M.line_larmor[l] * (R) st[RHG_CF_B * fs] * ivb;
-        R phi_sm = R(0), phi_pi = R(0), phi_sp = R(0), H, F;
+        const T cosg = F(RHG_CF_COSG);
+        const T sin2_gamma = T(1.0) - cosg * cosg;
+        const T vB = T(M.line_larmor[l]) * F(RHG_CF_B) * ivb;
+        T phi_sm = T(0), phi_pi = T(0), phi_sp = T(0), H, Fv;
         const int c0 = M.line_comp_ptr[l], c1 = M.line_comp_ptr[l + 1];
         for (int nz = c0; nz < c1; nz++) {
-          H = voigt_humlicek<R>(adamp, v - (R) M.comp_shift[nz] * vB, &F);
+          H = voigt_humlicek<T>(adamp, v - T(M.comp_shift[nz]) * vB, &Fv);
           const int q = M.comp_q[nz];
-          const R sH = (R) M.comp_strength[nz] * H;
+          const T sH = T(M.comp_strength[nz]) * H;
           if (q == -1) phi_sm += sH;
           else if (q == 0) phi_pi += sH;
           else phi_sp += sH;
         }
-        const R phi_sigma = phi_sp + phi_sm;
-        const R phi_delta = R(0.5) * phi_pi - R(0.25) * phi_sigma;
-        phi = (phi_delta * sin2_gamma + R(0.5) * phi_sigma) * sv;
-        phi_Q = phi_delta * sin2_gamma * (R) st[RHG_CF_COS2X * fs] * sv;
-        phi_U = phi_delta * sin2_gamma * (R) st[RHG_CF_SIN2X * fs] * sv;
-        phi_V = R(0.5) * (phi_sp - phi_sm) * cosg * sv;
+        const T phi_sigma = phi_sp + phi_sm;
+        const T phi_delta = T(0.5) * phi_pi - T(0.25) * phi_sigma;
+        phi = (phi_delta * sin2_gamma + T(0.5) * phi_sigma) * sv;
+        phi_Q = phi_delta * sin2_gamma * F(RHG_CF_COS2X) * sv;
+        phi_U = phi_delta * sin2_gamma * F(RHG_CF_SIN2X) * sv;
+        phi_V = T(0.5) * (phi_sp - phi_sm) * cosg * sv;
       } else {
-        phi = (R) voigt_armstrong((double) adamp, (double) v) * sv;
+        phi = voigt_armstrong_t(adamp, v) * sv;
       }
     } else {
-      phi = (fabs(v) <= R(MAX_GAUSS_DOPPLER)) ? rexp(-v * v) : R(0);
+      phi = (fabs(v) <= T(MAX_GAUSS_DOPPLER)) ? rexp(-v * v) : T(0);
       phi = phi * sv;
     }
-    if (phi != R(0)) {
+    if (phi != T(0)) {
       chiI += chi_l * phi;
       etaI += eta_l * phi;
       if (polar && M.line_Grad[l] != 0.0) {
@@ -165,6 +167,14 @@ __device__ __forceinline__ void line_point(const DevModel &M, int ls, const doub
       }
     }
   }
+}
+
+// the same with the fields read from the column state at st (stride fs)
+template <typename R>
+__device__ __forceinline__ void line_point(const DevModel &M, int ls, const double *st, int64_t fs,
+                                           R mu, R &chiI, R *chiQ, R &etaI, R *etaQ, int moving) {
+  auto F = [st, fs](int f) { return (R) st[f * fs]; };
+  line_point_f<R>(M, ls, F, mu, chiI, chiQ, etaI, etaQ, moving);
 }
 
 // Work census of one emergent ray (column, wavelength) over depth indices
@@ -215,6 +225,28 @@ __global__ void k_formal_scalar(const DevModel *M, const int32_t *ndep, const in
                                 int ncol, int Nmax, const double *state, const int32_t *out_ls,
                                 const int32_t *io_list, int nlist, int nout, const R *cchi,
                                 const R *ceta, double mu, double *stokes);
+constexpr int NCARRY = 15;   // carried scalars of the DELO recursion state
+__global__ void k_rf_perturb(const double *atmos, double *out, int64_t n, int Nmax, int row,
+                             double delta);
+template <typename O>
+__global__ void k_rf_diff(const double *a, const double *b, int64_t n, double scale, O *out);
+template <typename R>
+__global__ void k_rf_forward(const DevModel *M, const int32_t *ndep, const int32_t *moving, int ncol,
+                             int Nmax, const double *state, const int32_t *out_ls,
+                             const int32_t *io_list, int nlist, int nout, const R *cchi,
+                             const R *ceta, double mu, double *stokes, R *pts, R *car);
+template <typename R>
+__global__ void k_rf_reverse(const DevModel *M, const int32_t *ndep, const int32_t *moving, int ncol,
+                             int Nmax, const double *state, const int32_t *out_ls,
+                             const int32_t *io_list, int nlist, double mu, const R *pts,
+                             const R *car, R *aP, R *aZ, R *aB);
+template <typename R>
+__global__ void k_rf_chain(const DevModel *M, const int32_t *ndep, const int32_t *moving, int ncol,
+                           int Nmax, const double *state, const double *dstate, const double *chiref,
+                           const double *dchiref, const R *cchi, const R *ceta, const R *dcchi,
+                           const R *dceta, const int32_t *out_ls, const int32_t *io_list, int nlist,
+                           int nout, double mu, int atm_scale, const R *aP, const R *aZ,
+                           const R *aB, R *rf);
 __global__ void k_census(const DevModel *M, const int32_t *ndep, const int32_t *moving, int ncol,
                          int Nmax, const double *state, int nfield, const int32_t *out_ls,
                          int nout, double mu, int32_t *census);

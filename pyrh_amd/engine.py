@@ -20,7 +20,12 @@ _LIB = None
 
 RHG_NSTAGE = 4
 RHG_NWORK = 12
+RF_NPARAM = 6
+RF_PARAMS = ("T", "vlos", "B", "gamma", "chi", "vmic")
+RF_ROWS = (1, 3, 5, 6, 7, 4)        # atmosphere rows of the parameters
 STAGES = ("state", "geometry", "continuum", "formal")
+# the four intervals rhg_last_timings reports after rhg_rf_lte
+RF_STAGES = ("synthesis_inputs", "state_tangents", "tape", "reverse_and_chain")
 
 
 class EngineError(RuntimeError):
@@ -60,6 +65,8 @@ def lib():
         L.rhg_synth_lte_diag.argtypes = [C.c_void_p, C.POINTER(RhgColumns), C.c_int32, PD, PD, PD]
         L.rhg_last_timings.argtypes = [C.c_void_p, PD]
         L.rhg_count_work.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+        L.rhg_rf_lte.argtypes = [C.c_void_p, C.POINTER(RhgColumns), C.c_int32, C.c_int32, PD,
+                                 C.c_void_p]
         if L.rhg_model_sizeof() != C.sizeof(RhgModel) or \
                 L.rhg_columns_sizeof() != C.sizeof(RhgColumns):
             raise EngineError("rhg_model_t / rhg_columns_t layout mismatch between the "
@@ -126,6 +133,40 @@ class Engine:
         s.atm_scale, s.mu = int(atm_scale), float(mu)
         _check(lib().rhg_synth_lte(self._h, C.byref(s), int(precision), 1,
                                    C.cast(C.c_void_p(d_out), C.POINTER(C.c_double))))
+
+    def rf_lte(self, atmos, mu=1.0, atm_scale=0, ndep=None, precision=0):
+        """Stokes [ncol, 4, nout] and response functions
+        [ncol, RF_NPARAM, ndep_max, 4, nout] (d Stokes / d parameter at each
+        depth; parameters T, v_LOS, B, gamma, chi, v_mic in the atmosphere's
+        units) for host atmospheres [ncol, 9, ndep_max]."""
+        cs = ColumnsStruct(atmos, ndep, atm_scale, mu)
+        ncol, nd = cs.struct.ncol, cs.struct.ndep_max
+        out = np.empty((ncol, 4, self.nout))
+        rf = np.empty((RF_NPARAM, nd, 4, self.nout, ncol),
+                      dtype=np.float32 if precision == 1 else np.float64)
+        _check(lib().rhg_rf_lte(self._h, cs.ptr(), int(precision), 0, _pd(out),
+                                rf.ctypes.data_as(C.c_void_p)))
+        return out, np.ascontiguousarray(np.moveaxis(rf, -1, 0))
+
+    def rf_lte_device(self, d_atmos, d_ndep, ncol, ndep_max, d_out, d_rf, mu=1.0, atm_scale=0,
+                      precision=0):
+        """rf_lte with device pointers: Stokes [ncol, 4, nout] float64 and the
+        response functions in the library's layout [RF_NPARAM, ndep_max, 4,
+        nout, ncol] (float32 for precision 1, else float64)."""
+        s = RhgColumns()
+        s.ncol, s.ndep_max = int(ncol), int(ndep_max)
+        s.ndep = C.cast(C.c_void_p(d_ndep), C.POINTER(C.c_int32))
+        s.atmos = C.cast(C.c_void_p(d_atmos), C.POINTER(C.c_double))
+        s.atm_scale, s.mu = int(atm_scale), float(mu)
+        _check(lib().rhg_rf_lte(self._h, C.byref(s), int(precision), 1,
+                                C.cast(C.c_void_p(d_out), C.POINTER(C.c_double)),
+                                C.c_void_p(d_rf)))
+
+    def rf_timings(self):
+        """Per-interval kernel milliseconds of the last rf_lte call."""
+        t = np.zeros(RHG_NSTAGE)
+        _check(lib().rhg_last_timings(self._h, _pd(t)))
+        return dict(zip(RF_STAGES, t))
 
     def count_work(self):
         """Work census of the last synthesis (slots RHG_WORK_*, see

@@ -86,3 +86,9 @@ def stokes_map(ncol, seed=SEED_BASE + 1):
     atm[:, 7] = phi[:, None]
     atm[:, 8] = nH[None, :]
     return atm
+
+
+def rf_batch(ncol=1024, seed=SEED_BASE + 2):
+    """The response-function batch: columns drawn as in the Stokes map,
+    from their own seed."""
+    return stokes_map(ncol, seed=seed)

@@ -1,0 +1,119 @@
+"""GPU response functions against central finite differences of the FP64
+synthesis, and the FP32 form against the FP64 adjoint.
+
+The synthesis is not smooth everywhere.  Humlicek's four regions are
+separate approximations that jump by up to 1e-4 where a component's
+(a, v) crosses a region boundary (rh/voigt.c:396-413).  The Fritsch-Butland
+limiter and the max(., 0) clamps of the Bezier control points
+(rh/bezier_aux.c:34-51, bezier_1D.c:183-188) have kinks.  A central
+difference that straddles one of these points differs from the derivative
+on either side.  The fraction of such points is proportional to the step
+(measured: 3% of (lambda, Stokes) points at dv = 10 m/s, 0.3% at 1 m/s).
+The checks are therefore on the median and on the share of points beyond
+the tolerance, with steps 10x below the documented ones (dT = 0.1 K,
+dv = 1 m/s, dB = 1 G, dangle = 1e-3 rad); the documented steps are
+exercised too, with proportionally looser shares.
+"""
+import numpy as np
+import pytest
+
+from pyrh_amd import model, workloads
+from pyrh_amd.engine import RF_PARAMS, RF_ROWS
+
+pytestmark = pytest.mark.gpu
+
+STEP_DOC = {"T": 1.0, "vlos": 0.01, "B": 1.0, "gamma": 0.01, "chi": 0.01, "vmic": 0.01}
+STEP_FINE = {"T": 0.1, "vlos": 0.001, "B": 1.0, "gamma": 0.001, "chi": 0.001, "vmic": 0.001}
+DEPTHS = (3, 20, 40, 50, 58, 66, 74, 80)
+
+
+@pytest.fixture(scope="module")
+def rf_case(setups):
+    from pyrh_amd.engine import Engine
+    wave = workloads.wavelengths_630()
+    eng = Engine(model.build_model(setups["lte"], wave))
+    atm = workloads.stokes_map(3, seed=71)
+    st, rf = eng.rf_lte(atm)
+    return eng, atm, st, rf
+
+
+def _fd_errors(eng, atm, rf, param, h):
+    """|adjoint - central difference| / max over (Stokes, lambda) of the
+    difference quotient, per column and depth; all (column, depth, Stokes,
+    lambda) points with a nonzero scale."""
+    ip = RF_PARAMS.index(param)
+    errs = []
+    for k in DEPTHS:
+        a = atm.copy(); a[:, RF_ROWS[ip], k] += h
+        b = atm.copy(); b[:, RF_ROWS[ip], k] -= h
+        fd = (eng.synth_lte(a) - eng.synth_lte(b)) / (2 * h)       # [ncol, 4, nout]
+        ad = rf[:, ip, k]
+        scale = np.max(np.abs(fd), axis=(1, 2), keepdims=True)
+        ok = scale[:, 0, 0] > 0
+        errs.append((np.abs(ad - fd) / np.maximum(scale, 1e-300))[ok].ravel())
+    return np.concatenate(errs)
+
+
+def test_rf_stokes_equal_synthesis(rf_case):
+    eng, atm, st, _ = rf_case
+    np.testing.assert_allclose(st, eng.synth_lte(atm), rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("param", RF_PARAMS)
+def test_rf_matches_central_differences(rf_case, param):
+    eng, atm, _, rf = rf_case
+    e = _fd_errors(eng, atm, rf, param, STEP_FINE[param])
+    med, share = float(np.median(e)), float(np.mean(e > 1e-3))
+    print(f"{param} h={STEP_FINE[param]}: median {med:.2e}  p99 {np.percentile(e, 99):.2e}  "
+          f"max {e.max():.2e}  share>1e-3 {share:.4f}", flush=True)
+    assert med < 1e-6
+    assert share < 0.03
+    if param == "T":                 # no profile-shape kinks reach T at this step
+        assert e.max() < 1e-4
+
+
+@pytest.mark.parametrize("param", RF_PARAMS)
+def test_rf_documented_steps(rf_case, param):
+    eng, atm, _, rf = rf_case
+    e = _fd_errors(eng, atm, rf, param, STEP_DOC[param])
+    med, share = float(np.median(e)), float(np.mean(e > 1e-3))
+    print(f"{param} h={STEP_DOC[param]}: median {med:.2e}  p99 {np.percentile(e, 99):.2e}  "
+          f"share>1e-3 {share:.4f}", flush=True)
+    assert med < 1e-5
+    assert share < 0.15
+
+
+def test_rf_fp32_form_against_fp64(rf_case):
+    """FP32-form response functions against the FP64 adjoint, relative to
+    each (column, depth)'s largest response over Stokes and wavelength."""
+    eng, atm, st, rf = rf_case
+    st32, rf32 = eng.rf_lte(atm, precision=1)
+    for ip, name in enumerate(RF_PARAMS):
+        a, b = rf[:, ip], rf32[:, ip].astype(np.float64)
+        scale = np.max(np.abs(a), axis=(2, 3), keepdims=True)
+        sel = scale[..., 0, 0] > 0
+        e = (np.abs(a - b) / np.maximum(scale, 1e-300))[sel]
+        p99, share = float(np.percentile(e, 99)), float(np.mean(e > 1e-4))
+        print(f"FP32 RF {name}: p99 {p99:.2e}  max {e.max():.2e}  share>1e-4 {share:.4f}",
+              flush=True)
+        assert p99 < 1e-4 and share < 0.01
+        if name == "T":
+            assert e.max() < 1e-4
+
+
+def test_rf_azimuth_rotation_sum(rf_case):
+    """Turning the field azimuth by the same angle at every depth rotates the
+    frame about the line of sight at mu = 1: I and V stay, (Q, U) turn by
+    twice the angle.  Summed over depth, the azimuth response functions are
+    therefore 0 for I and V and (-2U, 2Q) (up to the sense of rotation)
+    for Q and U."""
+    eng, atm, st, rf = rf_case
+    r = rf[:, RF_PARAMS.index("chi")]              # [ncol, ndep, 4, nout]
+    tot = r.sum(axis=1)
+    Ic = st[:, 0, :1]
+    assert np.max(np.abs(tot[:, 0]) / Ic) < 1e-9
+    assert np.max(np.abs(tot[:, 3]) / Ic) < 1e-9
+    sense = np.sign(np.sum(tot[:, 2] * st[:, 1]))
+    assert sense != 0
+    np.testing.assert_allclose(tot[:, 1] / Ic, -2 * sense * st[:, 2] / Ic, rtol=0, atol=1e-8)
+    np.testing.assert_allclose(tot[:, 2] / Ic, 2 * sense * st[:, 1] / Ic, rtol=0, atol=1e-8)
