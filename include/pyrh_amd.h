@@ -244,6 +244,22 @@ enum rhg_rf_param {
 int rhg_rf_lte(rhg_context_t *ctx, const rhg_columns_t *cols, int32_t precision,
                int32_t device_ptrs, double *stokes_out, void *rf_out);
 
+/* d I / d log gf of the emergent intensity, the reference's analytic atomic
+ * response functions: compute1d(get_atomic_rfs=True) (pyrh.pyx:604-660;
+ * rh/rhf1d/bezier_1D.c:416-536, rh/kurucz.c:224-260, 696-699,
+ * rh/rhf1d/formal.c:59-65).  Parameter p perturbs the log gf of model line
+ * par_line[p] (an index into the model's line table; < 0 for a parameter
+ * with no response, such as the reference's never-filled wavelength-shift
+ * entries).  The response is nonzero only at the wavelengths the scalar
+ * cubic Bezier solves (moving columns), as in the reference.  stokes_out is
+ * as in rhg_synth_lte; rf_out is [ncol][nspect-1][npar] float64, host or
+ * device memory (device if `device_ptrs`); par_line is host memory.  FP64.
+ * At most RHG_RF_MAX_COLUMNS columns and RHG_RF_MAX_ATOMIC parameters. */
+#define RHG_RF_MAX_ATOMIC 256
+int rhg_rf_loggf(rhg_context_t *ctx, const rhg_columns_t *cols, int32_t npar,
+                 const int32_t *par_line, int32_t device_ptrs, double *stokes_out,
+                 double *rf_out);
+
 /* Same as rhg_synth_lte but also returns, per column, the depth scale and
  * the background opacity at the reference wavelength (for testing):
  * height_out/chiref_out are [ncol][ndep_max] float64 host buffers (NULL to

@@ -56,6 +56,11 @@ def lib():
         L.rho_formal_bezier.argtypes = [C.c_int, P, C.c_double, P, P, P, P]
         L.rho_formal_feautrier.argtypes = [C.c_int, P, C.c_double, P, P, P]
         L.rho_formal_feautrier.restype = C.c_double
+        L.rho_rf_loggf.argtypes = [C.POINTER(RhgModel), C.POINTER(RhgColumns), C.c_int,
+                                   C.POINTER(C.c_int), C.c_int, P, P]
+        L.rho_rf_loggf.restype = C.c_int
+        L.rho_formal_bezier_in.argtypes = [C.c_int, P, C.c_double, P, P, P]
+        L.rho_formal_bezier_rf.argtypes = [C.c_int, P, C.c_double, P, P, P, C.c_int, P, P, P, P]
         L.rho_model_sizeof.restype = C.c_int
         L.rho_columns_sizeof.restype = C.c_int
         _LIB = L
@@ -81,6 +86,46 @@ def synth_lte(model, atmos, mu=1.0, atm_scale=0, ndep=None, inverse_fp32=False,
     if rc != 0:
         raise RuntimeError(f"oracle failed with code {rc}")
     return (st, h, cr) if diag else st
+
+
+def rf_loggf(model, atmos, par_line, mu=1.0, atm_scale=0, ndep=None, nthreads=0):
+    """Stokes [ncol, 4, nlw] and the reference's analytic d I / d log gf
+    [ncol, nlw, npar] for parameter lines par_line."""
+    ms = ModelStruct(model)
+    cs = ColumnsStruct(atmos, ndep, atm_scale, mu)
+    ncol = cs.struct.ncol
+    nout = model.scalars["nspect"] - 1
+    par = np.ascontiguousarray(par_line, dtype=np.int32)
+    st = np.zeros((ncol, 4, nout))
+    rf = np.zeros((ncol, nout, par.size))
+    rc = lib().rho_rf_loggf(ms.ptr(), cs.ptr(), int(par.size),
+                            par.ctypes.data_as(C.POINTER(C.c_int)), int(nthreads), _p(st), _p(rf))
+    if rc != 0:
+        raise RuntimeError(f"oracle failed with code {rc}")
+    return st, rf
+
+
+def formal_bezier_in(z, mu, chi, S):
+    """Scalar Bezier intensity along the inward ray at every depth."""
+    z, chi, S = (np.ascontiguousarray(x, np.float64) for x in (z, chi, S))
+    I = np.zeros(z.size)
+    lib().rho_formal_bezier_in(z.size, _p(z), float(mu), _p(chi), _p(S), _p(I))
+    return I
+
+
+def formal_bezier_rf(z, mu, chi, S, B2, Iin, dchi, deta):
+    """Emergent intensity at every depth and the reference's analytic
+    responses [npar] for the opacity/emissivity derivatives dchi, deta
+    [npar, n]; Iin holds the inward intensities."""
+    z, chi, S, B2 = (np.ascontiguousarray(x, np.float64) for x in (z, chi, S, B2))
+    I = np.array(Iin, np.float64)
+    dchi = np.ascontiguousarray(dchi, np.float64)
+    deta = np.ascontiguousarray(deta, np.float64)
+    npar = dchi.shape[0]
+    dI0 = np.zeros(npar)
+    lib().rho_formal_bezier_rf(z.size, _p(z), float(mu), _p(chi), _p(S), _p(B2), npar,
+                               _p(dchi), _p(deta), _p(I), _p(dI0))
+    return I, dI0
 
 
 def voigt_humlicek(a, v):

@@ -627,6 +627,167 @@ void rho_formal_bezier(int Ndep, const double *z, double mu, const double *chi,
   (void) dchi_up;
 }
 
+/* Scalar cubic Bezier along the inward ray (to_obs = 0: depth 0 to n-1)
+   with zero incident intensity at the top (ZERO boundary,
+   rh/rhf1d/multiatmos.c:62-66).  rh/rhf1d/bezier_1D.c:306-541 */
+void rho_formal_bezier_in(int Ndep, const double *z, double mu, const double *chi,
+                          const double *S, double *I)
+{
+  const int dk = 1, k_start = 0, k_end = Ndep - 1;
+  double zmu = 1.0 / mu, dtau_uw, dtau_dw = 0.0, dS_uw, I_upw, c1, c2, w[3];
+  double dsup, dsdn, dt03, co[5], dS_up, dS_c, dchi_up, dchi_c, dchi_dn = 0.0, dsdn2;
+  int k;
+
+  I_upw = 0.0;
+  I[k_start] = I_upw;
+  k = k_start + dk;
+  dsup = fabs(z[k] - z[k - dk]) * zmu;
+  dsdn = fabs(z[k + dk] - z[k]) * zmu;
+  dchi_up = (chi[k] - chi[k - dk]) / dsup;
+  dchi_c = cent_deriv(dsup, dsdn, chi[k - dk], chi[k], chi[k + dk]);
+  c1 = MAXV(chi[k] - (dsup / 3.0) * dchi_c, 0.0);
+  c2 = MAXV(chi[k - dk] + (dsup / 3.0) * dchi_up, 0.0);
+  dtau_uw = dsup * (chi[k] + chi[k - dk] + c1 + c2) * 0.25;
+  dS_up = (S[k] - S[k - dk]) / dtau_uw;
+
+  for (k = k_start + dk; k != k_end + dk; k += dk) {
+    if (k != k_end) {
+      dsdn = fabs(z[k + dk] - z[k]) * zmu;
+      if (abs(k - k_end) > 1) {
+        dsdn2 = fabs(z[k + 2 * dk] - z[k + dk]) * zmu;
+        dchi_dn = cent_deriv(dsdn, dsdn2, chi[k], chi[k + dk], chi[k + 2 * dk]);
+      } else
+        dchi_dn = (chi[k + dk] - chi[k]) / dsdn;
+      c1 = MAXV(chi[k] + (dsdn / 3.0) * dchi_c, 0.0);
+      c2 = MAXV(chi[k + dk] - (dsdn / 3.0) * dchi_dn, 0.0);
+      dtau_dw = dsdn * (chi[k] + chi[k + dk] + c1 + c2) * 0.25;
+      dt03 = dtau_uw / 3.0;
+      rho_bezier3_coeffs(dtau_uw, co);
+      dS_c = cent_deriv(dtau_uw, dtau_dw, S[k - dk], S[k], S[k + dk]);
+      c1 = MAXV(S[k] - dt03 * dS_c, 0.0);
+      c2 = MAXV(S[k - dk] + dt03 * dS_up, 0.0);
+      I[k] = I_upw * co[4] + co[0] * S[k] + co[1] * S[k - dk] + co[2] * c1 + co[3] * c2;
+    } else {
+      dtau_uw = 0.5 * zmu * (chi[k] + chi[k - dk]) * fabs(z[k] - z[k - dk]);
+      dS_uw = -(S[k] - S[k - dk]) / dtau_uw;
+      rho_w3(dtau_uw, w);
+      I[k] = (1.0 - w[0]) * I_upw + w[0] * S[k] + w[1] * dS_uw;
+    }
+    I_upw = I[k];
+    dsup = dsdn;
+    dchi_up = dchi_c;
+    dchi_c = dchi_dn;
+    dtau_uw = dtau_dw;
+    dS_up = dS_c;
+  }
+  (void) dsup;
+  (void) dchi_up;
+}
+
+/* Scalar cubic Bezier along the emergent ray with the reference's analytic
+   d I / d log gf recursion (rh/rhf1d/bezier_1D.c:416-536):
+   Z = (-dchi I + deta) / chi through the intensity's Bezier weights, where
+   I is the one array the inward ray filled first (rh/rhf1d/formal.c:168-172),
+   so the downwind I in Z is still the inward value.  On entry I holds the
+   inward intensities; on return the emergent ones.  dchi, deta are
+   [npar][n]; dI0 receives d I(top) / d p for each parameter. */
+void rho_formal_bezier_rf(int Ndep, const double *z, double mu, const double *chi,
+                          const double *S, const double *B2, int npar,
+                          const double *dchi, const double *deta, double *I, double *dI0)
+{
+  const int dk = -1, k_start = Ndep - 1, k_end = 0;
+  double zmu = 1.0 / mu, dtau_uw, dtau_dw = 0.0, dS_uw, I_upw, c1, c2, w[3];
+  double dsup, dsdn, dt03, co[5], dS_up, dS_c, dchi_up, dchi_c, dchi_dn = 0.0, dsdn2;
+  double Zk, Zkm1, Zkp1, *dZk, *dZup, *dI_upw, *dI;
+  int k, p;
+#define DCHI(p, k) dchi[(size_t) (p) * Ndep + (k)]
+#define DETA(p, k) deta[(size_t) (p) * Ndep + (k)]
+
+  dZk = calloc((size_t) 4 * npar + 1, sizeof(double));
+  dZup = dZk + npar;
+  dI_upw = dZup + npar;
+  dI = dI_upw + npar;
+  dtau_uw = 0.5 * zmu * (chi[k_start] + chi[k_start + dk]) *
+            fabs(z[k_start] - z[k_start + dk]);
+  I_upw = B2[1] - (B2[0] - B2[1]) / dtau_uw;
+  I[k_start] = I_upw;
+
+  k = k_start + dk;
+  dsup = fabs(z[k] - z[k - dk]) * zmu;
+  dsdn = fabs(z[k + dk] - z[k]) * zmu;
+  dchi_up = (chi[k] - chi[k - dk]) / dsup;
+  dchi_c = cent_deriv(dsup, dsdn, chi[k - dk], chi[k], chi[k + dk]);
+  c1 = MAXV(chi[k] - (dsup / 3.0) * dchi_c, 0.0);
+  c2 = MAXV(chi[k - dk] + (dsup / 3.0) * dchi_up, 0.0);
+  dtau_uw = dsup * (chi[k] + chi[k - dk] + c1 + c2) * 0.25;
+  dS_up = (S[k] - S[k - dk]) / dtau_uw;
+  for (p = 0; p < npar; p++) {
+    dI_upw[p] = 0.0;
+    Zk = -DCHI(p, k) / chi[k] * I[k] + DETA(p, k) / chi[k];
+    Zkm1 = -DCHI(p, k - dk) / chi[k - dk] * I[k - dk] + DETA(p, k - dk) / chi[k - dk];
+    dZup[p] = (Zk - Zkm1) / dtau_uw;
+  }
+
+  for (k = k_start + dk; k != k_end + dk; k += dk) {
+    if (k != k_end) {
+      dsdn = fabs(z[k + dk] - z[k]) * zmu;
+      if (abs(k - k_end) > 1) {
+        dsdn2 = fabs(z[k + 2 * dk] - z[k + dk]) * zmu;
+        dchi_dn = cent_deriv(dsdn, dsdn2, chi[k], chi[k + dk], chi[k + 2 * dk]);
+      } else
+        dchi_dn = (chi[k + dk] - chi[k]) / dsdn;
+      c1 = MAXV(chi[k] + (dsdn / 3.0) * dchi_c, 0.0);
+      c2 = MAXV(chi[k + dk] - (dsdn / 3.0) * dchi_dn, 0.0);
+      dtau_dw = dsdn * (chi[k] + chi[k + dk] + c1 + c2) * 0.25;
+      dt03 = dtau_uw / 3.0;
+      rho_bezier3_coeffs(dtau_uw, co);
+      dS_c = cent_deriv(dtau_uw, dtau_dw, S[k - dk], S[k], S[k + dk]);
+      c1 = MAXV(S[k] - dt03 * dS_c, 0.0);
+      c2 = MAXV(S[k - dk] + dt03 * dS_up, 0.0);
+      I[k] = I_upw * co[4] + co[0] * S[k] + co[1] * S[k - dk] + co[2] * c1 + co[3] * c2;
+      for (p = 0; p < npar; p++) {
+        Zk = -DCHI(p, k) * I[k] + DETA(p, k);
+        Zk /= chi[k];
+        Zkm1 = -DCHI(p, k - dk) * I[k - dk] + DETA(p, k - dk);
+        Zkm1 /= chi[k - dk];
+        Zkp1 = -DCHI(p, k + dk) * I[k + dk] + DETA(p, k + dk);
+        Zkp1 /= chi[k + dk];
+        dZk[p] = cent_deriv(dtau_uw, dtau_dw, Zkm1, Zk, Zkp1);
+        c1 = MAXV(Zk - dt03 * dZk[p], 0.0);
+        c2 = MAXV(Zkm1 + dt03 * dZup[p], 0.0);
+        dI[p] = dI_upw[p] * co[4] + co[0] * Zk + co[1] * Zkm1 + co[2] * c1 + co[3] * c2;
+      }
+    } else {
+      dtau_uw = 0.5 * zmu * (chi[k] + chi[k - dk]) * fabs(z[k] - z[k - dk]);
+      dS_uw = -(S[k] - S[k - dk]) / dtau_uw;
+      rho_w3(dtau_uw, w);
+      I[k] = (1.0 - w[0]) * I_upw + w[0] * S[k] + w[1] * dS_uw;
+      for (p = 0; p < npar; p++) {
+        Zk = DCHI(p, k) / chi[k] * I[k] - DETA(p, k) / chi[k];
+        Zkm1 = DCHI(p, k - dk) / chi[k - dk] * I[k - dk] - DETA(p, k - dk) / chi[k - dk];
+        dZk[p] = -(Zk - Zkm1) / dtau_uw;
+        dI[p] = (1.0 - w[0]) * dI_upw[p] + w[0] * Zk + w[1] * dZk[p];
+      }
+    }
+    I_upw = I[k];
+    dsup = dsdn;
+    dchi_up = dchi_c;
+    dchi_c = dchi_dn;
+    dtau_uw = dtau_dw;
+    dS_up = dS_c;
+    for (p = 0; p < npar; p++) {
+      dI_upw[p] = dI[p];
+      dZup[p] = dZk[p];
+    }
+  }
+  for (p = 0; p < npar; p++) dI0[p] = dI_upw[p];
+  free(dZk);
+  (void) dsup;
+  (void) dchi_up;
+#undef DCHI
+#undef DETA
+}
+
 /* Feautrier solution, STANDARD order, ZERO upper boundary, thermalised
    lower boundary; returns the emergent intensity.  rh/rhf1d/feautrier.c:56-202 */
 double rho_formal_feautrier(int Ndep, const double *z, double mu,
@@ -1097,8 +1258,12 @@ static void background_continuum(const rhg_model_t *m, const column_t *c, int ls
 /* Kurucz-line opacity for the ray leaving the top (to_obs = 1),
    rh/kurucz.c:511-827.  Writes chi/eta [4][Ndep] from zero; returns 0 if
    the wavelength has no contributing line. */
-static int line_opacity(const rhg_model_t *m, const column_t *c, int ls,
-                        double mu, double *chi, double *eta)
+/* Kurucz-line opacity at wavelength ls for the ray of direction cosine mu
+   (mu < 0: the inward ray's Doppler shift, rh/kurucz.c:744-749);
+   sel >= 0 keeps only model line sel (its chi_l phi and eta_l phi, as
+   rh/kurucz.c:696-699 records for the log gf response). */
+static int line_opacity_sel(const rhg_model_t *m, const column_t *c, int ls,
+                            double mu, double *chi, double *eta, int sel)
 {
   const double lambda = m->lamconst[(size_t) ls * RHG_LC_NCONST + RHG_LC_LAMBDA];
   int p0 = m->lam_line_ptr[ls], p1 = m->lam_line_ptr[ls + 1], e, k, N = c->Ndep;
@@ -1107,6 +1272,7 @@ static int line_opacity(const rhg_model_t *m, const column_t *c, int ls,
   for (k = 0; k < 4 * N; k++) { chi[k] = 0.0; eta[k] = 0.0; }
   for (e = p0; e < p1; e++) {
     int l = m->lam_line_idx[e], ie = m->line_el[l], st = m->line_stage[l];
+    if (sel >= 0 && l != sel) continue;
     double lambda0 = m->line_lambda0[l], weight = m->el_weight[ie];
     double vtherm = 2.0 * KBOLTZMANN / (AMU * weight);
     const double *lp = m->el_logpf + m->el_pf_ptr[ie] + (size_t) st * m->npf;
@@ -1179,10 +1345,17 @@ static int line_opacity(const rhg_model_t *m, const column_t *c, int ls,
   return 1;
 }
 
+static int line_opacity(const rhg_model_t *m, const column_t *c, int ls,
+                        double mu, double *chi, double *eta)
+{
+  return line_opacity_sel(m, c, ls, mu, chi, eta, -1);
+}
+
 static int synth_column_ex(const rhg_model_t *m, const rhg_columns_t *cols, int ic,
                            int fp32, double *stokes, double *height_out,
                            double *chiref_out, int diag_ls, double *diag_chi,
-                           double *diag_eta, double *diag_pop)
+                           double *diag_eta, double *diag_pop, int npar,
+                           const int *par_line, double *rf)
 {
   const int N = cols->ndep[ic], Nmax = cols->ndep_max, nspect = m->nspect;
   const int nlev = m->atom_level_ptr[m->natom], natom = m->natom;
@@ -1341,6 +1514,32 @@ static int synth_column_ex(const rhg_model_t *m, const rhg_columns_t *cols, int 
         for (k = 0; k < N; k++) S4[n * N + k] = (0.0 + eta_c[n * N + k]) / chi_c[k];
       rho_formal_stokes(N, c.height, mu, chi_c, S4, chi_c + N, B2, fp32, I4);
       for (n = 0; n < 4; n++) Iout[n] = I4[n * N];
+    } else if (solver == RHG_SOLVER_BEZIER && rf) {
+      /* the inward ray first, then the emergent one with the log gf
+         response (rh/rhf1d/formal.c:168-172, 281-284) */
+      double *dchi = calloc((size_t) 2 * npar * N + 1, sizeof(double)), *deta;
+      int p;
+      if (!dchi) { free(mem); return -1; }
+      deta = dchi + (size_t) npar * N;
+      /* pyrh keeps one background array per wavelength
+         (rh/readj.c:241-268, filled by writeBackground rh/readj.c:300-345);
+         the last pass that wrote it is the emergent ray's
+         (rh/background.c:470-580), so the inward ray runs on the emergent
+         ray's opacity and source function */
+      for (k = 0; k < N; k++) S4[k] = (0.0 + eta_c[k]) / chi_c[k];
+      rho_formal_bezier_in(N, c.height, mu, chi_c, S4, I4);
+      for (p = 0; p < npar; p++) {
+        if (par_line[p] < 0) continue;
+        if (!line_opacity_sel(m, &c, ls, mu, lchi, leta, par_line[p])) continue;
+        for (k = 0; k < N; k++) {
+          dchi[(size_t) p * N + k] = lchi[k] * LG10;
+          deta[(size_t) p * N + k] = leta[k] * LG10;
+        }
+      }
+      rho_formal_bezier_rf(N, c.height, mu, chi_c, S4, B2, npar, dchi, deta, I4,
+                           rf + (size_t) nout * npar);
+      Iout[0] = I4[0];
+      free(dchi);
     } else if (solver == RHG_SOLVER_BEZIER) {
       for (k = 0; k < N; k++) S4[k] = (0.0 + eta_c[k]) / chi_c[k];
       rho_formal_bezier(N, c.height, mu, chi_c, S4, B2, I4);
@@ -1361,7 +1560,7 @@ static int synth_column(const rhg_model_t *m, const rhg_columns_t *cols, int ic,
                         double *chiref_out)
 {
   return synth_column_ex(m, cols, ic, fp32, stokes, height_out, chiref_out,
-                         -1, NULL, NULL, NULL);
+                         -1, NULL, NULL, NULL, 0, NULL, NULL);
 }
 
 /* Diagnostic: continuum (chi[0:N], eta[0:N]) and line (chi/eta[4N:8N])
@@ -1370,7 +1569,8 @@ static int synth_column(const rhg_model_t *m, const rhg_columns_t *cols, int ic,
 int rho_background(const rhg_model_t *m, const rhg_columns_t *cols, int ic, int ls,
                    double *chi8, double *eta8, double *pop6)
 {
-  return synth_column_ex(m, cols, ic, 0, NULL, NULL, NULL, ls, chi8, eta8, pop6);
+  return synth_column_ex(m, cols, ic, 0, NULL, NULL, NULL, ls, chi8, eta8, pop6, 0, NULL,
+                         NULL);
 }
 
 int rho_synth_lte(const rhg_model_t *m, const rhg_columns_t *cols,
@@ -1389,6 +1589,31 @@ int rho_synth_lte(const rhg_model_t *m, const rhg_columns_t *cols,
     if (synth_column(m, cols, ic, inverse_fp32, stokes_out + (size_t) ic * 4 * nout,
                      height_out ? height_out + (size_t) ic * cols->ndep_max : NULL,
                      chiref_out ? chiref_out + (size_t) ic * cols->ndep_max : NULL))
+      err |= 1;
+  }
+  (void) nthreads;
+  return err ? RHG_ENOMEM : RHG_OK;
+}
+
+/* The reference's analytic d I / d log gf with the synthesis
+   (compute1d(get_atomic_rfs=True), rh/rhf1d/bezier_1D.c:416-536): rf_out is
+   [ncol][nspect-1][npar], zero at the wavelengths the scalar Bezier does
+   not solve. */
+int rho_rf_loggf(const rhg_model_t *m, const rhg_columns_t *cols, int npar,
+                 const int *par_line, int nthreads, double *stokes_out, double *rf_out)
+{
+  int ic, err = 0;
+  const int nout = m->nspect - 1;
+  if (m->nnuclei + m->nmol > NEQ_MAX) return RHG_EUNSUPPORTED;
+  memset(rf_out, 0, (size_t) cols->ncol * nout * npar * sizeof(double));
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads) reduction(|:err)
+#endif
+  for (ic = 0; ic < cols->ncol; ic++) {
+    if (synth_column_ex(m, cols, ic, 0, stokes_out + (size_t) ic * 4 * nout, NULL, NULL, -1,
+                        NULL, NULL, NULL, npar, par_line,
+                        rf_out + (size_t) ic * nout * npar))
       err |= 1;
   }
   (void) nthreads;

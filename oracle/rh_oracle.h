@@ -26,6 +26,12 @@ int rho_synth_lte(const rhg_model_t *m, const rhg_columns_t *cols,
                   int inverse_fp32, int nthreads, double *stokes_out,
                   double *height_out, double *chiref_out);
 
+/* The reference's analytic d I / d log gf (compute1d(get_atomic_rfs=True),
+ * rh/rhf1d/bezier_1D.c:416-536) with the FP64 synthesis: parameter p is
+ * model line par_line[p] (< 0: no response).  rf_out [ncol][nspect-1][npar]. */
+int rho_rf_loggf(const rhg_model_t *m, const rhg_columns_t *cols, int npar,
+                 const int *par_line, int nthreads, double *stokes_out, double *rf_out);
+
 /* Single-function entry points for closed-form and invariant tests. */
 double rho_voigt_humlicek(double a, double v, double *F);
 double rho_voigt_armstrong(double a, double v);
@@ -44,6 +50,15 @@ void rho_formal_stokes(int n, const double *z, double mu, const double *chi,
                        const double *B2, int inverse_fp32, double *I4);
 void rho_formal_bezier(int n, const double *z, double mu, const double *chi,
                        const double *S, const double *B2, double *I);
+/* The same solver along the inward ray (depth 0 to n-1, zero incident
+ * intensity at the top), and along the emergent ray with the reference's
+ * d I / d log gf recursion: I holds the inward intensities on entry;
+ * dchi, deta are [npar][n]; dI0 receives the emergent responses [npar]. */
+void rho_formal_bezier_in(int n, const double *z, double mu, const double *chi,
+                          const double *S, double *I);
+void rho_formal_bezier_rf(int n, const double *z, double mu, const double *chi,
+                          const double *S, const double *B2, int npar,
+                          const double *dchi, const double *deta, double *I, double *dI0);
 double rho_formal_feautrier(int n, const double *z, double mu,
                             const double *chi, const double *S,
                             const double *B2);

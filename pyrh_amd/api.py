@@ -74,9 +74,10 @@ def compute1d(cwd, mu, atm_scale, atmosphere, wave, loggf_ids=None, loggf_values
               atomic_number=None, atomic_abundance=None, get_atomic_rfs=False,
               get_populations=False):
     """pyrh.compute1d (pyrh.pyx:537-668): returns (I, Q, U, V, lam) for one
-    column.  Arrays own their memory and the inputs are not modified."""
-    if get_atomic_rfs:
-        raise NotImplementedError("get_atomic_rfs is not implemented by the GPU engine")
+    column, and with get_atomic_rfs ((I, Q, U, V, lam), rf) where rf is
+    d I / d log gf [len(loggf_ids) + len(lam_ids), nlw] (the wavelength-shift
+    rows are zero, as in the reference).  Arrays own their memory and the
+    inputs are not modified."""
     if get_populations:
         raise NotImplementedError("get_populations applies to active (NLTE) atoms, which "
                                   "this LTE engine does not solve")
@@ -95,6 +96,14 @@ def compute1d(cwd, mu, atm_scale, atmosphere, wave, loggf_ids=None, loggf_values
         raise ValueError("atmosphere must be [>=9, Ndep]")
     e = engine_for(cwd, wave, atomic_number, atomic_abundance, loggf_ids, loggf_values,
                    lam_ids, lam_values)
-    st = e.synth_lte(atm[None, :9], mu, atm_scale)
     lam = e.model.lam[e.model.out_index].copy()
+    if get_atomic_rfs:
+        nlam = 0 if lam_ids is None else len(lam_ids)
+        par = _model.atomic_rf_lines(e.model.setup, loggf_ids, nlam)
+        if par.size == 0:
+            raise ValueError("get_atomic_rfs needs loggf_ids or lam_ids")
+        st, rf = e.rf_loggf(atm[None, :9], par, mu, atm_scale)
+        out = st[0, 0].copy(), st[0, 1].copy(), st[0, 2].copy(), st[0, 3].copy(), lam
+        return out, np.ascontiguousarray(rf[0].T)
+    st = e.synth_lte(atm[None, :9], mu, atm_scale)
     return st[0, 0].copy(), st[0, 1].copy(), st[0, 2].copy(), st[0, 3].copy(), lam

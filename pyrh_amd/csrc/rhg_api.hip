@@ -61,6 +61,8 @@ struct rhg_context {
   // response functions: perturbed inputs/states, tangents, tape, adjoints
   DevBuf rf_atm[2], rf_state[2], rf_cont[4], rf_chiref[2], rf_moving, rf_dstate, rf_dcont[2],
       rf_dchiref, rf_pts, rf_car, rf_aP, rf_aZ, rf_aB, rf_out;
+  // log gf response functions: inward intensities, output, parameter lines
+  DevBuf at_iin, at_out, at_par;
   hipEvent_t ev[RHG_NSTAGE + 2]{};
   float last_ms[RHG_NSTAGE + 1]{};
   // the last batch, for rhg_count_work
@@ -731,6 +733,47 @@ int rhg_rf_lte(rhg_context_t *c, const rhg_columns_t *cols, int32_t precision, i
   const int rc = precision == RHG_PREC_FP32 ? rf_run<float>(c, cols, device_ptrs, stokes_out, rf_out)
                                             : rf_run<double>(c, cols, device_ptrs, stokes_out, rf_out);
   if (rc) return rc;
+  g_err.clear();
+  return 0;
+}
+
+int rhg_rf_loggf(rhg_context_t *c, const rhg_columns_t *cols, int32_t npar,
+                 const int32_t *par_line, int32_t device_ptrs, double *stokes_out,
+                 double *rf_out) {
+  if (!c || !cols || !stokes_out || !rf_out || (npar > 0 && !par_line))
+    return fail(RHG_EINVAL, "null argument");
+  if (npar < 1 || npar > RHG_RF_MAX_ATOMIC)
+    return fail(RHG_EINVAL, "npar must be in [1, RHG_RF_MAX_ATOMIC]");
+  for (int p = 0; p < npar; p++)
+    if (par_line[p] >= c->nline) return fail(RHG_EINVAL, "par_line entry beyond the model's lines");
+  const int ncol = cols->ncol, Nmax = cols->ndep_max, nout = c->nout;
+  if (ncol < 1 || ncol > RHG_RF_MAX_COLUMNS)
+    return fail(RHG_EINVAL, "bad batch shape (at most RHG_RF_MAX_COLUMNS columns per call)");
+  // the synthesis leaves the column state, the FP64 continuum and the
+  // depth counts of this batch in the context
+  int rc = synth(c, cols, RHG_PREC_FP64, device_ptrs, stokes_out, nullptr, nullptr);
+  if (rc) return rc;
+  const size_t rf_b = (size_t) ncol * nout * npar * sizeof(double);
+  const int nscal = c->n_scal > 0 ? c->n_scal : 1;
+  if (c->at_iin.ensure((size_t) nscal * Nmax * ncol * sizeof(double)) ||
+      c->at_par.ensure((size_t) npar * sizeof(int32_t)) || (!device_ptrs && c->at_out.ensure(rf_b)))
+    return fail(RHG_ENOMEM, "device allocation for the log gf response functions failed");
+  double *d_rf = device_ptrs ? rf_out : (double *) c->at_out.p;
+  HIPCHECK(hipMemcpyAsync(c->at_par.p, par_line, npar * sizeof(int32_t), hipMemcpyHostToDevice,
+                          c->stream));
+  HIPCHECK(hipMemsetAsync(d_rf, 0, rf_b, c->stream));
+  if (c->n_scal > 0) {
+    dim3 blk(64), grd((unsigned) rhg::xcd_grid((ncol + 63) / 64, c->n_scal));
+    hipLaunchKernelGGL(rhg::k_rf_loggf, grd, blk, 0, c->stream, c->d_model, c->last_ndep,
+                       (const int32_t *) c->moving.p, ncol, Nmax, (const double *) c->state.p,
+                       c->d_out_ls, c->d_io_scal, c->n_scal, nout, (const double *) c->cchi.p,
+                       (const double *) c->ceta.p, cols->mu, npar, (const int32_t *) c->at_par.p,
+                       (double *) c->at_iin.p, d_rf);
+    HIPCHECK(hipGetLastError());
+  }
+  if (!device_ptrs)
+    HIPCHECK(hipMemcpyAsync(rf_out, d_rf, rf_b, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
   g_err.clear();
   return 0;
 }

@@ -24,9 +24,9 @@ __device__ __forceinline__ void form_point(const DevModel &M, int ls, int has_li
   form_point_f<R>(M, ls, has_line, F, mu, moving, cchi, ceta, p);
 }
 
-// Common set-up of one emergent ray: lane -> column, workgroup -> (column
-// block, entry t of the wavelength list io_list).
-#define RAY_SETUP()                                                                      \
+// Common set-up of one ray: lane -> column, workgroup -> (column block,
+// entry t of the wavelength list io_list).
+#define RAY_SETUP_BASE()                                                                 \
   const DevModel &M = *Mp;                                                               \
   int cb, t;                                                                             \
   xcd_tile(blockIdx.x, nlist, cb, t);                                                    \
@@ -48,7 +48,11 @@ __device__ __forceinline__ void form_point(const DevModel &M, int ls, int has_li
   const R mu = (R) mu_d;                                                                 \
   double B2[2];                                                                          \
   B2[0] = planck(Tc[(int64_t) (N - 2) * ncol], lc[RHG_LC_HC_KLA], lc[RHG_LC_TWOHNU3_C2]); \
-  B2[1] = planck(Tc[(int64_t) (N - 1) * ncol], lc[RHG_LC_HC_KLA], lc[RHG_LC_TWOHNU3_C2]); \
+  B2[1] = planck(Tc[(int64_t) (N - 1) * ncol], lc[RHG_LC_HC_KLA], lc[RHG_LC_TWOHNU3_C2]);
+
+// the same, with the column's Stokes output for wavelength io
+#define RAY_SETUP()                                                                      \
+  RAY_SETUP_BASE();                                                                      \
   double *out = stokes + (int64_t) ic * 4 * nout + io;
 
 #define Z(kk) zc[(int64_t) (kk) * ncol]
@@ -219,10 +223,210 @@ k_formal_scalar(const DevModel *__restrict__ Mp, const int32_t *__restrict__ nde
   out[2 * (int64_t) nout] = 0.0;
   out[3 * (int64_t) nout] = 0.0;
 }
+// d I / d log gf of the emergent intensity: the analytic response of
+// rh/rhf1d/bezier_1D.c:416-536 (Piecewise_Bezier3_1D with get_atomic_rfs),
+// for the wavelengths of io_list that the scalar Bezier solves (a moving
+// column at a BEZIER wavelength; the others keep the zero that
+// rh/rhf1d/formal.c:59-65 sets).  Parameter p perturbs the log gf of model
+// line par_line[p] (< 0: no line, zero response).  The recursion is the
+// reference's, not the exact derivative: Z = (-dchi I + deta) / chi with
+// dchi = ln10 chi_l phi and deta = ln10 eta_l phi of that line's emergent-ray
+// profile (rh/kurucz.c:696-699), carried with the intensity's Bezier weights
+// and control-point clamps; the downwind I in Z is still the inward ray's
+// (rh/rhf1d/formal.c:168-172 solves to_obs = 0 first into the same array),
+// and in the last, linear interval Z enters with the opposite sign.  pyrh
+// keeps one background array per wavelength (rh/readj.c:241-268), last
+// written by the emergent ray's pass (rh/background.c:470-580), so the
+// inward ray runs on the emergent ray's opacity and source function.  The
+// inward ray (zero incident intensity at the top) is solved first into
+// Iin[(t * Nmax + k) * ncol + ic]; each parameter then takes one emergent
+// sweep.  Output rf[(ic * nout + io) * npar + p]; FP64 only.
+__global__ void __launch_bounds__(64)
+k_rf_loggf(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep,
+           const int32_t *__restrict__ moving_arr, int ncol, int Nmax,
+           const double *__restrict__ state, const int32_t *__restrict__ out_ls,
+           const int32_t *__restrict__ io_list, int nlist, int nout,
+           const double *__restrict__ cchi, const double *__restrict__ ceta, double mu_d,
+           int npar, const int32_t *__restrict__ par_line, double *__restrict__ Iin,
+           double *__restrict__ rf) {
+  using R = double;
+  RAY_SETUP_BASE();
+  if (M.solver[ls] != RHG_SOLVER_BEZIER || !moving) return;
+  const double zmu = 1.0 / mu_d;
+  const int lb = line_field_base(M.ncont);
+  double *Ii = Iin + (int64_t) t * Nmax * ncol + ic;
+#define IIN(kk) Ii[(int64_t) (kk) * ncol]
+#define POINT_IN(kk, P) POINT(kk, P)
+
+  // inward ray, to_obs = 0: depth 0 down to N-1
+  {
+    Point<R> Pm, P0, Pp, Pn;
+    POINT_IN(0, Pm);
+    POINT_IN(1, P0);
+    POINT_IN(2, Pp);
+    IIN(0) = 0.0;
+    double dsup = DZ(1) * zmu, dsdn = DZ(2) * zmu;
+    double dchi_up = (P0.chi - Pm.chi) / dsup;
+    double dchi_c = cent_deriv<R>(dsup, dsdn, Pm.chi, P0.chi, Pp.chi);
+    double c1 = rmax0(P0.chi - (dsup / 3.0) * dchi_c);
+    double c2 = rmax0(Pm.chi + (dsup / 3.0) * dchi_up);
+    double dtau_uw = dsup * (P0.chi + Pm.chi + c1 + c2) * 0.25;
+    double dS_up = (P0.S[0] - Pm.S[0]) / dtau_uw;
+    double I_upw = 0.0;
+    for (int k = 1; k < N - 1; k++) {
+      dsdn = DZ(k + 1) * zmu;
+      double dchi_dn;
+      if (N - 1 - k > 1) {
+        POINT_IN(k + 2, Pn);
+        const double dsdn2 = DZ(k + 2) * zmu;
+        dchi_dn = cent_deriv<R>(dsdn, dsdn2, P0.chi, Pp.chi, Pn.chi);
+      } else {
+        dchi_dn = (Pp.chi - P0.chi) / dsdn;
+      }
+      c1 = rmax0(P0.chi + (dsdn / 3.0) * dchi_c);
+      c2 = rmax0(Pp.chi - (dsdn / 3.0) * dchi_dn);
+      const double dtau_dw = dsdn * (P0.chi + Pp.chi + c1 + c2) * 0.25;
+      const double dt03 = dtau_uw / 3.0;
+      const Bezier3<R> bz = bezier3(dtau_uw);
+      const double dS_c = cent_deriv<R>(dtau_uw, dtau_dw, Pm.S[0], P0.S[0], Pp.S[0]);
+      c1 = rmax0(P0.S[0] - dt03 * dS_c);
+      c2 = rmax0(Pm.S[0] + dt03 * dS_up);
+      const double I = I_upw * bz.eps + bz.alpha * P0.S[0] + bz.beta * Pm.S[0] + bz.gamma * c1 +
+                       bz.theta * c2;
+      IIN(k) = I;
+      I_upw = I;
+      dsup = dsdn;
+      dchi_up = dchi_c;
+      dchi_c = dchi_dn;
+      dtau_uw = dtau_dw;
+      dS_up = dS_c;
+      Pm = P0;
+      P0 = Pp;
+      Pp = Pn;
+    }
+    double w[3];
+    const double dtl = 0.5 * zmu * (P0.chi + Pm.chi) * DZ(N - 1);
+    const double dS_uw = -(P0.S[0] - Pm.S[0]) / dtl;
+    w3(dtl, w);
+    IIN(N - 1) = (1.0 - w[0]) * I_upw + w[0] * P0.S[0] + w[1] * dS_uw;
+  }
+
+  const int p0 = M.lam_line_ptr[ls], p1 = M.lam_line_ptr[ls + 1];
+  for (int p = 0; p < npar; p++) {
+    const int lp = par_line[p];
+    int ep = -1;
+    for (int e = p0; e < p1; e++)
+      if (M.lam_line_idx[e] == lp) ep = e;
+    if (lp < 0 || ep < 0) continue;
+    // d chi and d eta of the parameter's line at depth kk
+    auto dline = [&](int kk, double &a, double &b) {
+      const double *sp = st0 + (int64_t) kk * ncol;
+      auto F = [sp, fs](int f) { return sp[f * fs]; };
+      double pq, pu, pv;
+      const double phi = line_profile_f<double>(M, ep, lp, F, mu, F(RHG_CF_VLOS), moving, pq, pu, pv);
+      a = b = 0.0;
+      if (phi != 0.0) {
+        a = F(lb + 4 * lp) * phi * LG10;
+        b = F(lb + 4 * lp + 1) * phi * LG10;
+      }
+    };
+    Point<R> Pu, P0, Pd, Pn;
+    double au = 0, bu = 0, a0 = 0, b0 = 0, ad = 0, bd = 0, an = 0, bn = 0;
+    double I_upw = 0, Iu = 0, dI_upw = 0, dZup = 0;
+    double dtau_uw = 0, dsup = 0, dchi_up = 0, dchi_c = 0, dS_up = 0;
+    for (int j = N - 1; j >= -1; j--) {
+      if (j >= 0) {
+        POINT(j, Pn);
+        dline(j, an, bn);
+      }
+      if (j == N - 1) {
+        Pu = Pn; au = an; bu = bn;
+        continue;
+      }
+      if (j == N - 2) {
+        P0 = Pn; a0 = an; b0 = bn;
+        continue;
+      }
+      if (j == N - 3) {
+        Pd = Pn; ad = an; bd = bn;
+        const int k = N - 2;
+        const double dtau0 = 0.5 * zmu * (Pu.chi + P0.chi) * DZ(N - 1);
+        I_upw = B2[1] - (B2[0] - B2[1]) / dtau0;
+        Iu = I_upw;
+        dsup = DZ(k + 1) * zmu;
+        const double dsdn = DZ(k) * zmu;
+        dchi_up = (P0.chi - Pu.chi) / dsup;
+        dchi_c = cent_deriv<R>(dsup, dsdn, Pu.chi, P0.chi, Pd.chi);
+        const double c1 = rmax0(P0.chi - (dsup / 3.0) * dchi_c);
+        const double c2 = rmax0(Pu.chi + (dsup / 3.0) * dchi_up);
+        dtau_uw = dsup * (P0.chi + Pu.chi + c1 + c2) * 0.25;
+        dS_up = (P0.S[0] - Pu.S[0]) / dtau_uw;
+        dI_upw = 0.0;
+        const double Zk = -a0 / P0.chi * IIN(k) + b0 / P0.chi;
+        const double Zkm1 = -au / Pu.chi * Iu + bu / Pu.chi;
+        dZup = (Zk - Zkm1) / dtau_uw;
+        continue;
+      }
+      const int k = j + 2;
+      const double dsdn = DZ(k) * zmu;
+      double dchi_dn;
+      if (j >= 0) {
+        const double dsdn2 = DZ(k - 1) * zmu;
+        dchi_dn = cent_deriv<R>(dsdn, dsdn2, P0.chi, Pd.chi, Pn.chi);
+      } else {
+        dchi_dn = (Pd.chi - P0.chi) / dsdn;
+      }
+      double c1 = rmax0(P0.chi + (dsdn / 3.0) * dchi_c);
+      double c2 = rmax0(Pd.chi - (dsdn / 3.0) * dchi_dn);
+      const double dtau_dw = dsdn * (P0.chi + Pd.chi + c1 + c2) * 0.25;
+      const double dt03 = dtau_uw / 3.0;
+      const Bezier3<R> bz = bezier3(dtau_uw);
+      const double dS_c = cent_deriv<R>(dtau_uw, dtau_dw, Pu.S[0], P0.S[0], Pd.S[0]);
+      c1 = rmax0(P0.S[0] - dt03 * dS_c);
+      c2 = rmax0(Pu.S[0] + dt03 * dS_up);
+      const double I = I_upw * bz.eps + bz.alpha * P0.S[0] + bz.beta * Pu.S[0] + bz.gamma * c1 +
+                       bz.theta * c2;
+      // the response at k, with the inward ray's intensity at k - 1
+      const double Zk = (-a0 * I + b0) / P0.chi;
+      const double Zkm1 = (-au * Iu + bu) / Pu.chi;
+      const double Zkp1 = (-ad * IIN(k - 1) + bd) / Pd.chi;
+      const double dZk = cent_deriv<R>(dtau_uw, dtau_dw, Zkm1, Zk, Zkp1);
+      c1 = rmax0(Zk - dt03 * dZk);
+      c2 = rmax0(Zkm1 + dt03 * dZup);
+      const double dI = dI_upw * bz.eps + bz.alpha * Zk + bz.beta * Zkm1 + bz.gamma * c1 +
+                        bz.theta * c2;
+      I_upw = I;
+      Iu = I;
+      dI_upw = dI;
+      dZup = dZk;
+      Pu = P0; au = a0; bu = b0;
+      P0 = Pd; a0 = ad; b0 = bd;
+      if (j >= 0) { Pd = Pn; ad = an; bd = bn; }
+      dsup = dsdn;
+      dchi_up = dchi_c;
+      dchi_c = dchi_dn;
+      dtau_uw = dtau_dw;
+      dS_up = dS_c;
+    }
+    double w[3];
+    const double dtl = 0.5 * zmu * (P0.chi + Pu.chi) * DZ(1);
+    const double dS_uw = -(P0.S[0] - Pu.S[0]) / dtl;
+    w3(dtl, w);
+    const double I0 = (1.0 - w[0]) * I_upw + w[0] * P0.S[0] + w[1] * dS_uw;
+    const double Zk = a0 / P0.chi * I0 - b0 / P0.chi;
+    const double Zkm1 = au / Pu.chi * Iu - bu / Pu.chi;
+    const double dZk = -(Zk - Zkm1) / dtl;
+    rf[((int64_t) ic * nout + io) * npar + p] = (1.0 - w[0]) * dI_upw + w[0] * Zk + w[1] * dZk;
+  }
+#undef IIN
+#undef POINT_IN
+}
+
 #undef POINT
 #undef Z
 #undef DZ
 #undef RAY_SETUP
+#undef RAY_SETUP_BASE
 
 #define RHG_KFORMAL(KNAME, R)                                                                 \
   template __global__ void KNAME<R>(const DevModel *, const int32_t *, const int32_t *, int, int, \

@@ -102,11 +102,55 @@ __host__ __device__ __forceinline__ int state_nfield(int ncont, int nline) {
   return RHG_CF_NFIXED + ncont + 4 * nline + 1;
 }
 
+// Profile of entry e (line l) of wavelength ls's line list at one depth,
+// RLKProfile rh/kurucz.c:729-827 for the emergent ray (to_obs = 1), in the
+// number type T (double, float, or a dual number of either).  F(field)
+// returns a column-state field at that depth as a T; the Doppler offset
+// (lambda/lambda0 - 1) c comes from the model in FP64.  Returns phi;
+// phi_Q,U,V are set for a polarizable line with damping.
+template <typename T, typename Field>
+__device__ __forceinline__ T line_profile_f(const DevModel &M, int e, int l, const Field &F, T mu,
+                                            T vz, int moving, T &phi_Q, T &phi_U, T &phi_V) {
+  const int lb = line_field_base(M.ncont);
+  const T adamp = F(lb + 4 * l + 2), ivb = F(lb + 4 * l + 3);
+  T v = T(M.lam_line_dv[e]) * ivb;
+  if (moving) v += (mu * vz) * ivb;
+  const T sv = ivb * T(INV_SQRTPI);
+  T phi;
+  if (M.line_Grad[l] != 0.0) {
+    if (M.line_polar[l] != 0) {
+      const T cosg = F(RHG_CF_COSG);
+      const T sin2_gamma = T(1.0) - cosg * cosg;
+      const T vB = T(M.line_larmor[l]) * F(RHG_CF_B) * ivb;
+      T phi_sm = T(0), phi_pi = T(0), phi_sp = T(0), H, Fv;
+      const int c0 = M.line_comp_ptr[l], c1 = M.line_comp_ptr[l + 1];
+      for (int nz = c0; nz < c1; nz++) {
+        H = voigt_humlicek<T>(adamp, v - T(M.comp_shift[nz]) * vB, &Fv);
+        const int q = M.comp_q[nz];
+        const T sH = T(M.comp_strength[nz]) * H;
+        if (q == -1) phi_sm += sH;
+        else if (q == 0) phi_pi += sH;
+        else phi_sp += sH;
+      }
+      const T phi_sigma = phi_sp + phi_sm;
+      const T phi_delta = T(0.5) * phi_pi - T(0.25) * phi_sigma;
+      phi = (phi_delta * sin2_gamma + T(0.5) * phi_sigma) * sv;
+      phi_Q = phi_delta * sin2_gamma * F(RHG_CF_COS2X) * sv;
+      phi_U = phi_delta * sin2_gamma * F(RHG_CF_SIN2X) * sv;
+      phi_V = T(0.5) * (phi_sp - phi_sm) * cosg * sv;
+    } else {
+      phi = voigt_armstrong_t(adamp, v) * sv;
+    }
+  } else {
+    phi = (fabs(v) <= T(MAX_GAUSS_DOPPLER)) ? rexp(-v * v) : T(0);
+    phi = phi * sv;
+  }
+  return phi;
+}
+
 // Kurucz-line opacity and emissivity of the emergent ray (to_obs = 1) at
-// wavelength ls and one depth, rh/kurucz.c:600-715 and RLKProfile
-// rh/kurucz.c:729-827, in the number type T (double, float, or a dual number
-// of either).  F(field) returns a column-state field at that depth as a T.
-// The Doppler offset (lambda/lambda0 - 1) c comes from the model in FP64.
+// wavelength ls and one depth, rh/kurucz.c:600-715: the sum over the
+// wavelength's line list of line_profile_f times the line's chi_l, eta_l.
 // chiI/etaI and chiQ[3]/etaQ[3] accumulate from zero.
 template <typename T, typename Field>
 __device__ __forceinline__ void line_point_f(const DevModel &M, int ls, const Field &F, T mu,
@@ -120,44 +164,12 @@ __device__ __forceinline__ void line_point_f(const DevModel &M, int ls, const Fi
   for (int e = p0; e < p1; e++) {
     const int l = M.lam_line_idx[e];
     const T chi_l = F(lb + 4 * l), eta_l = F(lb + 4 * l + 1);
-    const T adamp = F(lb + 4 * l + 2), ivb = F(lb + 4 * l + 3);
-    T v = T(M.lam_line_dv[e]) * ivb;
-    if (moving) v += (mu * vz) * ivb;
-    const T sv = ivb * T(INV_SQRTPI);
-    T phi, phi_Q = T(0), phi_U = T(0), phi_V = T(0);
-    const bool polar = M.line_polar[l] != 0;
-    if (M.line_Grad[l] != 0.0) {
-      if (polar) {
-        const T cosg = F(RHG_CF_COSG);
-        const T sin2_gamma = T(1.0) - cosg * cosg;
-        const T vB = T(M.line_larmor[l]) * F(RHG_CF_B) * ivb;
-        T phi_sm = T(0), phi_pi = T(0), phi_sp = T(0), H, Fv;
-        const int c0 = M.line_comp_ptr[l], c1 = M.line_comp_ptr[l + 1];
-        for (int nz = c0; nz < c1; nz++) {
-          H = voigt_humlicek<T>(adamp, v - T(M.comp_shift[nz]) * vB, &Fv);
-          const int q = M.comp_q[nz];
-          const T sH = T(M.comp_strength[nz]) * H;
-          if (q == -1) phi_sm += sH;
-          else if (q == 0) phi_pi += sH;
-          else phi_sp += sH;
-        }
-        const T phi_sigma = phi_sp + phi_sm;
-        const T phi_delta = T(0.5) * phi_pi - T(0.25) * phi_sigma;
-        phi = (phi_delta * sin2_gamma + T(0.5) * phi_sigma) * sv;
-        phi_Q = phi_delta * sin2_gamma * F(RHG_CF_COS2X) * sv;
-        phi_U = phi_delta * sin2_gamma * F(RHG_CF_SIN2X) * sv;
-        phi_V = T(0.5) * (phi_sp - phi_sm) * cosg * sv;
-      } else {
-        phi = voigt_armstrong_t(adamp, v) * sv;
-      }
-    } else {
-      phi = (fabs(v) <= T(MAX_GAUSS_DOPPLER)) ? rexp(-v * v) : T(0);
-      phi = phi * sv;
-    }
+    T phi_Q = T(0), phi_U = T(0), phi_V = T(0);
+    const T phi = line_profile_f<T>(M, e, l, F, mu, vz, moving, phi_Q, phi_U, phi_V);
     if (phi != T(0)) {
       chiI += chi_l * phi;
       etaI += eta_l * phi;
-      if (polar && M.line_Grad[l] != 0.0) {
+      if (M.line_polar[l] != 0 && M.line_Grad[l] != 0.0) {
         chiQ[0] += chi_l * phi_Q;
         chiQ[1] += chi_l * phi_U;
         chiQ[2] += chi_l * phi_V;
@@ -225,6 +237,11 @@ __global__ void k_formal_scalar(const DevModel *M, const int32_t *ndep, const in
                                 int ncol, int Nmax, const double *state, const int32_t *out_ls,
                                 const int32_t *io_list, int nlist, int nout, const R *cchi,
                                 const R *ceta, double mu, double *stokes);
+__global__ void k_rf_loggf(const DevModel *M, const int32_t *ndep, const int32_t *moving, int ncol,
+                           int Nmax, const double *state, const int32_t *out_ls,
+                           const int32_t *io_list, int nlist, int nout, const double *cchi,
+                           const double *ceta, double mu, int npar, const int32_t *par_line,
+                           double *Iin, double *rf);
 constexpr int NCARRY = 15;   // carried scalars of the DELO recursion state
 __global__ void k_rf_perturb(const double *atmos, double *out, int64_t n, int Nmax, int row,
                              double delta);

@@ -67,6 +67,8 @@ def lib():
         L.rhg_count_work.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
         L.rhg_rf_lte.argtypes = [C.c_void_p, C.POINTER(RhgColumns), C.c_int32, C.c_int32, PD,
                                  C.c_void_p]
+        L.rhg_rf_loggf.argtypes = [C.c_void_p, C.POINTER(RhgColumns), C.c_int32,
+                                   C.POINTER(C.c_int32), C.c_int32, PD, PD]
         if L.rhg_model_sizeof() != C.sizeof(RhgModel) or \
                 L.rhg_columns_sizeof() != C.sizeof(RhgColumns):
             raise EngineError("rhg_model_t / rhg_columns_t layout mismatch between the "
@@ -147,6 +149,19 @@ class Engine:
         _check(lib().rhg_rf_lte(self._h, cs.ptr(), int(precision), 0, _pd(out),
                                 rf.ctypes.data_as(C.c_void_p)))
         return out, np.ascontiguousarray(np.moveaxis(rf, -1, 0))
+
+    def rf_loggf(self, atmos, par_line, mu=1.0, atm_scale=0, ndep=None):
+        """Stokes [ncol, 4, nout] and d I / d log gf [ncol, nout, npar] (the
+        reference's analytic atomic response functions) for host atmospheres
+        [ncol, 9, ndep_max]; par_line from pyrh_amd.model.atomic_rf_lines."""
+        cs = ColumnsStruct(atmos, ndep, atm_scale, mu)
+        ncol = cs.struct.ncol
+        par = np.ascontiguousarray(par_line, dtype=np.int32)
+        out = np.empty((ncol, 4, self.nout))
+        rf = np.empty((ncol, self.nout, par.size))
+        _check(lib().rhg_rf_loggf(self._h, cs.ptr(), int(par.size),
+                                  par.ctypes.data_as(C.POINTER(C.c_int32)), 0, _pd(out), _pd(rf)))
+        return out, rf
 
     def rf_lte_device(self, d_atmos, d_ndep, ncol, ndep_max, d_out, d_rf, mu=1.0, atm_scale=0,
                       precision=0):

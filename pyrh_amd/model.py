@@ -640,6 +640,29 @@ def _check_supported(setup):
         raise NotImplementedError("NON_ICE = TRUE")
 
 
+def atomic_rf_lines(setup: RunSetup, loggf_ids=None, nlam=0) -> np.ndarray:
+    """Model line (index into setup.rlk) of each atomic response-function
+    parameter, in pyrh's order: the log gf ids, then the nlam wavelength-shift
+    ids (pyrh.pyx:658-660).  -1 where the reference fills nothing: a line
+    takes the last log gf id that names its file-local index
+    (rh/kurucz.c:241-251), and the wavelength-shift entries are never filled
+    (rh/kurucz.c:225-232)."""
+    ids = [] if loggf_ids is None else [int(x) for x in loggf_ids]
+    par = np.full(len(ids) + int(nlam), -1, np.int32)
+    for l, r in enumerate(setup.rlk):
+        last = None
+        for idl, lid in enumerate(ids):
+            if lid == r.line_index:
+                last = idl
+        if last is not None:
+            if par[last] != -1:
+                raise NotImplementedError(
+                    f"log gf id {ids[last]} names a line in more than one Kurucz file; the "
+                    "reference's response then depends on the order of its opacity loop")
+            par[last] = l
+    return par
+
+
 def build_model(setup: RunSetup, wave) -> Model:
     """Wavelength grid and all column-independent tables for LTE synthesis."""
     _check_supported(setup)

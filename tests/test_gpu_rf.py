@@ -54,6 +54,48 @@ def _fd_errors(eng, atm, rf, param, h):
     return np.concatenate(errs)
 
 
+def _one_sided_errors(eng, atm, rf, param, h):
+    """Smallest of the errors against the central and the two one-sided
+    second-order differences (points x-2h..x, x-h..x+h, x..x+2h), relative
+    as in _fd_errors.  A kink or jump of the model between x and x+2h spoils
+    only the forward and central stencils, one between x-2h and x only the
+    backward and central ones."""
+    ip = RF_PARAMS.index(param)
+    errs = []
+    for k in DEPTHS:
+        f = {}
+        for m in (-2, -1, 0, 1, 2):
+            a = atm.copy(); a[:, RF_ROWS[ip], k] += m * h
+            f[m] = eng.synth_lte(a)
+        stencils = ((f[1] - f[-1]) / (2 * h),
+                    (-3 * f[0] + 4 * f[1] - f[2]) / (2 * h),
+                    (3 * f[0] - 4 * f[-1] + f[-2]) / (2 * h))
+        ad = rf[:, ip, k]
+        scale = np.max(np.abs(stencils[0]), axis=(1, 2), keepdims=True)
+        ok = scale[:, 0, 0] > 0
+        e = np.min([np.abs(ad - d) for d in stencils], axis=0) / np.maximum(scale, 1e-300)
+        errs.append(e[ok].ravel())
+    return np.concatenate(errs)
+
+
+@pytest.mark.parametrize("param", RF_PARAMS)
+def test_rf_one_sided_differences(rf_case, param):
+    """Convergence of the best of three stencils: near a kink or jump of the
+    model one stencil is usually smooth, and the share of points where none
+    matches the adjoint falls in proportion to the step (the density of
+    non-smooth points per unit parameter is fixed).  An error of the adjoint
+    at a fixed set of points would not shrink with the step."""
+    eng, atm, _, rf = rf_case
+    h = STEP_FINE[param]
+    e1 = _one_sided_errors(eng, atm, rf, param, h)
+    e2 = _one_sided_errors(eng, atm, rf, param, h / 4)
+    s1, s2 = float(np.mean(e1 > 1e-4)), float(np.mean(e2 > 1e-4))
+    print(f"{param}: best-stencil share>1e-4 {s1:.5f} at h={h}, {s2:.5f} at h/4; "
+          f"p99 {np.percentile(e2, 99):.2e} at h/4", flush=True)
+    assert s2 <= 0.5 * s1 + 2e-4
+    assert s2 < 0.02
+
+
 def test_rf_stokes_equal_synthesis(rf_case):
     eng, atm, st, _ = rf_case
     np.testing.assert_allclose(st, eng.synth_lte(atm), rtol=1e-13, atol=0)

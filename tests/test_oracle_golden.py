@@ -31,6 +31,31 @@ def test_oracle_matches_reference(fixtures, setups, name):
     assert np.max(np.abs(st[:, 1:] - ref[:, 1:]) / Ic[:, None]) <= 1e-13
 
 
+@pytest.mark.parametrize("name", ["rf_nostokes", "overrides"])
+def test_oracle_loggf_rf_matches_reference(fixtures, setups, name):
+    """The oracle's d I / d log gf against the reference's own output
+    (NO_STOKES: the scalar Bezier fills it; FULL_STOKES: all zero).  Solving
+    the inward ray on its own Doppler-shifted profile instead of the emergent
+    ray's background, as pyrh does, misses these by 1e-6."""
+    from pyrh_amd import model
+    from conftest import override_kwargs
+    F = fixtures
+    m = golden_model(F, setups, name)
+    kw = override_kwargs(F, name)
+    par = model.atomic_rf_lines(m.setup, kw.get("loggf_ids"), len(kw.get("lam_ids", [])))
+    st, rf = pyoracle.rf_loggf(m, F[name + "__atmos"], par, mu=float(F[name + "__mu"]),
+                               atm_scale=int(F[name + "__scale"]))
+    ref = F[name + "__rf_loggf"]
+    ng = ref.shape[2]
+    big = np.max(np.abs(ref), axis=1, keepdims=True)
+    err = np.abs(rf[:, :, :ng] - ref) / np.where(big > 0, big, 1e-300)
+    assert err.max() <= 1e-13
+    assert np.all(rf[:, :, ng:] == 0.0)
+    np.testing.assert_array_equal(st, pyoracle.synth_lte(m, F[name + "__atmos"],
+                                                         mu=float(F[name + "__mu"]),
+                                                         atm_scale=int(F[name + "__scale"])))
+
+
 def test_fixture_cases_cover_every_solver(fixtures, setups):
     """Stokes DELO-Bezier, scalar Bezier and Feautrier all occur."""
     from pyrh_amd import model
