@@ -191,6 +191,52 @@ typedef struct rhg_columns {
   double mu;
 } rhg_columns_t;
 
+/* One active (NLTE) atom with its radiative transitions on the merged
+ * wavelength grid of the model it is solved with, as pyrh's rhf1d() sets it
+ * up for the iteration (rh/readatom.c:62-615, rh/getlambda.c:42-220,
+ * rh/sortlambda.c:42-560, rh/collision.c:450-760, rh/broad.c:34-316,
+ * rh/gaussleg.c:30-64); built by pyrh_amd.nlte.build_nlte.  The model passed
+ * to rhg_open for the NLTE solve is the background model on the same grid,
+ * without the active atom's continua.  Transitions are ordered continua
+ * first, then lines, as the active sets list them. */
+typedef struct {
+  int32_t nlevel;           /* levels incl. the continuum level            */
+  int32_t atom_index;       /* the atom in the model's atom list           */
+  int32_t he_atom;          /* index of He in the model's atoms, -1: none  */
+  const double *E, *g;      /* [nlevel] energies [J], statistical weights  */
+  const int32_t *stage;     /* [nlevel] ionisation stage                   */
+  double weight;            /* atomic weight [amu]                         */
+  int32_t ntrans;
+  const int32_t *tr_type;   /* 0 line, 1 continuum                         */
+  const int32_t *tr_i, *tr_j;          /* lower, upper level               */
+  const int32_t *tr_nblue, *tr_nlam;   /* span of the merged grid          */
+  const int32_t *tr_ptr;    /* [ntrans+1] offsets into tr_wlam/tr_alpha    */
+  const int32_t *tr_line;   /* line index of a line transition, else -1    */
+  const double *tr_wlam;    /* quadrature weight per point of the span:
+                               lines in Doppler units (rh/getlambda.c:185),
+                               continua in nm (rh/getlambda.c:205)         */
+  const double *tr_alpha;   /* continuum cross-section per point [m^2]     */
+  int32_t nline;
+  const double *ln_lambda0, *ln_Aji, *ln_Bji, *ln_Bij, *ln_Grad;
+  const int32_t *ln_vdw;    /* 0 none, 1 Unsold                            */
+  const double *ln_vdw_cross;
+  const int32_t *ln_stark;  /* 0 none, 1 |c| ne, 2 quadratic (Traving)     */
+  const double *ln_stark_c, *ln_stark_C, *ln_stark_Cm;
+  const int32_t *as_ptr;    /* [nspect+1] active sets over the model grid  */
+  const int32_t *as_tr;     /* transition indices                          */
+  const int32_t *as_bezier; /* [nspect] 1: scalar Bezier rays, 0: Feautrier */
+  int32_t ncoll;
+  const int32_t *coll_type; /* OMEGA CE CI CP CH CH0 CH+ = 0..6             */
+  const int32_t *coll_i, *coll_j, *coll_ptr;
+  const double *coll_T, *coll_c, *coll_M;  /* table, values, spline M      */
+  int32_t nrays;
+  const double *muz, *wmu;  /* Gauss-Legendre rays on (0, 1)               */
+  const double *ray_H, *ray_He, *ray_H2;   /* [nspect] Rayleigh sigma      */
+  double sigma_thomson;
+  int32_t nmax_iter, nmax_scatter, ng_order, ng_delay, ng_period, isum;
+  double iter_limit;
+} rhg_nlte_t;
+
 typedef struct rhg_context rhg_context_t;
 
 /* Returns the last error message on this thread ("" if none). */

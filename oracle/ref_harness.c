@@ -233,6 +233,19 @@ void Piece_Stokes_Bezier3_1D(int nspect, int mu, bool_t to_obs, double *chi,
   for (n = 0; n < 4; n++) rec("fs_I", nspect, mu, n, I[n], N);
 }
 
+double Feautrier(int nspect, int mu, double *chi, double *S, enum FeautrierOrder F_order,
+                 double *P, double *Psi)
+{
+  REAL(Feautrier);
+  int N = atmos.Nspace;
+  double r = real_Feautrier(nspect, mu, chi, S, F_order, P, Psi);
+  rec("fe_chi", nspect, mu, 0, chi, N);
+  rec("fe_S", nspect, mu, 0, S, N);
+  rec("fe_P", nspect, mu, 0, P, N);
+  if (Psi) rec("fe_Psi", nspect, mu, 0, Psi, N);
+  return r;
+}
+
 void Piecewise_Bezier3_1D(int nspect, int mu, bool_t to_obs, double *chi,
                           double *S, double *I, double *Psi, double **dI)
 {
@@ -243,6 +256,84 @@ void Piecewise_Bezier3_1D(int nspect, int mu, bool_t to_obs, double *chi,
   rec("fs1_chi", nspect, mu, to_obs, chi, N);
   rec("fs1_S", nspect, mu, to_obs, S, N);
   rec("fs1_I", nspect, mu, to_obs, I, N);
+}
+
+/* --- NLTE: set-up and iteration intermediates ------------------------ */
+void CollisionRate(struct Atom *atom, FILE *fp_atom)
+{
+  REAL(CollisionRate);
+  int ij;
+  real_CollisionRate(atom, fp_atom);
+  if (!atom->active) return;
+  for (ij = 0; ij < atom->Nlevel * atom->Nlevel; ij++)
+    rec("coll_C", ij, 0, 0, atom->C[ij], atmos.Nspace);
+  for (ij = 0; ij < atom->Nlevel; ij++)
+    rec("nlte_nstar", ij, 0, 0, atom->nstar[ij], atmos.Nspace);
+}
+
+void Damping(AtomicLine *line, double *adamp)
+{
+  REAL(Damping);
+  real_Damping(line, adamp);
+  if (line->atom->active) rec("adamp", (int) (line - line->atom->line), 0, 0, adamp, atmos.Nspace);
+}
+
+void getProfiles(void)
+{
+  REAL(getProfiles);
+  int n, kr;
+  real_getProfiles();
+  for (n = 0; n < atmos.Nactiveatom; n++) {
+    Atom *a = atmos.activeatoms[n];
+    for (kr = 0; kr < a->Nline; kr++) {
+      rec("wphi", kr, 0, 0, a->line[kr].wphi, atmos.Nspace);
+      rec("phi0", kr, 0, 0, a->line[kr].phi[0], atmos.Nspace);
+    }
+  }
+}
+
+void initScatter(void)
+{
+  REAL(initScatter);
+  int n, i;
+  for (n = 0; n < atmos.Nactiveatom; n++)
+    for (i = 0; i < atmos.activeatoms[n]->Nlevel; i++)
+      rec("n_init", i, 0, 0, atmos.activeatoms[n]->n[i], atmos.Nspace);
+  real_initScatter();
+  if (spectrum.J)
+    for (n = 0; n < spectrum.Nspect; n++) rec("J_scat", n, 0, 0, spectrum.J[n], atmos.Nspace);
+}
+
+double updatePopulations(int niter)
+{
+  REAL(updatePopulations);
+  int n, ij;
+  for (n = 0; n < atmos.Nactiveatom && niter == 1; n++) {
+    Atom *a = atmos.activeatoms[n];
+    for (ij = 0; ij < a->Nlevel * a->Nlevel; ij++)
+      rec("gamma1", ij, 0, 0, a->Gamma[ij], atmos.Nspace);
+  }
+  double d = real_updatePopulations(niter), r[2];
+  r[0] = niter;
+  r[1] = d;
+  rec("iter", -1, 0, 0, r, 2);
+  return d;
+}
+
+/* called once the iterations are over (rh/rhf1d/pyrh_compute1dray.c:352) */
+void _solveray(double muz, mySpectrum *spec)
+{
+  REAL(_solveray);
+  int n, i;
+  for (n = 0; n < atmos.Nactiveatom; n++) {
+    Atom *a = atmos.activeatoms[n];
+    for (i = 0; i < a->Nlevel; i++) {
+      rec("pops_n", n, i, 0, a->n[i], atmos.Nspace);
+      rec("pops_nstar", n, i, 0, a->nstar[i], atmos.Nspace);
+    }
+  }
+  rec("lambda", -1, 0, 0, spectrum.lambda, spectrum.Nspect);
+  real__solveray(muz, spec);
 }
 
 /* ------------------------------------------------------------------- */

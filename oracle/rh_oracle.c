@@ -13,6 +13,7 @@
  * source it follows.
  */
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -39,6 +40,8 @@
 #define PI          3.14159265358979
 #define SQRTPI      1.77245385090551
 #define LG10        2.30258509299404568402
+#define RBOHR       5.29177349E-11
+#define E_RYDBERG   2.1798741E-18
 #define SQ(x)       ((x)*(x))
 #define CUBE(x)     ((x)*(x)*(x))
 #define POW10(x)    exp(LG10 * (x))
@@ -1351,11 +1354,16 @@ static int line_opacity(const rhg_model_t *m, const column_t *c, int ls,
   return line_opacity_sel(m, c, ls, mu, chi, eta, -1);
 }
 
+struct nlte_out { double *pops; int *niter; double *dpops; };
+static int nlte_column(const rhg_model_t *m, const rhg_nlte_t *na, const column_t *c,
+                       struct nlte_out *o);
+
 static int synth_column_ex(const rhg_model_t *m, const rhg_columns_t *cols, int ic,
                            int fp32, double *stokes, double *height_out,
                            double *chiref_out, int diag_ls, double *diag_chi,
                            double *diag_eta, double *diag_pop, int npar,
-                           const int *par_line, double *rf)
+                           const int *par_line, double *rf, const rhg_nlte_t *na,
+                           struct nlte_out *no)
 {
   const int N = cols->ndep[ic], Nmax = cols->ndep_max, nspect = m->nspect;
   const int nlev = m->atom_level_ptr[m->natom], natom = m->natom;
@@ -1442,6 +1450,20 @@ static int synth_column_ex(const rhg_model_t *m, const rhg_columns_t *cols, int 
   /* background at the reference wavelength, then convertScales
      (rh/rhf1d/multiatmos.c:100-177) */
   background_continuum(m, &c, m->ref_index, chiref, eta_ai);
+  if (na) {
+    /* solving NLTE, the background opacity includes the scattering
+       opacity (rh/background.c:461-463), also at the reference wavelength
+       that sets the depth scale */
+    const int lr = m->ref_index, nlev_all = m->atom_level_ptr[m->natom];
+    for (k = 0; k < N; k++) {
+      double sca = c.ne[k] * na->sigma_thomson;
+      sca += na->ray_H[lr] * c.nH0[k];
+      if (na->he_atom >= 0)
+        sca += na->ray_He[lr] * c.nstar[(size_t) k * nlev_all + m->atom_level_ptr[na->he_atom]];
+      sca += na->ray_H2[lr] * c.nH2[k];
+      chiref[k] += sca;
+    }
+  }
   if (line_opacity(m, &c, m->ref_index, mu, lchi, leta))
     for (k = 0; k < N; k++) chiref[k] += lchi[k];
   {
@@ -1493,6 +1515,12 @@ static int synth_column_ex(const rhg_model_t *m, const rhg_columns_t *cols, int 
       for (k = 0; k < 4 * N; k++) { diag_chi[4 * N + k] = 0.0; diag_eta[4 * N + k] = 0.0; }
     free(mem);
     return 0;
+  }
+
+  if (na) {
+    int rc = nlte_column(m, na, &c, no);
+    free(mem);
+    return rc;
   }
 
   /* formal solution of the emergent ray per wavelength (formal.c:157-275) */
@@ -1560,7 +1588,7 @@ static int synth_column(const rhg_model_t *m, const rhg_columns_t *cols, int ic,
                         double *chiref_out)
 {
   return synth_column_ex(m, cols, ic, fp32, stokes, height_out, chiref_out,
-                         -1, NULL, NULL, NULL, 0, NULL, NULL);
+                         -1, NULL, NULL, NULL, 0, NULL, NULL, NULL, NULL);
 }
 
 /* Diagnostic: continuum (chi[0:N], eta[0:N]) and line (chi/eta[4N:8N])
@@ -1570,7 +1598,7 @@ int rho_background(const rhg_model_t *m, const rhg_columns_t *cols, int ic, int 
                    double *chi8, double *eta8, double *pop6)
 {
   return synth_column_ex(m, cols, ic, 0, NULL, NULL, NULL, ls, chi8, eta8, pop6, 0, NULL,
-                         NULL);
+                         NULL, NULL, NULL);
 }
 
 int rho_synth_lte(const rhg_model_t *m, const rhg_columns_t *cols,
@@ -1613,7 +1641,727 @@ int rho_rf_loggf(const rhg_model_t *m, const rhg_columns_t *cols, int npar,
   for (ic = 0; ic < cols->ncol; ic++) {
     if (synth_column_ex(m, cols, ic, 0, stokes_out + (size_t) ic * 4 * nout, NULL, NULL, -1,
                         NULL, NULL, NULL, npar, par_line,
-                        rf_out + (size_t) ic * nout * npar))
+                        rf_out + (size_t) ic * nout * npar, NULL, NULL))
+      err |= 1;
+  }
+  (void) nthreads;
+  return err ? RHG_ENOMEM : RHG_OK;
+}
+
+
+/* ------------------------------------------------------------------ */
+/* NLTE: one active atom, accelerated lambda iteration as rhf1d() runs */
+/* it (rh/rhf1d/pyrh_compute1dray.c:280-352, rh/iterate.c:48-143).     */
+
+static double m_lambda(const rhg_model_t *m, int ls)
+{
+  return m->lamconst[(size_t) ls * RHG_LC_NCONST + RHG_LC_LAMBDA];
+}
+
+/* RH's splineEval with the table's second derivatives M precomputed
+   (rh/spline.c:70-97) */
+static double spline_eval_M(int n, const double *x, const double *y, const double *M,
+                            double xv)
+{
+  int j;
+  double hj, fx, fx1;
+  if (xv <= x[0]) return y[0];
+  if (xv >= x[n - 1]) return y[n - 1];
+  j = locate_asc(n, x, xv);
+  hj = x[j + 1] - x[j];
+  fx = (xv - x[j]) / hj;
+  fx1 = 1 - fx;
+  return fx1 * y[j] + fx * y[j + 1] +
+         (fx1 * (SQ(fx1) - 1) * M[j] + fx * (SQ(fx) - 1) * M[j + 1]) * SQ(hj) / 6.0;
+}
+
+/* Scalar cubic Bezier along one ray with the diagonal operator
+   (rh/rhf1d/bezier_1D.c:306-541): to_obs = 1 from the thermalised bottom
+   (B2 = B(T[N-2]), B(T[N-1])), to_obs = 0 from a dark top.  I and Psi at
+   every depth. */
+static void bezier_ray(int Ndep, const double *z, double zmu, const double *chi,
+                       const double *S, const double *B2, int to_obs, double *I, double *Psi)
+{
+  int dk, k_start, k_end, k;
+  double dtau_uw, dtau_dw = 0.0, dS_uw, I_upw, c1, c2, w[3], dsup, dsdn, dt03, co[5], dS_up,
+         dS_c, dchi_up, dchi_c, dchi_dn = 0.0, dsdn2;
+  if (to_obs) { dk = -1; k_start = Ndep - 1; k_end = 0; }
+  else        { dk = 1;  k_start = 0;        k_end = Ndep - 1; }
+  dtau_uw = 0.5 * zmu * (chi[k_start] + chi[k_start + dk]) *
+            fabs(z[k_start] - z[k_start + dk]);
+  I_upw = to_obs ? B2[1] - (B2[0] - B2[1]) / dtau_uw : 0.0;
+  I[k_start] = I_upw;
+  Psi[k_start] = 0.0;
+  k = k_start + dk;
+  dsup = fabs(z[k] - z[k - dk]) * zmu;
+  dsdn = fabs(z[k + dk] - z[k]) * zmu;
+  dchi_up = (chi[k] - chi[k - dk]) / dsup;
+  dchi_c = cent_deriv(dsup, dsdn, chi[k - dk], chi[k], chi[k + dk]);
+  c1 = MAXV(chi[k] - (dsup / 3.0) * dchi_c, 0.0);
+  c2 = MAXV(chi[k - dk] + (dsup / 3.0) * dchi_up, 0.0);
+  dtau_uw = dsup * (chi[k] + chi[k - dk] + c1 + c2) * 0.25;
+  dS_up = (S[k] - S[k - dk]) / dtau_uw;
+  for (k = k_start + dk; k != k_end + dk; k += dk) {
+    if (k != k_end) {
+      dsdn = fabs(z[k + dk] - z[k]) * zmu;
+      if (abs(k - k_end) > 1) {
+        dsdn2 = fabs(z[k + 2 * dk] - z[k + dk]) * zmu;
+        dchi_dn = cent_deriv(dsdn, dsdn2, chi[k], chi[k + dk], chi[k + 2 * dk]);
+      } else
+        dchi_dn = (chi[k + dk] - chi[k]) / dsdn;
+      c1 = MAXV(chi[k] + (dsdn / 3.0) * dchi_c, 0.0);
+      c2 = MAXV(chi[k + dk] - (dsdn / 3.0) * dchi_dn, 0.0);
+      dtau_dw = dsdn * (chi[k] + chi[k + dk] + c1 + c2) * 0.25;
+      dt03 = dtau_uw / 3.0;
+      rho_bezier3_coeffs(dtau_uw, co);
+      dS_c = cent_deriv(dtau_uw, dtau_dw, S[k - dk], S[k], S[k + dk]);
+      c1 = MAXV(S[k] - dt03 * dS_c, 0.0);
+      c2 = MAXV(S[k - dk] + dt03 * dS_up, 0.0);
+      I[k] = I_upw * co[4] + co[0] * S[k] + co[1] * S[k - dk] + co[2] * c1 + co[3] * c2;
+      Psi[k] = co[0] + co[2];
+    } else {
+      dtau_uw = 0.5 * zmu * (chi[k] + chi[k - dk]) * fabs(z[k] - z[k - dk]);
+      dS_uw = -(S[k] - S[k - dk]) / dtau_uw;
+      rho_w3(dtau_uw, w);
+      I[k] = (1.0 - w[0]) * I_upw + w[0] * S[k] + w[1] * dS_uw;
+      Psi[k] = w[0] - w[1] / dtau_uw;
+    }
+    I_upw = I[k];
+    dsup = dsdn;
+    dchi_up = dchi_c;
+    dchi_c = dchi_dn;
+    dtau_uw = dtau_dw;
+    dS_up = dS_c;
+  }
+  (void) dsup;
+  (void) dchi_up;
+}
+
+/* Feautrier, STANDARD order, dark top, thermalised bottom, with P and the
+   two-sided diagonal operator at every depth (rh/rhf1d/feautrier.c:56-202) */
+static void feautrier_full(int Ndep, const double *z, double mu, const double *chi,
+                           const double *S, const double *B2, double *P, double *Psi,
+                           double *scr)
+{
+  double *dtau = scr, *abc = scr + Ndep, *A1 = scr + 2 * Ndep, *C1 = scr + 3 * Ndep,
+         *F = scr + 4 * Ndep, *G = scr + 5 * Ndep, *ztmp = scr + 6 * Ndep, *Stmp = scr + 7 * Ndep;
+  double zmu = 0.5 / mu, r0 = 0.0, h0 = 0.0, rN = 0.0, hN, f0, fN, dtau_mid;
+  int k;
+  for (k = 0; k < Ndep - 1; k++) dtau[k] = zmu * (chi[k] + chi[k + 1]) * (z[k] - z[k + 1]);
+  f0 = (1.0 - r0) / (1.0 + r0);
+  abc[0] = 1.0 + 2.0 * f0 / dtau[0];
+  C1[0] = 2.0 / SQ(dtau[0]);
+  Stmp[0] = S[0] + 2.0 * h0 / ((1.0 + r0) * dtau[0]);
+  hN = B2[1] - (B2[0] - B2[1]) / dtau[Ndep - 2];
+  fN = (1.0 - rN) / (1.0 + rN);
+  abc[Ndep - 1] = 1.0 + 2.0 * fN / dtau[Ndep - 2];
+  A1[Ndep - 1] = 2.0 / SQ(dtau[Ndep - 2]);
+  Stmp[Ndep - 1] = S[Ndep - 1] + 2.0 * hN / ((1.0 + rN) * dtau[Ndep - 2]);
+  for (k = 1; k < Ndep - 1; k++) {
+    dtau_mid = 0.5 * (dtau[k] + dtau[k - 1]);
+    A1[k] = 1.0 / (dtau_mid * dtau[k - 1]);
+    C1[k] = 1.0 / (dtau_mid * dtau[k]);
+    abc[k] = 1.0;
+    Stmp[k] = S[k];
+  }
+  F[0] = abc[0] / C1[0];
+  ztmp[0] = Stmp[0] / (abc[0] + C1[0]);
+  for (k = 1; k < Ndep - 1; k++) {
+    F[k] = (abc[k] + A1[k] * F[k - 1] / (1.0 + F[k - 1])) / C1[k];
+    ztmp[k] = (Stmp[k] + A1[k] * ztmp[k - 1]) / (C1[k] * (1.0 + F[k]));
+  }
+  P[Ndep - 1] = (Stmp[Ndep - 1] + A1[Ndep - 1] * ztmp[Ndep - 2]) /
+                (abc[Ndep - 1] + A1[Ndep - 1] * (F[Ndep - 2] / (1.0 + F[Ndep - 2])));
+  for (k = Ndep - 2; k >= 0; k--) P[k] = P[k + 1] / (1.0 + F[k]) + ztmp[k];
+  G[Ndep - 1] = abc[Ndep - 1] / A1[Ndep - 1];
+  for (k = Ndep - 2; k >= 1; k--) G[k] = (abc[k] + C1[k] * G[k + 1] / (1.0 + G[k + 1])) / A1[k];
+  Psi[0] = 1.0 / (abc[0] + C1[0] * G[1] / (1.0 + G[1]));
+  for (k = 1; k < Ndep - 1; k++)
+    Psi[k] = 1.0 / (abc[k] + A1[k] * F[k - 1] / (1.0 + F[k - 1]) + C1[k] * G[k + 1] / (1.0 + G[k + 1]));
+  Psi[Ndep - 1] = 1.0 / (abc[Ndep - 1] + A1[Ndep - 1] * F[Ndep - 2] / (1.0 + F[Ndep - 2]));
+}
+
+#define NLEV_MAX 16
+
+typedef struct {
+  const rhg_model_t *m;
+  const rhg_nlte_t *na;
+  const column_t *c;
+  int N, nl, nrays;
+  double *n, *nstar, *ntotal, *vbroad, *C, *Gamma, *J, *Jdag;
+  double *phi, *wphi;               /* per line: [la][mu][dir][k], [k] */
+  const int *phi_off;               /* line -> offset into phi */
+  double *bg_chi, *bg_eta, *bg_sca; /* [nspect][N] (active wavelengths) */
+  double *chi, *S, *I, *Psi, *eta_at, *Ieff, *scr;
+  double *V, *gij, *wla;            /* [MAX][N] per transition of the set */
+  double *chi_up, *chi_down, *Uji;  /* [nl][N] */
+  FILE *dump;                       /* first formal pass only */
+} nlte_ws;
+
+static void nlte_dump(FILE *f, const char *tag, int idx, const double *x, int n);
+
+/* Opacity of the active set at wavelength ls for ray (mu, to_obs):
+   Vij, gij and wla per transition, the set's chi and the atom's eta
+   (rh/opacity.c:62-300) */
+static void nlte_opacity(nlte_ws *w, int ls, int mu, int to_obs, int initialize)
+{
+  const rhg_nlte_t *na = w->na;
+  const double hc = HPLANCK * CLIGHT, fourPI = 4.0 * PI, hc_4PI = hc / fourPI,
+               twohc = 2.0 * hc / CUBE(NM_TO_M), hc_k = hc / (KBOLTZMANN * NM_TO_M);
+  const double *T = w->c->T;
+  const int N = w->N, p0 = na->as_ptr[ls], p1 = na->as_ptr[ls + 1];
+  int e, k;
+  for (k = 0; k < N; k++) { w->chi[k] = 0.0; w->eta_at[k] = 0.0; }
+  for (e = p0; e < p1; e++) {
+    const int n = e - p0, t = na->as_tr[e], i = na->tr_i[t], j = na->tr_j[t];
+    const int la = ls - na->tr_nblue[t];
+    const double *ni = w->n + (size_t) i * N, *nj = w->n + (size_t) j * N;
+    double *V = w->V + (size_t) n * N, *gij = w->gij + (size_t) n * N, *wla = w->wla + (size_t) n * N;
+    double twohnu3_c2;
+    if (na->tr_type[t] == 0) {
+      const int l = na->tr_line[t];
+      const double *phi = w->phi + w->phi_off[l] +
+                          ((size_t) (la * w->nrays + mu) * 2 + to_obs) * N;
+      const double gijk = na->ln_Bji[l] / na->ln_Bij[l];
+      const double Bijxhc_4PI = hc_4PI * na->ln_Bij[l] * 1.0;
+      twohnu3_c2 = na->ln_Aji[l] / na->ln_Bji[l];
+      for (k = 0; k < N; k++) { gij[k] = gijk; V[k] = Bijxhc_4PI * phi[k]; }
+      if (initialize) {
+        const double wlambda = na->tr_wlam[na->tr_ptr[t] + la];
+        for (k = 0; k < N; k++) wla[k] = wlambda * w->wphi[(size_t) l * N + k] / hc_4PI;
+      }
+    } else {
+      const double lam = m_lambda(w->m, ls);
+      twohnu3_c2 = twohc / CUBE(lam);
+      if (initialize) {
+        const double wlambda = na->tr_wlam[na->tr_ptr[t] + la];
+        const double alpha = na->tr_alpha[na->tr_ptr[t] + la];
+        for (k = 0; k < N; k++) {
+          V[k] = alpha;
+          gij[k] = w->nstar[(size_t) i * N + k] / w->nstar[(size_t) j * N + k] *
+                   exp(-hc_k / (lam * T[k]));
+          wla[k] = fourPI / HPLANCK * (wlambda / lam);
+        }
+      }
+    }
+    for (k = 0; k < N; k++) {
+      w->chi[k] += V[k] * (ni[k] - gij[k] * nj[k]);
+      w->eta_at[k] += twohnu3_c2 * gij[k] * V[k] * nj[k];
+    }
+  }
+}
+
+static double nlte_twohnu3(const nlte_ws *w, int ls, int t)
+{
+  const rhg_nlte_t *na = w->na;
+  if (na->tr_type[t] == 0) return na->ln_Aji[na->tr_line[t]] / na->ln_Bji[na->tr_line[t]];
+  {
+    const double lam = m_lambda(w->m, ls);
+    return 2.0 * HPLANCK * CLIGHT / CUBE(NM_TO_M) / CUBE(lam);
+  }
+}
+
+/* addtoCoupling, rh/fillgamma.c:251-332 */
+static void nlte_coupling(nlte_ws *w, int ls)
+{
+  const rhg_nlte_t *na = w->na;
+  const int N = w->N, p0 = na->as_ptr[ls], p1 = na->as_ptr[ls + 1];
+  int e, k;
+  for (e = p0; e < p1; e++) {
+    const int t = na->as_tr[e];
+    for (k = 0; k < N; k++) {
+      w->chi_up[(size_t) na->tr_i[t] * N + k] = 0.0;
+      w->chi_down[(size_t) na->tr_j[t] * N + k] = 0.0;
+      w->Uji[(size_t) na->tr_j[t] * N + k] = 0.0;
+    }
+  }
+  for (e = p0; e < p1; e++) {
+    const int n = e - p0, t = na->as_tr[e], i = na->tr_i[t], j = na->tr_j[t];
+    const double twohnu3_c2 = nlte_twohnu3(w, ls, t);
+    const double *V = w->V + (size_t) n * N, *gij = w->gij + (size_t) n * N,
+                 *wla = w->wla + (size_t) n * N;
+    const double *ni = w->n + (size_t) i * N, *nj = w->n + (size_t) j * N;
+    if (twohnu3_c2) {
+      for (k = 0; k < N; k++) {
+        const double chicc = V[k] * wla[k] * (ni[k] - gij[k] * nj[k]);
+        w->chi_up[(size_t) i * N + k] += chicc;
+        w->chi_down[(size_t) j * N + k] += chicc;
+        w->Uji[(size_t) j * N + k] += twohnu3_c2 * gij[k] * V[k];
+      }
+    }
+  }
+}
+
+/* addtoGamma, rh/fillgamma.c:82-246 (Psi already divided by chi) */
+static void nlte_gamma(nlte_ws *w, int ls, double wmu)
+{
+  const rhg_nlte_t *na = w->na;
+  const int N = w->N, nl = w->nl, p0 = na->as_ptr[ls], p1 = na->as_ptr[ls + 1];
+  int e, e2, k;
+  for (k = 0; k < N; k++) w->Ieff[k] = w->I[k] - w->Psi[k] * w->eta_at[k];
+  for (e = p0; e < p1; e++) {
+    const int n = e - p0, t = na->as_tr[e], i = na->tr_i[t], j = na->tr_j[t];
+    const double twohnu3_c2 = nlte_twohnu3(w, ls, t);
+    const double *V = w->V + (size_t) n * N, *gij = w->gij + (size_t) n * N,
+                 *wla = w->wla + (size_t) n * N;
+    double *Gij = w->Gamma + (size_t) (i * nl + j) * N, *Gji = w->Gamma + (size_t) (j * nl + i) * N;
+    for (k = 0; k < N; k++) {
+      const double wlamu = V[k] * wla[k] * wmu;
+      Gji[k] += w->Ieff[k] * wlamu;
+      Gij[k] += (twohnu3_c2 + w->Ieff[k]) * gij[k] * wlamu;
+    }
+    for (k = 0; k < N; k++)
+      Gij[k] -= w->chi_up[(size_t) i * N + k] * w->Psi[k] * w->Uji[(size_t) j * N + k] * wmu;
+    for (e2 = p0; e2 < p1; e2++) {
+      if (na->tr_j[na->as_tr[e2]] == i) {
+        for (k = 0; k < N; k++)
+          Gji[k] += w->chi_down[(size_t) j * N + k] * w->Psi[k] * w->Uji[(size_t) i * N + k] * wmu;
+      }
+    }
+  }
+}
+
+/* Formal solution at one wavelength with the active set: rays, J and
+   (eval_operator) the rate operator; returns max |1 - Jdag/J|
+   (rh/rhf1d/formal.c:44-346) */
+static double nlte_formal(nlte_ws *w, int ls, int eval_operator)
+{
+  const rhg_nlte_t *na = w->na;
+  const column_t *c = w->c;
+  const int N = w->N;
+  const double lam = m_lambda(w->m, ls);
+  double *Jl = w->J + (size_t) ls * N, B2[2], dJmax = 0.0;
+  const double *chic = w->bg_chi + (size_t) ls * N, *etac = w->bg_eta + (size_t) ls * N,
+               *scac = w->bg_sca + (size_t) ls * N;
+  int k, mu, to_obs;
+  for (k = 0; k < N; k++) { w->Jdag[k] = Jl[k]; Jl[k] = 0.0; }
+  B2[0] = rho_planck(c->T[N - 2], lam);
+  B2[1] = rho_planck(c->T[N - 1], lam);
+  if (na->as_bezier[ls]) {
+    for (mu = 0; mu < w->nrays; mu++) {
+      const double wmu = 0.5 * na->wmu[mu];
+      for (to_obs = 0; to_obs <= 1; to_obs++) {
+        nlte_opacity(w, ls, mu, to_obs, mu == 0 && to_obs == 0);
+        if (eval_operator) nlte_coupling(w, ls);
+        for (k = 0; k < N; k++) {
+          const double chi_as = w->chi[k];
+          w->chi[k] = chi_as + chic[k];
+          w->S[k] = w->eta_at[k] + etac[k];
+          w->S[k] += scac[k] * w->Jdag[k];
+        }
+        for (k = 0; k < N; k++) w->S[k] /= w->chi[k];
+        bezier_ray(N, c->height, 1.0 / na->muz[mu], w->chi, w->S, B2, to_obs, w->I, w->Psi);
+        if (w->dump && to_obs) {
+          char tg[24];
+          snprintf(tg, sizeof(tg), "fs1_chi_%d", mu);
+          nlte_dump(w->dump, tg, ls, w->chi, N);
+          snprintf(tg, sizeof(tg), "fs1_S_%d", mu);
+          nlte_dump(w->dump, tg, ls, w->S, N);
+          snprintf(tg, sizeof(tg), "fs1_I_%d", mu);
+          nlte_dump(w->dump, tg, ls, w->I, N);
+          snprintf(tg, sizeof(tg), "fs1_eat_%d", mu);
+          nlte_dump(w->dump, tg, ls, w->eta_at, N);
+        }
+        if (eval_operator) {
+          for (k = 0; k < N; k++) w->Psi[k] /= w->chi[k];
+          nlte_gamma(w, ls, wmu);
+        }
+        for (k = 0; k < N; k++) Jl[k] += wmu * w->I[k];
+      }
+    }
+  } else {
+    nlte_opacity(w, ls, 0, 0, 1);
+    if (eval_operator) nlte_coupling(w, ls);
+    for (k = 0; k < N; k++) {
+      const double chi_as = w->chi[k];
+      w->chi[k] = chi_as + chic[k];
+      w->S[k] = (w->eta_at[k] + etac[k] + scac[k] * w->Jdag[k]) / w->chi[k];
+    }
+    for (mu = 0; mu < w->nrays; mu++) {
+      feautrier_full(N, c->height, na->muz[mu], w->chi, w->S, B2, w->I, w->Psi, w->scr);
+      if (w->dump) {
+        char tg[24];
+        snprintf(tg, sizeof(tg), "fe_chi_%d", mu);
+        nlte_dump(w->dump, tg, ls, w->chi, N);
+        snprintf(tg, sizeof(tg), "fe_S_%d", mu);
+        nlte_dump(w->dump, tg, ls, w->S, N);
+        snprintf(tg, sizeof(tg), "fe_P_%d", mu);
+        nlte_dump(w->dump, tg, ls, w->I, N);
+      }
+      if (eval_operator) {
+        for (k = 0; k < N; k++) w->Psi[k] /= w->chi[k];
+        nlte_gamma(w, ls, na->wmu[mu]);
+      }
+      for (k = 0; k < N; k++) Jl[k] += w->I[k] * na->wmu[mu];
+    }
+  }
+  for (k = 0; k < N; k++) {
+    const double dJ = fabs(1.0 - w->Jdag[k] / Jl[k]);
+    dJmax = MAXV(dJmax, dJ);
+  }
+  return dJmax;
+}
+
+/* Ng acceleration, rh/accelerate.c:34-146, on the flattened populations */
+typedef struct { int N, Norder, Nperiod, Ndelay, count; double *prev; } ng_t;
+
+static void ng_init(ng_t *g, int N, int Ndelay, int Norder, int Nperiod, const double *sol)
+{
+  int k;
+  g->N = N; g->Norder = Norder; g->Nperiod = Nperiod;
+  g->Ndelay = MAXV(Ndelay, Norder + 2);
+  g->prev = calloc((size_t) (Norder + 2) * N, sizeof(double));
+  for (k = 0; k < N; k++) g->prev[k] = sol[k];
+  g->count = 1;
+}
+
+static void ng_accelerate(ng_t *g, double *sol)
+{
+  const int Norder = g->Norder, N = g->N;
+  int i, j, k, ip, ipp, i0;
+  double A[NEQ_MAX][NEQ_MAX], b[NEQ_MAX], *Delta, *weight;
+  double *cur = g->prev + (size_t) (g->count % (Norder + 2)) * N;
+  for (k = 0; k < N; k++) cur[k] = sol[k];
+  g->count++;
+  if (!(Norder > 0 && g->count >= g->Ndelay && !((g->count - g->Ndelay) % g->Nperiod))) return;
+  Delta = malloc((size_t) (Norder + 1) * N * sizeof(double));
+  weight = malloc((size_t) N * sizeof(double));
+  for (i = 0; i <= Norder; i++) {
+    ip = (g->count - 1 - i) % (Norder + 2);
+    ipp = (g->count - 2 - i) % (Norder + 2);
+    for (k = 0; k < N; k++)
+      Delta[(size_t) i * N + k] = g->prev[(size_t) ip * N + k] - g->prev[(size_t) ipp * N + k];
+  }
+  for (k = 0; k < N; k++) weight[k] = 1.0 / fabs(sol[k]);
+  for (i = 0; i < Norder; i++) { b[i] = 0.0; for (j = 0; j < Norder; j++) A[i][j] = 0.0; }
+  for (j = 0; j < Norder; j++) {
+    for (k = 0; k < N; k++)
+      b[j] += weight[k] * Delta[k] * (Delta[k] - Delta[(size_t) (j + 1) * N + k]);
+    for (i = 0; i < Norder; i++)
+      for (k = 0; k < N; k++)
+        A[i][j] += weight[k] * (Delta[(size_t) (j + 1) * N + k] - Delta[k]) *
+                   (Delta[(size_t) (i + 1) * N + k] - Delta[k]);
+  }
+  solve_linear_improve(Norder, A, b);
+  i0 = (g->count - 1) % (Norder + 2);
+  for (i = 0; i < Norder; i++) {
+    ip = (g->count - 2 - i) % (Norder + 2);
+    for (k = 0; k < N; k++)
+      sol[k] += b[i] * (g->prev[(size_t) ip * N + k] - g->prev[(size_t) i0 * N + k]);
+  }
+  for (k = 0; k < N; k++) g->prev[(size_t) i0 * N + k] = sol[k];
+  free(weight);
+  free(Delta);
+}
+
+/* MaxChange, rh/maxchange.c:32-50 */
+static double ng_maxchange(const ng_t *g)
+{
+  double dmax = 0.0;
+  const double *old, *cur;
+  int k;
+  if (g->count < 2) return dmax;
+  old = g->prev + (size_t) ((g->count - 2) % (g->Norder + 2)) * g->N;
+  cur = g->prev + (size_t) ((g->count - 1) % (g->Norder + 2)) * g->N;
+  for (k = 0; k < g->N; k++)
+    if (cur[k]) dmax = MAXV(dmax, fabs((cur[k] - old[k]) / cur[k]));
+  return dmax;
+}
+
+/* Intermediate arrays of the first column solved, written in the capture
+   record format of ref_harness.c when RHO_NLTE_DUMP names a file (tests
+   compare them with the reference's). */
+static FILE *nlte_dump_open(void)
+{
+  static int done = 0;
+  const char *p = getenv("RHO_NLTE_DUMP");
+  FILE *f = NULL;
+#ifdef _OPENMP
+#pragma omp critical(nlte_dump)
+#endif
+  {
+    if (p && !done) { f = fopen(p, "wb"); done = 1; }
+  }
+  return f;
+}
+
+static void nlte_dump(FILE *f, const char *tag, int idx, const double *x, int n)
+{
+  char t[16];
+  int hdr[5] = {0, idx, 0, 0, n};
+  if (!f) return;
+  memset(t, 0, sizeof(t));
+  strncpy(t, tag, 15);
+  fwrite(t, 1, 16, f);
+  fwrite(hdr, sizeof(int), 5, f);
+  fwrite(x, sizeof(double), n, f);
+}
+
+static int nlte_column(const rhg_model_t *m, const rhg_nlte_t *na, const column_t *c,
+                       struct nlte_out *o)
+{
+  FILE *dump = nlte_dump_open();
+  const int N = c->Ndep, nl = na->nlevel, nspect = m->nspect, nrays = na->nrays;
+  const int nlev_all = m->atom_level_ptr[m->natom], l0 = m->atom_level_ptr[na->atom_index];
+  const double vtherm = 2.0 * KBOLTZMANN / (AMU * na->weight);
+  int k, i, l, t, ls, mu, to_obs, la, niter, maxset = 1, *phi_off;
+  size_t nphi = 0;
+  double *adamp, dpops = 0.0;
+  nlte_ws w;
+  ng_t ng;
+  if (nl > NLEV_MAX) return -1;
+  memset(&w, 0, sizeof(w));
+  w.m = m; w.na = na; w.c = c; w.N = N; w.nl = nl; w.nrays = nrays;
+  for (ls = 0; ls < nspect; ls++) maxset = MAXV(maxset, na->as_ptr[ls + 1] - na->as_ptr[ls]);
+  phi_off = calloc((size_t) na->nline + 1, sizeof(int));
+  for (l = 0; l < na->nline; l++) {
+    phi_off[l] = (int) nphi;
+    for (t = 0; t < na->ntrans; t++)
+      if (na->tr_line[t] == l) nphi += (size_t) na->tr_nlam[t] * nrays * 2 * N;
+  }
+  w.phi_off = phi_off;
+  w.n = calloc((size_t) nl * N, sizeof(double));
+  w.nstar = calloc((size_t) nl * N, sizeof(double));
+  w.ntotal = calloc(N, sizeof(double));
+  w.vbroad = calloc(N, sizeof(double));
+  w.C = calloc((size_t) nl * nl * N, sizeof(double));
+  w.Gamma = calloc((size_t) nl * nl * N, sizeof(double));
+  w.J = calloc((size_t) nspect * N, sizeof(double));
+  w.Jdag = calloc(N, sizeof(double));
+  w.phi = calloc(nphi + 1, sizeof(double));
+  w.wphi = calloc((size_t) (na->nline + 1) * N, sizeof(double));
+  w.bg_chi = calloc((size_t) nspect * N, sizeof(double));
+  w.bg_eta = calloc((size_t) nspect * N, sizeof(double));
+  w.bg_sca = calloc((size_t) nspect * N, sizeof(double));
+  w.chi = calloc(N, sizeof(double)); w.S = calloc(N, sizeof(double));
+  w.I = calloc(N, sizeof(double)); w.Psi = calloc(N, sizeof(double));
+  w.eta_at = calloc(N, sizeof(double)); w.Ieff = calloc(N, sizeof(double));
+  w.scr = calloc((size_t) 8 * N, sizeof(double));
+  w.V = calloc((size_t) maxset * N, sizeof(double));
+  w.gij = calloc((size_t) maxset * N, sizeof(double));
+  w.wla = calloc((size_t) maxset * N, sizeof(double));
+  w.chi_up = calloc((size_t) nl * N, sizeof(double));
+  w.chi_down = calloc((size_t) nl * N, sizeof(double));
+  w.Uji = calloc((size_t) nl * N, sizeof(double));
+  adamp = calloc(N, sizeof(double));
+
+  /* LTE populations and Doppler widths (rh/readatom.c:213-220) */
+  for (i = 0; i < nl; i++)
+    for (k = 0; k < N; k++) w.nstar[(size_t) i * N + k] = c->nstar[(size_t) k * nlev_all + l0 + i];
+  for (k = 0; k < N; k++) {
+    w.ntotal[k] = m->atom_abund[na->atom_index] * c->nHtot[k];
+    w.vbroad[k] = sqrt(vtherm * c->T[k] + SQ(c->vturb[k]));
+  }
+
+  /* line profiles and their normalisation (rh/profile.c:67-371 with
+     Damping, rh/broad.c:273-316) */
+  for (l = 0; l < na->nline; l++) {
+    const double lambda0 = na->ln_lambda0[l], cDop = (NM_TO_M * lambda0) / (4.0 * PI);
+    double *wphi = w.wphi + (size_t) l * N;
+    for (t = 0; t < na->ntrans && na->tr_line[t] != l; t++);
+    for (k = 0; k < N; k++) {
+      double Qelast = 0.0, G;
+      if (na->ln_vdw[l]) {
+        G = na->ln_vdw_cross[l] * pow(c->T[k], 0.3);
+        G *= c->nH0[k];
+        Qelast += G;
+      }
+      if (na->ln_stark[l] == 1) {
+        Qelast += na->ln_stark_c[l] * c->ne[k];
+      } else if (na->ln_stark[l] == 2) {
+        const double vrel = pow(na->ln_stark_C[l] * c->T[k], 0.16666667) * na->ln_stark_Cm[l];
+        Qelast += na->ln_stark_c[l] * vrel * c->ne[k];
+      }
+      adamp[k] = (na->ln_Grad[l] + Qelast) * cDop / w.vbroad[k];
+      wphi[k] = 0.0;
+    }
+    nlte_dump(dump, "adamp", l, adamp, N);
+    for (la = 0; la < na->tr_nlam[t]; la++) {
+      const double lamla = m_lambda(m, na->tr_nblue[t] + la);
+      const double wl = na->tr_wlam[na->tr_ptr[t] + la];
+      for (mu = 0; mu < nrays; mu++) {
+        const double wlamu = wl * 0.5 * na->wmu[mu];
+        for (to_obs = 0; to_obs <= 1; to_obs++) {
+          const double sign = to_obs ? 1.0 : -1.0;
+          double *phi = w.phi + phi_off[l] + ((size_t) (la * nrays + mu) * 2 + to_obs) * N;
+          for (k = 0; k < N; k++) {
+            const double v = (lamla - lambda0 - 0.0) * CLIGHT / (w.vbroad[k] * lambda0);
+            const double vlos = na->muz[mu] * c->vz[k] / w.vbroad[k];
+            const double vk = v + sign * vlos;
+            phi[k] = 0.0;
+            phi[k] += rho_voigt_armstrong(adamp[k], vk) * 1.0 / (SQRTPI * w.vbroad[k]);
+            wphi[k] += phi[k] * wlamu;
+          }
+        }
+      }
+    }
+    for (k = 0; k < N; k++) wphi[k] = 1.0 / wphi[k];
+  }
+
+  /* collisional rates (rh/collision.c:450-760) */
+  {
+    const double C0 = ((E_RYDBERG / sqrt(M_ELECTRON)) * PI * SQ(RBOHR)) * sqrt(8.0 / (PI * KBOLTZMANN));
+    int r;
+    for (r = 0; r < na->ncoll; r++) {
+      const int i1 = na->coll_i[r], j1 = na->coll_j[r], nT = na->coll_ptr[r + 1] - na->coll_ptr[r];
+      const double *Tt = na->coll_T + na->coll_ptr[r], *ct = na->coll_c + na->coll_ptr[r],
+                   *Mt = na->coll_M + na->coll_ptr[r];
+      double *Cij = w.C + (size_t) (i1 * nl + j1) * N, *Cji = w.C + (size_t) (j1 * nl + i1) * N;
+      const double *nsi = w.nstar + (size_t) i1 * N, *nsj = w.nstar + (size_t) j1 * N;
+      for (k = 0; k < N; k++) {
+        const double T = c->T[k];
+        const double Cv = nT > 2 ? spline_eval_M(nT, Tt, ct, Mt, T) : linear1(nT, Tt, ct, T);
+        double Cdown, Cup;
+        switch (na->coll_type[r]) {
+        case 0:
+          Cdown = C0 * c->ne[k] * Cv / (na->g[j1] * sqrt(T));
+          Cij[k] += Cdown; Cji[k] += Cdown * nsj[k] / nsi[k];
+          break;
+        case 1:
+          Cdown = Cv * c->ne[k] * (na->g[i1] / na->g[j1]) * sqrt(T);
+          Cij[k] += Cdown; Cji[k] += Cdown * nsj[k] / nsi[k];
+          break;
+        case 2:
+          Cup = Cv * c->ne[k] * exp(-(na->E[j1] - na->E[i1]) / (KBOLTZMANN * T)) * sqrt(T);
+          Cji[k] += Cup; Cij[k] += Cup * nsi[k] / nsj[k];
+          break;
+        case 3:
+          Cdown = c->np[k] * Cv;
+          Cij[k] += Cdown; Cji[k] += Cdown * nsj[k] / nsi[k];
+          break;
+        case 4:
+          Cup = c->nH0[k] * Cv;
+          Cji[k] += Cup; Cij[k] += Cup * nsi[k] / nsj[k];
+          break;
+        case 5: Cij[k] += c->nH0[k] * Cv; break;
+        case 6: Cji[k] += c->np[k] * Cv; break;
+        }
+      }
+    }
+  }
+
+  /* background at the wavelengths with an active set: the LTE continuum
+     and the scattering opacity (rh/background.c:342-465), which NLTE adds
+     to chi and takes J-weighted into the source function */
+  for (ls = 0; ls < nspect; ls++) {
+    double *bc = w.bg_chi + (size_t) ls * N, *be = w.bg_eta + (size_t) ls * N,
+           *bs = w.bg_sca + (size_t) ls * N;
+    if (na->as_ptr[ls + 1] == na->as_ptr[ls]) continue;
+    background_continuum(m, c, ls, bc, be);
+    for (k = 0; k < N; k++) {
+      double sca = c->ne[k] * na->sigma_thomson;
+      sca += na->ray_H[ls] * c->nH0[k];
+      if (na->he_atom >= 0)
+        sca += na->ray_He[ls] * c->nstar[(size_t) k * nlev_all + m->atom_level_ptr[na->he_atom]];
+      sca += na->ray_H2[ls] * c->nH2[k];
+      bs[k] = sca;
+      bc[k] += sca;
+    }
+  }
+
+  for (ls = 0; ls < nspect; ls++) {
+    nlte_dump(dump, "bgc_chi", ls, w.bg_chi + (size_t) ls * N, N);
+    nlte_dump(dump, "bgc_eta", ls, w.bg_eta + (size_t) ls * N, N);
+    nlte_dump(dump, "bgc_sca", ls, w.bg_sca + (size_t) ls * N, N);
+  }
+  for (i = 0; i < nl * nl; i++) nlte_dump(dump, "coll_C", i, w.C + (size_t) i * N, N);
+  for (i = 0; i < nl; i++) nlte_dump(dump, "nlte_nstar", i, w.nstar + (size_t) i * N, N);
+  for (l = 0; l < na->nline; l++) {
+    nlte_dump(dump, "wphi", l, w.wphi + (size_t) l * N, N);
+    nlte_dump(dump, "phi0", l, w.phi + phi_off[l], N);
+  }
+
+  w.dump = dump;
+  /* initial solution (LTE populations, J = 0) and background scattering
+     iterations (rh/initial_xdr.c:276-282, rh/initscatter.c:60-75) */
+  for (i = 0; i < nl * N; i++) w.n[i] = w.nstar[i];
+  if (na->nmax_iter > 0) {
+    int it = 0;
+    while (it < na->nmax_scatter) {
+      double dJmax = 0.0;
+      for (ls = 0; ls < nspect; ls++)
+        if (na->as_ptr[ls + 1] > na->as_ptr[ls]) {
+          const double dJ = nlte_formal(&w, ls, 0);
+          dJmax = MAXV(dJmax, dJ);
+        }
+      w.dump = NULL;
+      if (dJmax < na->iter_limit) break;
+      it++;
+    }
+  }
+
+  for (ls = 0; ls < nspect; ls++) nlte_dump(dump, "J_scat", ls, w.J + (size_t) ls * N, N);
+
+  /* main iterations (rh/iterate.c:48-143, rh/statequil.c:36-216) */
+  ng_init(&ng, nl * N, na->ng_delay, na->ng_order, na->ng_period, w.n);
+  for (niter = 1; niter <= na->nmax_iter; niter++) {
+    for (i = 0; i < nl * nl * N; i++) w.Gamma[i] = w.C[i];
+    for (ls = 0; ls < nspect; ls++)
+      if (na->as_ptr[ls + 1] > na->as_ptr[ls]) nlte_formal(&w, ls, 1);
+    if (niter == 1)
+      for (i = 0; i < nl * nl; i++) nlte_dump(dump, "gamma1", i, w.Gamma + (size_t) i * N, N);
+    for (k = 0; k < N; k++) {
+      double A[NEQ_MAX][NEQ_MAX], nk[NEQ_MAX], nmax = 0.0;
+      int j, ie = 0;
+      for (i = 0; i < nl; i++) {
+        nk[i] = w.n[(size_t) i * N + k];
+        for (j = 0; j < nl; j++) A[i][j] = w.Gamma[(size_t) (i * nl + j) * N + k];
+      }
+      if (na->isum == -1) {
+        for (i = 0; i < nl; i++) if (nk[i] > nmax) { nmax = nk[i]; ie = i; }
+      } else
+        ie = na->isum;
+      for (i = 0; i < nl; i++) {
+        double GamDiag = 0.0;
+        A[i][i] = 0.0;
+        nk[i] = 0.0;
+        for (j = 0; j < nl; j++) GamDiag += A[j][i];
+        A[i][i] = -GamDiag;
+      }
+      nk[ie] = w.ntotal[k];
+      for (j = 0; j < nl; j++) A[ie][j] = 1.0;
+      solve_linear_improve(nl, A, nk);
+      for (i = 0; i < nl; i++) w.n[(size_t) i * N + k] = nk[i];
+    }
+    ng_accelerate(&ng, w.n);
+    dpops = ng_maxchange(&ng);
+    if (dpops < na->iter_limit) break;
+  }
+  if (niter > na->nmax_iter) niter = na->nmax_iter;
+  for (i = 0; i < nl * N; i++) o->pops[i] = w.n[i];
+  *o->niter = niter;
+  *o->dpops = dpops;
+
+  if (dump) fclose(dump);
+  free(ng.prev);
+  free(adamp); free(phi_off);
+  free(w.n); free(w.nstar); free(w.ntotal); free(w.vbroad); free(w.C); free(w.Gamma);
+  free(w.J); free(w.Jdag); free(w.phi); free(w.wphi); free(w.bg_chi); free(w.bg_eta);
+  free(w.bg_sca); free(w.chi); free(w.S); free(w.I); free(w.Psi); free(w.eta_at);
+  free(w.Ieff); free(w.scr); free(w.V); free(w.gij); free(w.wla); free(w.chi_up);
+  free(w.chi_down); free(w.Uji);
+  return 0;
+}
+
+/* NLTE populations of the active atom per column: pops_out
+   [ncol][nlevel][ndep_max], iterations and final max relative change */
+int rho_nlte_solve(const rhg_model_t *m, const rhg_nlte_t *na, const rhg_columns_t *cols,
+                   int nthreads, double *pops_out, int *niter_out, double *dpops_out)
+{
+  int ic, err = 0;
+  if (m->nnuclei + m->nmol > NEQ_MAX) return RHG_EUNSUPPORTED;
+  if (na->nlevel > NLEV_MAX || na->ng_order + 2 > NEQ_MAX) return RHG_EUNSUPPORTED;
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads) reduction(|:err)
+#endif
+  for (ic = 0; ic < cols->ncol; ic++) {
+    struct nlte_out o;
+    o.pops = pops_out + (size_t) ic * na->nlevel * cols->ndep_max;
+    o.niter = niter_out + ic;
+    o.dpops = dpops_out + ic;
+    if (synth_column_ex(m, cols, ic, 0, NULL, NULL, NULL, -1, NULL, NULL, NULL, 0, NULL, NULL,
+                        na, &o))
       err |= 1;
   }
   (void) nthreads;

@@ -32,6 +32,14 @@ int rho_synth_lte(const rhg_model_t *m, const rhg_columns_t *cols,
 int rho_rf_loggf(const rhg_model_t *m, const rhg_columns_t *cols, int npar,
                  const int *par_line, int nthreads, double *stokes_out, double *rf_out);
 
+/* NLTE populations of the active atom `na` (accelerated lambda iteration
+ * as rhf1d() runs it, rh/iterate.c:48-143): the model is the background
+ * model on the merged grid.  pops_out [ncol][nlevel][ndep_max]; niter_out
+ * and dpops_out [ncol] get the iterations done and the last max relative
+ * change. */
+int rho_nlte_solve(const rhg_model_t *m, const rhg_nlte_t *na, const rhg_columns_t *cols,
+                   int nthreads, double *pops_out, int *niter_out, double *dpops_out);
+
 /* Single-function entry points for closed-form and invariant tests. */
 double rho_voigt_humlicek(double a, double v, double *F);
 double rho_voigt_armstrong(double a, double v);

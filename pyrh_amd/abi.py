@@ -110,3 +110,59 @@ class ColumnsStruct:
 
     def ptr(self):
         return C.byref(self.struct)
+
+
+# rhg_nlte_t, in declaration order
+NLTE_FIELDS = [
+    ("nlevel", C.c_int32, "s"), ("atom_index", C.c_int32, "s"), ("he_atom", C.c_int32, "s"),
+    ("E", P_D, "d"), ("g", P_D, "d"), ("stage", P_I, "i"), ("weight", C.c_double, "s"),
+    ("ntrans", C.c_int32, "s"),
+    ("tr_type", P_I, "i"), ("tr_i", P_I, "i"), ("tr_j", P_I, "i"),
+    ("tr_nblue", P_I, "i"), ("tr_nlam", P_I, "i"), ("tr_ptr", P_I, "i"), ("tr_line", P_I, "i"),
+    ("tr_wlam", P_D, "d"), ("tr_alpha", P_D, "d"),
+    ("nline", C.c_int32, "s"),
+    ("ln_lambda0", P_D, "d"), ("ln_Aji", P_D, "d"), ("ln_Bji", P_D, "d"), ("ln_Bij", P_D, "d"),
+    ("ln_Grad", P_D, "d"), ("ln_vdw", P_I, "i"), ("ln_vdw_cross", P_D, "d"),
+    ("ln_stark", P_I, "i"), ("ln_stark_c", P_D, "d"), ("ln_stark_C", P_D, "d"),
+    ("ln_stark_Cm", P_D, "d"),
+    ("as_ptr", P_I, "i"), ("as_tr", P_I, "i"), ("as_bezier", P_I, "i"),
+    ("ncoll", C.c_int32, "s"),
+    ("coll_type", P_I, "i"), ("coll_i", P_I, "i"), ("coll_j", P_I, "i"), ("coll_ptr", P_I, "i"),
+    ("coll_T", P_D, "d"), ("coll_c", P_D, "d"), ("coll_M", P_D, "d"),
+    ("nrays", C.c_int32, "s"), ("muz", P_D, "d"), ("wmu", P_D, "d"),
+    ("ray_H", P_D, "d"), ("ray_He", P_D, "d"), ("ray_H2", P_D, "d"),
+    ("sigma_thomson", C.c_double, "s"),
+    ("nmax_iter", C.c_int32, "s"), ("nmax_scatter", C.c_int32, "s"),
+    ("ng_order", C.c_int32, "s"), ("ng_delay", C.c_int32, "s"), ("ng_period", C.c_int32, "s"),
+    ("isum", C.c_int32, "s"), ("iter_limit", C.c_double, "s"),
+]
+
+
+class RhgNlte(C.Structure):
+    _fields_ = [(n, t) for n, t, _ in NLTE_FIELDS]
+
+
+class NlteStruct:
+    """Owns contiguous copies of an NLTEModel's arrays and the rhg_nlte_t
+    pointing at them."""
+
+    def __init__(self, nm):
+        self._keep = {}
+        s = RhgNlte()
+        counts = {"ntrans": int(nm.tr_type.size), "nline": int(nm.ln_lambda0.size),
+                  "ncoll": int(nm.coll_type.size), "nrays": int(nm.muz.size)}
+        for name, ctype, kind in NLTE_FIELDS:
+            if kind == "s":
+                v = counts[name] if name in counts else getattr(nm, name)
+                setattr(s, name, ctype(v).value)
+            else:
+                dt = np.float64 if kind == "d" else np.int32
+                arr = np.ascontiguousarray(getattr(nm, name), dtype=dt).ravel()
+                if arr.size == 0:
+                    arr = np.zeros(1, dtype=dt)
+                self._keep[name] = arr
+                setattr(s, name, arr.ctypes.data_as(ctype))
+        self.struct = s
+
+    def ptr(self):
+        return C.byref(self.struct)

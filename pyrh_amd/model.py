@@ -614,7 +614,7 @@ class Model:
     cont_desc: list          # (atom index, continuum object) per continuum
 
 
-def _check_supported(setup):
+def _check_supported(setup, nlte=False):
     kw = setup.kw
     if not kw.bool("HYDROGEN_LTE"):
         raise NotImplementedError("HYDROGEN_LTE = FALSE (NLTE hydrogen in the background)")
@@ -632,8 +632,9 @@ def _check_supported(setup):
         raise NotImplementedError(f"STOKES_MODE = {kw.str('STOKES_MODE')}")
     if kw.bool("MAGNETO_OPTICAL"):
         raise NotImplementedError("MAGNETO_OPTICAL = TRUE")
-    if any(a.active for a in setup.atoms):
-        raise NotImplementedError("active atoms (NLTE) are not implemented in the LTE path")
+    if not nlte and any(a.active for a in setup.atoms):
+        raise NotImplementedError("active atoms (NLTE) are not solved by the LTE path; "
+                                  "use pyrh_amd.nlte")
     if kw.float("METALLICITY") != 0.0:
         pass  # handled in read_abundance
     if kw.bool("NON_ICE"):
@@ -663,9 +664,9 @@ def atomic_rf_lines(setup: RunSetup, loggf_ids=None, nlam=0) -> np.ndarray:
     return par
 
 
-def build_model(setup: RunSetup, wave) -> Model:
+def build_model(setup: RunSetup, wave, nlte=False) -> Model:
     """Wavelength grid and all column-independent tables for LTE synthesis."""
-    _check_supported(setup)
+    _check_supported(setup, nlte)
     kw = setup.kw
     el = setup.elements
     atoms = setup.atoms
@@ -737,6 +738,8 @@ def build_model(setup: RunSetup, wave) -> Model:
               * K.HPLANCK / (2.0 * K.E_RYDBERG))
     cont_list = []   # (atom idx, cont, is_H)
     for ia, at in enumerate(atoms):
+        if at.active:       # the active atom's continua are in its active set
+            continue        # (rh/metal.c:90)
         for c in at.conts:
             cont_list.append((ia, c))
     per_lam = [[] for _ in range(nspect)]   # (global cont index, alpha)
@@ -780,6 +783,8 @@ def build_model(setup: RunSetup, wave) -> Model:
     # --- passive bound-bound lines of the model atoms would add opacity
     #     (rh/metal.c:174-330); detect and refuse (not implemented)
     for ia, at in enumerate(atoms):
+        if at.active:
+            continue
         for ln in at.lines:
             dl = ln.lambda0 * ln.qwing * (vchar / K.CLIGHT)
             hit = np.abs(lam - ln.lambda0) <= dl

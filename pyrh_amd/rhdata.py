@@ -297,6 +297,7 @@ class Atom:
     lines: list
     conts: list
     raw_lines: list = field(default_factory=list)  # remaining text (collisions)
+    Nfixed: int = 0
 
 
 def _atom_lines_after(path):
@@ -413,7 +414,8 @@ def read_atom(path, active, elements: ElementTable, initial="LTE_POPULATIONS") -
             raise RHInputError(f"Invalid value for wavelength dependence string: {nudep}")
     rest = L[p:]
     return Atom(aid, str(path), active, initial, pt, float(elements.abund[pt]),
-                float(elements.weight[pt]), E, g, stage, labels, abo, lines, conts, rest)
+                float(elements.weight[pt]), E, g, stage, labels, abo, lines, conts, rest,
+                Nfixed=int(Nfixed))
 
 
 def read_atoms_input(path, pyrh_path, elements) -> list:

@@ -1,0 +1,76 @@
+"""Generate the NLTE golden vectors in tests/golden/nlte_fixtures.npz.
+
+Runs pyrh's rhf1d() from the reference library compiled from its C sources
+(oracle/_ref, variant "o2") on the NLTE run directory tests/golden/run_nlte
+(the Ca II-like five-level atom with continuum, active, accelerated lambda
+iteration to a maximum relative population change of 1e-3) and stores the
+inputs, the converged populations of the active atom, the iteration count
+and last change, the merged wavelength grid, and the iteration history.
+Needs /root/reference; run from the repository root:
+
+    PYRH_PATH=tests/golden/rhdata python3 tests/golden/make_golden_nlte.py
+"""
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+sys.path.insert(0, str(HERE.parents[1]))
+sys.path.insert(0, str(HERE.parents[1] / "oracle"))
+
+import refrun  # noqa: E402
+from pyrh_amd import workloads as W  # noqa: E402
+
+RUN = HERE / "run_nlte"
+WAVE = np.array([393.4776, 396.9591, 854.44])
+
+
+def cases():
+    mp = W.stokes_map(6, seed=W.SEED_BASE + 3)
+    yield "map6", mp
+    # the single workload column and a slow, strongly perturbed one
+    c0 = W.single_column()[None]
+    hot = mp[:1].copy()
+    hot[:, 1] *= 1.08
+    yield "single", c0
+    yield "hot", hot
+
+
+def main():
+    out = {}
+    for name, atm in cases():
+        res = refrun.run_synth(RUN, atm, WAVE, mu=1.0, atm_scale=0, variant="o2",
+                               capture=True, timeout=3600)
+        cap = res[-1]
+        ncol = atm.shape[0]
+        # records: pops_n with (column, atom, level) in (col, nspect, mu)
+        nl = len([r for r in cap if r[0] == "pops_n" and r[1] == 0])
+        pops = np.zeros((ncol, nl, atm.shape[2]))
+        niter = np.zeros(ncol, np.int32)
+        dpops = np.zeros(ncol)
+        hist = np.zeros((ncol, 100))
+        for c in range(ncol):
+            for r in cap:
+                if r[0] == "pops_n" and r[1] == c and r[2] == 0:
+                    pops[c, r[3]] = r[5]
+            its = [r[5] for r in cap if r[0] == "iter" and r[1] == c]
+            niter[c] = len(its)
+            dpops[c] = its[-1][1]
+            hist[c, :len(its)] = [x[1] for x in its]
+        lam = [r[5] for r in cap if r[0] == "lambda"][0]
+        out[f"{name}__atmos"] = atm
+        out[f"{name}__pops"] = pops
+        out[f"{name}__niter"] = niter
+        out[f"{name}__dpops"] = dpops
+        out[f"{name}__history"] = hist
+        out[f"{name}__lam"] = lam
+        print(name, atm.shape, "iterations", niter.tolist())
+    out["wave"] = WAVE
+    np.savez_compressed(HERE / "nlte_fixtures.npz", **out)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,106 @@
+"""NLTE host model and oracle against the reference's own NLTE runs.
+
+The golden vectors (tests/golden/nlte_fixtures.npz, tests/golden/
+make_golden_nlte.py) come from rhf1d() compiled from the reference sources:
+converged populations of the active Ca II-like atom, iteration counts, the
+last maximum relative change and the merged wavelength grid, for eight
+columns.  The oracle reproduces them to round-off; the host model builds the
+same wavelength grid bit for bit.
+"""
+import numpy as np
+import pytest
+
+import pyoracle
+from conftest import GOLDEN
+
+RUN_NLTE = GOLDEN / "run_nlte"
+
+
+@pytest.fixture(scope="module")
+def nlte_fix():
+    return np.load(GOLDEN / "nlte_fixtures.npz")
+
+
+@pytest.fixture(scope="module")
+def nm(nlte_fix):
+    from pyrh_amd import model, nlte
+    return nlte.build_nlte(model.open_run(str(RUN_NLTE)), nlte_fix["wave"])
+
+
+CASES = ("map6", "single", "hot")
+
+
+def test_merged_grid_matches_reference(nm, nlte_fix):
+    for name in CASES:
+        np.testing.assert_array_equal(nm.lam, nlte_fix[name + "__lam"])
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_oracle_populations_match_reference(nm, nlte_fix, name):
+    atm = nlte_fix[name + "__atmos"]
+    pops, niter, dpops = pyoracle.nlte_solve(nm, atm)
+    ref = nlte_fix[name + "__pops"]
+    np.testing.assert_array_equal(niter, nlte_fix[name + "__niter"])
+    # the last change is a difference of nearly equal populations
+    np.testing.assert_allclose(dpops, nlte_fix[name + "__dpops"], rtol=1e-6, atol=0)
+    # bit-identical for the moving map columns; the static column picks up a
+    # one-ulp difference in its first rate matrix that the iteration grows
+    # to ~1e-9 by convergence
+    assert np.max(np.abs(pops - ref) / ref) <= 1e-8
+    # particle conservation closes every depth's rate equations
+    ntot = pops.sum(axis=1)
+    np.testing.assert_allclose(ntot / ntot[:, :1], ref.sum(axis=1) / ref.sum(axis=1)[:, :1],
+                               rtol=1e-12)
+
+
+def test_gauss_legendre_closed_form():
+    from pyrh_amd import nlte
+    for n in (1, 2, 3, 5, 8):
+        x, w = nlte.gauss_legendre(n)
+        xr, wr = np.polynomial.legendre.leggauss(n)
+        np.testing.assert_allclose(x, 0.5 * (xr + 1.0), rtol=0, atol=1e-14)
+        np.testing.assert_allclose(w, 0.5 * wr, rtol=1e-13)
+        # exact for polynomials of degree 2n-1 on (0, 1)
+        for p in range(2 * n):
+            assert abs(np.sum(w * x ** p) - 1.0 / (p + 1)) < 1e-13
+
+
+def test_line_grid_shape(nm):
+    """getLambda: an odd, symmetric set of offsets around lambda0 reaching
+    qwing Doppler units of VMICRO_CHAR, spacing growing outward."""
+    from pyrh_amd import model, nlte
+    s = model.open_run(str(RUN_NLTE))
+    atom = [a for a in s.atoms if a.active][0]
+    vchar = s.kw.float("VMICRO_CHAR") * 1e3
+    for ln in atom.lines:
+        g = nlte.line_grid(ln, False, vchar)
+        assert g.size % 2 == 1 and g[g.size // 2] == ln.lambda0
+        d = g - ln.lambda0
+        np.testing.assert_allclose(d + d[::-1], 0.0, atol=1e-12)
+        assert np.all(np.diff(np.diff(g[g.size // 2:])) > 0)
+        qmax = (g[-1] - ln.lambda0) / (ln.lambda0 * vchar / 2.99792458e8)
+        assert abs(qmax - ln.qwing) < 1e-9 * ln.qwing
+
+
+def test_active_sets_cover_each_transition_once(nm):
+    for t in range(nm.tr_type.size):
+        span = range(nm.tr_nblue[t], nm.tr_nblue[t] + nm.tr_nlam[t])
+        for ls in range(nm.lam.size):
+            members = nm.as_tr[nm.as_ptr[ls]:nm.as_ptr[ls + 1]].tolist()
+            assert (t in members) == (ls in span)
+    # continua are listed before lines in every set (the reference's order)
+    for ls in range(nm.lam.size):
+        ty = nm.tr_type[nm.as_tr[nm.as_ptr[ls]:nm.as_ptr[ls + 1]]]
+        assert np.all(np.diff(ty) <= 0)
+
+
+def test_quadrature_weights_sum_to_span(nm):
+    """Trapezoid weights over each transition's span add up to its width
+    (times c/lambda0 for lines, in Doppler units)."""
+    for t in range(nm.tr_type.size):
+        w = nm.tr_wlam[nm.tr_ptr[t]:nm.tr_ptr[t + 1]]
+        lo, hi = nm.lam[nm.tr_nblue[t]], nm.lam[nm.tr_nblue[t] + nm.tr_nlam[t] - 1]
+        width = hi - lo
+        if nm.tr_type[t] == 0:
+            width *= 2.99792458e8 / nm.ln_lambda0[nm.tr_line[t]]
+        assert abs(w.sum() - width) < 1e-12 * width
