@@ -306,6 +306,35 @@ int rhg_rf_loggf(rhg_context_t *ctx, const rhg_columns_t *cols, int32_t npar,
                  const int32_t *par_line, int32_t device_ptrs, double *stokes_out,
                  double *rf_out);
 
+/* NLTE populations of one active atom in every column of a batch: the work
+ * of rhf1d() with the atom ACTIVE up to the converged populations
+ * (rh/rhf1d/rhf1d.c:80-140 through rh/iterate.c:48-143: initSolution with
+ * LTE populations and J = 0, rh/initial_xdr.c:276-282; N_MAX_SCATTER
+ * passes of background scattering, rh/initscatter.c:60-75; then accelerated
+ * lambda iterations, the formal solution of every active wavelength with the
+ * diagonal operator, rh/rhf1d/formal.c:44-346, rh/fillgamma.c:82-332,
+ * statistical equilibrium, rh/statequil.c:36-216, Ng acceleration,
+ * rh/accelerate.c:34-146, until the maximum relative change is below
+ * ITER_LIMIT or N_MAX_ITER).  Each column stops iterating when it converges
+ * (its own rhf1d() would).  The context is opened with the background model
+ * on the atom's merged wavelength grid (pyrh_amd.nlte.build_nlte);
+ * columns as in rhg_synth_lte.  pops_out is [ncol][nlevel][ndep_max]
+ * float64 (zero past ndep), niter_out [ncol] the iterations done and
+ * dpops_out [ncol] the last maximum relative change; host memory, or device
+ * memory for all three and the columns if `device_ptrs`.  FP64.
+ * Synchronous. */
+int rhg_nlte_solve(rhg_context_t *ctx, const rhg_nlte_t *atom, const rhg_columns_t *cols,
+                   int32_t device_ptrs, double *pops_out, int32_t *niter_out,
+                   double *dpops_out);
+
+/* Timings of the last rhg_nlte_solve in ms and its work counts:
+ * [0] set-up (equilibria, depth scale, background, profiles, collisions),
+ * [1] formal solutions of the scattering passes, [2] formal solutions of the
+ * iterations, [3] statistical equilibrium, [4] Ng acceleration, [5] total,
+ * [6] formal-solution launches, [7] (column, wavelength) pairs solved. */
+#define RHG_NLTE_NTIMING 8
+int rhg_nlte_timings(rhg_context_t *ctx, double *out);
+
 /* Same as rhg_synth_lte but also returns, per column, the depth scale and
  * the background opacity at the reference wavelength (for testing):
  * height_out/chiref_out are [ncol][ndep_max] float64 host buffers (NULL to

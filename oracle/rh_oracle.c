@@ -1354,7 +1354,7 @@ static int line_opacity(const rhg_model_t *m, const column_t *c, int ls,
   return line_opacity_sel(m, c, ls, mu, chi, eta, -1);
 }
 
-struct nlte_out { double *pops; int *niter; double *dpops; };
+struct nlte_out { double *pops; int *niter; double *dpops; int ndep_max; };
 static int nlte_column(const rhg_model_t *m, const rhg_nlte_t *na, const column_t *c,
                        struct nlte_out *o);
 
@@ -2328,7 +2328,9 @@ static int nlte_column(const rhg_model_t *m, const rhg_nlte_t *na, const column_
     if (dpops < na->iter_limit) break;
   }
   if (niter > na->nmax_iter) niter = na->nmax_iter;
-  for (i = 0; i < nl * N; i++) o->pops[i] = w.n[i];
+  for (i = 0; i < nl; i++)
+    for (k = 0; k < o->ndep_max; k++)
+      o->pops[(size_t) i * o->ndep_max + k] = k < N ? w.n[(size_t) i * N + k] : 0.0;
   *o->niter = niter;
   *o->dpops = dpops;
 
@@ -2360,6 +2362,7 @@ int rho_nlte_solve(const rhg_model_t *m, const rhg_nlte_t *na, const rhg_columns
     o.pops = pops_out + (size_t) ic * na->nlevel * cols->ndep_max;
     o.niter = niter_out + ic;
     o.dpops = dpops_out + ic;
+    o.ndep_max = cols->ndep_max;
     if (synth_column_ex(m, cols, ic, 0, NULL, NULL, NULL, -1, NULL, NULL, NULL, 0, NULL, NULL,
                         na, &o))
       err |= 1;

@@ -10,7 +10,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libpyrh_amd.so"
-SOURCES = ["rhg_api.hip", "rhg_kernels.hip", "rhg_formal.hip", "rhg_rf.hip"]
+SOURCES = ["rhg_api.hip", "rhg_kernels.hip", "rhg_formal.hip", "rhg_rf.hip", "rhg_nlte.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",

@@ -8,7 +8,7 @@
 #include <string>
 #include <vector>
 
-#include "rhg_kernels.hpp"
+#include "rhg_nlte.hpp"
 
 namespace {
 
@@ -63,6 +63,14 @@ struct rhg_context {
       rf_dchiref, rf_pts, rf_car, rf_aP, rf_aZ, rf_aB, rf_out;
   // log gf response functions: inward intensities, output, parameter lines
   DevBuf at_iin, at_out, at_par;
+  // NLTE: the active atom (packed arrays and its DevNlte), the batch arrays
+  // of rhg::NlteBufs, active-column list, per-column reductions, output
+  DevBuf nl_atom, nl_dev, nl_nstar, nl_pops, nl_ntot, nl_vbroad, nl_nHe, nl_adamp, nl_wphi,
+      nl_Cmat, nl_Gam, nl_phi, nl_csca, nl_J, nl_acc, nl_scr, nl_dJ, nl_hist, nl_act, nl_red,
+      nl_out, nl_iolist, nl_atm, nl_ndep;
+  double nlte_ms[RHG_NLTE_NTIMING]{};
+  // host copies of model tables the NLTE checks need
+  std::vector<int32_t> h_atom_level_ptr, h_nucl_atom;
   hipEvent_t ev[RHG_NSTAGE + 2]{};
   float last_ms[RHG_NSTAGE + 1]{};
   // the last batch, for rhg_count_work
@@ -304,6 +312,8 @@ int rhg_open(const rhg_model_t *m, int device, rhg_context_t **out) {
   }
   if (upload(c, &c->dm, 1, &c->d_model)) { std::string e = g_err; rhg_close(c); return fail(RHG_ENOMEM, e); }
 
+  c->h_atom_level_ptr.assign(m->atom_level_ptr, m->atom_level_ptr + m->natom + 1);
+  c->h_nucl_atom.assign(m->nucl_atom, m->nucl_atom + m->nnuclei);
   c->nspect = ns;
   c->nout = nout;
   c->ncont = m->ncont;
@@ -320,6 +330,28 @@ int rhg_open(const rhg_model_t *m, int device, rhg_context_t **out) {
   }
   *out = c;
   g_err.clear();
+  return 0;
+}
+
+// Stage 1 (k_state) for a batch: one thread per (column, depth), the
+// chemical-equilibrium kernel instantiated for the model's nucleus count.
+static int launch_state(rhg_context_t *c, const double *d_atm, const int32_t *d_ndep, int ncol,
+                        int Nmax, double mu, int atm_scale, double *state, int nfield) {
+  const int64_t npts = (int64_t) ncol * Nmax;
+  dim3 blk(64), grd((unsigned) ((npts + 63) / 64));
+  const size_t lds = (size_t) 3 * c->dm.nmol * 64 * sizeof(double);
+  switch (c->dm.nnuclei) {
+#define RHG_LAUNCH_STATE(NN)                                                                    \
+  case NN:                                                                                      \
+    hipLaunchKernelGGL(rhg::k_state<NN>, grd, blk, lds, c->stream, c->d_model, d_atm, d_ndep, ncol, \
+                       Nmax, mu, atm_scale, state, nfield);                                     \
+    break;
+    RHG_LAUNCH_STATE(1) RHG_LAUNCH_STATE(2) RHG_LAUNCH_STATE(3) RHG_LAUNCH_STATE(4)
+    RHG_LAUNCH_STATE(5) RHG_LAUNCH_STATE(6) RHG_LAUNCH_STATE(7) RHG_LAUNCH_STATE(8)
+#undef RHG_LAUNCH_STATE
+    default: return fail(RHG_EUNSUPPORTED, "number of molecule nuclei out of range");
+  }
+  HIPCHECK(hipGetLastError());
   return 0;
 }
 
@@ -373,20 +405,8 @@ static int synth(rhg_context_t *c, const rhg_columns_t *cols, int32_t precision,
 
   HIPCHECK(hipEventRecord(c->ev[0], c->stream));
   {
-    dim3 blk(64), grd((unsigned) ((npts + 63) / 64));
-    const size_t lds = (size_t) 3 * c->dm.nmol * 64 * sizeof(double);
-    switch (c->dm.nnuclei) {
-#define RHG_LAUNCH_STATE(NN)                                                                   \
-  case NN:                                                                                     \
-    hipLaunchKernelGGL(rhg::k_state<NN>, grd, blk, lds, c->stream, c->d_model, d_atm, d_ndep, ncol, \
-                       Nmax, cols->mu, cols->atm_scale, state, nfield);                        \
-    break;
-      RHG_LAUNCH_STATE(1) RHG_LAUNCH_STATE(2) RHG_LAUNCH_STATE(3) RHG_LAUNCH_STATE(4)
-      RHG_LAUNCH_STATE(5) RHG_LAUNCH_STATE(6) RHG_LAUNCH_STATE(7) RHG_LAUNCH_STATE(8)
-#undef RHG_LAUNCH_STATE
-      default: return fail(RHG_EUNSUPPORTED, "number of molecule nuclei out of range");
-    }
-    HIPCHECK(hipGetLastError());
+    const int rc = launch_state(c, d_atm, d_ndep, ncol, Nmax, cols->mu, cols->atm_scale, state, nfield);
+    if (rc) return rc;
   }
   HIPCHECK(hipEventRecord(c->ev[1], c->stream));
   {
@@ -775,6 +795,420 @@ int rhg_rf_loggf(rhg_context_t *c, const rhg_columns_t *cols, int32_t npar,
     HIPCHECK(hipMemcpyAsync(rf_out, d_rf, rf_b, hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
   g_err.clear();
+  return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Host-side checks of an active atom against the model of the context and
+// what the NLTE kernels implement; fills the derived tables.
+struct NlteHost {
+  std::vector<int32_t> ln_tr, ln_phi, io_list;
+  int nentry = 0, nphi = 0;
+};
+
+int nlte_check(const rhg_context_t *c, const rhg_nlte_t *a, NlteHost &h) {
+  const int ns = c->nspect, nl = a->nlevel;
+  if (nl < 2 || nl > rhg::NLTE_MAXLEV) return fail(RHG_EUNSUPPORTED, "active atom: 2..16 levels");
+  if (a->atom_index < 0 || a->atom_index >= c->natom) return fail(RHG_EINVAL, "atom_index out of range");
+  if (c->h_atom_level_ptr[a->atom_index + 1] - c->h_atom_level_ptr[a->atom_index] != nl)
+    return fail(RHG_EINVAL, "active atom: level count differs from the model's atom");
+  if (a->he_atom >= c->natom) return fail(RHG_EINVAL, "he_atom out of range");
+  for (int32_t ia : c->h_nucl_atom)
+    if (ia == a->atom_index || ia == a->he_atom)
+      return fail(RHG_EUNSUPPORTED, "active atom or He bound in molecules");
+  if (a->nrays < 1 || a->nrays > 64) return fail(RHG_EINVAL, "nrays out of range");
+  if (a->ng_order < 0 || a->ng_order > rhg::NLTE_MAXNG) return fail(RHG_EUNSUPPORTED, "Ng order > 4");
+  if (a->ng_order > 0 && a->ng_period < 1) return fail(RHG_EINVAL, "Ng period < 1");
+  if (a->isum < -1 || a->isum >= nl) return fail(RHG_EINVAL, "isum out of range");
+  if (a->nmax_iter < 0 || a->nmax_scatter < 0) return fail(RHG_EINVAL, "negative iteration count");
+  if (a->ntrans < 1 || a->nline < 0) return fail(RHG_EINVAL, "no transitions");
+  h.ln_tr.assign(a->nline > 0 ? a->nline : 1, -1);
+  h.ln_phi.assign(a->nline > 0 ? a->nline : 1, 0);
+  if (a->tr_ptr[0] != 0) return fail(RHG_EINVAL, "tr_ptr[0] != 0");
+  for (int t = 0; t < a->ntrans; t++) {
+    const int i = a->tr_i[t], j = a->tr_j[t];
+    if (i < 0 || j <= i || j >= nl) return fail(RHG_EINVAL, "transition levels out of range");
+    if (a->tr_nblue[t] < 0 || a->tr_nlam[t] < 1 || a->tr_nblue[t] + a->tr_nlam[t] > ns)
+      return fail(RHG_EINVAL, "transition span outside the grid");
+    if (a->tr_ptr[t + 1] - a->tr_ptr[t] != a->tr_nlam[t]) return fail(RHG_EINVAL, "tr_ptr inconsistent");
+    if (a->tr_type[t] == 0) {
+      const int l = a->tr_line[t];
+      if (l < 0 || l >= a->nline || h.ln_tr[l] >= 0) return fail(RHG_EINVAL, "bad tr_line");
+      h.ln_tr[l] = t;
+      h.ln_phi[l] = h.nphi;
+      h.nphi += a->tr_nlam[t];
+    } else if (a->tr_type[t] != 1) {
+      return fail(RHG_EINVAL, "transition type must be 0 (line) or 1 (continuum)");
+    }
+  }
+  for (int l = 0; l < a->nline; l++)
+    if (h.ln_tr[l] < 0) return fail(RHG_EINVAL, "line without a transition");
+  h.nentry = a->tr_ptr[a->ntrans];
+  if (a->as_ptr[0] != 0) return fail(RHG_EINVAL, "as_ptr[0] != 0");
+  for (int ls = 0; ls < ns; ls++) {
+    const int n = a->as_ptr[ls + 1] - a->as_ptr[ls];
+    if (n < 0 || n > rhg::NLTE_MAXSET) return fail(RHG_EUNSUPPORTED, "more than 8 transitions active at one wavelength");
+    for (int e = a->as_ptr[ls]; e < a->as_ptr[ls + 1]; e++) {
+      const int t = a->as_tr[e];
+      if (t < 0 || t >= a->ntrans || ls < a->tr_nblue[t] || ls >= a->tr_nblue[t] + a->tr_nlam[t])
+        return fail(RHG_EINVAL, "active set entry outside its transition's span");
+    }
+    if (n > 0 && ls == c->dm.ref_index)
+      return fail(RHG_EUNSUPPORTED, "active transition at the reference wavelength");
+    if (n > 0) h.io_list.push_back(ls < c->dm.ref_index ? ls : ls - 1);
+  }
+  for (int t = 0; t < a->ntrans; t++)
+    for (int ls = a->tr_nblue[t]; ls < a->tr_nblue[t] + a->tr_nlam[t]; ls++) {
+      bool found = false;
+      for (int e = a->as_ptr[ls]; e < a->as_ptr[ls + 1]; e++) found |= a->as_tr[e] == t;
+      if (!found) return fail(RHG_EINVAL, "transition missing from an active set in its span");
+    }
+  for (int r = 0; r < a->ncoll; r++) {
+    const int n = a->coll_ptr[r + 1] - a->coll_ptr[r];
+    if (n < 2) return fail(RHG_EINVAL, "collision table with fewer than 2 temperatures");
+    for (int q = 1; q < n; q++)
+      if (!(a->coll_T[a->coll_ptr[r] + q] > a->coll_T[a->coll_ptr[r] + q - 1]))
+        return fail(RHG_EINVAL, "collision temperatures must ascend");
+    if (a->coll_i[r] < 0 || a->coll_i[r] >= nl || a->coll_j[r] < 0 || a->coll_j[r] >= nl ||
+        a->coll_type[r] < 0 || a->coll_type[r] > 6)
+      return fail(RHG_EINVAL, "bad collision entry");
+  }
+  if (h.io_list.empty()) return fail(RHG_EINVAL, "no active wavelength");
+  return 0;
+}
+
+// Copy the atom into one device buffer and point a DevNlte at it.
+int nlte_upload(rhg_context_t *c, const rhg_nlte_t *a, const NlteHost &h) {
+  std::vector<char> blob;
+  std::vector<std::pair<size_t, const void **>> fix;
+  rhg::DevNlte d{};
+  auto add = [&](const void *src, size_t n, size_t esz, const void **dst) {
+    size_t off = (blob.size() + 255) & ~(size_t) 255;
+    blob.resize(off + (n > 0 ? n : 1) * esz, 0);
+    if (n > 0 && src) memcpy(blob.data() + off, src, n * esz);
+    fix.push_back({off, dst});
+  };
+#define ADD(field, n) add(a->field, (size_t) (n), sizeof(*a->field), (const void **) &d.field)
+  const int nl = a->nlevel, nt = a->ntrans, nln = a->nline, ns = c->nspect;
+  const int ncw = a->ncoll > 0 ? a->coll_ptr[a->ncoll] : 0;
+  ADD(E, nl); ADD(g, nl); ADD(stage, nl);
+  ADD(tr_type, nt); ADD(tr_i, nt); ADD(tr_j, nt); ADD(tr_nblue, nt); ADD(tr_nlam, nt);
+  ADD(tr_ptr, nt + 1); ADD(tr_line, nt); ADD(tr_wlam, h.nentry); ADD(tr_alpha, h.nentry);
+  ADD(ln_lambda0, nln); ADD(ln_Aji, nln); ADD(ln_Bji, nln); ADD(ln_Bij, nln); ADD(ln_Grad, nln);
+  ADD(ln_vdw_cross, nln); ADD(ln_stark_c, nln); ADD(ln_stark_C, nln); ADD(ln_stark_Cm, nln);
+  ADD(ln_vdw, nln); ADD(ln_stark, nln);
+  ADD(as_ptr, ns + 1); ADD(as_tr, a->as_ptr[ns]); ADD(as_bezier, ns);
+  ADD(coll_type, a->ncoll); ADD(coll_i, a->ncoll); ADD(coll_j, a->ncoll); ADD(coll_ptr, a->ncoll + 1);
+  ADD(coll_T, ncw); ADD(coll_c, ncw); ADD(coll_M, ncw);
+  ADD(muz, a->nrays); ADD(wmu, a->nrays); ADD(ray_H, ns); ADD(ray_He, ns); ADD(ray_H2, ns);
+#undef ADD
+  add(h.ln_tr.data(), h.ln_tr.size(), sizeof(int32_t), (const void **) &d.ln_tr);
+  add(h.ln_phi.data(), h.ln_phi.size(), sizeof(int32_t), (const void **) &d.ln_phi);
+  if (c->nl_atom.ensure(blob.size()) || c->nl_dev.ensure(sizeof(rhg::DevNlte)))
+    return fail(RHG_ENOMEM, "device allocation for the active atom failed");
+  for (auto &f : fix) *f.second = (const char *) c->nl_atom.p + f.first;
+  d.nl = nl;
+  d.atom_index = a->atom_index;
+  d.he_atom = a->he_atom;
+  d.ntrans = nt;
+  d.nline = nln;
+  d.ncoll = a->ncoll;
+  d.nrays = a->nrays;
+  d.nentry = h.nentry;
+  d.nphi = h.nphi;
+  d.weight = a->weight;
+  d.sigma_thomson = a->sigma_thomson;
+  HIPCHECK(hipMemcpyAsync(c->nl_atom.p, blob.data(), blob.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(c->nl_dev.p, &d, sizeof(d), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+struct NlteTimer {
+  hipEvent_t e[2]{};
+  ~NlteTimer() {
+    for (auto &x : e)
+      if (x) (void) hipEventDestroy(x);
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int rhg_nlte_solve(rhg_context_t *c, const rhg_nlte_t *a, const rhg_columns_t *cols,
+                   int32_t device_ptrs, double *pops_out, int32_t *niter_out,
+                   double *dpops_out) {
+  if (!c || !a || !cols || !pops_out || !niter_out || !dpops_out)
+    return fail(RHG_EINVAL, "null argument");
+  const int ncol_all = cols->ncol, Nmax = cols->ndep_max, nout = c->nout, nl = a->nlevel;
+  if (ncol_all < 1 || Nmax < 4 || Nmax > RHG_MAX_DEPTH) return fail(RHG_EINVAL, "bad batch shape");
+  if (!(cols->mu > 0.0 && cols->mu <= 1.0)) return fail(RHG_EINVAL, "mu must be in (0, 1]");
+  if (cols->atm_scale < 0 || cols->atm_scale > 2) return fail(RHG_EINVAL, "bad atm_scale");
+  NlteHost h;
+  if (int rc = nlte_check(c, a, h)) return rc;
+  std::vector<int32_t> hndep(ncol_all);
+  HIPCHECK(hipSetDevice(c->device));
+  if (device_ptrs) {
+    HIPCHECK(hipMemcpy(hndep.data(), cols->ndep, ncol_all * sizeof(int32_t), hipMemcpyDeviceToHost));
+  } else {
+    memcpy(hndep.data(), cols->ndep, ncol_all * sizeof(int32_t));
+  }
+  for (int i = 0; i < ncol_all; i++)
+    if (hndep[i] < 4 || hndep[i] > Nmax) return fail(RHG_EINVAL, "ndep out of range");
+  if (int rc = nlte_upload(c, a, h)) return rc;
+  const rhg::DevNlte *d_at = (const rhg::DevNlte *) c->nl_dev.p;
+  const int nlist = (int) h.io_list.size(), nrays = a->nrays, NS = a->ng_order + 2;
+  const int nfield = rhg::state_nfield(c->ncont, c->nline);
+
+  // device bytes per column; columns are solved in chunks that fit in 3/4
+  // of the free device memory
+  const size_t per_col =
+      (size_t) Nmax * sizeof(double) *
+          ((size_t) nfield + 5 * (size_t) nout + 2 * (size_t) nl + 3 + 2 * (size_t) a->nline +
+           2 * (size_t) nl * nl + (size_t) h.nphi * nrays * 2 + 2 * (size_t) h.nentry +
+           (size_t) rhg::NLTE_NSLOT * nlist + (size_t) NS * nl) +
+      (size_t) nlist * sizeof(double) + 9 * (size_t) Nmax * sizeof(double) + 64;
+  size_t freeb = 0, totb = 0;
+  HIPCHECK(hipMemGetInfo(&freeb, &totb));
+  const size_t budget = freeb / 4 * 3;
+  int chunk = (int) std::min<size_t>((size_t) ncol_all, budget / per_col);
+  if (chunk > 64) chunk = chunk / 64 * 64;
+  if (chunk < 1) return fail(RHG_ENOMEM, "device memory too small for one NLTE column");
+
+  NlteTimer tm_all, tm_k;
+  for (auto &x : tm_all.e) HIPCHECK(hipEventCreate(&x));
+  for (auto &x : tm_k.e) HIPCHECK(hipEventCreate(&x));
+  for (double &x : c->nlte_ms) x = 0.0;
+  HIPCHECK(hipEventRecord(tm_all.e[0], c->stream));
+  float ms = 0.0f;
+  // time one launch sequence on the stream into nlte_ms[slot]
+  auto timed = [&](int slot, auto &&launch) -> int {
+    HIPCHECK(hipEventRecord(tm_k.e[0], c->stream));
+    if (int rc = launch()) return rc;
+    HIPCHECK(hipEventRecord(tm_k.e[1], c->stream));
+    HIPCHECK(hipEventSynchronize(tm_k.e[1]));
+    HIPCHECK(hipEventElapsedTime(&ms, tm_k.e[0], tm_k.e[1]));
+    c->nlte_ms[slot] += ms;
+    return 0;
+  };
+
+  if (c->nl_iolist.ensure((size_t) nlist * sizeof(int32_t)))
+    return fail(RHG_ENOMEM, "device allocation (NLTE wavelength list)");
+  HIPCHECK(hipMemcpyAsync(c->nl_iolist.p, h.io_list.data(), nlist * sizeof(int32_t),
+                          hipMemcpyHostToDevice, c->stream));
+  const int32_t *d_iolist = (const int32_t *) c->nl_iolist.p;
+
+  for (int c0 = 0; c0 < ncol_all; c0 += chunk) {
+    const int ncol = std::min(chunk, ncol_all - c0);
+    const size_t pts = (size_t) ncol * Nmax * sizeof(double);
+    const int ncb_max = (ncol + 63) / 64;
+    if (c->state.ensure(pts * nfield) || c->cchi.ensure(pts * nout) || c->ceta.ensure(pts * nout) ||
+        c->moving.ensure(ncol * sizeof(int32_t)) || c->nl_nstar.ensure(pts * nl) ||
+        c->nl_pops.ensure(pts * nl) || c->nl_ntot.ensure(pts) || c->nl_vbroad.ensure(pts) ||
+        c->nl_nHe.ensure(pts) || c->nl_adamp.ensure(pts * std::max(a->nline, 1)) ||
+        c->nl_wphi.ensure(pts * std::max(a->nline, 1)) || c->nl_Cmat.ensure(pts * nl * nl) ||
+        c->nl_Gam.ensure(pts * nl * nl) ||
+        c->nl_phi.ensure(pts * std::max(h.nphi, 1) * nrays * 2) || c->nl_csca.ensure(pts * nout) ||
+        c->nl_J.ensure(pts * nout) || c->nl_acc.ensure(pts * h.nentry * 2) ||
+        c->nl_scr.ensure((size_t) rhg::NLTE_NSLOT * Nmax * nlist * ncb_max * 64 * sizeof(double)) ||
+        c->nl_dJ.ensure((size_t) nlist * ncol * sizeof(double)) || c->nl_hist.ensure(pts * NS * nl) ||
+        c->nl_act.ensure(ncol * sizeof(int32_t)) || c->nl_red.ensure(ncol * sizeof(double)) ||
+        (!device_ptrs && (c->nl_atm.ensure(pts * 9) || c->nl_ndep.ensure(ncol * sizeof(int32_t)) ||
+                          c->nl_out.ensure(pts * nl))))
+      return fail(RHG_ENOMEM, "device allocation for the NLTE batch failed");
+    const double *d_atm = (const double *) cols->atmos + (size_t) c0 * 9 * Nmax;
+    const int32_t *d_ndep = cols->ndep + c0;
+    if (!device_ptrs) {
+      HIPCHECK(hipMemcpyAsync(c->nl_atm.p, d_atm, pts * 9, hipMemcpyHostToDevice, c->stream));
+      HIPCHECK(hipMemcpyAsync(c->nl_ndep.p, d_ndep, ncol * sizeof(int32_t), hipMemcpyHostToDevice,
+                              c->stream));
+      d_atm = (const double *) c->nl_atm.p;
+      d_ndep = (const int32_t *) c->nl_ndep.p;
+    }
+    rhg::NlteBufs B{};
+    B.ncol = ncol;
+    B.Nmax = Nmax;
+    B.nfield = nfield;
+    B.nout = nout;
+    B.ndep = d_ndep;
+    B.state = (double *) c->state.p;
+    B.nstar = (double *) c->nl_nstar.p;
+    B.pops = (double *) c->nl_pops.p;
+    B.ntot = (double *) c->nl_ntot.p;
+    B.vbroad = (double *) c->nl_vbroad.p;
+    B.nHe = (double *) c->nl_nHe.p;
+    B.adamp = (double *) c->nl_adamp.p;
+    B.wphi = (double *) c->nl_wphi.p;
+    B.Cmat = (double *) c->nl_Cmat.p;
+    B.Gam = (double *) c->nl_Gam.p;
+    B.phi = (double *) c->nl_phi.p;
+    B.cchi = (double *) c->cchi.p;
+    B.ceta = (double *) c->ceta.p;
+    B.csca = (double *) c->nl_csca.p;
+    B.J = (double *) c->nl_J.p;
+    B.acc = (double *) c->nl_acc.p;
+    B.scr = (double *) c->nl_scr.p;
+    B.dJ = (double *) c->nl_dJ.p;
+    B.hist = (double *) c->nl_hist.p;
+    int32_t *d_act = (int32_t *) c->nl_act.p;
+    double *d_red = (double *) c->nl_red.p;
+    const int64_t npts = (int64_t) ncol * Nmax;
+    const dim3 b64(64), gpts((unsigned) ((npts + 63) / 64));
+
+    // set-up: column state, scattering in chi_ref, depth scale, background,
+    // populations, collisions, profiles
+    int rc = timed(0, [&]() -> int {
+      if (int r = launch_state(c, d_atm, d_ndep, ncol, Nmax, cols->mu, cols->atm_scale, B.state, nfield))
+        return r;
+      hipLaunchKernelGGL(rhg::k_nlte_pre, gpts, b64, 0, c->stream, c->d_model, d_at, B);
+      HIPCHECK(hipGetLastError());
+      hipLaunchKernelGGL(rhg::k_geometry, dim3((unsigned) ((ncol + 63) / 64)), b64, 0, c->stream,
+                         c->d_model, d_ndep, ncol, Nmax, cols->atm_scale, cols->mu, B.state, nfield,
+                         (int32_t *) c->moving.p, (double *) nullptr);
+      HIPCHECK(hipGetLastError());
+      hipLaunchKernelGGL(rhg::k_continuum<double>,
+                         dim3((unsigned) ((npts + rhg::CONT_BLOCK - 1) / rhg::CONT_BLOCK)),
+                         dim3(rhg::CONT_BLOCK), 0, c->stream, c->d_model, d_ndep, ncol, Nmax,
+                         B.state, nfield, c->d_out_ls, nout, B.cchi, B.ceta);
+      HIPCHECK(hipGetLastError());
+      hipLaunchKernelGGL(rhg::k_nlte_bg, gpts, b64, 0, c->stream, c->d_model, d_at, B,
+                         c->d_out_ls);
+      HIPCHECK(hipGetLastError());
+      if (a->nline > 0) {
+        const int64_t np = npts * a->nline;
+        hipLaunchKernelGGL(rhg::k_nlte_profile, dim3((unsigned) ((np + 63) / 64)), b64, 0,
+                           c->stream, c->d_model, d_at, B);
+        HIPCHECK(hipGetLastError());
+      }
+      HIPCHECK(hipMemsetAsync(B.J, 0, pts * nout, c->stream));
+      return 0;
+    });
+    if (rc) return rc;
+
+    std::vector<int32_t> act(ncol), keep;
+    std::vector<double> red(ncol);
+    for (int i = 0; i < ncol; i++) act[i] = i;
+    auto upload_act = [&]() -> int {
+      HIPCHECK(hipMemcpyAsync(d_act, act.data(), act.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                              c->stream));
+      return 0;
+    };
+    auto rt = [&](int slot, int eval) -> int {
+      const int nact = (int) act.size();
+      return timed(slot, [&]() -> int {
+        dim3 grd((unsigned) rhg::xcd_grid((nact + 63) / 64, nlist));
+        hipLaunchKernelGGL(rhg::k_nlte_rt, grd, b64, 0, c->stream, c->d_model, d_at, B, d_act, nact,
+                           c->d_out_ls, d_iolist, nlist, eval);
+        HIPCHECK(hipGetLastError());
+        c->nlte_ms[6] += 1.0;
+        c->nlte_ms[7] += (double) nact * nlist;
+        return 0;
+      });
+    };
+    // per-column reduction of the active columns into red[]
+    auto fetch_red = [&]() -> int {
+      HIPCHECK(hipMemcpyAsync(red.data(), d_red, ncol * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIPCHECK(hipStreamSynchronize(c->stream));
+      return 0;
+    };
+
+    // background scattering passes without the operator (each column stops
+    // when its own maximum change of J is below ITER_LIMIT)
+    if (a->nmax_iter > 0) {
+      if ((rc = upload_act())) return rc;
+      for (int it = 0; it < a->nmax_scatter && !act.empty(); it++) {
+        if ((rc = rt(1, 0))) return rc;
+        const int nact = (int) act.size();
+        hipLaunchKernelGGL(rhg::k_nlte_djmax, dim3((unsigned) ((nact + 63) / 64)), b64, 0, c->stream,
+                           B, d_act, nact, nlist, d_red);
+        HIPCHECK(hipGetLastError());
+        if ((rc = fetch_red())) return rc;
+        keep.clear();
+        for (int ic : act)
+          if (!(red[ic] < a->iter_limit)) keep.push_back(ic);
+        act.swap(keep);
+        if ((rc = upload_act())) return rc;
+      }
+    }
+
+    // accelerated lambda iterations
+    std::vector<int32_t> niter(ncol, 0);
+    std::vector<double> dpops(ncol, 0.0);
+    act.resize(ncol);
+    for (int i = 0; i < ncol; i++) act[i] = i;
+    if ((rc = upload_act())) return rc;
+    HIPCHECK(hipMemcpyAsync(B.hist, B.pops, pts * nl, hipMemcpyDeviceToDevice, c->stream));
+    const int ndelay = std::max(a->ng_delay, a->ng_order + 2);
+    for (int it = 1; it <= a->nmax_iter && !act.empty(); it++) {
+      const int nact = (int) act.size();
+      if ((rc = rt(2, 1))) return rc;
+      rc = timed(3, [&]() -> int {
+        const int64_t n = (int64_t) nact * Nmax;
+        const dim3 g((unsigned) ((n + 63) / 64));
+        if (nl <= 4)
+          hipLaunchKernelGGL(rhg::k_nlte_stateq<4>, g, b64, 0, c->stream, d_at, B, d_act, nact, a->isum);
+        else if (nl <= 8)
+          hipLaunchKernelGGL(rhg::k_nlte_stateq<8>, g, b64, 0, c->stream, d_at, B, d_act, nact, a->isum);
+        else
+          hipLaunchKernelGGL(rhg::k_nlte_stateq<16>, g, b64, 0, c->stream, d_at, B, d_act, nact, a->isum);
+        HIPCHECK(hipGetLastError());
+        return 0;
+      });
+      if (rc) return rc;
+      rc = timed(4, [&]() -> int {
+        hipLaunchKernelGGL(rhg::k_nlte_ng, dim3((unsigned) ((nact + 63) / 64)), b64, 0, c->stream, d_at,
+                           B, d_act, nact, it, a->ng_order, ndelay, a->ng_period, d_red);
+        HIPCHECK(hipGetLastError());
+        return 0;
+      });
+      if (rc) return rc;
+      if ((rc = fetch_red())) return rc;
+      keep.clear();
+      for (int ic : act) {
+        niter[ic] = it;
+        dpops[ic] = red[ic];
+        if (!(red[ic] < a->iter_limit)) keep.push_back(ic);
+      }
+      act.swap(keep);
+      if (!act.empty() && (rc = upload_act())) return rc;
+    }
+
+    // outputs of the chunk
+    double *d_out = device_ptrs ? pops_out + (size_t) c0 * nl * Nmax : (double *) c->nl_out.p;
+    {
+      const int64_t n = (int64_t) ncol * nl * Nmax;
+      hipLaunchKernelGGL(rhg::k_nlte_out, dim3((unsigned) ((n + 63) / 64)), b64, 0, c->stream, d_at, B,
+                         d_out);
+      HIPCHECK(hipGetLastError());
+    }
+    if (device_ptrs) {
+      HIPCHECK(hipMemcpyAsync(niter_out + c0, niter.data(), ncol * sizeof(int32_t), hipMemcpyHostToDevice,
+                              c->stream));
+      HIPCHECK(hipMemcpyAsync(dpops_out + c0, dpops.data(), ncol * sizeof(double), hipMemcpyHostToDevice,
+                              c->stream));
+    } else {
+      HIPCHECK(hipMemcpyAsync(pops_out + (size_t) c0 * nl * Nmax, d_out, pts * nl, hipMemcpyDeviceToHost,
+                              c->stream));
+      memcpy(niter_out + c0, niter.data(), ncol * sizeof(int32_t));
+      memcpy(dpops_out + c0, dpops.data(), ncol * sizeof(double));
+    }
+    HIPCHECK(hipStreamSynchronize(c->stream));
+  }
+  HIPCHECK(hipEventRecord(tm_all.e[1], c->stream));
+  HIPCHECK(hipEventSynchronize(tm_all.e[1]));
+  HIPCHECK(hipEventElapsedTime(&ms, tm_all.e[0], tm_all.e[1]));
+  c->nlte_ms[5] = ms;
+  g_err.clear();
+  return 0;
+}
+
+int rhg_nlte_timings(rhg_context_t *ctx, double *out) {
+  if (!ctx || !out) return fail(RHG_EINVAL, "null argument");
+  for (int s = 0; s < RHG_NLTE_NTIMING; s++) out[s] = ctx->nlte_ms[s];
   return 0;
 }
 

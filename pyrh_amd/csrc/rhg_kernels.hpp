@@ -102,6 +102,27 @@ __host__ __device__ __forceinline__ int state_nfield(int ncont, int nline) {
   return RHG_CF_NFIXED + ncont + 4 * nline + 1;
 }
 
+// Saha-Boltzmann ratio of level i to level 0 (before normalisation),
+// rh/ltepops.c:87-100
+__device__ __forceinline__ double level_ratio(const DevModel &M, int l0, int i, double T,
+                                              double cNe_T) {
+  double dE = M.level_E[l0 + i] - M.level_E[l0];
+  double gi0 = M.level_g[l0 + i] / M.level_g[l0];
+  int dZ = M.level_stage[l0 + i] - M.level_stage[l0];
+  double dE_kT = dE / (KBOLTZMANN * T);
+  double r = gi0 * exp(-dE_kT);
+  for (int m = 1; m <= dZ; m++) r /= cNe_T;
+  return r;
+}
+
+// 1 + sum of the level ratios of an atom (the Saha-Boltzmann normalisation)
+__device__ __forceinline__ double atom_sum(const DevModel &M, int l0, int nl, double T,
+                                           double cNe_T) {
+  double sum = 1.0;
+  for (int i = 1; i < nl; i++) sum += level_ratio(M, l0, i, T, cNe_T);
+  return sum;
+}
+
 // Profile of entry e (line l) of wavelength ls's line list at one depth,
 // RLKProfile rh/kurucz.c:729-827 for the emergent ray (to_obs = 1), in the
 // number type T (double, float, or a dual number of either).  F(field)

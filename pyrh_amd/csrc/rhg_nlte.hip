@@ -1,0 +1,877 @@
+// NLTE populations of one active atom per column: the accelerated lambda
+// iteration of rhf1d() (rh/iterate.c:48-143) as a sequence of batched
+// kernels over all columns at once.
+//
+// Set-up (once per batch, after k_state of the background model)
+//   k_nlte_pre      (column, depth): scattering opacity at the reference
+//                   wavelength (the depth scale), LTE populations, Doppler
+//                   widths, damping and the collision matrix.
+//   k_nlte_bg       (column, depth): scattering opacity at every wavelength,
+//                   added to the LTE continuum opacity.
+//   k_nlte_profile  (column, depth, line): absorption profiles for every
+//                   ray and direction, and their normalisation.
+// Iteration (per pass, over the columns still iterating)
+//   k_nlte_rt       (column, wavelength): the rays of the wavelength with the
+//                   diagonal operator; new mean intensity and, with the
+//                   operator, the radiative rate terms of every transition
+//                   active there.
+//   k_nlte_stateq   (column, depth): rate matrix = collisions + the rate
+//                   terms summed over the transitions' wavelengths; the
+//                   statistical-equilibrium solve.
+//   k_nlte_ng       (column): Ng acceleration and the maximum relative
+//                   change.
+#include "rhg_nlte.hpp"
+
+namespace rhg {
+
+namespace {
+
+// rh/planck.c:38-63 for a wavelength in nm
+__device__ __forceinline__ double planck_lam(double T, double lambda) {
+  const double hc_kla = (HPLANCK * CLIGHT) / (KBOLTZMANN * NM_TO_M * lambda);
+  const double twohnu3_c2 = (2.0 * HPLANCK * CLIGHT) / ((NM_TO_M * lambda) * (NM_TO_M * lambda) *
+                                                         (NM_TO_M * lambda));
+  const double hc_Tkla = hc_kla / T;
+  return (hc_Tkla <= MAX_EXPONENT) ? twohnu3_c2 / (exp(hc_Tkla) - 1.0) : 0.0;
+}
+
+// Fritsch & Butland centred derivative with the reference's divisions,
+// rh/bezier_aux.c:34-51
+__device__ __forceinline__ double cent_div(double dsup, double dsdn, double up, double c,
+                                           double dn) {
+  const double fim1 = (c - up) / dsup, fi = (dn - c) / dsdn;
+  if (fim1 * fi > 0) {
+    const double alpha = 0.333333333333333333333333 * (1.0 + dsdn / (dsdn + dsup));
+    return (fim1 * fi) / ((1.0 - alpha) * fim1 + alpha * fi);
+  }
+  return 0.0;
+}
+
+__device__ __forceinline__ double max0(double x) { return x > 0.0 ? x : 0.0; }
+
+// RH's splineEval with precomputed second derivatives, rh/spline.c:70-97
+__device__ double spline_eval_M(int n, const double *x, const double *y, const double *Mv,
+                                double xv) {
+  if (xv <= x[0]) return y[0];
+  if (xv >= x[n - 1]) return y[n - 1];
+  const int j = locate_asc(n, x, xv);
+  const double hj = x[j + 1] - x[j];
+  const double fx = (xv - x[j]) / hj;
+  const double fx1 = 1 - fx;
+  return fx1 * y[j] + fx * y[j + 1] +
+         (fx1 * (fx1 * fx1 - 1) * Mv[j] + fx * (fx * fx - 1) * Mv[j + 1]) * (hj * hj) / 6.0;
+}
+
+constexpr double HC = HPLANCK * CLIGHT;
+constexpr double FOURPI = 4.0 * PI;
+constexpr double HC_4PI = HC / FOURPI;
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Set-up per (column, depth).  Scattering opacity at the reference
+// wavelength joins chi_ref before convertScales (rh/background.c:461-463);
+// LTE populations of the active atom (rh/ltepops.c:33-113), its total and
+// Doppler width (rh/readatom.c:213-220); damping of each line
+// (rh/broad.c:273-316); collision rates (rh/collision.c:450-760).
+__global__ void k_nlte_pre(const DevModel *__restrict__ Mp, const DevNlte *__restrict__ Ap,
+                           NlteBufs B) {
+  const DevModel &M = *Mp;
+  const DevNlte &A = *Ap;
+  const int64_t p = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  const int ncol = B.ncol, Nmax = B.Nmax;
+  if (p >= (int64_t) ncol * Nmax) return;
+  const int ic = (int) (p % ncol), k = (int) (p / ncol);
+  if (k >= B.ndep[ic]) return;
+  const int64_t fs = (int64_t) Nmax * ncol;
+  double *st = B.state + (int64_t) k * ncol + ic;
+  const double T = st[RHG_CF_T * fs], ne = st[RHG_CF_NE * fs], nHtot = st[RHG_CF_NHTOT * fs];
+  const double nH0 = st[RHG_CF_NH0 * fs], np = st[RHG_CF_NP * fs], nH2 = st[RHG_CF_NH2 * fs];
+  const double vturb = st[RHG_CF_VTURB * fs];
+  const double c1 = (HPLANCK / (2.0 * PI * M_ELECTRON)) * (HPLANCK / KBOLTZMANN);
+  const double cNe_T = 0.5 * ne * pow(c1 / T, 1.5);
+  const int64_t pk = (int64_t) k * ncol + ic;
+
+  double nHe = 0.0;
+  if (A.he_atom >= 0) {
+    const int l0 = M.atom_level_ptr[A.he_atom], nl = M.atom_level_ptr[A.he_atom + 1] - l0;
+    nHe = (M.atom_abund[A.he_atom] * nHtot) / atom_sum(M, l0, nl, T, cNe_T);
+  }
+  B.nHe[pk] = nHe;
+  {
+    const int lr = M.ref_index;
+    double sca = ne * A.sigma_thomson;
+    sca += A.ray_H[lr] * nH0;
+    if (A.he_atom >= 0) sca += A.ray_He[lr] * nHe;
+    sca += A.ray_H2[lr] * nH2;
+    st[RHG_CF_HEIGHT * fs] += sca;
+  }
+
+  const int nl = A.nl, a = A.atom_index, l0 = M.atom_level_ptr[a];
+  const double ntotal = M.atom_abund[a] * nHtot;
+  const double n0 = ntotal / atom_sum(M, l0, nl, T, cNe_T);
+  for (int i = 0; i < nl; i++) {
+    const double ns = (i == 0) ? n0 : level_ratio(M, l0, i, T, cNe_T) * n0;
+    B.nstar[(int64_t) i * fs + pk] = ns;
+    B.pops[(int64_t) i * fs + pk] = ns;
+  }
+  B.ntot[pk] = ntotal;
+  const double vtherm = 2.0 * KBOLTZMANN / (AMU * A.weight);
+  const double vbroad = sqrt(vtherm * T + vturb * vturb);
+  B.vbroad[pk] = vbroad;
+
+  for (int l = 0; l < A.nline; l++) {
+    const double cDop = (NM_TO_M * A.ln_lambda0[l]) / (4.0 * PI);
+    double Qelast = 0.0;
+    if (A.ln_vdw[l]) {
+      double G = A.ln_vdw_cross[l] * pow(T, 0.3);
+      G *= nH0;
+      Qelast += G;
+    }
+    if (A.ln_stark[l] == 1) {
+      Qelast += A.ln_stark_c[l] * ne;
+    } else if (A.ln_stark[l] == 2) {
+      const double vrel = pow(A.ln_stark_C[l] * T, 0.16666667) * A.ln_stark_Cm[l];
+      Qelast += A.ln_stark_c[l] * vrel * ne;
+    }
+    B.adamp[(int64_t) l * fs + pk] = (A.ln_Grad[l] + Qelast) * cDop / vbroad;
+  }
+
+  for (int q = 0; q < nl * nl; q++) B.Cmat[(int64_t) q * fs + pk] = 0.0;
+  const double C0 = ((E_RYDBERG / sqrt(M_ELECTRON)) * PI * (RBOHR * RBOHR)) *
+                    sqrt(8.0 / (PI * KBOLTZMANN));
+  for (int r = 0; r < A.ncoll; r++) {
+    const int i1 = A.coll_i[r], j1 = A.coll_j[r], c0 = A.coll_ptr[r], nT = A.coll_ptr[r + 1] - c0;
+    const double Cv = nT > 2 ? spline_eval_M(nT, A.coll_T + c0, A.coll_c + c0, A.coll_M + c0, T)
+                             : linear1(nT, A.coll_T + c0, A.coll_c + c0, T);
+    double *Cij = B.Cmat + (int64_t) (i1 * nl + j1) * fs + pk;
+    double *Cji = B.Cmat + (int64_t) (j1 * nl + i1) * fs + pk;
+    const double nsi = B.nstar[(int64_t) i1 * fs + pk], nsj = B.nstar[(int64_t) j1 * fs + pk];
+    double Cdown, Cup;
+    switch (A.coll_type[r]) {
+      case 0:   // OMEGA
+        Cdown = C0 * ne * Cv / (A.g[j1] * sqrt(T));
+        *Cij += Cdown;
+        *Cji += Cdown * nsj / nsi;
+        break;
+      case 1:   // CE
+        Cdown = Cv * ne * (A.g[i1] / A.g[j1]) * sqrt(T);
+        *Cij += Cdown;
+        *Cji += Cdown * nsj / nsi;
+        break;
+      case 2:   // CI
+        Cup = Cv * ne * exp(-(A.E[j1] - A.E[i1]) / (KBOLTZMANN * T)) * sqrt(T);
+        *Cji += Cup;
+        *Cij += Cup * nsi / nsj;
+        break;
+      case 3:   // CP
+        Cdown = np * Cv;
+        *Cij += Cdown;
+        *Cji += Cdown * nsj / nsi;
+        break;
+      case 4:   // CH
+        Cup = nH0 * Cv;
+        *Cji += Cup;
+        *Cij += Cup * nsi / nsj;
+        break;
+      case 5:   // CH0
+        *Cij += nH0 * Cv;
+        break;
+      case 6:   // CH+
+        *Cji += np * Cv;
+        break;
+    }
+  }
+}
+
+// Scattering opacity (Thomson and Rayleigh on H, He and H2) at every output
+// wavelength: stored, and added to the LTE continuum opacity
+// (rh/background.c:342-465).
+__global__ void k_nlte_bg(const DevModel *__restrict__ Mp, const DevNlte *__restrict__ Ap,
+                          NlteBufs B, const int32_t *__restrict__ out_ls) {
+  const DevNlte &A = *Ap;
+  const int64_t p = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  const int ncol = B.ncol, Nmax = B.Nmax;
+  if (p >= (int64_t) ncol * Nmax) return;
+  const int ic = (int) (p % ncol), k = (int) (p / ncol);
+  if (k >= B.ndep[ic]) return;
+  const int64_t fs = (int64_t) Nmax * ncol;
+  const double *st = B.state + (int64_t) k * ncol + ic;
+  const double ne = st[RHG_CF_NE * fs], nH0 = st[RHG_CF_NH0 * fs], nH2 = st[RHG_CF_NH2 * fs];
+  const double nHe = B.nHe[(int64_t) k * ncol + ic];
+  (void) Mp;
+  for (int io = 0; io < B.nout; io++) {
+    const int ls = out_ls[io];
+    double sca = ne * A.sigma_thomson;
+    sca += A.ray_H[ls] * nH0;
+    if (A.he_atom >= 0) sca += A.ray_He[ls] * nHe;
+    sca += A.ray_H2[ls] * nH2;
+    const int64_t q = (int64_t) io * fs + (int64_t) k * ncol + ic;
+    B.csca[q] = sca;
+    B.cchi[q] += sca;
+  }
+}
+
+// Field-free absorption profiles of the atom's lines for every ray and
+// direction and their normalisation over the line's quadrature,
+// rh/profile.c:67-371 (Voigt function of Armstrong, rh/voigt.c:126-243).
+__global__ void k_nlte_profile(const DevModel *__restrict__ Mp, const DevNlte *__restrict__ Ap,
+                               NlteBufs B) {
+  const DevModel &M = *Mp;
+  const DevNlte &A = *Ap;
+  const int64_t p = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  const int ncol = B.ncol, Nmax = B.Nmax;
+  if (p >= (int64_t) ncol * Nmax * A.nline) return;
+  const int ic = (int) (p % ncol);
+  const int64_t q = p / ncol;
+  const int k = (int) (q % Nmax), l = (int) (q / Nmax);
+  if (k >= B.ndep[ic]) return;
+  const int64_t fs = (int64_t) Nmax * ncol, pk = (int64_t) k * ncol + ic;
+  const double vb = B.vbroad[pk], vz = B.state[RHG_CF_VLOS * fs + pk];
+  const double adamp = B.adamp[(int64_t) l * fs + pk];
+  const int t = A.ln_tr[l], nrays = A.nrays;
+  const double lambda0 = A.ln_lambda0[l];
+  double wphi = 0.0;
+  for (int la = 0; la < A.tr_nlam[t]; la++) {
+    const int ls = A.tr_nblue[t] + la;
+    const double lamla = M.lamconst[(int64_t) ls * RHG_LC_NCONST + RHG_LC_LAMBDA];
+    const double wl = A.tr_wlam[A.tr_ptr[t] + la];
+    const double v = (lamla - lambda0 - 0.0) * CLIGHT / (vb * lambda0);
+    for (int mu = 0; mu < nrays; mu++) {
+      const double wlamu = wl * 0.5 * A.wmu[mu];
+      const double vlos = A.muz[mu] * vz / vb;
+      for (int to_obs = 0; to_obs <= 1; to_obs++) {
+        const double sign = to_obs ? 1.0 : -1.0;
+        const double vk = v + sign * vlos;
+        const double ph = voigt_armstrong(adamp, vk) * 1.0 / (SQRTPI * vb);
+        B.phi[((((int64_t) (A.ln_phi[l] + la) * nrays + mu) * 2 + to_obs) * Nmax) * ncol + pk] = ph;
+        wphi += ph * wlamu;
+      }
+    }
+  }
+  B.wphi[(int64_t) l * fs + pk] = 1.0 / wphi;
+}
+
+// ---------------------------------------------------------------------------
+// One wavelength of one column: the formal solution of every ray with the
+// transitions active there (rh/rhf1d/formal.c:44-346), the new mean
+// intensity, and with eval_op the radiative rate terms of the transitions
+// (addtoCoupling and addtoGamma, rh/fillgamma.c:82-332).
+//
+// Wavelengths with an active line are solved ray by ray with the scalar
+// cubic Bezier method and its diagonal operator (rh/rhf1d/bezier_1D.c:
+// 306-541), both directions; the others with the Feautrier method per
+// ray, whose P is the mean of the two directions (rh/rhf1d/feautrier.c:
+// 56-202).  For each ray the kernel first evaluates the opacity and source
+// function at every depth into its scratch rows, then sweeps the ray; as
+// each depth's intensity and operator are known, its rate terms are added
+// to the accumulators of the transitions' quadrature points.  Each
+// accumulator belongs to one (column, wavelength) thread, so the sums over
+// the rays need no atomics; the sum over wavelengths is done per depth by
+// k_nlte_stateq, in wavelength order.
+//
+// Workgroups map to (64 columns of the active list, list wavelength) by
+// xcd_tile, so a block of columns' state and populations stay in one
+// XCD's L2 across the wavelengths.
+__global__ void __launch_bounds__(64)
+k_nlte_rt(const DevModel *__restrict__ Mp, const DevNlte *__restrict__ Ap, NlteBufs B,
+          const int32_t *__restrict__ act, int nact, const int32_t *__restrict__ out_ls,
+          const int32_t *__restrict__ io_list, int nlist, int eval_op) {
+  const DevModel &M = *Mp;
+  const DevNlte &A = *Ap;
+  const int ncb = (nact + 63) / 64;
+  int cb, il;
+  xcd_tile(blockIdx.x, nlist, cb, il);
+  if (cb >= ncb) return;
+  const int slot = cb * 64 + threadIdx.x;
+  if (slot >= nact) return;
+  const int ic = act[slot], io = io_list[il], ls = out_ls[io];
+  const int ncol = B.ncol, Nmax = B.Nmax, N = B.ndep[ic], nl = A.nl;
+  const int64_t fs = (int64_t) Nmax * ncol;
+  const int64_t nthr = (int64_t) nlist * ncb * 64;
+  double *scr = B.scr + (int64_t) il * ncb * 64 + slot;
+#define SCR(s, k) scr[((int64_t) (s) * Nmax + (k)) * nthr]
+#define STF(f, k) B.state[(int64_t) (f) * fs + (int64_t) (k) * ncol + ic]
+#define PK(k) ((int64_t) (k) * ncol + ic)
+  const int64_t wo = (int64_t) io * fs;
+  double *Jl = B.J + wo;
+  const double *chic = B.cchi + wo, *etac = B.ceta + wo, *scac = B.csca + wo;
+  const int p0 = A.as_ptr[ls], nset = A.as_ptr[ls + 1] - p0;
+  const double lam = M.lamconst[(int64_t) ls * RHG_LC_NCONST + RHG_LC_LAMBDA];
+  const double lam3 = lam * lam * lam;
+  const double twohc = 2.0 * HC / (NM_TO_M * NM_TO_M * NM_TO_M);
+  const double hc_k = HC / (KBOLTZMANN * NM_TO_M);
+  const int nrays = A.nrays;
+
+  // the set's transitions, levels and quadrature entries
+  int tt[NLTE_MAXSET], ti[NLTE_MAXSET], tj[NLTE_MAXSET], tent[NLTE_MAXSET];
+#pragma unroll
+  for (int n = 0; n < NLTE_MAXSET; n++) {
+    if (n < nset) {
+      const int t = A.as_tr[p0 + n];
+      tt[n] = t;
+      ti[n] = A.tr_i[t];
+      tj[n] = A.tr_j[t];
+      tent[n] = A.tr_ptr[t] + (ls - A.tr_nblue[t]);
+    } else {
+      tt[n] = ti[n] = tj[n] = tent[n] = 0;
+    }
+  }
+
+  // previous J, and the new one from zero
+  for (int k = 0; k < N; k++) {
+    SCR(NS_JDAG, k) = Jl[PK(k)];
+    Jl[PK(k)] = 0.0;
+  }
+  if (eval_op) {
+    for (int n = 0; n < nset; n++)
+      for (int s = 0; s < 2; s++) {
+        double *a = B.acc + ((int64_t) tent[n] * 2 + s) * fs;
+        for (int k = 0; k < N; k++) a[PK(k)] = 0.0;
+      }
+  }
+
+  // opacity terms of set member n at depth k for ray (mu, dir):
+  // V (absorption per population), gij, the emission coefficient factor
+  // 2hnu^3/c^2, and the quadrature weight wla (rh/opacity.c:62-300)
+  auto member = [&](int n, int k, int mu, int dir, double T, double &V, double &gij,
+                    double &tw, double &wla) {
+    const int t = tt[n];
+    const int la = ls - A.tr_nblue[t];
+    if (A.tr_type[t] == 0) {
+      const int l = A.tr_line[t];
+      const double ph =
+          B.phi[((((int64_t) (A.ln_phi[l] + la) * nrays + mu) * 2 + dir) * Nmax) * ncol + PK(k)];
+      gij = A.ln_Bji[l] / A.ln_Bij[l];
+      V = (HC_4PI * A.ln_Bij[l] * 1.0) * ph;
+      tw = A.ln_Aji[l] / A.ln_Bji[l];
+      wla = A.tr_wlam[A.tr_ptr[t] + la] * B.wphi[(int64_t) l * fs + PK(k)] / HC_4PI;
+    } else {
+      tw = twohc / lam3;
+      V = A.tr_alpha[A.tr_ptr[t] + la];
+      gij = B.nstar[(int64_t) ti[n] * fs + PK(k)] / B.nstar[(int64_t) tj[n] * fs + PK(k)] *
+            exp(-hc_k / (lam * T));
+      wla = FOURPI / HPLANCK * (A.tr_wlam[A.tr_ptr[t] + la] / lam);
+    }
+  };
+  auto pop = [&](int i, int k) { return B.pops[(int64_t) i * fs + PK(k)]; };
+
+  // opacity and source function of one ray at every depth
+  auto fill = [&](int mu, int dir) {
+    for (int k = 0; k < N; k++) {
+      const double T = STF(RHG_CF_T, k);
+      double chi = 0.0, eta = 0.0;
+      for (int n = 0; n < nset; n++) {
+        double V, gij, tw, wla;
+        member(n, k, mu, dir, T, V, gij, tw, wla);
+        const double ni = pop(ti[n], k), nj = pop(tj[n], k);
+        chi += V * (ni - gij * nj);
+        eta += tw * gij * V * nj;
+      }
+      const double chi_t = chi + chic[PK(k)];
+      double S = eta + etac[PK(k)];
+      S += scac[PK(k)] * SCR(NS_JDAG, k);
+      S /= chi_t;
+      SCR(NS_CHI, k) = chi_t;
+      SCR(NS_S, k) = S;
+      SCR(NS_ETA, k) = eta;
+    }
+  };
+
+  // rate terms of depth k for one ray: I, Psi (already divided by chi),
+  // quadrature weight of the ray wmu
+  auto gamma_point = [&](int k, double I, double Psi, int mu, int dir, double wmu) {
+    const double T = STF(RHG_CF_T, k);
+    double V[NLTE_MAXSET], gij[NLTE_MAXSET], tw[NLTE_MAXSET], wla[NLTE_MAXSET],
+        chicc[NLTE_MAXSET], U[NLTE_MAXSET];
+#pragma unroll
+    for (int n = 0; n < NLTE_MAXSET; n++) {
+      if (n < nset) {
+        member(n, k, mu, dir, T, V[n], gij[n], tw[n], wla[n]);
+        const double ni = pop(ti[n], k), nj = pop(tj[n], k);
+        chicc[n] = V[n] * wla[n] * (ni - gij[n] * nj);
+        U[n] = tw[n] * gij[n] * V[n];
+      } else {
+        V[n] = gij[n] = tw[n] = wla[n] = chicc[n] = U[n] = 0.0;
+      }
+    }
+    const double Ieff = I - Psi * SCR(NS_ETA, k);
+#pragma unroll
+    for (int n = 0; n < NLTE_MAXSET; n++) {
+      if (n < nset) {
+        // chi_up of the lower level, Uji of the upper level (over the
+        // members with nonzero 2hnu^3/c^2, rh/fillgamma.c:251-332)
+        double chi_up = 0.0, Uj = 0.0, chi_dn = 0.0, Ui = 0.0;
+        int ncross = 0;
+#pragma unroll
+        for (int m = 0; m < NLTE_MAXSET; m++) {
+          if (m < nset && tw[m] != 0.0) {
+            if (ti[m] == ti[n]) chi_up += chicc[m];
+            if (tj[m] == tj[n]) { Uj += U[m]; chi_dn += chicc[m]; }
+            if (tj[m] == ti[n]) Ui += U[m];
+          }
+          if (m < nset && tj[m] == ti[n]) ncross++;
+        }
+        double *aij = B.acc + ((int64_t) tent[n] * 2) * fs + PK(k);
+        double *aji = aij + fs;
+        const double wlamu = V[n] * wla[n] * wmu;
+        double gji = *aji, gij_ = *aij;
+        gji += Ieff * wlamu;
+        gij_ += (tw[n] + Ieff) * gij[n] * wlamu;
+        gij_ -= chi_up * Psi * Uj * wmu;
+        for (int c = 0; c < ncross; c++) gji += chi_dn * Psi * Ui * wmu;
+        *aji = gji;
+        *aij = gij_;
+      }
+    }
+  };
+
+  const double B2_0 = planck_lam(STF(RHG_CF_T, N - 2), lam);
+  const double B2_1 = planck_lam(STF(RHG_CF_T, N - 1), lam);
+#define CHI(k) SCR(NS_CHI, k)
+#define SRC(k) SCR(NS_S, k)
+
+  if (A.as_bezier[ls]) {
+    for (int mu = 0; mu < nrays; mu++) {
+      const double wmu = 0.5 * A.wmu[mu];
+      const double zmu = 1.0 / A.muz[mu];
+      for (int to_obs = 0; to_obs <= 1; to_obs++) {
+        fill(mu, to_obs);
+        auto emit = [&](int k, double I, double Psi) {
+          Jl[PK(k)] += wmu * I;
+          if (eval_op) gamma_point(k, I, Psi / CHI(k), mu, to_obs, wmu);
+        };
+        // |z[a] - z[b]| for neighbouring depths
+        auto dzb = [&](int a, int b) { return STF(RHG_CF_DZ, a > b ? a : b); };
+        const int dk = to_obs ? -1 : 1, ks = to_obs ? N - 1 : 0, ke = to_obs ? 0 : N - 1;
+        double dtau_uw = 0.5 * zmu * (CHI(ks) + CHI(ks + dk)) * dzb(ks, ks + dk);
+        double I_upw = to_obs ? B2_1 - (B2_0 - B2_1) / dtau_uw : 0.0;
+        emit(ks, I_upw, 0.0);
+        int k = ks + dk;
+        double dsup = dzb(k, k - dk) * zmu, dsdn = dzb(k + dk, k) * zmu;
+        double dchi_up = (CHI(k) - CHI(k - dk)) / dsup;
+        double dchi_c = cent_div(dsup, dsdn, CHI(k - dk), CHI(k), CHI(k + dk));
+        double c1 = max0(CHI(k) - (dsup / 3.0) * dchi_c);
+        double c2 = max0(CHI(k - dk) + (dsup / 3.0) * dchi_up);
+        dtau_uw = dsup * (CHI(k) + CHI(k - dk) + c1 + c2) * 0.25;
+        double dS_up = (SRC(k) - SRC(k - dk)) / dtau_uw;
+        double dchi_dn = 0.0, dtau_dw = 0.0, dS_c = 0.0;
+        for (k = ks + dk; k != ke + dk; k += dk) {
+          double I, Psi;
+          if (k != ke) {
+            dsdn = dzb(k + dk, k) * zmu;
+            if (abs(k - ke) > 1) {
+              const double dsdn2 = dzb(k + 2 * dk, k + dk) * zmu;
+              dchi_dn = cent_div(dsdn, dsdn2, CHI(k), CHI(k + dk), CHI(k + 2 * dk));
+            } else {
+              dchi_dn = (CHI(k + dk) - CHI(k)) / dsdn;
+            }
+            c1 = max0(CHI(k) + (dsdn / 3.0) * dchi_c);
+            c2 = max0(CHI(k + dk) - (dsdn / 3.0) * dchi_dn);
+            dtau_dw = dsdn * (CHI(k) + CHI(k + dk) + c1 + c2) * 0.25;
+            const double dt03 = dtau_uw / 3.0;
+            double alpha, beta, gamma, theta, eps;
+            bezier3_coeffs(dtau_uw, alpha, beta, gamma, theta, eps);
+            dS_c = cent_div(dtau_uw, dtau_dw, SRC(k - dk), SRC(k), SRC(k + dk));
+            c1 = max0(SRC(k) - dt03 * dS_c);
+            c2 = max0(SRC(k - dk) + dt03 * dS_up);
+            I = I_upw * eps + alpha * SRC(k) + beta * SRC(k - dk) + gamma * c1 + theta * c2;
+            Psi = alpha + gamma;
+          } else {
+            dtau_uw = 0.5 * zmu * (CHI(k) + CHI(k - dk)) * dzb(k, k - dk);
+            const double dS_uw = -(SRC(k) - SRC(k - dk)) / dtau_uw;
+            double w[3];
+            w3(dtau_uw, w);
+            I = (1.0 - w[0]) * I_upw + w[0] * SRC(k) + w[1] * dS_uw;
+            Psi = w[0] - w[1] / dtau_uw;
+          }
+          emit(k, I, Psi);
+          I_upw = I;
+          dsup = dsdn;
+          dchi_up = dchi_c;
+          dchi_c = dchi_dn;
+          dtau_uw = dtau_dw;
+          dS_up = dS_c;
+        }
+        (void) dchi_up;
+        (void) dsup;
+      }
+    }
+  } else {
+    // continua only: one opacity for all rays
+    fill(0, 0);
+#define HGT(k) STF(RHG_CF_HEIGHT, k)
+    for (int mu = 0; mu < nrays; mu++) {
+      const double wmu = A.wmu[mu];
+      const double zmu = 0.5 / A.muz[mu];
+      auto emit = [&](int k, double P, double Psi) {
+        Jl[PK(k)] += P * wmu;
+        if (eval_op) gamma_point(k, P, Psi / CHI(k), 0, 0, wmu);
+      };
+      auto dtau = [&](int k) { return zmu * (CHI(k) + CHI(k + 1)) * (HGT(k) - HGT(k + 1)); };
+      // forward elimination, rh/rhf1d/feautrier.c:56-202 (dark top,
+      // thermalised bottom, STANDARD order)
+      const double dt0 = dtau(0);
+      const double abc0 = 1.0 + 2.0 * 1.0 / dt0;
+      const double C1_0 = 2.0 / (dt0 * dt0);
+      const double S0 = SRC(0) + 2.0 * 0.0 / ((1.0 + 0.0) * dt0);
+      double F = abc0 / C1_0, z = S0 / (abc0 + C1_0);
+      SCR(NS_F, 0) = F;
+      SCR(NS_Z, 0) = z;
+      double dtm1 = dt0;
+      for (int k = 1; k < N - 1; k++) {
+        const double dtk = dtau(k);
+        const double dtau_mid = 0.5 * (dtk + dtm1);
+        const double A1 = 1.0 / (dtau_mid * dtm1), C1 = 1.0 / (dtau_mid * dtk);
+        F = (1.0 + A1 * F / (1.0 + F)) / C1;
+        z = (SRC(k) + A1 * z) / (C1 * (1.0 + F));
+        SCR(NS_F, k) = F;
+        SCR(NS_Z, k) = z;
+        dtm1 = dtk;
+      }
+      // back substitution with the two-sided operator
+      const double dtN = dtm1;   // dtau[N-2]
+      const double hN = B2_1 - (B2_0 - B2_1) / dtN;
+      const double abcN = 1.0 + 2.0 * 1.0 / dtN;
+      const double A1N = 2.0 / (dtN * dtN);
+      const double SN = SRC(N - 1) + 2.0 * hN / ((1.0 + 0.0) * dtN);
+      const double FN2 = SCR(NS_F, N - 2);
+      double P = (SN + A1N * SCR(NS_Z, N - 2)) / (abcN + A1N * (FN2 / (1.0 + FN2)));
+      emit(N - 1, P, 1.0 / (abcN + A1N * FN2 / (1.0 + FN2)));
+      double G = abcN / A1N;   // G[N-1]
+      double dtk = dtN;        // dtau[k] for k = N-2
+      for (int k = N - 2; k >= 1; k--) {
+        const double Fk = SCR(NS_F, k), Fm = SCR(NS_F, k - 1);
+        P = P / (1.0 + Fk) + SCR(NS_Z, k);
+        const double dtkm = dtau(k - 1);
+        const double dtau_mid = 0.5 * (dtk + dtkm);
+        const double A1 = 1.0 / (dtau_mid * dtkm), C1 = 1.0 / (dtau_mid * dtk);
+        emit(k, P, 1.0 / (1.0 + A1 * Fm / (1.0 + Fm) + C1 * G / (1.0 + G)));
+        G = (1.0 + C1 * G / (1.0 + G)) / A1;
+        dtk = dtkm;
+      }
+      P = P / (1.0 + SCR(NS_F, 0)) + SCR(NS_Z, 0);
+      emit(0, P, 1.0 / (abc0 + C1_0 * G / (1.0 + G)));
+    }
+#undef HGT
+  }
+
+  double dJmax = 0.0;
+  for (int k = 0; k < N; k++) {
+    const double dJ = fabs(1.0 - SCR(NS_JDAG, k) / Jl[PK(k)]);
+    dJmax = dJmax > dJ ? dJmax : dJ;
+  }
+  B.dJ[(int64_t) il * ncol + ic] = dJmax;
+#undef CHI
+#undef SRC
+#undef SCR
+#undef STF
+#undef PK
+}
+
+// max over the list wavelengths of the per-wavelength dJ, per column
+__global__ void k_nlte_djmax(NlteBufs B, const int32_t *__restrict__ act, int nact, int nlist,
+                             double *__restrict__ djmax) {
+  const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= nact) return;
+  const int ic = act[slot];
+  double d = 0.0;
+  for (int il = 0; il < nlist; il++) {
+    const double x = B.dJ[(int64_t) il * B.ncol + ic];
+    d = d > x ? d : x;
+  }
+  djmax[ic] = d;
+}
+
+// ---------------------------------------------------------------------------
+// Crout LU decomposition with implicit pivoting, back substitution and one
+// step of iterative improvement, rh/ludcmp.c:36-177, on an NLP x NLP system
+// held in registers (compile-time indices; the dynamic pivot row is
+// selected by comparison).  A system of n < NLP equations is padded with
+// identity rows and zero right-hand sides, which leaves every operation on
+// the first n rows and columns unchanged.
+template <int NLP>
+__device__ __forceinline__ void lu_decomp_r(double (&a)[NLP][NLP], int (&index)[NLP]) {
+  double vv[NLP];
+#pragma unroll
+  for (int i = 0; i < NLP; i++) {
+    double big = 0.0;
+#pragma unroll
+    for (int j = 0; j < NLP; j++) {
+      const double temp = fabs(a[i][j]);
+      if (temp > big) big = temp;
+    }
+    vv[i] = 1.0 / big;
+  }
+#pragma unroll
+  for (int j = 0; j < NLP; j++) {
+#pragma unroll
+    for (int i = 0; i < j; i++) {
+      double sum = a[i][j];
+#pragma unroll
+      for (int k = 0; k < i; k++) sum -= a[i][k] * a[k][j];
+      a[i][j] = sum;
+    }
+    double big = 0.0;
+    int imax = 0;
+#pragma unroll
+    for (int i = j; i < NLP; i++) {
+      double sum = a[i][j];
+#pragma unroll
+      for (int k = 0; k < j; k++) sum -= a[i][k] * a[k][j];
+      a[i][j] = sum;
+      const double dum = vv[i] * fabs(sum);
+      if (dum >= big) {
+        big = dum;
+        imax = i;
+      }
+    }
+#pragma unroll
+    for (int i = j + 1; i < NLP; i++) {
+      if (i == imax) {
+#pragma unroll
+        for (int k = 0; k < NLP; k++) {
+          const double d = a[i][k];
+          a[i][k] = a[j][k];
+          a[j][k] = d;
+        }
+        vv[i] = vv[j];
+      }
+    }
+    index[j] = imax;
+    if (a[j][j] == 0.0) a[j][j] = 1.0e-20;
+    const double dum = 1.0 / a[j][j];
+#pragma unroll
+    for (int i = j + 1; i < NLP; i++) a[i][j] *= dum;
+  }
+}
+
+template <int NLP>
+__device__ __forceinline__ void lu_backsubst_r(const double (&a)[NLP][NLP],
+                                               const int (&index)[NLP], double (&b)[NLP]) {
+  int ii = -1;
+#pragma unroll
+  for (int i = 0; i < NLP; i++) {
+    const int ip = index[i];
+    double sum = 0.0;
+#pragma unroll
+    for (int q = 0; q < NLP; q++)
+      if (q == ip) sum = b[q];
+#pragma unroll
+    for (int q = 0; q < NLP; q++)
+      if (q == ip) b[q] = b[i];
+    if (ii >= 0) {
+#pragma unroll
+      for (int j = 0; j < i; j++)
+        if (j >= ii) sum -= a[i][j] * b[j];
+    } else if (sum) {
+      ii = i;
+    }
+    b[i] = sum;
+  }
+#pragma unroll
+  for (int i = NLP - 1; i >= 0; i--) {
+    double sum = b[i];
+#pragma unroll
+    for (int j = i + 1; j < NLP; j++) sum -= a[i][j] * b[j];
+    b[i] = sum / a[i][i];
+  }
+}
+
+template <int NLP>
+__device__ __forceinline__ void solve_improve_r(double (&a)[NLP][NLP], double (&b)[NLP]) {
+  double ac[NLP][NLP], bc[NLP], res[NLP];
+  int index[NLP];
+#pragma unroll
+  for (int i = 0; i < NLP; i++) {
+    bc[i] = b[i];
+#pragma unroll
+    for (int j = 0; j < NLP; j++) ac[i][j] = a[i][j];
+  }
+  lu_decomp_r<NLP>(a, index);
+  lu_backsubst_r<NLP>(a, index, b);
+#pragma unroll
+  for (int i = 0; i < NLP; i++) {
+    res[i] = bc[i];
+#pragma unroll
+    for (int j = 0; j < NLP; j++) res[i] -= ac[i][j] * b[j];
+  }
+  lu_backsubst_r<NLP>(a, index, res);
+#pragma unroll
+  for (int i = 0; i < NLP; i++) b[i] += res[i];
+}
+
+// Rate matrix and statistical equilibrium at one (column, depth),
+// rh/statequil.c:36-216: Gamma = C + the rate terms of each transition
+// summed over its wavelengths in order; the diagonal is minus the column
+// sums; the equation of the most populated level (isum = -1, else level
+// isum) is replaced by particle conservation.
+template <int NLP>
+__global__ void __launch_bounds__(64)
+k_nlte_stateq(const DevNlte *__restrict__ Ap, NlteBufs B, const int32_t *__restrict__ act,
+              int nact, int isum) {
+  const DevNlte &A = *Ap;
+  const int64_t p = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  const int Nmax = B.Nmax, ncol = B.ncol, nl = A.nl;
+  if (p >= (int64_t) nact * Nmax) return;
+  const int slot = (int) (p % nact), k = (int) (p / nact);
+  const int ic = act[slot];
+  if (k >= B.ndep[ic]) return;
+  const int64_t fs = (int64_t) Nmax * ncol, pk = (int64_t) k * ncol + ic;
+  for (int q = 0; q < nl * nl; q++) B.Gam[(int64_t) q * fs + pk] = B.Cmat[(int64_t) q * fs + pk];
+  for (int t = 0; t < A.ntrans; t++) {
+    const int i = A.tr_i[t], j = A.tr_j[t], e0 = A.tr_ptr[t], ne = A.tr_nlam[t];
+    double *gij = B.Gam + (int64_t) (i * nl + j) * fs + pk;
+    double *gji = B.Gam + (int64_t) (j * nl + i) * fs + pk;
+    double sij = *gij, sji = *gji;
+    for (int e = e0; e < e0 + ne; e++) {
+      sij += B.acc[((int64_t) e * 2) * fs + pk];
+      sji += B.acc[((int64_t) e * 2 + 1) * fs + pk];
+    }
+    *gij = sij;
+    *gji = sji;
+  }
+  double a[NLP][NLP], nk[NLP];
+#pragma unroll
+  for (int i = 0; i < NLP; i++) {
+    nk[i] = (i < nl) ? B.pops[(int64_t) i * fs + pk] : 0.0;
+#pragma unroll
+    for (int j = 0; j < NLP; j++)
+      a[i][j] = (i < nl && j < nl) ? B.Gam[(int64_t) (i * nl + j) * fs + pk] : (i == j ? 1.0 : 0.0);
+  }
+  int ie = 0;
+  if (isum == -1) {
+    double nmax = 0.0;
+#pragma unroll
+    for (int i = 0; i < NLP; i++)
+      if (i < nl && nk[i] > nmax) {
+        nmax = nk[i];
+        ie = i;
+      }
+  } else {
+    ie = isum;
+  }
+#pragma unroll
+  for (int i = 0; i < NLP; i++) {
+    if (i < nl) {
+      a[i][i] = 0.0;
+      nk[i] = 0.0;
+      double GamDiag = 0.0;
+#pragma unroll
+      for (int j = 0; j < NLP; j++)
+        if (j < nl) GamDiag += a[j][i];
+      a[i][i] = -GamDiag;
+    }
+  }
+  const double ntotal = B.ntot[pk];
+#pragma unroll
+  for (int i = 0; i < NLP; i++) {
+    if (i == ie) {
+      nk[i] = ntotal;
+#pragma unroll
+      for (int j = 0; j < NLP; j++)
+        if (j < nl) a[i][j] = 1.0;
+    }
+  }
+  solve_improve_r<NLP>(a, nk);
+#pragma unroll
+  for (int i = 0; i < NLP; i++)
+    if (i < nl) B.pops[(int64_t) i * fs + pk] = nk[i];
+}
+
+template __global__ void k_nlte_stateq<4>(const DevNlte *, NlteBufs, const int32_t *, int, int);
+template __global__ void k_nlte_stateq<8>(const DevNlte *, NlteBufs, const int32_t *, int, int);
+template __global__ void k_nlte_stateq<16>(const DevNlte *, NlteBufs, const int32_t *, int, int);
+
+// ---------------------------------------------------------------------------
+// Ng acceleration of one column's populations and the maximum relative
+// change, rh/accelerate.c:34-146 and rh/maxchange.c:32-50, over the
+// populations flattened level by level (index i * N + k).  The history
+// holds norder + 2 solutions; ng_init stored the starting solution in slot
+// 0, so before iteration niter (1-based) the count is niter.
+__global__ void k_nlte_ng(const DevNlte *__restrict__ Ap, NlteBufs B,
+                          const int32_t *__restrict__ act, int nact, int niter, int norder,
+                          int ndelay, int nperiod, double *__restrict__ dpops) {
+  const DevNlte &A = *Ap;
+  const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= nact) return;
+  const int ic = act[slot], N = B.ndep[ic], nl = A.nl, Nmax = B.Nmax, ncol = B.ncol;
+  const int NS = norder + 2;
+  const int64_t fs = (int64_t) Nmax * ncol, hs = (int64_t) nl * fs;
+  auto H = [&](int s, int i, int k) -> double & {
+    return B.hist[(int64_t) s * hs + (int64_t) i * fs + (int64_t) k * ncol + ic];
+  };
+  auto POP = [&](int i, int k) -> double & { return B.pops[(int64_t) i * fs + (int64_t) k * ncol + ic]; };
+  const int cur = niter % NS;
+  for (int i = 0; i < nl; i++)
+    for (int k = 0; k < N; k++) H(cur, i, k) = POP(i, k);
+  const int count = niter + 1;
+  if (norder > 0 && count >= ndelay && !((count - ndelay) % nperiod)) {
+    double Am[NLTE_MAXNG][NLTE_MAXNG], bv[NLTE_MAXNG];
+#pragma unroll
+    for (int i = 0; i < NLTE_MAXNG; i++) {
+      bv[i] = 0.0;
+#pragma unroll
+      for (int j = 0; j < NLTE_MAXNG; j++) Am[i][j] = (i < norder || j < norder) ? 0.0 : (i == j ? 1.0 : 0.0);
+    }
+    for (int i = 0; i < nl; i++)
+      for (int k = 0; k < N; k++) {
+        double D[NLTE_MAXNG + 1];
+#pragma unroll
+        for (int q = 0; q <= NLTE_MAXNG; q++) {
+          D[q] = 0.0;
+          // count >= ndelay >= norder + 2, so both slots are >= 0
+          if (q <= norder) D[q] = H((count - 1 - q) % NS, i, k) - H((count - 2 - q) % NS, i, k);
+        }
+        const double w = 1.0 / fabs(POP(i, k));
+#pragma unroll
+        for (int j = 0; j < NLTE_MAXNG; j++) {
+          if (j < norder) {
+            bv[j] += w * D[0] * (D[0] - D[j + 1]);
+#pragma unroll
+            for (int ii = 0; ii < NLTE_MAXNG; ii++)
+              if (ii < norder) Am[ii][j] += w * (D[j + 1] - D[0]) * (D[ii + 1] - D[0]);
+          }
+        }
+      }
+    solve_improve_r<NLTE_MAXNG>(Am, bv);
+    const int i0 = (count - 1) % NS;
+    for (int i = 0; i < nl; i++)
+      for (int k = 0; k < N; k++) {
+        double s = POP(i, k);
+        const double h0 = H(i0, i, k);
+#pragma unroll
+        for (int q = 0; q < NLTE_MAXNG; q++)
+          if (q < norder) s += bv[q] * (H((count - 2 - q) % NS, i, k) - h0);
+        POP(i, k) = s;
+        H(i0, i, k) = s;
+      }
+  }
+  double dmax = 0.0;
+  {
+    const int so = (count - 2) % NS, sc = (count - 1) % NS;
+    for (int i = 0; i < nl; i++)
+      for (int k = 0; k < N; k++) {
+        const double c = H(sc, i, k);
+        if (c) {
+          const double d = fabs((c - H(so, i, k)) / c);
+          dmax = dmax > d ? dmax : d;
+        }
+      }
+  }
+  dpops[ic] = dmax;
+}
+
+// populations to the caller's layout [ncol][nl][Nmax], zero past ndep
+__global__ void k_nlte_out(const DevNlte *__restrict__ Ap, NlteBufs B, double *__restrict__ out) {
+  const DevNlte &A = *Ap;
+  const int64_t p = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  const int ncol = B.ncol, Nmax = B.Nmax, nl = A.nl;
+  if (p >= (int64_t) ncol * nl * Nmax) return;
+  const int k = (int) (p % Nmax);
+  const int64_t q = p / Nmax;
+  const int i = (int) (q % nl), ic = (int) (q / nl);
+  out[p] = k < B.ndep[ic] ? B.pops[((int64_t) i * Nmax + k) * ncol + ic] : 0.0;
+}
+
+}  // namespace rhg

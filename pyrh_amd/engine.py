@@ -26,6 +26,9 @@ RF_ROWS = (1, 3, 5, 6, 7, 4)        # atmosphere rows of the parameters
 STAGES = ("state", "geometry", "continuum", "formal")
 # the four intervals rhg_last_timings reports after rhg_rf_lte
 RF_STAGES = ("synthesis_inputs", "state_tangents", "tape", "reverse_and_chain")
+# rhg_nlte_timings slots (RHG_NLTE_NTIMING)
+NLTE_TIMINGS = ("setup_ms", "scatter_rt_ms", "iter_rt_ms", "stateq_ms", "ng_ms", "total_ms",
+                "rt_launches", "rt_pairs")
 
 
 class EngineError(RuntimeError):
@@ -69,6 +72,9 @@ def lib():
                                  C.c_void_p]
         L.rhg_rf_loggf.argtypes = [C.c_void_p, C.POINTER(RhgColumns), C.c_int32,
                                    C.POINTER(C.c_int32), C.c_int32, PD, PD]
+        L.rhg_nlte_solve.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(RhgColumns), C.c_int32,
+                                     PD, C.POINTER(C.c_int32), PD]
+        L.rhg_nlte_timings.argtypes = [C.c_void_p, PD]
         if L.rhg_model_sizeof() != C.sizeof(RhgModel) or \
                 L.rhg_columns_sizeof() != C.sizeof(RhgColumns):
             raise EngineError("rhg_model_t / rhg_columns_t layout mismatch between the "
@@ -176,6 +182,45 @@ class Engine:
         _check(lib().rhg_rf_lte(self._h, C.byref(s), int(precision), 1,
                                 C.cast(C.c_void_p(d_out), C.POINTER(C.c_double)),
                                 C.c_void_p(d_rf)))
+
+    def nlte_solve(self, nm, atmos, mu=1.0, atm_scale=0, ndep=None):
+        """NLTE populations of the active atom of nm (pyrh_amd.nlte.build_nlte;
+        the engine must hold nm.lte) for host atmospheres [ncol, 9,
+        ndep_max]: populations [ncol, nlevel, ndep_max], iterations [ncol]
+        and the last maximum relative change [ncol]."""
+        from .abi import NlteStruct
+        ns = NlteStruct(nm)
+        cs = ColumnsStruct(atmos, ndep, atm_scale, mu)
+        ncol, nd = cs.struct.ncol, cs.struct.ndep_max
+        pops = np.zeros((ncol, nm.nlevel, nd))
+        niter = np.zeros(ncol, np.int32)
+        dpops = np.zeros(ncol)
+        _check(lib().rhg_nlte_solve(self._h, C.addressof(ns.struct), cs.ptr(), 0, _pd(pops),
+                                    niter.ctypes.data_as(C.POINTER(C.c_int32)), _pd(dpops)))
+        return pops, niter, dpops
+
+    def nlte_solve_device(self, nm, d_atmos, d_ndep, ncol, ndep_max, d_pops, d_niter, d_dpops,
+                          mu=1.0, atm_scale=0):
+        """nlte_solve with device pointers (ints) for the atmospheres, depth
+        counts and the three outputs."""
+        from .abi import NlteStruct
+        ns = NlteStruct(nm)
+        s = RhgColumns()
+        s.ncol, s.ndep_max = int(ncol), int(ndep_max)
+        s.ndep = C.cast(C.c_void_p(d_ndep), C.POINTER(C.c_int32))
+        s.atmos = C.cast(C.c_void_p(d_atmos), C.POINTER(C.c_double))
+        s.atm_scale, s.mu = int(atm_scale), float(mu)
+        _check(lib().rhg_nlte_solve(self._h, C.addressof(ns.struct), C.byref(s), 1,
+                                    C.cast(C.c_void_p(d_pops), C.POINTER(C.c_double)),
+                                    C.cast(C.c_void_p(d_niter), C.POINTER(C.c_int32)),
+                                    C.cast(C.c_void_p(d_dpops), C.POINTER(C.c_double))))
+
+    def nlte_timings(self):
+        """Milliseconds and work counts of the last NLTE solve (keys of
+        NLTE_TIMINGS)."""
+        t = np.zeros(len(NLTE_TIMINGS))
+        _check(lib().rhg_nlte_timings(self._h, _pd(t)))
+        return dict(zip(NLTE_TIMINGS, t))
 
     def rf_timings(self):
         """Per-interval kernel milliseconds of the last rf_lte call."""

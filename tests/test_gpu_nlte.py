@@ -1,0 +1,95 @@
+"""NLTE populations on the GPU (rhg_nlte_solve) against the reference's own
+NLTE runs (golden vectors from rhf1d() compiled from the reference sources,
+tests/golden/make_golden_nlte.py) and against the oracle restatement on
+further seeded columns, ragged depth counts included.
+
+Tolerances: populations within 1e-6 relative of the reference and of the
+oracle at the same convergence criterion (ITER_LIMIT 1e-3); iteration counts
+equal; the last maximum relative change, a difference of nearly equal
+populations, within 1e-5 relative."""
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+RUN_NLTE = GOLDEN / "run_nlte"
+CASES = ("map6", "single", "hot")
+TOL_POPS = 1e-6
+TOL_DPOPS = 1e-5
+
+
+@pytest.fixture(scope="module")
+def nlte_fix():
+    return np.load(GOLDEN / "nlte_fixtures.npz")
+
+
+@pytest.fixture(scope="module")
+def nm(nlte_fix):
+    from pyrh_amd import model, nlte
+    return nlte.build_nlte(model.open_run(str(RUN_NLTE)), nlte_fix["wave"])
+
+
+@pytest.fixture(scope="module")
+def eng(nm):
+    from pyrh_amd.engine import Engine
+    e = Engine(nm.lte)
+    yield e
+    e.close()
+
+
+def _relerr(a, b):
+    return float(np.max(np.abs(a - b) / np.abs(b)))
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_nlte_matches_reference(eng, nm, nlte_fix, name):
+    atm = nlte_fix[name + "__atmos"]
+    pops, niter, dpops = eng.nlte_solve(nm, atm)
+    ref = nlte_fix[name + "__pops"]
+    err = _relerr(pops, ref)
+    print(f"{name}: niter {niter.tolist()}  max rel err {err:.2e}", flush=True)
+    np.testing.assert_array_equal(niter, nlte_fix[name + "__niter"])
+    assert err < TOL_POPS
+    np.testing.assert_allclose(dpops, nlte_fix[name + "__dpops"], rtol=TOL_DPOPS, atol=0)
+    # particle conservation: the levels add up to the atom's total
+    ntot = pops.sum(axis=1)
+    np.testing.assert_allclose(ntot, ref.sum(axis=1), rtol=1e-12)
+
+
+def test_nlte_matches_oracle_on_map_columns(eng, nm):
+    """24 seeded map columns with ragged depth counts, against the oracle."""
+    import pyoracle
+    from pyrh_amd import workloads
+    atm = workloads.stokes_map(24, seed=workloads.SEED_BASE + 11)
+    nd = atm.shape[2]
+    ndep = np.full(24, nd, np.int32)
+    ndep[5], ndep[11], ndep[17] = nd - 1, nd - 7, nd // 2
+    for c in range(24):
+        atm[c, :, ndep[c]:] = 0.0
+    pops, niter, dpops = eng.nlte_solve(nm, atm, ndep=ndep)
+    cp, cn, cd = pyoracle.nlte_solve(nm, atm, ndep=ndep)
+    for c in range(24):
+        n = ndep[c]
+        assert np.all(pops[c, :, n:] == 0.0)
+        e = _relerr(pops[c, :, :n], cp[c, :, :n])
+        assert e < TOL_POPS, (c, n, e)
+    print(f"oracle: niter {niter.tolist()}  max rel err "
+          f"{max(_relerr(pops[c, :, :ndep[c]], cp[c, :, :ndep[c]]) for c in range(24)):.2e}",
+          flush=True)
+    np.testing.assert_array_equal(niter, cn)
+    np.testing.assert_allclose(dpops, cd, rtol=TOL_DPOPS, atol=0)
+
+
+def test_nlte_batch_order_independent(eng, nm, nlte_fix):
+    """A column's result does not depend on its batch neighbours or on when
+    they converge and retire."""
+    atm = nlte_fix["map6__atmos"]
+    p1, n1, d1 = eng.nlte_solve(nm, atm)
+    perm = np.array([3, 0, 5, 1, 4, 2])
+    p2, n2, d2 = eng.nlte_solve(nm, atm[perm])
+    np.testing.assert_array_equal(p2, p1[perm])
+    np.testing.assert_array_equal(n2, n1[perm])
+    p3, n3, _ = eng.nlte_solve(nm, atm[2:3])
+    np.testing.assert_array_equal(p3[0], p1[2])
