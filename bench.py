@@ -13,9 +13,13 @@ time on the library's stream), and a CPU baseline timed on this host on a
 bounded sample of the same columns.  The response-function leg (object
 response_functions) times the adjoint of a 1024-column batch in the FP32
 form and in FP64 against central differences of the plain FP64 synthesis
-(2 x 6 x 82 perturbed syntheses per column) and reports the errors.
+(2 x 6 x 82 perturbed syntheses per column) and reports the errors.  The
+NLTE leg (object nlte) solves the populations of a Ca II-like atom in 4096
+map columns to convergence and reports columns/s, the iteration histogram
+and the roofline of its formal-solution kernel.
 
     python bench.py [--steps K] [--warmup W] [--ncol N] [--no-cpu] [--no-rf]
+                    [--no-nlte] [--nlte-ncol N] [--nlte-only]
 """
 from __future__ import annotations
 
@@ -41,6 +45,16 @@ RF_WORKLOAD = "lte_stokes_rf_1024"
 # central-difference steps of the finite-difference response functions, in
 # the atmosphere's units (K, km/s, G, rad, rad, km/s)
 RF_STEP = {"T": 1.0, "vlos": 0.01, "B": 1.0, "gamma": 0.01, "chi": 0.01, "vmic": 0.01}
+# NLTE leg: the Ca II-like five-level atom with continuum, ACTIVE, in the
+# run directory tests/golden/run_nlte (5 Gauss-Legendre rays, Ng 2/10/3,
+# ITER_LIMIT 1e-3), on 4096 map columns
+NLTE_RUN = ROOT / "tests" / "golden" / "run_nlte"
+NLTE_WORKLOAD = "nlte_caii_4096"
+NLTE_WAVE = (393.4776, 396.9591, 854.44)
+# algorithmic FP64 operations per (column, wavelength, ray direction, depth)
+# of an iteration's formal solution with the operator: opacity of the active
+# set, the Bezier step, J and the rate terms (about 150 per point)
+NLTE_FLOP_PER_POINT = 150.0
 
 
 def _traffic_from_profile(ncol):
@@ -189,6 +203,98 @@ def _rf_leg(eng, W, torch, dev, reps):
     return res
 
 
+def _nlte_leg(torch, dev, ncol, do_cpu, nworkers):
+    """NLTE populations of the 4096-column batch: one rhg_nlte_solve of the
+    whole batch with the atmospheres resident in HBM (set-up, scattering
+    passes, iterations until every column has converged), columns/s, the
+    iteration histogram, the time split and the roofline of the formal
+    solution kernel k_nlte_rt; the populations of 16 columns against the
+    oracle; the reference's rhf1d() on a bounded sample of the same columns
+    on this host's cores."""
+    from pyrh_amd import model as M, nlte as NL, roofline as R, workloads as W
+    from pyrh_amd.engine import Engine
+    sys.path.insert(0, str(ROOT / "oracle"))
+    wave = np.array(NLTE_WAVE)
+    nm = NL.build_nlte(M.open_run(str(NLTE_RUN)), wave)
+    atm = W.stokes_map(ncol, seed=W.SEED_BASE + 5)
+    nd = atm.shape[2]
+    eng = Engine(nm.lte)
+    d_atm = torch.from_numpy(atm).to(dev)
+    d_ndep = torch.full((ncol,), nd, dtype=torch.int32, device=dev)
+    d_pops = torch.empty((ncol, nm.nlevel, nd), dtype=torch.float64, device=dev)
+    d_niter = torch.empty(ncol, dtype=torch.int32, device=dev)
+    d_dp = torch.empty(ncol, dtype=torch.float64, device=dev)
+
+    def solve(n):
+        eng.nlte_solve_device(nm, d_atm.data_ptr(), d_ndep.data_ptr(), n, nd, d_pops.data_ptr(),
+                              d_niter.data_ptr(), d_dp.data_ptr())
+
+    solve(min(ncol, 64))            # warm-up (module load, allocations)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solve(ncol)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    tm = eng.nlte_timings()
+    niter = d_niter.cpu().numpy()
+    pops = d_pops[:16].cpu().numpy()
+    import pyoracle
+    cp, cn, _ = pyoracle.nlte_solve(nm, atm[:16], nthreads=nworkers)
+    err = float(np.max(np.abs(pops - cp) / cp))
+    # the dominant kernel, k_nlte_bezier (wavelengths with a line): its
+    # points are (column, wavelength) pairs x 2 directions x rays x depths
+    nset = np.diff(nm.as_ptr)
+    nbez = int(np.sum((nset > 0) & (nm.as_bezier != 0)))
+    pairs_bez = tm["rt_pairs"] * nbez / int(np.sum(nset > 0))
+    pts = pairs_bez * 2 * nm.muz.size * nd
+    rt_ms = tm["scatter_bezier_ms"] + tm["iter_bezier_ms"]
+    achieved = NLTE_FLOP_PER_POINT * pts / (rt_ms * 1e-3) / 1e12
+    hist = {int(k): int(v) for k, v in zip(*np.unique(niter, return_counts=True))}
+    res = {
+        "workload": NLTE_WORKLOAD if ncol == 4096 else f"nlte_caii_{ncol}",
+        "value": round(ncol / dt, 2), "unit": "columns/s", "wall_s": round(dt, 3),
+        "dtype": "f64", "ncol": ncol, "ndep": nd, "nlevel": int(nm.nlevel),
+        "ntrans": int(nm.tr_type.size), "nlambda": int(nm.lam.size), "nrays": int(nm.muz.size),
+        "iter_limit": float(nm.iter_limit), "iterations_histogram": hist,
+        "time_ms": {k: round(float(v), 3) for k, v in tm.items() if k.endswith("_ms")},
+        "formal_passes": int(tm["rt_passes"]),
+        "roofline": {"bound": "valu", "kernel": "k_nlte_bezier", "achieved": round(achieved, 3),
+                     "peak": R.PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / R.PEAK_FP64_TFLOPS, 4), "traffic": None,
+                     "flop_per_point": NLTE_FLOP_PER_POINT,
+                     "avg_launch_ms": round(rt_ms / max(tm["rt_passes"], 1), 3)},
+        "oracle_check": {"columns": 16, "max_rel_err_pops": float(f"{err:.3g}"),
+                         "iterations_equal": bool(np.array_equal(niter[:16], cn))},
+        "cpu_baseline": None,
+    }
+    if do_cpu:
+        import refrun
+        ns = min(ncol, 2 * nworkers)
+        if refrun.ref_available("o1"):
+            from concurrent.futures import ThreadPoolExecutor
+            chunks = np.array_split(atm[:ns], nworkers)
+            t0 = time.perf_counter()
+            with ThreadPoolExecutor(nworkers) as ex:
+                list(ex.map(lambda x: refrun.run_synth(NLTE_RUN, x, wave, variant="o1"), chunks))
+            dtc = time.perf_counter() - t0
+            res["cpu_baseline"] = {
+                "value": round(ns / dtc, 3), "unit": "columns/s", "cores": nworkers,
+                "kind": "reference",
+                "sample": f"{ns} columns of the batch, rhf1d() per column to convergence plus its "
+                          f"emergent spectrum, {nworkers} processes, {dtc:.1f} s wall"}
+        else:
+            t0 = time.perf_counter()
+            pyoracle.nlte_solve(nm, atm[:ns], nthreads=nworkers)
+            dtc = time.perf_counter() - t0
+            res["cpu_baseline"] = {
+                "value": round(ns / dtc, 3), "unit": "columns/s", "cores": nworkers, "kind": "port",
+                "sample": f"{ns} columns, C restatement, OpenMP {nworkers} threads, {dtc:.1f} s wall"}
+    del d_atm, d_pops
+    eng.close()
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -197,6 +303,10 @@ def main():
     ap.add_argument("--ncol", type=int, default=16384)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-rf", action="store_true", help="skip the response-function leg")
+    ap.add_argument("--no-nlte", action="store_true", help="skip the NLTE leg")
+    ap.add_argument("--nlte-ncol", type=int, default=4096)
+    ap.add_argument("--nlte-only", action="store_true",
+                    help="run only the NLTE leg and print its object (for profiling)")
     ap.add_argument("--profile-step", action="store_true",
                     help="one FP32-form synthesis and one 1 GiB device copy, for rocprofv3 --pmc")
     a = ap.parse_args()
@@ -205,6 +315,13 @@ def main():
               f"independent and a multi-GPU run would shard them, which is not built.",
               file=sys.stderr)
         sys.exit(2)
+
+    nw = max(1, min(16, len(os.sched_getaffinity(0))))
+    if a.nlte_only:
+        import torch
+        res = _nlte_leg(torch, torch.device("cuda:0"), a.nlte_ncol, not a.no_cpu, nw)
+        print(json.dumps(res), flush=True)
+        return
 
     from pyrh_amd import model as M, roofline as R, workloads as W
     setup = M.open_run(str(RUN_DIR))
@@ -261,6 +378,7 @@ def main():
     ach64 = fl["formal_kernel"] / (stages64["formal"] * 1e-3) / 1e12
 
     rf = None if a.no_rf else _rf_leg(eng, W, torch, dev, reps=3)
+    nlte = None if a.no_nlte else _nlte_leg(torch, dev, a.nlte_ncol, not a.no_cpu, nw)
 
     # host-buffer rate (PCIe both ways), reported beside the resident rate
     t1 = time.perf_counter()
@@ -269,7 +387,6 @@ def main():
 
     cpu = None
     if not a.no_cpu:
-        nw = max(1, min(16, len(os.sched_getaffinity(0))))
         cpu = _cpu_baseline(setup, wave, atm, nw)
 
     line = {
@@ -305,6 +422,7 @@ def main():
                                     "unit": "TFLOP/s",
                                     "frac": round(ach64 / R.PEAK_FP64_TFLOPS, 4)}},
         "response_functions": rf,
+        "nlte": nlte,
         "pipeline_flops": fl["total"],
         "host_buffers_columns_per_s": round(host_rate, 1),
     }

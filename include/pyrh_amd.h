@@ -329,10 +329,12 @@ int rhg_nlte_solve(rhg_context_t *ctx, const rhg_nlte_t *atom, const rhg_columns
 
 /* Timings of the last rhg_nlte_solve in ms and its work counts:
  * [0] set-up (equilibria, depth scale, background, profiles, collisions),
- * [1] formal solutions of the scattering passes, [2] formal solutions of the
- * iterations, [3] statistical equilibrium, [4] Ng acceleration, [5] total,
- * [6] formal-solution launches, [7] (column, wavelength) pairs solved. */
-#define RHG_NLTE_NTIMING 8
+ * [1] formal solutions at the wavelengths with a line (Bezier rays) in the
+ * scattering passes, [2] the same in the iterations, [3] statistical
+ * equilibrium, [4] Ng acceleration, [5] total, [6] formal-solution passes,
+ * [7] (column, wavelength) pairs solved, [8] and [9] the continuum-only
+ * wavelengths (Feautrier) in the scattering passes and the iterations. */
+#define RHG_NLTE_NTIMING 10
 int rhg_nlte_timings(rhg_context_t *ctx, double *out);
 
 /* Same as rhg_synth_lte but also returns, per column, the depth scale and

@@ -66,7 +66,7 @@ struct rhg_context {
   // NLTE: the active atom (packed arrays and its DevNlte), the batch arrays
   // of rhg::NlteBufs, active-column list, per-column reductions, output
   DevBuf nl_atom, nl_dev, nl_nstar, nl_pops, nl_ntot, nl_vbroad, nl_nHe, nl_adamp, nl_wphi,
-      nl_Cmat, nl_Gam, nl_phi, nl_csca, nl_J, nl_acc, nl_scr, nl_dJ, nl_hist, nl_act, nl_red,
+      nl_Cmat, nl_Gam, nl_phi, nl_csca, nl_J, nl_jpar, nl_acc, nl_scr, nl_dJ, nl_hist, nl_act, nl_red,
       nl_out, nl_iolist, nl_atm, nl_ndep;
   double nlte_ms[RHG_NLTE_NTIMING]{};
   // host copies of model tables the NLTE checks need
@@ -805,8 +805,10 @@ namespace {
 // Host-side checks of an active atom against the model of the context and
 // what the NLTE kernels implement; fills the derived tables.
 struct NlteHost {
+  // output-wavelength positions with an active set: those with a line
+  // (Bezier rays) first, then the continuum-only ones (Feautrier)
   std::vector<int32_t> ln_tr, ln_phi, io_list;
-  int nentry = 0, nphi = 0;
+  int nbez = 0, nentry = 0, nphi = 0, maxset_bez = 0;
 };
 
 int nlte_check(const rhg_context_t *c, const rhg_nlte_t *a, NlteHost &h) {
@@ -858,8 +860,15 @@ int nlte_check(const rhg_context_t *c, const rhg_nlte_t *a, NlteHost &h) {
     }
     if (n > 0 && ls == c->dm.ref_index)
       return fail(RHG_EUNSUPPORTED, "active transition at the reference wavelength");
-    if (n > 0) h.io_list.push_back(ls < c->dm.ref_index ? ls : ls - 1);
+    if (n > 0 && a->as_bezier[ls]) {
+      h.io_list.push_back(ls < c->dm.ref_index ? ls : ls - 1);
+      h.maxset_bez = std::max(h.maxset_bez, n);
+    }
   }
+  h.nbez = (int) h.io_list.size();
+  for (int ls = 0; ls < ns; ls++)
+    if (a->as_ptr[ls + 1] > a->as_ptr[ls] && !a->as_bezier[ls])
+      h.io_list.push_back(ls < c->dm.ref_index ? ls : ls - 1);
   for (int t = 0; t < a->ntrans; t++)
     for (int ls = a->tr_nblue[t]; ls < a->tr_nblue[t] + a->tr_nlam[t]; ls++) {
       bool found = false;
@@ -970,7 +979,7 @@ int rhg_nlte_solve(rhg_context_t *c, const rhg_nlte_t *a, const rhg_columns_t *c
       (size_t) Nmax * sizeof(double) *
           ((size_t) nfield + 5 * (size_t) nout + 2 * (size_t) nl + 3 + 2 * (size_t) a->nline +
            2 * (size_t) nl * nl + (size_t) h.nphi * nrays * 2 + 2 * (size_t) h.nentry +
-           (size_t) rhg::NLTE_NSLOT * nlist + (size_t) NS * nl) +
+           (size_t) rhg::NLTE_NSLOT * (nlist - h.nbez) + (size_t) NS * nl + (size_t) nout) +
       (size_t) nlist * sizeof(double) + 9 * (size_t) Nmax * sizeof(double) + 64;
   size_t freeb = 0, totb = 0;
   HIPCHECK(hipMemGetInfo(&freeb, &totb));
@@ -1013,8 +1022,10 @@ int rhg_nlte_solve(rhg_context_t *c, const rhg_nlte_t *a, const rhg_columns_t *c
         c->nl_wphi.ensure(pts * std::max(a->nline, 1)) || c->nl_Cmat.ensure(pts * nl * nl) ||
         c->nl_Gam.ensure(pts * nl * nl) ||
         c->nl_phi.ensure(pts * std::max(h.nphi, 1) * nrays * 2) || c->nl_csca.ensure(pts * nout) ||
-        c->nl_J.ensure(pts * nout) || c->nl_acc.ensure(pts * h.nentry * 2) ||
-        c->nl_scr.ensure((size_t) rhg::NLTE_NSLOT * Nmax * nlist * ncb_max * 64 * sizeof(double)) ||
+        c->nl_J.ensure(pts * nout * 2) || c->nl_acc.ensure(pts * h.nentry * 2) ||
+        c->nl_scr.ensure((size_t) rhg::NLTE_NSLOT * Nmax * std::max(nlist - h.nbez, 1) * ncb_max *
+                         64 * sizeof(double)) ||
+        c->nl_jpar.ensure(ncol * sizeof(int32_t)) ||
         c->nl_dJ.ensure((size_t) nlist * ncol * sizeof(double)) || c->nl_hist.ensure(pts * NS * nl) ||
         c->nl_act.ensure(ncol * sizeof(int32_t)) || c->nl_red.ensure(ncol * sizeof(double)) ||
         (!device_ptrs && (c->nl_atm.ensure(pts * 9) || c->nl_ndep.ensure(ncol * sizeof(int32_t)) ||
@@ -1050,6 +1061,7 @@ int rhg_nlte_solve(rhg_context_t *c, const rhg_nlte_t *a, const rhg_columns_t *c
     B.ceta = (double *) c->ceta.p;
     B.csca = (double *) c->nl_csca.p;
     B.J = (double *) c->nl_J.p;
+    B.jpar = (int32_t *) c->nl_jpar.p;
     B.acc = (double *) c->nl_acc.p;
     B.scr = (double *) c->nl_scr.p;
     B.dJ = (double *) c->nl_dJ.p;
@@ -1085,6 +1097,7 @@ int rhg_nlte_solve(rhg_context_t *c, const rhg_nlte_t *a, const rhg_columns_t *c
         HIPCHECK(hipGetLastError());
       }
       HIPCHECK(hipMemsetAsync(B.J, 0, pts * nout, c->stream));
+      HIPCHECK(hipMemsetAsync(B.jpar, 0, ncol * sizeof(int32_t), c->stream));
       return 0;
     });
     if (rc) return rc;
@@ -1099,15 +1112,37 @@ int rhg_nlte_solve(rhg_context_t *c, const rhg_nlte_t *a, const rhg_columns_t *c
     };
     auto rt = [&](int slot, int eval) -> int {
       const int nact = (int) act.size();
-      return timed(slot, [&]() -> int {
-        dim3 grd((unsigned) rhg::xcd_grid((nact + 63) / 64, nlist));
-        hipLaunchKernelGGL(rhg::k_nlte_rt, grd, b64, 0, c->stream, c->d_model, d_at, B, d_act, nact,
-                           c->d_out_ls, d_iolist, nlist, eval);
-        HIPCHECK(hipGetLastError());
+      const int ncb = (nact + 63) / 64, nfe = nlist - h.nbez;
+      int r = timed(slot, [&]() -> int {
+        if (h.nbez > 0) {
+          dim3 grd((unsigned) rhg::xcd_grid(ncb, h.nbez));
+#define RHG_LAUNCH_BEZ(MS)                                                                       \
+  hipLaunchKernelGGL((rhg::k_nlte_bezier<rhg::NLTE_RB, MS>), grd, b64, 0, c->stream, c->d_model, \
+                     d_at, B, d_act, nact, c->d_out_ls, d_iolist, h.nbez, 0, eval)
+          if (h.maxset_bez <= 2) RHG_LAUNCH_BEZ(2);
+          else if (h.maxset_bez <= 4) RHG_LAUNCH_BEZ(4);
+          else RHG_LAUNCH_BEZ(8);
+#undef RHG_LAUNCH_BEZ
+          HIPCHECK(hipGetLastError());
+        }
         c->nlte_ms[6] += 1.0;
         c->nlte_ms[7] += (double) nact * nlist;
         return 0;
       });
+      if (r) return r;
+      r = timed(8 + (slot == 2), [&]() -> int {
+        if (nfe > 0) {
+          dim3 grd((unsigned) rhg::xcd_grid(ncb, nfe));
+          hipLaunchKernelGGL(rhg::k_nlte_feautrier, grd, b64, 0, c->stream, c->d_model, d_at, B,
+                             d_act, nact, c->d_out_ls, d_iolist + h.nbez, nfe, h.nbez, eval);
+          HIPCHECK(hipGetLastError());
+        }
+        hipLaunchKernelGGL(rhg::k_nlte_flip, dim3((unsigned) ((nact + 63) / 64)), b64, 0, c->stream,
+                           B, d_act, nact);
+        HIPCHECK(hipGetLastError());
+        return 0;
+      });
+      return r;
     };
     // per-column reduction of the active columns into red[]
     auto fetch_red = [&]() -> int {

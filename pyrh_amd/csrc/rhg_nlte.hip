@@ -11,10 +11,11 @@
 //   k_nlte_profile  (column, depth, line): absorption profiles for every
 //                   ray and direction, and their normalisation.
 // Iteration (per pass, over the columns still iterating)
-//   k_nlte_rt       (column, wavelength): the rays of the wavelength with the
-//                   diagonal operator; new mean intensity and, with the
-//                   operator, the radiative rate terms of every transition
-//                   active there.
+//   k_nlte_bezier   (column, wavelength with a line): the rays of the
+//   k_nlte_feautrier (column, continuum-only wavelength)
+//                   wavelength with the diagonal operator; new mean
+//                   intensity and, with the operator, the radiative rate
+//                   terms of every transition active there.
 //   k_nlte_stateq   (column, depth): rate matrix = collisions + the rate
 //                   terms summed over the transitions' wavelengths; the
 //                   statistical-equilibrium solve.
@@ -253,30 +254,146 @@ __global__ void k_nlte_profile(const DevModel *__restrict__ Mp, const DevNlte *_
 }
 
 // ---------------------------------------------------------------------------
-// One wavelength of one column: the formal solution of every ray with the
-// transitions active there (rh/rhf1d/formal.c:44-346), the new mean
-// intensity, and with eval_op the radiative rate terms of the transitions
-// (addtoCoupling and addtoGamma, rh/fillgamma.c:82-332).
+// Radiative rate terms at one depth for the rays of one direction
+// (addtoCoupling and addtoGamma, rh/fillgamma.c:82-332), summed over the
+// rays.  The members of the active set are described per wavelength by
+// SetInfo; per depth the kernel passes the members' populations, gij and
+// quadrature weights, and per ray the absorption V of each member.
+template <int MS>
+struct SetInfo {
+  int n;                                  // members
+  int ti[MS], tj[MS], ent[MS], line[MS], prow[MS];
+  double tw[MS], gline[MS], cV[MS], alpha[MS], wlam[MS];
+  // per member: members with the same lower level / upper level / whose
+  // upper level is this member's lower level (with 2hnu^3/c^2 != 0 for the
+  // sums, rh/fillgamma.c:251-332), and the count of the last without that
+  // condition (rh/fillgamma.c:204-214)
+  uint32_t m_up[MS], m_jj[MS], m_ji[MS];
+  int ncross[MS];
+};
+
+// per-depth member data
+template <int MS>
+struct SetDepth {
+  double ni[MS], nj[MS], gij[MS], wla[MS];
+};
+
+// accumulate the rate terms of ray r (intensity I, operator Psi / chi in
+// PsiC, direction weight wmu, absorption V of each member) into sIJ, sJI
+template <int MS>
+__device__ __forceinline__ void rate_terms(const SetInfo<MS> &si, const SetDepth<MS> &sd,
+                                           const double (&V)[MS], double I, double PsiC,
+                                           double wmu, double (&sIJ)[MS], double (&sJI)[MS]) {
+  double eta = 0.0;
+#pragma unroll
+  for (int n = 0; n < MS; n++)
+    if (n < si.n) eta += si.tw[n] * sd.gij[n] * V[n] * sd.nj[n];
+  const double Ieff = I - PsiC * eta;
+  double chicc[MS], U[MS];
+#pragma unroll
+  for (int n = 0; n < MS; n++) {
+    chicc[n] = V[n] * sd.wla[n] * (sd.ni[n] - sd.gij[n] * sd.nj[n]);
+    U[n] = si.tw[n] * sd.gij[n] * V[n];
+  }
+#pragma unroll
+  for (int n = 0; n < MS; n++) {
+    if (n < si.n) {
+      double chi_up = 0.0, Uj = 0.0, chi_dn = 0.0, Ui = 0.0;
+#pragma unroll
+      for (int m = 0; m < MS; m++) {
+        if ((si.m_up[n] >> m) & 1u) chi_up += chicc[m];
+        if ((si.m_jj[n] >> m) & 1u) { Uj += U[m]; chi_dn += chicc[m]; }
+        if ((si.m_ji[n] >> m) & 1u) Ui += U[m];
+      }
+      const double wlamu = V[n] * sd.wla[n] * wmu;
+      double gji = sJI[n], gij = sIJ[n];
+      gji += Ieff * wlamu;
+      gij += (si.tw[n] + Ieff) * sd.gij[n] * wlamu;
+      gij -= chi_up * PsiC * Uj * wmu;
+      for (int c = 0; c < si.ncross[n]; c++) gji += chi_dn * PsiC * Ui * wmu;
+      sJI[n] = gji;
+      sIJ[n] = gij;
+    }
+  }
+}
+
+// The active set of wavelength ls: members, their constants and coupling
+// masks.
+template <int MS>
+__device__ __forceinline__ void set_info(const DevNlte &A, int ls, double lam, SetInfo<MS> &si) {
+  const int p0 = A.as_ptr[ls];
+  si.n = A.as_ptr[ls + 1] - p0;
+  const double twohc = 2.0 * HC / (NM_TO_M * NM_TO_M * NM_TO_M);
+#pragma unroll
+  for (int n = 0; n < MS; n++) {
+    si.ti[n] = si.tj[n] = si.ent[n] = si.line[n] = si.prow[n] = 0;
+    si.tw[n] = si.gline[n] = si.cV[n] = si.alpha[n] = si.wlam[n] = 0.0;
+    if (n < si.n) {
+      const int t = A.as_tr[p0 + n], la = ls - A.tr_nblue[t];
+      si.ti[n] = A.tr_i[t];
+      si.tj[n] = A.tr_j[t];
+      si.ent[n] = A.tr_ptr[t] + la;
+      si.wlam[n] = A.tr_wlam[A.tr_ptr[t] + la];
+      if (A.tr_type[t] == 0) {
+        const int l = A.tr_line[t];
+        si.line[n] = l;
+        si.prow[n] = (A.ln_phi[l] + la) * A.nrays;
+        si.gline[n] = A.ln_Bji[l] / A.ln_Bij[l];
+        si.cV[n] = HC_4PI * A.ln_Bij[l] * 1.0;
+        si.tw[n] = A.ln_Aji[l] / A.ln_Bji[l];
+      } else {
+        si.line[n] = -1;
+        si.tw[n] = twohc / (lam * lam * lam);
+        si.alpha[n] = A.tr_alpha[A.tr_ptr[t] + la];
+      }
+    }
+  }
+#pragma unroll
+  for (int n = 0; n < MS; n++) {
+    uint32_t up = 0, jj = 0, ji = 0;
+    int nc = 0;
+#pragma unroll
+    for (int m = 0; m < MS; m++) {
+      if (m < si.n) {
+        if (si.tw[m] != 0.0) {
+          if (si.ti[m] == si.ti[n]) up |= 1u << m;
+          if (si.tj[m] == si.tj[n]) jj |= 1u << m;
+          if (si.tj[m] == si.ti[n]) ji |= 1u << m;
+        }
+        if (si.tj[m] == si.ti[n]) nc++;
+      }
+    }
+    si.m_up[n] = up;
+    si.m_jj[n] = jj;
+    si.m_ji[n] = ji;
+    si.ncross[n] = nc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Wavelengths with an active line, one (column, wavelength) per thread:
+// the rays with the scalar cubic Bezier method and its diagonal operator
+// (rh/rhf1d/bezier_1D.c:306-541), both directions, the new mean intensity
+// and with eval_op the radiative rate terms (rh/rhf1d/formal.c:44-346,
+// rh/fillgamma.c:82-332).
 //
-// Wavelengths with an active line are solved ray by ray with the scalar
-// cubic Bezier method and its diagonal operator (rh/rhf1d/bezier_1D.c:
-// 306-541), both directions; the others with the Feautrier method per
-// ray, whose P is the mean of the two directions (rh/rhf1d/feautrier.c:
-// 56-202).  For each ray the kernel first evaluates the opacity and source
-// function at every depth into its scratch rows, then sweeps the ray; as
-// each depth's intensity and operator are known, its rate terms are added
-// to the accumulators of the transitions' quadrature points.  Each
-// accumulator belongs to one (column, wavelength) thread, so the sums over
-// the rays need no atomics; the sum over wavelengths is done per depth by
-// k_nlte_stateq, in wavelength order.
+// The RB rays of a direction are swept together, depth by depth: RB
+// independent recursions per thread.  The opacity and source function of
+// each ray are evaluated two depths ahead of the recursion into a window
+// in registers, so no per-depth array is kept.  At each depth the rays'
+// contributions to J and to each member's rate terms are summed in
+// registers and stored once per direction: the inward sweep writes them,
+// the outward sweep adds its own.  J is double-buffered per column
+// (jpar): the previous J enters the source function, the new one is built.
 //
 // Workgroups map to (64 columns of the active list, list wavelength) by
 // xcd_tile, so a block of columns' state and populations stay in one
 // XCD's L2 across the wavelengths.
+template <int RB, int MS>
 __global__ void __launch_bounds__(64)
-k_nlte_rt(const DevModel *__restrict__ Mp, const DevNlte *__restrict__ Ap, NlteBufs B,
-          const int32_t *__restrict__ act, int nact, const int32_t *__restrict__ out_ls,
-          const int32_t *__restrict__ io_list, int nlist, int eval_op) {
+k_nlte_bezier(const DevModel *__restrict__ Mp, const DevNlte *__restrict__ Ap, NlteBufs B,
+              const int32_t *__restrict__ act, int nact, const int32_t *__restrict__ out_ls,
+              const int32_t *__restrict__ io_list, int nlist, int il_off, int eval_op) {
   const DevModel &M = *Mp;
   const DevNlte &A = *Ap;
   const int ncb = (nact + 63) / 64;
@@ -286,287 +403,417 @@ k_nlte_rt(const DevModel *__restrict__ Mp, const DevNlte *__restrict__ Ap, NlteB
   const int slot = cb * 64 + threadIdx.x;
   if (slot >= nact) return;
   const int ic = act[slot], io = io_list[il], ls = out_ls[io];
-  const int ncol = B.ncol, Nmax = B.Nmax, N = B.ndep[ic], nl = A.nl;
+  const int ncol = B.ncol, Nmax = B.Nmax, N = B.ndep[ic], nrays = A.nrays;
+  const int64_t fs = (int64_t) Nmax * ncol;
+#define STF(f, k) B.state[(int64_t) (f) * fs + (int64_t) (k) * ncol + ic]
+#define PK(k) ((int64_t) (k) * ncol + ic)
+  const int64_t wo = (int64_t) io * fs, jb = (int64_t) B.nout * fs;
+  const int par = B.jpar[ic];
+  const double *Jold = B.J + par * jb + wo;
+  double *Jnew = B.J + (1 - par) * jb + wo;
+  const double *chic = B.cchi + wo, *etac = B.ceta + wo, *scac = B.csca + wo;
+  const double lam = M.lamconst[(int64_t) ls * RHG_LC_NCONST + RHG_LC_LAMBDA];
+  const double hc_k = HC / (KBOLTZMANN * NM_TO_M);
+  SetInfo<MS> si;
+  set_info<MS>(A, ls, lam, si);
+  const double B2_0 = planck_lam(STF(RHG_CF_T, N - 2), lam);
+  const double B2_1 = planck_lam(STF(RHG_CF_T, N - 1), lam);
+  double dJmax = 0.0;
+
+  // member data at depth k
+  auto depth = [&](int k, SetDepth<MS> &sd) {
+    const double T = STF(RHG_CF_T, k);
+#pragma unroll
+    for (int n = 0; n < MS; n++) {
+      sd.ni[n] = sd.nj[n] = sd.gij[n] = sd.wla[n] = 0.0;
+      if (n < si.n) {
+        sd.ni[n] = B.pops[(int64_t) si.ti[n] * fs + PK(k)];
+        sd.nj[n] = B.pops[(int64_t) si.tj[n] * fs + PK(k)];
+        if (si.line[n] >= 0) {
+          sd.gij[n] = si.gline[n];
+          sd.wla[n] = si.wlam[n] * B.wphi[(int64_t) si.line[n] * fs + PK(k)] / HC_4PI;
+        } else {
+          sd.gij[n] = B.nstar[(int64_t) si.ti[n] * fs + PK(k)] /
+                      B.nstar[(int64_t) si.tj[n] * fs + PK(k)] * exp(-hc_k / (lam * T));
+          sd.wla[n] = FOURPI / HPLANCK * (si.wlam[n] / lam);
+        }
+      }
+    }
+  };
+  // absorption per population of each member for ray mu, direction dir
+  auto absorb = [&](int k, int mu, int dir, double (&V)[MS]) {
+#pragma unroll
+    for (int n = 0; n < MS; n++) {
+      V[n] = 0.0;
+      if (n < si.n) {
+        if (si.line[n] >= 0)
+          V[n] = si.cV[n] *
+                 B.phi[(((int64_t) (si.prow[n] + mu) * 2 + dir) * Nmax) * ncol + PK(k)];
+        else
+          V[n] = si.alpha[n];
+      }
+    }
+  };
+
+  for (int c0 = 0; c0 < nrays; c0 += RB) {
+    const int nrc = nrays - c0 < RB ? nrays - c0 : RB;
+    const bool last_chunk = c0 + RB >= nrays;
+    double zmu[RB], wmu[RB];
+#pragma unroll
+    for (int r = 0; r < RB; r++) {
+      zmu[r] = 1.0;
+      wmu[r] = 0.0;
+      if (r < nrc) {
+        zmu[r] = 1.0 / A.muz[c0 + r];
+        wmu[r] = 0.5 * A.wmu[c0 + r];
+      }
+    }
+    for (int dir = 0; dir <= 1; dir++) {
+      const bool first = (c0 == 0 && dir == 0);
+      // opacity and source function of the rays at depth k
+      auto opac = [&](int k, double (&C)[RB], double (&S)[RB]) {
+        SetDepth<MS> sd;
+        depth(k, sd);
+        const double cc = chic[PK(k)], ee = etac[PK(k)], ss = scac[PK(k)], jd = Jold[PK(k)];
+#pragma unroll
+        for (int r = 0; r < RB; r++) {
+          C[r] = 1.0;
+          S[r] = 0.0;
+          if (r < nrc) {
+            double V[MS];
+            absorb(k, c0 + r, dir, V);
+            double chi = 0.0, eta = 0.0;
+#pragma unroll
+            for (int n = 0; n < MS; n++) {
+              if (n < si.n) {
+                chi += V[n] * (sd.ni[n] - sd.gij[n] * sd.nj[n]);
+                eta += si.tw[n] * sd.gij[n] * V[n] * sd.nj[n];
+              }
+            }
+            C[r] = chi + cc;
+            double s = eta + ee;
+            s += ss * jd;
+            S[r] = s / C[r];
+          }
+        }
+      };
+      // J and rate terms of depth k
+      auto emit = [&](int k, const double (&I)[RB], const double (&Psi)[RB],
+                      const double (&C)[RB]) {
+        double js = 0.0;
+#pragma unroll
+        for (int r = 0; r < RB; r++)
+          if (r < nrc) js += wmu[r] * I[r];
+        double *jp = Jnew + PK(k);
+        const double jn = first ? js : *jp + js;
+        *jp = jn;
+        if (last_chunk && dir == 1) {
+          const double dJ = fabs(1.0 - Jold[PK(k)] / jn);
+          dJmax = dJmax > dJ ? dJmax : dJ;
+        }
+        if (!eval_op) return;
+        SetDepth<MS> sd;
+        depth(k, sd);
+        double sIJ[MS], sJI[MS];
+#pragma unroll
+        for (int n = 0; n < MS; n++) sIJ[n] = sJI[n] = 0.0;
+#pragma unroll
+        for (int r = 0; r < RB; r++) {
+          if (r < nrc) {
+            double V[MS];
+            absorb(k, c0 + r, dir, V);
+            rate_terms<MS>(si, sd, V, I[r], Psi[r] / C[r], wmu[r], sIJ, sJI);
+          }
+        }
+#pragma unroll
+        for (int n = 0; n < MS; n++) {
+          if (n < si.n) {
+            double *aij = B.acc + ((int64_t) si.ent[n] * 2) * fs + PK(k);
+            double *aji = aij + fs;
+            *aij = first ? sIJ[n] : *aij + sIJ[n];
+            *aji = first ? sJI[n] : *aji + sJI[n];
+          }
+        }
+      };
+      // |z[a] - z[b]| for neighbouring depths
+      auto dzb = [&](int a, int b) { return STF(RHG_CF_DZ, a > b ? a : b); };
+      const int dk = dir ? -1 : 1, ks = dir ? N - 1 : 0, ke = dir ? 0 : N - 1;
+      double Cm[RB], C0[RB], Cp[RB], Cn[RB], Sm[RB], S0[RB], Sp[RB], Sn[RB];
+      double I_upw[RB], dchi_c[RB], dtau_uw[RB], dS_up[RB], I[RB], Psi[RB];
+      opac(ks, Cm, Sm);
+      opac(ks + dk, C0, S0);
+      opac(ks + 2 * dk, Cp, Sp);
+      {
+        const double dz0 = dzb(ks, ks + dk);
+#pragma unroll
+        for (int r = 0; r < RB; r++) {
+          const double dt = 0.5 * zmu[r] * (Cm[r] + C0[r]) * dz0;
+          I_upw[r] = dir ? B2_1 - (B2_0 - B2_1) / dt : 0.0;
+          Psi[r] = 0.0;
+        }
+        emit(ks, I_upw, Psi, Cm);
+        const int k = ks + dk;
+        const double dzu = dzb(k, k - dk), dzd = dzb(k + dk, k);
+#pragma unroll
+        for (int r = 0; r < RB; r++) {
+          const double dsup = dzu * zmu[r], dsdn = dzd * zmu[r];
+          const double dchi_up = (C0[r] - Cm[r]) / dsup;
+          dchi_c[r] = cent_div(dsup, dsdn, Cm[r], C0[r], Cp[r]);
+          const double c1 = max0(C0[r] - (dsup / 3.0) * dchi_c[r]);
+          const double c2 = max0(Cm[r] + (dsup / 3.0) * dchi_up);
+          dtau_uw[r] = dsup * (C0[r] + Cm[r] + c1 + c2) * 0.25;
+          dS_up[r] = (S0[r] - Sm[r]) / dtau_uw[r];
+        }
+      }
+      for (int k = ks + dk; k != ke + dk; k += dk) {
+        if (k != ke) {
+          const double dzd = dzb(k + dk, k);
+          const bool far = abs(k - ke) > 1;
+          double dzd2 = 0.0;
+          if (far) {
+            opac(k + 2 * dk, Cn, Sn);
+            dzd2 = dzb(k + 2 * dk, k + dk);
+          } else {
+#pragma unroll
+            for (int r = 0; r < RB; r++) {
+              Cn[r] = 1.0;
+              Sn[r] = 0.0;
+            }
+          }
+#pragma unroll
+          for (int r = 0; r < RB; r++) {
+            const double dsdn = dzd * zmu[r];
+            const double dchi_dn = far ? cent_div(dsdn, dzd2 * zmu[r], C0[r], Cp[r], Cn[r])
+                                       : (Cp[r] - C0[r]) / dsdn;
+            double c1 = max0(C0[r] + (dsdn / 3.0) * dchi_c[r]);
+            double c2 = max0(Cp[r] - (dsdn / 3.0) * dchi_dn);
+            const double dtau_dw = dsdn * (C0[r] + Cp[r] + c1 + c2) * 0.25;
+            const double dt03 = dtau_uw[r] / 3.0;
+            double alpha, beta, gamma, theta, eps;
+            bezier3_coeffs(dtau_uw[r], alpha, beta, gamma, theta, eps);
+            const double dS_c = cent_div(dtau_uw[r], dtau_dw, Sm[r], S0[r], Sp[r]);
+            c1 = max0(S0[r] - dt03 * dS_c);
+            c2 = max0(Sm[r] + dt03 * dS_up[r]);
+            I[r] = I_upw[r] * eps + alpha * S0[r] + beta * Sm[r] + gamma * c1 + theta * c2;
+            Psi[r] = alpha + gamma;
+            I_upw[r] = I[r];
+            dchi_c[r] = dchi_dn;
+            dtau_uw[r] = dtau_dw;
+            dS_up[r] = dS_c;
+          }
+        } else {
+          const double dzu = dzb(k, k - dk);
+#pragma unroll
+          for (int r = 0; r < RB; r++) {
+            const double dt = 0.5 * zmu[r] * (C0[r] + Cm[r]) * dzu;
+            const double dS_uw = -(S0[r] - Sm[r]) / dt;
+            double w[3];
+            w3(dt, w);
+            I[r] = (1.0 - w[0]) * I_upw[r] + w[0] * S0[r] + w[1] * dS_uw;
+            Psi[r] = w[0] - w[1] / dt;
+          }
+        }
+        emit(k, I, Psi, C0);
+#pragma unroll
+        for (int r = 0; r < RB; r++) {
+          Cm[r] = C0[r];
+          C0[r] = Cp[r];
+          Cp[r] = Cn[r];
+          Sm[r] = S0[r];
+          S0[r] = Sp[r];
+          Sp[r] = Sn[r];
+        }
+      }
+    }
+  }
+  B.dJ[(int64_t) (il_off + il) * ncol + ic] = dJmax;
+#undef STF
+#undef PK
+}
+
+#define RHG_INST_BEZ(MS)                                                                        \
+  template __global__ void k_nlte_bezier<NLTE_RB, MS>(const DevModel *, const DevNlte *, NlteBufs, \
+                                                      const int32_t *, int, const int32_t *,     \
+                                                      const int32_t *, int, int, int);
+RHG_INST_BEZ(2)
+RHG_INST_BEZ(4)
+RHG_INST_BEZ(8)
+#undef RHG_INST_BEZ
+
+// ---------------------------------------------------------------------------
+// Wavelengths with continua only, one (column, wavelength) per thread: one
+// opacity for all rays, and per ray the Feautrier solution with its
+// two-sided diagonal operator (rh/rhf1d/feautrier.c:56-202; P is the mean
+// of the two directions, weighted by the full ray weight).  The opacity,
+// source function and the elimination coefficients F and z live in the
+// thread's scratch rows; the rate terms of each depth are added as the back
+// substitution reaches it.
+__global__ void __launch_bounds__(64)
+k_nlte_feautrier(const DevModel *__restrict__ Mp, const DevNlte *__restrict__ Ap, NlteBufs B,
+                 const int32_t *__restrict__ act, int nact, const int32_t *__restrict__ out_ls,
+                 const int32_t *__restrict__ io_list, int nlist, int il_off, int eval_op) {
+  constexpr int MS = NLTE_MAXSET;
+  const DevModel &M = *Mp;
+  const DevNlte &A = *Ap;
+  const int ncb = (nact + 63) / 64;
+  int cb, il;
+  xcd_tile(blockIdx.x, nlist, cb, il);
+  if (cb >= ncb) return;
+  const int slot = cb * 64 + threadIdx.x;
+  if (slot >= nact) return;
+  const int ic = act[slot], io = io_list[il], ls = out_ls[io];
+  const int ncol = B.ncol, Nmax = B.Nmax, N = B.ndep[ic], nrays = A.nrays;
   const int64_t fs = (int64_t) Nmax * ncol;
   const int64_t nthr = (int64_t) nlist * ncb * 64;
   double *scr = B.scr + (int64_t) il * ncb * 64 + slot;
 #define SCR(s, k) scr[((int64_t) (s) * Nmax + (k)) * nthr]
 #define STF(f, k) B.state[(int64_t) (f) * fs + (int64_t) (k) * ncol + ic]
 #define PK(k) ((int64_t) (k) * ncol + ic)
-  const int64_t wo = (int64_t) io * fs;
-  double *Jl = B.J + wo;
+  const int64_t wo = (int64_t) io * fs, jb = (int64_t) B.nout * fs;
+  const int par = B.jpar[ic];
+  const double *Jold = B.J + par * jb + wo;
+  double *Jnew = B.J + (1 - par) * jb + wo;
   const double *chic = B.cchi + wo, *etac = B.ceta + wo, *scac = B.csca + wo;
-  const int p0 = A.as_ptr[ls], nset = A.as_ptr[ls + 1] - p0;
   const double lam = M.lamconst[(int64_t) ls * RHG_LC_NCONST + RHG_LC_LAMBDA];
-  const double lam3 = lam * lam * lam;
-  const double twohc = 2.0 * HC / (NM_TO_M * NM_TO_M * NM_TO_M);
   const double hc_k = HC / (KBOLTZMANN * NM_TO_M);
-  const int nrays = A.nrays;
+  SetInfo<MS> si;
+  set_info<MS>(A, ls, lam, si);
 
-  // the set's transitions, levels and quadrature entries
-  int tt[NLTE_MAXSET], ti[NLTE_MAXSET], tj[NLTE_MAXSET], tent[NLTE_MAXSET];
-#pragma unroll
-  for (int n = 0; n < NLTE_MAXSET; n++) {
-    if (n < nset) {
-      const int t = A.as_tr[p0 + n];
-      tt[n] = t;
-      ti[n] = A.tr_i[t];
-      tj[n] = A.tr_j[t];
-      tent[n] = A.tr_ptr[t] + (ls - A.tr_nblue[t]);
-    } else {
-      tt[n] = ti[n] = tj[n] = tent[n] = 0;
-    }
-  }
-
-  // previous J, and the new one from zero
-  for (int k = 0; k < N; k++) {
-    SCR(NS_JDAG, k) = Jl[PK(k)];
-    Jl[PK(k)] = 0.0;
-  }
-  if (eval_op) {
-    for (int n = 0; n < nset; n++)
-      for (int s = 0; s < 2; s++) {
-        double *a = B.acc + ((int64_t) tent[n] * 2 + s) * fs;
-        for (int k = 0; k < N; k++) a[PK(k)] = 0.0;
-      }
-  }
-
-  // opacity terms of set member n at depth k for ray (mu, dir):
-  // V (absorption per population), gij, the emission coefficient factor
-  // 2hnu^3/c^2, and the quadrature weight wla (rh/opacity.c:62-300)
-  auto member = [&](int n, int k, int mu, int dir, double T, double &V, double &gij,
-                    double &tw, double &wla) {
-    const int t = tt[n];
-    const int la = ls - A.tr_nblue[t];
-    if (A.tr_type[t] == 0) {
-      const int l = A.tr_line[t];
-      const double ph =
-          B.phi[((((int64_t) (A.ln_phi[l] + la) * nrays + mu) * 2 + dir) * Nmax) * ncol + PK(k)];
-      gij = A.ln_Bji[l] / A.ln_Bij[l];
-      V = (HC_4PI * A.ln_Bij[l] * 1.0) * ph;
-      tw = A.ln_Aji[l] / A.ln_Bji[l];
-      wla = A.tr_wlam[A.tr_ptr[t] + la] * B.wphi[(int64_t) l * fs + PK(k)] / HC_4PI;
-    } else {
-      tw = twohc / lam3;
-      V = A.tr_alpha[A.tr_ptr[t] + la];
-      gij = B.nstar[(int64_t) ti[n] * fs + PK(k)] / B.nstar[(int64_t) tj[n] * fs + PK(k)] *
-            exp(-hc_k / (lam * T));
-      wla = FOURPI / HPLANCK * (A.tr_wlam[A.tr_ptr[t] + la] / lam);
-    }
-  };
-  auto pop = [&](int i, int k) { return B.pops[(int64_t) i * fs + PK(k)]; };
-
-  // opacity and source function of one ray at every depth
-  auto fill = [&](int mu, int dir) {
-    for (int k = 0; k < N; k++) {
-      const double T = STF(RHG_CF_T, k);
-      double chi = 0.0, eta = 0.0;
-      for (int n = 0; n < nset; n++) {
-        double V, gij, tw, wla;
-        member(n, k, mu, dir, T, V, gij, tw, wla);
-        const double ni = pop(ti[n], k), nj = pop(tj[n], k);
-        chi += V * (ni - gij * nj);
-        eta += tw * gij * V * nj;
-      }
-      const double chi_t = chi + chic[PK(k)];
-      double S = eta + etac[PK(k)];
-      S += scac[PK(k)] * SCR(NS_JDAG, k);
-      S /= chi_t;
-      SCR(NS_CHI, k) = chi_t;
-      SCR(NS_S, k) = S;
-      SCR(NS_ETA, k) = eta;
-    }
-  };
-
-  // rate terms of depth k for one ray: I, Psi (already divided by chi),
-  // quadrature weight of the ray wmu
-  auto gamma_point = [&](int k, double I, double Psi, int mu, int dir, double wmu) {
+  auto depth = [&](int k, SetDepth<MS> &sd, double (&V)[MS]) {
     const double T = STF(RHG_CF_T, k);
-    double V[NLTE_MAXSET], gij[NLTE_MAXSET], tw[NLTE_MAXSET], wla[NLTE_MAXSET],
-        chicc[NLTE_MAXSET], U[NLTE_MAXSET];
 #pragma unroll
-    for (int n = 0; n < NLTE_MAXSET; n++) {
-      if (n < nset) {
-        member(n, k, mu, dir, T, V[n], gij[n], tw[n], wla[n]);
-        const double ni = pop(ti[n], k), nj = pop(tj[n], k);
-        chicc[n] = V[n] * wla[n] * (ni - gij[n] * nj);
-        U[n] = tw[n] * gij[n] * V[n];
-      } else {
-        V[n] = gij[n] = tw[n] = wla[n] = chicc[n] = U[n] = 0.0;
-      }
-    }
-    const double Ieff = I - Psi * SCR(NS_ETA, k);
-#pragma unroll
-    for (int n = 0; n < NLTE_MAXSET; n++) {
-      if (n < nset) {
-        // chi_up of the lower level, Uji of the upper level (over the
-        // members with nonzero 2hnu^3/c^2, rh/fillgamma.c:251-332)
-        double chi_up = 0.0, Uj = 0.0, chi_dn = 0.0, Ui = 0.0;
-        int ncross = 0;
-#pragma unroll
-        for (int m = 0; m < NLTE_MAXSET; m++) {
-          if (m < nset && tw[m] != 0.0) {
-            if (ti[m] == ti[n]) chi_up += chicc[m];
-            if (tj[m] == tj[n]) { Uj += U[m]; chi_dn += chicc[m]; }
-            if (tj[m] == ti[n]) Ui += U[m];
-          }
-          if (m < nset && tj[m] == ti[n]) ncross++;
+    for (int n = 0; n < MS; n++) {
+      sd.ni[n] = sd.nj[n] = sd.gij[n] = sd.wla[n] = V[n] = 0.0;
+      if (n < si.n) {
+        sd.ni[n] = B.pops[(int64_t) si.ti[n] * fs + PK(k)];
+        sd.nj[n] = B.pops[(int64_t) si.tj[n] * fs + PK(k)];
+        if (si.line[n] >= 0) {
+          // a line at a Feautrier wavelength (not produced by the set-up):
+          // its profile of the first ray
+          sd.gij[n] = si.gline[n];
+          sd.wla[n] = si.wlam[n] * B.wphi[(int64_t) si.line[n] * fs + PK(k)] / HC_4PI;
+          V[n] = si.cV[n] * B.phi[(((int64_t) si.prow[n] * 2) * Nmax) * ncol + PK(k)];
+        } else {
+          sd.gij[n] = B.nstar[(int64_t) si.ti[n] * fs + PK(k)] /
+                      B.nstar[(int64_t) si.tj[n] * fs + PK(k)] * exp(-hc_k / (lam * T));
+          sd.wla[n] = FOURPI / HPLANCK * (si.wlam[n] / lam);
+          V[n] = si.alpha[n];
         }
-        double *aij = B.acc + ((int64_t) tent[n] * 2) * fs + PK(k);
-        double *aji = aij + fs;
-        const double wlamu = V[n] * wla[n] * wmu;
-        double gji = *aji, gij_ = *aij;
-        gji += Ieff * wlamu;
-        gij_ += (tw[n] + Ieff) * gij[n] * wlamu;
-        gij_ -= chi_up * Psi * Uj * wmu;
-        for (int c = 0; c < ncross; c++) gji += chi_dn * Psi * Ui * wmu;
-        *aji = gji;
-        *aij = gij_;
       }
     }
   };
 
+  for (int k = 0; k < N; k++) {
+    SetDepth<MS> sd;
+    double V[MS];
+    depth(k, sd, V);
+    double chi = 0.0, eta = 0.0;
+#pragma unroll
+    for (int n = 0; n < MS; n++) {
+      if (n < si.n) {
+        chi += V[n] * (sd.ni[n] - sd.gij[n] * sd.nj[n]);
+        eta += si.tw[n] * sd.gij[n] * V[n] * sd.nj[n];
+      }
+    }
+    const double chi_t = chi + chic[PK(k)];
+    double S = eta + etac[PK(k)];
+    S += scac[PK(k)] * Jold[PK(k)];
+    SCR(NS_CHI, k) = chi_t;
+    SCR(NS_S, k) = S / chi_t;
+  }
   const double B2_0 = planck_lam(STF(RHG_CF_T, N - 2), lam);
   const double B2_1 = planck_lam(STF(RHG_CF_T, N - 1), lam);
 #define CHI(k) SCR(NS_CHI, k)
 #define SRC(k) SCR(NS_S, k)
-
-  if (A.as_bezier[ls]) {
-    for (int mu = 0; mu < nrays; mu++) {
-      const double wmu = 0.5 * A.wmu[mu];
-      const double zmu = 1.0 / A.muz[mu];
-      for (int to_obs = 0; to_obs <= 1; to_obs++) {
-        fill(mu, to_obs);
-        auto emit = [&](int k, double I, double Psi) {
-          Jl[PK(k)] += wmu * I;
-          if (eval_op) gamma_point(k, I, Psi / CHI(k), mu, to_obs, wmu);
-        };
-        // |z[a] - z[b]| for neighbouring depths
-        auto dzb = [&](int a, int b) { return STF(RHG_CF_DZ, a > b ? a : b); };
-        const int dk = to_obs ? -1 : 1, ks = to_obs ? N - 1 : 0, ke = to_obs ? 0 : N - 1;
-        double dtau_uw = 0.5 * zmu * (CHI(ks) + CHI(ks + dk)) * dzb(ks, ks + dk);
-        double I_upw = to_obs ? B2_1 - (B2_0 - B2_1) / dtau_uw : 0.0;
-        emit(ks, I_upw, 0.0);
-        int k = ks + dk;
-        double dsup = dzb(k, k - dk) * zmu, dsdn = dzb(k + dk, k) * zmu;
-        double dchi_up = (CHI(k) - CHI(k - dk)) / dsup;
-        double dchi_c = cent_div(dsup, dsdn, CHI(k - dk), CHI(k), CHI(k + dk));
-        double c1 = max0(CHI(k) - (dsup / 3.0) * dchi_c);
-        double c2 = max0(CHI(k - dk) + (dsup / 3.0) * dchi_up);
-        dtau_uw = dsup * (CHI(k) + CHI(k - dk) + c1 + c2) * 0.25;
-        double dS_up = (SRC(k) - SRC(k - dk)) / dtau_uw;
-        double dchi_dn = 0.0, dtau_dw = 0.0, dS_c = 0.0;
-        for (k = ks + dk; k != ke + dk; k += dk) {
-          double I, Psi;
-          if (k != ke) {
-            dsdn = dzb(k + dk, k) * zmu;
-            if (abs(k - ke) > 1) {
-              const double dsdn2 = dzb(k + 2 * dk, k + dk) * zmu;
-              dchi_dn = cent_div(dsdn, dsdn2, CHI(k), CHI(k + dk), CHI(k + 2 * dk));
-            } else {
-              dchi_dn = (CHI(k + dk) - CHI(k)) / dsdn;
-            }
-            c1 = max0(CHI(k) + (dsdn / 3.0) * dchi_c);
-            c2 = max0(CHI(k + dk) - (dsdn / 3.0) * dchi_dn);
-            dtau_dw = dsdn * (CHI(k) + CHI(k + dk) + c1 + c2) * 0.25;
-            const double dt03 = dtau_uw / 3.0;
-            double alpha, beta, gamma, theta, eps;
-            bezier3_coeffs(dtau_uw, alpha, beta, gamma, theta, eps);
-            dS_c = cent_div(dtau_uw, dtau_dw, SRC(k - dk), SRC(k), SRC(k + dk));
-            c1 = max0(SRC(k) - dt03 * dS_c);
-            c2 = max0(SRC(k - dk) + dt03 * dS_up);
-            I = I_upw * eps + alpha * SRC(k) + beta * SRC(k - dk) + gamma * c1 + theta * c2;
-            Psi = alpha + gamma;
-          } else {
-            dtau_uw = 0.5 * zmu * (CHI(k) + CHI(k - dk)) * dzb(k, k - dk);
-            const double dS_uw = -(SRC(k) - SRC(k - dk)) / dtau_uw;
-            double w[3];
-            w3(dtau_uw, w);
-            I = (1.0 - w[0]) * I_upw + w[0] * SRC(k) + w[1] * dS_uw;
-            Psi = w[0] - w[1] / dtau_uw;
-          }
-          emit(k, I, Psi);
-          I_upw = I;
-          dsup = dsdn;
-          dchi_up = dchi_c;
-          dchi_c = dchi_dn;
-          dtau_uw = dtau_dw;
-          dS_up = dS_c;
-        }
-        (void) dchi_up;
-        (void) dsup;
-      }
-    }
-  } else {
-    // continua only: one opacity for all rays
-    fill(0, 0);
 #define HGT(k) STF(RHG_CF_HEIGHT, k)
-    for (int mu = 0; mu < nrays; mu++) {
-      const double wmu = A.wmu[mu];
-      const double zmu = 0.5 / A.muz[mu];
-      auto emit = [&](int k, double P, double Psi) {
-        Jl[PK(k)] += P * wmu;
-        if (eval_op) gamma_point(k, P, Psi / CHI(k), 0, 0, wmu);
-      };
-      auto dtau = [&](int k) { return zmu * (CHI(k) + CHI(k + 1)) * (HGT(k) - HGT(k + 1)); };
-      // forward elimination, rh/rhf1d/feautrier.c:56-202 (dark top,
-      // thermalised bottom, STANDARD order)
-      const double dt0 = dtau(0);
-      const double abc0 = 1.0 + 2.0 * 1.0 / dt0;
-      const double C1_0 = 2.0 / (dt0 * dt0);
-      const double S0 = SRC(0) + 2.0 * 0.0 / ((1.0 + 0.0) * dt0);
-      double F = abc0 / C1_0, z = S0 / (abc0 + C1_0);
-      SCR(NS_F, 0) = F;
-      SCR(NS_Z, 0) = z;
-      double dtm1 = dt0;
-      for (int k = 1; k < N - 1; k++) {
-        const double dtk = dtau(k);
-        const double dtau_mid = 0.5 * (dtk + dtm1);
-        const double A1 = 1.0 / (dtau_mid * dtm1), C1 = 1.0 / (dtau_mid * dtk);
-        F = (1.0 + A1 * F / (1.0 + F)) / C1;
-        z = (SRC(k) + A1 * z) / (C1 * (1.0 + F));
-        SCR(NS_F, k) = F;
-        SCR(NS_Z, k) = z;
-        dtm1 = dtk;
+  for (int mu = 0; mu < nrays; mu++) {
+    const double wmu = A.wmu[mu];
+    const double zmu = 0.5 / A.muz[mu];
+    auto emit = [&](int k, double P, double Psi) {
+      double *jp = Jnew + PK(k);
+      *jp = (mu == 0 ? 0.0 : *jp) + P * wmu;
+      if (!eval_op) return;
+      SetDepth<MS> sd;
+      double V[MS], sIJ[MS], sJI[MS];
+      depth(k, sd, V);
+#pragma unroll
+      for (int n = 0; n < MS; n++) sIJ[n] = sJI[n] = 0.0;
+      rate_terms<MS>(si, sd, V, P, Psi / CHI(k), wmu, sIJ, sJI);
+#pragma unroll
+      for (int n = 0; n < MS; n++) {
+        if (n < si.n) {
+          double *aij = B.acc + ((int64_t) si.ent[n] * 2) * fs + PK(k);
+          double *aji = aij + fs;
+          *aij = (mu == 0 ? 0.0 : *aij) + sIJ[n];
+          *aji = (mu == 0 ? 0.0 : *aji) + sJI[n];
+        }
       }
-      // back substitution with the two-sided operator
-      const double dtN = dtm1;   // dtau[N-2]
-      const double hN = B2_1 - (B2_0 - B2_1) / dtN;
-      const double abcN = 1.0 + 2.0 * 1.0 / dtN;
-      const double A1N = 2.0 / (dtN * dtN);
-      const double SN = SRC(N - 1) + 2.0 * hN / ((1.0 + 0.0) * dtN);
-      const double FN2 = SCR(NS_F, N - 2);
-      double P = (SN + A1N * SCR(NS_Z, N - 2)) / (abcN + A1N * (FN2 / (1.0 + FN2)));
-      emit(N - 1, P, 1.0 / (abcN + A1N * FN2 / (1.0 + FN2)));
-      double G = abcN / A1N;   // G[N-1]
-      double dtk = dtN;        // dtau[k] for k = N-2
-      for (int k = N - 2; k >= 1; k--) {
-        const double Fk = SCR(NS_F, k), Fm = SCR(NS_F, k - 1);
-        P = P / (1.0 + Fk) + SCR(NS_Z, k);
-        const double dtkm = dtau(k - 1);
-        const double dtau_mid = 0.5 * (dtk + dtkm);
-        const double A1 = 1.0 / (dtau_mid * dtkm), C1 = 1.0 / (dtau_mid * dtk);
-        emit(k, P, 1.0 / (1.0 + A1 * Fm / (1.0 + Fm) + C1 * G / (1.0 + G)));
-        G = (1.0 + C1 * G / (1.0 + G)) / A1;
-        dtk = dtkm;
-      }
-      P = P / (1.0 + SCR(NS_F, 0)) + SCR(NS_Z, 0);
-      emit(0, P, 1.0 / (abc0 + C1_0 * G / (1.0 + G)));
+    };
+    auto dtau = [&](int k) { return zmu * (CHI(k) + CHI(k + 1)) * (HGT(k) - HGT(k + 1)); };
+    // forward elimination, rh/rhf1d/feautrier.c:56-202 (dark top,
+    // thermalised bottom, STANDARD order)
+    const double dt0 = dtau(0);
+    const double abc0 = 1.0 + 2.0 * 1.0 / dt0;
+    const double C1_0 = 2.0 / (dt0 * dt0);
+    const double S0 = SRC(0) + 2.0 * 0.0 / ((1.0 + 0.0) * dt0);
+    double F = abc0 / C1_0, z = S0 / (abc0 + C1_0);
+    SCR(NS_F, 0) = F;
+    SCR(NS_Z, 0) = z;
+    double dtm1 = dt0;
+    for (int k = 1; k < N - 1; k++) {
+      const double dtk = dtau(k);
+      const double dtau_mid = 0.5 * (dtk + dtm1);
+      const double A1 = 1.0 / (dtau_mid * dtm1), C1 = 1.0 / (dtau_mid * dtk);
+      F = (1.0 + A1 * F / (1.0 + F)) / C1;
+      z = (SRC(k) + A1 * z) / (C1 * (1.0 + F));
+      SCR(NS_F, k) = F;
+      SCR(NS_Z, k) = z;
+      dtm1 = dtk;
     }
-#undef HGT
+    // back substitution with the two-sided operator
+    const double dtN = dtm1;   // dtau[N-2]
+    const double hN = B2_1 - (B2_0 - B2_1) / dtN;
+    const double abcN = 1.0 + 2.0 * 1.0 / dtN;
+    const double A1N = 2.0 / (dtN * dtN);
+    const double SN = SRC(N - 1) + 2.0 * hN / ((1.0 + 0.0) * dtN);
+    const double FN2 = SCR(NS_F, N - 2);
+    double P = (SN + A1N * SCR(NS_Z, N - 2)) / (abcN + A1N * (FN2 / (1.0 + FN2)));
+    emit(N - 1, P, 1.0 / (abcN + A1N * FN2 / (1.0 + FN2)));
+    double G = abcN / A1N;   // G[N-1]
+    double dtk = dtN;        // dtau[k] for k = N-2
+    for (int k = N - 2; k >= 1; k--) {
+      const double Fk = SCR(NS_F, k), Fm = SCR(NS_F, k - 1);
+      P = P / (1.0 + Fk) + SCR(NS_Z, k);
+      const double dtkm = dtau(k - 1);
+      const double dtau_mid = 0.5 * (dtk + dtkm);
+      const double A1 = 1.0 / (dtau_mid * dtkm), C1 = 1.0 / (dtau_mid * dtk);
+      emit(k, P, 1.0 / (1.0 + A1 * Fm / (1.0 + Fm) + C1 * G / (1.0 + G)));
+      G = (1.0 + C1 * G / (1.0 + G)) / A1;
+      dtk = dtkm;
+    }
+    P = P / (1.0 + SCR(NS_F, 0)) + SCR(NS_Z, 0);
+    emit(0, P, 1.0 / (abc0 + C1_0 * G / (1.0 + G)));
   }
-
   double dJmax = 0.0;
   for (int k = 0; k < N; k++) {
-    const double dJ = fabs(1.0 - SCR(NS_JDAG, k) / Jl[PK(k)]);
+    const double dJ = fabs(1.0 - Jold[PK(k)] / Jnew[PK(k)]);
     dJmax = dJmax > dJ ? dJmax : dJ;
   }
-  B.dJ[(int64_t) il * ncol + ic] = dJmax;
+  B.dJ[(int64_t) (il_off + il) * ncol + ic] = dJmax;
+#undef HGT
 #undef CHI
 #undef SRC
 #undef SCR
 #undef STF
 #undef PK
+}
+
+// the new J becomes the previous one for the columns just solved
+__global__ void k_nlte_flip(NlteBufs B, const int32_t *__restrict__ act, int nact) {
+  const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= nact) return;
+  const int ic = act[slot];
+  B.jpar[ic] = 1 - B.jpar[ic];
 }
 
 // max over the list wavelengths of the per-wavelength dJ, per column

@@ -14,10 +14,15 @@ constexpr double E_RYDBERG = 2.1798741E-18;
 constexpr int NLTE_MAXSET = 8;
 constexpr int NLTE_MAXLEV = 16;
 constexpr int NLTE_MAXNG = 4;                  // Ng order
-// per-thread scratch rows of k_nlte_rt: total opacity, source function,
-// atomic emissivity, the previous mean intensity, and the Feautrier
-// elimination coefficients F and z
-enum { NS_CHI = 0, NS_S, NS_ETA, NS_JDAG, NS_F, NS_Z, NLTE_NSLOT };
+// per-thread scratch rows of k_nlte_feautrier: total opacity, source
+// function and the elimination coefficients F and z
+enum { NS_CHI = 0, NS_S, NS_F, NS_Z, NLTE_NSLOT };
+// rays swept together by a k_nlte_bezier thread (more rays are swept in
+// chunks of this many)
+#ifndef RHG_NLTE_RB
+#define RHG_NLTE_RB 2
+#endif
+constexpr int NLTE_RB = RHG_NLTE_RB;
 
 // Device mirror of rhg_nlte_t, plus two derived tables: the transition of
 // each line (ln_tr) and each line's first row in the profile array
@@ -53,9 +58,10 @@ struct NlteBufs {
   double *phi;                    // [(nphi * nrays + mu) * 2 + to_obs]: profiles
   double *cchi, *ceta, *csca;     // [nout]: background opacity (with scattering),
                                   //         emissivity and scattering opacity
-  double *J;                      // [nout]: mean intensity
+  double *J;                      // [2 * nout]: mean intensity, previous and new
+  int32_t *jpar;                  // per column: which half of J is the previous
   double *acc;                    // [nentry * 2 + s]: rate terms per quadrature point
-  double *scr;                    // [NLTE_NSLOT] per k_nlte_rt thread
+  double *scr;                    // [NLTE_NSLOT] per k_nlte_feautrier thread
   double *dJ;                     // [nlist] x ncol: max |1 - J_old / J| per wavelength
   double *hist;                   // [(ng_order + 2) * nl]: Ng history
 };
@@ -64,9 +70,14 @@ __global__ void k_nlte_pre(const DevModel *Mp, const DevNlte *Ap, NlteBufs B);
 __global__ void k_nlte_bg(const DevModel *Mp, const DevNlte *Ap, NlteBufs B,
                           const int32_t *out_ls);
 __global__ void k_nlte_profile(const DevModel *Mp, const DevNlte *Ap, NlteBufs B);
-__global__ void k_nlte_rt(const DevModel *Mp, const DevNlte *Ap, NlteBufs B,
-                          const int32_t *act, int nact, const int32_t *out_ls,
-                          const int32_t *io_list, int nlist, int eval_op);
+template <int RB, int MS>
+__global__ void k_nlte_bezier(const DevModel *Mp, const DevNlte *Ap, NlteBufs B,
+                              const int32_t *act, int nact, const int32_t *out_ls,
+                              const int32_t *io_list, int nlist, int il_off, int eval_op);
+__global__ void k_nlte_feautrier(const DevModel *Mp, const DevNlte *Ap, NlteBufs B,
+                                 const int32_t *act, int nact, const int32_t *out_ls,
+                                 const int32_t *io_list, int nlist, int il_off, int eval_op);
+__global__ void k_nlte_flip(NlteBufs B, const int32_t *act, int nact);
 __global__ void k_nlte_djmax(NlteBufs B, const int32_t *act, int nact, int nlist,
                              double *djmax);
 template <int NLP>

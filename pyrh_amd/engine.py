@@ -27,8 +27,8 @@ STAGES = ("state", "geometry", "continuum", "formal")
 # the four intervals rhg_last_timings reports after rhg_rf_lte
 RF_STAGES = ("synthesis_inputs", "state_tangents", "tape", "reverse_and_chain")
 # rhg_nlte_timings slots (RHG_NLTE_NTIMING)
-NLTE_TIMINGS = ("setup_ms", "scatter_rt_ms", "iter_rt_ms", "stateq_ms", "ng_ms", "total_ms",
-                "rt_launches", "rt_pairs")
+NLTE_TIMINGS = ("setup_ms", "scatter_bezier_ms", "iter_bezier_ms", "stateq_ms", "ng_ms",
+                "total_ms", "rt_passes", "rt_pairs", "scatter_feautrier_ms", "iter_feautrier_ms")
 
 
 class EngineError(RuntimeError):
