@@ -206,7 +206,9 @@ def _rf_leg(eng, W, torch, dev, reps):
 def _nlte_leg(torch, dev, ncol, do_cpu, nworkers):
     """NLTE populations of the 4096-column batch: one rhg_nlte_solve of the
     whole batch with the atmospheres resident in HBM (set-up, scattering
-    passes, iterations until every column has converged), columns/s, the
+    passes, iterations until every column has converged, J with the
+    converged populations and the emergent ray at mu = 1, everything an
+    rhf1d() call does), columns/s, the
     iteration histogram, the time split and the roofline of the formal
     solution kernel k_nlte_rt; the populations of 16 columns against the
     oracle; the reference's rhf1d() on a bounded sample of the same columns
@@ -224,10 +226,11 @@ def _nlte_leg(torch, dev, ncol, do_cpu, nworkers):
     d_pops = torch.empty((ncol, nm.nlevel, nd), dtype=torch.float64, device=dev)
     d_niter = torch.empty(ncol, dtype=torch.int32, device=dev)
     d_dp = torch.empty(ncol, dtype=torch.float64, device=dev)
+    d_st = torch.empty((ncol, 4, nm.lam.size - 1), dtype=torch.float64, device=dev)
 
     def solve(n):
         eng.nlte_solve_device(nm, d_atm.data_ptr(), d_ndep.data_ptr(), n, nd, d_pops.data_ptr(),
-                              d_niter.data_ptr(), d_dp.data_ptr())
+                              d_niter.data_ptr(), d_dp.data_ptr(), d_stokes=d_st.data_ptr())
 
     solve(min(ncol, 64))            # warm-up (module load, allocations)
     torch.cuda.synchronize()

@@ -320,12 +320,19 @@ int rhg_rf_loggf(rhg_context_t *ctx, const rhg_columns_t *cols, int32_t npar,
  * on the atom's merged wavelength grid (pyrh_amd.nlte.build_nlte);
  * columns as in rhg_synth_lte.  pops_out is [ncol][nlevel][ndep_max]
  * float64 (zero past ndep), niter_out [ncol] the iterations done and
- * dpops_out [ncol] the last maximum relative change; host memory, or device
- * memory for all three and the columns if `device_ptrs`.  FP64.
+ * dpops_out [ncol] the last maximum relative change.  stokes_out (NULL to
+ * skip) receives what rhf1d() returns afterwards: J updated with the
+ * converged populations (up to N_MAX_SCATTER passes,
+ * rh/rhf1d/pyrh_compute1dray.c:332-337), then the emergent intensity of
+ * one ray at cols->mu (_solveray, rh/rhf1d/pyrh_solveray.c:82-150),
+ * [ncol][4][nspect-1] like rhg_synth_lte's, Q, U, V zero.  nstar_out
+ * (NULL to skip) receives the atom's LTE populations in the layout of
+ * pops_out (pyrh's Populations.nstar, pyrh.pyx:646-656).  Host memory, or
+ * device memory for the outputs and the columns if `device_ptrs`.  FP64.
  * Synchronous. */
 int rhg_nlte_solve(rhg_context_t *ctx, const rhg_nlte_t *atom, const rhg_columns_t *cols,
                    int32_t device_ptrs, double *pops_out, int32_t *niter_out,
-                   double *dpops_out);
+                   double *dpops_out, double *stokes_out, double *nstar_out);
 
 /* Timings of the last rhg_nlte_solve in ms and its work counts:
  * [0] set-up (equilibria, depth scale, background, profiles, collisions),

@@ -105,28 +105,30 @@ def rf_loggf(model, atmos, par_line, mu=1.0, atm_scale=0, ndep=None, nthreads=0)
     return st, rf
 
 
-def nlte_solve(nm, atmos, ndep=None, atm_scale=0, nthreads=0):
+def nlte_solve(nm, atmos, ndep=None, atm_scale=0, nthreads=0, mu=1.0, stokes=False):
     """NLTE populations [ncol, nlevel, ndep_max], iterations [ncol] and the
     last max relative change [ncol] for the NLTE model nm
-    (pyrh_amd.nlte.build_nlte)."""
+    (pyrh_amd.nlte.build_nlte); with stokes=True also the emergent spectrum
+    at mu afterwards [ncol, 4, nspect-1]."""
     from pyrh_amd.abi import NlteStruct
     L = lib()
     L.rho_nlte_solve.argtypes = [C.POINTER(RhgModel), C.c_void_p, C.POINTER(RhgColumns), C.c_int,
                                  C.POINTER(C.c_double), C.POINTER(C.c_int),
-                                 C.POINTER(C.c_double)]
+                                 C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.rho_nlte_solve.restype = C.c_int
     ms = ModelStruct(nm.lte)
     ns = NlteStruct(nm)
-    cs = ColumnsStruct(atmos, ndep, atm_scale, 1.0)
+    cs = ColumnsStruct(atmos, ndep, atm_scale, mu)
     ncol, nd = cs.struct.ncol, cs.struct.ndep_max
     pops = np.zeros((ncol, nm.nlevel, nd))
     niter = np.zeros(ncol, np.int32)
     dpops = np.zeros(ncol)
+    st = np.zeros((ncol, 4, nm.lam.size - 1)) if stokes else None
     rc = L.rho_nlte_solve(ms.ptr(), C.addressof(ns.struct), cs.ptr(), int(nthreads), _p(pops),
-                          niter.ctypes.data_as(C.POINTER(C.c_int)), _p(dpops))
+                          niter.ctypes.data_as(C.POINTER(C.c_int)), _p(dpops), _p(st))
     if rc != 0:
         raise RuntimeError(f"oracle failed with code {rc}")
-    return pops, niter, dpops
+    return (pops, niter, dpops, st) if stokes else (pops, niter, dpops)
 
 
 def formal_bezier_in(z, mu, chi, S):

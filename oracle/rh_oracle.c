@@ -1354,7 +1354,7 @@ static int line_opacity(const rhg_model_t *m, const column_t *c, int ls,
   return line_opacity_sel(m, c, ls, mu, chi, eta, -1);
 }
 
-struct nlte_out { double *pops; int *niter; double *dpops; int ndep_max; };
+struct nlte_out { double *pops; int *niter; double *dpops; int ndep_max; double *stokes; double mu; };
 static int nlte_column(const rhg_model_t *m, const rhg_nlte_t *na, const column_t *c,
                        struct nlte_out *o);
 
@@ -1790,6 +1790,7 @@ typedef struct {
   int N, nl, nrays;
   double *n, *nstar, *ntotal, *vbroad, *C, *Gamma, *J, *Jdag;
   double *phi, *wphi;               /* per line: [la][mu][dir][k], [k] */
+  double *adamp_l;                  /* per line: damping [k] */
   const int *phi_off;               /* line -> offset into phi */
   double *bg_chi, *bg_eta, *bg_sca; /* [nspect][N] (active wavelengths) */
   double *chi, *S, *I, *Psi, *eta_at, *Ieff, *scr;
@@ -2144,6 +2145,7 @@ static int nlte_column(const rhg_model_t *m, const rhg_nlte_t *na, const column_
   w.chi_down = calloc((size_t) nl * N, sizeof(double));
   w.Uji = calloc((size_t) nl * N, sizeof(double));
   adamp = calloc(N, sizeof(double));
+  w.adamp_l = calloc((size_t) (na->nline + 1) * N, sizeof(double));
 
   /* LTE populations and Doppler widths (rh/readatom.c:213-220) */
   for (i = 0; i < nl; i++)
@@ -2176,6 +2178,7 @@ static int nlte_column(const rhg_model_t *m, const rhg_nlte_t *na, const column_
       wphi[k] = 0.0;
     }
     nlte_dump(dump, "adamp", l, adamp, N);
+    for (k = 0; k < N; k++) w.adamp_l[(size_t) l * N + k] = adamp[k];
     for (la = 0; la < na->tr_nlam[t]; la++) {
       const double lamla = m_lambda(m, na->tr_nblue[t] + la);
       const double wl = na->tr_wlam[na->tr_ptr[t] + la];
@@ -2246,7 +2249,7 @@ static int nlte_column(const rhg_model_t *m, const rhg_nlte_t *na, const column_
   for (ls = 0; ls < nspect; ls++) {
     double *bc = w.bg_chi + (size_t) ls * N, *be = w.bg_eta + (size_t) ls * N,
            *bs = w.bg_sca + (size_t) ls * N;
-    if (na->as_ptr[ls + 1] == na->as_ptr[ls]) continue;
+    if (ls == m->ref_index) continue;
     background_continuum(m, c, ls, bc, be);
     for (k = 0; k < N; k++) {
       double sca = c->ne[k] * na->sigma_thomson;
@@ -2280,7 +2283,7 @@ static int nlte_column(const rhg_model_t *m, const rhg_nlte_t *na, const column_
     while (it < na->nmax_scatter) {
       double dJmax = 0.0;
       for (ls = 0; ls < nspect; ls++)
-        if (na->as_ptr[ls + 1] > na->as_ptr[ls]) {
+        if (ls != m->ref_index) {
           const double dJ = nlte_formal(&w, ls, 0);
           dJmax = MAXV(dJmax, dJ);
         }
@@ -2297,7 +2300,7 @@ static int nlte_column(const rhg_model_t *m, const rhg_nlte_t *na, const column_
   for (niter = 1; niter <= na->nmax_iter; niter++) {
     for (i = 0; i < nl * nl * N; i++) w.Gamma[i] = w.C[i];
     for (ls = 0; ls < nspect; ls++)
-      if (na->as_ptr[ls + 1] > na->as_ptr[ls]) nlte_formal(&w, ls, 1);
+      if (ls != m->ref_index) nlte_formal(&w, ls, 1);
     if (niter == 1)
       for (i = 0; i < nl * nl; i++) nlte_dump(dump, "gamma1", i, w.Gamma + (size_t) i * N, N);
     for (k = 0; k < N; k++) {
@@ -2328,6 +2331,96 @@ static int nlte_column(const rhg_model_t *m, const rhg_nlte_t *na, const column_
     if (dpops < na->iter_limit) break;
   }
   if (niter > na->nmax_iter) niter = na->nmax_iter;
+
+  /* J with the converged populations: up to N_MAX_SCATTER passes without
+     the operator, stopping at max dJ <= ITER_LIMIT
+     (rh/rhf1d/pyrh_compute1dray.c:332-337) */
+  if (o->stokes) {
+    int it = 0;
+    while (it < na->nmax_scatter) {
+      double dJmax = 0.0;
+      for (ls = 0; ls < nspect; ls++)
+        if (ls != m->ref_index) {
+          const double dJ = nlte_formal(&w, ls, 0);
+          dJmax = MAXV(dJmax, dJ);
+        }
+      if (dJmax <= na->iter_limit) break;
+      it++;
+    }
+  }
+
+  /* the emergent spectrum at the column's mu with the converged
+     populations and the last J (pyrh's _solveray, rh/rhf1d/pyrh_solveray.c:
+     82-107: one ray of weight 1, profiles for it, background scattering
+     from the stored J, no update of J; rh/rhf1d/formal.c:44-346) */
+  if (o->stokes) {
+    const double muf = o->mu;
+    int io = 0;
+    for (ls = 0; ls < nspect; ls++) {
+      const double lam = m_lambda(m, ls);
+      const double *chic = w.bg_chi + (size_t) ls * N, *etac = w.bg_eta + (size_t) ls * N,
+                   *scac = w.bg_sca + (size_t) ls * N, *Jl = w.J + (size_t) ls * N;
+      const int p0 = na->as_ptr[ls], p1 = na->as_ptr[ls + 1];
+      double B2[2], Iout;
+      int e;
+      if (ls == m->ref_index) continue;
+      for (k = 0; k < N; k++) { w.chi[k] = 0.0; w.eta_at[k] = 0.0; }
+      for (e = p0; e < p1; e++) {
+        const int t = na->as_tr[e], it = na->tr_i[t], jt = na->tr_j[t];
+        const int la2 = ls - na->tr_nblue[t];
+        const double *ni = w.n + (size_t) it * N, *nj = w.n + (size_t) jt * N;
+        for (k = 0; k < N; k++) {
+          double V, gij, tw;
+          if (na->tr_type[t] == 0) {
+            const int l = na->tr_line[t];
+            const double lambda0 = na->ln_lambda0[l];
+            const double v = (lam - lambda0 - 0.0) * CLIGHT / (w.vbroad[k] * lambda0);
+            const double vlos = muf * c->vz[k] / w.vbroad[k];
+            const double ph = rho_voigt_armstrong(w.adamp_l[(size_t) l * N + k], v + 1.0 * vlos) *
+                              1.0 / (SQRTPI * w.vbroad[k]);
+            gij = na->ln_Bji[l] / na->ln_Bij[l];
+            V = (HPLANCK * CLIGHT / (4.0 * PI) * na->ln_Bij[l] * 1.0) * ph;
+            tw = na->ln_Aji[l] / na->ln_Bji[l];
+          } else {
+            tw = 2.0 * HPLANCK * CLIGHT / CUBE(NM_TO_M) / CUBE(lam);
+            V = na->tr_alpha[na->tr_ptr[t] + la2];
+            gij = w.nstar[(size_t) it * N + k] / w.nstar[(size_t) jt * N + k] *
+                  exp(-HPLANCK * CLIGHT / (KBOLTZMANN * NM_TO_M) / (lam * c->T[k]));
+          }
+          w.chi[k] += V * (ni[k] - gij * nj[k]);
+          w.eta_at[k] += tw * gij * V * nj[k];
+        }
+      }
+      B2[0] = rho_planck(c->T[N - 2], lam);
+      B2[1] = rho_planck(c->T[N - 1], lam);
+      if (na->as_bezier[ls]) {
+        for (k = 0; k < N; k++) {
+          const double chi_as = w.chi[k];
+          w.chi[k] = chi_as + chic[k];
+          w.S[k] = w.eta_at[k] + etac[k];
+          w.S[k] += scac[k] * Jl[k];
+        }
+        for (k = 0; k < N; k++) w.S[k] /= w.chi[k];
+        bezier_ray(N, c->height, 1.0 / muf, w.chi, w.S, B2, 1, w.I, w.Psi);
+        nlte_dump(dump, "fin_chi", ls, w.chi, N);
+        nlte_dump(dump, "fin_S", ls, w.S, N);
+        nlte_dump(dump, "fin_J", ls, Jl, N);
+        Iout = w.I[0];
+      } else {
+        for (k = 0; k < N; k++) {
+          const double chi_as = w.chi[k];
+          w.chi[k] = chi_as + chic[k];
+          w.S[k] = (w.eta_at[k] + etac[k] + scac[k] * Jl[k]) / w.chi[k];
+        }
+        Iout = rho_formal_feautrier(N, c->height, muf, w.chi, w.S, B2);
+      }
+      o->stokes[0 * (nspect - 1) + io] = Iout;
+      o->stokes[1 * (nspect - 1) + io] = 0.0;
+      o->stokes[2 * (nspect - 1) + io] = 0.0;
+      o->stokes[3 * (nspect - 1) + io] = 0.0;
+      io++;
+    }
+  }
   for (i = 0; i < nl; i++)
     for (k = 0; k < o->ndep_max; k++)
       o->pops[(size_t) i * o->ndep_max + k] = k < N ? w.n[(size_t) i * N + k] : 0.0;
@@ -2336,7 +2429,7 @@ static int nlte_column(const rhg_model_t *m, const rhg_nlte_t *na, const column_
 
   if (dump) fclose(dump);
   free(ng.prev);
-  free(adamp); free(phi_off);
+  free(adamp); free(phi_off); free(w.adamp_l);
   free(w.n); free(w.nstar); free(w.ntotal); free(w.vbroad); free(w.C); free(w.Gamma);
   free(w.J); free(w.Jdag); free(w.phi); free(w.wphi); free(w.bg_chi); free(w.bg_eta);
   free(w.bg_sca); free(w.chi); free(w.S); free(w.I); free(w.Psi); free(w.eta_at);
@@ -2348,7 +2441,8 @@ static int nlte_column(const rhg_model_t *m, const rhg_nlte_t *na, const column_
 /* NLTE populations of the active atom per column: pops_out
    [ncol][nlevel][ndep_max], iterations and final max relative change */
 int rho_nlte_solve(const rhg_model_t *m, const rhg_nlte_t *na, const rhg_columns_t *cols,
-                   int nthreads, double *pops_out, int *niter_out, double *dpops_out)
+                   int nthreads, double *pops_out, int *niter_out, double *dpops_out,
+                   double *stokes_out)
 {
   int ic, err = 0;
   if (m->nnuclei + m->nmol > NEQ_MAX) return RHG_EUNSUPPORTED;
@@ -2363,6 +2457,8 @@ int rho_nlte_solve(const rhg_model_t *m, const rhg_nlte_t *na, const rhg_columns
     o.niter = niter_out + ic;
     o.dpops = dpops_out + ic;
     o.ndep_max = cols->ndep_max;
+    o.stokes = stokes_out ? stokes_out + (size_t) ic * 4 * (m->nspect - 1) : NULL;
+    o.mu = cols->mu;
     if (synth_column_ex(m, cols, ic, 0, NULL, NULL, NULL, -1, NULL, NULL, NULL, 0, NULL, NULL,
                         na, &o))
       err |= 1;

@@ -36,9 +36,12 @@ int rho_rf_loggf(const rhg_model_t *m, const rhg_columns_t *cols, int npar,
  * as rhf1d() runs it, rh/iterate.c:48-143): the model is the background
  * model on the merged grid.  pops_out [ncol][nlevel][ndep_max]; niter_out
  * and dpops_out [ncol] get the iterations done and the last max relative
- * change. */
+ * change; stokes_out (NULL to skip) [ncol][4][nspect-1] the emergent
+ * spectrum at cols->mu afterwards (pyrh's _solveray,
+ * rh/rhf1d/pyrh_solveray.c:82-107), Q, U, V zero. */
 int rho_nlte_solve(const rhg_model_t *m, const rhg_nlte_t *na, const rhg_columns_t *cols,
-                   int nthreads, double *pops_out, int *niter_out, double *dpops_out);
+                   int nthreads, double *pops_out, int *niter_out, double *dpops_out,
+                   double *stokes_out);
 
 /* Single-function entry points for closed-form and invariant tests. */
 double rho_voigt_humlicek(double a, double v, double *F);

@@ -1,8 +1,10 @@
 """pyrh-compatible Python API on top of the GPU engine.
 
 ``compute1d`` keeps the signature and return tuple of pyrh.compute1d
-(pyrh.pyx:537-668) for the LTE path; ``synth_batch`` is the batched form
-that the GPU is built for.  Parsed inputs, models and device contexts are
+(pyrh.pyx:537-668): LTE runs go through the LTE synthesis, runs with an
+ACTIVE atom through the NLTE solve (populations, then the emergent ray);
+``synth_batch`` and ``nlte_batch`` are the batched forms that the GPU is
+built for.  Parsed inputs, models and device contexts are
 cached per run directory and wavelength grid, so repeated calls do not
 re-read files (the reference re-reads everything on every call,
 rh/rhf1d/pyrh_compute1dray.c:130-301).
@@ -15,10 +17,24 @@ import os
 import numpy as np
 
 from . import model as _model
+from . import nlte as _nlte
 from .engine import Engine
 
 _SETUPS = {}
 _ENGINES = {}
+_NLTE = {}
+
+
+class Populations:
+    """Populations of one active atom, pyrh.Populations (pyrh.pyx:170-176):
+    ID, nlevel, nz (depth points), n and nstar [nlevel, nz]."""
+
+    def __init__(self, ID, nlevel, nz, n, nstar):
+        self.ID = ID
+        self.nlevel = nlevel
+        self.nz = nz
+        self.n = n
+        self.nstar = nstar
 
 
 def _key(*arrays):
@@ -58,6 +74,40 @@ def engine_for(cwd, wave, atomic_number=None, atomic_abundance=None, loggf_ids=N
     return e
 
 
+def nlte_engine_for(cwd, wave, atomic_number=None, atomic_abundance=None, loggf_ids=None,
+                    loggf_values=None, lam_ids=None, lam_values=None, device=0):
+    """(engine, NLTE model) for a run directory with one ACTIVE atom."""
+    skey, s = _setup(cwd, atomic_number, atomic_abundance, loggf_ids, loggf_values,
+                     lam_ids, lam_values)
+    key = (skey, _key(np.asarray(wave, np.float64)), device)
+    r = _NLTE.get(key)
+    if r is None:
+        nm = _nlte.build_nlte(s, np.asarray(wave, np.float64))
+        r = (Engine(nm.lte, device), nm)
+        _NLTE[key] = r
+    return r
+
+
+def has_active_atom(cwd):
+    s = _setup(cwd, None, None, None, None, None, None)[1]
+    return any(a.active for a in s.atoms)
+
+
+def nlte_batch(cwd, mu, atm_scale, atmospheres, wave, ndep=None, device=0):
+    """NLTE solve of a batch of columns [ncol, >=9, ndep_max] for a run
+    directory with one ACTIVE atom: a dict with the populations n and nstar
+    [ncol, nlevel, ndep_max], the iterations and last maximum relative change
+    per column, the emergent Stokes I, Q, U, V [ncol, nlw] at mu and the
+    wavelengths [nlw]."""
+    e, nm = nlte_engine_for(cwd, wave, device=device)
+    atm = np.asarray(atmospheres, np.float64)
+    pops, niter, dpops, st, nst = e.nlte_solve(nm, atm, mu, atm_scale, ndep, stokes=True,
+                                               nstar=True)
+    lam = e.model.lam[e.model.out_index]
+    return {"n": pops, "nstar": nst, "niter": niter, "dpops": dpops, "I": st[:, 0],
+            "Q": st[:, 1], "U": st[:, 2], "V": st[:, 3], "lam": lam}
+
+
 def synth_batch(cwd, mu, atm_scale, atmospheres, wave, ndep=None, precision="fp64",
                 atomic_number=None, atomic_abundance=None, device=0):
     """Stokes I, Q, U, V [ncol, nlw] and the wavelengths [nlw] for a batch
@@ -76,11 +126,12 @@ def compute1d(cwd, mu, atm_scale, atmosphere, wave, loggf_ids=None, loggf_values
     """pyrh.compute1d (pyrh.pyx:537-668): returns (I, Q, U, V, lam) for one
     column, and with get_atomic_rfs ((I, Q, U, V, lam), rf) where rf is
     d I / d log gf [len(loggf_ids) + len(lam_ids), nlw] (the wavelength-shift
-    rows are zero, as in the reference).  Arrays own their memory and the
-    inputs are not modified."""
-    if get_populations:
-        raise NotImplementedError("get_populations applies to active (NLTE) atoms, which "
-                                  "this LTE engine does not solve")
+    rows are zero, as in the reference).  With get_populations the result is
+    (that, populations): a tuple with one Populations per active atom (empty
+    for an LTE run).  A run directory with an ACTIVE atom is solved in NLTE
+    (rhf1d's Iterate and final ray) and returns the emergent spectrum of the
+    converged atom.  Arrays own their memory and the inputs are not
+    modified."""
     if fudge_wave is not None or fudge_value is not None:
         raise NotImplementedError("opacity fudge factors are not implemented")
     if loggf_ids is not None and loggf_values is not None and \
@@ -94,6 +145,21 @@ def compute1d(cwd, mu, atm_scale, atmosphere, wave, loggf_ids=None, loggf_values
     atm = np.ascontiguousarray(atmosphere, dtype=np.float64)
     if atm.ndim != 2 or atm.shape[0] < 9:
         raise ValueError("atmosphere must be [>=9, Ndep]")
+    skey, s = _setup(cwd, atomic_number, atomic_abundance, loggf_ids, loggf_values,
+                     lam_ids, lam_values)
+    if any(a.active for a in s.atoms):
+        if get_atomic_rfs:
+            raise NotImplementedError("get_atomic_rfs with an ACTIVE atom is not implemented")
+        e, nm = nlte_engine_for(cwd, wave, atomic_number, atomic_abundance, loggf_ids,
+                                loggf_values, lam_ids, lam_values)
+        pops, _, _, st, nst = e.nlte_solve(nm, atm[None, :9], mu, atm_scale, stokes=True,
+                                           nstar=True)
+        lam = e.model.lam[e.model.out_index].copy()
+        out = st[0, 0].copy(), st[0, 1].copy(), st[0, 2].copy(), st[0, 3].copy(), lam
+        if get_populations:
+            nd = atm.shape[1]
+            return out, (Populations(nm.ID, int(nm.nlevel), nd, pops[0].copy(), nst[0].copy()),)
+        return out
     e = engine_for(cwd, wave, atomic_number, atomic_abundance, loggf_ids, loggf_values,
                    lam_ids, lam_values)
     lam = e.model.lam[e.model.out_index].copy()
@@ -104,6 +170,8 @@ def compute1d(cwd, mu, atm_scale, atmosphere, wave, loggf_ids=None, loggf_values
             raise ValueError("get_atomic_rfs needs loggf_ids or lam_ids")
         st, rf = e.rf_loggf(atm[None, :9], par, mu, atm_scale)
         out = st[0, 0].copy(), st[0, 1].copy(), st[0, 2].copy(), st[0, 3].copy(), lam
-        return out, np.ascontiguousarray(rf[0].T)
+        out = out, np.ascontiguousarray(rf[0].T)
+        return (out, ()) if get_populations else out
     st = e.synth_lte(atm[None, :9], mu, atm_scale)
-    return st[0, 0].copy(), st[0, 1].copy(), st[0, 2].copy(), st[0, 3].copy(), lam
+    out = st[0, 0].copy(), st[0, 1].copy(), st[0, 2].copy(), st[0, 3].copy(), lam
+    return (out, ()) if get_populations else out

@@ -67,7 +67,7 @@ struct rhg_context {
   // of rhg::NlteBufs, active-column list, per-column reductions, output
   DevBuf nl_atom, nl_dev, nl_nstar, nl_pops, nl_ntot, nl_vbroad, nl_nHe, nl_adamp, nl_wphi,
       nl_Cmat, nl_Gam, nl_phi, nl_csca, nl_J, nl_jpar, nl_acc, nl_scr, nl_dJ, nl_hist, nl_act, nl_red,
-      nl_out, nl_iolist, nl_atm, nl_ndep;
+      nl_out, nl_iolist, nl_atm, nl_ndep, nl_st;
   double nlte_ms[RHG_NLTE_NTIMING]{};
   // host copies of model tables the NLTE checks need
   std::vector<int32_t> h_atom_level_ptr, h_nucl_atom;
@@ -866,8 +866,10 @@ int nlte_check(const rhg_context_t *c, const rhg_nlte_t *a, NlteHost &h) {
     }
   }
   h.nbez = (int) h.io_list.size();
+  // every other output wavelength, active or not: J is iterated there too,
+  // as the reference solves every wavelength in each pass
   for (int ls = 0; ls < ns; ls++)
-    if (a->as_ptr[ls + 1] > a->as_ptr[ls] && !a->as_bezier[ls])
+    if (ls != c->dm.ref_index && !(a->as_ptr[ls + 1] > a->as_ptr[ls] && a->as_bezier[ls]))
       h.io_list.push_back(ls < c->dm.ref_index ? ls : ls - 1);
   for (int t = 0; t < a->ntrans; t++)
     for (int ls = a->tr_nblue[t]; ls < a->tr_nblue[t] + a->tr_nlam[t]; ls++) {
@@ -885,7 +887,7 @@ int nlte_check(const rhg_context_t *c, const rhg_nlte_t *a, NlteHost &h) {
         a->coll_type[r] < 0 || a->coll_type[r] > 6)
       return fail(RHG_EINVAL, "bad collision entry");
   }
-  if (h.io_list.empty()) return fail(RHG_EINVAL, "no active wavelength");
+  if (h.nbez == 0 && a->as_ptr[ns] == 0) return fail(RHG_EINVAL, "no active wavelength");
   return 0;
 }
 
@@ -950,7 +952,7 @@ extern "C" {
 
 int rhg_nlte_solve(rhg_context_t *c, const rhg_nlte_t *a, const rhg_columns_t *cols,
                    int32_t device_ptrs, double *pops_out, int32_t *niter_out,
-                   double *dpops_out) {
+                   double *dpops_out, double *stokes_out, double *nstar_out) {
   if (!c || !a || !cols || !pops_out || !niter_out || !dpops_out)
     return fail(RHG_EINVAL, "null argument");
   const int ncol_all = cols->ncol, Nmax = cols->ndep_max, nout = c->nout, nl = a->nlevel;
@@ -979,7 +981,8 @@ int rhg_nlte_solve(rhg_context_t *c, const rhg_nlte_t *a, const rhg_columns_t *c
       (size_t) Nmax * sizeof(double) *
           ((size_t) nfield + 5 * (size_t) nout + 2 * (size_t) nl + 3 + 2 * (size_t) a->nline +
            2 * (size_t) nl * nl + (size_t) h.nphi * nrays * 2 + 2 * (size_t) h.nentry +
-           (size_t) rhg::NLTE_NSLOT * (nlist - h.nbez) + (size_t) NS * nl + (size_t) nout) +
+           (size_t) rhg::NLTE_NSLOT * (stokes_out ? nlist : nlist - h.nbez) + (size_t) NS * nl +
+           (size_t) nout) +
       (size_t) nlist * sizeof(double) + 9 * (size_t) Nmax * sizeof(double) + 64;
   size_t freeb = 0, totb = 0;
   HIPCHECK(hipMemGetInfo(&freeb, &totb));
@@ -1023,13 +1026,15 @@ int rhg_nlte_solve(rhg_context_t *c, const rhg_nlte_t *a, const rhg_columns_t *c
         c->nl_Gam.ensure(pts * nl * nl) ||
         c->nl_phi.ensure(pts * std::max(h.nphi, 1) * nrays * 2) || c->nl_csca.ensure(pts * nout) ||
         c->nl_J.ensure(pts * nout * 2) || c->nl_acc.ensure(pts * h.nentry * 2) ||
-        c->nl_scr.ensure((size_t) rhg::NLTE_NSLOT * Nmax * std::max(nlist - h.nbez, 1) * ncb_max *
-                         64 * sizeof(double)) ||
+        c->nl_scr.ensure((size_t) rhg::NLTE_NSLOT * Nmax *
+                         std::max(stokes_out ? nlist : nlist - h.nbez, 1) * ncb_max * 64 *
+                         sizeof(double)) ||
+        (stokes_out && !device_ptrs && c->nl_st.ensure((size_t) ncol * 4 * nout * sizeof(double))) ||
         c->nl_jpar.ensure(ncol * sizeof(int32_t)) ||
         c->nl_dJ.ensure((size_t) nlist * ncol * sizeof(double)) || c->nl_hist.ensure(pts * NS * nl) ||
         c->nl_act.ensure(ncol * sizeof(int32_t)) || c->nl_red.ensure(ncol * sizeof(double)) ||
         (!device_ptrs && (c->nl_atm.ensure(pts * 9) || c->nl_ndep.ensure(ncol * sizeof(int32_t)) ||
-                          c->nl_out.ensure(pts * nl))))
+                          c->nl_out.ensure(pts * nl * (nstar_out ? 2 : 1)))))
       return fail(RHG_ENOMEM, "device allocation for the NLTE batch failed");
     const double *d_atm = (const double *) cols->atmos + (size_t) c0 * 9 * Nmax;
     const int32_t *d_ndep = cols->ndep + c0;
@@ -1212,13 +1217,55 @@ int rhg_nlte_solve(rhg_context_t *c, const rhg_nlte_t *a, const rhg_columns_t *c
       if (!act.empty() && (rc = upload_act())) return rc;
     }
 
+    // the emergent spectrum: J with the converged populations (up to
+    // N_MAX_SCATTER passes, each column stopping at max dJ <= ITER_LIMIT,
+    // rh/rhf1d/pyrh_compute1dray.c:332-337), then the final ray
+    if (stokes_out) {
+      act.resize(ncol);
+      for (int i = 0; i < ncol; i++) act[i] = i;
+      if ((rc = upload_act())) return rc;
+      for (int it = 0; it < a->nmax_scatter && !act.empty(); it++) {
+        if ((rc = rt(1, 0))) return rc;
+        const int nact = (int) act.size();
+        hipLaunchKernelGGL(rhg::k_nlte_djmax, dim3((unsigned) ((nact + 63) / 64)), b64, 0, c->stream,
+                           B, d_act, nact, nlist, d_red);
+        HIPCHECK(hipGetLastError());
+        if ((rc = fetch_red())) return rc;
+        keep.clear();
+        for (int ic : act)
+          if (!(red[ic] <= a->iter_limit)) keep.push_back(ic);
+        act.swap(keep);
+        if (!act.empty() && (rc = upload_act())) return rc;
+      }
+      double *d_st = device_ptrs ? stokes_out + (size_t) c0 * 4 * nout : (double *) c->nl_st.p;
+      dim3 grd((unsigned) rhg::xcd_grid((ncol + 63) / 64, nlist));
+      hipLaunchKernelGGL(rhg::k_nlte_emergent, grd, b64, 0, c->stream, c->d_model, d_at, B,
+                         c->d_out_ls, d_iolist, nlist, cols->mu, d_st);
+      HIPCHECK(hipGetLastError());
+      if (!device_ptrs)
+        HIPCHECK(hipMemcpyAsync(stokes_out + (size_t) c0 * 4 * nout, d_st,
+                                (size_t) ncol * 4 * nout * sizeof(double), hipMemcpyDeviceToHost,
+                                c->stream));
+    }
+
     // outputs of the chunk
     double *d_out = device_ptrs ? pops_out + (size_t) c0 * nl * Nmax : (double *) c->nl_out.p;
+    double *d_nst = !nstar_out ? nullptr
+                    : device_ptrs ? nstar_out + (size_t) c0 * nl * Nmax
+                                  : (double *) c->nl_out.p + (size_t) ncol * nl * Nmax;
     {
       const int64_t n = (int64_t) ncol * nl * Nmax;
       hipLaunchKernelGGL(rhg::k_nlte_out, dim3((unsigned) ((n + 63) / 64)), b64, 0, c->stream, d_at, B,
-                         d_out);
+                         (const double *) B.pops, d_out);
       HIPCHECK(hipGetLastError());
+      if (d_nst) {
+        hipLaunchKernelGGL(rhg::k_nlte_out, dim3((unsigned) ((n + 63) / 64)), b64, 0, c->stream, d_at,
+                           B, (const double *) B.nstar, d_nst);
+        HIPCHECK(hipGetLastError());
+        if (!device_ptrs)
+          HIPCHECK(hipMemcpyAsync(nstar_out + (size_t) c0 * nl * Nmax, d_nst, pts * nl,
+                                  hipMemcpyDeviceToHost, c->stream));
+      }
     }
     if (device_ptrs) {
       HIPCHECK(hipMemcpyAsync(niter_out + c0, niter.data(), ncol * sizeof(int32_t), hipMemcpyHostToDevice,

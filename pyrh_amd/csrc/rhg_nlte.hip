@@ -808,6 +808,173 @@ k_nlte_feautrier(const DevModel *__restrict__ Mp, const DevNlte *__restrict__ Ap
 #undef PK
 }
 
+// ---------------------------------------------------------------------------
+// The emergent spectrum after convergence, pyrh's final ray
+// (rh/rhf1d/pyrh_solveray.c:82-107): one ray at the caller's mu with the
+// converged populations, profiles for that ray (Armstrong, as the set-up's),
+// background scattering from the last J, no update of J.  One (column,
+// output wavelength) per thread: the opacity and source function into the
+// scratch rows, then the outward Bezier ray (wavelengths with a line) or
+// the Feautrier solution at mu (rh/rhf1d/formal.c:44-346).  I goes to
+// stokes[(column * 4) * nout + io]; Q, U, V are zero (field-free).
+__global__ void __launch_bounds__(64)
+k_nlte_emergent(const DevModel *__restrict__ Mp, const DevNlte *__restrict__ Ap, NlteBufs B,
+                const int32_t *__restrict__ out_ls, const int32_t *__restrict__ io_list, int nlist,
+                double mu, double *__restrict__ stokes) {
+  constexpr int MS = NLTE_MAXSET;
+  const DevModel &M = *Mp;
+  const DevNlte &A = *Ap;
+  const int ncol = B.ncol, Nmax = B.Nmax, ncb = (ncol + 63) / 64;
+  int cb, il;
+  xcd_tile(blockIdx.x, nlist, cb, il);
+  if (cb >= ncb) return;
+  const int ic = cb * 64 + threadIdx.x;
+  if (ic >= ncol) return;
+  const int io = io_list[il], ls = out_ls[io], N = B.ndep[ic];
+  const int64_t fs = (int64_t) Nmax * ncol;
+  const int64_t nthr = (int64_t) nlist * ncb * 64;
+  double *scr = B.scr + (int64_t) il * ncb * 64 + ic;
+#define SCR(s, k) scr[((int64_t) (s) * Nmax + (k)) * nthr]
+#define STF(f, k) B.state[(int64_t) (f) * fs + (int64_t) (k) * ncol + ic]
+#define PK(k) ((int64_t) (k) * ncol + ic)
+  const int64_t wo = (int64_t) io * fs, jb = (int64_t) B.nout * fs;
+  const double *Jl = B.J + B.jpar[ic] * jb + wo;
+  const double *chic = B.cchi + wo, *etac = B.ceta + wo, *scac = B.csca + wo;
+  const double lam = M.lamconst[(int64_t) ls * RHG_LC_NCONST + RHG_LC_LAMBDA];
+  const double hc_k = HC / (KBOLTZMANN * NM_TO_M);
+  SetInfo<MS> si;
+  set_info<MS>(A, ls, lam, si);
+  const bool bez = si.n > 0 && A.as_bezier[ls];
+  for (int k = 0; k < N; k++) {
+    const double T = STF(RHG_CF_T, k), vb = B.vbroad[PK(k)], vz = STF(RHG_CF_VLOS, k);
+    double chi = 0.0, eta = 0.0;
+#pragma unroll
+    for (int n = 0; n < MS; n++) {
+      if (n < si.n) {
+        const double ni = B.pops[(int64_t) si.ti[n] * fs + PK(k)];
+        const double nj = B.pops[(int64_t) si.tj[n] * fs + PK(k)];
+        double V, gij;
+        if (si.line[n] >= 0) {
+          const int l = si.line[n];
+          const double lambda0 = A.ln_lambda0[l];
+          const double v = (lam - lambda0 - 0.0) * CLIGHT / (vb * lambda0);
+          const double vlos = mu * vz / vb;
+          const double ph = voigt_armstrong(B.adamp[(int64_t) l * fs + PK(k)], v + 1.0 * vlos) *
+                            1.0 / (SQRTPI * vb);
+          gij = si.gline[n];
+          V = si.cV[n] * ph;
+        } else {
+          gij = B.nstar[(int64_t) si.ti[n] * fs + PK(k)] / B.nstar[(int64_t) si.tj[n] * fs + PK(k)] *
+                exp(-hc_k / (lam * T));
+          V = si.alpha[n];
+        }
+        chi += V * (ni - gij * nj);
+        eta += si.tw[n] * gij * V * nj;
+      }
+    }
+    const double chi_t = chi + chic[PK(k)];
+    double S = eta + etac[PK(k)];
+    S += scac[PK(k)] * Jl[PK(k)];
+    SCR(NS_CHI, k) = chi_t;
+    SCR(NS_S, k) = S / chi_t;
+  }
+  const double B2_0 = planck_lam(STF(RHG_CF_T, N - 2), lam);
+  const double B2_1 = planck_lam(STF(RHG_CF_T, N - 1), lam);
+#define CHI(k) SCR(NS_CHI, k)
+#define SRC(k) SCR(NS_S, k)
+  double Iout;
+  if (bez) {
+    // outward ray from the thermalised bottom, rh/rhf1d/bezier_1D.c:306-541
+    const double zmu = 1.0 / mu;
+    auto dzb = [&](int k) { return STF(RHG_CF_DZ, k); };   // |z[k-1] - z[k]|
+    double dtau_uw = 0.5 * zmu * (CHI(N - 1) + CHI(N - 2)) * dzb(N - 1);
+    double I_upw = B2_1 - (B2_0 - B2_1) / dtau_uw;
+    int k = N - 2;
+    double dsup = dzb(k + 1) * zmu, dsdn = dzb(k) * zmu;
+    double dchi_up = (CHI(k) - CHI(k + 1)) / dsup;
+    double dchi_c = cent_div(dsup, dsdn, CHI(k + 1), CHI(k), CHI(k - 1));
+    double c1 = max0(CHI(k) - (dsup / 3.0) * dchi_c);
+    double c2 = max0(CHI(k + 1) + (dsup / 3.0) * dchi_up);
+    dtau_uw = dsup * (CHI(k) + CHI(k + 1) + c1 + c2) * 0.25;
+    double dS_up = (SRC(k) - SRC(k + 1)) / dtau_uw;
+    for (k = N - 2; k >= 1; k--) {
+      dsdn = dzb(k) * zmu;
+      double dchi_dn;
+      if (k > 1) {
+        const double dsdn2 = dzb(k - 1) * zmu;
+        dchi_dn = cent_div(dsdn, dsdn2, CHI(k), CHI(k - 1), CHI(k - 2));
+      } else {
+        dchi_dn = (CHI(k - 1) - CHI(k)) / dsdn;
+      }
+      c1 = max0(CHI(k) + (dsdn / 3.0) * dchi_c);
+      c2 = max0(CHI(k - 1) - (dsdn / 3.0) * dchi_dn);
+      const double dtau_dw = dsdn * (CHI(k) + CHI(k - 1) + c1 + c2) * 0.25;
+      const double dt03 = dtau_uw / 3.0;
+      double alpha, beta, gamma, theta, eps;
+      bezier3_coeffs(dtau_uw, alpha, beta, gamma, theta, eps);
+      const double dS_c = cent_div(dtau_uw, dtau_dw, SRC(k + 1), SRC(k), SRC(k - 1));
+      c1 = max0(SRC(k) - dt03 * dS_c);
+      c2 = max0(SRC(k + 1) + dt03 * dS_up);
+      I_upw = I_upw * eps + alpha * SRC(k) + beta * SRC(k + 1) + gamma * c1 + theta * c2;
+      dchi_c = dchi_dn;
+      dtau_uw = dtau_dw;
+      dS_up = dS_c;
+    }
+    {
+      const double dt = 0.5 * zmu * (CHI(0) + CHI(1)) * dzb(1);
+      const double dS_uw = -(SRC(0) - SRC(1)) / dt;
+      double w[3];
+      w3(dt, w);
+      Iout = (1.0 - w[0]) * I_upw + w[0] * SRC(0) + w[1] * dS_uw;
+    }
+    (void) dchi_up;
+  } else {
+    // Feautrier at mu, rh/rhf1d/feautrier.c:56-202; the emergent intensity
+    // is (1 + f0) P[0] - h0 / (1 + r0) with r0 = h0 = 0
+#define HGT(k) STF(RHG_CF_HEIGHT, k)
+    const double zmu = 0.5 / mu;
+    auto dtau = [&](int k) { return zmu * (CHI(k) + CHI(k + 1)) * (HGT(k) - HGT(k + 1)); };
+    const double dt0 = dtau(0);
+    const double abc0 = 1.0 + 2.0 * 1.0 / dt0;
+    const double C1_0 = 2.0 / (dt0 * dt0);
+    const double S0 = SRC(0) + 2.0 * 0.0 / ((1.0 + 0.0) * dt0);
+    double F = abc0 / C1_0, z = S0 / (abc0 + C1_0);
+    SCR(NS_F, 0) = F;
+    SCR(NS_Z, 0) = z;
+    double dtm1 = dt0;
+    for (int k = 1; k < N - 1; k++) {
+      const double dtk = dtau(k);
+      const double dtau_mid = 0.5 * (dtk + dtm1);
+      const double A1 = 1.0 / (dtau_mid * dtm1), C1 = 1.0 / (dtau_mid * dtk);
+      F = (1.0 + A1 * F / (1.0 + F)) / C1;
+      z = (SRC(k) + A1 * z) / (C1 * (1.0 + F));
+      SCR(NS_F, k) = F;
+      SCR(NS_Z, k) = z;
+      dtm1 = dtk;
+    }
+    const double dtN = dtm1;
+    const double hN = B2_1 - (B2_0 - B2_1) / dtN;
+    const double abcN = 1.0 + 2.0 * 1.0 / dtN;
+    const double A1N = 2.0 / (dtN * dtN);
+    const double SN = SRC(N - 1) + 2.0 * hN / ((1.0 + 0.0) * dtN);
+    const double FN2 = SCR(NS_F, N - 2);
+    double P = (SN + A1N * SCR(NS_Z, N - 2)) / (abcN + A1N * (FN2 / (1.0 + FN2)));
+    for (int k = N - 2; k >= 0; k--) P = P / (1.0 + SCR(NS_F, k)) + SCR(NS_Z, k);
+    Iout = (1.0 + 1.0) * P - 0.0 / (1.0 + 0.0);
+#undef HGT
+  }
+  double *o = stokes + (int64_t) ic * 4 * B.nout + io;
+  o[0] = Iout;
+  o[B.nout] = 0.0;
+  o[2 * (int64_t) B.nout] = 0.0;
+  o[3 * (int64_t) B.nout] = 0.0;
+#undef CHI
+#undef SRC
+#undef SCR
+#undef STF
+#undef PK
+}
+
 // the new J becomes the previous one for the columns just solved
 __global__ void k_nlte_flip(NlteBufs B, const int32_t *__restrict__ act, int nact) {
   const int slot = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1109,8 +1276,10 @@ __global__ void k_nlte_ng(const DevNlte *__restrict__ Ap, NlteBufs B,
   dpops[ic] = dmax;
 }
 
-// populations to the caller's layout [ncol][nl][Nmax], zero past ndep
-__global__ void k_nlte_out(const DevNlte *__restrict__ Ap, NlteBufs B, double *__restrict__ out) {
+// populations (src: current or LTE) to the caller's layout
+// [ncol][nl][Nmax], zero past ndep
+__global__ void k_nlte_out(const DevNlte *__restrict__ Ap, NlteBufs B, const double *__restrict__ src,
+                           double *__restrict__ out) {
   const DevNlte &A = *Ap;
   const int64_t p = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
   const int ncol = B.ncol, Nmax = B.Nmax, nl = A.nl;
@@ -1118,7 +1287,7 @@ __global__ void k_nlte_out(const DevNlte *__restrict__ Ap, NlteBufs B, double *_
   const int k = (int) (p % Nmax);
   const int64_t q = p / Nmax;
   const int i = (int) (q % nl), ic = (int) (q / nl);
-  out[p] = k < B.ndep[ic] ? B.pops[((int64_t) i * Nmax + k) * ncol + ic] : 0.0;
+  out[p] = k < B.ndep[ic] ? src[((int64_t) i * Nmax + k) * ncol + ic] : 0.0;
 }
 
 }  // namespace rhg

@@ -78,6 +78,9 @@ __global__ void k_nlte_feautrier(const DevModel *Mp, const DevNlte *Ap, NlteBufs
                                  const int32_t *act, int nact, const int32_t *out_ls,
                                  const int32_t *io_list, int nlist, int il_off, int eval_op);
 __global__ void k_nlte_flip(NlteBufs B, const int32_t *act, int nact);
+__global__ void k_nlte_emergent(const DevModel *Mp, const DevNlte *Ap, NlteBufs B,
+                                const int32_t *out_ls, const int32_t *io_list, int nlist,
+                                double mu, double *stokes);
 __global__ void k_nlte_djmax(NlteBufs B, const int32_t *act, int nact, int nlist,
                              double *djmax);
 template <int NLP>
@@ -85,6 +88,6 @@ __global__ void k_nlte_stateq(const DevNlte *Ap, NlteBufs B, const int32_t *act,
                               int isum);
 __global__ void k_nlte_ng(const DevNlte *Ap, NlteBufs B, const int32_t *act, int nact,
                           int niter, int norder, int ndelay, int nperiod, double *dpops);
-__global__ void k_nlte_out(const DevNlte *Ap, NlteBufs B, double *pops_out);
+__global__ void k_nlte_out(const DevNlte *Ap, NlteBufs B, const double *src, double *out);
 
 }  // namespace rhg

@@ -73,7 +73,7 @@ def lib():
         L.rhg_rf_loggf.argtypes = [C.c_void_p, C.POINTER(RhgColumns), C.c_int32,
                                    C.POINTER(C.c_int32), C.c_int32, PD, PD]
         L.rhg_nlte_solve.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(RhgColumns), C.c_int32,
-                                     PD, C.POINTER(C.c_int32), PD]
+                                     PD, C.POINTER(C.c_int32), PD, PD, PD]
         L.rhg_nlte_timings.argtypes = [C.c_void_p, PD]
         if L.rhg_model_sizeof() != C.sizeof(RhgModel) or \
                 L.rhg_columns_sizeof() != C.sizeof(RhgColumns):
@@ -183,11 +183,13 @@ class Engine:
                                 C.cast(C.c_void_p(d_out), C.POINTER(C.c_double)),
                                 C.c_void_p(d_rf)))
 
-    def nlte_solve(self, nm, atmos, mu=1.0, atm_scale=0, ndep=None):
+    def nlte_solve(self, nm, atmos, mu=1.0, atm_scale=0, ndep=None, stokes=False, nstar=False):
         """NLTE populations of the active atom of nm (pyrh_amd.nlte.build_nlte;
         the engine must hold nm.lte) for host atmospheres [ncol, 9,
         ndep_max]: populations [ncol, nlevel, ndep_max], iterations [ncol]
-        and the last maximum relative change [ncol]."""
+        and the last maximum relative change [ncol]; with stokes=True also
+        the emergent spectrum at mu afterwards [ncol, 4, nout], with
+        nstar=True also the LTE populations (appended in that order)."""
         from .abi import NlteStruct
         ns = NlteStruct(nm)
         cs = ColumnsStruct(atmos, ndep, atm_scale, mu)
@@ -195,14 +197,17 @@ class Engine:
         pops = np.zeros((ncol, nm.nlevel, nd))
         niter = np.zeros(ncol, np.int32)
         dpops = np.zeros(ncol)
+        st = np.zeros((ncol, 4, self.nout)) if stokes else None
+        nst = np.zeros_like(pops) if nstar else None
         _check(lib().rhg_nlte_solve(self._h, C.addressof(ns.struct), cs.ptr(), 0, _pd(pops),
-                                    niter.ctypes.data_as(C.POINTER(C.c_int32)), _pd(dpops)))
-        return pops, niter, dpops
+                                    niter.ctypes.data_as(C.POINTER(C.c_int32)), _pd(dpops),
+                                    _pd(st), _pd(nst)))
+        return (pops, niter, dpops) + ((st,) if stokes else ()) + ((nst,) if nstar else ())
 
     def nlte_solve_device(self, nm, d_atmos, d_ndep, ncol, ndep_max, d_pops, d_niter, d_dpops,
-                          mu=1.0, atm_scale=0):
+                          mu=1.0, atm_scale=0, d_stokes=None):
         """nlte_solve with device pointers (ints) for the atmospheres, depth
-        counts and the three outputs."""
+        counts and the outputs (d_stokes None: no emergent spectrum)."""
         from .abi import NlteStruct
         ns = NlteStruct(nm)
         s = RhgColumns()
@@ -213,7 +218,9 @@ class Engine:
         _check(lib().rhg_nlte_solve(self._h, C.addressof(ns.struct), C.byref(s), 1,
                                     C.cast(C.c_void_p(d_pops), C.POINTER(C.c_double)),
                                     C.cast(C.c_void_p(d_niter), C.POINTER(C.c_int32)),
-                                    C.cast(C.c_void_p(d_dpops), C.POINTER(C.c_double))))
+                                    C.cast(C.c_void_p(d_dpops), C.POINTER(C.c_double)),
+                                    None if d_stokes is None else
+                                    C.cast(C.c_void_p(d_stokes), C.POINTER(C.c_double)), None))
 
     def nlte_timings(self):
         """Milliseconds and work counts of the last NLTE solve (keys of

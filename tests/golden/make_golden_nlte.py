@@ -5,7 +5,8 @@ Runs pyrh's rhf1d() from the reference library compiled from its C sources
 (the Ca II-like five-level atom with continuum, active, accelerated lambda
 iteration to a maximum relative population change of 1e-3) and stores the
 inputs, the converged populations of the active atom, the iteration count
-and last change, the merged wavelength grid, and the iteration history.
+and last change, the merged wavelength grid, the iteration history and the
+emergent spectrum that rhf1d() returns afterwards at the case's mu.
 Needs /root/reference; run from the repository root:
 
     PYRH_PATH=tests/golden/rhdata python3 tests/golden/make_golden_nlte.py
@@ -30,25 +31,28 @@ WAVE = np.array([393.4776, 396.9591, 854.44])
 
 def cases():
     mp = W.stokes_map(6, seed=W.SEED_BASE + 3)
-    yield "map6", mp
+    yield "map6", mp, 1.0
     # the single workload column and a slow, strongly perturbed one
     c0 = W.single_column()[None]
     hot = mp[:1].copy()
     hot[:, 1] *= 1.08
-    yield "single", c0
-    yield "hot", hot
+    yield "single", c0, 1.0
+    yield "hot", hot, 1.0
+    # an inclined final ray
+    yield "mu07", mp[2:4].copy(), 0.7
 
 
 def main():
     out = {}
-    for name, atm in cases():
-        res = refrun.run_synth(RUN, atm, WAVE, mu=1.0, atm_scale=0, variant="o2",
+    for name, atm, mu in cases():
+        res = refrun.run_synth(RUN, atm, WAVE, mu=mu, atm_scale=0, variant="o2",
                                capture=True, timeout=3600)
         cap = res[-1]
         ncol = atm.shape[0]
         # records: pops_n with (column, atom, level) in (col, nspect, mu)
         nl = len([r for r in cap if r[0] == "pops_n" and r[1] == 0])
         pops = np.zeros((ncol, nl, atm.shape[2]))
+        nstar = np.zeros((ncol, nl, atm.shape[2]))
         niter = np.zeros(ncol, np.int32)
         dpops = np.zeros(ncol)
         hist = np.zeros((ncol, 100))
@@ -56,6 +60,8 @@ def main():
             for r in cap:
                 if r[0] == "pops_n" and r[1] == c and r[2] == 0:
                     pops[c, r[3]] = r[5]
+                if r[0] == "pops_nstar" and r[1] == c and r[2] == 0:
+                    nstar[c, r[3]] = r[5]
             its = [r[5] for r in cap if r[0] == "iter" and r[1] == c]
             niter[c] = len(its)
             dpops[c] = its[-1][1]
@@ -63,10 +69,14 @@ def main():
         lam = [r[5] for r in cap if r[0] == "lambda"][0]
         out[f"{name}__atmos"] = atm
         out[f"{name}__pops"] = pops
+        out[f"{name}__nstar"] = nstar
         out[f"{name}__niter"] = niter
         out[f"{name}__dpops"] = dpops
         out[f"{name}__history"] = hist
         out[f"{name}__lam"] = lam
+        out[f"{name}__mu"] = mu
+        out[f"{name}__stokes"] = np.stack(res[1:5], axis=1)     # [ncol, 4, nlw]
+        out[f"{name}__lamout"] = res[0]
         print(name, atm.shape, "iterations", niter.tolist())
     out["wave"] = WAVE
     np.savez_compressed(HERE / "nlte_fixtures.npz", **out)

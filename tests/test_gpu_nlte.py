@@ -6,7 +6,9 @@ further seeded columns, ragged depth counts included.
 Tolerances: populations within 1e-6 relative of the reference and of the
 oracle at the same convergence criterion (ITER_LIMIT 1e-3); iteration counts
 equal; the last maximum relative change, a difference of nearly equal
-populations, within 1e-5 relative."""
+populations, within 1e-5 relative; the emergent spectrum that rhf1d()
+returns afterwards within 1e-7 relative (it inherits the populations'
+differences)."""
 import numpy as np
 import pytest
 
@@ -15,9 +17,10 @@ from conftest import GOLDEN
 pytestmark = pytest.mark.gpu
 
 RUN_NLTE = GOLDEN / "run_nlte"
-CASES = ("map6", "single", "hot")
+CASES = ("map6", "single", "hot", "mu07")
 TOL_POPS = 1e-6
 TOL_DPOPS = 1e-5
+TOL_I = 1e-7
 
 
 @pytest.fixture(scope="module")
@@ -46,10 +49,15 @@ def _relerr(a, b):
 @pytest.mark.parametrize("name", CASES)
 def test_nlte_matches_reference(eng, nm, nlte_fix, name):
     atm = nlte_fix[name + "__atmos"]
-    pops, niter, dpops = eng.nlte_solve(nm, atm)
+    pops, niter, dpops, st = eng.nlte_solve(nm, atm, mu=float(nlte_fix[name + "__mu"]),
+                                            stokes=True)
     ref = nlte_fix[name + "__pops"]
     err = _relerr(pops, ref)
-    print(f"{name}: niter {niter.tolist()}  max rel err {err:.2e}", flush=True)
+    sref = nlte_fix[name + "__stokes"]
+    eI = _relerr(st[:, 0], sref[:, 0])
+    print(f"{name}: niter {niter.tolist()}  pops max rel err {err:.2e}  I {eI:.2e}", flush=True)
+    assert eI < TOL_I
+    assert np.all(st[:, 1:] == 0.0)
     np.testing.assert_array_equal(niter, nlte_fix[name + "__niter"])
     assert err < TOL_POPS
     np.testing.assert_allclose(dpops, nlte_fix[name + "__dpops"], rtol=TOL_DPOPS, atol=0)
@@ -68,13 +76,14 @@ def test_nlte_matches_oracle_on_map_columns(eng, nm):
     ndep[5], ndep[11], ndep[17] = nd - 1, nd - 7, nd // 2
     for c in range(24):
         atm[c, :, ndep[c]:] = 0.0
-    pops, niter, dpops = eng.nlte_solve(nm, atm, ndep=ndep)
-    cp, cn, cd = pyoracle.nlte_solve(nm, atm, ndep=ndep)
+    pops, niter, dpops, st = eng.nlte_solve(nm, atm, ndep=ndep, mu=0.85, stokes=True)
+    cp, cn, cd, cs = pyoracle.nlte_solve(nm, atm, ndep=ndep, mu=0.85, stokes=True)
     for c in range(24):
         n = ndep[c]
         assert np.all(pops[c, :, n:] == 0.0)
         e = _relerr(pops[c, :, :n], cp[c, :, :n])
         assert e < TOL_POPS, (c, n, e)
+    assert _relerr(st[:, 0], cs[:, 0]) < TOL_I
     print(f"oracle: niter {niter.tolist()}  max rel err "
           f"{max(_relerr(pops[c, :, :ndep[c]], cp[c, :, :ndep[c]]) for c in range(24)):.2e}",
           flush=True)
@@ -93,3 +102,25 @@ def test_nlte_batch_order_independent(eng, nm, nlte_fix):
     np.testing.assert_array_equal(n2, n1[perm])
     p3, n3, _ = eng.nlte_solve(nm, atm[2:3])
     np.testing.assert_array_equal(p3[0], p1[2])
+
+
+def test_compute1d_nlte_run(nlte_fix):
+    """pyrh.compute1d on the NLTE run directory: the emergent spectrum at mu
+    and, with get_populations, the active atom's n and nstar."""
+    from pyrh_amd import api
+    name = "mu07"
+    atm = nlte_fix[name + "__atmos"][1]
+    (I, Q, U, V, lam), pops = api.compute1d(str(RUN_NLTE), float(nlte_fix[name + "__mu"]), 0, atm,
+                                            nlte_fix["wave"], get_populations=True)
+    np.testing.assert_array_equal(lam, nlte_fix[name + "__lamout"])
+    assert _relerr(I, nlte_fix[name + "__stokes"][1, 0]) < TOL_I
+    assert np.all(Q == 0.0) and np.all(U == 0.0) and np.all(V == 0.0)
+    assert len(pops) == 1 and pops[0].nlevel == 6 and pops[0].nz == atm.shape[1]
+    assert _relerr(pops[0].n, nlte_fix[name + "__pops"][1]) < TOL_POPS
+    assert _relerr(pops[0].nstar, nlte_fix[name + "__nstar"][1]) < 1e-12
+    # an LTE run returns no active atoms
+    from conftest import RUNS
+    from pyrh_amd import workloads
+    out, p0 = api.compute1d(str(RUNS["lte"]), 1.0, 0, workloads.single_column(),
+                            workloads.wavelengths_630(), get_populations=True)
+    assert p0 == () and len(out) == 5

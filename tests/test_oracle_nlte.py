@@ -5,7 +5,9 @@ make_golden_nlte.py) come from rhf1d() compiled from the reference sources:
 converged populations of the active Ca II-like atom, iteration counts, the
 last maximum relative change and the merged wavelength grid, for eight
 columns.  The oracle reproduces them to round-off; the host model builds the
-same wavelength grid bit for bit.
+same wavelength grid bit for bit.  The emergent spectrum that rhf1d()
+returns after convergence (J updated with the converged populations, then
+one ray at the case's mu) is checked against the reference's output.
 """
 import numpy as np
 import pytest
@@ -27,7 +29,7 @@ def nm(nlte_fix):
     return nlte.build_nlte(model.open_run(str(RUN_NLTE)), nlte_fix["wave"])
 
 
-CASES = ("map6", "single", "hot")
+CASES = ("map6", "single", "hot", "mu07")
 
 
 def test_merged_grid_matches_reference(nm, nlte_fix):
@@ -38,7 +40,8 @@ def test_merged_grid_matches_reference(nm, nlte_fix):
 @pytest.mark.parametrize("name", CASES)
 def test_oracle_populations_match_reference(nm, nlte_fix, name):
     atm = nlte_fix[name + "__atmos"]
-    pops, niter, dpops = pyoracle.nlte_solve(nm, atm)
+    pops, niter, dpops, st = pyoracle.nlte_solve(nm, atm, mu=float(nlte_fix[name + "__mu"]),
+                                                 stokes=True)
     ref = nlte_fix[name + "__pops"]
     np.testing.assert_array_equal(niter, nlte_fix[name + "__niter"])
     # the last change is a difference of nearly equal populations
@@ -51,6 +54,12 @@ def test_oracle_populations_match_reference(nm, nlte_fix, name):
     ntot = pops.sum(axis=1)
     np.testing.assert_allclose(ntot / ntot[:, :1], ref.sum(axis=1) / ref.sum(axis=1)[:, :1],
                                rtol=1e-12)
+    # the emergent spectrum: to ~1e-11 where the populations are identical;
+    # the static column carries its ~1e-9 population difference into I
+    sref = nlte_fix[name + "__stokes"]
+    eI = np.max(np.abs(st[:, 0] - sref[:, 0]) / sref[:, 0])
+    assert eI < (1e-8 if name == "single" else 1e-10), eI
+    assert np.all(st[:, 1:] == 0.0) and np.all(sref[:, 1:] == 0.0)
 
 
 def test_gauss_legendre_closed_form():
