@@ -68,6 +68,9 @@ struct rhg_context {
   DevBuf nl_atom, nl_dev, nl_nstar, nl_pops, nl_ntot, nl_vbroad, nl_nHe, nl_adamp, nl_wphi,
       nl_Cmat, nl_Gam, nl_phi, nl_csca, nl_J, nl_jpar, nl_acc, nl_scr, nl_dJ, nl_hist, nl_act, nl_red,
       nl_out, nl_iolist, nl_atm, nl_ndep, nl_st;
+  // batches of uneven depth counts run sorted by depth count: the caller's
+  // columns gathered into these, the outputs scattered back
+  DevBuf srt_atm, srt_ndep, srt_perm, srt_out, srt_rf;
   double nlte_ms[RHG_NLTE_NTIMING]{};
   // host copies of model tables the NLTE checks need
   std::vector<int32_t> h_atom_level_ptr, h_nucl_atom;
@@ -481,9 +484,110 @@ static int synth(rhg_context_t *c, const rhg_columns_t *cols, int32_t precision,
   return 0;
 }
 
+}  // extern "C"
+
+namespace {
+
+// A batch whose depth counts differ runs with its columns sorted by depth
+// count, longest first: a wavefront holds 64 columns at one wavelength, and
+// each lane's depth recursion is as long as its column, so a wavefront of
+// mixed depths waits for its deepest column.  Returns the permutation
+// (empty when the batch is small or already ordered); perm[i] is the
+// caller's column at sorted position i.
+int sort_order(rhg_context_t *c, const rhg_columns_t *cols, int32_t device_ptrs,
+               std::vector<int32_t> &perm, std::vector<int32_t> &sorted_ndep) {
+  perm.clear();
+  const int ncol = cols->ncol;
+  if (ncol < 128 || !cols->ndep) return 0;
+  std::vector<int32_t> hndep(ncol);
+  if (device_ptrs) {
+    HIPCHECK(hipSetDevice(c->device));
+    HIPCHECK(hipMemcpy(hndep.data(), cols->ndep, ncol * sizeof(int32_t), hipMemcpyDeviceToHost));
+  } else {
+    memcpy(hndep.data(), cols->ndep, ncol * sizeof(int32_t));
+  }
+  bool ordered = true;
+  for (int i = 1; i < ncol; i++) ordered &= hndep[i] <= hndep[i - 1];
+  if (ordered) return 0;
+  perm.resize(ncol);
+  for (int i = 0; i < ncol; i++) perm[i] = i;
+  std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return hndep[a] > hndep[b]; });
+  sorted_ndep.resize(ncol);
+  for (int i = 0; i < ncol; i++) sorted_ndep[i] = hndep[perm[i]];
+  return 0;
+}
+
+// Gather the caller's atmospheres into the sorted order; fills cols2.
+int gather_sorted(rhg_context_t *c, const rhg_columns_t *cols, int32_t device_ptrs,
+                  const std::vector<int32_t> &perm, const std::vector<int32_t> &sorted_ndep,
+                  rhg_columns_t &cols2) {
+  const int ncol = cols->ncol, Nmax = cols->ndep_max;
+  const size_t atm_b = (size_t) ncol * 9 * Nmax * sizeof(double);
+  if (c->srt_atm.ensure(atm_b) || c->srt_ndep.ensure(ncol * sizeof(int32_t)) ||
+      c->srt_perm.ensure(ncol * sizeof(int32_t)) || (!device_ptrs && c->atmos.ensure(atm_b)))
+    return fail(RHG_ENOMEM, "device allocation for the sorted batch failed");
+  HIPCHECK(hipSetDevice(c->device));
+  const double *src = (const double *) cols->atmos;
+  if (!device_ptrs) {
+    HIPCHECK(hipMemcpyAsync(c->atmos.p, cols->atmos, atm_b, hipMemcpyHostToDevice, c->stream));
+    src = (const double *) c->atmos.p;
+  }
+  HIPCHECK(hipMemcpyAsync(c->srt_perm.p, perm.data(), ncol * sizeof(int32_t), hipMemcpyHostToDevice,
+                          c->stream));
+  HIPCHECK(hipMemcpyAsync(c->srt_ndep.p, sorted_ndep.data(), ncol * sizeof(int32_t),
+                          hipMemcpyHostToDevice, c->stream));
+  const int64_t n = (int64_t) ncol * 9 * Nmax;
+  hipLaunchKernelGGL(rhg::k_gather_cols, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, c->stream,
+                     src, (double *) c->srt_atm.p, (const int32_t *) c->srt_perm.p, ncol,
+                     (int64_t) 9 * Nmax);
+  HIPCHECK(hipGetLastError());
+  // the synthesis reads the sorted depth counts back with a blocking copy,
+  // which does not order with this (non-blocking) stream
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  cols2 = *cols;
+  cols2.atmos = (const double *) c->srt_atm.p;
+  cols2.ndep = (const int32_t *) c->srt_ndep.p;
+  return 0;
+}
+
+// Stokes [ncol][4][nout] from the sorted order back to the caller's.
+int scatter_stokes(rhg_context_t *c, int ncol, int32_t device_ptrs, const double *src,
+                   double *stokes_out) {
+  const int64_t len = (int64_t) 4 * c->nout;
+  const size_t b = (size_t) ncol * len * sizeof(double);
+  if (!device_ptrs && c->stokes.ensure(b)) return fail(RHG_ENOMEM, "device allocation failed");
+  double *dst = device_ptrs ? stokes_out : (double *) c->stokes.p;
+  const int64_t n = (int64_t) ncol * len;
+  hipLaunchKernelGGL(rhg::k_scatter_cols, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, c->stream,
+                     src, dst, (const int32_t *) c->srt_perm.p, ncol, len);
+  HIPCHECK(hipGetLastError());
+  if (!device_ptrs) HIPCHECK(hipMemcpyAsync(stokes_out, dst, b, hipMemcpyDeviceToHost, c->stream));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
 int rhg_synth_lte(rhg_context_t *ctx, const rhg_columns_t *cols, int32_t precision,
                   int32_t device_ptrs, double *stokes_out) {
-  return synth(ctx, cols, precision, device_ptrs, stokes_out, nullptr, nullptr);
+  if (!ctx || !cols || !stokes_out) return fail(RHG_EINVAL, "null argument");
+  std::vector<int32_t> perm, sndep;
+  if (cols->ncol > 0 && cols->ncol <= (1 << 28))
+    if (int rc = sort_order(ctx, cols, device_ptrs, perm, sndep)) return rc;
+  if (perm.empty()) return synth(ctx, cols, precision, device_ptrs, stokes_out, nullptr, nullptr);
+  for (int32_t n : sndep)
+    if (n < 4 || n > cols->ndep_max) return fail(RHG_EINVAL, "ndep out of range");
+  rhg_columns_t cols2;
+  if (int rc = gather_sorted(ctx, cols, device_ptrs, perm, sndep, cols2)) return rc;
+  if (ctx->srt_out.ensure((size_t) cols->ncol * 4 * ctx->nout * sizeof(double)))
+    return fail(RHG_ENOMEM, "device allocation for the sorted batch failed");
+  if (int rc = synth(ctx, &cols2, precision, 1, (double *) ctx->srt_out.p, nullptr, nullptr)) return rc;
+  if (int rc = scatter_stokes(ctx, cols->ncol, device_ptrs, (const double *) ctx->srt_out.p, stokes_out))
+    return rc;
+  HIPCHECK(hipStreamSynchronize(ctx->stream));
+  g_err.clear();
+  return 0;
 }
 
 int rhg_synth_lte_diag(rhg_context_t *ctx, const rhg_columns_t *cols, int32_t precision,
@@ -750,9 +854,40 @@ int rhg_rf_lte(rhg_context_t *c, const rhg_columns_t *cols, int32_t precision, i
   for (int i = 0; i < ncol; i++)
     if (hndep[i] < 4 || hndep[i] > Nmax) return fail(RHG_EINVAL, "ndep out of range");
   HIPCHECK(hipSetDevice(c->device));
-  const int rc = precision == RHG_PREC_FP32 ? rf_run<float>(c, cols, device_ptrs, stokes_out, rf_out)
-                                            : rf_run<double>(c, cols, device_ptrs, stokes_out, rf_out);
+  std::vector<int32_t> perm, sndep;
+  if (int rc = sort_order(c, cols, device_ptrs, perm, sndep)) return rc;
+  if (perm.empty()) {
+    const int rc = precision == RHG_PREC_FP32 ? rf_run<float>(c, cols, device_ptrs, stokes_out, rf_out)
+                                              : rf_run<double>(c, cols, device_ptrs, stokes_out, rf_out);
+    if (rc) return rc;
+    g_err.clear();
+    return 0;
+  }
+  // uneven depth counts: run sorted by depth count, scatter both outputs back
+  rhg_columns_t cols2;
+  if (int rc = gather_sorted(c, cols, device_ptrs, perm, sndep, cols2)) return rc;
+  const size_t esz = precision == RHG_PREC_FP32 ? sizeof(float) : sizeof(double);
+  const int64_t nouter = (int64_t) RHG_RF_NPARAM * Nmax * 4 * c->nout;
+  const size_t rf_b = (size_t) nouter * ncol * esz;
+  if (c->srt_out.ensure((size_t) ncol * 4 * c->nout * sizeof(double)) || c->srt_rf.ensure(rf_b) ||
+      (!device_ptrs && c->rf_out.ensure(rf_b)))
+    return fail(RHG_ENOMEM, "device allocation for the sorted batch failed");
+  int rc = precision == RHG_PREC_FP32
+               ? rf_run<float>(c, &cols2, 1, (double *) c->srt_out.p, c->srt_rf.p)
+               : rf_run<double>(c, &cols2, 1, (double *) c->srt_out.p, c->srt_rf.p);
   if (rc) return rc;
+  if ((rc = scatter_stokes(c, ncol, device_ptrs, (const double *) c->srt_out.p, stokes_out))) return rc;
+  void *rdst = device_ptrs ? rf_out : c->rf_out.p;
+  const dim3 g((unsigned) ((nouter * ncol + 255) / 256)), b(256);
+  if (precision == RHG_PREC_FP32)
+    hipLaunchKernelGGL(rhg::k_scatter_last<float>, g, b, 0, c->stream, (const float *) c->srt_rf.p,
+                       (float *) rdst, (const int32_t *) c->srt_perm.p, ncol, nouter);
+  else
+    hipLaunchKernelGGL(rhg::k_scatter_last<double>, g, b, 0, c->stream, (const double *) c->srt_rf.p,
+                       (double *) rdst, (const int32_t *) c->srt_perm.p, ncol, nouter);
+  HIPCHECK(hipGetLastError());
+  if (!device_ptrs) HIPCHECK(hipMemcpyAsync(rf_out, rdst, rf_b, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
   g_err.clear();
   return 0;
 }

@@ -119,6 +119,78 @@ __device__ __forceinline__ void delo_prologue(DeloState<T> &s, const Point<T> &P
   for (int n = 0; n < 3; n++) s.dqu[n] = (s.P0.q[n] - s.Pu.q[n]) * idt;
 }
 
+// The right-hand side V0 of a step and the solve Md Iv = V0 are formed in
+// Wide<T>: the step's own type in FP64, FP64 in the FP32 form.  Near the
+// core of a strong Zeeman line with an optically thick step, |q| -> 1 and
+// Md = I + c K0 + g (K0^2 + dK0) becomes nearly singular (eigenvalues
+// 1 -/+ c|q| + g|q|^2 with c ~ 2, g ~ 1).  The exact solution (I + K)^-1 S
+// for a constant source only needs eps = 1 - (alpha + beta + gamma + theta),
+// which float weights miss by ~1e-7; the near-singular solve amplifies that
+// and float rounding of V0 to ~1e-3 of I.  So the FP32 form widens its
+// (float) weights and takes eps from that identity, and builds V0 and solves
+// in FP64; the FP64 form keeps the reference's eps = exp(-dtau).
+template <typename T> struct Wide { using type = T; static constexpr bool widened = false; };
+template <> struct Wide<float> { using type = double; static constexpr bool widened = true; };
+template <> struct Wide<Dual<float>> { using type = Dual<double>; static constexpr bool widened = true; };
+
+__device__ __forceinline__ double widen(double x) { return x; }
+__device__ __forceinline__ double widen(float x) { return (double) x; }
+__device__ __forceinline__ Dual<double> widen(Dual<double> x) { return x; }
+__device__ __forceinline__ Dual<double> widen(Dual<float> x) {
+  return Dual<double>((double) x.v, (double) x.d);
+}
+__device__ __forceinline__ void narrow(double x, double &y) { y = x; }
+__device__ __forceinline__ void narrow(double x, float &y) { y = (float) x; }
+__device__ __forceinline__ void narrow(Dual<double> x, Dual<double> &y) { y = x; }
+__device__ __forceinline__ void narrow(Dual<double> x, Dual<float> &y) {
+  y = Dual<float>((float) x.v, (float) x.d);
+}
+
+template <typename T>
+__device__ __forceinline__ void delo_rhs_solve(DeloState<T> &s, T alpha_, T beta_, T gamma_,
+                                               T theta_, T eps_, T dt03_, const T *dS0_,
+                                               const T *dq0_) {
+  using C = typename Wide<T>::type;
+  const C alpha = widen(alpha_), beta = widen(beta_), gamma = widen(gamma_),
+          theta = widen(theta_), dt03 = widen(dt03_);
+  const C eps = Wide<T>::widened ? C(1) - (alpha + beta + gamma + theta) : widen(eps_);
+  C Iv[4], Su[4], S0[4], dS0[4], dSu[4], qu[3], q0[3], dqu[3], dq0[3];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    Iv[i] = widen(s.Iv[i]);
+    Su[i] = widen(s.Pu.S[i]);
+    S0[i] = widen(s.P0.S[i]);
+    dS0[i] = widen(dS0_[i]);
+    dSu[i] = widen(s.dSu[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    qu[i] = widen(s.Pu.q[i]);
+    q0[i] = widen(s.P0.q[i]);
+    dqu[i] = widen(s.dqu[i]);
+    dq0[i] = widen(dq0_[i]);
+  }
+  // V0 = Ma Iv + Mb Su + Mc S0 + dt03 (gamma dS0 - theta dSu) with
+  // Ma = eps I - beta Ku + theta (dt03 (Ku^2 + dKu + Ku) - Ku),
+  // Mb = beta I + theta (I - dt03 Ku), Mc = alpha I + gamma (I + dt03 K0)
+  C KI[4], KKI[4], dKI[4], KSu[4], KS0[4], V0[4], x[4];
+  kmul(qu, Iv, KI);
+  kmul(qu, KI, KKI);
+  kmul(dqu, Iv, dKI);
+  kmul(qu, Su, KSu);
+  kmul(q0, S0, KS0);
+  const C ca = theta * (dt03 - C(1)) - beta, gu = theta * dt03;
+  const C cb0 = beta + theta, cc0 = alpha + gamma, g0 = gamma * dt03;
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+    V0[i] = eps * Iv[i] + ca * KI[i] + gu * (KKI[i] + dKI[i]) + cb0 * Su[i] - gu * KSu[i] +
+            cc0 * S0[i] + g0 * KS0[i] + dt03 * (gamma * dS0[i] - theta * dSu[i]);
+  // Md = I + alpha K0 + gamma (dt03 (K0^2 + dK0 + K0) + K0)
+  solve_md<C>(alpha + gamma * (C(1) + dt03), g0, q0, dq0, V0, x);
+#pragma unroll
+  for (int i = 0; i < 4; i++) narrow(x[i], s.Iv[i]);
+}
+
 // One step at depth k: Iv(k) from Iv(k+1).  Pn is the point at k-2 (has_pn;
 // at k = 1 there is none and the downwind derivative of chi is one-sided).
 // dzk = |z[k-1] - z[k]|, dzk1 = |z[k-2] - z[k-1]|.
@@ -148,20 +220,7 @@ __device__ __forceinline__ void delo_step(DeloState<T> &s, const Point<T> &Pn, b
   // V0 = Ma Iv + Mb Su + Mc S0 + dt03 (gamma dS0 - theta dSu) with
   // Ma = eps I - beta Ku + theta (dt03 (Ku^2 + dKu + Ku) - Ku),
   // Mb = beta I + theta (I - dt03 Ku), Mc = alpha I + gamma (I + dt03 K0)
-  T KI[4], KKI[4], dKI[4], KSu[4], KS0[4], V0[4];
-  kmul(s.Pu.q, s.Iv, KI);
-  kmul(s.Pu.q, KI, KKI);
-  kmul(s.dqu, s.Iv, dKI);
-  kmul(s.Pu.q, s.Pu.S, KSu);
-  kmul(s.P0.q, s.P0.S, KS0);
-  const T ca = theta * (dt03 - T(1)) - beta, gu = theta * dt03;
-  const T cb0 = beta + theta, cc0 = alpha + gamma, g0 = gamma * dt03;
-#pragma unroll
-  for (int i = 0; i < 4; i++)
-    V0[i] = eps * s.Iv[i] + ca * KI[i] + gu * (KKI[i] + dKI[i]) + cb0 * s.Pu.S[i] - gu * KSu[i] +
-            cc0 * s.P0.S[i] + g0 * KS0[i] + dt03 * (gamma * dS0[i] - theta * s.dSu[i]);
-  // Md = I + alpha K0 + gamma (dt03 (K0^2 + dK0 + K0) + K0)
-  solve_md<T>(alpha + gamma * (T(1) + dt03), g0, s.P0.q, dq0, V0, s.Iv);
+  delo_rhs_solve(s, alpha, beta, gamma, theta, eps, dt03, dS0, dq0);
   s.Pu = s.P0;
   s.P0 = s.Pd;
   if (has_pn) s.Pd = Pn;

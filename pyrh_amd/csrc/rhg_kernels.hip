@@ -787,4 +787,35 @@ template __global__ void k_continuum<double>(const DevModel *, const int32_t *, 
 template __global__ void k_continuum<float>(const DevModel *, const int32_t *, int, int, const double *,
                                             int, const int32_t *, int, float *, float *);
 
+// ---------------------------------------------------------------------------
+// Column reordering.  perm[i] is the caller's column at sorted position i.
+__global__ void k_gather_cols(const double *__restrict__ src, double *__restrict__ dst,
+                              const int32_t *__restrict__ perm, int ncol, int64_t len) {
+  const int64_t p = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (int64_t) ncol * len) return;
+  const int64_t i = p / len, j = p % len;
+  dst[p] = src[(int64_t) perm[i] * len + j];
+}
+
+__global__ void k_scatter_cols(const double *__restrict__ src, double *__restrict__ dst,
+                               const int32_t *__restrict__ perm, int ncol, int64_t len) {
+  const int64_t p = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (int64_t) ncol * len) return;
+  const int64_t i = p / len, j = p % len;
+  dst[(int64_t) perm[i] * len + j] = src[p];
+}
+
+template <typename T>
+__global__ void k_scatter_last(const T *__restrict__ src, T *__restrict__ dst,
+                               const int32_t *__restrict__ perm, int ncol, int64_t nouter) {
+  const int64_t p = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (int64_t) ncol * nouter) return;
+  const int64_t o = p / ncol;
+  const int i = (int) (p % ncol);
+  dst[o * ncol + perm[i]] = src[p];
+}
+template __global__ void k_scatter_last<float>(const float *, float *, const int32_t *, int, int64_t);
+template __global__ void k_scatter_last<double>(const double *, double *, const int32_t *, int,
+                                                int64_t);
+
 }  // namespace rhg

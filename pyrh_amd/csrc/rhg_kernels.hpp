@@ -289,5 +289,15 @@ __global__ void k_census(const DevModel *M, const int32_t *ndep, const int32_t *
                          int Nmax, const double *state, int nfield, const int32_t *out_ls,
                          int nout, double mu, int32_t *census);
 __global__ void k_census_sum(const int32_t *census, int ncol, int nout, int64_t *col_sum);
+// column reordering for batches of uneven depth counts: records of len
+// values per column, gathered into (k_gather_cols) or scattered back from
+// (k_scatter_cols) the sorted order; k_scatter_last for arrays whose last
+// index is the column
+__global__ void k_gather_cols(const double *src, double *dst, const int32_t *perm, int ncol,
+                              int64_t len);
+__global__ void k_scatter_cols(const double *src, double *dst, const int32_t *perm, int ncol,
+                               int64_t len);
+template <typename T>
+__global__ void k_scatter_last(const T *src, T *dst, const int32_t *perm, int ncol, int64_t nouter);
 
 }  // namespace rhg
