@@ -16,10 +16,14 @@ form and in FP64 against central differences of the plain FP64 synthesis
 (2 x 6 x 82 perturbed syntheses per column) and reports the errors.  The
 NLTE leg (object nlte) solves the populations of a Ca II-like atom in 4096
 map columns to convergence and reports columns/s, the iteration histogram
-and the roofline of its formal-solution kernel.
+and the roofline of its formal-solution kernel.  The mixed leg (object
+mixed) synthesises 8192 columns of 64-256 depth points with 400 Zeeman
+lines on 2000 wavelengths at three mu, with adjoint response functions on
+a subset and the oracle on 128 columns.
 
     python bench.py [--steps K] [--warmup W] [--ncol N] [--no-cpu] [--no-rf]
                     [--no-nlte] [--nlte-ncol N] [--nlte-only]
+                    [--no-mixed] [--mixed-ncol N] [--mixed-only]
 """
 from __future__ import annotations
 
@@ -298,6 +302,119 @@ def _nlte_leg(torch, dev, ncol, do_cpu, nworkers):
     return res
 
 
+MIXED_RUN = ROOT / "tests" / "golden" / "run_mixed"
+MIXED_WORKLOAD = "mixed_zeeman400_8192"
+
+
+def _mixed_leg(torch, dev, ncol, do_cpu, nworkers, rf_cols=256, oracle_cols=128):
+    """The mixed-size batch: 8192 columns with 64-256 depth points (seeded),
+    400 synthetic Fe I Zeeman lines on 2000 wavelengths, FP32 Stokes at
+    mu = 1, 0.7, 0.4 with the batch resident in HBM (one rhg_synth_lte per
+    mu; the library runs the batch sorted by depth count); FP32 adjoint
+    response functions on the first rf_cols columns at mu = 1 (all six
+    parameters; the tape of 2000 wavelengths x 256 depths takes ~0.2 GB per
+    column); the FP64 oracle on oracle_cols columns at mu = 0.7; the
+    reference's rhf1d() per (column, mu) on a bounded sample.  The unit is
+    (column, mu) syntheses per second."""
+    from pyrh_amd import model as M, roofline as R, workloads as W
+    from pyrh_amd.engine import Engine
+    sys.path.insert(0, str(ROOT / "oracle"))
+    wave = W.mixed_wavelengths()
+    mdl = M.build_model(M.open_run(str(MIXED_RUN)), wave)
+    atm, nd = W.mixed_batch(ncol, seed=W.SEED_BASE + 8)
+    nmax = atm.shape[2]
+    eng = Engine(mdl)
+    nout = mdl.scalars["nspect"] - 1
+    d_atm = torch.from_numpy(atm).to(dev)
+    d_nd = torch.from_numpy(nd).to(dev)
+    d_out = torch.empty((len(W.MIXED_MU), ncol, 4, nout), dtype=torch.float64, device=dev)
+
+    def synth(prec):
+        for i, mu in enumerate(W.MIXED_MU):
+            eng.synth_lte_device(d_atm.data_ptr(), d_nd.data_ptr(), ncol, nmax, d_out[i].data_ptr(),
+                                 mu=mu, precision=prec)
+
+    synth(1)                       # warm-up (module load, allocations)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    synth(1)
+    torch.cuda.synchronize()
+    dt32 = time.perf_counter() - t0
+    formal_ms = eng.timings()["formal"]
+    fl = R.flops(eng.count_work(), mdl.scalars["nline"])
+    st32 = d_out[1, :oracle_cols].cpu().numpy()
+    t0 = time.perf_counter()
+    synth(0)
+    torch.cuda.synchronize()
+    dt64 = time.perf_counter() - t0
+    st64 = d_out[1, :oracle_cols].cpu().numpy()
+    import pyoracle
+    ref = pyoracle.synth_lte(mdl, atm[:oracle_cols], mu=W.MIXED_MU[1], ndep=nd[:oracle_cols],
+                             nthreads=nworkers)
+    Ic = ref[:, 0, :1]
+    achieved = fl["formal_kernel"] / (formal_ms * 1e-3) / 1e12
+    res = {
+        "workload": MIXED_WORKLOAD if ncol == 8192 else f"mixed_zeeman400_{ncol}",
+        "unit": "(column, mu) syntheses/s", "ncol": ncol, "mu": list(W.MIXED_MU),
+        "ndep_counts": {int(k): int(v) for k, v in zip(*np.unique(nd, return_counts=True))},
+        "nlambda": nout, "nline": int(mdl.scalars["nline"]),
+        "fp32": {"value": round(len(W.MIXED_MU) * ncol / dt32, 1), "wall_s": round(dt32, 3)},
+        "fp64": {"value": round(len(W.MIXED_MU) * ncol / dt64, 1), "wall_s": round(dt64, 3)},
+        "roofline_fp32_mu04": {"kernel": "k_formal_stokes<float>", "achieved": round(achieved, 3),
+                               "peak": R.PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                               "frac": round(achieved / R.PEAK_FP32_TFLOPS, 4),
+                               "flops_per_launch": fl["formal_kernel"],
+                               "launch_ms": round(formal_ms, 3)},
+        "oracle_check_mu07": {
+            "columns": oracle_cols,
+            "fp64_I_rel_err_max": float(f"{np.max(np.abs(st64[:, 0] - ref[:, 0]) / ref[:, 0]):.3g}"),
+            "fp64_QUV_err_over_Ic_max": float(f"{np.max(np.abs(st64[:, 1:] - ref[:, 1:]) / Ic[:, None]):.3g}"),
+            "fp32_I_err_over_Ic_max": float(f"{np.max(np.abs(st32[:, 0] - ref[:, 0]) / Ic):.3g}"),
+            "fp32_QUV_err_over_Ic_max": float(f"{np.max(np.abs(st32[:, 1:] - ref[:, 1:]) / Ic[:, None]):.3g}")},
+    }
+    del d_out
+    torch.cuda.empty_cache()
+    # FP32 adjoint response functions on a subset
+    nrf = min(rf_cols, ncol)
+    d_st = torch.empty((nrf, 4, nout), dtype=torch.float64, device=dev)
+    d_rf = torch.empty((6, nmax, 4, nout, nrf), dtype=torch.float32, device=dev)
+
+    def rf():
+        eng.rf_lte_device(d_atm.data_ptr(), d_nd.data_ptr(), nrf, nmax, d_st.data_ptr(),
+                          d_rf.data_ptr(), mu=1.0, precision=1)
+    rf()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rf()
+    torch.cuda.synchronize()
+    dtr = time.perf_counter() - t0
+    res["adjoint_rf_fp32"] = {"columns": nrf, "mu": 1.0, "params": 6, "ms": round(dtr * 1e3, 2),
+                              "columns_per_s": round(nrf / dtr, 1),
+                              "stage_ms": {k: round(float(v), 3) for k, v in eng.rf_timings().items()},
+                              "finite": bool(torch.isfinite(d_rf).all().item())}
+    del d_rf, d_st
+    eng.close()
+    torch.cuda.empty_cache()
+    res["cpu_baseline"] = None
+    if do_cpu:
+        import refrun
+        if refrun.ref_available("o1"):
+            from concurrent.futures import ThreadPoolExecutor
+            sample = [(c, mu) for c in range(nworkers) for mu in W.MIXED_MU[:1]]
+            t0 = time.perf_counter()
+            with ThreadPoolExecutor(nworkers) as ex:
+                list(ex.map(lambda cm: refrun.run_synth(MIXED_RUN, atm[cm[0]:cm[0] + 1, :, :nd[cm[0]]],
+                                                        wave, mu=cm[1], variant="o1"), sample))
+            dtc = time.perf_counter() - t0
+            res["cpu_baseline"] = {
+                "value": round(len(sample) / dtc, 3), "unit": "(column, mu) syntheses/s",
+                "cores": nworkers, "kind": "reference",
+                "sample": f"{len(sample)} columns of the batch (depths {sorted(set(nd[:nworkers].tolist()))}) "
+                          f"at mu = 1, rhf1d() per column with its input parsing, {nworkers} processes, "
+                          f"{dtc:.1f} s wall"}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -310,6 +427,10 @@ def main():
     ap.add_argument("--nlte-ncol", type=int, default=4096)
     ap.add_argument("--nlte-only", action="store_true",
                     help="run only the NLTE leg and print its object (for profiling)")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-size batch leg")
+    ap.add_argument("--mixed-ncol", type=int, default=8192)
+    ap.add_argument("--mixed-only", action="store_true",
+                    help="run only the mixed-size batch leg and print its object")
     ap.add_argument("--profile-step", action="store_true",
                     help="one FP32-form synthesis and one 1 GiB device copy, for rocprofv3 --pmc")
     a = ap.parse_args()
@@ -320,9 +441,11 @@ def main():
         sys.exit(2)
 
     nw = max(1, min(16, len(os.sched_getaffinity(0))))
-    if a.nlte_only:
+    if a.nlte_only or a.mixed_only:
         import torch
-        res = _nlte_leg(torch, torch.device("cuda:0"), a.nlte_ncol, not a.no_cpu, nw)
+        leg = _nlte_leg if a.nlte_only else _mixed_leg
+        res = leg(torch, torch.device("cuda:0"), a.nlte_ncol if a.nlte_only else a.mixed_ncol,
+                  not a.no_cpu, nw)
         print(json.dumps(res), flush=True)
         return
 
@@ -382,6 +505,7 @@ def main():
 
     rf = None if a.no_rf else _rf_leg(eng, W, torch, dev, reps=3)
     nlte = None if a.no_nlte else _nlte_leg(torch, dev, a.nlte_ncol, not a.no_cpu, nw)
+    mixed = None if a.no_mixed else _mixed_leg(torch, dev, a.mixed_ncol, not a.no_cpu, nw)
 
     # host-buffer rate (PCIe both ways), reported beside the resident rate
     t1 = time.perf_counter()
@@ -426,6 +550,7 @@ def main():
                                     "frac": round(ach64 / R.PEAK_FP64_TFLOPS, 4)}},
         "response_functions": rf,
         "nlte": nlte,
+        "mixed": mixed,
         "pipeline_flops": fl["total"],
         "host_buffers_columns_per_s": round(host_rate, 1),
     }
