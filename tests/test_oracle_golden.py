@@ -117,3 +117,24 @@ def test_overrides_change_the_spectrum(fixtures, setups):
     st0 = pyoracle.synth_lte(m0, F["overrides__atmos"])
     ref = F["overrides__stokes_o2"]
     assert np.max(np.abs(st0[:, 0] - ref[:, 0]) / ref[:, 0]) > 1e-3
+
+
+@pytest.mark.parametrize("name", ("mixed_mu1", "mixed_mu04"))
+def test_oracle_matches_reference_mixed(name):
+    """The mixed-size batch run (400 synthetic Fe I lines with Lande factors
+    read from the records, LS_LANDE = FALSE, one column of each depth count
+    64..256) against the reference's output (FP64 inverse)."""
+    import pyoracle
+    from conftest import GOLDEN
+    from pyrh_amd import model
+    F = np.load(GOLDEN / "mixed_fixtures.npz")
+    m = model.build_model(model.open_run(str(GOLDEN / "run_mixed")), F["wave"])
+    np.testing.assert_array_equal(m.lam[m.out_index], F[name + "__lam"])
+    st = pyoracle.synth_lte(m, F[name + "__atmos"], mu=float(F[name + "__mu"]),
+                            ndep=F[name + "__ndep"])
+    ref = F[name + "__stokes_o2"]
+    Ic = ref[:, 0, :1]
+    assert np.max(np.abs(st[:, 0] - ref[:, 0]) / ref[:, 0]) < 5e-12
+    assert np.max(np.abs(st[:, 1:] - ref[:, 1:]) / Ic[:, None]) < 5e-12
+    # the lines are Zeeman-split: the case exercises the polarized solver
+    assert np.max(np.abs(ref[:, 3]) / Ic) > 0.05
