@@ -237,6 +237,33 @@ typedef struct {
   double iter_limit;
 } rhg_nlte_t;
 
+/* Tables of the hydrostatic-equilibrium builder (pyrh's hse(),
+ * rh/rhf1d/pyrh_hse.c:67-399) beyond the model: every element that enters
+ * the electron-density Newton step of get_ne (rh/rhf1d/pyrh_background.c:
+ * 459-549, rh/solvene.c:145-211), hydrogen first, and the scattering
+ * cross-sections at the wavelength the opacity is evaluated at.  Built by
+ * pyrh_amd.hse.build_hse.  The model passed to rhg_open is the background
+ * model on the grid SortLambda makes of {LAMBDA_REF, 500 nm}, without
+ * Kurucz lines and with every atom passive. */
+typedef struct {
+  int32_t nel;              /* elements, hydrogen first                    */
+  const int32_t *el_nstage; /* [nel] ionisation stages                     */
+  const int32_t *el_pf_ptr; /* [nel] offset of the element's first stage   */
+  const double *el_logpf;   /* ln U per stage on the model's Tpf grid      */
+  const double *el_ionpot;  /* ionisation potential per stage [J], same
+                               offsets (stride npf per element)            */
+  const double *el_abund;   /* [nel] abundance relative to hydrogen        */
+  int32_t lam_index;        /* model wavelength of the opacity: the last of
+                               the grid (pyrh_Background overwrites its
+                               total opacity at every wavelength)          */
+  int32_t he_atom;          /* index of He in the model's atoms, -1: none  */
+  double ray_H, ray_He, ray_H2;  /* Rayleigh cross-sections there [m^2]    */
+  double sigma_thomson;
+  double hmin_fudge, scatt_fudge;  /* opacity fudge factors there (1: none) */
+  int32_t nmax_iter;        /* per-layer pressure iterations, 50           */
+  double iter_limit;        /* relative change of nHtot, 1e-2              */
+} rhg_hse_t;
+
 typedef struct rhg_context rhg_context_t;
 
 /* Returns the last error message on this thread ("" if none). */
@@ -343,6 +370,34 @@ int rhg_nlte_solve(rhg_context_t *ctx, const rhg_nlte_t *atom, const rhg_columns
  * wavelengths (Feautrier) in the scattering passes and the iterations. */
 #define RHG_NLTE_NTIMING 10
 int rhg_nlte_timings(rhg_context_t *ctx, double *out);
+
+/* Gas pressure in hydrostatic equilibrium for a batch of temperature
+ * columns: the work of one pyrh hse() call per column (pyrh.pyx:427-487,
+ * rh/rhf1d/pyrh_hse.c:67-399).  Layer by layer from the top: p_g from the
+ * layer above (on the tau500 scale the trapezoid in log tau of
+ * rho tau / chi_500, pyrh_hse.c:312-318; on the height scale the
+ * geometric mean of rho, :320-328), n_e from LTE ionisation of every
+ * element (get_ne), LTE populations and chemical equilibrium, the
+ * continuum opacity at 500 nm (pyrh_Background, without metal bound-free),
+ * and n_Htot = (p_g/kT - n_e)/totalAbund, repeated until n_Htot changes by
+ * at most iter_limit or nmax_iter times.  scale and temp are
+ * [ncol][ndep_max] (log10 tau500 for atm_scale 0, height in km for 2;
+ * column mass, 1, is rejected: the reference leaves its pressure step
+ * undefined); pg_top [ncol] the gas pressure at the top [Pa].  Outputs are
+ * [ncol][ndep_max] in SI units as pyrh returns them (n_e, n_Htot [m^-3],
+ * rho [kg m^-3], p_g [Pa]), zero past ndep; niter_out (NULL to skip) the
+ * pressure iterations per layer.  Host memory, or device memory for all
+ * arrays except `hse` if `device_ptrs`.  FP64.  Synchronous. */
+int rhg_hse(rhg_context_t *ctx, const rhg_hse_t *hse, int32_t ncol, int32_t ndep_max,
+            const int32_t *ndep, int32_t atm_scale, const double *scale,
+            const double *temp, const double *pg_top, int32_t device_ptrs,
+            double *ne_out, double *nHtot_out, double *rho_out, double *pg_out,
+            int32_t *niter_out);
+
+/* Duration of the hydrostatic-equilibrium kernel of the last rhg_hse call
+ * in ms (HIP events on the context's stream) and the lanes per column it
+ * ran with. */
+int rhg_hse_timing(rhg_context_t *ctx, double *ms_out, int32_t *lanes_out);
 
 /* Same as rhg_synth_lte but also returns, per column, the depth scale and
  * the background opacity at the reference wavelength (for testing):

@@ -228,3 +228,37 @@ def background(model, atmos, ls, mu=1.0, atm_scale=0, ic=0):
         raise RuntimeError("oracle background failed")
     return chi[:N], eta[:N], chi[4 * N:].reshape(4, N), eta[4 * N:].reshape(4, N), \
         pop.reshape(6, N)
+
+
+def hse(hm, scale, temp, pg_top=0.1, atm_scale=0, ndep=None, nthreads=0):
+    """pyrh's hse() on a batch of columns: scale, temp [ncol, ndep_max]
+    (a single column may be 1-D), pg_top scalar or [ncol].  Returns ne,
+    nHtot, rho, pg [ncol, ndep_max] (SI) and the pressure iterations per
+    layer."""
+    from pyrh_amd.abi import HseStruct
+    L = lib()
+    L.rho_hse.argtypes = [C.POINTER(RhgModel), C.c_void_p, C.c_int, C.c_int,
+                          C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_double),
+                          C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int,
+                          C.POINTER(C.c_double), C.POINTER(C.c_double),
+                          C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    L.rho_hse.restype = C.c_int
+    scale = np.ascontiguousarray(np.atleast_2d(scale), np.float64)
+    temp = np.ascontiguousarray(np.atleast_2d(temp), np.float64)
+    ncol, nd = temp.shape
+    if scale.shape != temp.shape:
+        scale = np.ascontiguousarray(np.broadcast_to(scale, temp.shape))
+    ndep = (np.full(ncol, nd, np.int32) if ndep is None
+            else np.ascontiguousarray(ndep, np.int32))
+    pgt = np.ascontiguousarray(np.broadcast_to(np.asarray(pg_top, np.float64), (ncol,)))
+    ms = ModelStruct(hm.model)
+    hs = HseStruct(hm)
+    out = [np.zeros((ncol, nd)) for _ in range(4)]
+    niter = np.zeros((ncol, nd), np.int32)
+    rc = L.rho_hse(ms.ptr(), C.addressof(hs.struct), ncol, nd,
+                   ndep.ctypes.data_as(C.POINTER(C.c_int)), int(atm_scale), _p(scale), _p(temp),
+                   _p(pgt), int(nthreads), *(_p(o) for o in out),
+                   niter.ctypes.data_as(C.POINTER(C.c_int)))
+    if rc != 0:
+        raise RuntimeError(f"oracle hse failed with code {rc}")
+    return (*out, niter)

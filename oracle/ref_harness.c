@@ -16,7 +16,8 @@
            rhref_run --hse <rundir> <in.bin> <out.bin>
 
    hse mode: in.bin as above (only rows 0 and 1 of each column are used,
-   row 0 = scale, row 1 = T; the wave list is ignored, mu holds pg_top),
+   row 0 = scale, row 1 = T; the wave list is ignored, mu holds pg_top;
+   overrides "abun id value" and "fudge lambda v0 v1 v2" are passed on),
    out.bin per column float64 ne[Ndep], nHtot[Ndep], rho[Ndep], pg[Ndep]
    from pyrh's hse() (rh/rhf1d/pyrh_hse.c:67).
 
@@ -348,6 +349,8 @@ typedef struct {
   int nloggf, nlam, nabun;
   int loggf_ids[64], lam_ids[64], abun_ids[64];
   double loggf_val[64], lam_val[64], abun_val[64];
+  int nfudge;                       /* hse(): opacity fudge table          */
+  double fudge_lam[64], fudge_row[64][3], fudge_val[3 * 64];
   FILE *rf_out;
 } Overrides;
 
@@ -366,6 +369,12 @@ static int read_overrides(const char *path, Overrides *ov) {
       if (!ov->rf_out) { perror("rf output"); fclose(f); return -1; }
       continue;
     }
+    if (strcmp(key, "fudge") == 0 && ov->nfudge < 64) {
+      double *r = ov->fudge_row[ov->nfudge];
+      if (sscanf(line, "%*s %lf %lf %lf %lf", &ov->fudge_lam[ov->nfudge], r, r + 1, r + 2) == 4)
+        ov->nfudge++;
+      continue;
+    }
     if (sscanf(line, "%*s %d %lf", &id, &v) != 2) continue;
     if (strcmp(key, "loggf") == 0 && ov->nloggf < 64) {
       ov->loggf_ids[ov->nloggf] = id; ov->loggf_val[ov->nloggf++] = v;
@@ -376,6 +385,8 @@ static int read_overrides(const char *path, Overrides *ov) {
     }
   }
   fclose(f);
+  for (int n = 0; n < 3; n++)
+    for (int k = 0; k < ov->nfudge; k++) ov->fudge_val[n * ov->nfudge + k] = ov->fudge_row[k][n];
   return 0;
 }
 
@@ -419,7 +430,9 @@ int main(int argc, char **argv)
       double *o = calloc(4 * (size_t) Ndep, sizeof(double));
       o[3 * Ndep] = mu;   /* pg[0] = pg_top on input */
       hse(".", Ndep, atm, atm + Ndep, o, o + Ndep, o + 2 * Ndep, o + 3 * Ndep,
-          atm_scale, 0, NULL, NULL, 0, NULL, NULL);
+          atm_scale, ov.nfudge, ov.nfudge ? ov.fudge_lam : NULL,
+          ov.nfudge ? ov.fudge_val : NULL, ov.nabun, ov.nabun ? ov.abun_ids : NULL,
+          ov.nabun ? ov.abun_val : NULL);
       fwrite(o, sizeof(double), 4 * (size_t) Ndep, fo);
       free(o);
       fflush(fo);

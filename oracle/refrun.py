@@ -104,19 +104,34 @@ def run_synth(rundir, atmos, wave, mu=1.0, atm_scale=0, variant="o2",
         return out
 
 
-def run_hse(rundir, logtau, T, pg_top=0.1, variant="o2", timeout=600):
-    """pyrh.hse() on the tau500 scale; returns ne, nHtot, rho, pg [m^-3 ...]
-    exactly as the C entry point leaves them (units of pyrh_hse.c)."""
+def run_hse(rundir, scale, T, pg_top=0.1, variant="o2", timeout=600, atm_scale=0,
+            abun=(), fudge=None):
+    """pyrh.hse() once per column of T [ncol, ndep] on `scale` (log10 tau500
+    for atm_scale 0, height [km] for 2; 1-D or per column); returns ne,
+    nHtot, rho, pg [ncol, 4, ndep] exactly as the C entry point leaves them
+    (SI units of pyrh_hse.c).  abun: (atomic number, abundance) pairs;
+    fudge: (fudge_wave [n], fudge_value [3][n])."""
     exe = REF_DIR / f"rhref_run_{variant}"
     env = dict(os.environ, PYRH_PATH=str(DATA_DIR))
     T = np.atleast_2d(np.asarray(T, np.float64))
     ncol, ndep = T.shape
     atmos = np.zeros((ncol, 9, ndep))
-    atmos[:, 0] = logtau
+    atmos[:, 0] = scale
     atmos[:, 1] = T
     with tempfile.TemporaryDirectory() as td:
         fin, fout = os.path.join(td, "in"), os.path.join(td, "out")
-        _write_input(fin, atmos, np.zeros(1), pg_top, 0)
+        if abun or fudge is not None:
+            fov = os.path.join(td, "overrides")
+            with open(fov, "w") as f:
+                for i, v in abun:
+                    f.write(f"abun {int(i)} {float(v)!r}\n")
+                if fudge is not None:
+                    fw, fv = fudge
+                    for k in range(len(fw)):
+                        f.write(f"fudge {float(fw[k])!r} {float(fv[0][k])!r} "
+                                f"{float(fv[1][k])!r} {float(fv[2][k])!r}\n")
+            env["RHREF_OVERRIDES"] = fov
+        _write_input(fin, atmos, np.zeros(1), pg_top, atm_scale)
         r = subprocess.run([str(exe), "--hse", str(rundir), fin, fout],
                            env=env, capture_output=True, timeout=timeout)
         if r.returncode != 0:

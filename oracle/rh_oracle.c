@@ -2470,3 +2470,294 @@ int rho_nlte_solve(const rhg_model_t *m, const rhg_nlte_t *na, const rhg_columns
 /* layout check for the host bindings */
 int rho_model_sizeof(void) { return (int) sizeof(rhg_model_t); }
 int rho_columns_sizeof(void) { return (int) sizeof(rhg_columns_t); }
+
+/* ------------------------------------------------------------------ */
+/* hydrostatic equilibrium (pyrh's hse())                              */
+
+#define HSE_MAX_ELECTRON_ERROR 1.0E-2     /* rh/rhf1d/pyrh_background.h:1-2 */
+#define HSE_N_MAX_ELECTRON_ITER 10
+
+/* getfjk, rh/solvene.c:145-197 (LTE branch: Kurucz partition functions,
+   getKuruczpf rh/solvene.c:202-211): ionisation fractions f_j and their
+   derivatives with ne for element e of the table */
+static void hse_getfjk(const rhg_model_t *m, const rhg_hse_t *h, int e, double T,
+                       double ne, double *fjk, double *dfjk)
+{
+  const double C1 = (HPLANCK / (2.0 * PI * M_ELECTRON)) * (HPLANCK / KBOLTZMANN);
+  const int ns = h->el_nstage[e], npf = m->npf;
+  const double *lp = h->el_logpf + h->el_pf_ptr[e], *ip = h->el_ionpot + h->el_pf_ptr[e];
+  double CT_ne = 2.0 * pow(C1 / T, -1.5) / ne, sum1 = 1.0, sum2 = 0.0, Uk, Ukp1;
+  int j;
+  fjk[0] = 1.0;
+  dfjk[0] = 0.0;
+  Uk = linear1(npf, m->Tpf, lp, T);
+  for (j = 1; j < ns; j++) {
+    Ukp1 = linear1(npf, m->Tpf, lp + j * npf, T);
+    fjk[j] = fjk[j - 1] * CT_ne * exp(Ukp1 - Uk - ip[j - 1] / (KBOLTZMANN * T));
+    dfjk[j] = -j * fjk[j] / ne;
+    sum1 += fjk[j];
+    sum2 += dfjk[j];
+    Uk = Ukp1;
+  }
+  for (j = 0; j < ns; j++) {
+    fjk[j] /= sum1;
+    dfjk[j] = (dfjk[j] - fjk[j] * sum2) / sum1;
+  }
+}
+
+/* get_ne(fromscratch = TRUE) with hydrogen in LTE,
+   rh/rhf1d/pyrh_background.c:459-549: starting value from hydrogen
+   ionisation with U = 0, Newton steps over all elements and H-. */
+static double hse_get_ne(const rhg_model_t *m, const rhg_hse_t *h, double T, double nHtot)
+{
+  const double C1 = (HPLANCK / (2.0 * PI * M_ELECTRON)) * (HPLANCK / KBOLTZMANN);
+  double fjk[NSTAGE_MAX], dfjk[NSTAGE_MAX], error, sum, akj, PhiH, PhiHmin, ne, ne_old, dne;
+  double Uk = 0.0;
+  int niter = 0, n, j;
+  PhiH = 0.5 * pow(C1 / T, 1.5) * exp(Uk + h->el_ionpot[h->el_pf_ptr[0]] / (KBOLTZMANN * T));
+  ne_old = (sqrt(1.0 + 4.0 * nHtot * PhiH) - 1.0) / (2.0 * PhiH);
+  ne = ne_old;
+  while (niter < HSE_N_MAX_ELECTRON_ITER) {
+    error = ne_old / nHtot;
+    sum = 0.0;
+    for (n = 0; n < h->nel; n++) {
+      hse_getfjk(m, h, n, T, ne_old, fjk, dfjk);
+      if (n == 0) {
+        PhiHmin = 0.25 * pow(C1 / T, 1.5) * exp(0.754 * EV / (KBOLTZMANN * T));
+        error += ne_old * fjk[0] * PhiHmin;
+        sum -= (fjk[0] + ne_old * dfjk[0]) * PhiHmin;
+      }
+      for (j = 1; j < h->el_nstage[n]; j++) {
+        akj = h->el_abund[n] * j;
+        error -= akj * fjk[j];
+        sum += akj * dfjk[j];
+      }
+    }
+    ne = ne_old - nHtot * error / (1.0 - nHtot * sum);
+    dne = fabs((ne - ne_old) / ne_old);
+    ne_old = ne;
+    if (dne <= HSE_MAX_ELECTRON_ERROR) break;
+    niter++;
+  }
+  return ne;
+}
+
+/* Continuum opacity at wavelength ls in one layer as pyrh_Background sums
+   it, rh/rhf1d/pyrh_background.c:256-400: H- bound-free and free-free
+   (times the H- fudge factor), OH and CH bound-free, H bound-free and
+   free-free, H2+ and H2- free-free; plus the scattering (Thomson, Rayleigh
+   on H, He and H2, times the scattering fudge factor).  No metal
+   bound-free (commented out there).  ns: all level populations. */
+static double hse_opacity(const rhg_model_t *m, const rhg_hse_t *h, int ls, double T, double ne,
+                          const double *ns, double nHmin, const double *nmol)
+{
+  const double *lc = m->lamconst + (size_t) ls * RHG_LC_NCONST;
+  const int lH = m->atom_level_ptr[0], nlH = m->atom_level_ptr[1] - lH;
+  const double nH0 = ns[lH], np = ns[lH + nlH - 1];
+  const double nH2 = m->mol_h2 >= 0 ? nmol[m->mol_h2] : 0.0;
+  const double nOH = m->mol_oh >= 0 ? nmol[m->mol_oh] : 0.0;
+  const double nCH = m->mol_ch >= 0 ? nmol[m->mol_ch] : 0.0;
+  double C0 = SQ(Q_ELECTRON) / (4.0 * PI * EPSILON_0) / sqrt(M_ELECTRON);
+  double sigma_ff = 4.0 / 3.0 * sqrt(2.0 * PI / (3.0 * KBOLTZMANN)) * CUBE(C0) /
+                    (HPLANCK * CLIGHT);
+  double total = 0.0, sca, chi, stim;
+  int e, cp0 = m->lam_cont_ptr[ls], cp1 = m->lam_cont_ptr[ls + 1];
+  /* Thomson, rh/thomson.c:30-48 */
+  sca = ne * h->sigma_thomson;
+  if (lc[RHG_LC_HMIN_BF_ON] != 0.0) {
+    stim = exp(-lc[RHG_LC_HC_KLA] / T);
+    total += nHmin * (1.0 - stim) * lc[RHG_LC_HMIN_BF_ALPHA];
+  }
+  if (lc[RHG_LC_HMIN_FF_LIDX] >= 0.0) {
+    double ti = clamp_index(NTHETA, m->hminff_theta, THETA0 / T);
+    double pe = ne * KBOLTZMANN * T;
+    double kappa = bilinear(NTHETA, NFF, m->hminff_kappa, ti, lc[RHG_LC_HMIN_FF_LIDX]);
+    chi = (nH0 * 1.0E-29) * pe * kappa;
+  } else {
+    double Ck = (KBOLTZMANN * THETA0 * 1.0E-32), theta_n = 1.0, sqrt_theta = sqrt(THETA0 / T);
+    int n;
+    chi = 0.0;
+    for (n = 1; n < 6; n++) {
+      theta_n *= sqrt_theta;
+      chi += theta_n * lc[RHG_LC_HFF_LONG0 + n - 1];
+    }
+    chi *= nH0 * (ne * Ck);
+  }
+  total += chi;
+  total *= h->hmin_fudge;
+  if (lc[RHG_LC_OH_EIDX] >= 0.0) {
+    if (T < m->oh_T[0] || T > m->oh_T[NTOH - 1]) chi = 0.0;
+    else {
+      int i2 = locate_asc(NTOH, m->oh_T, T);
+      double ti = (i2 == NTOH - 1) ? (double) i2 :
+        (double) i2 + (T - m->oh_T[i2]) / (m->oh_T[i2 + 1] - m->oh_T[i2]);
+      double kappa = exp(LG10 * bilinear(NTOH, NEOH, m->oh_cross, ti,
+                                         lc[RHG_LC_OH_EIDX])) * SQ(CM_TO_M);
+      stim = exp(-lc[RHG_LC_HC_KLA] / T);
+      chi = nOH * (1.0 - stim) * kappa;
+    }
+    total += chi;
+  }
+  if (lc[RHG_LC_CH_EIDX] >= 0.0) {
+    if (T < m->ch_T[0] || T > m->ch_T[NTCH - 1]) chi = 0.0;
+    else {
+      int i2 = locate_asc(NTCH, m->ch_T, T);
+      double ti = (i2 == NTCH - 1) ? (double) i2 :
+        (double) i2 + (T - m->ch_T[i2]) / (m->ch_T[i2 + 1] - m->ch_T[i2]);
+      double kappa = exp(LG10 * bilinear(NTCH, NECH, m->ch_cross, ti,
+                                         lc[RHG_LC_CH_EIDX])) * SQ(CM_TO_M);
+      stim = exp(-lc[RHG_LC_HC_KLA] / T);
+      chi = nCH * (1.0 - stim) * kappa;
+    }
+    total += chi;
+  }
+  /* H bound-free, rh/hydrogen.c:150-208 (active-layer branch) */
+  {
+    int any = 0;
+    chi = 0.0;
+    for (e = cp0; e < cp1; e++) {
+      int ci = m->lam_cont_idx[e];
+      if (m->cont_atom[ci] != 0) continue;
+      any = 1;
+      {
+        double expla = exp(-lc[RHG_LC_BF_HC_KLA] / T);
+        chi += m->lam_cont_alpha[e] * (1.0 - expla) * ns[lH + m->cont_i[ci]];
+      }
+    }
+    if (any) total += chi;
+  }
+  {
+    double y = (lc[RHG_LC_GFF_CY] * T) / (HPLANCK * CLIGHT), x3 = lc[RHG_LC_GFF_X3], g_ff;
+    double gIII = 1.0 + 0.1728 * x3 * (1.0 + y) -
+                  0.0496 * SQ(x3) * (1.0 + (1.0 + y) * 0.33333333 * y);
+    g_ff = (gIII > 1.0) ? gIII : 1.0;
+    stim = 1.0 - exp(-lc[RHG_LC_HC_KLA] / T);
+    total += sigma_ff / sqrt(T) * lc[RHG_LC_HFF_NU3] * ne * np * stim * g_ff;
+  }
+  /* Rayleigh on H and He, rh/rayleigh.c:35-100 */
+  if (h->ray_H > 0.0) sca += h->ray_H * nH0;
+  if (h->he_atom >= 0 && h->ray_He > 0.0) sca += h->ray_He * ns[m->atom_level_ptr[h->he_atom]];
+  if (lc[RHG_LC_H2P_LIDX] >= 0.0) {
+    double ti = clamp_index(NTEMP_H2P, m->h2pff_temp, T);
+    double kappa = bilinear(NTEMP_H2P, NFF_H2P, m->h2pff_kappa, ti, lc[RHG_LC_H2P_LIDX]);
+    total += (nH0 * 1.0E-29) * (np * 1.0E-20) * kappa;
+  }
+  /* Rayleigh on H2, rh/hydrogen.c:938-995 */
+  if (h->ray_H2 > 0.0) sca += h->ray_H2 * nH2;
+  if (lc[RHG_LC_H2M_LIDX] >= 0.0) {
+    if (nH2 > 0.0) {
+      double ti = clamp_index(NTHETA_H2, m->h2mff_theta, THETA0 / T);
+      double pe = ne * KBOLTZMANN * T;
+      double kappa = bilinear(NTHETA_H2, NFF_H2, m->h2mff_kappa, ti, lc[RHG_LC_H2M_LIDX]);
+      chi = (nH2 * 1.0E-29) * pe * kappa;
+    } else chi = 0.0;
+    total += chi;
+  }
+  sca *= h->scatt_fudge;
+  return total + sca;
+}
+
+/* One pass of the layer loop body, rh/rhf1d/pyrh_hse.c:226-238 and
+   :295-307: ne from get_ne, LTE populations of every model atom
+   (SetLTEQuantities, rh/ltepops.c:222-249), chemical equilibrium and the
+   opacity.  Scratch: ns [nlevel], ntot [natom], nmol [nmol]. */
+static void hse_layer(const rhg_model_t *m, const rhg_hse_t *h, double T, double nHtot,
+                      double *ns, double *ntot, double *nmol, double *ne_out, double *opac_out)
+{
+  double ne = hse_get_ne(m, h, T, nHtot), nHmin;
+  int ia;
+  for (ia = 0; ia < m->natom; ia++) {
+    ntot[ia] = m->atom_abund[ia] * nHtot;
+    ltepops(m, ia, T, ne, ntot[ia], ns + m->atom_level_ptr[ia]);
+  }
+  nHmin = chemeq(m, T, ne, nHtot, ns, ntot, nmol);
+  *ne_out = ne;
+  *opac_out = hse_opacity(m, h, h->lam_index, T, ne, ns, nHmin, nmol);
+}
+
+/* pyrh's hse() for one column, rh/rhf1d/pyrh_hse.c:210-351 */
+static int hse_column(const rhg_model_t *m, const rhg_hse_t *h, int N, int atm_scale,
+                      const double *scale, const double *T, double pg_top, double *ne,
+                      double *nHtot, double *rho, double *pg, int *niter)
+{
+  const double g = m->gravity, LOG10 = log(10.0);
+  double *ns = malloc(sizeof(double) * (m->atom_level_ptr[m->natom] + m->natom + m->nmol + 2 * N + 1));
+  double *ntot, *nmol, *opac, *z, nHtot_old, eta, deltaP = 0.0;
+  int k, iter;
+  if (!ns) return 1;
+  ntot = ns + m->atom_level_ptr[m->natom];
+  nmol = ntot + m->natom;
+  opac = nmol + m->nmol;
+  z = opac + N;
+  for (k = 0; k < N; k++) z[k] = (atm_scale == 0) ? POW10(scale[k]) : scale[k] * KM_TO_M;
+  pg[0] = pg_top;
+  ne[0] = 0.0;
+  nHtot[0] = (pg[0] / KBOLTZMANN / T[0] - ne[0]) / m->totalAbund;
+  iter = 0;
+  while (iter < h->nmax_iter) {
+    rho[0] = (AMU * m->wght_per_H) * nHtot[0];
+    hse_layer(m, h, T[0], nHtot[0], ns, ntot, nmol, ne, opac);
+    nHtot_old = nHtot[0];
+    nHtot[0] = (pg[0] / KBOLTZMANN / T[0] - ne[0]) / m->totalAbund;
+    iter++;
+    eta = fabs((nHtot[0] - nHtot_old) / nHtot[0]);
+    if (eta <= h->iter_limit) break;
+  }
+  if (niter) niter[0] = iter;
+  for (k = 1; k < N; k++) {
+    iter = 0;
+    if (atm_scale == 2) deltaP = g * rho[k - 1] * (z[k - 1] - z[k]);
+    else deltaP = g * rho[k - 1] / opac[k - 1] * (z[k] - z[k - 1]);
+    pg[k] = pg[k - 1] + deltaP;
+    nHtot[k] = (pg[k] / KBOLTZMANN / T[k] - ne[k - 1]) / m->totalAbund;
+    while (iter < h->nmax_iter) {
+      rho[k] = (AMU * m->wght_per_H) * nHtot[k];
+      hse_layer(m, h, T[k], nHtot[k], ns, ntot, nmol, ne + k, opac + k);
+      if (atm_scale == 0) {
+        double dlogtau = log10(z[k]) - log10(z[k - 1]);
+        double beta1 = rho[k - 1] / opac[k - 1] * z[k - 1];
+        double beta2 = rho[k] / opac[k] * z[k];
+        pg[k] = pg[k - 1] + LOG10 * g * (beta2 + beta1) / 2 * dlogtau;
+      } else {
+        deltaP = g * (z[k - 1] - z[k]) * sqrt(rho[k] * rho[k - 1]);
+        pg[k] = pg[k - 1] + deltaP;
+      }
+      nHtot_old = nHtot[k];
+      nHtot[k] = (pg[k] / KBOLTZMANN / T[k] - ne[k]) / m->totalAbund;
+      iter++;
+      eta = fabs((nHtot[k] - nHtot_old) / nHtot[k]);
+      if (eta <= h->iter_limit) break;
+    }
+    if (niter) niter[k] = iter;
+  }
+  free(ns);
+  return 0;
+}
+
+int rho_hse(const rhg_model_t *m, const rhg_hse_t *h, int ncol, int ndep_max,
+            const int *ndep, int atm_scale, const double *scale, const double *temp,
+            const double *pg_top, int nthreads, double *ne, double *nHtot, double *rho,
+            double *pg, int *niter)
+{
+  int ic, err = 0;
+  if (m->nnuclei + m->nmol > NEQ_MAX) return RHG_EUNSUPPORTED;
+  if (atm_scale != 0 && atm_scale != 2) return RHG_EUNSUPPORTED;
+  for (ic = 0; ic < h->nel; ic++) if (h->el_nstage[ic] > NSTAGE_MAX) return RHG_EUNSUPPORTED;
+  memset(ne, 0, sizeof(double) * ncol * ndep_max);
+  memset(nHtot, 0, sizeof(double) * ncol * ndep_max);
+  memset(rho, 0, sizeof(double) * ncol * ndep_max);
+  memset(pg, 0, sizeof(double) * ncol * ndep_max);
+  if (niter) memset(niter, 0, sizeof(int) * ncol * ndep_max);
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads) reduction(|:err)
+#endif
+  for (ic = 0; ic < ncol; ic++) {
+    size_t o = (size_t) ic * ndep_max;
+    if (hse_column(m, h, ndep[ic], atm_scale, scale + o, temp + o, pg_top[ic], ne + o,
+                   nHtot + o, rho + o, pg + o, niter ? niter + o : NULL))
+      err |= 1;
+  }
+  (void) nthreads;
+  return err ? RHG_ENOMEM : RHG_OK;
+}

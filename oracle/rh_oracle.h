@@ -43,6 +43,17 @@ int rho_nlte_solve(const rhg_model_t *m, const rhg_nlte_t *na, const rhg_columns
                    int nthreads, double *pops_out, int *niter_out, double *dpops_out,
                    double *stokes_out);
 
+/* pyrh's hse() for a batch of columns (rh/rhf1d/pyrh_hse.c:67-351): the
+ * model is the background model of pyrh_amd.hse.build_hse, h its element
+ * and scattering tables.  scale/temp [ncol][ndep_max], pg_top [ncol];
+ * outputs [ncol][ndep_max] in SI units, zero past ndep; niter (NULL to
+ * skip) the pressure iterations per layer.  atm_scale 0 (tau500) or 2
+ * (height). */
+int rho_hse(const rhg_model_t *m, const rhg_hse_t *h, int ncol, int ndep_max,
+            const int *ndep, int atm_scale, const double *scale, const double *temp,
+            const double *pg_top, int nthreads, double *ne, double *nHtot, double *rho,
+            double *pg, int *niter);
+
 /* Single-function entry points for closed-form and invariant tests. */
 double rho_voigt_humlicek(double a, double v, double *F);
 double rho_voigt_armstrong(double a, double v);

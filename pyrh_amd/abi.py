@@ -166,3 +166,42 @@ class NlteStruct:
 
     def ptr(self):
         return C.byref(self.struct)
+
+
+# rhg_hse_t, in declaration order
+HSE_FIELDS = [
+    ("nel", C.c_int32, "s"), ("el_nstage", P_I, "i"), ("el_pf_ptr", P_I, "i"),
+    ("el_logpf", P_D, "d"), ("el_ionpot", P_D, "d"), ("el_abund", P_D, "d"),
+    ("lam_index", C.c_int32, "s"), ("he_atom", C.c_int32, "s"),
+    ("ray_H", C.c_double, "s"), ("ray_He", C.c_double, "s"), ("ray_H2", C.c_double, "s"),
+    ("sigma_thomson", C.c_double, "s"),
+    ("hmin_fudge", C.c_double, "s"), ("scatt_fudge", C.c_double, "s"),
+    ("nmax_iter", C.c_int32, "s"), ("iter_limit", C.c_double, "s"),
+]
+
+
+class RhgHse(C.Structure):
+    _fields_ = [(n, t) for n, t, _ in HSE_FIELDS]
+
+
+class HseStruct:
+    """Owns contiguous copies of an HSEModel's tables and the rhg_hse_t
+    pointing at them."""
+
+    def __init__(self, hm):
+        self._keep = {}
+        s = RhgHse()
+        for name, ctype, kind in HSE_FIELDS:
+            if kind == "s":
+                setattr(s, name, ctype(getattr(hm, name)).value)
+            else:
+                dt = np.float64 if kind == "d" else np.int32
+                arr = np.ascontiguousarray(getattr(hm, name), dtype=dt).ravel()
+                if arr.size == 0:
+                    arr = np.zeros(1, dtype=dt)
+                self._keep[name] = arr
+                setattr(s, name, arr.ctypes.data_as(ctype))
+        self.struct = s
+
+    def ptr(self):
+        return C.byref(self.struct)

@@ -4,7 +4,8 @@
 (pyrh.pyx:537-668): LTE runs go through the LTE synthesis, runs with an
 ACTIVE atom through the NLTE solve (populations, then the emergent ray);
 ``synth_batch`` and ``nlte_batch`` are the batched forms that the GPU is
-built for.  Parsed inputs, models and device contexts are
+built for.  ``hse`` keeps the signature of pyrh.hse (pyrh.pyx:427-487),
+``hse_batch`` is its batched form.  Parsed inputs, models and device contexts are
 cached per run directory and wavelength grid, so repeated calls do not
 re-read files (the reference re-reads everything on every call,
 rh/rhf1d/pyrh_compute1dray.c:130-301).
@@ -16,6 +17,7 @@ import os
 
 import numpy as np
 
+from . import hse as _hse
 from . import model as _model
 from . import nlte as _nlte
 from .engine import Engine
@@ -23,6 +25,7 @@ from .engine import Engine
 _SETUPS = {}
 _ENGINES = {}
 _NLTE = {}
+_HSE = {}
 
 
 class Populations:
@@ -86,6 +89,51 @@ def nlte_engine_for(cwd, wave, atomic_number=None, atomic_abundance=None, loggf_
         r = (Engine(nm.lte, device), nm)
         _NLTE[key] = r
     return r
+
+
+def hse_engine_for(cwd, fudge_wave=None, fudge_value=None, atomic_number=None,
+                   atomic_abundance=None, device=0):
+    """(engine, HSE model) of pyrh's hse() for a run directory."""
+    skey, s = _setup(cwd, atomic_number, atomic_abundance, None, None, None, None)
+    key = (skey, _key(fudge_wave, fudge_value), device)
+    r = _HSE.get(key)
+    if r is None:
+        hm = _hse.build_hse(s, fudge_wave, fudge_value)
+        r = (Engine(hm.model, device), hm)
+        _HSE[key] = r
+    return r
+
+
+def hse(cwd, atm_scale, scale, temp, pg_top=0.1, fudge_wave=None, fudge_value=None,
+        atomic_number=None, atomic_abundance=None, full_output=False):
+    """pyrh.hse (pyrh.pyx:427-487): gas pressure in hydrostatic equilibrium
+    for one temperature column on `scale` (log10 tau500 for atm_scale 0,
+    height [km] for 2), starting from pg_top [Pa] at the top.  Returns
+    (ne, nHtot) in m^-3, or (ne, nHtot, rho, pg) in SI units with
+    full_output."""
+    scale = np.ascontiguousarray(scale, np.float64)
+    temp = np.ascontiguousarray(temp, np.float64)
+    if scale.ndim != 1 or temp.shape != scale.shape:
+        raise ValueError("scale and temp must be 1-D arrays of the same length")
+    _hse.check_scale(atm_scale)
+    e, hm = hse_engine_for(cwd, fudge_wave, fudge_value, atomic_number, atomic_abundance)
+    ne, nH, rho, pg, _ = e.hse(hm, scale[None], temp[None], pg_top, atm_scale)
+    if full_output:
+        return ne[0], nH[0], rho[0], pg[0]
+    return ne[0], nH[0]
+
+
+def hse_batch(cwd, atm_scale, scale, temp, pg_top=0.1, ndep=None, fudge_wave=None,
+              fudge_value=None, atomic_number=None, atomic_abundance=None, device=0):
+    """hse for a batch of columns: temp [ncol, ndep_max], scale the same
+    shape or one row for all, pg_top scalar or [ncol], ndep [ncol] the depth
+    points each column uses.  A dict with ne, nHtot, rho, pg [ncol,
+    ndep_max] (SI units, zero past ndep) and the pressure iterations per
+    layer."""
+    _hse.check_scale(atm_scale)
+    e, hm = hse_engine_for(cwd, fudge_wave, fudge_value, atomic_number, atomic_abundance, device)
+    ne, nH, rho, pg, it = e.hse(hm, scale, temp, pg_top, atm_scale, ndep)
+    return {"ne": ne, "nHtot": nH, "rho": rho, "pg": pg, "niter": it}
 
 
 def has_active_atom(cwd):

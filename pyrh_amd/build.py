@@ -10,7 +10,8 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libpyrh_amd.so"
-SOURCES = ["rhg_api.hip", "rhg_kernels.hip", "rhg_formal.hip", "rhg_rf.hip", "rhg_nlte.hip"]
+SOURCES = ["rhg_api.hip", "rhg_kernels.hip", "rhg_formal.hip", "rhg_rf.hip", "rhg_nlte.hip",
+           "rhg_hse.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",

@@ -3,11 +3,13 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstdlib>
 #include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
 
+#include "rhg_hse.hpp"
 #include "rhg_nlte.hpp"
 
 namespace {
@@ -72,6 +74,10 @@ struct rhg_context {
   // columns gathered into these, the outputs scattered back
   DevBuf srt_atm, srt_ndep, srt_perm, srt_out, srt_rf;
   double nlte_ms[RHG_NLTE_NTIMING]{};
+  // hydrostatic equilibrium: element tables, inputs, outputs, timing
+  DevBuf hs_nstage, hs_ptr, hs_logpf, hs_ionpot, hs_abund, hs_in, hs_ndep, hs_out, hs_niter;
+  float hse_ms = 0.0f;
+  int hse_lpc = 0;
   // host copies of model tables the NLTE checks need
   std::vector<int32_t> h_atom_level_ptr, h_nucl_atom;
   hipEvent_t ev[RHG_NSTAGE + 2]{};
@@ -1420,6 +1426,145 @@ int rhg_nlte_solve(rhg_context_t *c, const rhg_nlte_t *a, const rhg_columns_t *c
   HIPCHECK(hipEventElapsedTime(&ms, tm_all.e[0], tm_all.e[1]));
   c->nlte_ms[5] = ms;
   g_err.clear();
+  return 0;
+}
+
+int rhg_hse(rhg_context_t *c, const rhg_hse_t *h, int32_t ncol, int32_t Nmax, const int32_t *ndep,
+            int32_t atm_scale, const double *scale, const double *temp, const double *pg_top,
+            int32_t device_ptrs, double *ne_out, double *nHtot_out, double *rho_out,
+            double *pg_out, int32_t *niter_out) {
+  if (!c || !h || !ndep || !scale || !temp || !pg_top || !ne_out || !nHtot_out || !rho_out ||
+      !pg_out)
+    return fail(RHG_EINVAL, "null argument");
+  if (ncol < 1 || Nmax < 1 || Nmax > RHG_MAX_DEPTH) return fail(RHG_EINVAL, "bad batch shape");
+  if ((int64_t) ncol * Nmax > (int64_t) INT32_MAX / 16) return fail(RHG_EINVAL, "batch too large");
+  if (atm_scale == 1)
+    return fail(RHG_EUNSUPPORTED, "hse on the column-mass scale: the reference leaves the "
+                                  "pressure step undefined there (pyrh_hse.c:282-290)");
+  if (atm_scale != 0 && atm_scale != 2) return fail(RHG_EINVAL, "bad atm_scale");
+  const rhg::DevModel &dm = c->dm;
+  if (dm.nnuclei < 1 || dm.nnuclei > 8) return fail(RHG_EUNSUPPORTED, "number of molecule nuclei out of range");
+  if (dm.natom < 1 || c->h_nucl_atom.empty() || c->h_nucl_atom[0] != 0)
+    return fail(RHG_EINVAL, "hydrogen must be the first atom and the first nucleus");
+  if (h->nel < 1 || !h->el_nstage || !h->el_pf_ptr || !h->el_logpf || !h->el_ionpot || !h->el_abund)
+    return fail(RHG_EINVAL, "empty element table");
+  if (h->lam_index < 0 || h->lam_index >= c->nspect) return fail(RHG_EINVAL, "lam_index out of range");
+  if (h->he_atom >= dm.natom) return fail(RHG_EINVAL, "he_atom out of range");
+  if (h->nmax_iter < 1) return fail(RHG_EINVAL, "nmax_iter must be positive");
+  // the element tables are packed: element e's stages start where e-1's end
+  int64_t npfel = 0;
+  for (int e = 0; e < h->nel; e++) {
+    if (h->el_nstage[e] < 1 || h->el_nstage[e] > rhg::NSTAGE_MAX)
+      return fail(RHG_EUNSUPPORTED, "element with too many ionisation stages");
+    if (h->el_pf_ptr[e] != npfel) return fail(RHG_EINVAL, "element tables are not packed");
+    npfel += (int64_t) h->el_nstage[e] * dm.npf;
+  }
+  std::vector<int32_t> hndep(ncol);
+  if (device_ptrs) {
+    HIPCHECK(hipMemcpy(hndep.data(), ndep, ncol * sizeof(int32_t), hipMemcpyDeviceToHost));
+  } else {
+    memcpy(hndep.data(), ndep, ncol * sizeof(int32_t));
+  }
+  for (int i = 0; i < ncol; i++)
+    if (hndep[i] < 1 || hndep[i] > Nmax) return fail(RHG_EINVAL, "ndep out of range");
+  HIPCHECK(hipSetDevice(c->device));
+
+  const size_t col_b = (size_t) ncol * Nmax * sizeof(double);
+  if (c->hs_nstage.ensure(h->nel * sizeof(int32_t)) || c->hs_ptr.ensure(h->nel * sizeof(int32_t)) ||
+      c->hs_logpf.ensure(npfel * sizeof(double)) || c->hs_ionpot.ensure(npfel * sizeof(double)) ||
+      c->hs_abund.ensure(h->nel * sizeof(double)) ||
+      (!device_ptrs && (c->hs_in.ensure(2 * col_b + ncol * sizeof(double)) ||
+                        c->hs_ndep.ensure(ncol * sizeof(int32_t)) || c->hs_out.ensure(4 * col_b) ||
+                        (niter_out && c->hs_niter.ensure((size_t) ncol * Nmax * sizeof(int32_t))))))
+    return fail(RHG_ENOMEM, "device allocation for the batch failed");
+  HIPCHECK(hipMemcpyAsync(c->hs_nstage.p, h->el_nstage, h->nel * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(c->hs_ptr.p, h->el_pf_ptr, h->nel * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(c->hs_logpf.p, h->el_logpf, npfel * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(c->hs_ionpot.p, h->el_ionpot, npfel * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(c->hs_abund.p, h->el_abund, h->nel * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  rhg::DevHse H;
+  H.nel = h->nel;
+  H.el_nstage = (const int32_t *) c->hs_nstage.p;
+  H.el_pf_ptr = (const int32_t *) c->hs_ptr.p;
+  H.el_logpf = (const double *) c->hs_logpf.p;
+  H.el_ionpot = (const double *) c->hs_ionpot.p;
+  H.el_abund = (const double *) c->hs_abund.p;
+  H.lam_index = h->lam_index;
+  H.he_atom = h->he_atom;
+  H.ray_H = h->ray_H;
+  H.ray_He = h->ray_He;
+  H.ray_H2 = h->ray_H2;
+  H.sigma_thomson = h->sigma_thomson;
+  H.hmin_fudge = h->hmin_fudge;
+  H.scatt_fudge = h->scatt_fudge;
+  H.nmax_iter = h->nmax_iter;
+  H.iter_limit = h->iter_limit;
+
+  const double *d_scale = scale, *d_temp = temp, *d_pgt = pg_top;
+  const int32_t *d_ndep = ndep;
+  double *d_ne = ne_out, *d_nH = nHtot_out, *d_rho = rho_out, *d_pg = pg_out;
+  int32_t *d_niter = niter_out;
+  if (!device_ptrs) {
+    double *in = (double *) c->hs_in.p;
+    HIPCHECK(hipMemcpyAsync(in, scale, col_b, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(in + (size_t) ncol * Nmax, temp, col_b, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(in + (size_t) 2 * ncol * Nmax, pg_top, ncol * sizeof(double),
+                            hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(c->hs_ndep.p, ndep, ncol * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    d_scale = in;
+    d_temp = in + (size_t) ncol * Nmax;
+    d_pgt = in + (size_t) 2 * ncol * Nmax;
+    d_ndep = (const int32_t *) c->hs_ndep.p;
+    double *out = (double *) c->hs_out.p;
+    d_ne = out;
+    d_nH = out + (size_t) ncol * Nmax;
+    d_rho = out + (size_t) 2 * ncol * Nmax;
+    d_pg = out + (size_t) 3 * ncol * Nmax;
+    d_niter = niter_out ? (int32_t *) c->hs_niter.p : nullptr;
+  }
+  // lanes per column: enough lane groups to give every SIMD of the chip a
+  // few waves (256 CUs x 4 SIMDs x 64 lanes x 2), at most 16
+  int lpc = 1;
+  while (lpc < 16 && (int64_t) ncol * lpc < 131072) lpc <<= 1;
+  if (const char *lanes_env = getenv("RHG_HSE_LANES")) {
+    const int v = atoi(lanes_env);
+    if (v == 1 || v == 2 || v == 4 || v == 8 || v == 16) lpc = v;
+  }
+  c->hse_lpc = lpc;
+  const int64_t nthr = (int64_t) ncol * lpc;
+  dim3 blk(rhg::HSE_BLOCK), grd((unsigned) ((nthr + rhg::HSE_BLOCK - 1) / rhg::HSE_BLOCK));
+  const size_t lds = (size_t) 3 * dm.nmol * rhg::HSE_BLOCK * sizeof(double);
+  HIPCHECK(hipEventRecord(c->ev[0], c->stream));
+  switch (dm.nnuclei) {
+#define RHG_LAUNCH_HSE(NN)                                                                          \
+  case NN:                                                                                          \
+    hipLaunchKernelGGL(rhg::k_hse<NN>, grd, blk, lds, c->stream, c->d_model, H, lpc, ncol, Nmax,    \
+                       d_ndep, atm_scale, d_scale, d_temp, d_pgt, d_ne, d_nH, d_rho, d_pg, d_niter); \
+    break;
+    RHG_LAUNCH_HSE(1) RHG_LAUNCH_HSE(2) RHG_LAUNCH_HSE(3) RHG_LAUNCH_HSE(4)
+    RHG_LAUNCH_HSE(5) RHG_LAUNCH_HSE(6) RHG_LAUNCH_HSE(7) RHG_LAUNCH_HSE(8)
+#undef RHG_LAUNCH_HSE
+  }
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipEventRecord(c->ev[1], c->stream));
+  if (!device_ptrs) {
+    HIPCHECK(hipMemcpyAsync(ne_out, d_ne, col_b, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipMemcpyAsync(nHtot_out, d_nH, col_b, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipMemcpyAsync(rho_out, d_rho, col_b, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipMemcpyAsync(pg_out, d_pg, col_b, hipMemcpyDeviceToHost, c->stream));
+    if (niter_out)
+      HIPCHECK(hipMemcpyAsync(niter_out, d_niter, (size_t) ncol * Nmax * sizeof(int32_t),
+                              hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  HIPCHECK(hipEventElapsedTime(&c->hse_ms, c->ev[0], c->ev[1]));
+  return 0;
+}
+
+int rhg_hse_timing(rhg_context_t *ctx, double *ms_out, int32_t *lanes_out) {
+  if (!ctx) return fail(RHG_EINVAL, "null argument");
+  if (ms_out) *ms_out = ctx->hse_ms;
+  if (lanes_out) *lanes_out = ctx->hse_lpc;
   return 0;
 }
 

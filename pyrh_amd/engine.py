@@ -75,6 +75,10 @@ def lib():
         L.rhg_nlte_solve.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(RhgColumns), C.c_int32,
                                      PD, C.POINTER(C.c_int32), PD, PD, PD]
         L.rhg_nlte_timings.argtypes = [C.c_void_p, PD]
+        L.rhg_hse.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int32,
+                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,
+                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.rhg_hse_timing.argtypes = [C.c_void_p, PD, C.POINTER(C.c_int32)]
         if L.rhg_model_sizeof() != C.sizeof(RhgModel) or \
                 L.rhg_columns_sizeof() != C.sizeof(RhgColumns):
             raise EngineError("rhg_model_t / rhg_columns_t layout mismatch between the "
@@ -221,6 +225,48 @@ class Engine:
                                     C.cast(C.c_void_p(d_dpops), C.POINTER(C.c_double)),
                                     None if d_stokes is None else
                                     C.cast(C.c_void_p(d_stokes), C.POINTER(C.c_double)), None))
+
+    def hse(self, hm, scale, temp, pg_top=0.1, atm_scale=0, ndep=None):
+        """Hydrostatic equilibrium (pyrh's hse()) for the HSE model hm
+        (pyrh_amd.hse.build_hse; the engine must hold hm.model) on host
+        columns: scale, temp [ncol, ndep_max] (1-D: one column; scale may be
+        one row for all), pg_top scalar or [ncol].  Returns ne, nHtot, rho,
+        pg [ncol, ndep_max] in SI units and the pressure iterations per
+        layer."""
+        from .abi import HseStruct
+        temp = np.ascontiguousarray(np.atleast_2d(temp), np.float64)
+        ncol, nd = temp.shape
+        scale = np.ascontiguousarray(np.broadcast_to(np.atleast_2d(np.asarray(scale, np.float64)),
+                                                     temp.shape))
+        ndep = (np.full(ncol, nd, np.int32) if ndep is None
+                else np.ascontiguousarray(ndep, np.int32))
+        if ndep.shape != (ncol,):
+            raise ValueError("ndep must have one entry per column")
+        pgt = np.ascontiguousarray(np.broadcast_to(np.asarray(pg_top, np.float64), (ncol,)))
+        hs = HseStruct(hm)
+        out = [np.zeros((ncol, nd)) for _ in range(4)]
+        niter = np.zeros((ncol, nd), np.int32)
+        _check(lib().rhg_hse(self._h, C.addressof(hs.struct), ncol, nd, ndep.ctypes.data,
+                             int(atm_scale), scale.ctypes.data, temp.ctypes.data, pgt.ctypes.data, 0,
+                             *(o.ctypes.data for o in out), niter.ctypes.data))
+        return (*out, niter)
+
+    def hse_device(self, hm, d_scale, d_temp, d_pg_top, d_ndep, ncol, ndep_max, d_ne, d_nHtot,
+                   d_rho, d_pg, atm_scale=0, d_niter=None):
+        """hse with device pointers (ints) for every array; d_niter None
+        skips the iteration counts."""
+        from .abi import HseStruct
+        hs = HseStruct(hm)
+        _check(lib().rhg_hse(self._h, C.addressof(hs.struct), int(ncol), int(ndep_max), d_ndep,
+                             int(atm_scale), d_scale, d_temp, d_pg_top, 1, d_ne, d_nHtot, d_rho,
+                             d_pg, d_niter))
+
+    def hse_timing(self):
+        """(kernel ms, lanes per column) of the last hse call."""
+        ms = C.c_double()
+        lanes = C.c_int32()
+        _check(lib().rhg_hse_timing(self._h, C.byref(ms), C.byref(lanes)))
+        return ms.value, lanes.value
 
     def nlte_timings(self):
         """Milliseconds and work counts of the last NLTE solve (keys of

@@ -664,9 +664,12 @@ def atomic_rf_lines(setup: RunSetup, loggf_ids=None, nlam=0) -> np.ndarray:
     return par
 
 
-def build_model(setup: RunSetup, wave, nlte=False) -> Model:
-    """Wavelength grid and all column-independent tables for LTE synthesis."""
-    _check_supported(setup, nlte)
+def build_model(setup: RunSetup, wave, nlte=False, check=True) -> Model:
+    """Wavelength grid and all column-independent tables for LTE synthesis.
+    check=False skips the keyword checks of the synthesis (for the
+    hydrostatic-equilibrium builder, which needs only the background)."""
+    if check:
+        _check_supported(setup, nlte)
     kw = setup.kw
     el = setup.elements
     atoms = setup.atoms

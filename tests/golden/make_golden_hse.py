@@ -1,0 +1,92 @@
+"""Generate the hydrostatic-equilibrium golden vectors in
+tests/golden/hse_fixtures.npz.
+
+Runs pyrh's hse() from the reference library compiled from its C sources
+(oracle/_ref, variant "o2"; rh/rhf1d/pyrh_hse.c:67) once per column on the
+run directory tests/golden/run_falc and stores the inputs and the four
+arrays hse() fills (n_e, n_Htot, rho, p_g in SI units).  Cases: the
+workload's temperature profile and seeded map columns on the tau500 scale,
+a hotter and a cooler column, other top pressures, the FALC model's own
+temperatures, ragged depth counts, the height scale, opacity fudge factors
+and abundance overrides.  Needs /root/reference; run from the repository
+root:
+
+    PYRH_PATH=tests/golden/rhdata python3 tests/golden/make_golden_hse.py
+"""
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+sys.path.insert(0, str(HERE.parents[1]))
+sys.path.insert(0, str(HERE))
+sys.path.insert(0, str(HERE.parents[1] / "oracle"))
+
+import falc  # noqa: E402
+import refrun  # noqa: E402
+from pyrh_amd import workloads as W  # noqa: E402
+
+RUN = HERE / "run_falc"
+FUDGE_WAVE = np.array([300.0, 450.0, 600.0, 900.0])
+FUDGE_VALUE = np.array([[1.3, 1.2, 1.1, 1.0], [0.7, 0.8, 0.9, 1.0], [1.0, 1.0, 1.0, 1.0]])
+ABUN = ((26, 7.62), (11, 6.1), (14, 7.4))
+HEIGHT_KM = np.linspace(1500.0, -100.0, W.NDEP)
+
+
+def cases():
+    """(name, scale [ncol, ndep], T [ncol, ndep], ndep [ncol], pg_top,
+    atm_scale, abun, fudge)"""
+    nd = W.NDEP
+    t0 = W.t0_profile()[None]
+    tau = W.LOGTAU[None]
+    mp = W.stokes_map(6, seed=W.SEED_BASE + 41)[:, 1]
+    full = lambda n: np.full(n, nd, np.int32)  # noqa: E731
+    yield "single", tau, t0, full(1), 0.1, 0, (), None
+    yield "map6", np.repeat(tau, 6, 0), mp, full(6), 0.1, 0, (), None
+    yield "hot", tau, t0 * 1.12, full(1), 0.1, 0, (), None
+    yield "cool", tau, t0 * 0.9, full(1), 0.1, 0, (), None
+    yield "pg1", tau, t0, full(1), 1.0, 0, (), None
+    yield "pg003", tau, mp[:1], full(1), 0.03, 0, (), None
+    a = falc.atmosphere()
+    yield "falc", a[None, 0], a[None, 1], np.array([a.shape[1]], np.int32), 0.1, 0, (), None
+    yield "height", np.repeat(HEIGHT_KM[None], 2, 0), np.vstack([t0, mp[1:2]]), full(2), 0.1, 2, \
+        (), None
+    yield "fudge", tau, t0, full(1), 0.1, 0, (), (FUDGE_WAVE, FUDGE_VALUE)
+    yield "abund", tau, mp[2:3], full(1), 0.1, 0, ABUN, None
+    # ragged: the reference run on each column's first ndep points
+    yield "ragged", np.repeat(tau, 3, 0), mp[3:6], np.array([nd, nd - 9, 40], np.int32), 0.1, 0, \
+        (), None
+
+
+def main():
+    out = {"fudge_wave": FUDGE_WAVE, "fudge_value": FUDGE_VALUE,
+           "abun_id": np.array([a for a, _ in ABUN], np.int32),
+           "abun_val": np.array([v for _, v in ABUN])}
+    for name, scale, T, ndep, pgt, ats, abun, fudge in cases():
+        ncol, nd = T.shape
+        res = np.zeros((ncol, 4, nd))
+        for c in range(ncol):
+            n = int(ndep[c])
+            res[c, :, :n] = refrun.run_hse(RUN, scale[c, :n], T[c, :n], pg_top=pgt, atm_scale=ats,
+                                           abun=abun, fudge=fudge)[0]
+        sc = scale.copy()
+        Tz = T.copy()
+        for c in range(ncol):
+            sc[c, ndep[c]:] = 0.0
+            Tz[c, ndep[c]:] = 0.0
+        out[name + "__scale"] = sc
+        out[name + "__T"] = Tz
+        out[name + "__ndep"] = ndep
+        out[name + "__pg_top"] = np.float64(pgt)
+        out[name + "__atm_scale"] = np.int32(ats)
+        out[name + "__out"] = res
+        print(name, ncol, nd, "pg bottom", res[:, 3, ndep - 1].diagonal() if ncol > 1 else
+              res[0, 3, ndep[0] - 1], flush=True)
+    np.savez_compressed(HERE / "hse_fixtures.npz", **out)
+
+
+if __name__ == "__main__":
+    main()

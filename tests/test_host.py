@@ -10,6 +10,8 @@ from pyrh_amd import abi, constants as K, model, workloads
 from pyrh_amd.cscan import sscanf
 from pyrh_amd.rhdata import RHInputError, read_keywords
 
+from conftest import GOLDEN
+
 ROOT = Path(__file__).resolve().parents[1]
 
 
@@ -127,3 +129,28 @@ def test_library_reports_errors_without_device():
     h = C.c_void_p()
     rc = lib.rhg_open(None, 0, C.byref(h))
     assert rc == -1 and b"null" in lib.rhg_last_error()
+
+
+def test_hse_host_tables():
+    """The HSE model: packed element tables with hydrogen first, the opacity
+    wavelength (the last of {LAMBDA_REF, 500 nm}), the fudge factors there
+    (Linear over the fudge table) and the rejected column-mass scale."""
+    import pytest
+    from pyrh_amd import api, hse, model
+    s = model.open_run(str(GOLDEN / "run_falc"))
+    hm = hse.build_hse(s)
+    npf = s.elements.Tpf.size
+    assert hm.el_pf_ptr[0] == 0 and hm.el_abund[0] == 1.0
+    np.testing.assert_array_equal(np.diff(hm.el_pf_ptr), hm.el_nstage[:-1] * npf)
+    assert hm.el_logpf.size == hm.el_nstage.sum() * npf
+    assert hm.model.lam[hm.lam_index] == 500.0 and hm.lam_index == hm.model.lam.size - 1
+    assert hm.hmin_fudge == 1.0 and hm.scatt_fudge == 1.0
+    fw = np.array([300.0, 450.0, 600.0])
+    fv = np.array([[1.3, 1.2, 1.1], [0.7, 0.8, 0.9], [1.0, 1.0, 1.0]])
+    hf = hse.build_hse(s, fw, fv)
+    assert abs(hf.hmin_fudge - (1.2 + (1.1 - 1.2) * 50.0 / 150.0)) < 1e-15
+    assert abs(hf.scatt_fudge - (0.8 + (0.9 - 0.8) * 50.0 / 150.0)) < 1e-15
+    with pytest.raises(NotImplementedError):
+        api.hse(str(GOLDEN / "run_falc"), 1, np.zeros(5), np.full(5, 5000.0))
+    with pytest.raises(ValueError):
+        hse.build_hse(s, fw, None)
