@@ -19,11 +19,15 @@ map columns to convergence and reports columns/s, the iteration histogram
 and the roofline of its formal-solution kernel.  The mixed leg (object
 mixed) synthesises 8192 columns of 64-256 depth points with 400 Zeeman
 lines on 2000 wavelengths at three mu, with adjoint response functions on
-a subset and the oracle on 128 columns.
+a subset and the oracle on 128 columns.  The HSE leg (object hse) builds
+the hydrostatic structure (pyrh's hse()) of 16384 map temperature columns
+and reports columns/s, the pressure-iteration census and the roofline of
+its kernel.
 
     python bench.py [--steps K] [--warmup W] [--ncol N] [--no-cpu] [--no-rf]
                     [--no-nlte] [--nlte-ncol N] [--nlte-only]
                     [--no-mixed] [--mixed-ncol N] [--mixed-only]
+                    [--no-hse] [--hse-ncol N] [--hse-only]
 """
 from __future__ import annotations
 
@@ -302,6 +306,103 @@ def _nlte_leg(torch, dev, ncol, do_cpu, nworkers):
     return res
 
 
+HSE_WORKLOAD = "hse_map_16384"
+
+
+def _hse_leg(torch, dev, ncol, do_cpu, nworkers, sample=128):
+    """Hydrostatic equilibrium (pyrh's hse()) of the map's temperature
+    columns: one rhg_hse call on ncol columns of 82 depths with the inputs
+    resident in HBM, columns/s of the whole call and of the kernel, the
+    pressure-iteration census and the roofline of k_hse (algorithmic FP64
+    operations of the reference's arithmetic, pyrh_amd.roofline.hse_flops,
+    with the Newton and chemical-equilibrium counts per pressure iteration
+    from the oracle's census of `sample` columns); the GPU result on those
+    columns against the oracle; the reference's hse() on a bounded sample on
+    this host's cores."""
+    from pyrh_amd import model as M, hse as HS, roofline as R, workloads as W
+    from pyrh_amd.engine import Engine
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle
+    hm = HS.build_hse(M.open_run(str(RUN_DIR)))
+    T = np.ascontiguousarray(W.stokes_map(ncol, seed=W.SEED_BASE + 44)[:, 1])
+    nd = T.shape[1]
+    sc = np.ascontiguousarray(np.broadcast_to(W.LOGTAU, T.shape))
+    eng = Engine(hm.model)
+    d = {k: torch.from_numpy(v).to(dev) for k, v in
+         (("sc", sc), ("T", T), ("pg", np.full(ncol, 0.1)), ("nd", np.full(ncol, nd, np.int32)))}
+    outs = [torch.empty((ncol, nd), dtype=torch.float64, device=dev) for _ in range(4)]
+    d_it = torch.empty((ncol, nd), dtype=torch.int32, device=dev)
+
+    def run(n):
+        eng.hse_device(hm, d["sc"].data_ptr(), d["T"].data_ptr(), d["pg"].data_ptr(),
+                       d["nd"].data_ptr(), n, nd, *(o.data_ptr() for o in outs), 0,
+                       d_it.data_ptr())
+
+    run(min(ncol, 256))
+    torch.cuda.synchronize()
+    walls, kms = [], []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        run(ncol)
+        torch.cuda.synchronize()
+        walls.append(time.perf_counter() - t0)
+        kms.append(eng.hse_timing()[0])
+    lanes = eng.hse_timing()[1]
+    it = d_it.cpu().numpy()
+    got = [o[:sample].cpu().numpy() for o in outs]
+    *ref, rit, (newton, chem) = pyoracle.hse(hm, sc[:sample], T[:sample], 0.1, nthreads=nworkers,
+                                             census=True)
+    err = max(float(np.max(np.abs(a - b) / np.abs(b))) for a, b in zip(got, ref))
+    L_s = int(rit.sum())
+    fl, per = R.hse_flops(hm, int(it.sum()), newton / L_s, chem / L_s)
+    k_ms = min(kms)
+    achieved = fl / (k_ms * 1e-3) / 1e12
+    res = {
+        "workload": HSE_WORKLOAD if ncol == 16384 else f"hse_map_{ncol}",
+        "value": round(ncol / min(walls), 1), "unit": "columns/s",
+        "kernel_columns_per_s": round(ncol / (k_ms * 1e-3), 1), "kernel_ms": round(k_ms, 3),
+        "wall_ms": round(min(walls) * 1e3, 3), "dtype": "f64", "ncol": ncol, "ndep": nd,
+        "lanes_per_column": lanes, "pressure_iterations": int(it.sum()),
+        "top_layer_iterations_mean": round(float(it[:, 0].mean()), 2),
+        "census_per_pressure_iteration": {"newton_steps": round(newton / L_s, 4),
+                                          "chem_iterations": round(chem / L_s, 4)},
+        "roofline": {"bound": "valu", "kernel": "k_hse", "achieved": round(achieved, 3),
+                     "peak": R.PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / R.PEAK_FP64_TFLOPS, 4), "traffic": None,
+                     "flops_per_launch": fl, "flop_model": per},
+        "oracle_check": {"columns": sample, "max_rel_err": float(f"{err:.3g}"),
+                         "iterations_equal": bool(np.array_equal(it[:sample], rit))},
+        "cpu_baseline": None,
+    }
+    if do_cpu:
+        import refrun
+        ns = min(ncol, 64 * nworkers)
+        if refrun.ref_available("o1"):
+            from concurrent.futures import ThreadPoolExecutor
+            chunks = np.array_split(np.arange(ns), nworkers)
+            t0 = time.perf_counter()
+            with ThreadPoolExecutor(nworkers) as ex:
+                list(ex.map(lambda ix: refrun.run_hse(RUN_DIR, W.LOGTAU, T[ix], variant="o1"),
+                            chunks))
+            dtc = time.perf_counter() - t0
+            res["cpu_baseline"] = {
+                "value": round(ns / dtc, 3), "unit": "columns/s", "cores": nworkers,
+                "kind": "reference",
+                "sample": f"{ns} columns of the batch, hse() per column, {nworkers} processes, "
+                          f"{dtc:.1f} s wall"}
+        else:
+            t0 = time.perf_counter()
+            pyoracle.hse(hm, sc[:ns], T[:ns], 0.1, nthreads=nworkers)
+            dtc = time.perf_counter() - t0
+            res["cpu_baseline"] = {
+                "value": round(ns / dtc, 3), "unit": "columns/s", "cores": nworkers, "kind": "port",
+                "sample": f"{ns} columns, C restatement, OpenMP {nworkers} threads, {dtc:.1f} s wall"}
+    del outs, d
+    eng.close()
+    torch.cuda.empty_cache()
+    return res
+
+
 MIXED_RUN = ROOT / "tests" / "golden" / "run_mixed"
 MIXED_WORKLOAD = "mixed_zeeman400_8192"
 
@@ -431,6 +532,10 @@ def main():
     ap.add_argument("--mixed-ncol", type=int, default=8192)
     ap.add_argument("--mixed-only", action="store_true",
                     help="run only the mixed-size batch leg and print its object")
+    ap.add_argument("--no-hse", action="store_true", help="skip the hydrostatic-equilibrium leg")
+    ap.add_argument("--hse-ncol", type=int, default=16384)
+    ap.add_argument("--hse-only", action="store_true",
+                    help="run only the hydrostatic-equilibrium leg and print its object")
     ap.add_argument("--profile-step", action="store_true",
                     help="one FP32-form synthesis and one 1 GiB device copy, for rocprofv3 --pmc")
     a = ap.parse_args()
@@ -441,11 +546,11 @@ def main():
         sys.exit(2)
 
     nw = max(1, min(16, len(os.sched_getaffinity(0))))
-    if a.nlte_only or a.mixed_only:
+    if a.nlte_only or a.mixed_only or a.hse_only:
         import torch
-        leg = _nlte_leg if a.nlte_only else _mixed_leg
-        res = leg(torch, torch.device("cuda:0"), a.nlte_ncol if a.nlte_only else a.mixed_ncol,
-                  not a.no_cpu, nw)
+        leg, n = ((_nlte_leg, a.nlte_ncol) if a.nlte_only else
+                  (_mixed_leg, a.mixed_ncol) if a.mixed_only else (_hse_leg, a.hse_ncol))
+        res = leg(torch, torch.device("cuda:0"), n, not a.no_cpu, nw)
         print(json.dumps(res), flush=True)
         return
 
@@ -506,6 +611,7 @@ def main():
     rf = None if a.no_rf else _rf_leg(eng, W, torch, dev, reps=3)
     nlte = None if a.no_nlte else _nlte_leg(torch, dev, a.nlte_ncol, not a.no_cpu, nw)
     mixed = None if a.no_mixed else _mixed_leg(torch, dev, a.mixed_ncol, not a.no_cpu, nw)
+    hse = None if a.no_hse else _hse_leg(torch, dev, a.hse_ncol, not a.no_cpu, nw)
 
     # host-buffer rate (PCIe both ways), reported beside the resident rate
     t1 = time.perf_counter()
@@ -551,6 +657,7 @@ def main():
         "response_functions": rf,
         "nlte": nlte,
         "mixed": mixed,
+        "hse": hse,
         "pipeline_flops": fl["total"],
         "host_buffers_columns_per_s": round(host_rate, 1),
     }

@@ -394,6 +394,32 @@ int rhg_hse(rhg_context_t *ctx, const rhg_hse_t *hse, int32_t ncol, int32_t ndep
             double *ne_out, double *nHtot_out, double *rho_out, double *pg_out,
             int32_t *niter_out);
 
+/* pyrh's get_ne_from_nH for a batch (pyrh.pyx:396-423,
+ * rh/rhf1d/pyrh_hse.c:555-678): n_e at every depth point from the
+ * temperature [K] and n_Htot [cm^-3] by the LTE ionisation equilibrium of
+ * every element (Solve_ne from scratch with hydrogen in LTE,
+ * rh/solvene.c:53-135).  temp, nH and ne_out are [ncol][ndep_max], n_e in
+ * cm^-3 as the reference returns it, zero past ndep.  The element tables
+ * and the context are those of rhg_hse.  Host memory, or device memory for
+ * the arrays if `device_ptrs`.  FP64.  Synchronous. */
+int rhg_ne_from_nh(rhg_context_t *ctx, const rhg_hse_t *hse, int32_t ncol, int32_t ndep_max,
+                   const int32_t *ndep, const double *temp, const double *nH,
+                   int32_t device_ptrs, double *ne_out);
+
+/* pyrh's get_scales for a batch (pyrh.pyx:491-533,
+ * rh/rhf1d/pyrh_hse.c:402-553): the background opacity at the model's
+ * reference wavelength (the lam_ref asked for; rhg_open with the model of
+ * that single wavelength) and convertScales (rh/rhf1d/multiatmos.c:100-177)
+ * with all three depth scales kept.  Columns as in rhg_synth_lte (rows 1,
+ * 2, 3, 4 and 8 are read; mu is not).  tau_out, height_out [m] and
+ * cmass_out [kg m^-2] are [ncol][ndep_max], zero past ndep; the input
+ * scale comes back converted to SI, where the reference leaves that output
+ * unset.  A model with a Kurucz line at the reference wavelength is
+ * rejected (RHG_EUNSUPPORTED).  Host memory, or device memory if
+ * `device_ptrs`.  FP64.  Synchronous. */
+int rhg_scales(rhg_context_t *ctx, const rhg_columns_t *cols, int32_t device_ptrs,
+               double *tau_out, double *height_out, double *cmass_out);
+
 /* Duration of the hydrostatic-equilibrium kernel of the last rhg_hse call
  * in ms (HIP events on the context's stream) and the lanes per column it
  * ran with. */

@@ -230,11 +230,12 @@ def background(model, atmos, ls, mu=1.0, atm_scale=0, ic=0):
         pop.reshape(6, N)
 
 
-def hse(hm, scale, temp, pg_top=0.1, atm_scale=0, ndep=None, nthreads=0):
+def hse(hm, scale, temp, pg_top=0.1, atm_scale=0, ndep=None, nthreads=0, census=False):
     """pyrh's hse() on a batch of columns: scale, temp [ncol, ndep_max]
     (a single column may be 1-D), pg_top scalar or [ncol].  Returns ne,
     nHtot, rho, pg [ncol, ndep_max] (SI) and the pressure iterations per
-    layer."""
+    layer; with census=True also (Newton steps of get_ne, chemical-
+    equilibrium iterations) summed over the batch."""
     from pyrh_amd.abi import HseStruct
     L = lib()
     L.rho_hse.argtypes = [C.POINTER(RhgModel), C.c_void_p, C.c_int, C.c_int,
@@ -243,6 +244,8 @@ def hse(hm, scale, temp, pg_top=0.1, atm_scale=0, ndep=None, nthreads=0):
                           C.POINTER(C.c_double), C.POINTER(C.c_double),
                           C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int)]
     L.rho_hse.restype = C.c_int
+    L.rho_hse_census.argtypes = L.rho_hse.argtypes + [C.POINTER(C.c_longlong)]
+    L.rho_hse_census.restype = C.c_int
     scale = np.ascontiguousarray(np.atleast_2d(scale), np.float64)
     temp = np.ascontiguousarray(np.atleast_2d(temp), np.float64)
     ncol, nd = temp.shape
@@ -255,10 +258,52 @@ def hse(hm, scale, temp, pg_top=0.1, atm_scale=0, ndep=None, nthreads=0):
     hs = HseStruct(hm)
     out = [np.zeros((ncol, nd)) for _ in range(4)]
     niter = np.zeros((ncol, nd), np.int32)
-    rc = L.rho_hse(ms.ptr(), C.addressof(hs.struct), ncol, nd,
-                   ndep.ctypes.data_as(C.POINTER(C.c_int)), int(atm_scale), _p(scale), _p(temp),
-                   _p(pgt), int(nthreads), *(_p(o) for o in out),
-                   niter.ctypes.data_as(C.POINTER(C.c_int)))
+    cen = (C.c_longlong * 2)()
+    rc = L.rho_hse_census(ms.ptr(), C.addressof(hs.struct), ncol, nd,
+                          ndep.ctypes.data_as(C.POINTER(C.c_int)), int(atm_scale), _p(scale),
+                          _p(temp), _p(pgt), int(nthreads), *(_p(o) for o in out),
+                          niter.ctypes.data_as(C.POINTER(C.c_int)), cen)
     if rc != 0:
         raise RuntimeError(f"oracle hse failed with code {rc}")
-    return (*out, niter)
+    return (*out, niter) + ((int(cen[0]), int(cen[1])),) if census else (*out, niter)
+
+
+def ne_from_nh(hm, temp, nH, ndep=None, nthreads=0):
+    """pyrh's get_ne_from_nH on [ncol, ndep_max] arrays: n_e in cm^-3."""
+    from pyrh_amd.abi import HseStruct
+    L = lib()
+    L.rho_ne_from_nh.argtypes = [C.POINTER(RhgModel), C.c_void_p, C.c_int, C.c_int,
+                                 C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double)]
+    L.rho_ne_from_nh.restype = C.c_int
+    temp = np.ascontiguousarray(np.atleast_2d(temp), np.float64)
+    nH = np.ascontiguousarray(np.atleast_2d(nH), np.float64)
+    ncol, nd = temp.shape
+    ndep = (np.full(ncol, nd, np.int32) if ndep is None
+            else np.ascontiguousarray(ndep, np.int32))
+    ms, hs = ModelStruct(hm.model), HseStruct(hm)
+    ne = np.zeros((ncol, nd))
+    rc = L.rho_ne_from_nh(ms.ptr(), C.addressof(hs.struct), ncol, nd,
+                          ndep.ctypes.data_as(C.POINTER(C.c_int)), _p(temp), _p(nH),
+                          int(nthreads), _p(ne))
+    if rc != 0:
+        raise RuntimeError(f"oracle ne_from_nh failed with code {rc}")
+    return ne
+
+
+def scales(model, atmos, atm_scale=0, ndep=None, nthreads=0):
+    """pyrh's get_scales on a model of the single reference wavelength:
+    tau, height [m], cmass [kg m^-2], each [ncol, ndep_max]."""
+    L = lib()
+    L.rho_scales.argtypes = [C.POINTER(RhgModel), C.POINTER(RhgColumns), C.c_int,
+                             C.POINTER(C.c_double), C.POINTER(C.c_double),
+                             C.POINTER(C.c_double)]
+    L.rho_scales.restype = C.c_int
+    ms = ModelStruct(model)
+    cs = ColumnsStruct(atmos, ndep, atm_scale, 1.0)
+    ncol, nd = cs.struct.ncol, cs.struct.ndep_max
+    out = [np.zeros((ncol, nd)) for _ in range(3)]
+    rc = L.rho_scales(ms.ptr(), cs.ptr(), int(nthreads), *(_p(o) for o in out))
+    if rc != 0:
+        raise RuntimeError(f"oracle scales failed with code {rc}")
+    return tuple(out)

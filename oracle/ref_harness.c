@@ -15,6 +15,12 @@
    Usage:  rhref_run <rundir> <in.bin> <out.bin> [capture.bin]
            rhref_run --hse <rundir> <in.bin> <out.bin>
 
+   rhref_run --nefromnh <rundir> <in.bin> <out.bin>: pyrh's get_ne_from_nH
+   (rh/rhf1d/pyrh_hse.c:555) per column from rows 0, 1 and 8; out.bin per
+   column float64 ne[Ndep] in cm^-3.
+   rhref_run --scales <rundir> <in.bin> <out.bin>: pyrh's get_scales
+   (rh/rhf1d/pyrh_hse.c:402) per column with lam_ref in the mu field;
+   out.bin per column float64 tau[Ndep], height[Ndep], cmass[Ndep].
    hse mode: in.bin as above (only rows 0 and 1 of each column are used,
    row 0 = scale, row 1 = T; the wave list is ignored, mu holds pg_top;
    overrides "abun id value" and "fudge lambda v0 v1 v2" are passed on),
@@ -43,6 +49,12 @@
 #include "background.h"
 #include "rhf1d/pyrh_compute1dray.h"
 
+void get_ne_from_nH(char *cwd, int atm_scale, int Ndep, double *scale, double *temp,
+                    double *nH, double *ne);
+void get_scales(char *cwd, int Ndep, double *scale, double *temp, double *ne, double *vz,
+                double *vmic, double *nH, int atm_scale, double lam_ref, double *tau,
+                double *height, double *cmass, int Nabun, int *abundance_id,
+                double *abundance_value);
 void hse(char *cwd, int Ndep, double *scale, double *temp, double *ne,
          double *nHtot, double *rho, double *pg, int atm_scale,
          int fudge_num, double *fudge_lam, double *fudge,
@@ -396,9 +408,11 @@ int main(int argc, char **argv)
   double mu, *wave, *atm, *w;
   FILE *fi, *fo;
 
-  int hse_mode = 0;
+  int hse_mode = 0, ne_mode = 0, scales_mode = 0;
   Overrides ov;
   if (argc > 1 && strcmp(argv[1], "--hse") == 0) { hse_mode = 1; argv++; argc--; }
+  else if (argc > 1 && strcmp(argv[1], "--nefromnh") == 0) { ne_mode = 1; argv++; argc--; }
+  else if (argc > 1 && strcmp(argv[1], "--scales") == 0) { scales_mode = 1; argv++; argc--; }
   if (argc < 4) {
     fprintf(stderr, "usage: %s rundir in.bin out.bin [capture.bin]\n", argv[0]);
     return 2;
@@ -426,6 +440,24 @@ int main(int argc, char **argv)
     mySpectrum spec;
     double *zero;
     if (fread(atm, sizeof(double), 9 * Ndep, fi) != (size_t) 9 * Ndep) return 2;
+    if (ne_mode) {
+      double *o = calloc((size_t) Ndep, sizeof(double));
+      get_ne_from_nH(".", atm_scale, Ndep, atm, atm + Ndep, atm + 8 * Ndep, o);
+      fwrite(o, sizeof(double), (size_t) Ndep, fo);
+      free(o);
+      fflush(fo);
+      continue;
+    }
+    if (scales_mode) {
+      double *o = calloc(3 * (size_t) Ndep, sizeof(double));
+      get_scales(".", Ndep, atm, atm + Ndep, atm + 2 * Ndep, atm + 3 * Ndep, atm + 4 * Ndep,
+                 atm + 8 * Ndep, atm_scale, mu, o, o + Ndep, o + 2 * Ndep, ov.nabun,
+                 ov.nabun ? ov.abun_ids : NULL, ov.nabun ? ov.abun_val : NULL);
+      fwrite(o, sizeof(double), 3 * (size_t) Ndep, fo);
+      free(o);
+      fflush(fo);
+      continue;
+    }
     if (hse_mode) {
       double *o = calloc(4 * (size_t) Ndep, sizeof(double));
       o[3 * Ndep] = mu;   /* pg[0] = pg_top on input */

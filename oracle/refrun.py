@@ -138,3 +138,38 @@ def run_hse(rundir, scale, T, pg_top=0.1, variant="o2", timeout=600, atm_scale=0
             raise RuntimeError(f"reference hse failed: {r.stderr.decode()[-2000:]}")
         o = np.fromfile(fout, np.float64).reshape(ncol, 4, ndep)
     return o
+
+
+def _run_mode(mode, rundir, atmos, mu, atm_scale, nout, timeout=600, abun=()):
+    exe = REF_DIR / "rhref_run_o2"
+    env = dict(os.environ, PYRH_PATH=str(DATA_DIR))
+    atmos = np.ascontiguousarray(atmos, np.float64)
+    if atmos.ndim == 2:
+        atmos = atmos[None]
+    ncol, _, ndep = atmos.shape
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in"), os.path.join(td, "out")
+        if abun:
+            fov = os.path.join(td, "overrides")
+            with open(fov, "w") as f:
+                for i, v in abun:
+                    f.write(f"abun {int(i)} {float(v)!r}\n")
+            env["RHREF_OVERRIDES"] = fov
+        _write_input(fin, atmos, np.zeros(1), mu, atm_scale)
+        r = subprocess.run([str(exe), mode, str(rundir), fin, fout],
+                           env=env, capture_output=True, timeout=timeout)
+        if r.returncode != 0:
+            raise RuntimeError(f"reference {mode} failed: {r.stderr.decode()[-2000:]}")
+        return np.fromfile(fout, np.float64).reshape(ncol, nout, ndep)
+
+
+def run_ne_from_nh(rundir, atmos, atm_scale=0):
+    """pyrh.get_ne_from_nH per column of atmos [ncol, 9, ndep] (rows 0, 1,
+    8 used): n_e [ncol, ndep] in cm^-3."""
+    return _run_mode("--nefromnh", rundir, atmos, 0.0, atm_scale, 1)[:, 0]
+
+
+def run_scales(rundir, atmos, lam_ref, atm_scale=0, abun=()):
+    """pyrh.get_scales per column of atmos [ncol, 9, ndep]: tau, height [m],
+    cmass [kg m^-2] as [ncol, 3, ndep]."""
+    return _run_mode("--scales", rundir, atmos, lam_ref, atm_scale, 3, abun=abun)

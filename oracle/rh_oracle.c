@@ -977,6 +977,9 @@ static void solve_linear_improve(int N, double A[][NEQ_MAX], double *b)
    nstar_all holds all level populations of all model atoms (offsets
    atom_level_ptr) and is rescaled in place for atoms bound in molecules;
    ntotal[atom] likewise.  Returns nHmin; nmol_out gets molecule numbers. */
+/* iterations of chemeq on this thread (work census of rho_hse) */
+static _Thread_local long long g_chem_iter = 0;
+
 static double chemeq(const rhg_model_t *m, double T, double ne, double nHtot,
                      double *nstar_all, double *ntotal, double *nmol_out)
 {
@@ -1043,6 +1046,7 @@ static double chemeq(const rhg_model_t *m, double T, double ne, double nHtot,
       for (i = 0; i < N; i++)
         if (nw[i]) dnmax = MAXV(dnmax, fabs((nw[i] - old[i]) / nw[i]));
     }
+    g_chem_iter++;
     if (dnmax <= CHEM_ITER_LIMIT) break;
     niter++;
   }
@@ -1358,6 +1362,10 @@ struct nlte_out { double *pops; int *niter; double *dpops; int ndep_max; double 
 static int nlte_column(const rhg_model_t *m, const rhg_nlte_t *na, const column_t *c,
                        struct nlte_out *o);
 
+/* the depth scales convertScales leaves (rho_scales): where to put tau_ref
+   and cmass of the column being solved on this thread (NULL: nowhere) */
+static _Thread_local double *g_tau_out = NULL, *g_cmass_out = NULL;
+
 static int synth_column_ex(const rhg_model_t *m, const rhg_columns_t *cols, int ic,
                            int fp32, double *stokes, double *height_out,
                            double *chiref_out, int diag_ls, double *diag_chi,
@@ -1492,6 +1500,18 @@ static int synth_column_ex(const rhg_model_t *m, const rhg_columns_t *cols, int 
       h_zero = linear1(N, c.tau_ref, c.height, unity);
       for (k = 0; k < N; k++) c.height[k] = c.height[k] - h_zero;
     }
+    if (cols->atm_scale == 2 && N > 1) {
+      c.cmass[0] = (c.nHtot[0] * m->totalAbund + c.ne[0]) * (KBOLTZMANN * c.T[0] / m->gravity);
+      c.tau_ref[0] = 0.5 * chiref[0] * (c.height[0] - c.height[1]);
+      if (c.tau_ref[0] > 1.0) c.tau_ref[0] = 0.0;
+      for (k = 1; k < N; k++) {
+        c.cmass[k] = c.cmass[k - 1] + 0.5 * (rho[k - 1] + rho[k]) * (c.height[k - 1] - c.height[k]);
+        c.tau_ref[k] = c.tau_ref[k - 1] + 0.5 * (chiref[k - 1] + chiref[k]) *
+                       (c.height[k - 1] - c.height[k]);
+      }
+    }
+    if (g_tau_out) for (k = 0; k < N; k++) g_tau_out[k] = c.tau_ref[k];
+    if (g_cmass_out) for (k = 0; k < N; k++) g_cmass_out[k] = c.cmass[k];
   }
   if (height_out) for (k = 0; k < N; k++) height_out[k] = c.height[k];
   if (chiref_out) for (k = 0; k < N; k++) chiref_out[k] = chiref[k];
@@ -2507,12 +2527,16 @@ static void hse_getfjk(const rhg_model_t *m, const rhg_hse_t *h, int e, double T
 
 /* get_ne(fromscratch = TRUE) with hydrogen in LTE,
    rh/rhf1d/pyrh_background.c:459-549: starting value from hydrogen
-   ionisation with U = 0, Newton steps over all elements and H-. */
-static double hse_get_ne(const rhg_model_t *m, const rhg_hse_t *h, double T, double nHtot)
+   ionisation with U = Uk (0 there; Solve_ne, rh/solvene.c:78-88, takes
+   the Kurucz partition function of neutral hydrogen), Newton steps over
+   all elements and H- (the same in both). */
+static _Thread_local long long g_newton_iter = 0;
+
+static double hse_get_ne(const rhg_model_t *m, const rhg_hse_t *h, double T, double nHtot,
+                         double Uk)
 {
   const double C1 = (HPLANCK / (2.0 * PI * M_ELECTRON)) * (HPLANCK / KBOLTZMANN);
   double fjk[NSTAGE_MAX], dfjk[NSTAGE_MAX], error, sum, akj, PhiH, PhiHmin, ne, ne_old, dne;
-  double Uk = 0.0;
   int niter = 0, n, j;
   PhiH = 0.5 * pow(C1 / T, 1.5) * exp(Uk + h->el_ionpot[h->el_pf_ptr[0]] / (KBOLTZMANN * T));
   ne_old = (sqrt(1.0 + 4.0 * nHtot * PhiH) - 1.0) / (2.0 * PhiH);
@@ -2536,6 +2560,7 @@ static double hse_get_ne(const rhg_model_t *m, const rhg_hse_t *h, double T, dou
     ne = ne_old - nHtot * error / (1.0 - nHtot * sum);
     dne = fabs((ne - ne_old) / ne_old);
     ne_old = ne;
+    g_newton_iter++;
     if (dne <= HSE_MAX_ELECTRON_ERROR) break;
     niter++;
   }
@@ -2664,7 +2689,7 @@ static double hse_opacity(const rhg_model_t *m, const rhg_hse_t *h, int ls, doub
 static void hse_layer(const rhg_model_t *m, const rhg_hse_t *h, double T, double nHtot,
                       double *ns, double *ntot, double *nmol, double *ne_out, double *opac_out)
 {
-  double ne = hse_get_ne(m, h, T, nHtot), nHmin;
+  double ne = hse_get_ne(m, h, T, nHtot, 0.0), nHmin;
   int ia;
   for (ia = 0; ia < m->natom; ia++) {
     ntot[ia] = m->atom_abund[ia] * nHtot;
@@ -2739,7 +2764,17 @@ int rho_hse(const rhg_model_t *m, const rhg_hse_t *h, int ncol, int ndep_max,
             const double *pg_top, int nthreads, double *ne, double *nHtot, double *rho,
             double *pg, int *niter)
 {
+  return rho_hse_census(m, h, ncol, ndep_max, ndep, atm_scale, scale, temp, pg_top, nthreads,
+                        ne, nHtot, rho, pg, niter, NULL);
+}
+
+int rho_hse_census(const rhg_model_t *m, const rhg_hse_t *h, int ncol, int ndep_max,
+                   const int *ndep, int atm_scale, const double *scale, const double *temp,
+                   const double *pg_top, int nthreads, double *ne, double *nHtot, double *rho,
+                   double *pg, int *niter, long long *census)
+{
   int ic, err = 0;
+  long long newton = 0, chem = 0;
   if (m->nnuclei + m->nmol > NEQ_MAX) return RHG_EUNSUPPORTED;
   if (atm_scale != 0 && atm_scale != 2) return RHG_EUNSUPPORTED;
   for (ic = 0; ic < h->nel; ic++) if (h->el_nstage[ic] > NSTAGE_MAX) return RHG_EUNSUPPORTED;
@@ -2750,13 +2785,78 @@ int rho_hse(const rhg_model_t *m, const rhg_hse_t *h, int ncol, int ndep_max,
   if (niter) memset(niter, 0, sizeof(int) * ncol * ndep_max);
 #ifdef _OPENMP
   if (nthreads <= 0) nthreads = omp_get_max_threads();
-#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads) reduction(|:err)
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads) reduction(|:err) \
+    reduction(+:newton, chem)
 #endif
   for (ic = 0; ic < ncol; ic++) {
     size_t o = (size_t) ic * ndep_max;
+    long long n0 = g_newton_iter, c0 = g_chem_iter;
     if (hse_column(m, h, ndep[ic], atm_scale, scale + o, temp + o, pg_top[ic], ne + o,
                    nHtot + o, rho + o, pg + o, niter ? niter + o : NULL))
       err |= 1;
+    newton += g_newton_iter - n0;
+    chem += g_chem_iter - c0;
+  }
+  (void) nthreads;
+  if (census) {
+    census[0] = newton;
+    census[1] = chem;
+  }
+  return err ? RHG_ENOMEM : RHG_OK;
+}
+
+/* pyrh's get_ne_from_nH (rh/rhf1d/pyrh_hse.c:555-678): n_e per depth from
+   T and n_Htot [cm^-3] by Solve_ne (rh/solvene.c:53-135) from scratch with
+   hydrogen in LTE; returned in cm^-3 as the entry point converts back. */
+int rho_ne_from_nh(const rhg_model_t *m, const rhg_hse_t *h, int ncol, int ndep_max,
+                   const int *ndep, const double *temp, const double *nH, int nthreads,
+                   double *ne_out)
+{
+  int ic;
+  for (ic = 0; ic < h->nel; ic++) if (h->el_nstage[ic] > NSTAGE_MAX) return RHG_EUNSUPPORTED;
+  memset(ne_out, 0, sizeof(double) * ncol * ndep_max);
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+#endif
+  for (ic = 0; ic < ncol; ic++) {
+    size_t o = (size_t) ic * ndep_max;
+    int k;
+    for (k = 0; k < ndep[ic]; k++) {
+      double T = temp[o + k], nHtot = nH[o + k] / CUBE(CM_TO_M);
+      double Uk = linear1(m->npf, m->Tpf, h->el_logpf + h->el_pf_ptr[0], T);
+      ne_out[o + k] = hse_get_ne(m, h, T, nHtot, Uk) * CUBE(CM_TO_M);
+    }
+  }
+  (void) nthreads;
+  return RHG_OK;
+}
+
+/* pyrh's get_scales (rh/rhf1d/pyrh_hse.c:402-553): the background
+   opacity at the model's reference wavelength and convertScales
+   (rh/rhf1d/multiatmos.c:100-177) for each column; tau, height [m] and
+   cmass [kg m^-2] are [ncol][ndep_max], zero past ndep. */
+int rho_scales(const rhg_model_t *m, const rhg_columns_t *cols, int nthreads, double *tau_out,
+               double *height_out, double *cmass_out)
+{
+  int ic, err = 0;
+  const int Nmax = cols->ndep_max;
+  if (m->nnuclei + m->nmol > NEQ_MAX) return RHG_EUNSUPPORTED;
+  memset(tau_out, 0, sizeof(double) * cols->ncol * Nmax);
+  memset(height_out, 0, sizeof(double) * cols->ncol * Nmax);
+  memset(cmass_out, 0, sizeof(double) * cols->ncol * Nmax);
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads) reduction(|:err)
+#endif
+  for (ic = 0; ic < cols->ncol; ic++) {
+    g_tau_out = tau_out + (size_t) ic * Nmax;
+    g_cmass_out = cmass_out + (size_t) ic * Nmax;
+    if (m->nspect != 1) { err |= 1; continue; }
+    if (synth_column_ex(m, cols, ic, 0, NULL, height_out + (size_t) ic * Nmax,
+                        NULL, -1, NULL, NULL, NULL, 0, NULL, NULL, NULL, NULL))
+      err |= 1;
+    g_tau_out = g_cmass_out = NULL;
   }
   (void) nthreads;
   return err ? RHG_ENOMEM : RHG_OK;

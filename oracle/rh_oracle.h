@@ -53,6 +53,26 @@ int rho_hse(const rhg_model_t *m, const rhg_hse_t *h, int ncol, int ndep_max,
             const int *ndep, int atm_scale, const double *scale, const double *temp,
             const double *pg_top, int nthreads, double *ne, double *nHtot, double *rho,
             double *pg, int *niter);
+/* The same, with a work census (NULL to skip): census[0] the Newton steps
+ * of get_ne, census[1] the iterations of the chemical equilibrium, summed
+ * over the batch. */
+int rho_hse_census(const rhg_model_t *m, const rhg_hse_t *h, int ncol, int ndep_max,
+                   const int *ndep, int atm_scale, const double *scale, const double *temp,
+                   const double *pg_top, int nthreads, double *ne, double *nHtot, double *rho,
+                   double *pg, int *niter, long long *census);
+
+/* pyrh's get_ne_from_nH (rh/rhf1d/pyrh_hse.c:555-678): n_e [cm^-3] per
+ * depth from temp [K] and nH [cm^-3], all [ncol][ndep_max]; h as for
+ * rho_hse. */
+int rho_ne_from_nh(const rhg_model_t *m, const rhg_hse_t *h, int ncol, int ndep_max,
+                   const int *ndep, const double *temp, const double *nH, int nthreads,
+                   double *ne_out);
+
+/* pyrh's get_scales (rh/rhf1d/pyrh_hse.c:402-553) for a batch of columns
+ * on a model whose reference wavelength is the lam_ref asked for: tau500,
+ * height [m] and column mass [kg m^-2], [ncol][ndep_max]. */
+int rho_scales(const rhg_model_t *m, const rhg_columns_t *cols, int nthreads, double *tau_out,
+               double *height_out, double *cmass_out);
 
 /* Single-function entry points for closed-form and invariant tests. */
 double rho_voigt_humlicek(double a, double v, double *F);

@@ -4,8 +4,9 @@
 (pyrh.pyx:537-668): LTE runs go through the LTE synthesis, runs with an
 ACTIVE atom through the NLTE solve (populations, then the emergent ray);
 ``synth_batch`` and ``nlte_batch`` are the batched forms that the GPU is
-built for.  ``hse`` keeps the signature of pyrh.hse (pyrh.pyx:427-487),
-``hse_batch`` is its batched form.  Parsed inputs, models and device contexts are
+built for.  ``hse``, ``get_ne_from_nH`` and ``get_scales`` keep the
+signatures of pyrh's (pyrh.pyx:396-533); ``hse_batch`` is hse's batched
+form.  Parsed inputs, models and device contexts are
 cached per run directory and wavelength grid, so repeated calls do not
 re-read files (the reference re-reads everything on every call,
 rh/rhf1d/pyrh_compute1dray.c:130-301).
@@ -134,6 +135,56 @@ def hse_batch(cwd, atm_scale, scale, temp, pg_top=0.1, ndep=None, fudge_wave=Non
     e, hm = hse_engine_for(cwd, fudge_wave, fudge_value, atomic_number, atomic_abundance, device)
     ne, nH, rho, pg, it = e.hse(hm, scale, temp, pg_top, atm_scale, ndep)
     return {"ne": ne, "nHtot": nH, "rho": rho, "pg": pg, "niter": it}
+
+
+def get_ne_from_nH(cwd, atm_scale, scale, temperature, nH):
+    """pyrh.get_ne_from_nH (pyrh.pyx:396-423, rh/rhf1d/pyrh_hse.c:555-678):
+    the electron density [cm^-3] of LTE ionisation at every depth from the
+    temperature [K] and n_Htot [cm^-3].  The depth scale does not enter
+    (the reference sets it up and does not use it)."""
+    temperature = np.ascontiguousarray(temperature, np.float64)
+    nH = np.ascontiguousarray(nH, np.float64)
+    if temperature.ndim != 1 or nH.shape != temperature.shape or np.size(scale) != nH.size:
+        raise ValueError("scale, temperature and nH must be 1-D arrays of the same length")
+    e, hm = hse_engine_for(cwd)
+    return e.ne_from_nh(hm, temperature[None], nH[None])[0]
+
+
+_SCALES = {}
+
+
+def scales_engine_for(cwd, lam_ref, atomic_number=None, atomic_abundance=None, device=0):
+    """Engine for pyrh's get_scales: the background model of the run
+    directory on the single wavelength lam_ref, which is also its reference
+    wavelength (pyrh_hse.c:475-479; Kurucz lines are read,
+    rh/background.c:288)."""
+    skey, s = _setup(cwd, atomic_number, atomic_abundance, None, None, None, None)
+    key = (skey, float(lam_ref), device)
+    e = _SCALES.get(key)
+    if e is None:
+        if any(a.active for a in s.atoms):
+            raise NotImplementedError("get_scales with an ACTIVE atom in the atom list")
+        m = _model.build_model(s, np.array([float(lam_ref)]), check=False, lam_ref=lam_ref)
+        e = Engine(m, device)
+        _SCALES[key] = e
+    return e
+
+
+def get_scales(cwd, atm_scale, scale, atmosphere, lam_ref, atomic_number=None,
+               atomic_abundance=None):
+    """pyrh.get_scales (pyrh.pyx:491-533, rh/rhf1d/pyrh_hse.c:402-553):
+    tau (at lam_ref), height [m] and column mass [kg m^-2] of an atmosphere
+    (rows as compute1d's; T, n_e, v_LOS, v_mic and n_H are read) given on
+    `scale`.  The input scale is returned converted to SI units, where the
+    reference leaves that array unset."""
+    atm = np.array(atmosphere, np.float64)
+    if atm.ndim != 2 or atm.shape[0] < 9:
+        raise ValueError("atmosphere must be [>=9, ndep]")
+    atm = np.ascontiguousarray(atm[:9])
+    atm[0] = np.asarray(scale, np.float64)
+    e = scales_engine_for(cwd, lam_ref, atomic_number, atomic_abundance)
+    tau, h, m = e.scales(atm[None], atm_scale)
+    return tau[0], h[0], m[0]
 
 
 def has_active_atom(cwd):

@@ -75,11 +75,13 @@ struct rhg_context {
   DevBuf srt_atm, srt_ndep, srt_perm, srt_out, srt_rf;
   double nlte_ms[RHG_NLTE_NTIMING]{};
   // hydrostatic equilibrium: element tables, inputs, outputs, timing
-  DevBuf hs_nstage, hs_ptr, hs_logpf, hs_ionpot, hs_abund, hs_in, hs_ndep, hs_out, hs_niter;
+  DevBuf hs_nstage, hs_ptr, hs_logpf, hs_ionpot, hs_abund, hs_in, hs_ndep, hs_out, hs_niter,
+      hs_scr;
   float hse_ms = 0.0f;
+  bool ref_has_line = false;   // a Kurucz line covers the reference wavelength
   int hse_lpc = 0;
   // host copies of model tables the NLTE checks need
-  std::vector<int32_t> h_atom_level_ptr, h_nucl_atom;
+  std::vector<int32_t> h_atom_level_ptr, h_nucl_atom, h_level_stage;
   hipEvent_t ev[RHG_NSTAGE + 2]{};
   float last_ms[RHG_NSTAGE + 1]{};
   // the last batch, for rhg_count_work
@@ -135,7 +137,9 @@ int rhg_open(const rhg_model_t *m, int device, rhg_context_t **out) {
   if (!m || !out) return fail(RHG_EINVAL, "null argument");
   *out = nullptr;
   if (m->abi_version != RHG_ABI_VERSION) return fail(RHG_EINVAL, "model ABI version mismatch");
-  if (m->nspect < 2 || m->ref_index < 0 || m->ref_index >= m->nspect)
+  // a grid of the reference wavelength alone serves the hydrostatic
+  // builder; the synthesis entry points need an output wavelength
+  if (m->nspect < 1 || m->ref_index < 0 || m->ref_index >= m->nspect)
     return fail(RHG_EINVAL, "bad wavelength grid");
   if (m->magneto_optical) return fail(RHG_EUNSUPPORTED, "magneto-optical terms are not implemented");
   if (m->natom < 1 || m->natom > RHG_MAX_ATOM) return fail(RHG_EUNSUPPORTED, "too many model atoms");
@@ -323,16 +327,19 @@ int rhg_open(const rhg_model_t *m, int device, rhg_context_t **out) {
 
   c->h_atom_level_ptr.assign(m->atom_level_ptr, m->atom_level_ptr + m->natom + 1);
   c->h_nucl_atom.assign(m->nucl_atom, m->nucl_atom + m->nnuclei);
+  c->ref_has_line = m->lam_line_ptr[m->ref_index + 1] > m->lam_line_ptr[m->ref_index];
+  c->h_level_stage.assign(m->level_stage, m->level_stage + m->atom_level_ptr[m->natom]);
   c->nspect = ns;
   c->nout = nout;
   c->ncont = m->ncont;
   c->nline = m->nline;
   c->natom = m->natom;
-  if (hipMalloc(&c->d_out_ls, ols.size() * sizeof(int32_t)) != hipSuccess) {
+  if (hipMalloc(&c->d_out_ls, std::max<size_t>(ols.size(), 1) * sizeof(int32_t)) != hipSuccess) {
     rhg_close(c);
     return fail(RHG_ENOMEM, "hipMalloc (wavelength index)");
   }
-  if (hipMemcpy(c->d_out_ls, ols.data(), ols.size() * sizeof(int32_t), hipMemcpyHostToDevice) !=
+  if (!ols.empty() &&
+      hipMemcpy(c->d_out_ls, ols.data(), ols.size() * sizeof(int32_t), hipMemcpyHostToDevice) !=
       hipSuccess) {
     rhg_close(c);
     return fail(RHG_EDEVICE, "hipMemcpy (wavelength index)");
@@ -368,6 +375,7 @@ static int synth(rhg_context_t *c, const rhg_columns_t *cols, int32_t precision,
                  int32_t device_ptrs, double *stokes_out, double *height_out,
                  double *chiref_out) {
   if (!c || !cols || !stokes_out) return fail(RHG_EINVAL, "null argument");
+  if (c->nout < 1) return fail(RHG_EINVAL, "the model has no output wavelength");
   if (precision != RHG_PREC_FP64 && precision != RHG_PREC_FP32)
     return fail(RHG_EINVAL, "precision must be RHG_PREC_FP64 or RHG_PREC_FP32");
   const int ncol = cols->ncol, Nmax = cols->ndep_max, nout = c->nout;
@@ -844,6 +852,7 @@ extern "C" {
 int rhg_rf_lte(rhg_context_t *c, const rhg_columns_t *cols, int32_t precision, int32_t device_ptrs,
                double *stokes_out, void *rf_out) {
   if (!c || !cols || !stokes_out || !rf_out) return fail(RHG_EINVAL, "null argument");
+  if (c->nout < 1) return fail(RHG_EINVAL, "the model has no output wavelength");
   if (precision != RHG_PREC_FP64 && precision != RHG_PREC_FP32)
     return fail(RHG_EINVAL, "precision must be RHG_PREC_FP64 or RHG_PREC_FP32");
   const int ncol = cols->ncol, Nmax = cols->ndep_max;
@@ -903,6 +912,7 @@ int rhg_rf_loggf(rhg_context_t *c, const rhg_columns_t *cols, int32_t npar,
                  double *rf_out) {
   if (!c || !cols || !stokes_out || !rf_out || (npar > 0 && !par_line))
     return fail(RHG_EINVAL, "null argument");
+  if (c->nout < 1) return fail(RHG_EINVAL, "the model has no output wavelength");
   if (npar < 1 || npar > RHG_RF_MAX_ATOMIC)
     return fail(RHG_EINVAL, "npar must be in [1, RHG_RF_MAX_ATOMIC]");
   for (int p = 0; p < npar; p++)
@@ -1096,6 +1106,7 @@ int rhg_nlte_solve(rhg_context_t *c, const rhg_nlte_t *a, const rhg_columns_t *c
                    double *dpops_out, double *stokes_out, double *nstar_out) {
   if (!c || !a || !cols || !pops_out || !niter_out || !dpops_out)
     return fail(RHG_EINVAL, "null argument");
+  if (c->nout < 1) return fail(RHG_EINVAL, "the model has no output wavelength");
   const int ncol_all = cols->ncol, Nmax = cols->ndep_max, nout = c->nout, nl = a->nlevel;
   if (ncol_all < 1 || Nmax < 4 || Nmax > RHG_MAX_DEPTH) return fail(RHG_EINVAL, "bad batch shape");
   if (!(cols->mu > 0.0 && cols->mu <= 1.0)) return fail(RHG_EINVAL, "mu must be in (0, 1]");
@@ -1429,6 +1440,54 @@ int rhg_nlte_solve(rhg_context_t *c, const rhg_nlte_t *a, const rhg_columns_t *c
   return 0;
 }
 
+// Validates the element tables of an rhg_hse_t, uploads them on the
+// context's stream and fills H (without its scratch).
+static int hse_tables(rhg_context_t *c, const rhg_hse_t *h, rhg::DevHse &H) {
+  const rhg::DevModel &dm = c->dm;
+  if (h->nel < 1 || !h->el_nstage || !h->el_pf_ptr || !h->el_logpf || !h->el_ionpot || !h->el_abund)
+    return fail(RHG_EINVAL, "empty element table");
+  if (h->lam_index < 0 || h->lam_index >= c->nspect) return fail(RHG_EINVAL, "lam_index out of range");
+  if (h->he_atom >= dm.natom) return fail(RHG_EINVAL, "he_atom out of range");
+  if (h->nmax_iter < 1) return fail(RHG_EINVAL, "nmax_iter must be positive");
+  // the element tables are packed: element e's stages start where e-1's end
+  int64_t npfel = 0;
+  for (int e = 0; e < h->nel; e++) {
+    if (h->el_nstage[e] < 1 || h->el_nstage[e] > rhg::NSTAGE_MAX)
+      return fail(RHG_EUNSUPPORTED, "element with too many ionisation stages");
+    if (h->el_pf_ptr[e] != npfel) return fail(RHG_EINVAL, "element tables are not packed");
+    npfel += (int64_t) h->el_nstage[e] * dm.npf;
+  }
+  if (c->hs_nstage.ensure(h->nel * sizeof(int32_t)) || c->hs_ptr.ensure(h->nel * sizeof(int32_t)) ||
+      c->hs_logpf.ensure(npfel * sizeof(double)) || c->hs_ionpot.ensure(npfel * sizeof(double)) ||
+      c->hs_abund.ensure(h->nel * sizeof(double)))
+    return fail(RHG_ENOMEM, "device allocation for the element tables failed");
+  HIPCHECK(hipSetDevice(c->device));
+  HIPCHECK(hipMemcpyAsync(c->hs_nstage.p, h->el_nstage, h->nel * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(c->hs_ptr.p, h->el_pf_ptr, h->nel * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(c->hs_logpf.p, h->el_logpf, npfel * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(c->hs_ionpot.p, h->el_ionpot, npfel * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(c->hs_abund.p, h->el_abund, h->nel * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  H.nel = h->nel;
+  H.el_nstage = (const int32_t *) c->hs_nstage.p;
+  H.el_pf_ptr = (const int32_t *) c->hs_ptr.p;
+  H.el_logpf = (const double *) c->hs_logpf.p;
+  H.el_ionpot = (const double *) c->hs_ionpot.p;
+  H.el_abund = (const double *) c->hs_abund.p;
+  H.lam_index = h->lam_index;
+  H.he_atom = h->he_atom;
+  H.ray_H = h->ray_H;
+  H.ray_He = h->ray_He;
+  H.ray_H2 = h->ray_H2;
+  H.sigma_thomson = h->sigma_thomson;
+  H.hmin_fudge = h->hmin_fudge;
+  H.scatt_fudge = h->scatt_fudge;
+  H.nmax_iter = h->nmax_iter;
+  H.iter_limit = h->iter_limit;
+  H.scr = nullptr;
+  H.nslot_stage = 0;
+  return 0;
+}
+
 int rhg_hse(rhg_context_t *c, const rhg_hse_t *h, int32_t ncol, int32_t Nmax, const int32_t *ndep,
             int32_t atm_scale, const double *scale, const double *temp, const double *pg_top,
             int32_t device_ptrs, double *ne_out, double *nHtot_out, double *rho_out,
@@ -1446,19 +1505,6 @@ int rhg_hse(rhg_context_t *c, const rhg_hse_t *h, int32_t ncol, int32_t Nmax, co
   if (dm.nnuclei < 1 || dm.nnuclei > 8) return fail(RHG_EUNSUPPORTED, "number of molecule nuclei out of range");
   if (dm.natom < 1 || c->h_nucl_atom.empty() || c->h_nucl_atom[0] != 0)
     return fail(RHG_EINVAL, "hydrogen must be the first atom and the first nucleus");
-  if (h->nel < 1 || !h->el_nstage || !h->el_pf_ptr || !h->el_logpf || !h->el_ionpot || !h->el_abund)
-    return fail(RHG_EINVAL, "empty element table");
-  if (h->lam_index < 0 || h->lam_index >= c->nspect) return fail(RHG_EINVAL, "lam_index out of range");
-  if (h->he_atom >= dm.natom) return fail(RHG_EINVAL, "he_atom out of range");
-  if (h->nmax_iter < 1) return fail(RHG_EINVAL, "nmax_iter must be positive");
-  // the element tables are packed: element e's stages start where e-1's end
-  int64_t npfel = 0;
-  for (int e = 0; e < h->nel; e++) {
-    if (h->el_nstage[e] < 1 || h->el_nstage[e] > rhg::NSTAGE_MAX)
-      return fail(RHG_EUNSUPPORTED, "element with too many ionisation stages");
-    if (h->el_pf_ptr[e] != npfel) return fail(RHG_EINVAL, "element tables are not packed");
-    npfel += (int64_t) h->el_nstage[e] * dm.npf;
-  }
   std::vector<int32_t> hndep(ncol);
   if (device_ptrs) {
     HIPCHECK(hipMemcpy(hndep.data(), ndep, ncol * sizeof(int32_t), hipMemcpyDeviceToHost));
@@ -1470,36 +1516,12 @@ int rhg_hse(rhg_context_t *c, const rhg_hse_t *h, int32_t ncol, int32_t Nmax, co
   HIPCHECK(hipSetDevice(c->device));
 
   const size_t col_b = (size_t) ncol * Nmax * sizeof(double);
-  if (c->hs_nstage.ensure(h->nel * sizeof(int32_t)) || c->hs_ptr.ensure(h->nel * sizeof(int32_t)) ||
-      c->hs_logpf.ensure(npfel * sizeof(double)) || c->hs_ionpot.ensure(npfel * sizeof(double)) ||
-      c->hs_abund.ensure(h->nel * sizeof(double)) ||
-      (!device_ptrs && (c->hs_in.ensure(2 * col_b + ncol * sizeof(double)) ||
-                        c->hs_ndep.ensure(ncol * sizeof(int32_t)) || c->hs_out.ensure(4 * col_b) ||
-                        (niter_out && c->hs_niter.ensure((size_t) ncol * Nmax * sizeof(int32_t))))))
-    return fail(RHG_ENOMEM, "device allocation for the batch failed");
-  HIPCHECK(hipMemcpyAsync(c->hs_nstage.p, h->el_nstage, h->nel * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
-  HIPCHECK(hipMemcpyAsync(c->hs_ptr.p, h->el_pf_ptr, h->nel * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
-  HIPCHECK(hipMemcpyAsync(c->hs_logpf.p, h->el_logpf, npfel * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  HIPCHECK(hipMemcpyAsync(c->hs_ionpot.p, h->el_ionpot, npfel * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  HIPCHECK(hipMemcpyAsync(c->hs_abund.p, h->el_abund, h->nel * sizeof(double), hipMemcpyHostToDevice, c->stream));
   rhg::DevHse H;
-  H.nel = h->nel;
-  H.el_nstage = (const int32_t *) c->hs_nstage.p;
-  H.el_pf_ptr = (const int32_t *) c->hs_ptr.p;
-  H.el_logpf = (const double *) c->hs_logpf.p;
-  H.el_ionpot = (const double *) c->hs_ionpot.p;
-  H.el_abund = (const double *) c->hs_abund.p;
-  H.lam_index = h->lam_index;
-  H.he_atom = h->he_atom;
-  H.ray_H = h->ray_H;
-  H.ray_He = h->ray_He;
-  H.ray_H2 = h->ray_H2;
-  H.sigma_thomson = h->sigma_thomson;
-  H.hmin_fudge = h->hmin_fudge;
-  H.scatt_fudge = h->scatt_fudge;
-  H.nmax_iter = h->nmax_iter;
-  H.iter_limit = h->iter_limit;
-
+  if (int rc = hse_tables(c, h, H)) return rc;
+  if (!device_ptrs && (c->hs_in.ensure(2 * col_b + ncol * sizeof(double)) ||
+                       c->hs_ndep.ensure(ncol * sizeof(int32_t)) || c->hs_out.ensure(4 * col_b) ||
+                       (niter_out && c->hs_niter.ensure((size_t) ncol * Nmax * sizeof(int32_t)))))
+    return fail(RHG_ENOMEM, "device allocation for the batch failed");
   const double *d_scale = scale, *d_temp = temp, *d_pgt = pg_top;
   const int32_t *d_ndep = ndep;
   double *d_ne = ne_out, *d_nH = nHtot_out, *d_rho = rho_out, *d_pg = pg_out;
@@ -1523,9 +1545,9 @@ int rhg_hse(rhg_context_t *c, const rhg_hse_t *h, int32_t ncol, int32_t Nmax, co
     d_niter = niter_out ? (int32_t *) c->hs_niter.p : nullptr;
   }
   // lanes per column: enough lane groups to give every SIMD of the chip a
-  // few waves (256 CUs x 4 SIMDs x 64 lanes x 2), at most 16
+  // wave (256 CUs x 4 SIMDs x 64 lanes), at most 16
   int lpc = 1;
-  while (lpc < 16 && (int64_t) ncol * lpc < 131072) lpc <<= 1;
+  while (lpc < 16 && (int64_t) ncol * lpc < 65536) lpc <<= 1;
   if (const char *lanes_env = getenv("RHG_HSE_LANES")) {
     const int v = atoi(lanes_env);
     if (v == 1 || v == 2 || v == 4 || v == 8 || v == 16) lpc = v;
@@ -1533,6 +1555,31 @@ int rhg_hse(rhg_context_t *c, const rhg_hse_t *h, int32_t ncol, int32_t Nmax, co
   c->hse_lpc = lpc;
   const int64_t nthr = (int64_t) ncol * lpc;
   dim3 blk(rhg::HSE_BLOCK), grd((unsigned) ((nthr + rhg::HSE_BLOCK - 1) / rhg::HSE_BLOCK));
+  // stage-factor slots: the most stages (beyond the first) any group lane
+  // handles, its elements being sub, sub + lpc, ...
+  int nslot = 1;
+  for (int sub = 0; sub < lpc; sub++) {
+    int n = 0;
+    for (int e = sub; e < h->nel; e += lpc) n += h->el_nstage[e] - 1;
+    nslot = std::max(nslot, n);
+  }
+  // the level groups: every molecule nucleus and He, ionisation differences
+  // below HSE_NDZ
+  for (int i = 0; i <= dm.nnuclei; i++) {
+    const int ia = i < dm.nnuclei ? c->h_nucl_atom[i] : h->he_atom;
+    if (ia < 0) continue;
+    const int l0 = c->h_atom_level_ptr[ia], l1 = c->h_atom_level_ptr[ia + 1];
+    for (int l = l0; l < l1; l++)
+      if (c->h_level_stage[l] - c->h_level_stage[l0] >= rhg::HSE_NDZ)
+        return fail(RHG_EUNSUPPORTED, "model atom with levels four or more ionisation stages "
+                                      "above its ground level");
+  }
+  const int64_t nlaunch = (int64_t) grd.x * rhg::HSE_BLOCK;
+  const int64_t nslot_all = nslot + (int64_t) (dm.nnuclei + 1) * (rhg::HSE_NDZ + 1);
+  if (c->hs_scr.ensure((size_t) nslot_all * nlaunch * sizeof(double)))
+    return fail(RHG_ENOMEM, "device allocation for the layer tables failed");
+  H.scr = (double *) c->hs_scr.p;
+  H.nslot_stage = nslot;
   const size_t lds = (size_t) 3 * dm.nmol * rhg::HSE_BLOCK * sizeof(double);
   HIPCHECK(hipEventRecord(c->ev[0], c->stream));
   switch (dm.nnuclei) {
@@ -1558,6 +1605,106 @@ int rhg_hse(rhg_context_t *c, const rhg_hse_t *h, int32_t ncol, int32_t Nmax, co
   }
   HIPCHECK(hipStreamSynchronize(c->stream));
   HIPCHECK(hipEventElapsedTime(&c->hse_ms, c->ev[0], c->ev[1]));
+  return 0;
+}
+
+int rhg_ne_from_nh(rhg_context_t *c, const rhg_hse_t *h, int32_t ncol, int32_t Nmax,
+                   const int32_t *ndep, const double *temp, const double *nH,
+                   int32_t device_ptrs, double *ne_out) {
+  if (!c || !h || !ndep || !temp || !nH || !ne_out) return fail(RHG_EINVAL, "null argument");
+  if (ncol < 1 || Nmax < 1 || Nmax > RHG_MAX_DEPTH) return fail(RHG_EINVAL, "bad batch shape");
+  if ((int64_t) ncol * Nmax > (int64_t) INT32_MAX / 16) return fail(RHG_EINVAL, "batch too large");
+  std::vector<int32_t> hndep(ncol);
+  if (device_ptrs) {
+    HIPCHECK(hipMemcpy(hndep.data(), ndep, ncol * sizeof(int32_t), hipMemcpyDeviceToHost));
+  } else {
+    memcpy(hndep.data(), ndep, ncol * sizeof(int32_t));
+  }
+  for (int i = 0; i < ncol; i++)
+    if (hndep[i] < 1 || hndep[i] > Nmax) return fail(RHG_EINVAL, "ndep out of range");
+  HIPCHECK(hipSetDevice(c->device));
+  rhg::DevHse H;
+  if (int rc = hse_tables(c, h, H)) return rc;
+  const size_t col_b = (size_t) ncol * Nmax * sizeof(double);
+  const double *d_T = temp, *d_nH = nH;
+  const int32_t *d_ndep = ndep;
+  double *d_ne = ne_out;
+  if (!device_ptrs) {
+    if (c->hs_in.ensure(2 * col_b) || c->hs_ndep.ensure(ncol * sizeof(int32_t)) ||
+        c->hs_out.ensure(col_b))
+      return fail(RHG_ENOMEM, "device allocation for the batch failed");
+    double *in = (double *) c->hs_in.p;
+    HIPCHECK(hipMemcpyAsync(in, temp, col_b, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(in + (size_t) ncol * Nmax, nH, col_b, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(c->hs_ndep.p, ndep, ncol * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    d_T = in;
+    d_nH = in + (size_t) ncol * Nmax;
+    d_ndep = (const int32_t *) c->hs_ndep.p;
+    d_ne = (double *) c->hs_out.p;
+  }
+  const int64_t npts = (int64_t) ncol * Nmax;
+  dim3 blk(rhg::HSE_BLOCK), grd((unsigned) ((npts + rhg::HSE_BLOCK - 1) / rhg::HSE_BLOCK));
+  hipLaunchKernelGGL(rhg::k_ne_from_nh, grd, blk, 0, c->stream, c->d_model, H, ncol, Nmax, d_ndep,
+                     d_T, d_nH, d_ne);
+  HIPCHECK(hipGetLastError());
+  if (!device_ptrs) HIPCHECK(hipMemcpyAsync(ne_out, d_ne, col_b, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int rhg_scales(rhg_context_t *c, const rhg_columns_t *cols, int32_t device_ptrs, double *tau_out,
+               double *height_out, double *cmass_out) {
+  if (!c || !cols || !tau_out || !height_out || !cmass_out) return fail(RHG_EINVAL, "null argument");
+  const int ncol = cols->ncol, Nmax = cols->ndep_max;
+  if (ncol < 1 || Nmax < 2 || Nmax > RHG_MAX_DEPTH) return fail(RHG_EINVAL, "bad batch shape");
+  if ((int64_t) ncol * Nmax > (int64_t) INT32_MAX / 16) return fail(RHG_EINVAL, "batch too large");
+  if (cols->atm_scale < 0 || cols->atm_scale > 2) return fail(RHG_EINVAL, "bad atm_scale");
+  if (c->ref_has_line)
+    return fail(RHG_EUNSUPPORTED, "a Kurucz line at the reference wavelength: its opacity in "
+                                  "the depth scales depends on the ray in moving atmospheres");
+  if (c->dm.nnuclei < 1 || c->dm.nnuclei > 8)
+    return fail(RHG_EUNSUPPORTED, "number of molecule nuclei out of range");
+  std::vector<int32_t> hndep(ncol);
+  if (device_ptrs) {
+    HIPCHECK(hipMemcpy(hndep.data(), cols->ndep, ncol * sizeof(int32_t), hipMemcpyDeviceToHost));
+  } else {
+    memcpy(hndep.data(), cols->ndep, ncol * sizeof(int32_t));
+  }
+  for (int i = 0; i < ncol; i++)
+    if (hndep[i] < 2 || hndep[i] > Nmax) return fail(RHG_EINVAL, "ndep out of range");
+  HIPCHECK(hipSetDevice(c->device));
+  const int nfield = rhg::state_nfield(c->ncont, c->nline);
+  const size_t atm_b = (size_t) ncol * 9 * Nmax * sizeof(double);
+  const size_t col_b = (size_t) ncol * Nmax * sizeof(double);
+  if (c->state.ensure((size_t) ncol * nfield * Nmax * sizeof(double)) ||
+      (!device_ptrs && (c->atmos.ensure(atm_b) || c->ndep.ensure(ncol * sizeof(int32_t)) ||
+                        c->hs_out.ensure(3 * col_b))))
+    return fail(RHG_ENOMEM, "device allocation for the batch failed");
+  const double *d_atm = cols->atmos;
+  const int32_t *d_ndep = cols->ndep;
+  double *d_tau = tau_out, *d_h = height_out, *d_m = cmass_out;
+  if (!device_ptrs) {
+    HIPCHECK(hipMemcpyAsync(c->atmos.p, cols->atmos, atm_b, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(c->ndep.p, cols->ndep, ncol * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    d_atm = (const double *) c->atmos.p;
+    d_ndep = (const int32_t *) c->ndep.p;
+    d_tau = (double *) c->hs_out.p;
+    d_h = d_tau + (size_t) ncol * Nmax;
+    d_m = d_h + (size_t) ncol * Nmax;
+  }
+  double *state = (double *) c->state.p;
+  if (int rc = launch_state(c, d_atm, d_ndep, ncol, Nmax, 1.0, cols->atm_scale, state, nfield))
+    return rc;
+  dim3 blk(64), grd((unsigned) ((ncol + 63) / 64));
+  hipLaunchKernelGGL(rhg::k_scales, grd, blk, 0, c->stream, c->d_model, d_ndep, ncol, Nmax,
+                     cols->atm_scale, state, nfield, d_tau, d_h, d_m);
+  HIPCHECK(hipGetLastError());
+  if (!device_ptrs) {
+    HIPCHECK(hipMemcpyAsync(tau_out, d_tau, col_b, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipMemcpyAsync(height_out, d_h, col_b, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipMemcpyAsync(cmass_out, d_m, col_b, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHECK(hipStreamSynchronize(c->stream));
   return 0;
 }
 

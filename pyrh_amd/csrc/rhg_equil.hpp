@@ -106,7 +106,8 @@ __device__ __forceinline__ void small_solve(double (&a)[NN][NN], double (&b)[NN]
 template <int NN>
 __device__ void chem_eq(const DevModel &M, double T, double ne, double nHtot,
                         const double (&fn0)[NN], double (&nnuc)[NN], double *lds_phi,
-                        double *lds_nmol, double *lds_saha, int ls_stride, double &PhiHmin_out) {
+                        double *lds_nmol, double *lds_saha, int ls_stride, double &PhiHmin_out,
+                        bool phi_ready = false) {
   const int Nm = M.nmol;
   const double CI = (HPLANCK / (2.0 * PI * M_ELECTRON)) * (HPLANCK / KBOLTZMANN);
   double a[NN];
@@ -114,8 +115,10 @@ __device__ void chem_eq(const DevModel &M, double T, double ne, double nHtot,
   for (int i = 0; i < NN; i++) a[i] = M.nucl_abund[i] * nHtot;
   double PhiHmin = 0.25 * pow(CI / T, 1.5) * exp(0.754 * EV / (KBOLTZMANN * T));
   double fHmin = ne * fn0[0] * PhiHmin;
+  // phi_ready: lds_phi already holds the constants at this T (a caller
+  // that solves the same temperature again)
   for (int m = 0; m < Nm; m++) {
-    lds_phi[m * ls_stride] = equilconstant(M, m, T);
+    if (!phi_ready) lds_phi[m * ls_stride] = equilconstant(M, m, T);
     lds_nmol[m * ls_stride] = 0.0;
   }
 #pragma unroll

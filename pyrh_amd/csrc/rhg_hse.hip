@@ -18,19 +18,16 @@ namespace {
 constexpr int HSE_N_MAX_ELECTRON_ITER = 10;  // rh/rhf1d/pyrh_background.h:1-2
 constexpr double HSE_MAX_ELECTRON_ERROR = 1.0E-2;
 
-// get_ne(fromscratch = TRUE) with hydrogen in LTE: the starting value from
-// hydrogen ionisation with U = 0 (pyrh_background.c:489-502), then Newton
-// steps on charge conservation over every element's LTE ionisation
-// fractions (getfjk, rh/solvene.c:145-197, partition functions by
-// getKuruczpf = Linear on Tpf, rh/solvene.c:202-211) and H-.  The group's
-// lanes take elements sub, sub + lpc, ...; lane 0 starts the error from
-// n_e/n_Htot, so with lpc = 1 the sums run in the reference's order.
-__device__ double hse_get_ne(const DevModel &M, const DevHse &H, double T, double nHtot, int sub,
-                             int lpc) {
-  const double C1 = (HPLANCK / (2.0 * PI * M_ELECTRON)) * (HPLANCK / KBOLTZMANN);
+// Stage factors of the LTE ionisation fractions of the group lane's
+// elements at temperature T: r = exp(U_j - U_{j-1} - chi_{j-1}/kT) per
+// stage j >= 1 (getfjk, rh/solvene.c:180-190; U from getKuruczpf = Linear
+// on Tpf, rh/solvene.c:202-211).  They depend on T alone, so each layer
+// computes them once into the lane's scratch slots (sc, stride ss) and
+// every Newton step of every pressure iteration of the layer reads them.
+__device__ void hse_stage_factors(const DevModel &M, const DevHse &H, double T, int sub, int lpc,
+                                  double *sc, int64_t ss) {
   const double kT = KBOLTZMANN * T;
   const int npf = M.npf;
-  // Linear (rh/linear.c:22-54) at T, shared by every partition function
   int jT = 0, edge = 0;
   double fx = 0.0;
   if (T <= M.Tpf[0]) edge = -1;
@@ -44,35 +41,128 @@ __device__ double hse_get_ne(const DevModel &M, const DevHse &H, double T, doubl
     if (edge > 0) return yt[npf - 1];
     return fx * yt[jT] + (1 - fx) * yt[jT + 1];
   };
-  const double Uk0 = 0.0;
-  const double PhiH = 0.5 * pow(C1 / T, 1.5) * exp(Uk0 + H.el_ionpot[H.el_pf_ptr[0]] / kT);
+  int slot = 0;
+  for (int e = sub; e < H.nel; e += lpc) {
+    const int ns = H.el_nstage[e];
+    const double *lp = H.el_logpf + H.el_pf_ptr[e], *ip = H.el_ionpot + H.el_pf_ptr[e];
+    double Uk = pf(lp);
+    for (int j = 1; j < ns; j++, slot++) {
+      const double Ukp1 = pf(lp + j * npf);
+      if (slot < H.nslot_stage) sc[slot * ss] = exp(Ukp1 - Uk - ip[j - 1] / kT);
+      Uk = Ukp1;
+    }
+  }
+}
+
+// Saha-Boltzmann level sums of one model atom at temperature T, grouped by
+// the ionisation difference dZ to the ground level: g[d] = sum over levels
+// i >= 1 with dZ_i = d of g_i/g_0 exp(-(E_i - E_0)/kT) (rh/ltepops.c:87-100),
+// d < HSE_NDZ, and g[HSE_NDZ] the same sum over the neutral levels before
+// the first ionised one (the neutral fraction of rh/chemequil.c:222-236).
+// The LTE normalisation is then 1 + sum_d g[d] / (cNe_T)^d for any n_e.
+__device__ void hse_level_groups(const DevModel &M, int ia, double T, double *g, int64_t ss) {
+  const int l0 = M.atom_level_ptr[ia], nl = M.atom_level_ptr[ia + 1] - l0;
+  double acc[HSE_NDZ + 1];
+#pragma unroll
+  for (int d = 0; d <= HSE_NDZ; d++) acc[d] = 0.0;
+  bool neutral = M.level_stage[l0] == 0;
+  for (int i = 1; i < nl; i++) {
+    const double dE = M.level_E[l0 + i] - M.level_E[l0];
+    const double r = (M.level_g[l0 + i] / M.level_g[l0]) * exp(-(dE / (KBOLTZMANN * T)));
+    const int dZ = M.level_stage[l0 + i] - M.level_stage[l0];
+    if (M.level_stage[l0 + i] > 0) neutral = false;
+#pragma unroll
+    for (int d = 0; d < HSE_NDZ; d++)
+      if (dZ == d) acc[d] += r;
+    if (neutral) acc[HSE_NDZ] += r;
+  }
+#pragma unroll
+  for (int d = 0; d <= HSE_NDZ; d++) g[d * ss] = acc[d];
+}
+
+// 1 + sum_d g[d] / cNe_T^d
+__device__ __forceinline__ double hse_level_sum(const double *g, int64_t ss, double cNe_T) {
+  const double ci = 1.0 / cNe_T;
+  double sum = 1.0, p = 1.0;
+#pragma unroll
+  for (int d = 0; d < HSE_NDZ; d++) {
+    sum += g[d * ss] * p;
+    p *= ci;
+  }
+  return sum;
+}
+
+// get_ne(fromscratch = TRUE) with hydrogen in LTE: the starting value from
+// hydrogen ionisation with U = 0 (pyrh_background.c:489-502), then Newton
+// steps on charge conservation over every element's LTE ionisation
+// fractions (getfjk, rh/solvene.c:145-197, with the stage factors of
+// hse_stage_factors) and H-.  The group's lanes take elements sub,
+// sub + lpc, ...; lane 0 starts the error from n_e/n_Htot, so with lpc = 1
+// the sums run in the reference's order.  The quotients by sum1 and n_e
+// are products with their reciprocals (a last-place difference).
+// CACHED = false: the stage factors are formed in place (one (column,
+// depth) point per lane, get_ne_from_nH), and Uk0 is the starting value's
+// partition function (0 in hse(), that of neutral hydrogen in Solve_ne,
+// rh/solvene.c:78-88).
+template <bool CACHED>
+__device__ double hse_get_ne(const DevModel &M, const DevHse &H, double T, double nHtot, int sub,
+                             int lpc, const double *sc, int64_t ss, double Uk0 = 0.0) {
+  const double C1 = (HPLANCK / (2.0 * PI * M_ELECTRON)) * (HPLANCK / KBOLTZMANN);
+  const double kT = KBOLTZMANN * T;
+  const int npf = M.npf;
+  int jT = 0, edge = 0;
+  double fx = 0.0;
+  if (!CACHED) {
+    if (T <= M.Tpf[0]) edge = -1;
+    else if (T >= M.Tpf[npf - 1]) edge = 1;
+    else {
+      jT = locate_asc(npf, M.Tpf, T);
+      fx = (M.Tpf[jT + 1] - T) / (M.Tpf[jT + 1] - M.Tpf[jT]);
+    }
+  }
+  auto pf = [&](const double *yt) -> double {
+    if (edge < 0) return yt[0];
+    if (edge > 0) return yt[npf - 1];
+    return fx * yt[jT] + (1 - fx) * yt[jT + 1];
+  };
+  const double PhiH = 0.5 * pow(C1 / T, 1.5) * exp(Uk0 + H.el_ionpot[0] / kT);
   double ne_old = (sqrt(1.0 + 4.0 * nHtot * PhiH) - 1.0) / (2.0 * PhiH), ne = ne_old;
   const double PhiHmin = 0.25 * pow(C1 / T, 1.5) * exp(0.754 * EV / kT);
   const double CTp = pow(C1 / T, -1.5);
   for (int niter = 0; niter < HSE_N_MAX_ELECTRON_ITER; niter++) {
-    const double CT_ne = 2.0 * CTp / ne_old;
+    const double CT_ne = 2.0 * CTp / ne_old, inv_ne = 1.0 / ne_old;
     double err = (sub == 0) ? ne_old / nHtot : 0.0, sum = 0.0;
+    int slot = 0;
     for (int e = sub; e < H.nel; e += lpc) {
       const int ns = H.el_nstage[e];
-      const double *lp = H.el_logpf + H.el_pf_ptr[e], *ip = H.el_ionpot + H.el_pf_ptr[e];
       double u[NSTAGE_MAX], du[NSTAGE_MAX];
       u[0] = 1.0;
       du[0] = 0.0;
-      double sum1 = 1.0, sum2 = 0.0, Uk = pf(lp);
+      double sum1 = 1.0, sum2 = 0.0;
+      const double *lp = H.el_logpf + H.el_pf_ptr[e], *ip = H.el_ionpot + H.el_pf_ptr[e];
+      double Uk = CACHED ? 0.0 : pf(lp);
 #pragma unroll
       for (int j = 1; j < NSTAGE_MAX; j++) {
         u[j] = du[j] = 0.0;
         if (j < ns) {
-          const double Ukp1 = pf(lp + j * npf);
-          u[j] = u[j - 1] * CT_ne * exp(Ukp1 - Uk - ip[j - 1] / kT);
-          du[j] = (double) (-j) * u[j] / ne_old;
+          double r;
+          if (CACHED) {
+            r = sc[(slot + j - 1) * ss];
+          } else {
+            const double Ukp1 = pf(lp + j * npf);
+            r = exp(Ukp1 - Uk - ip[j - 1] / kT);
+            Uk = Ukp1;
+          }
+          u[j] = u[j - 1] * CT_ne * r;
+          du[j] = (double) (-j) * u[j] * inv_ne;
           sum1 += u[j];
           sum2 += du[j];
-          Uk = Ukp1;
         }
       }
+      slot += ns - 1;
+      const double inv1 = 1.0 / sum1;
       if (e == 0) {
-        const double f0 = u[0] / sum1, df0 = (du[0] - f0 * sum2) / sum1;
+        const double f0 = u[0] * inv1, df0 = (du[0] - f0 * sum2) * inv1;
         err += ne_old * f0 * PhiHmin;
         sum -= (f0 + ne_old * df0) * PhiHmin;
       }
@@ -80,7 +170,7 @@ __device__ double hse_get_ne(const DevModel &M, const DevHse &H, double T, doubl
 #pragma unroll
       for (int j = 1; j < NSTAGE_MAX; j++) {
         if (j < ns) {
-          const double f = u[j] / sum1, df = (du[j] - f * sum2) / sum1, akj = ab * j;
+          const double f = u[j] * inv1, df = (du[j] - f * sum2) * inv1, akj = ab * j;
           err -= akj * f;
           sum += akj * df;
         }
@@ -208,28 +298,28 @@ __device__ double hse_opacity(const DevModel &M, const DevHse &H, int ls, double
 // opacity.
 template <int NN>
 __device__ void hse_layer(const DevModel &M, const DevHse &H, double T, double nHtot, int sub,
-                          int lpc, bool opacity, double *lds_phi, double *lds_nmol,
-                          double *lds_saha, int lstride, double &ne_out, double &opac_out) {
-  const double ne = hse_get_ne(M, H, T, nHtot, sub, lpc);
+                          int lpc, bool opacity, bool first, const double *sc, int64_t ss,
+                          double *lds_phi, double *lds_nmol, double *lds_saha, int lstride,
+                          double &ne_out, double &opac_out) {
+  const double ne = hse_get_ne<true>(M, H, T, nHtot, sub, lpc, sc, ss);
   ne_out = ne;
   if (!opacity) return;
   const double c1 = (HPLANCK / (2.0 * PI * M_ELECTRON)) * (HPLANCK / KBOLTZMANN);
   const double cNe_T = 0.5 * ne * pow(c1 / T, 1.5);
+  // level groups of the nuclei (and He) at this layer's T, after the
+  // stage factors in the lane's slots
+  const double *lg = sc + (int64_t) H.nslot_stage * ss;
   double nstar0n[NN], fn0[NN];
 #pragma unroll
   for (int i = 0; i < NN; i++) {
-    const int ia = M.nucl_atom[i], l0 = M.atom_level_ptr[ia], nl = M.atom_level_ptr[ia + 1] - l0;
+    const int ia = M.nucl_atom[i];
+    const double *g = lg + (int64_t) i * (HSE_NDZ + 1) * ss;
     const double ntot = M.atom_abund[ia] * nHtot;
-    nstar0n[i] = ntot / atom_sum(M, l0, nl, T, cNe_T);
-    double f = 0.0;
-    for (int j = 0; j < nl; j++) {
-      if (M.level_stage[l0 + j] > 0) break;
-      f += (j == 0) ? nstar0n[i] : level_ratio(M, l0, j, T, cNe_T) * nstar0n[i];
-    }
-    fn0[i] = f / ntot;
+    nstar0n[i] = ntot / hse_level_sum(g, ss, cNe_T);
+    fn0[i] = (nstar0n[i] + g[HSE_NDZ * ss] * nstar0n[i]) / ntot;
   }
   double nnuc[NN], PhiHmin;
-  chem_eq<NN>(M, T, ne, nHtot, fn0, nnuc, lds_phi, lds_nmol, lds_saha, lstride, PhiHmin);
+  chem_eq<NN>(M, T, ne, nHtot, fn0, nnuc, lds_phi, lds_nmol, lds_saha, lstride, PhiHmin, !first);
   double fracn[NN];
 #pragma unroll
   for (int i = 0; i < NN; i++) fracn[i] = nnuc[i] / (M.atom_abund[M.nucl_atom[i]] * nHtot);
@@ -242,8 +332,9 @@ __device__ void hse_layer(const DevModel &M, const DevHse &H, double T, double n
   const double nCH = M.mol_ch >= 0 ? lds_nmol[M.mol_ch * lstride] : 0.0;
   double nHe0 = 0.0;
   if (H.he_atom >= 0 && H.ray_He > 0.0) {
-    const int ia = H.he_atom, l0 = M.atom_level_ptr[ia], nl = M.atom_level_ptr[ia + 1] - l0;
-    nHe0 = (M.atom_abund[ia] * nHtot) / atom_sum(M, l0, nl, T, cNe_T);
+    const int ia = H.he_atom;
+    nHe0 = (M.atom_abund[ia] * nHtot) / hse_level_sum(lg + (int64_t) NN * (HSE_NDZ + 1) * ss, ss,
+                                                       cNe_T);
 #pragma unroll
     for (int i = 0; i < NN; i++)
       if (M.nucl_atom[i] == ia) nHe0 = nstar0n[i] * fracn[i];
@@ -258,7 +349,7 @@ __device__ void hse_layer(const DevModel &M, const DevHse &H, double T, double n
 }  // namespace
 
 template <int NN>
-__global__ void __launch_bounds__(HSE_BLOCK)
+__global__ void __launch_bounds__(HSE_BLOCK, 2)
 k_hse(const DevModel *__restrict__ Mp, DevHse H, int lpc, int ncol, int Nmax,
       const int32_t *__restrict__ ndep, int atm_scale, const double *__restrict__ scale,
       const double *__restrict__ temp, const double *__restrict__ pg_top, double *__restrict__ ne_out,
@@ -281,13 +372,29 @@ k_hse(const DevModel *__restrict__ Mp, DevHse H, int lpc, int ncol, int Nmax,
   const double g = M.gravity, LOG10 = log(10.0), wH = AMU * M.wght_per_H;
   const double tA = M.totalAbund;
 
+  // this lane's stage-factor slots
+  const int64_t ss = (int64_t) gridDim.x * blockDim.x;
+  double *sc_t = H.scr + gid;
+
   // top layer, pyrh_hse.c:210-273
   double T = Tc[0], pg = pg_top[ic], ne = 0.0, opac = 0.0, rho = 0.0;
+  auto layer_tables = [&](double Tl) {
+    hse_stage_factors(M, H, Tl, sub, lpc, sc_t, ss);
+    if (!tau) return;
+    double *lg = sc_t + (int64_t) H.nslot_stage * ss;
+#pragma unroll
+    for (int i = 0; i < NN; i++)
+      hse_level_groups(M, M.nucl_atom[i], Tl, lg + (int64_t) i * (HSE_NDZ + 1) * ss, ss);
+    if (H.he_atom >= 0 && H.ray_He > 0.0)
+      hse_level_groups(M, H.he_atom, Tl, lg + (int64_t) NN * (HSE_NDZ + 1) * ss, ss);
+  };
+  layer_tables(T);
   double nH = (pg / KBOLTZMANN / T - ne) / tA;
   int iter = 0;
   while (iter < H.nmax_iter) {
     rho = wH * nH;
-    hse_layer<NN>(M, H, T, nH, sub, lpc, tau, lds_phi, lds_nmol, lds_saha, lstride, ne, opac);
+    hse_layer<NN>(M, H, T, nH, sub, lpc, tau, iter == 0, sc_t, ss, lds_phi, lds_nmol, lds_saha,
+                  lstride, ne, opac);
     const double old = nH;
     nH = (pg / KBOLTZMANN / T - ne) / tA;
     iter++;
@@ -309,13 +416,15 @@ k_hse(const DevModel *__restrict__ Mp, DevHse H, int lpc, int ncol, int Nmax,
     if (tau) deltaP = g * rho_prev / opac_prev * (z - z_prev);
     else deltaP = g * rho_prev * (z_prev - z);
     T = Tc[k];
+    layer_tables(T);
     pg = pg_prev + deltaP;
     nH = (pg / KBOLTZMANN / T - ne_prev) / tA;
     const double dlogtau = tau ? log10(z) - log10(z_prev) : 0.0;
     iter = 0;
     while (iter < H.nmax_iter) {
       rho = wH * nH;
-      hse_layer<NN>(M, H, T, nH, sub, lpc, tau, lds_phi, lds_nmol, lds_saha, lstride, ne, opac);
+      hse_layer<NN>(M, H, T, nH, sub, lpc, tau, iter == 0, sc_t, ss, lds_phi, lds_nmol, lds_saha,
+                  lstride, ne, opac);
       if (tau) {
         const double beta1 = rho_prev / opac_prev * z_prev;
         const double beta2 = rho / opac * z;
@@ -346,6 +455,28 @@ k_hse(const DevModel *__restrict__ Mp, DevHse H, int lpc, int ncol, int Nmax,
       if (niter_out) niter_out[o + k] = 0;
     }
   }
+}
+
+// get_ne_from_nH (rh/rhf1d/pyrh_hse.c:555-678): n_e at every (column,
+// depth) point from T and n_Htot in cm^-3 (Solve_ne from scratch,
+// rh/solvene.c:53-135), one point per lane; n_e returned in cm^-3 as the
+// entry point converts back (and zero past ndep).
+__global__ void __launch_bounds__(HSE_BLOCK)
+k_ne_from_nh(const DevModel *__restrict__ Mp, DevHse H, int ncol, int Nmax,
+             const int32_t *__restrict__ ndep, const double *__restrict__ temp,
+             const double *__restrict__ nH, double *__restrict__ ne_out) {
+  const DevModel &M = *Mp;
+  const int64_t gid = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (int64_t) ncol * Nmax) return;
+  const int ic = (int) (gid / Nmax), k = (int) (gid % Nmax);
+  if (k >= ndep[ic]) {
+    ne_out[gid] = 0.0;
+    return;
+  }
+  const double CUBE_CM = CM_TO_M * CM_TO_M * CM_TO_M;
+  const double T = temp[gid], nHtot = nH[gid] / CUBE_CM;
+  const double Uk = linear1(M.npf, M.Tpf, H.el_logpf, T);
+  ne_out[gid] = hse_get_ne<false>(M, H, T, nHtot, 0, 1, nullptr, 0, Uk) * CUBE_CM;
 }
 
 #define RHG_INST_HSE(NN)                                                                         \

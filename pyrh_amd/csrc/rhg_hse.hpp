@@ -15,9 +15,16 @@ struct DevHse {
   double ray_H, ray_He, ray_H2, sigma_thomson, hmin_fudge, scatt_fudge;
   int nmax_iter;
   double iter_limit;
+  // per-lane scratch of the layer's temperature tables, slot s of thread t
+  // at scr[s * (launched threads) + t]: nslot_stage ionisation stage
+  // factors, then HSE_NDZ + 1 level groups per molecule nucleus and He
+  double *scr;
+  int nslot_stage;
 };
 
 constexpr int HSE_BLOCK = 64;
+// ionisation differences to the ground level the level groups distinguish
+constexpr int HSE_NDZ = 4;
 
 // One column per group of `lpc` consecutive lanes (lpc a power of two up
 // to 16, groups aligned within the wavefront), serial over depth from the
@@ -31,5 +38,9 @@ __global__ void k_hse(const DevModel *Mp, DevHse H, int lpc, int ncol, int Nmax,
                       const int32_t *ndep, int atm_scale, const double *scale, const double *temp,
                       const double *pg_top, double *ne_out, double *nH_out, double *rho_out,
                       double *pg_out, int32_t *niter_out);
+
+__global__ void k_ne_from_nh(const DevModel *Mp, DevHse H, int ncol, int Nmax,
+                             const int32_t *ndep, const double *temp, const double *nH,
+                             double *ne_out);
 
 }  // namespace rhg

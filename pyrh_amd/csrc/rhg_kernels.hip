@@ -422,6 +422,73 @@ __global__ void k_geometry(const DevModel *__restrict__ Mp, const int32_t *__res
 }
 
 // ---------------------------------------------------------------------------
+// pyrh's get_scales (rh/rhf1d/pyrh_hse.c:402-553): convertScales
+// (rh/rhf1d/multiatmos.c:100-177) with all three depth scales kept, from the
+// column state k_state left (the background opacity at the reference
+// wavelength in the height field).  One thread per column; tau, height [m]
+// and cmass [kg m^-2] are [ncol][Nmax], zero past ndep.  The input scale is
+// returned converted to SI (the reference leaves that output unset).
+__global__ void k_scales(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep,
+                         int ncol, int Nmax, int atm_scale, const double *__restrict__ state,
+                         int nfield, double *__restrict__ tau, double *__restrict__ height,
+                         double *__restrict__ cmass) {
+  const DevModel &M = *Mp;
+  const int ic = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ic >= ncol) return;
+  const int N = ndep[ic];
+  const int64_t fs = (int64_t) Nmax * ncol;
+  const double *st0 = state + ic;
+  auto CHI = [&](int k) { return st0[(int64_t) RHG_CF_HEIGHT * fs + (int64_t) k * ncol]; };
+  auto SC = [&](int k) { return st0[(int64_t) (nfield - 1) * fs + (int64_t) k * ncol]; };
+  auto NHT = [&](int k) { return st0[(int64_t) RHG_CF_NHTOT * fs + (int64_t) k * ncol]; };
+  const double rho_f = AMU * M.wght_per_H;
+  double *tc = tau + (int64_t) ic * Nmax, *hc = height + (int64_t) ic * Nmax,
+         *mc = cmass + (int64_t) ic * Nmax;
+  for (int k = 0; k < N; k++) {
+    if (atm_scale == 0) tc[k] = SC(k);
+    else if (atm_scale == 1) mc[k] = SC(k);
+    else hc[k] = SC(k);
+  }
+  if (atm_scale == 1) {
+    hc[0] = 0.0;
+    tc[0] = CHI(0) / (rho_f * NHT(0)) * mc[0];
+    for (int k = 1; k < N; k++) {
+      hc[k] = hc[k - 1] - 2.0 * (mc[k] - mc[k - 1]) / (rho_f * NHT(k - 1) + rho_f * NHT(k));
+      tc[k] = tc[k - 1] + 0.5 * (CHI(k - 1) + CHI(k)) * (hc[k - 1] - hc[k]);
+    }
+  } else if (atm_scale == 0) {
+    hc[0] = 0.0;
+    mc[0] = (tc[0] / CHI(0)) * (rho_f * NHT(0));
+    for (int k = 1; k < N; k++) {
+      hc[k] = hc[k - 1] - 2.0 * (tc[k] - tc[k - 1]) / (CHI(k - 1) + CHI(k));
+      mc[k] = mc[k - 1] + 0.5 * (rho_f * NHT(k - 1) + rho_f * NHT(k)) * (hc[k - 1] - hc[k]);
+    }
+  } else if (N > 1) {
+    const double T0 = st0[(int64_t) RHG_CF_T * fs], ne0 = st0[(int64_t) RHG_CF_NE * fs];
+    mc[0] = (NHT(0) * M.totalAbund + ne0) * (KBOLTZMANN * T0 / M.gravity);
+    tc[0] = 0.5 * CHI(0) * (hc[0] - hc[1]);
+    if (tc[0] > 1.0) tc[0] = 0.0;
+    for (int k = 1; k < N; k++) {
+      mc[k] = mc[k - 1] + 0.5 * (rho_f * NHT(k - 1) + rho_f * NHT(k)) * (hc[k - 1] - hc[k]);
+      tc[k] = tc[k - 1] + 0.5 * (CHI(k - 1) + CHI(k)) * (hc[k - 1] - hc[k]);
+    }
+  }
+  if (atm_scale == 0 || atm_scale == 1) {
+    // h_zero = Linear(tau, height, 1.0), rh/linear.c:22-54
+    double hz;
+    if (1.0 <= tc[0]) hz = hc[0];
+    else if (1.0 >= tc[N - 1]) hz = hc[N - 1];
+    else {
+      const int j = locate_asc(N, tc, 1.0);
+      const double fx = (tc[j + 1] - 1.0) / (tc[j + 1] - tc[j]);
+      hz = fx * hc[j] + (1 - fx) * hc[j + 1];
+    }
+    for (int k = 0; k < N; k++) hc[k] = hc[k] - hz;
+  }
+  for (int k = N; k < Nmax; k++) tc[k] = hc[k] = mc[k] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
 // Stage 3: continuum opacity and emissivity for every output wavelength at
 // one (column, depth), the terms of continuum_point() with what depends on
 // T and ne alone evaluated once.  The bound-free sums

@@ -244,6 +244,9 @@ __global__ void k_state(const DevModel *M, const double *atmos, const int32_t *n
 __global__ void k_geometry(const DevModel *M, const int32_t *ndep, int ncol, int Nmax, int atm_scale,
                            double mu, double *state, int nfield, int32_t *moving_out,
                            double *chiref_out);
+__global__ void k_scales(const DevModel *Mp, const int32_t *ndep, int ncol, int Nmax, int atm_scale,
+                         const double *state, int nfield, double *tau, double *height,
+                         double *cmass);
 template <typename R>
 __global__ void k_continuum(const DevModel *M, const int32_t *ndep, int ncol, int Nmax,
                             const double *state, int nfield, const int32_t *out_ls, int nout,

@@ -79,6 +79,9 @@ def lib():
                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,
                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.rhg_hse_timing.argtypes = [C.c_void_p, PD, C.POINTER(C.c_int32)]
+        L.rhg_ne_from_nh.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
+                                     C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
+        L.rhg_scales.argtypes = [C.c_void_p, C.POINTER(RhgColumns), C.c_int32, PD, PD, PD]
         if L.rhg_model_sizeof() != C.sizeof(RhgModel) or \
                 L.rhg_columns_sizeof() != C.sizeof(RhgColumns):
             raise EngineError("rhg_model_t / rhg_columns_t layout mismatch between the "
@@ -260,6 +263,34 @@ class Engine:
         _check(lib().rhg_hse(self._h, C.addressof(hs.struct), int(ncol), int(ndep_max), d_ndep,
                              int(atm_scale), d_scale, d_temp, d_pg_top, 1, d_ne, d_nHtot, d_rho,
                              d_pg, d_niter))
+
+    def ne_from_nh(self, hm, temp, nH, ndep=None):
+        """pyrh's get_ne_from_nH on host columns: temp [K] and nH [cm^-3]
+        [ncol, ndep_max] (1-D: one column).  Returns n_e [cm^-3] of the same
+        shape (2-D)."""
+        from .abi import HseStruct
+        temp = np.ascontiguousarray(np.atleast_2d(temp), np.float64)
+        nH = np.ascontiguousarray(np.atleast_2d(nH), np.float64)
+        if nH.shape != temp.shape:
+            raise ValueError("temp and nH must have the same shape")
+        ncol, nd = temp.shape
+        ndep = (np.full(ncol, nd, np.int32) if ndep is None
+                else np.ascontiguousarray(ndep, np.int32))
+        hs = HseStruct(hm)
+        ne = np.zeros((ncol, nd))
+        _check(lib().rhg_ne_from_nh(self._h, C.addressof(hs.struct), ncol, nd, ndep.ctypes.data,
+                                    temp.ctypes.data, nH.ctypes.data, 0, ne.ctypes.data))
+        return ne
+
+    def scales(self, atmos, atm_scale=0, ndep=None):
+        """pyrh's get_scales on host atmospheres [ncol, 9, ndep_max] (the
+        engine's model holds the single reference wavelength): tau, height
+        [m], cmass [kg m^-2], each [ncol, ndep_max]."""
+        cs = ColumnsStruct(atmos, ndep, atm_scale, 1.0)
+        ncol, nd = cs.struct.ncol, cs.struct.ndep_max
+        out = [np.zeros((ncol, nd)) for _ in range(3)]
+        _check(lib().rhg_scales(self._h, cs.ptr(), 0, *(_pd(o) for o in out)))
+        return tuple(out)
 
     def hse_timing(self):
         """(kernel ms, lanes per column) of the last hse call."""

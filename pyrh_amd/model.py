@@ -664,16 +664,17 @@ def atomic_rf_lines(setup: RunSetup, loggf_ids=None, nlam=0) -> np.ndarray:
     return par
 
 
-def build_model(setup: RunSetup, wave, nlte=False, check=True) -> Model:
+def build_model(setup: RunSetup, wave, nlte=False, check=True, lam_ref=None) -> Model:
     """Wavelength grid and all column-independent tables for LTE synthesis.
     check=False skips the keyword checks of the synthesis (for the
-    hydrostatic-equilibrium builder, which needs only the background)."""
+    hydrostatic-equilibrium builder and the depth scales, which need only
+    the background); lam_ref replaces LAMBDA_REF (get_scales)."""
     if check:
         _check_supported(setup, nlte)
     kw = setup.kw
     el = setup.elements
     atoms = setup.atoms
-    lam_ref = kw.float("LAMBDA_REF")
+    lam_ref = kw.float("LAMBDA_REF") if lam_ref is None else float(lam_ref)
     vchar = kw.float("VMICRO_CHAR") * K.KM_TO_M
     vmacro = kw.float("VMACRO_TRESH") * K.KM_TO_M
 

@@ -68,3 +68,35 @@ def flops(counts, nline: int) -> dict:
 def voigt_transcendentals(counts) -> int:
     c = census_dict(counts)
     return 3 * c["voigt_IV"]
+
+
+def hse_flops(hm, layer_iters, newton_per_iter, chem_per_iter):
+    """Algorithmic FP64 operations of pyrh's hse() (rh/rhf1d/pyrh_hse.c:67-399)
+    for `layer_iters` pressure iterations, counting each add, multiply,
+    divide, exp, pow, log and sqrt of the reference's arithmetic as one:
+
+    * a Newton step of get_ne (rh/rhf1d/pyrh_background.c:509-538) over the
+      element table: 11 per element (its ne factor, first partition
+      function, normalisation of stage 0) + 24 per further stage (partition
+      function, Saha factor, derivative, sums, normalisation, error and
+      derivative terms) + 20;
+    * an iteration of the chemical equilibrium (rh/chemequil.c:262-330): LU
+      solution of the N = nuclei + molecules system with one refinement,
+      2/3 N^3 + 4 N^2, and 10 per molecule to build it;
+    * per pressure iteration: LTE populations of every atom, 9 per level
+      (rh/ltepops.c:87-110), 15 per molecular equilibrium constant, 150 for
+      the 500 nm opacity and the pressure step.
+
+    newton_per_iter and chem_per_iter are the mean counts per pressure
+    iteration (from the oracle's census of a sample)."""
+    m = hm.model.scalars
+    nel = int(hm.nel)
+    nst = int(np.sum(hm.el_nstage)) - nel
+    newton = 11 * nel + 24 * nst + 20
+    N = int(m["nnuclei"]) + int(m["nmol"])
+    chem = 2.0 / 3.0 * N ** 3 + 4 * N * N + 10 * int(m["nmol"])
+    nlev = int(hm.model.arrays["atom_level_ptr"][-1])
+    layer = 9 * nlev + 15 * int(m["nmol"]) + 150
+    per_iter = newton_per_iter * newton + chem_per_iter * chem + layer
+    return float(layer_iters) * per_iter, {"newton_step": newton, "chem_iteration": chem,
+                                           "layer_fixed": layer, "per_layer_iteration": per_iter}

@@ -8,8 +8,10 @@ arrays hse() fills (n_e, n_Htot, rho, p_g in SI units).  Cases: the
 workload's temperature profile and seeded map columns on the tau500 scale,
 a hotter and a cooler column, other top pressures, the FALC model's own
 temperatures, ragged depth counts, the height scale, opacity fudge factors
-and abundance overrides.  Needs /root/reference; run from the repository
-root:
+and abundance overrides.  Also pyrh's get_ne_from_nH (n_e in cm^-3 per
+column) and get_scales (tau, height, cmass per column; the input scale's own
+output array is left unset by the reference and stored as it comes back).
+Needs /root/reference; run from the repository root:
 
     PYRH_PATH=tests/golden/rhdata python3 tests/golden/make_golden_hse.py
 """
@@ -61,6 +63,34 @@ def cases():
         (), None
 
 
+def ne_cases():
+    """(name, atmos [ncol, 9, ndep]) for get_ne_from_nH: rows 0, 1, 8."""
+    a = W.stokes_map(4, seed=W.SEED_BASE + 45)
+    yield "ne_map4", a
+    hot = a[:1].copy()
+    hot[:, 1] *= 1.15
+    yield "ne_hot", hot
+    f = falc.atmosphere()[None]
+    yield "ne_falc", f
+
+
+def scale_cases():
+    """(name, atmos [ncol, 9, ndep], lam_ref, atm_scale, abun) for
+    get_scales; the column-mass and height cases use the scales the tau500
+    run returns for the same columns."""
+    a = W.stokes_map(3, seed=W.SEED_BASE + 46)
+    yield "sc_tau", a, 500.0, 0, ()
+    yield "sc_tau400", a[:1], 400.0, 0, ()
+    yield "sc_abund", a[1:2], 500.0, 0, ABUN
+    ref = refrun.run_scales(RUN, a, 500.0, 0)
+    cm = a.copy()
+    cm[:, 0] = np.log10(ref[:, 2] * 0.1)        # kg m^-2 -> g cm^-2
+    yield "sc_cmass", cm, 500.0, 1, ()
+    hz = a.copy()
+    hz[:, 0] = ref[:, 1] / 1e3                  # m -> km
+    yield "sc_height", hz, 500.0, 2, ()
+
+
 def main():
     out = {"fudge_wave": FUDGE_WAVE, "fudge_value": FUDGE_VALUE,
            "abun_id": np.array([a for a, _ in ABUN], np.int32),
@@ -85,6 +115,16 @@ def main():
         out[name + "__out"] = res
         print(name, ncol, nd, "pg bottom", res[:, 3, ndep - 1].diagonal() if ncol > 1 else
               res[0, 3, ndep[0] - 1], flush=True)
+    for name, a in ne_cases():
+        out[name + "__atmos"] = a
+        out[name + "__ne"] = refrun.run_ne_from_nh(RUN, a)
+        print(name, a.shape, flush=True)
+    for name, a, lam, ats, abun in scale_cases():
+        out[name + "__atmos"] = a
+        out[name + "__lam_ref"] = np.float64(lam)
+        out[name + "__atm_scale"] = np.int32(ats)
+        out[name + "__scales"] = refrun.run_scales(RUN, a, lam, ats, abun=abun)
+        print(name, a.shape, lam, ats, flush=True)
     np.savez_compressed(HERE / "hse_fixtures.npz", **out)
 
 

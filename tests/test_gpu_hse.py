@@ -153,3 +153,43 @@ def test_hse_large_batch_invariants():
     assert np.all(np.diff(b["pg"], axis=1) > 0.0)
     assert np.all(b["ne"] > 0.0) and np.all(np.isfinite(b["rho"]))
     assert np.all((b["niter"] >= 1) & (b["niter"] <= hm.nmax_iter))
+
+
+@pytest.mark.parametrize("name", ("ne_map4", "ne_hot", "ne_falc"))
+def test_ne_from_nh_matches_reference(fix, name):
+    from pyrh_amd import api
+    a = fix[name + "__atmos"]
+    ref = fix[name + "__ne"]
+    e, hm = api.hse_engine_for(str(RUN))
+    ne = e.ne_from_nh(hm, a[:, 1], a[:, 8])
+    err = relerr(ne, ref)
+    one = api.get_ne_from_nH(str(RUN), 0, a[0, 0], a[0, 1], a[0, 8])
+    print(f"{name}: max rel err {err:.2e}", flush=True)
+    assert err < TOL
+    np.testing.assert_array_equal(one, ne[0])
+
+
+@pytest.mark.parametrize("name", ("sc_tau", "sc_tau400", "sc_abund", "sc_cmass", "sc_height"))
+def test_scales_match_reference(fix, name):
+    from pyrh_amd import api
+    from test_oracle_hse import SC_FILLED
+    a = fix[name + "__atmos"]
+    lam = float(fix[name + "__lam_ref"])
+    ats = int(fix[name + "__atm_scale"])
+    kw = {}
+    if name == "sc_abund":
+        kw = dict(atomic_number=fix["abun_id"], atomic_abundance=fix["abun_val"])
+    e = api.scales_engine_for(str(RUN), lam, **kw)
+    got = np.stack(e.scales(a, ats), 1)
+    ref = fix[name + "__scales"]
+    worst = 0.0
+    for i in SC_FILLED[ats]:
+        scale = np.max(np.abs(ref[:, i]), axis=1, keepdims=True)
+        worst = max(worst, float(np.max(np.abs(got[:, i] - ref[:, i]) / scale)))
+    print(f"{name}: max err {worst:.2e} of the column's range", flush=True)
+    assert worst < TOL
+    # the input scale comes back in SI units
+    own = {0: 10.0 ** a[:, 0], 1: 10.0 ** a[:, 0] * 10.0, 2: a[:, 0] * 1e3}[ats]
+    np.testing.assert_allclose(got[:, {0: 0, 1: 2, 2: 1}[ats]], own, rtol=1e-13)
+    t1, h1, m1 = api.get_scales(str(RUN), ats, a[0, 0], a[0], lam, **kw)
+    np.testing.assert_array_equal(np.stack([t1, h1, m1]), got[0])

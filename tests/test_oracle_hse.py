@@ -91,3 +91,43 @@ def test_oracle_hse_reproduces_workload_structure():
     assert relerr(ne[0] * 1e-6, d["ne_cm3"]) < 1e-12
     assert relerr(nH[0] * 1e-6, d["nH_cm3"]) < 1e-12
     assert relerr(pg[0], d["pg_si"]) < 1e-12
+
+
+NE_CASES = ("ne_map4", "ne_hot", "ne_falc")
+SC_CASES = ("sc_tau", "sc_tau400", "sc_abund", "sc_cmass", "sc_height")
+# the outputs of get_scales that the reference fills, per input scale
+# (tau, height, cmass): the input scale's own array it leaves unset
+SC_FILLED = {0: (1, 2), 1: (0, 1), 2: (0, 2)}
+
+
+@pytest.mark.parametrize("name", NE_CASES)
+def test_oracle_ne_from_nh_matches_reference(fix, name):
+    import pyoracle
+    from pyrh_amd import hse, model
+    hm = hse.build_hse(model.open_run(str(RUN)))
+    a = fix[name + "__atmos"]
+    ne = pyoracle.ne_from_nh(hm, a[:, 1], a[:, 8])
+    assert relerr(ne, fix[name + "__ne"]) < TOL
+
+
+def scales_model(fix, name):
+    from pyrh_amd import model
+    kw = {}
+    if name == "sc_abund":
+        kw = dict(atomic_number=fix["abun_id"], atomic_abundance=fix["abun_val"])
+    s = model.open_run(str(RUN), **kw)
+    lam = float(fix[name + "__lam_ref"])
+    return model.build_model(s, np.array([lam]), check=False, lam_ref=lam)
+
+
+@pytest.mark.parametrize("name", SC_CASES)
+def test_oracle_scales_match_reference(fix, name):
+    import pyoracle
+    m = scales_model(fix, name)
+    ats = int(fix[name + "__atm_scale"])
+    got = np.stack(pyoracle.scales(m, fix[name + "__atmos"], ats), 1)
+    ref = fix[name + "__scales"]
+    for i in SC_FILLED[ats]:
+        # heights cross zero at tau = 1: compare relative to the column's range
+        scale = np.max(np.abs(ref[:, i]), axis=1, keepdims=True)
+        assert float(np.max(np.abs(got[:, i] - ref[:, i]) / scale)) < TOL, (name, i)
