@@ -1544,10 +1544,11 @@ int rhg_hse(rhg_context_t *c, const rhg_hse_t *h, int32_t ncol, int32_t Nmax, co
     d_pg = out + (size_t) 3 * ncol * Nmax;
     d_niter = niter_out ? (int32_t *) c->hs_niter.p : nullptr;
   }
-  // lanes per column: enough lane groups to give every SIMD of the chip a
-  // wave (256 CUs x 4 SIMDs x 64 lanes), at most 16
+  // lanes per column: enough lane groups to give every SIMD of the chip two
+  // waves (256 CUs x 4 SIMDs x 64 lanes x 2, the occupancy k_hse is built
+  // for), at most 16
   int lpc = 1;
-  while (lpc < 16 && (int64_t) ncol * lpc < 65536) lpc <<= 1;
+  while (lpc < 16 && (int64_t) ncol * lpc < 131072) lpc <<= 1;
   if (const char *lanes_env = getenv("RHG_HSE_LANES")) {
     const int v = atoi(lanes_env);
     if (v == 1 || v == 2 || v == 4 || v == 8 || v == 16) lpc = v;
