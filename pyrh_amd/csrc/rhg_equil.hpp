@@ -107,13 +107,16 @@ template <int NN>
 __device__ void chem_eq(const DevModel &M, double T, double ne, double nHtot,
                         const double (&fn0)[NN], double (&nnuc)[NN], double *lds_phi,
                         double *lds_nmol, double *lds_saha, int ls_stride, double &PhiHmin_out,
-                        bool phi_ready = false) {
+                        bool phi_ready = false, const double *tcache = nullptr) {
   const int Nm = M.nmol;
   const double CI = (HPLANCK / (2.0 * PI * M_ELECTRON)) * (HPLANCK / KBOLTZMANN);
   double a[NN];
 #pragma unroll
   for (int i = 0; i < NN; i++) a[i] = M.nucl_abund[i] * nHtot;
-  double PhiHmin = 0.25 * pow(CI / T, 1.5) * exp(0.754 * EV / (KBOLTZMANN * T));
+  // tcache: pow(CI/T, 1.5) and exp(E_ION_HMIN/kT) formed by a caller that
+  // solves this temperature again
+  double PhiHmin = tcache ? 0.25 * tcache[0] * tcache[1]
+                          : 0.25 * pow(CI / T, 1.5) * exp(0.754 * EV / (KBOLTZMANN * T));
   double fHmin = ne * fn0[0] * PhiHmin;
   // phi_ready: lds_phi already holds the constants at this T (a caller
   // that solves the same temperature again)
@@ -126,6 +129,12 @@ __device__ void chem_eq(const DevModel &M, double T, double ne, double nHtot,
 
   for (int niter = 1; niter <= N_MAX_CHEM_ITER; niter++) {
     double f1[NN], S[NN][NN], r[NN], x1[NN];
+    // the Jacobian's count / n_nu (rh/chemequil.c:300-304) as count * (1 / n_nu):
+    // one division per nucleus instead of one per (molecule, nucleus), a
+    // last-place difference
+    double inv_n[NN];
+#pragma unroll
+    for (int i = 0; i < NN; i++) inv_n[i] = 1.0 / nnuc[i];
 #pragma unroll
     for (int i = 0; i < NN; i++) f1[i] = nnuc[i] - a[i];
     f1[0] += fHmin * nnuc[0];
@@ -160,7 +169,7 @@ __device__ void chem_eq(const DevModel &M, double T, double ne, double nHtot,
       const double f2 = lds_nmol[m * ls_stride] - sv;
       double C[NN];
 #pragma unroll
-      for (int j = 0; j < NN; j++) C[j] = -sv * (cnt[j] / nnuc[j]);
+      for (int j = 0; j < NN; j++) C[j] = -sv * (cnt[j] * inv_n[j]);
 #pragma unroll
       for (int i = 0; i < NN; i++)
         if (cnt[i] != 0.0) {
@@ -200,7 +209,7 @@ __device__ void chem_eq(const DevModel &M, double T, double ne, double nHtot,
       double x2 = nold - sv;
 #pragma unroll
       for (int j = 0; j < NN; j++)
-        if (cnt[j] != 0.0) x2 -= (-sv * (cnt[j] / nnuc[j])) * x1[j];
+        if (cnt[j] != 0.0) x2 -= (-sv * (cnt[j] * inv_n[j])) * x1[j];
       const double nnew = nold - x2;
       lds_nmol[m * ls_stride] = nnew;
       if (nnew) dnmax = fmax(dnmax, fabs((nnew - nold) / nnew));

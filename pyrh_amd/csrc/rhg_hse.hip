@@ -18,6 +18,24 @@ namespace {
 constexpr int HSE_N_MAX_ELECTRON_ITER = 10;  // rh/rhf1d/pyrh_background.h:1-2
 constexpr double HSE_MAX_ELECTRON_ERROR = 1.0E-2;
 
+// What the layer's temperature fixes in the Saha factors: pow(C1/T, 1.5),
+// exp(E_ION_HMIN/kT) (these two in this order, as chem_eq takes them),
+// pow(C1/T, -1.5) and exp(Uk0 + chi_H/kT).
+struct HseT {
+  double p15, eHmin, pm15, eH;
+};
+
+__device__ __forceinline__ HseT hse_tcache(const DevHse &H, double T, double Uk0) {
+  const double C1 = (HPLANCK / (2.0 * PI * M_ELECTRON)) * (HPLANCK / KBOLTZMANN);
+  const double kT = KBOLTZMANN * T;
+  HseT t;
+  t.p15 = pow(C1 / T, 1.5);
+  t.eHmin = exp(0.754 * EV / kT);
+  t.pm15 = pow(C1 / T, -1.5);
+  t.eH = exp(Uk0 + H.el_ionpot[0] / kT);
+  return t;
+}
+
 // Stage factors of the LTE ionisation fractions of the group lane's
 // elements at temperature T: r = exp(U_j - U_{j-1} - chi_{j-1}/kT) per
 // stage j >= 1 (getfjk, rh/solvene.c:180-190; U from getKuruczpf = Linear
@@ -106,8 +124,7 @@ __device__ __forceinline__ double hse_level_sum(const double *g, int64_t ss, dou
 // rh/solvene.c:78-88).
 template <bool CACHED>
 __device__ double hse_get_ne(const DevModel &M, const DevHse &H, double T, double nHtot, int sub,
-                             int lpc, const double *sc, int64_t ss, double Uk0 = 0.0) {
-  const double C1 = (HPLANCK / (2.0 * PI * M_ELECTRON)) * (HPLANCK / KBOLTZMANN);
+                             int lpc, const double *sc, int64_t ss, const HseT &tc) {
   const double kT = KBOLTZMANN * T;
   const int npf = M.npf;
   int jT = 0, edge = 0;
@@ -125,10 +142,10 @@ __device__ double hse_get_ne(const DevModel &M, const DevHse &H, double T, doubl
     if (edge > 0) return yt[npf - 1];
     return fx * yt[jT] + (1 - fx) * yt[jT + 1];
   };
-  const double PhiH = 0.5 * pow(C1 / T, 1.5) * exp(Uk0 + H.el_ionpot[0] / kT);
+  const double PhiH = 0.5 * tc.p15 * tc.eH;
   double ne_old = (sqrt(1.0 + 4.0 * nHtot * PhiH) - 1.0) / (2.0 * PhiH), ne = ne_old;
-  const double PhiHmin = 0.25 * pow(C1 / T, 1.5) * exp(0.754 * EV / kT);
-  const double CTp = pow(C1 / T, -1.5);
+  const double PhiHmin = 0.25 * tc.p15 * tc.eHmin;
+  const double CTp = tc.pm15;
   for (int niter = 0; niter < HSE_N_MAX_ELECTRON_ITER; niter++) {
     const double CT_ne = 2.0 * CTp / ne_old, inv_ne = 1.0 / ne_old;
     double err = (sub == 0) ? ne_old / nHtot : 0.0, sum = 0.0;
@@ -299,13 +316,12 @@ __device__ double hse_opacity(const DevModel &M, const DevHse &H, int ls, double
 template <int NN>
 __device__ void hse_layer(const DevModel &M, const DevHse &H, double T, double nHtot, int sub,
                           int lpc, bool opacity, bool first, const double *sc, int64_t ss,
-                          double *lds_phi, double *lds_nmol, double *lds_saha, int lstride,
-                          double &ne_out, double &opac_out) {
-  const double ne = hse_get_ne<true>(M, H, T, nHtot, sub, lpc, sc, ss);
+                          const HseT &tc, double *lds_phi, double *lds_nmol, double *lds_saha,
+                          int lstride, double &ne_out, double &opac_out) {
+  const double ne = hse_get_ne<true>(M, H, T, nHtot, sub, lpc, sc, ss, tc);
   ne_out = ne;
   if (!opacity) return;
-  const double c1 = (HPLANCK / (2.0 * PI * M_ELECTRON)) * (HPLANCK / KBOLTZMANN);
-  const double cNe_T = 0.5 * ne * pow(c1 / T, 1.5);
+  const double cNe_T = 0.5 * ne * tc.p15;
   // level groups of the nuclei (and He) at this layer's T, after the
   // stage factors in the lane's slots
   const double *lg = sc + (int64_t) H.nslot_stage * ss;
@@ -319,7 +335,8 @@ __device__ void hse_layer(const DevModel &M, const DevHse &H, double T, double n
     fn0[i] = (nstar0n[i] + g[HSE_NDZ * ss] * nstar0n[i]) / ntot;
   }
   double nnuc[NN], PhiHmin;
-  chem_eq<NN>(M, T, ne, nHtot, fn0, nnuc, lds_phi, lds_nmol, lds_saha, lstride, PhiHmin, !first);
+  chem_eq<NN>(M, T, ne, nHtot, fn0, nnuc, lds_phi, lds_nmol, lds_saha, lstride, PhiHmin, !first,
+              &tc.p15);
   double fracn[NN];
 #pragma unroll
   for (int i = 0; i < NN; i++) fracn[i] = nnuc[i] / (M.atom_abund[M.nucl_atom[i]] * nHtot);
@@ -378,7 +395,9 @@ k_hse(const DevModel *__restrict__ Mp, DevHse H, int lpc, int ncol, int Nmax,
 
   // top layer, pyrh_hse.c:210-273
   double T = Tc[0], pg = pg_top[ic], ne = 0.0, opac = 0.0, rho = 0.0;
+  HseT tc;
   auto layer_tables = [&](double Tl) {
+    tc = hse_tcache(H, Tl, 0.0);
     hse_stage_factors(M, H, Tl, sub, lpc, sc_t, ss);
     if (!tau) return;
     double *lg = sc_t + (int64_t) H.nslot_stage * ss;
@@ -393,8 +412,8 @@ k_hse(const DevModel *__restrict__ Mp, DevHse H, int lpc, int ncol, int Nmax,
   int iter = 0;
   while (iter < H.nmax_iter) {
     rho = wH * nH;
-    hse_layer<NN>(M, H, T, nH, sub, lpc, tau, iter == 0, sc_t, ss, lds_phi, lds_nmol, lds_saha,
-                  lstride, ne, opac);
+    hse_layer<NN>(M, H, T, nH, sub, lpc, tau, iter == 0, sc_t, ss, tc, lds_phi, lds_nmol,
+                  lds_saha, lstride, ne, opac);
     const double old = nH;
     nH = (pg / KBOLTZMANN / T - ne) / tA;
     iter++;
@@ -423,8 +442,8 @@ k_hse(const DevModel *__restrict__ Mp, DevHse H, int lpc, int ncol, int Nmax,
     iter = 0;
     while (iter < H.nmax_iter) {
       rho = wH * nH;
-      hse_layer<NN>(M, H, T, nH, sub, lpc, tau, iter == 0, sc_t, ss, lds_phi, lds_nmol, lds_saha,
-                  lstride, ne, opac);
+      hse_layer<NN>(M, H, T, nH, sub, lpc, tau, iter == 0, sc_t, ss, tc, lds_phi, lds_nmol,
+                  lds_saha, lstride, ne, opac);
       if (tau) {
         const double beta1 = rho_prev / opac_prev * z_prev;
         const double beta2 = rho / opac * z;
@@ -476,7 +495,7 @@ k_ne_from_nh(const DevModel *__restrict__ Mp, DevHse H, int ncol, int Nmax,
   const double CUBE_CM = CM_TO_M * CM_TO_M * CM_TO_M;
   const double T = temp[gid], nHtot = nH[gid] / CUBE_CM;
   const double Uk = linear1(M.npf, M.Tpf, H.el_logpf, T);
-  ne_out[gid] = hse_get_ne<false>(M, H, T, nHtot, 0, 1, nullptr, 0, Uk) * CUBE_CM;
+  ne_out[gid] = hse_get_ne<false>(M, H, T, nHtot, 0, 1, nullptr, 0, hse_tcache(H, T, Uk)) * CUBE_CM;
 }
 
 #define RHG_INST_HSE(NN)                                                                         \
