@@ -66,7 +66,8 @@ enum rhg_lamconst {
   RHG_LC_OH_EIDX,        /* OH bf fractional energy index (-1 = off)      */
   RHG_LC_CH_EIDX,        /* CH bf fractional energy index (-1 = off)      */
   RHG_LC_HFF_LONG0,      /* 5 coefficients of the long-wave H- ff fit ... */
-  RHG_LC_NCONST = RHG_LC_HFF_LONG0 + 5
+  RHG_LC_HMIN_FUDGE = RHG_LC_HFF_LONG0 + 5, /* H- opacity fudge factor (1: none)  */
+  RHG_LC_NCONST
 };
 
 /* Per-(column, depth) quantities produced by the equilibria stage and

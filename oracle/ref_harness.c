@@ -476,7 +476,8 @@ int main(int argc, char **argv)
     spec = rhf1d(".", mu, Ndep, atm, atm + Ndep, atm + 2 * Ndep,
                  atm + 3 * Ndep, atm + 4 * Ndep, atm + 5 * Ndep,
                  atm + 6 * Ndep, atm + 7 * Ndep, atm + 8 * Ndep, atm_scale,
-                 Nwave, w, 0, NULL, NULL, ov.nloggf, ov.loggf_ids, ov.loggf_val,
+                 Nwave, w, ov.nfudge, ov.nfudge ? ov.fudge_lam : NULL,
+                 ov.nfudge ? ov.fudge_val : NULL, ov.nloggf, ov.loggf_ids, ov.loggf_val,
                  ov.nlam, ov.lam_ids, ov.lam_val, ov.nabun, ov.abun_ids, ov.abun_val,
                  ov.rf_out != NULL, 0, 0, ".");
     if (ov.rf_out && spec.rfs) {

@@ -55,7 +55,7 @@ def read_capture(path):
 
 
 def run_synth(rundir, atmos, wave, mu=1.0, atm_scale=0, variant="o2",
-              capture=False, timeout=3600, loggf=(), lam=(), abun=(), rf=False):
+              capture=False, timeout=3600, loggf=(), lam=(), abun=(), rf=False, fudge=None):
     """Call rhf1d() once per column.  Returns (lam, I, Q, U, V[, capture]
     [, rf]).  loggf / lam / abun are sequences of (index, value) pairs passed
     as pyrh.compute1d's loggf_ids/values, lam_ids/values and
@@ -67,7 +67,7 @@ def run_synth(rundir, atmos, wave, mu=1.0, atm_scale=0, variant="o2",
         fin, fout, fcap = (os.path.join(td, n) for n in ("in", "out", "cap"))
         frf = os.path.join(td, "rf")
         npar = len(loggf) + len(lam)
-        if loggf or lam or abun or rf:
+        if loggf or lam or abun or rf or fudge is not None:
             fov = os.path.join(td, "overrides")
             with open(fov, "w") as f:
                 for i, v in loggf:
@@ -78,6 +78,11 @@ def run_synth(rundir, atmos, wave, mu=1.0, atm_scale=0, variant="o2",
                     f.write(f"abun {int(i)} {float(v)!r}\n")
                 if rf:
                     f.write(f"rf {frf}\n")
+                if fudge is not None:
+                    fw, fv = fudge
+                    for k in range(len(fw)):
+                        f.write(f"fudge {float(fw[k])!r} {float(fv[0][k])!r} "
+                                f"{float(fv[1][k])!r} {float(fv[2][k])!r}\n")
             env["RHREF_OVERRIDES"] = fov
         ncol, ndep = _write_input(fin, atmos, wave, mu, atm_scale)
         cmd = [str(exe), str(rundir), fin, fout] + ([fcap] if capture else [])

@@ -1152,6 +1152,10 @@ static void background_continuum(const rhg_model_t *m, const column_t *c, int ls
     }
     chi_ai[k] += chi;
     eta_ai[k] += chi * Bnu;
+    /* opacity fudge factor of H-, rh/background.c:362-369 (the metal one,
+       background.c:438-451, is in the metal continua's cross-sections) */
+    chi_ai[k] *= lc[RHG_LC_HMIN_FUDGE];
+    eta_ai[k] *= lc[RHG_LC_HMIN_FUDGE];
     /* OH and CH bound-free, rh/ohchbf.c:33-394, 402-690 */
     if (lc[RHG_LC_OH_EIDX] >= 0.0) {
       if (T < m->oh_T[0] || T > m->oh_T[NTOH - 1]) { chi = 0.0; eta = 0.0; }

@@ -65,14 +65,27 @@ def _setup(cwd, atomic_number, atomic_abundance, loggf_ids, loggf_values, lam_id
     return key, s
 
 
+def _fudge(fudge_wave, fudge_value):
+    """pyrh's fudge_wave/fudge_value as one (wave, value) pair, or None."""
+    if fudge_wave is None and fudge_value is None:
+        return None
+    if fudge_wave is None or fudge_value is None:
+        raise ValueError("fudge_wave and fudge_value go together")
+    return (np.ascontiguousarray(fudge_wave, np.float64),
+            np.ascontiguousarray(fudge_value, np.float64))
+
+
 def engine_for(cwd, wave, atomic_number=None, atomic_abundance=None, loggf_ids=None,
-               loggf_values=None, lam_ids=None, lam_values=None, device=0):
+               loggf_values=None, lam_ids=None, lam_values=None, device=0, fudge_wave=None,
+               fudge_value=None):
     skey, s = _setup(cwd, atomic_number, atomic_abundance, loggf_ids, loggf_values,
                      lam_ids, lam_values)
-    key = (skey, _key(np.asarray(wave, np.float64)), device)
+    fudge = _fudge(fudge_wave, fudge_value)
+    key = (skey, _key(np.asarray(wave, np.float64)), device,
+           _key(*(fudge if fudge is not None else (None,))))
     e = _ENGINES.get(key)
     if e is None:
-        m = _model.build_model(s, np.asarray(wave, np.float64))
+        m = _model.build_model(s, np.asarray(wave, np.float64), fudge=fudge)
         e = Engine(m, device)
         _ENGINES[key] = e
     return e
@@ -208,10 +221,12 @@ def nlte_batch(cwd, mu, atm_scale, atmospheres, wave, ndep=None, device=0):
 
 
 def synth_batch(cwd, mu, atm_scale, atmospheres, wave, ndep=None, precision="fp64",
-                atomic_number=None, atomic_abundance=None, device=0):
+                atomic_number=None, atomic_abundance=None, device=0, fudge_wave=None,
+                fudge_value=None):
     """Stokes I, Q, U, V [ncol, nlw] and the wavelengths [nlw] for a batch
     of columns [ncol, >=9, ndep_max] (pyrh row order and units)."""
-    e = engine_for(cwd, wave, atomic_number, atomic_abundance, device=device)
+    e = engine_for(cwd, wave, atomic_number, atomic_abundance, device=device,
+                   fudge_wave=fudge_wave, fudge_value=fudge_value)
     prec = {"fp64": 0, "fp32": 1}[precision]
     st = e.synth_lte(np.asarray(atmospheres, np.float64), mu, atm_scale, ndep, prec)
     lam = e.model.lam[e.model.out_index]
@@ -231,8 +246,7 @@ def compute1d(cwd, mu, atm_scale, atmosphere, wave, loggf_ids=None, loggf_values
     (rhf1d's Iterate and final ray) and returns the emergent spectrum of the
     converged atom.  Arrays own their memory and the inputs are not
     modified."""
-    if fudge_wave is not None or fudge_value is not None:
-        raise NotImplementedError("opacity fudge factors are not implemented")
+    fudge = _fudge(fudge_wave, fudge_value)
     if loggf_ids is not None and loggf_values is not None and \
             len(loggf_ids) != len(loggf_values):
         raise ValueError("Different number of 'loggf_ids' and 'loggf_values'.")
@@ -247,6 +261,10 @@ def compute1d(cwd, mu, atm_scale, atmosphere, wave, loggf_ids=None, loggf_values
     skey, s = _setup(cwd, atomic_number, atomic_abundance, loggf_ids, loggf_values,
                      lam_ids, lam_values)
     if any(a.active for a in s.atoms):
+        if fudge is not None:
+            raise NotImplementedError("opacity fudge factors with an ACTIVE atom (the "
+                                      "scattering factor of the NLTE background) are not "
+                                      "implemented")
         if get_atomic_rfs:
             raise NotImplementedError("get_atomic_rfs with an ACTIVE atom is not implemented")
         e, nm = nlte_engine_for(cwd, wave, atomic_number, atomic_abundance, loggf_ids,
@@ -260,7 +278,7 @@ def compute1d(cwd, mu, atm_scale, atmosphere, wave, loggf_ids=None, loggf_values
             return out, (Populations(nm.ID, int(nm.nlevel), nd, pops[0].copy(), nst[0].copy()),)
         return out
     e = engine_for(cwd, wave, atomic_number, atomic_abundance, loggf_ids, loggf_values,
-                   lam_ids, lam_values)
+                   lam_ids, lam_values, fudge_wave=fudge_wave, fudge_value=fudge_value)
     lam = e.model.lam[e.model.out_index].copy()
     if get_atomic_rfs:
         nlam = 0 if lam_ids is None else len(lam_ids)

@@ -60,6 +60,9 @@ __device__ __forceinline__ void continuum_point(const DevModel &M, int ls, const
   }
   chi_ai += chi;
   eta_ai += chi * Bnu;
+  // the H- fudge factor scales what has been summed so far, rh/background.c:362-369
+  chi_ai *= lc[RHG_LC_HMIN_FUDGE];
+  eta_ai *= lc[RHG_LC_HMIN_FUDGE];
   if (lc[RHG_LC_OH_EIDX] >= 0.0) {
     if (T < M.oh_T[0] || T > M.oh_T[NTOH - 1]) { chi = 0.0; eta = 0.0; }
     else {
@@ -587,6 +590,8 @@ k_continuum(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep, i
       }
       chi_ai += chi;
       eta_ai += chi * Bnu;
+      chi_ai *= lc[RHG_LC_HMIN_FUDGE];
+      eta_ai *= lc[RHG_LC_HMIN_FUDGE];
       if (lc[RHG_LC_OH_EIDX] >= 0.0 && oh_in) {
         double kappa = exp(LG10 * bilinear(NTOH, NEOH, M.oh_cross, ti_oh, lc[RHG_LC_OH_EIDX])) *
                        (CM_TO_M * CM_TO_M);

@@ -664,11 +664,18 @@ def atomic_rf_lines(setup: RunSetup, loggf_ids=None, nlam=0) -> np.ndarray:
     return par
 
 
-def build_model(setup: RunSetup, wave, nlte=False, check=True, lam_ref=None) -> Model:
+def build_model(setup: RunSetup, wave, nlte=False, check=True, lam_ref=None,
+                fudge=None) -> Model:
     """Wavelength grid and all column-independent tables for LTE synthesis.
     check=False skips the keyword checks of the synthesis (for the
     hydrostatic-equilibrium builder and the depth scales, which need only
-    the background); lam_ref replaces LAMBDA_REF (get_scales)."""
+    the background); lam_ref replaces LAMBDA_REF (get_scales).  fudge =
+    (fudge_wave [n], fudge_value [3][n]) are pyrh's opacity fudge factors
+    (rh/rhf1d/pyrh_compute1dray.c:183-195): row 0 scales the H- opacity
+    (rh/background.c:362-369, the per-wavelength factor RHG_LC_HMIN_FUDGE),
+    row 2 the metal bound-free opacity (background.c:438-451, folded into
+    the metal continua's cross-sections), both by Linear in wavelength; row
+    1 scales the scattering opacity, which enters only an NLTE solution."""
     if check:
         _check_supported(setup, nlte)
     kw = setup.kw
@@ -698,8 +705,19 @@ def build_model(setup: RunSetup, wave, nlte=False, check=True, lam_ref=None) -> 
     hmbf_l = list(T.HMIN_BF_LAMBDA)
     hmbf_a = list(T.HMIN_BF_ALPHA)
     hmbf_M = spline_coef(hmbf_l, hmbf_a)
-    NC = 20
+    NC = 21
     lc = np.zeros((nspect, NC))
+    if fudge is not None:
+        fw = np.ascontiguousarray(fudge[0], np.float64).ravel()
+        fv = np.ascontiguousarray(fudge[1], np.float64)
+        if fv.ndim != 2 or fv.shape[0] != 3 or fv.shape[1] != fw.size or fw.size < 2:
+            raise ValueError("fudge_value must be [3][len(fudge_wave)] with at least two "
+                             "wavelengths")
+        hmin_f = np.array([linear(fw, fv[0], float(x)) for x in lam])
+        metal_f = np.array([linear(fw, fv[2], float(x)) for x in lam])
+    else:
+        hmin_f = metal_f = np.ones(nspect)
+    lc[:, 20] = hmin_f
     for n, la in enumerate(lam):
         lc[n, 0] = la
         lc[n, 1] = (K.HPLANCK * K.CLIGHT) / (K.KBOLTZMANN * K.NM_TO_M * la)
@@ -777,7 +795,7 @@ def build_model(setup: RunSetup, wave, nlte=False, check=True, lam_ref=None) -> 
     for n in range(nspect):
         for g, a in per_lam[n]:
             lam_cont_idx.append(remap[g])
-            lam_cont_alpha.append(a)
+            lam_cont_alpha.append(a if cont_list[g][0] == 0 else a * metal_f[n])
         lam_cont_ptr.append(len(lam_cont_idx))
     cont_atom = np.array([ia for ia, _ in cont_desc], np.int32)
     cont_i = np.array([c.i for _, c in cont_desc], np.int32)

@@ -1,0 +1,45 @@
+"""Generate the opacity-fudge golden vectors in tests/golden/fudge_fixtures.npz.
+
+Runs pyrh's rhf1d() from the reference library compiled from its C sources
+(oracle/_ref, variant "o2") with compute1d's fudge_wave / fudge_value
+(rh/rhf1d/pyrh_compute1dray.c:183-195: row 0 scales the H- opacity, row 1
+the scattering, row 2 the metal bound-free opacity, each by Linear in
+wavelength) on three seeded map columns of the LTE run directory at mu = 1
+and mu = 0.7.  Needs /root/reference; run from the repository root:
+
+    PYRH_PATH=tests/golden/rhdata python3 tests/golden/make_golden_fudge.py
+"""
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+sys.path.insert(0, str(HERE.parents[1]))
+sys.path.insert(0, str(HERE.parents[1] / "oracle"))
+
+import refrun  # noqa: E402
+from pyrh_amd import workloads as W  # noqa: E402
+
+RUN = HERE / "run_lte"
+FUDGE_WAVE = np.array([300.0, 450.0, 600.0, 900.0])
+FUDGE_VALUE = np.array([[1.3, 1.2, 1.1, 1.0], [0.7, 0.8, 0.9, 1.0], [1.5, 1.4, 1.3, 1.2]])
+
+
+def main():
+    atm = W.stokes_map(3, seed=W.SEED_BASE + 47)
+    wave = W.wavelengths_630()
+    out = {"atmos": atm, "wave": wave, "fudge_wave": FUDGE_WAVE, "fudge_value": FUDGE_VALUE}
+    for name, mu in (("mu1", 1.0), ("mu07", 0.7)):
+        lam, I, Q, U, V = refrun.run_synth(RUN, atm, wave, mu=mu, fudge=(FUDGE_WAVE, FUDGE_VALUE))[:5]
+        out[name + "__stokes"] = np.stack([I, Q, U, V], 1)
+        out[name + "__mu"] = np.float64(mu)
+        out["lam"] = lam
+        print(name, out[name + "__stokes"].shape, flush=True)
+    np.savez_compressed(HERE / "fudge_fixtures.npz", **out)
+
+
+if __name__ == "__main__":
+    main()

@@ -207,3 +207,30 @@ def test_gpu_compute1d_api(fixtures):
         assert np.max(np.abs(I - ref[0]) / ref[0]) <= TOL_I
         for n, X in ((1, Q), (2, U), (3, V)):
             assert np.max(np.abs(X - ref[n])) <= TOL_QUV * ref[0, 0]
+
+
+@pytest.mark.parametrize("name", ("mu1", "mu07"))
+def test_fudge_matches_reference(name):
+    """compute1d's opacity fudge factors on the GPU against the reference's
+    rhf1d() with the same table (tests/golden/make_golden_fudge.py): FP64
+    within 1e-10, the FP32 form within 1e-4 I_c, and compute1d with
+    fudge_wave / fudge_value on one column the same as the batch."""
+    from conftest import GOLDEN
+    from pyrh_amd import api
+    f = np.load(GOLDEN / "fudge_fixtures.npz")
+    mu = float(f[name + "__mu"])
+    run = str(GOLDEN / "run_lte")
+    e = api.engine_for(run, f["wave"], fudge_wave=f["fudge_wave"], fudge_value=f["fudge_value"])
+    st = e.synth_lte(f["atmos"], mu)
+    ref = f[name + "__stokes"]
+    Ic = ref[:, 0, :1]
+    eI = float(np.max(np.abs(st[:, 0] - ref[:, 0]) / ref[:, 0]))
+    eP = float(np.max(np.abs(st[:, 1:] - ref[:, 1:]) / Ic[:, None]))
+    print(f"fudge {name}: I {eI:.2e}  QUV/Ic {eP:.2e}", flush=True)
+    assert eI < 1e-10 and eP < 1e-10
+    st32 = e.synth_lte(f["atmos"], mu, precision=1)
+    assert float(np.max(np.abs(st32 - ref) / Ic[:, None])) < 1e-4
+    I, Q, U, V, lam = api.compute1d(run, mu, 0, f["atmos"][1], f["wave"],
+                                    fudge_wave=f["fudge_wave"], fudge_value=f["fudge_value"])
+    np.testing.assert_array_equal(np.stack([I, Q, U, V]), st[1])
+    np.testing.assert_array_equal(lam, f["lam"])

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import pyoracle
-from conftest import fixture_cases, golden_model
+from conftest import GOLDEN, fixture_cases, golden_model
 
 
 def _run(F, setups, name, fp32=False):
@@ -138,3 +138,31 @@ def test_oracle_matches_reference_mixed(name):
     assert np.max(np.abs(st[:, 1:] - ref[:, 1:]) / Ic[:, None]) < 5e-12
     # the lines are Zeeman-split: the case exercises the polarized solver
     assert np.max(np.abs(ref[:, 3]) / Ic) > 0.05
+
+
+@pytest.mark.parametrize("name", ("mu1", "mu07"))
+def test_oracle_matches_reference_fudge(name):
+    """compute1d's opacity fudge factors (H- and metal bound-free rows):
+    the oracle on the model built with them against the reference's own
+    rhf1d() with the same table (tests/golden/make_golden_fudge.py)."""
+    import pyoracle
+    from pyrh_amd import model
+    f = np.load(GOLDEN / "fudge_fixtures.npz")
+    s = model.open_run(str(GOLDEN / "run_lte"))
+    m = model.build_model(s, f["wave"], fudge=(f["fudge_wave"], f["fudge_value"]))
+    st = pyoracle.synth_lte(m, f["atmos"], mu=float(f[name + "__mu"]))
+    ref = f[name + "__stokes"]
+    Ic = ref[:, 0, :1]
+    assert float(np.max(np.abs(st[:, 0] - ref[:, 0]) / ref[:, 0])) < 1e-10
+    assert float(np.max(np.abs(st[:, 1:] - ref[:, 1:]) / Ic[:, None])) < 1e-10
+    # the factors matter: without them the spectrum is percent-level off
+    m0 = model.build_model(s, f["wave"])
+    st0 = pyoracle.synth_lte(m0, f["atmos"][:1], mu=float(f[name + "__mu"]))
+    assert float(np.max(np.abs(st0[0, 0] - ref[0, 0]) / ref[0, 0])) > 1e-3
+
+
+def test_fudge_with_active_atom_is_rejected():
+    from pyrh_amd import api
+    with pytest.raises(NotImplementedError):
+        api.compute1d(str(GOLDEN / "run_nlte"), 1.0, 0, np.zeros((9, 10)), [393.4],
+                      fudge_wave=[300.0, 900.0], fudge_value=np.ones((3, 2)))
