@@ -1473,6 +1473,7 @@ static int synth_column_ex(const rhg_model_t *m, const rhg_columns_t *cols, int 
       if (na->he_atom >= 0)
         sca += na->ray_He[lr] * c.nstar[(size_t) k * nlev_all + m->atom_level_ptr[na->he_atom]];
       sca += na->ray_H2[lr] * c.nH2[k];
+      sca *= m->lamconst[(size_t) lr * RHG_LC_NCONST + RHG_LC_SCATT_FUDGE];
       chiref[k] += sca;
     }
   }
@@ -2281,6 +2282,8 @@ static int nlte_column(const rhg_model_t *m, const rhg_nlte_t *na, const column_
       if (na->he_atom >= 0)
         sca += na->ray_He[ls] * c->nstar[(size_t) k * nlev_all + m->atom_level_ptr[na->he_atom]];
       sca += na->ray_H2[ls] * c->nH2[k];
+      /* scattering fudge factor, rh/background.c:453-462 */
+      sca *= m->lamconst[(size_t) ls * RHG_LC_NCONST + RHG_LC_SCATT_FUDGE];
       bs[k] = sca;
       bc[k] += sca;
     }

@@ -92,14 +92,17 @@ def engine_for(cwd, wave, atomic_number=None, atomic_abundance=None, loggf_ids=N
 
 
 def nlte_engine_for(cwd, wave, atomic_number=None, atomic_abundance=None, loggf_ids=None,
-                    loggf_values=None, lam_ids=None, lam_values=None, device=0):
+                    loggf_values=None, lam_ids=None, lam_values=None, device=0,
+                    fudge_wave=None, fudge_value=None):
     """(engine, NLTE model) for a run directory with one ACTIVE atom."""
     skey, s = _setup(cwd, atomic_number, atomic_abundance, loggf_ids, loggf_values,
                      lam_ids, lam_values)
-    key = (skey, _key(np.asarray(wave, np.float64)), device)
+    fudge = _fudge(fudge_wave, fudge_value)
+    key = (skey, _key(np.asarray(wave, np.float64)), device,
+           _key(*(fudge if fudge is not None else (None,))))
     r = _NLTE.get(key)
     if r is None:
-        nm = _nlte.build_nlte(s, np.asarray(wave, np.float64))
+        nm = _nlte.build_nlte(s, np.asarray(wave, np.float64), fudge=fudge)
         r = (Engine(nm.lte, device), nm)
         _NLTE[key] = r
     return r
@@ -261,14 +264,11 @@ def compute1d(cwd, mu, atm_scale, atmosphere, wave, loggf_ids=None, loggf_values
     skey, s = _setup(cwd, atomic_number, atomic_abundance, loggf_ids, loggf_values,
                      lam_ids, lam_values)
     if any(a.active for a in s.atoms):
-        if fudge is not None:
-            raise NotImplementedError("opacity fudge factors with an ACTIVE atom (the "
-                                      "scattering factor of the NLTE background) are not "
-                                      "implemented")
         if get_atomic_rfs:
             raise NotImplementedError("get_atomic_rfs with an ACTIVE atom is not implemented")
         e, nm = nlte_engine_for(cwd, wave, atomic_number, atomic_abundance, loggf_ids,
-                                loggf_values, lam_ids, lam_values)
+                                loggf_values, lam_ids, lam_values, fudge_wave=fudge_wave,
+                                fudge_value=fudge_value)
         pops, _, _, st, nst = e.nlte_solve(nm, atm[None, :9], mu, atm_scale, stokes=True,
                                            nstar=True)
         lam = e.model.lam[e.model.out_index].copy()

@@ -105,6 +105,7 @@ __global__ void k_nlte_pre(const DevModel *__restrict__ Mp, const DevNlte *__res
     sca += A.ray_H[lr] * nH0;
     if (A.he_atom >= 0) sca += A.ray_He[lr] * nHe;
     sca += A.ray_H2[lr] * nH2;
+    sca *= M.lamconst[(int64_t) lr * RHG_LC_NCONST + RHG_LC_SCATT_FUDGE];
     st[RHG_CF_HEIGHT * fs] += sca;
   }
 
@@ -200,13 +201,15 @@ __global__ void k_nlte_bg(const DevModel *__restrict__ Mp, const DevNlte *__rest
   const double *st = B.state + (int64_t) k * ncol + ic;
   const double ne = st[RHG_CF_NE * fs], nH0 = st[RHG_CF_NH0 * fs], nH2 = st[RHG_CF_NH2 * fs];
   const double nHe = B.nHe[(int64_t) k * ncol + ic];
-  (void) Mp;
+  const DevModel &M = *Mp;
   for (int io = 0; io < B.nout; io++) {
     const int ls = out_ls[io];
     double sca = ne * A.sigma_thomson;
     sca += A.ray_H[ls] * nH0;
     if (A.he_atom >= 0) sca += A.ray_He[ls] * nHe;
     sca += A.ray_H2[ls] * nH2;
+    // the scattering fudge factor, rh/background.c:453-462
+    sca *= M.lamconst[(int64_t) ls * RHG_LC_NCONST + RHG_LC_SCATT_FUDGE];
     const int64_t q = (int64_t) io * fs + (int64_t) k * ncol + ic;
     B.csca[q] = sca;
     B.cchi[q] += sca;

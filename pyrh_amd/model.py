@@ -674,8 +674,9 @@ def build_model(setup: RunSetup, wave, nlte=False, check=True, lam_ref=None,
     (rh/rhf1d/pyrh_compute1dray.c:183-195): row 0 scales the H- opacity
     (rh/background.c:362-369, the per-wavelength factor RHG_LC_HMIN_FUDGE),
     row 2 the metal bound-free opacity (background.c:438-451, folded into
-    the metal continua's cross-sections), both by Linear in wavelength; row
-    1 scales the scattering opacity, which enters only an NLTE solution."""
+    the metal continua's cross-sections), row 1 the scattering opacity
+    (background.c:453-462, RHG_LC_SCATT_FUDGE; it enters only an NLTE
+    solution), each by Linear in wavelength."""
     if check:
         _check_supported(setup, nlte)
     kw = setup.kw
@@ -705,7 +706,7 @@ def build_model(setup: RunSetup, wave, nlte=False, check=True, lam_ref=None,
     hmbf_l = list(T.HMIN_BF_LAMBDA)
     hmbf_a = list(T.HMIN_BF_ALPHA)
     hmbf_M = spline_coef(hmbf_l, hmbf_a)
-    NC = 21
+    NC = 22
     lc = np.zeros((nspect, NC))
     if fudge is not None:
         fw = np.ascontiguousarray(fudge[0], np.float64).ravel()
@@ -714,10 +715,12 @@ def build_model(setup: RunSetup, wave, nlte=False, check=True, lam_ref=None,
             raise ValueError("fudge_value must be [3][len(fudge_wave)] with at least two "
                              "wavelengths")
         hmin_f = np.array([linear(fw, fv[0], float(x)) for x in lam])
+        scatt_f = np.array([linear(fw, fv[1], float(x)) for x in lam])
         metal_f = np.array([linear(fw, fv[2], float(x)) for x in lam])
     else:
-        hmin_f = metal_f = np.ones(nspect)
+        hmin_f = scatt_f = metal_f = np.ones(nspect)
     lc[:, 20] = hmin_f
+    lc[:, 21] = scatt_f
     for n, la in enumerate(lam):
         lc[n, 0] = la
         lc[n, 1] = (K.HPLANCK * K.CLIGHT) / (K.KBOLTZMANN * K.NM_TO_M * la)

@@ -302,7 +302,7 @@ def _broadening(setup, atom, ln):
     return vdw, cross, stark, c0, C, Cm
 
 
-def build_nlte(setup: RunSetup, wave) -> NLTEModel:
+def build_nlte(setup: RunSetup, wave, fudge=None) -> NLTEModel:
     """The NLTE problem of run directory `setup` with output wavelengths
     `wave` [nm, vacuum]."""
     kw = setup.kw
@@ -430,7 +430,7 @@ def build_nlte(setup: RunSetup, wave) -> NLTEModel:
     sigma_T = 8.0 * K.PI / 3.0 * math.pow(
         K.Q_ELECTRON / (math.sqrt(4.0 * K.PI * K.EPSILON_0) * (math.sqrt(K.M_ELECTRON) * K.CLIGHT)), 4)
     wave_bg = [x for x in lam if x != lam_ref]
-    lte = build_model(setup, np.array(wave_bg), nlte=True)
+    lte = build_model(setup, np.array(wave_bg), nlte=True, fudge=fudge)
     if not np.array_equal(lte.lam, lam):
         raise RuntimeError("background grid differs from the merged grid")
     isum = kw.int("I_SUM")

@@ -5,7 +5,8 @@ Runs pyrh's rhf1d() from the reference library compiled from its C sources
 (rh/rhf1d/pyrh_compute1dray.c:183-195: row 0 scales the H- opacity, row 1
 the scattering, row 2 the metal bound-free opacity, each by Linear in
 wavelength) on three seeded map columns of the LTE run directory at mu = 1
-and mu = 0.7.  Needs /root/reference; run from the repository root:
+and mu = 0.7, and on two columns of the NLTE run directory (Ca II-like atom
+ACTIVE: the emergent spectrum after convergence).  Needs /root/reference; run from the repository root:
 
     PYRH_PATH=tests/golden/rhdata python3 tests/golden/make_golden_fudge.py
 """
@@ -38,6 +39,17 @@ def main():
         out[name + "__mu"] = np.float64(mu)
         out["lam"] = lam
         print(name, out[name + "__stokes"].shape, flush=True)
+    # NLTE: the Ca II-like atom ACTIVE (tests/golden/run_nlte), where the
+    # scattering factor enters the background; the emergent spectrum
+    natm = W.stokes_map(2, seed=W.SEED_BASE + 48)
+    nwave = np.array([393.4776, 396.9591, 854.44])
+    lam, I, Q, U, V = refrun.run_synth(HERE / "run_nlte", natm, nwave, mu=1.0,
+                                       fudge=(FUDGE_WAVE, FUDGE_VALUE))[:5]
+    out["nlte__atmos"] = natm
+    out["nlte__wave"] = nwave
+    out["nlte__lam"] = lam
+    out["nlte__I"] = np.asarray(I)
+    print("nlte", np.asarray(I).shape, flush=True)
     np.savez_compressed(HERE / "fudge_fixtures.npz", **out)
 
 

@@ -234,3 +234,19 @@ def test_fudge_matches_reference(name):
                                     fudge_wave=f["fudge_wave"], fudge_value=f["fudge_value"])
     np.testing.assert_array_equal(np.stack([I, Q, U, V]), st[1])
     np.testing.assert_array_equal(lam, f["lam"])
+
+
+def test_nlte_fudge_matches_reference():
+    """compute1d with fudge factors on the NLTE run: the emergent intensity
+    after convergence against the reference's (1e-7, as the NLTE runs)."""
+    from conftest import GOLDEN
+    from pyrh_amd import api
+    f = np.load(GOLDEN / "fudge_fixtures.npz")
+    for c in range(f["nlte__atmos"].shape[0]):
+        I, Q, U, V, lam = api.compute1d(str(GOLDEN / "run_nlte"), 1.0, 0, f["nlte__atmos"][c],
+                                        f["nlte__wave"], fudge_wave=f["fudge_wave"],
+                                        fudge_value=f["fudge_value"])
+        err = float(np.max(np.abs(I - f["nlte__I"][c]) / f["nlte__I"][c]))
+        print(f"NLTE fudge column {c}: I {err:.2e}", flush=True)
+        assert err < 1e-7
+        np.testing.assert_array_equal(lam, f["nlte__lam"])

@@ -161,8 +161,15 @@ def test_oracle_matches_reference_fudge(name):
     assert float(np.max(np.abs(st0[0, 0] - ref[0, 0]) / ref[0, 0])) > 1e-3
 
 
-def test_fudge_with_active_atom_is_rejected():
-    from pyrh_amd import api
-    with pytest.raises(NotImplementedError):
-        api.compute1d(str(GOLDEN / "run_nlte"), 1.0, 0, np.zeros((9, 10)), [393.4],
-                      fudge_wave=[300.0, 900.0], fudge_value=np.ones((3, 2)))
+def test_oracle_nlte_fudge_matches_reference():
+    """Fudge factors with the Ca II-like atom ACTIVE (the scattering factor
+    enters the NLTE background): the oracle's emergent intensity after
+    convergence against the reference's, 1e-7 as for the NLTE runs."""
+    import pyoracle
+    from pyrh_amd import model, nlte
+    f = np.load(GOLDEN / "fudge_fixtures.npz")
+    nm = nlte.build_nlte(model.open_run(str(GOLDEN / "run_nlte")), f["nlte__wave"],
+                         fudge=(f["fudge_wave"], f["fudge_value"]))
+    *_, st = pyoracle.nlte_solve(nm, f["nlte__atmos"], stokes=True)
+    ref = f["nlte__I"]
+    assert float(np.max(np.abs(st[:, 0] - ref) / ref)) < 1e-7
