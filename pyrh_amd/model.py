@@ -628,8 +628,8 @@ def _check_supported(setup, nlte=False):
         raise NotImplementedError("BACKGROUND_POLARIZATION = TRUE")
     if kw.str("ATMOS_ITOP") != "none":
         raise NotImplementedError("ATMOS_ITOP irradiation at the top boundary")
-    if kw.str("STOKES_MODE") not in ("NO_STOKES", "FULL_STOKES"):
-        raise NotImplementedError(f"STOKES_MODE = {kw.str('STOKES_MODE')}")
+    if kw.str("STOKES_MODE") not in STOKES_MODES:
+        raise RHInputError(f"Unknown STOKES_MODE = {kw.str('STOKES_MODE')}")
     if kw.bool("MAGNETO_OPTICAL"):
         raise NotImplementedError("MAGNETO_OPTICAL = TRUE")
     if not nlte and any(a.active for a in setup.atoms):

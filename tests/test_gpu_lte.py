@@ -250,3 +250,20 @@ def test_nlte_fudge_matches_reference():
         print(f"NLTE fudge column {c}: I {err:.2e}", flush=True)
         assert err < 1e-7
         np.testing.assert_array_equal(lam, f["nlte__lam"])
+
+
+@pytest.mark.parametrize("mode,tag", [(m, t) for m in ("field_free", "polarization_free")
+                                      for t in ("mu1", "mu07")])
+def test_stokes_modes_match_reference(mode, tag):
+    """STOKES_MODE = FIELD_FREE / POLARIZATION_FREE (no active atom) on the
+    GPU against the reference's rhf1d(), FP64 within 1e-10."""
+    from conftest import GOLDEN
+    from pyrh_amd import api
+    f = np.load(GOLDEN / "modes_fixtures.npz")
+    e = api.engine_for(str(GOLDEN / f"run_lte_{mode}"), f["wave"])
+    st = e.synth_lte(f["atmos"], float(f[f"{mode}_{tag}__mu"]))
+    ref = f[f"{mode}_{tag}__stokes"]
+    err = float(np.max(np.abs(st[:, 0] - ref[:, 0]) / ref[:, 0]))
+    print(f"{mode} {tag}: I {err:.2e}", flush=True)
+    assert err < 1e-10
+    assert np.all(st[:, 1:] == 0.0)

@@ -173,3 +173,21 @@ def test_oracle_nlte_fudge_matches_reference():
     *_, st = pyoracle.nlte_solve(nm, f["nlte__atmos"], stokes=True)
     ref = f["nlte__I"]
     assert float(np.max(np.abs(st[:, 0] - ref) / ref)) < 1e-7
+
+
+MODE_CASES = [(m, t) for m in ("field_free", "polarization_free") for t in ("mu1", "mu07")]
+
+
+@pytest.mark.parametrize("mode,tag", MODE_CASES)
+def test_oracle_matches_reference_stokes_modes(mode, tag):
+    """STOKES_MODE = FIELD_FREE / POLARIZATION_FREE with no active atom:
+    Stokes I of the scalar solvers on the Zeeman-split opacity, against the
+    reference (tests/golden/make_golden_modes.py)."""
+    import pyoracle
+    from pyrh_amd import model
+    f = np.load(GOLDEN / "modes_fixtures.npz")
+    m = model.build_model(model.open_run(str(GOLDEN / f"run_lte_{mode}")), f["wave"])
+    st = pyoracle.synth_lte(m, f["atmos"], mu=float(f[f"{mode}_{tag}__mu"]))
+    ref = f[f"{mode}_{tag}__stokes"]
+    assert float(np.max(np.abs(st[:, 0] - ref[:, 0]) / ref[:, 0])) < 1e-10
+    assert np.all(st[:, 1:] == 0.0) and np.all(ref[:, 1:] == 0.0)
