@@ -562,8 +562,8 @@ k_nlte_bezier(const DevModel *__restrict__ Mp, const DevNlte *__restrict__ Ap, N
           const double dsup = dzu * zmu[r], dsdn = dzd * zmu[r];
           const double dchi_up = (C0[r] - Cm[r]) / dsup;
           dchi_c[r] = cent_div(dsup, dsdn, Cm[r], C0[r], Cp[r]);
-          const double c1 = max0(C0[r] - (dsup / 3.0) * dchi_c[r]);
-          const double c2 = max0(Cm[r] + (dsup / 3.0) * dchi_up);
+          const double c1 = max0(C0[r] - (dsup * (1.0 / 3.0)) * dchi_c[r]);
+          const double c2 = max0(Cm[r] + (dsup * (1.0 / 3.0)) * dchi_up);
           dtau_uw[r] = dsup * (C0[r] + Cm[r] + c1 + c2) * 0.25;
           dS_up[r] = (S0[r] - Sm[r]) / dtau_uw[r];
         }
@@ -588,10 +588,10 @@ k_nlte_bezier(const DevModel *__restrict__ Mp, const DevNlte *__restrict__ Ap, N
             const double dsdn = dzd * zmu[r];
             const double dchi_dn = far ? cent_div(dsdn, dzd2 * zmu[r], C0[r], Cp[r], Cn[r])
                                        : (Cp[r] - C0[r]) / dsdn;
-            double c1 = max0(C0[r] + (dsdn / 3.0) * dchi_c[r]);
-            double c2 = max0(Cp[r] - (dsdn / 3.0) * dchi_dn);
+            double c1 = max0(C0[r] + (dsdn * (1.0 / 3.0)) * dchi_c[r]);
+            double c2 = max0(Cp[r] - (dsdn * (1.0 / 3.0)) * dchi_dn);
             const double dtau_dw = dsdn * (C0[r] + Cp[r] + c1 + c2) * 0.25;
-            const double dt03 = dtau_uw[r] / 3.0;
+            const double dt03 = dtau_uw[r] * (1.0 / 3.0);
             double alpha, beta, gamma, theta, eps;
             bezier3_coeffs(dtau_uw[r], alpha, beta, gamma, theta, eps);
             const double dS_c = cent_div(dtau_uw[r], dtau_dw, Sm[r], S0[r], Sp[r]);
@@ -896,8 +896,8 @@ k_nlte_emergent(const DevModel *__restrict__ Mp, const DevNlte *__restrict__ Ap,
     double dsup = dzb(k + 1) * zmu, dsdn = dzb(k) * zmu;
     double dchi_up = (CHI(k) - CHI(k + 1)) / dsup;
     double dchi_c = cent_div(dsup, dsdn, CHI(k + 1), CHI(k), CHI(k - 1));
-    double c1 = max0(CHI(k) - (dsup / 3.0) * dchi_c);
-    double c2 = max0(CHI(k + 1) + (dsup / 3.0) * dchi_up);
+    double c1 = max0(CHI(k) - (dsup * (1.0 / 3.0)) * dchi_c);
+    double c2 = max0(CHI(k + 1) + (dsup * (1.0 / 3.0)) * dchi_up);
     dtau_uw = dsup * (CHI(k) + CHI(k + 1) + c1 + c2) * 0.25;
     double dS_up = (SRC(k) - SRC(k + 1)) / dtau_uw;
     for (k = N - 2; k >= 1; k--) {
@@ -909,8 +909,8 @@ k_nlte_emergent(const DevModel *__restrict__ Mp, const DevNlte *__restrict__ Ap,
       } else {
         dchi_dn = (CHI(k - 1) - CHI(k)) / dsdn;
       }
-      c1 = max0(CHI(k) + (dsdn / 3.0) * dchi_c);
-      c2 = max0(CHI(k - 1) - (dsdn / 3.0) * dchi_dn);
+      c1 = max0(CHI(k) + (dsdn * (1.0 / 3.0)) * dchi_c);
+      c2 = max0(CHI(k - 1) - (dsdn * (1.0 / 3.0)) * dchi_dn);
       const double dtau_dw = dsdn * (CHI(k) + CHI(k - 1) + c1 + c2) * 0.25;
       const double dt03 = dtau_uw / 3.0;
       double alpha, beta, gamma, theta, eps;
