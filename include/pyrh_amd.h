@@ -68,6 +68,7 @@ enum rhg_lamconst {
   RHG_LC_HFF_LONG0,      /* 5 coefficients of the long-wave H- ff fit ... */
   RHG_LC_HMIN_FUDGE = RHG_LC_HFF_LONG0 + 5, /* H- opacity fudge factor (1: none)  */
   RHG_LC_SCATT_FUDGE,    /* scattering-opacity fudge factor (NLTE background)    */
+  RHG_LC_ITOP,           /* intensity irradiating the top (ATMOS_ITOP; 0: none)  */
   RHG_LC_NCONST
 };
 

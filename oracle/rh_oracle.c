@@ -793,10 +793,20 @@ void rho_formal_bezier_rf(int Ndep, const double *z, double mu, const double *ch
 
 /* Feautrier solution, STANDARD order, ZERO upper boundary, thermalised
    lower boundary; returns the emergent intensity.  rh/rhf1d/feautrier.c:56-202 */
+static double formal_feautrier_h0(int Ndep, const double *z, double mu, const double *chi,
+                                  const double *S, const double *B2, double h0);
+
 double rho_formal_feautrier(int Ndep, const double *z, double mu,
                             const double *chi, const double *S, const double *B2)
 {
-  double r0 = 0.0, h0 = 0.0, rN = 0.0, hN, f0, fN, dtau_mid, zmu = 0.5 / mu;
+  return formal_feautrier_h0(Ndep, z, mu, chi, S, B2, 0.0);
+}
+
+/* h0: the intensity irradiating the top (rh/rhf1d/feautrier.c:84-97) */
+static double formal_feautrier_h0(int Ndep, const double *z, double mu, const double *chi,
+                                  const double *S, const double *B2, double h0)
+{
+  double r0 = 0.0, rN = 0.0, hN, f0, fN, dtau_mid, zmu = 0.5 / mu;
   double *dtau = malloc(8 * (size_t) Ndep), *abc = malloc(8 * (size_t) Ndep),
          *A1 = malloc(8 * (size_t) Ndep), *C1 = malloc(8 * (size_t) Ndep),
          *F = malloc(8 * (size_t) Ndep), *ztmp = malloc(8 * (size_t) Ndep),
@@ -1599,7 +1609,8 @@ static int synth_column_ex(const rhg_model_t *m, const rhg_columns_t *cols, int 
       Iout[0] = I4[0];
     } else {
       for (k = 0; k < N; k++) S4[k] = (0.0 + eta_c[k] + 0.0) / chi_c[k];
-      Iout[0] = rho_formal_feautrier(N, c.height, mu, chi_c, S4, B2);
+      Iout[0] = formal_feautrier_h0(N, c.height, mu, chi_c, S4, B2,
+                                    m->lamconst[(size_t) ls * RHG_LC_NCONST + RHG_LC_ITOP]);
     }
     for (n = 0; n < 4; n++) stokes[n * (nspect - 1) + nout] = Iout[n];
     nout++;

@@ -189,7 +189,8 @@ k_formal_scalar(const DevModel *__restrict__ Mp, const int32_t *__restrict__ nde
         dtau_prev = zmu * (chi_prev + chi_j) * (Z(0) - Z(1));
         const double abc0 = 1.0 + 2.0 * 1.0 / dtau_prev;
         const double C10 = 2.0 / (dtau_prev * dtau_prev);
-        const double Stmp = S_prev + 2.0 * 0.0 / ((1.0 + 0.0) * dtau_prev);
+        // h0: the intensity irradiating the top (0 without ATMOS_ITOP), feautrier.c:84-97
+        const double Stmp = S_prev + 2.0 * lc[RHG_LC_ITOP] / ((1.0 + 0.0) * dtau_prev);
         F = abc0 / C10;
         ztmp = Stmp / (abc0 + C10);
         P0acc = ztmp;
@@ -217,7 +218,7 @@ k_formal_scalar(const DevModel *__restrict__ Mp, const int32_t *__restrict__ nde
     const double StmpN = S_prev + 2.0 * hN / ((1.0 + 0.0) * dtau_prev);
     const double PN = (StmpN + A1N * ztmp) / (abcN + A1N * (F / (1.0 + F)));
     const double P0 = P0acc + PN * prod;
-    out[0] = (1.0 + 1.0) * P0 - 0.0 / (1.0 + 0.0);
+    out[0] = (1.0 + 1.0) * P0 - lc[RHG_LC_ITOP] / (1.0 + 0.0);
   }
   out[nout] = 0.0;
   out[2 * (int64_t) nout] = 0.0;

@@ -626,8 +626,8 @@ def _check_supported(setup, nlte=False):
         raise NotImplementedError("RLK_SCATTER = TRUE")
     if kw.bool("BACKGROUND_POLARIZATION"):
         raise NotImplementedError("BACKGROUND_POLARIZATION = TRUE")
-    if kw.str("ATMOS_ITOP") != "none":
-        raise NotImplementedError("ATMOS_ITOP irradiation at the top boundary")
+    if nlte and kw.str("ATMOS_ITOP") != "none":
+        raise NotImplementedError("ATMOS_ITOP irradiation at the top boundary with an ACTIVE atom")
     if kw.str("STOKES_MODE") not in STOKES_MODES:
         raise RHInputError(f"Unknown STOKES_MODE = {kw.str('STOKES_MODE')}")
     if kw.bool("MAGNETO_OPTICAL"):
@@ -706,7 +706,7 @@ def build_model(setup: RunSetup, wave, nlte=False, check=True, lam_ref=None,
     hmbf_l = list(T.HMIN_BF_LAMBDA)
     hmbf_a = list(T.HMIN_BF_ALPHA)
     hmbf_M = spline_coef(hmbf_l, hmbf_a)
-    NC = 22
+    NC = 23
     lc = np.zeros((nspect, NC))
     if fudge is not None:
         fw = np.ascontiguousarray(fudge[0], np.float64).ravel()
@@ -721,6 +721,19 @@ def build_model(setup: RunSetup, wave, nlte=False, check=True, lam_ref=None,
         hmin_f = scatt_f = metal_f = np.ones(nspect)
     lc[:, 20] = hmin_f
     lc[:, 21] = scatt_f
+    # irradiation at the top (ATMOS_ITOP, rh/rhf1d/multiatmos.c:185-222): an
+    # XDR file of Nspect x Nrays doubles on the merged grid, read after
+    # rhf1d() has set one ray (rh/rhf1d/pyrh_compute1dray.c:293, 308); only
+    # the Feautrier solver of the emergent ray uses it (feautrier.c:94-96)
+    itop = kw.str("ATMOS_ITOP")
+    if itop != "none":
+        path = itop if os.path.isabs(itop) else os.path.join(setup.cwd, itop)
+        if not os.path.exists(path):
+            raise RHInputError(f"Unable to open inputfile {itop}")
+        v = np.fromfile(path, dtype=">f8")
+        if v.size < nspect:
+            raise RHInputError("Unable to read irradiation data at TOP of atmosphere")
+        lc[:, 22] = v[:nspect]
     for n, la in enumerate(lam):
         lc[n, 0] = la
         lc[n, 1] = (K.HPLANCK * K.CLIGHT) / (K.KBOLTZMANN * K.NM_TO_M * la)

@@ -267,3 +267,19 @@ def test_stokes_modes_match_reference(mode, tag):
     print(f"{mode} {tag}: I {err:.2e}", flush=True)
     assert err < 1e-10
     assert np.all(st[:, 1:] == 0.0)
+
+
+@pytest.mark.parametrize("tag", ("mu1", "mu07"))
+def test_itop_matches_reference(tag):
+    """ATMOS_ITOP on the GPU against the reference's rhf1d() (1e-10)."""
+    from conftest import GOLDEN
+    from pyrh_amd import api
+    f = np.load(GOLDEN / "itop_fixtures.npz")
+    e = api.engine_for(str(GOLDEN / "run_lte_itop"), f["wave"])
+    st = e.synth_lte(f["atmos"], float(f[tag + "__mu"]))
+    ref = f[tag + "__stokes"]
+    Ic = ref[:, 0, :1]
+    eI = float(np.max(np.abs(st[:, 0] - ref[:, 0]) / ref[:, 0]))
+    eP = float(np.max(np.abs(st[:, 1:] - ref[:, 1:]) / Ic[:, None]))
+    print(f"itop {tag}: I {eI:.2e}  QUV/Ic {eP:.2e}", flush=True)
+    assert eI < 1e-10 and eP < 1e-10

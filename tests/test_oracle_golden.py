@@ -191,3 +191,21 @@ def test_oracle_matches_reference_stokes_modes(mode, tag):
     ref = f[f"{mode}_{tag}__stokes"]
     assert float(np.max(np.abs(st[:, 0] - ref[:, 0]) / ref[:, 0])) < 1e-10
     assert np.all(st[:, 1:] == 0.0) and np.all(ref[:, 1:] == 0.0)
+
+
+@pytest.mark.parametrize("tag", ("mu1", "mu07"))
+def test_oracle_matches_reference_itop(tag):
+    """ATMOS_ITOP: the intensity irradiating the top enters the Feautrier
+    solution of the continuum wavelengths (rh/rhf1d/feautrier.c:94-96),
+    against the reference (tests/golden/make_golden_itop.py); the effect
+    (5e-4 of I) is well above the tolerance."""
+    import pyoracle
+    from pyrh_amd import model
+    f = np.load(GOLDEN / "itop_fixtures.npz")
+    m = model.build_model(model.open_run(str(GOLDEN / "run_lte_itop")), f["wave"])
+    st = pyoracle.synth_lte(m, f["atmos"], mu=float(f[tag + "__mu"]))
+    ref = f[tag + "__stokes"]
+    assert float(np.max(np.abs(st[:, 0] - ref[:, 0]) / ref[:, 0])) < 1e-10
+    m0 = model.build_model(model.open_run(str(GOLDEN / "run_lte")), f["wave"])
+    st0 = pyoracle.synth_lte(m0, f["atmos"][:1], mu=float(f[tag + "__mu"]))
+    assert float(np.max(np.abs(st0[0, 0] - ref[0, 0]) / ref[0, 0])) > 1e-4
