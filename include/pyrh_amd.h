@@ -39,11 +39,17 @@ enum rhg_status {
 };
 
 /* Solver used for one wavelength, decided from the background flags
- * exactly as rh/rhf1d/formal.c:78-103 does. */
+ * exactly as rh/rhf1d/formal.c:78-103 does.  BEZIER becomes FEAUTRIER in a
+ * static column (the angle-independent branch, formal.c:287-310);
+ * BEZIER_ANGLE does not: a polarizable Kurucz line outside FULL_STOKES still
+ * marks the wavelength polarized (pyrh sets atmos.Stokes,
+ * rh/rhf1d/pyrh_compute1dray.c:258; rh/kurucz.c:656-659), which keeps it
+ * angle-dependent (formal.c:98-101). */
 enum rhg_solver {
-  RHG_SOLVER_FEAUTRIER = 0, /* no line at this wavelength          */
-  RHG_SOLVER_BEZIER = 1,    /* unpolarized cubic Bezier (I only)    */
-  RHG_SOLVER_STOKES = 2     /* cubic DELO-Bezier, full Stokes       */
+  RHG_SOLVER_FEAUTRIER = 0,   /* no line at this wavelength                */
+  RHG_SOLVER_BEZIER = 1,      /* unpolarized cubic Bezier (I only)         */
+  RHG_SOLVER_STOKES = 2,      /* cubic DELO-Bezier, full Stokes            */
+  RHG_SOLVER_BEZIER_ANGLE = 3 /* cubic Bezier (I only) in every column     */
 };
 
 enum rhg_vdw { RHG_VDW_KURUCZ = 0, RHG_VDW_UNSOLD = 1, RHG_VDW_BARKLEM = 2 };
@@ -297,6 +303,7 @@ enum rhg_precision { RHG_PREC_FP64 = 0, RHG_PREC_FP32 = 1 };
 int rhg_synth_lte(rhg_context_t *ctx, const rhg_columns_t *cols,
                   int32_t precision, int32_t device_ptrs, double *stokes_out);
 
+
 /* Response functions of the emergent Stokes vector: dS(lambda)/dp(k) for
  * the atmosphere parameters below at every depth k, for the wavelengths
  * solved with the polarized DELO-Bezier solver (zero at the others).  Units
@@ -304,7 +311,11 @@ int rhg_synth_lte(rhg_context_t *ctx, const rhg_columns_t *cols,
  * columns and Stokes output as rhg_synth_lte; rf_out holds
  *   rf[((((p * ndep_max + k) * 4 + s) * (nspect-1) + l) * ncol + c)]
  * in float for RHG_PREC_FP32 and double for RHG_PREC_FP64 (device memory if
- * device_ptrs).  At most RHG_RF_MAX_COLUMNS columns per call.  The reference
+ * device_ptrs).  At most RHG_RF_MAX_COLUMNS columns per call, solved in
+ * column chunks whose work buffers fit in half of the free device memory
+ * (the forward sweep keeps 13 FP64 scalars per depth step and the reverse
+ * sweep 36 point and path-length adjoints per depth and Stokes parameter,
+ * for every polarized wavelength of the chunk's columns).  The reference
  * computes only d I / d log gf (rh/kurucz.c:224-260); these have no
  * counterpart there. */
 enum rhg_rf_param {
@@ -316,9 +327,19 @@ enum rhg_rf_param {
   RHG_RF_VMIC,     /* microturbulence, row 4                 */
   RHG_RF_NPARAM
 };
-#define RHG_RF_MAX_COLUMNS 4096
+#define RHG_RF_MAX_COLUMNS 65536  /* per call; run in column chunks that fit the device */
 int rhg_rf_lte(rhg_context_t *ctx, const rhg_columns_t *cols, int32_t precision,
                int32_t device_ptrs, double *stokes_out, void *rf_out);
+
+/* The same for a subset of the parameters: params[0..nparam-1] are
+ * distinct rhg_rf_param ids, and rf_out holds
+ *   rf[((((i * ndep_max + k) * 4 + s) * (nspect-1) + l) * ncol + c)]
+ * for the i-th of them.  rhg_rf_lte is this call with all six in enum
+ * order.  Only the requested parameters' state tangents and chain terms are
+ * computed.  No counterpart in the reference (see rhg_rf_lte). */
+int rhg_rf_lte_params(rhg_context_t *ctx, const rhg_columns_t *cols, int32_t precision,
+                      int32_t device_ptrs, int32_t nparam, const int32_t *params,
+                      double *stokes_out, void *rf_out);
 
 /* d I / d log gf of the emergent intensity, the reference's analytic atomic
  * response functions: compute1d(get_atomic_rfs=True) (pyrh.pyx:604-660;
@@ -442,6 +463,12 @@ int rhg_synth_lte_diag(rhg_context_t *ctx, const rhg_columns_t *cols,
  * formal solution. */
 #define RHG_NSTAGE 4
 int rhg_last_timings(rhg_context_t *ctx, double *ms_out);
+
+/* Kernel milliseconds of the last rhg_rf_lte / rhg_rf_lte_params call, 5
+ * entries: synthesis inputs (state, geometry, continuum), state tangents,
+ * forward sweep, reverse sweep, chain to the parameters (with the scalar
+ * wavelengths' synthesis). */
+int rhg_rf_timings(rhg_context_t *ctx, double *ms_out);
 
 /* Algorithmic work census of the last rhg_synth_lte call on this context
  * (its columns, depth counts and column state must still be alive): sums

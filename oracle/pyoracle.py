@@ -52,14 +52,14 @@ def lib():
         L.rho_planck.argtypes = [C.c_double, C.c_double]
         L.rho_planck.restype = C.c_double
         P = C.POINTER(C.c_double)
-        L.rho_formal_stokes.argtypes = [C.c_int, P, C.c_double, P, P, P, P, C.c_int, P]
+        L.rho_formal_stokes.argtypes = [C.c_int, P, C.c_double, P, P, P, P, P, C.c_int, P]
         L.rho_formal_bezier.argtypes = [C.c_int, P, C.c_double, P, P, P, P]
         L.rho_formal_feautrier.argtypes = [C.c_int, P, C.c_double, P, P, P]
         L.rho_formal_feautrier.restype = C.c_double
         L.rho_rf_loggf.argtypes = [C.POINTER(RhgModel), C.POINTER(RhgColumns), C.c_int,
                                    C.POINTER(C.c_int), C.c_int, P, P]
         L.rho_rf_loggf.restype = C.c_int
-        L.rho_formal_bezier_in.argtypes = [C.c_int, P, C.c_double, P, P, P]
+        L.rho_formal_bezier_in.argtypes = [C.c_int, P, C.c_double, P, P, C.c_double, P]
         L.rho_formal_bezier_rf.argtypes = [C.c_int, P, C.c_double, P, P, P, C.c_int, P, P, P, P]
         L.rho_model_sizeof.restype = C.c_int
         L.rho_columns_sizeof.restype = C.c_int
@@ -131,11 +131,12 @@ def nlte_solve(nm, atmos, ndep=None, atm_scale=0, nthreads=0, mu=1.0, stokes=Fal
     return (pops, niter, dpops, st) if stokes else (pops, niter, dpops)
 
 
-def formal_bezier_in(z, mu, chi, S):
-    """Scalar Bezier intensity along the inward ray at every depth."""
+def formal_bezier_in(z, mu, chi, S, Itop=0.0):
+    """Scalar Bezier intensity along the inward ray at every depth (incident
+    intensity Itop at the top)."""
     z, chi, S = (np.ascontiguousarray(x, np.float64) for x in (z, chi, S))
     I = np.zeros(z.size)
-    lib().rho_formal_bezier_in(z.size, _p(z), float(mu), _p(chi), _p(S), _p(I))
+    lib().rho_formal_bezier_in(z.size, _p(z), float(mu), _p(chi), _p(S), float(Itop), _p(I))
     return I
 
 
@@ -180,13 +181,14 @@ def planck(T, lam_nm):
     return lib().rho_planck(float(T), float(lam_nm))
 
 
-def formal_stokes(z, mu, chi, S4, chiQUV, B2=None, inverse_fp32=False):
+def formal_stokes(z, mu, chi, S4, chiQUV, B2=None, inverse_fp32=False, chip=None):
     n = len(z)
     z, chi = np.ascontiguousarray(z, float), np.ascontiguousarray(chi, float)
     S4, q = np.ascontiguousarray(S4, float), np.ascontiguousarray(chiQUV, float)
     b = None if B2 is None else np.ascontiguousarray(B2, float)
+    r = None if chip is None else np.ascontiguousarray(chip, float)
     I4 = np.zeros((4, n))
-    lib().rho_formal_stokes(n, _p(z), float(mu), _p(chi), _p(S4), _p(q), _p(b),
+    lib().rho_formal_stokes(n, _p(z), float(mu), _p(chi), _p(S4), _p(q), _p(r), _p(b),
                             int(inverse_fp32), _p(I4))
     return I4
 

@@ -48,6 +48,7 @@
 #include "spectrum.h"
 #include "background.h"
 #include "rhf1d/pyrh_compute1dray.h"
+#include "inputs.h"
 
 void get_ne_from_nH(char *cwd, int atm_scale, int Ndep, double *scale, double *temp,
                     double *nH, double *ne);
@@ -63,6 +64,7 @@ void hse(char *cwd, int Ndep, double *scale, double *temp, double *ne,
 extern Atmosphere atmos;
 extern Geometry geometry;
 extern Spectrum spectrum;
+extern InputData input;
 
 static FILE *cap = NULL;
 static int cur_col = 0;
@@ -96,6 +98,35 @@ int writeBackground(int nspect, int mu, bool_t to_obs, double *chi_c,
   rec("bg_eta", nspect, mu, to_obs, eta_c, nrec * N);
   rec("bg_sca", nspect, mu, to_obs, sca_c, N);
   return real_writeBackground(nspect, mu, to_obs, chi_c, eta_c, sca_c, chip_c);
+}
+
+/* --- storage of the background records (rh/sortlambda.c:564-585).  With
+   MAGNETO_OPTICAL the library gives each wavelength's magneto-optical record
+   3 * Nspace doubles, but writeBackground copies NrecStokes = 4 records
+   into it (rh/readj.c:328), writing Nspace doubles past the block.  This
+   allocation gives that record 4 * Nspace doubles, so the copy stays inside
+   its block; everything else is as the library allocates it, and the
+   library's freeOpacityEmissivity releases it.  Results read only the
+   first 3 * Nspace doubles (rh/readj.c:252-262). ------------------------ */
+void allocateOpacityEmissivity(void)
+{
+  int nspect, N = atmos.Nspace;
+
+  spectrum.chi_c_lam = (double **) malloc(spectrum.Nspect * sizeof(double *));
+  for (nspect = 0; nspect < spectrum.Nspect; nspect++)
+    spectrum.chi_c_lam[nspect] = (double *) malloc(4 * N * sizeof(double));
+  if (input.magneto_optical) {
+    spectrum.chip_c_lam = (double **) malloc(spectrum.Nspect * sizeof(double *));
+    for (nspect = 0; nspect < spectrum.Nspect; nspect++)
+      spectrum.chip_c_lam[nspect] = (double *) malloc(4 * N * sizeof(double));
+  }
+  spectrum.eta_c_lam = (double **) malloc(spectrum.Nspect * sizeof(double *));
+  for (nspect = 0; nspect < spectrum.Nspect; nspect++)
+    spectrum.eta_c_lam[nspect] = (double *) malloc(4 * N * sizeof(double));
+  spectrum.sca_c_lam = (double **) malloc(spectrum.Nspect * sizeof(double *));
+  for (nspect = 0; nspect < spectrum.Nspect; nspect++)
+    spectrum.sca_c_lam[nspect] = (double *) malloc(4 * N * sizeof(double));
+  if (getenv("RHREF_TRACE_ALLOC")) fprintf(stderr, "harness: background records allocated\n");
 }
 
 /* --- continuum processes (outputs per lambda) ----------------------- */

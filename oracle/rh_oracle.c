@@ -431,18 +431,21 @@ static void solve4(double Md[4][4], const double *V0, double *V1, int fp32)
   }
 }
 
-/* reduced propagation matrix K' (rh/stokesopac.c:28-88), MO terms off;
-   q[3] are chi_Q,U,V at this depth */
-static void stokesK(const double *q, double chi_I, double K[4][4])
+/* reduced propagation matrix K' (rh/stokesopac.c:28-88); q[3] are
+   chi_Q,U,V at this depth, rho[3] the magneto-optical chip_Q,U,V (NULL: the
+   terms are off) */
+static void stokesK(const double *q, const double *rho, double chi_I, double K[4][4])
 {
   int i, j;
   for (j = 0; j < 4; j++) for (i = 0; i < 4; i++) K[j][i] = 0.0;
   K[0][1] = q[0]; K[0][2] = q[1]; K[0][3] = q[2];
+  if (rho) { K[1][2] = rho[2]; K[1][3] = rho[1]; K[2][3] = rho[0]; }
   for (j = 0; j < 3; j++)
     for (i = j + 1; i < 4; i++) {
       K[j][i] /= chi_I;
       K[i][j] = K[j][i];
     }
+  if (rho) { K[1][3] *= -1.0; K[2][1] *= -1.0; K[3][2] *= -1.0; }
 }
 
 /* m4m: c = b . a as rh/bezier_aux.c:94-103 accumulates it */
@@ -458,7 +461,7 @@ static void m4m(double a[4][4], double b[4][4], double c[4][4])
 /* Cubic DELO-Bezier polarized solver for the ray leaving the top
    (to_obs), rh/rhf1d/bezier_1D.c:52-300.  S4/chiQUV/I4 are [4|3][n]. */
 void rho_formal_stokes(int Ndep, const double *z, double mu, const double *chi,
-                       const double *S4, const double *chiQUV,
+                       const double *S4, const double *chiQUV, const double *chip,
                        const double *B2, int fp32, double *I4)
 {
   static const double ident[4][4] = {{1,0,0,0},{0,1,0,0},{0,0,1,0},{0,0,0,1}};
@@ -467,12 +470,14 @@ void rho_formal_stokes(int Ndep, const double *z, double mu, const double *chi,
          I_upw[4], dchi_up, dchi_c, dt03, dsup, dsdn, dt, co[5];
   double Ku[4][4], K0[4][4], Kd[4][4], dKu[4][4], dK0[4][4];
   double Su[4], S0[4], Sd[4], dSu[4], dS0[4];
-  double A[4][4], Ma[4][4], Mb[4][4], Mc[4][4], Md[4][4], V0[4], V1[4], q[3];
+  double A[4][4], Ma[4][4], Mb[4][4], Mc[4][4], Md[4][4], V0[4], V1[4], q[3], r[3];
   int k, n, m, i, j;
 #define IQ(s, kk) I4[(s) * Ndep + (kk)]
 #define SV(s, kk) S4[(s) * Ndep + (kk)]
 #define LOADK(kk, KK) do { q[0] = chiQUV[kk]; q[1] = chiQUV[Ndep + (kk)]; \
-    q[2] = chiQUV[2 * Ndep + (kk)]; stokesK(q, chi[kk], KK); } while (0)
+    q[2] = chiQUV[2 * Ndep + (kk)]; \
+    if (chip) { r[0] = chip[kk]; r[1] = chip[Ndep + (kk)]; r[2] = chip[2 * Ndep + (kk)]; } \
+    stokesK(q, chip ? r : NULL, chi[kk], KK); } while (0)
 
   dtau_uw = 0.5 * imu * (chi[k_start] + chi[k_start + dk]) *
             fabs(z[k_start] - z[k_start + dk]);
@@ -634,14 +639,15 @@ void rho_formal_bezier(int Ndep, const double *z, double mu, const double *chi,
    with zero incident intensity at the top (ZERO boundary,
    rh/rhf1d/multiatmos.c:62-66).  rh/rhf1d/bezier_1D.c:306-541 */
 void rho_formal_bezier_in(int Ndep, const double *z, double mu, const double *chi,
-                          const double *S, double *I)
+                          const double *S, double Itop, double *I)
 {
   const int dk = 1, k_start = 0, k_end = Ndep - 1;
   double zmu = 1.0 / mu, dtau_uw, dtau_dw = 0.0, dS_uw, I_upw, c1, c2, w[3];
   double dsup, dsdn, dt03, co[5], dS_up, dS_c, dchi_up, dchi_c, dchi_dn = 0.0, dsdn2;
   int k;
 
-  I_upw = 0.0;
+  /* ZERO or IRRADIATED top boundary (rh/rhf1d/bezier_1D.c:370-380) */
+  I_upw = Itop;
   I[k_start] = I_upw;
   k = k_start + dk;
   dsup = fabs(z[k] - z[k - dk]) * zmu;
@@ -1284,13 +1290,14 @@ static void background_continuum(const rhg_model_t *m, const column_t *c, int ls
    sel >= 0 keeps only model line sel (its chi_l phi and eta_l phi, as
    rh/kurucz.c:696-699 records for the log gf response). */
 static int line_opacity_sel(const rhg_model_t *m, const column_t *c, int ls,
-                            double mu, double *chi, double *eta, int sel)
+                            double mu, double *chi, double *eta, double *chip, int sel)
 {
   const double lambda = m->lamconst[(size_t) ls * RHG_LC_NCONST + RHG_LC_LAMBDA];
   int p0 = m->lam_line_ptr[ls], p1 = m->lam_line_ptr[ls + 1], e, k, N = c->Ndep;
   const int nlev = m->atom_level_ptr[m->natom];
   if (p1 == p0) return 0;
   for (k = 0; k < 4 * N; k++) { chi[k] = 0.0; eta[k] = 0.0; }
+  if (chip) for (k = 0; k < 3 * N; k++) chip[k] = 0.0;
   for (e = p0; e < p1; e++) {
     int l = m->lam_line_idx[e], ie = m->line_el[l], st = m->line_stage[l];
     if (sel >= 0 && l != sel) continue;
@@ -1300,6 +1307,7 @@ static int line_opacity_sel(const rhg_model_t *m, const column_t *c, int ls,
     int c0 = m->line_comp_ptr[l], c1 = m->line_comp_ptr[l + 1];
     for (k = 0; k < N; k++) {
       double T = c->T[k], vbroad, v, sv, phi, phi_Q = 0, phi_U = 0, phi_V = 0;
+      double psi_Q = 0, psi_U = 0, psi_V = 0;
       double pf = linear1(m->npf, m->Tpf, lp, T);
       vbroad = sqrt(vtherm * T + SQ(c->vturb[k]));
       v = (lambda / lambda0 - 1.0) * CLIGHT / vbroad;
@@ -1320,13 +1328,14 @@ static int line_opacity_sel(const rhg_model_t *m, const column_t *c, int ls,
           double sin2_gamma = 1.0 - SQ(c->cosg[k]);
           double vB = m->line_larmor[l] * c->B[k] / vbroad, sign = 1.0;
           double phi_sm = 0, phi_pi = 0, phi_sp = 0, phi_sigma, phi_delta, H, F;
+          double psi_sm = 0, psi_pi = 0, psi_sp = 0, psi_sigma, psi_delta;
           int nz;
           for (nz = c0; nz < c1; nz++) {
             H = rho_voigt_humlicek(adamp, v - m->comp_shift[nz] * vB, &F);
             switch (m->comp_q[nz]) {
-            case -1: phi_sm += m->comp_strength[nz] * H; break;
-            case 0: phi_pi += m->comp_strength[nz] * H; break;
-            case 1: phi_sp += m->comp_strength[nz] * H; break;
+            case -1: phi_sm += m->comp_strength[nz] * H; psi_sm += m->comp_strength[nz] * F; break;
+            case 0: phi_pi += m->comp_strength[nz] * H; psi_pi += m->comp_strength[nz] * F; break;
+            case 1: phi_sp += m->comp_strength[nz] * H; psi_sp += m->comp_strength[nz] * F; break;
             }
           }
           phi_sigma = phi_sp + phi_sm;
@@ -1335,6 +1344,14 @@ static int line_opacity_sel(const rhg_model_t *m, const column_t *c, int ls,
           phi_Q = sign * phi_delta * sin2_gamma * c->cos2x[k] * sv;
           phi_U = phi_delta * sin2_gamma * c->sin2x[k] * sv;
           phi_V = sign * 0.5 * (phi_sp - phi_sm) * c->cosg[k] * sv;
+          /* the Faraday-Voigt profiles (rh/kurucz.c:811-819) */
+          if (chip) {
+            psi_sigma = psi_sp + psi_sm;
+            psi_delta = 0.5 * psi_pi - 0.25 * psi_sigma;
+            psi_Q = sign * psi_delta * sin2_gamma * c->cos2x[k] * sv;
+            psi_U = psi_delta * sin2_gamma * c->sin2x[k] * sv;
+            psi_V = sign * 0.5 * (psi_sp - psi_sm) * c->cosg[k] * sv;
+          }
         } else {
           phi = rho_voigt_armstrong(adamp, v) * sv;
         }
@@ -1358,6 +1375,11 @@ static int line_opacity_sel(const rhg_model_t *m, const column_t *c, int ls,
           eta[N + k] += eta_l * phi_Q;
           eta[2 * N + k] += eta_l * phi_U;
           eta[3 * N + k] += eta_l * phi_V;
+          if (chip) {   /* rh/kurucz.c:711-715 */
+            chip[k] += chi_l * psi_Q;
+            chip[N + k] += chi_l * psi_U;
+            chip[2 * N + k] += chi_l * psi_V;
+          }
         }
       }
     }
@@ -1367,9 +1389,9 @@ static int line_opacity_sel(const rhg_model_t *m, const column_t *c, int ls,
 }
 
 static int line_opacity(const rhg_model_t *m, const column_t *c, int ls,
-                        double mu, double *chi, double *eta)
+                        double mu, double *chi, double *eta, double *chip)
 {
-  return line_opacity_sel(m, c, ls, mu, chi, eta, -1);
+  return line_opacity_sel(m, c, ls, mu, chi, eta, chip, -1);
 }
 
 struct nlte_out { double *pops; int *niter; double *dpops; int ndep_max; double *stokes; double mu; };
@@ -1393,13 +1415,13 @@ static int synth_column_ex(const rhg_model_t *m, const rhg_columns_t *cols, int 
   const double mu = cols->mu;
   column_t c;
   double *mem, *chi_ai, *eta_ai, *chi_c, *eta_c, *lchi, *leta, *chiref, *S4, *I4,
-         *molbuf, *scr;
+         *molbuf, *scr, *chipb;
   int k, ls, i, nout = 0;
 
   memset(&c, 0, sizeof(c));
   c.Ndep = N;
   mem = calloc((size_t) N * (24 + nlev + natom + m->nel * NSTAGE_MAX + 30) +
-               (size_t) 4 * N * 8 + 64, sizeof(double));
+               (size_t) 4 * N * 8 + (size_t) 3 * N + 64, sizeof(double));
   if (!mem) return -1;
   scr = mem;
 #define TAKE(p, cnt) do { p = scr; scr += (cnt); } while (0)
@@ -1412,7 +1434,7 @@ static int synth_column_ex(const rhg_model_t *m, const rhg_columns_t *cols, int 
   TAKE(c.nel, (size_t) N * m->nel * NSTAGE_MAX);
   TAKE(chi_ai, N); TAKE(eta_ai, N); TAKE(chi_c, 4 * N); TAKE(eta_c, 4 * N);
   TAKE(lchi, 4 * N); TAKE(leta, 4 * N); TAKE(chiref, N); TAKE(S4, 4 * N);
-  TAKE(I4, 4 * N); TAKE(molbuf, 32);
+  TAKE(I4, 4 * N); TAKE(molbuf, 32); TAKE(chipb, 3 * N);
 #undef TAKE
 
   /* units and scales, rh/rhf1d/pyrh_compute1dray.c:230-278 */
@@ -1487,7 +1509,7 @@ static int synth_column_ex(const rhg_model_t *m, const rhg_columns_t *cols, int 
       chiref[k] += sca;
     }
   }
-  if (line_opacity(m, &c, m->ref_index, mu, lchi, leta))
+  if (line_opacity(m, &c, m->ref_index, mu, lchi, leta, NULL))
     for (k = 0; k < N; k++) chiref[k] += lchi[k];
   {
     double *rho = S4, h_zero, unity = 1.0;   /* S4 as scratch */
@@ -1544,7 +1566,7 @@ static int synth_column_ex(const rhg_model_t *m, const rhg_columns_t *cols, int 
     background_continuum(m, &c, diag_ls, chi_ai, eta_ai);
     for (k = 0; k < N; k++) { diag_chi[k] = chi_ai[k]; diag_eta[k] = eta_ai[k]; }
     for (k = N; k < 4 * N; k++) { diag_chi[k] = 0.0; diag_eta[k] = 0.0; }
-    if (line_opacity(m, &c, diag_ls, mu, lchi, leta))
+    if (line_opacity(m, &c, diag_ls, mu, lchi, leta, NULL))
       for (k = 0; k < 4 * N; k++) { diag_chi[4 * N + k] = lchi[k]; diag_eta[4 * N + k] = leta[k]; }
     else
       for (k = 0; k < 4 * N; k++) { diag_chi[4 * N + k] = 0.0; diag_eta[4 * N + k] = 0.0; }
@@ -1564,18 +1586,23 @@ static int synth_column_ex(const rhg_model_t *m, const rhg_columns_t *cols, int 
     double B2[2], Iout[4] = {0, 0, 0, 0};
     if (ls == m->ref_index) continue;
     background_continuum(m, &c, ls, chi_ai, eta_ai);
-    has = line_opacity(m, &c, ls, mu, lchi, leta);
+    has = line_opacity(m, &c, ls, mu, lchi, leta, m->magneto_optical ? chipb : NULL);
     for (k = 0; k < N; k++) { chi_c[k] = chi_ai[k]; eta_c[k] = eta_ai[k]; }
     for (k = N; k < 4 * N; k++) { chi_c[k] = 0.0; eta_c[k] = 0.0; }
     if (has) for (k = 0; k < 4 * N; k++) { chi_c[k] += lchi[k]; eta_c[k] += leta[k]; }
+    /* a BEZIER wavelength is angle-independent in a static column and
+       takes Feautrier (rh/rhf1d/formal.c:98-101, 287-310); a polarizable
+       background line keeps it angle-dependent (BEZIER_ANGLE) */
     solver = m->solver[ls];
     if (solver == RHG_SOLVER_BEZIER && !c.moving) solver = RHG_SOLVER_FEAUTRIER;
+    if (solver == RHG_SOLVER_BEZIER_ANGLE) solver = RHG_SOLVER_BEZIER;
     B2[0] = rho_planck(c.T[N - 2], m->lamconst[(size_t) ls * RHG_LC_NCONST]);
     B2[1] = rho_planck(c.T[N - 1], m->lamconst[(size_t) ls * RHG_LC_NCONST]);
     if (solver == RHG_SOLVER_STOKES) {
       for (n = 0; n < 4; n++)
         for (k = 0; k < N; k++) S4[n * N + k] = (0.0 + eta_c[n * N + k]) / chi_c[k];
-      rho_formal_stokes(N, c.height, mu, chi_c, S4, chi_c + N, B2, fp32, I4);
+      rho_formal_stokes(N, c.height, mu, chi_c, S4, chi_c + N,
+                        m->magneto_optical ? chipb : NULL, B2, fp32, I4);
       for (n = 0; n < 4; n++) Iout[n] = I4[n * N];
     } else if (solver == RHG_SOLVER_BEZIER && rf) {
       /* the inward ray first, then the emergent one with the log gf
@@ -1590,10 +1617,11 @@ static int synth_column_ex(const rhg_model_t *m, const rhg_columns_t *cols, int 
          (rh/background.c:470-580), so the inward ray runs on the emergent
          ray's opacity and source function */
       for (k = 0; k < N; k++) S4[k] = (0.0 + eta_c[k]) / chi_c[k];
-      rho_formal_bezier_in(N, c.height, mu, chi_c, S4, I4);
+      rho_formal_bezier_in(N, c.height, mu, chi_c, S4,
+                           m->lamconst[(size_t) ls * RHG_LC_NCONST + RHG_LC_ITOP], I4);
       for (p = 0; p < npar; p++) {
         if (par_line[p] < 0) continue;
-        if (!line_opacity_sel(m, &c, ls, mu, lchi, leta, par_line[p])) continue;
+        if (!line_opacity_sel(m, &c, ls, mu, lchi, leta, NULL, par_line[p])) continue;
         for (k = 0; k < N; k++) {
           dchi[(size_t) p * N + k] = lchi[k] * LG10;
           deta[(size_t) p * N + k] = leta[k] * LG10;

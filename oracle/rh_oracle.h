@@ -85,19 +85,20 @@ double rho_planck(double T, double lambda_nm);
  * leaving the top (to_obs), height z [m] decreasing with index, with the
  * thermalised lower boundary built from B2 = {B(T[n-2]), B(T[n-1])}
  * (B2 == NULL: zero incoming intensity).  S4 is [4][n] (source functions
- * of I,Q,U,V), chiQUV is [3][n] (chi_Q,U,V, not divided by chi);
+ * of I,Q,U,V), chiQUV is [3][n] (chi_Q,U,V, not divided by chi), chip
+ * [3][n] the magneto-optical chip_Q,U,V (NULL: terms off);
  * I4 receives I,Q,U,V at every depth, [4][n]; I receives [n]. */
 void rho_formal_stokes(int n, const double *z, double mu, const double *chi,
-                       const double *S4, const double *chiQUV,
+                       const double *S4, const double *chiQUV, const double *chip,
                        const double *B2, int inverse_fp32, double *I4);
 void rho_formal_bezier(int n, const double *z, double mu, const double *chi,
                        const double *S, const double *B2, double *I);
-/* The same solver along the inward ray (depth 0 to n-1, zero incident
- * intensity at the top), and along the emergent ray with the reference's
+/* The same solver along the inward ray (depth 0 to n-1, incident intensity
+ * Itop at the top: 0, or ATMOS_ITOP's value), and along the emergent ray with the reference's
  * d I / d log gf recursion: I holds the inward intensities on entry;
  * dchi, deta are [npar][n]; dI0 receives the emergent responses [npar]. */
 void rho_formal_bezier_in(int n, const double *z, double mu, const double *chi,
-                          const double *S, double *I);
+                          const double *S, double Itop, double *I);
 void rho_formal_bezier_rf(int n, const double *z, double mu, const double *chi,
                           const double *S, const double *B2, int npar,
                           const double *dchi, const double *deta, double *I, double *dI0);

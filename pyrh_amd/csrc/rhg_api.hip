@@ -47,6 +47,7 @@ struct DevBuf {
 
 }  // namespace
 
+
 struct rhg_context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -62,7 +63,7 @@ struct rhg_context {
   DevBuf atmos, ndep, state, moving, cchi, ceta, stokes, chiref, census, census_sum;
   // response functions: perturbed inputs/states, tangents, tape, adjoints
   DevBuf rf_atm[2], rf_state[2], rf_cont[4], rf_chiref[2], rf_moving, rf_dstate, rf_dcont[2],
-      rf_dchiref, rf_pts, rf_car, rf_aP, rf_aZ, rf_aB, rf_out;
+      rf_dchiref, rf_car, rf_aP, rf_aZ, rf_aB, rf_out;
   // log gf response functions: inward intensities, output, parameter lines
   DevBuf at_iin, at_out, at_par;
   // NLTE: the active atom (packed arrays and its DevNlte), the batch arrays
@@ -84,6 +85,7 @@ struct rhg_context {
   std::vector<int32_t> h_atom_level_ptr, h_nucl_atom, h_level_stage;
   hipEvent_t ev[RHG_NSTAGE + 2]{};
   float last_ms[RHG_NSTAGE + 1]{};
+  float rf_ms[5]{};   // the last response-function call's intervals
   // the last batch, for rhg_count_work
   int last_ncol = 0, last_Nmax = 0, last_nfield = 0;
   double last_mu = 1.0;
@@ -141,7 +143,6 @@ int rhg_open(const rhg_model_t *m, int device, rhg_context_t **out) {
   // builder; the synthesis entry points need an output wavelength
   if (m->nspect < 1 || m->ref_index < 0 || m->ref_index >= m->nspect)
     return fail(RHG_EINVAL, "bad wavelength grid");
-  if (m->magneto_optical) return fail(RHG_EUNSUPPORTED, "magneto-optical terms are not implemented");
   if (m->natom < 1 || m->natom > RHG_MAX_ATOM) return fail(RHG_EUNSUPPORTED, "too many model atoms");
   if (m->nnuclei < 1 || m->nnuclei > rhg::NNUC_MAX) return fail(RHG_EUNSUPPORTED, "too many molecule nuclei");
   if (m->nmol > rhg::NMOL_MAX) return fail(RHG_EUNSUPPORTED, "too many molecules");
@@ -371,6 +372,71 @@ static int launch_state(rhg_context_t *c, const double *d_atm, const int32_t *d_
   return 0;
 }
 
+// Stages 1-4 of a synthesis (k_state, k_geometry, k_continuum, the formal
+// kernels) on device buffers, with the stage events; no allocation, no copy,
+// no synchronisation.
+static int launch_synth(rhg_context_t *c, const double *d_atm, const int32_t *d_ndep, int ncol,
+                        int Nmax, double mu, int atm_scale, int32_t precision, int nfield,
+                        double *state, int32_t *moving, double *chiref, void *cchi, void *ceta,
+                        double *d_out) {
+  const int64_t npts = (int64_t) ncol * Nmax;
+  const int nout = c->nout;
+
+  HIPCHECK(hipEventRecord(c->ev[0], c->stream));
+  {
+    const int rc = launch_state(c, d_atm, d_ndep, ncol, Nmax, mu, atm_scale, state, nfield);
+    if (rc) return rc;
+  }
+  HIPCHECK(hipEventRecord(c->ev[1], c->stream));
+  {
+    dim3 blk(64), grd((unsigned) ((ncol + 63) / 64));
+    hipLaunchKernelGGL(rhg::k_geometry, grd, blk, 0, c->stream, c->d_model, d_ndep, ncol, Nmax,
+                       atm_scale, mu, state, nfield, moving, chiref);
+    HIPCHECK(hipGetLastError());
+  }
+  HIPCHECK(hipEventRecord(c->ev[2], c->stream));
+  {
+    dim3 blk(rhg::CONT_BLOCK), grd((unsigned) ((npts + rhg::CONT_BLOCK - 1) / rhg::CONT_BLOCK));
+    if (precision == RHG_PREC_FP32)
+      hipLaunchKernelGGL(rhg::k_continuum<float>, grd, blk, 0, c->stream, c->d_model, d_ndep, ncol, Nmax,
+                         state, nfield, c->d_out_ls, nout, (float *) cchi, (float *) ceta);
+    else
+      hipLaunchKernelGGL(rhg::k_continuum<double>, grd, blk, 0, c->stream, c->d_model, d_ndep, ncol, Nmax,
+                         state, nfield, c->d_out_ls, nout, (double *) cchi, (double *) ceta);
+    HIPCHECK(hipGetLastError());
+  }
+  HIPCHECK(hipEventRecord(c->ev[3], c->stream));
+  {
+    const int ncb = (ncol + 63) / 64;
+    dim3 blk(64);
+#define RHG_LAUNCH_FORMAL(K32, K64, LIST, NLIST)                                                 \
+  if (NLIST > 0) {                                                                               \
+    dim3 grd((unsigned) rhg::xcd_grid(ncb, NLIST));                                              \
+    if (precision == RHG_PREC_FP32)                                                              \
+      hipLaunchKernelGGL(K32, grd, blk, 0, c->stream, c->d_model, d_ndep, moving, ncol, Nmax,    \
+                         state, c->d_out_ls, LIST, NLIST, nout, (const float *) cchi,           \
+                         (const float *) ceta, mu, d_out);                                       \
+    else                                                                                         \
+      hipLaunchKernelGGL(K64, grd, blk, 0, c->stream, c->d_model, d_ndep, moving, ncol, Nmax,    \
+                         state, c->d_out_ls, LIST, NLIST, nout, (const double *) cchi,          \
+                         (const double *) ceta, mu, d_out);                                      \
+    HIPCHECK(hipGetLastError());                                                                 \
+  }
+    if (c->dm.magneto_optical) {
+      RHG_LAUNCH_FORMAL((rhg::k_formal_stokes<float, true>), (rhg::k_formal_stokes<double, true>),
+                        c->d_io_pol, c->n_pol)
+    } else {
+      RHG_LAUNCH_FORMAL((rhg::k_formal_stokes<float, false>), (rhg::k_formal_stokes<double, false>),
+                        c->d_io_pol, c->n_pol)
+    }
+    RHG_LAUNCH_FORMAL(rhg::k_formal_scalar<float>, rhg::k_formal_scalar<double>, c->d_io_scal,
+                      c->n_scal)
+#undef RHG_LAUNCH_FORMAL
+  }
+  HIPCHECK(hipEventRecord(c->ev[4], c->stream));
+  return 0;
+}
+
 static int synth(rhg_context_t *c, const rhg_columns_t *cols, int32_t precision,
                  int32_t device_ptrs, double *stokes_out, double *height_out,
                  double *chiref_out) {
@@ -418,54 +484,9 @@ static int synth(rhg_context_t *c, const rhg_columns_t *cols, int32_t precision,
   double *state = (double *) c->state.p;
   int32_t *moving = (int32_t *) c->moving.p;
   double *chiref = chiref_out ? (double *) c->chiref.p : nullptr;
-  const int64_t npts = (int64_t) ncol * Nmax;
-
-  HIPCHECK(hipEventRecord(c->ev[0], c->stream));
-  {
-    const int rc = launch_state(c, d_atm, d_ndep, ncol, Nmax, cols->mu, cols->atm_scale, state, nfield);
-    if (rc) return rc;
-  }
-  HIPCHECK(hipEventRecord(c->ev[1], c->stream));
-  {
-    dim3 blk(64), grd((unsigned) ((ncol + 63) / 64));
-    hipLaunchKernelGGL(rhg::k_geometry, grd, blk, 0, c->stream, c->d_model, d_ndep, ncol, Nmax,
-                       cols->atm_scale, cols->mu, state, nfield, moving, chiref);
-    HIPCHECK(hipGetLastError());
-  }
-  HIPCHECK(hipEventRecord(c->ev[2], c->stream));
-  {
-    dim3 blk(rhg::CONT_BLOCK), grd((unsigned) ((npts + rhg::CONT_BLOCK - 1) / rhg::CONT_BLOCK));
-    if (precision == RHG_PREC_FP32)
-      hipLaunchKernelGGL(rhg::k_continuum<float>, grd, blk, 0, c->stream, c->d_model, d_ndep, ncol, Nmax,
-                         state, nfield, c->d_out_ls, nout, (float *) c->cchi.p, (float *) c->ceta.p);
-    else
-      hipLaunchKernelGGL(rhg::k_continuum<double>, grd, blk, 0, c->stream, c->d_model, d_ndep, ncol, Nmax,
-                         state, nfield, c->d_out_ls, nout, (double *) c->cchi.p, (double *) c->ceta.p);
-    HIPCHECK(hipGetLastError());
-  }
-  HIPCHECK(hipEventRecord(c->ev[3], c->stream));
-  {
-    const int ncb = (ncol + 63) / 64;
-    dim3 blk(64);
-#define RHG_LAUNCH_FORMAL(KNAME, LIST, NLIST)                                                    \
-  if (NLIST > 0) {                                                                               \
-    dim3 grd((unsigned) rhg::xcd_grid(ncb, NLIST));                                              \
-    if (precision == RHG_PREC_FP32)                                                              \
-      hipLaunchKernelGGL(rhg::KNAME<float>, grd, blk, 0, c->stream, c->d_model, d_ndep, moving,  \
-                         ncol, Nmax, state, c->d_out_ls, LIST, NLIST, nout,                     \
-                         (const float *) c->cchi.p, (const float *) c->ceta.p, cols->mu, d_out); \
-    else                                                                                         \
-      hipLaunchKernelGGL(rhg::KNAME<double>, grd, blk, 0, c->stream, c->d_model, d_ndep, moving, \
-                         ncol, Nmax, state, c->d_out_ls, LIST, NLIST, nout,                     \
-                         (const double *) c->cchi.p, (const double *) c->ceta.p, cols->mu,      \
-                         d_out);                                                                 \
-    HIPCHECK(hipGetLastError());                                                                 \
-  }
-    RHG_LAUNCH_FORMAL(k_formal_stokes, c->d_io_pol, c->n_pol)
-    RHG_LAUNCH_FORMAL(k_formal_scalar, c->d_io_scal, c->n_scal)
-#undef RHG_LAUNCH_FORMAL
-  }
-  HIPCHECK(hipEventRecord(c->ev[4], c->stream));
+  if (int rc = launch_synth(c, d_atm, d_ndep, ncol, Nmax, cols->mu, cols->atm_scale, precision,
+                            nfield, state, moving, chiref, c->cchi.p, c->ceta.p, d_out))
+    return rc;
   if (!device_ptrs)
     HIPCHECK(hipMemcpyAsync(stokes_out, d_out, out_b, hipMemcpyDeviceToHost, c->stream));
   std::vector<double> hfield;
@@ -699,56 +720,17 @@ const RfParam kRfParam[RHG_RF_NPARAM] = {
 };
 
 template <typename R>
-int rf_run(rhg_context_t *c, const rhg_columns_t *cols, int32_t device_ptrs, double *stokes_out,
-           void *rf_out) {
-  const int ncol = cols->ncol, Nmax = cols->ndep_max, nout = c->nout;
+int rf_chunk(rhg_context_t *c, const double *d_atm, const int32_t *d_ndep, int ncol, int Nmax,
+             double mu, int scale, const rhg::RfParamList &pl, double *d_out, R *d_rf,
+             int64_t ldr) {
+  const int nout = c->nout;
   const int nfield = rhg::state_nfield(c->ncont, c->nline);
   const int64_t npts = (int64_t) ncol * Nmax;
-  const size_t atm_b = (size_t) ncol * 9 * Nmax * sizeof(double);
-  const size_t st_b = (size_t) npts * nfield * sizeof(double);
-  const size_t cc_b = (size_t) npts * nout * sizeof(R);
-  const size_t cc64_b = (size_t) npts * nout * sizeof(double);
-  const size_t out_b = (size_t) ncol * 4 * nout * sizeof(double);
-  const size_t rf_b = (size_t) RHG_RF_NPARAM * Nmax * 4 * nout * ncol * sizeof(R);
   const int npol = c->n_pol;
-  const size_t pts_b = (size_t) (npol > 0 ? npol : 1) * Nmax * 8 * ncol * sizeof(R);
-  const size_t car_b = (size_t) (npol > 0 ? npol : 1) * Nmax * rhg::NCARRY * ncol * sizeof(R);
-  const size_t aP_b = (size_t) (npol > 0 ? npol : 1) * 4 * Nmax * 8 * ncol * sizeof(R);
-  const size_t aZ_b = (size_t) (npol > 0 ? npol : 1) * 4 * Nmax * ncol * sizeof(R);
-  const size_t aB_b = (size_t) (npol > 0 ? npol : 1) * 4 * 2 * ncol * sizeof(R);
-  if (c->state.ensure(st_b) || c->moving.ensure(ncol * sizeof(int32_t)) || c->cchi.ensure(cc_b) ||
-      c->ceta.ensure(cc_b) || c->chiref.ensure((size_t) npts * sizeof(double)) ||
-      (!device_ptrs && (c->atmos.ensure(atm_b) || c->ndep.ensure(ncol * sizeof(int32_t)) ||
-                        c->stokes.ensure(out_b) || c->rf_out.ensure(rf_b))) ||
-      c->rf_atm[0].ensure(atm_b) || c->rf_atm[1].ensure(atm_b) || c->rf_state[0].ensure(st_b) ||
-      c->rf_state[1].ensure(st_b) || c->rf_cont[0].ensure(cc64_b) || c->rf_cont[1].ensure(cc64_b) ||
-      c->rf_cont[2].ensure(cc64_b) || c->rf_cont[3].ensure(cc64_b) ||
-      c->rf_chiref[0].ensure((size_t) npts * sizeof(double)) ||
-      c->rf_chiref[1].ensure((size_t) npts * sizeof(double)) ||
-      c->rf_moving.ensure(ncol * sizeof(int32_t)) ||
-      c->rf_dstate.ensure(st_b * RHG_RF_NPARAM) || c->rf_dcont[0].ensure(cc_b) ||
-      c->rf_dcont[1].ensure(cc_b) || c->rf_dchiref.ensure((size_t) npts * sizeof(double)) ||
-      c->rf_pts.ensure(pts_b) || c->rf_car.ensure(car_b) || c->rf_aP.ensure(aP_b) ||
-      c->rf_aZ.ensure(aZ_b) || c->rf_aB.ensure(aB_b))
-    return fail(RHG_ENOMEM, "device allocation for the response functions failed");
-  const double *d_atm = (const double *) cols->atmos;
-  const int32_t *d_ndep = cols->ndep;
-  double *d_out = stokes_out;
-  R *d_rf = (R *) rf_out;
-  if (!device_ptrs) {
-    HIPCHECK(hipMemcpyAsync(c->atmos.p, cols->atmos, atm_b, hipMemcpyHostToDevice, c->stream));
-    HIPCHECK(hipMemcpyAsync(c->ndep.p, cols->ndep, ncol * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
-    d_atm = (const double *) c->atmos.p;
-    d_ndep = (const int32_t *) c->ndep.p;
-    d_out = (double *) c->stokes.p;
-    d_rf = (R *) c->rf_out.p;
-  }
   double *state = (double *) c->state.p;
   int32_t *moving = (int32_t *) c->moving.p;
   R *cchi = (R *) c->cchi.p, *ceta = (R *) c->ceta.p;
   double *chiref = (double *) c->chiref.p;
-  const double mu = cols->mu;
-  const int scale = cols->atm_scale;
   HIPCHECK(hipEventRecord(c->ev[0], c->stream));
   int rc = launch_base<R>(c, d_atm, d_ndep, ncol, Nmax, mu, scale, state, nfield, moving, chiref,
                           cchi, ceta);
@@ -758,7 +740,8 @@ int rf_run(rhg_context_t *c, const rhg_columns_t *cols, int32_t device_ptrs, dou
   // 1. state tangents by central differences (every depth at once)
   const int64_t natm = (int64_t) ncol * 9 * Nmax;
   double *dstate = (double *) c->rf_dstate.p;
-  for (int p = 0; p < RHG_RF_NPARAM; p++) {
+  for (int ip = 0; ip < pl.npar; ip++) {
+    const int p = pl.p[ip];
     const RfParam &P = kRfParam[p];
     for (int sgn = 0; sgn < 2; sgn++) {
       dim3 blk(256), grd((unsigned) ((natm + 255) / 256));
@@ -801,28 +784,36 @@ int rf_run(rhg_context_t *c, const rhg_columns_t *cols, int32_t device_ptrs, dou
 
   // 2-4. tape, reverse sweep, chain (polarized wavelengths); the others by
   // the scalar solver with zero response functions
-  HIPCHECK(hipMemsetAsync(d_rf, 0, rf_b, c->stream));
   const int ncb = (ncol + 63) / 64;
   dim3 blk(64);
   if (npol > 0) {
     dim3 grd((unsigned) rhg::xcd_grid(ncb, npol));
-    R *pts = (R *) c->rf_pts.p, *car = (R *) c->rf_car.p;
+    double *car = (double *) c->rf_car.p;
     R *aP = (R *) c->rf_aP.p, *aZ = (R *) c->rf_aZ.p, *aB = (R *) c->rf_aB.p;
     hipLaunchKernelGGL(rhg::k_rf_forward<R>, grd, blk, 0, c->stream, c->d_model, d_ndep, moving, ncol,
-                       Nmax, state, c->d_out_ls, c->d_io_pol, npol, nout, cchi, ceta, mu, d_out, pts,
+                       Nmax, state, c->d_out_ls, c->d_io_pol, npol, nout, cchi, ceta, mu, d_out,
                        car);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipEventRecord(c->ev[3], c->stream));
     hipLaunchKernelGGL(rhg::k_rf_reverse<R>, grd, blk, 0, c->stream, c->d_model, d_ndep, moving, ncol,
-                       Nmax, state, c->d_out_ls, c->d_io_pol, npol, mu, pts, car, aP, aZ, aB);
+                       Nmax, state, c->d_out_ls, c->d_io_pol, npol, cchi, ceta, mu, car, aP, aZ,
+                       aB);
     HIPCHECK(hipGetLastError());
-    hipLaunchKernelGGL(rhg::k_rf_chain<R>, grd, blk, 0, c->stream, c->d_model, d_ndep, moving, ncol,
-                       Nmax, state, dstate, chiref, (const double *) c->rf_dchiref.p, cchi, ceta,
-                       (const R *) c->rf_dcont[0].p, (const R *) c->rf_dcont[1].p, c->d_out_ls,
-                       c->d_io_pol, npol, nout, mu, scale, aP, aZ, aB, d_rf);
+    HIPCHECK(hipEventRecord(c->ev[4], c->stream));
+    if (pl.npar <= 3)
+      hipLaunchKernelGGL((rhg::k_rf_chain<R, 3>), grd, blk, 0, c->stream, c->d_model, d_ndep, moving,
+                         ncol, Nmax, state, dstate, chiref, (const double *) c->rf_dchiref.p, cchi,
+                         ceta, (const R *) c->rf_dcont[0].p, (const R *) c->rf_dcont[1].p,
+                         c->d_out_ls, c->d_io_pol, npol, nout, mu, scale, pl, aP, aZ, aB, d_rf, ldr);
+    else
+      hipLaunchKernelGGL((rhg::k_rf_chain<R, 6>), grd, blk, 0, c->stream, c->d_model, d_ndep, moving,
+                         ncol, Nmax, state, dstate, chiref, (const double *) c->rf_dchiref.p, cchi,
+                         ceta, (const R *) c->rf_dcont[0].p, (const R *) c->rf_dcont[1].p,
+                         c->d_out_ls, c->d_io_pol, npol, nout, mu, scale, pl, aP, aZ, aB, d_rf, ldr);
     HIPCHECK(hipGetLastError());
   } else {
     HIPCHECK(hipEventRecord(c->ev[3], c->stream));
+    HIPCHECK(hipEventRecord(c->ev[4], c->stream));
   }
   if (c->n_scal > 0) {
     dim3 grd((unsigned) rhg::xcd_grid(ncb, c->n_scal));
@@ -831,17 +822,113 @@ int rf_run(rhg_context_t *c, const rhg_columns_t *cols, int32_t device_ptrs, dou
                        (const R *) cchi, (const R *) ceta, mu, d_out);
     HIPCHECK(hipGetLastError());
   }
-  HIPCHECK(hipEventRecord(c->ev[4], c->stream));
+  HIPCHECK(hipEventRecord(c->ev[5], c->stream));
+  HIPCHECK(hipEventSynchronize(c->ev[5]));
+  float ms;
+  for (int st = 0; st < 5; st++) {
+    HIPCHECK(hipEventElapsedTime(&ms, c->ev[st], c->ev[st + 1]));
+    c->rf_ms[st] += ms;
+  }
+  return 0;
+}
+
+// The response functions of a batch, in column chunks whose work buffers
+// (column state, continuum, tangents, the forward sweep's carried state and
+// the point adjoints: about npol * Nmax * 300 bytes per column) fit in half
+// of the device memory available; the outputs are the whole batch's.
+template <typename R>
+int rf_run(rhg_context_t *c, const rhg_columns_t *cols, int32_t device_ptrs,
+           const rhg::RfParamList &pl, double *stokes_out, void *rf_out) {
+  const int ncol_all = cols->ncol, Nmax = cols->ndep_max, nout = c->nout;
+  const int nfield = rhg::state_nfield(c->ncont, c->nline);
+  const size_t npl = (size_t) (c->n_pol > 0 ? c->n_pol : 1);
+  const size_t rsz = sizeof(R);
+  const size_t per_col =
+      (size_t) Nmax * ((size_t) nfield * 8 + 2 * nout * rsz + 8 + 2 * 9 * 8 + 2 * (size_t) nfield * 8 +
+                       4 * (size_t) nout * 8 + 2 * 8 + RHG_RF_NPARAM * (size_t) nfield * 8 +
+                       2 * nout * rsz + 8 + npl * rhg::RF_NCARRY * 8 + npl * 4 * 8 * rsz +
+                       npl * 4 * rsz) +
+      npl * 8 * rsz + 2 * sizeof(int32_t);
+  DevBuf *work[] = {&c->state, &c->moving, &c->cchi, &c->ceta, &c->chiref, &c->rf_atm[0],
+                    &c->rf_atm[1], &c->rf_state[0], &c->rf_state[1], &c->rf_cont[0],
+                    &c->rf_cont[1], &c->rf_cont[2], &c->rf_cont[3], &c->rf_chiref[0],
+                    &c->rf_chiref[1], &c->rf_moving, &c->rf_dstate, &c->rf_dcont[0],
+                    &c->rf_dcont[1], &c->rf_dchiref, &c->rf_car, &c->rf_aP, &c->rf_aZ, &c->rf_aB};
+  size_t held = 0;
+  for (DevBuf *b : work) held += b->bytes;
+  size_t freeb = 0, totb = 0;
+  HIPCHECK(hipMemGetInfo(&freeb, &totb));
+  const size_t out_b = (size_t) ncol_all * 4 * nout * sizeof(double);
+  const size_t rf_b = (size_t) pl.npar * Nmax * 4 * nout * ncol_all * rsz;
+  const size_t atm_all_b = (size_t) ncol_all * 9 * Nmax * sizeof(double);
+  size_t avail = freeb + held;
+  if (!device_ptrs) {
+    const size_t host_side = out_b + rf_b + atm_all_b + ncol_all * sizeof(int32_t);
+    avail = avail > host_side ? avail - host_side : 0;
+  }
+  int64_t chunk = (int64_t) (avail / 2 / per_col);
+  if (chunk < 1) return fail(RHG_ENOMEM, "device memory too small for one response-function column");
+  if (chunk > ncol_all) chunk = ncol_all;
+  if (chunk > 64) chunk = chunk / 64 * 64;
+  // PYRH_AMD_RF_CHUNK caps the chunk (tests of the chunked path)
+  if (const char *cap = getenv("PYRH_AMD_RF_CHUNK")) {
+    const long v = strtol(cap, nullptr, 10);
+    if (v >= 1 && v < chunk) chunk = v;
+  }
+  const int64_t npts = chunk * Nmax;
+  const size_t atm_b = (size_t) chunk * 9 * Nmax * sizeof(double);
+  const size_t st_b = (size_t) npts * nfield * sizeof(double);
+  const size_t cc_b = (size_t) npts * nout * rsz;
+  const size_t cc64_b = (size_t) npts * nout * sizeof(double);
+  const size_t car_b = npl * Nmax * rhg::RF_NCARRY * chunk * sizeof(double);
+  const size_t aP_b = npl * 4 * Nmax * 8 * chunk * rsz;
+  const size_t aZ_b = npl * 4 * Nmax * chunk * rsz;
+  const size_t aB_b = npl * 4 * 2 * chunk * rsz;
+  if (c->state.ensure(st_b) || c->moving.ensure(chunk * sizeof(int32_t)) || c->cchi.ensure(cc_b) ||
+      c->ceta.ensure(cc_b) || c->chiref.ensure((size_t) npts * sizeof(double)) ||
+      (!device_ptrs && (c->atmos.ensure(atm_all_b) || c->ndep.ensure(ncol_all * sizeof(int32_t)) ||
+                        c->stokes.ensure(out_b) || c->rf_out.ensure(rf_b))) ||
+      c->rf_atm[0].ensure(atm_b) || c->rf_atm[1].ensure(atm_b) || c->rf_state[0].ensure(st_b) ||
+      c->rf_state[1].ensure(st_b) || c->rf_cont[0].ensure(cc64_b) || c->rf_cont[1].ensure(cc64_b) ||
+      c->rf_cont[2].ensure(cc64_b) || c->rf_cont[3].ensure(cc64_b) ||
+      c->rf_chiref[0].ensure((size_t) npts * sizeof(double)) ||
+      c->rf_chiref[1].ensure((size_t) npts * sizeof(double)) ||
+      c->rf_moving.ensure(chunk * sizeof(int32_t)) ||
+      c->rf_dstate.ensure(st_b * RHG_RF_NPARAM) || c->rf_dcont[0].ensure(cc_b) ||
+      c->rf_dcont[1].ensure(cc_b) || c->rf_dchiref.ensure((size_t) npts * sizeof(double)) ||
+      c->rf_car.ensure(car_b) || c->rf_aP.ensure(aP_b) ||
+      c->rf_aZ.ensure(aZ_b) || c->rf_aB.ensure(aB_b))
+    return fail(RHG_ENOMEM, "device allocation for the response functions failed");
+  const double *d_atm = (const double *) cols->atmos;
+  const int32_t *d_ndep = cols->ndep;
+  double *d_out = stokes_out;
+  R *d_rf = (R *) rf_out;
+  if (!device_ptrs) {
+    HIPCHECK(hipMemcpyAsync(c->atmos.p, cols->atmos, atm_all_b, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(c->ndep.p, cols->ndep, ncol_all * sizeof(int32_t), hipMemcpyHostToDevice,
+                            c->stream));
+    d_atm = (const double *) c->atmos.p;
+    d_ndep = (const int32_t *) c->ndep.p;
+    d_out = (double *) c->stokes.p;
+    d_rf = (R *) c->rf_out.p;
+  }
+  // the wavelengths of the scalar solvers keep zero response functions
+  HIPCHECK(hipMemsetAsync(d_rf, 0, rf_b, c->stream));
+  for (float &x : c->rf_ms) x = 0.0f;
+  for (int64_t c0 = 0; c0 < ncol_all; c0 += chunk) {
+    const int n = (int) std::min<int64_t>(chunk, ncol_all - c0);
+    if (int rc = rf_chunk<R>(c, d_atm + c0 * 9 * Nmax, d_ndep + c0, n, Nmax, cols->mu,
+                             cols->atm_scale, pl, d_out + c0 * 4 * nout, d_rf + c0, ncol_all))
+      return rc;
+  }
   if (!device_ptrs) {
     HIPCHECK(hipMemcpyAsync(stokes_out, d_out, out_b, hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(hipMemcpyAsync(rf_out, d_rf, rf_b, hipMemcpyDeviceToHost, c->stream));
   }
   HIPCHECK(hipStreamSynchronize(c->stream));
-  float ms;
-  for (int st = 0; st < 4; st++) {
-    HIPCHECK(hipEventElapsedTime(&ms, c->ev[st], c->ev[st + 1]));
-    c->last_ms[st] = ms;
-  }
+  // rhg_last_timings' four intervals: inputs, tangents, forward, reverse + chain
+  for (int st = 0; st < 3; st++) c->last_ms[st] = c->rf_ms[st];
+  c->last_ms[3] = c->rf_ms[3] + c->rf_ms[4];
   return 0;
 }
 
@@ -851,7 +938,27 @@ extern "C" {
 
 int rhg_rf_lte(rhg_context_t *c, const rhg_columns_t *cols, int32_t precision, int32_t device_ptrs,
                double *stokes_out, void *rf_out) {
-  if (!c || !cols || !stokes_out || !rf_out) return fail(RHG_EINVAL, "null argument");
+  const int32_t all[RHG_RF_NPARAM] = {RHG_RF_T, RHG_RF_VLOS, RHG_RF_B, RHG_RF_GAMMA, RHG_RF_CHI,
+                                      RHG_RF_VMIC};
+  return rhg_rf_lte_params(c, cols, precision, device_ptrs, RHG_RF_NPARAM, all, stokes_out, rf_out);
+}
+
+int rhg_rf_lte_params(rhg_context_t *c, const rhg_columns_t *cols, int32_t precision,
+                      int32_t device_ptrs, int32_t nparam, const int32_t *params,
+                      double *stokes_out, void *rf_out) {
+  if (!c || !cols || !stokes_out || !rf_out || !params) return fail(RHG_EINVAL, "null argument");
+  if (nparam < 1 || nparam > RHG_RF_NPARAM) return fail(RHG_EINVAL, "bad number of parameters");
+  if (c->dm.magneto_optical)
+    return fail(RHG_EUNSUPPORTED,
+                "response functions with the magneto-optical terms are not implemented");
+  rhg::RfParamList pl{};
+  pl.npar = nparam;
+  for (int i = 0; i < nparam; i++) {
+    if (params[i] < 0 || params[i] >= RHG_RF_NPARAM) return fail(RHG_EINVAL, "bad parameter id");
+    for (int j = 0; j < i; j++)
+      if (params[j] == params[i]) return fail(RHG_EINVAL, "repeated parameter id");
+    pl.p[i] = params[i];
+  }
   if (c->nout < 1) return fail(RHG_EINVAL, "the model has no output wavelength");
   if (precision != RHG_PREC_FP64 && precision != RHG_PREC_FP32)
     return fail(RHG_EINVAL, "precision must be RHG_PREC_FP64 or RHG_PREC_FP32");
@@ -872,8 +979,9 @@ int rhg_rf_lte(rhg_context_t *c, const rhg_columns_t *cols, int32_t precision, i
   std::vector<int32_t> perm, sndep;
   if (int rc = sort_order(c, cols, device_ptrs, perm, sndep)) return rc;
   if (perm.empty()) {
-    const int rc = precision == RHG_PREC_FP32 ? rf_run<float>(c, cols, device_ptrs, stokes_out, rf_out)
-                                              : rf_run<double>(c, cols, device_ptrs, stokes_out, rf_out);
+    const int rc = precision == RHG_PREC_FP32
+                       ? rf_run<float>(c, cols, device_ptrs, pl, stokes_out, rf_out)
+                       : rf_run<double>(c, cols, device_ptrs, pl, stokes_out, rf_out);
     if (rc) return rc;
     g_err.clear();
     return 0;
@@ -882,14 +990,14 @@ int rhg_rf_lte(rhg_context_t *c, const rhg_columns_t *cols, int32_t precision, i
   rhg_columns_t cols2;
   if (int rc = gather_sorted(c, cols, device_ptrs, perm, sndep, cols2)) return rc;
   const size_t esz = precision == RHG_PREC_FP32 ? sizeof(float) : sizeof(double);
-  const int64_t nouter = (int64_t) RHG_RF_NPARAM * Nmax * 4 * c->nout;
+  const int64_t nouter = (int64_t) pl.npar * Nmax * 4 * c->nout;
   const size_t rf_b = (size_t) nouter * ncol * esz;
   if (c->srt_out.ensure((size_t) ncol * 4 * c->nout * sizeof(double)) || c->srt_rf.ensure(rf_b) ||
       (!device_ptrs && c->rf_out.ensure(rf_b)))
     return fail(RHG_ENOMEM, "device allocation for the sorted batch failed");
   int rc = precision == RHG_PREC_FP32
-               ? rf_run<float>(c, &cols2, 1, (double *) c->srt_out.p, c->srt_rf.p)
-               : rf_run<double>(c, &cols2, 1, (double *) c->srt_out.p, c->srt_rf.p);
+               ? rf_run<float>(c, &cols2, 1, pl, (double *) c->srt_out.p, c->srt_rf.p)
+               : rf_run<double>(c, &cols2, 1, pl, (double *) c->srt_out.p, c->srt_rf.p);
   if (rc) return rc;
   if ((rc = scatter_stokes(c, ncol, device_ptrs, (const double *) c->srt_out.p, stokes_out))) return rc;
   void *rdst = device_ptrs ? rf_out : c->rf_out.p;
@@ -1725,6 +1833,12 @@ int rhg_nlte_timings(rhg_context_t *ctx, double *out) {
 int rhg_last_timings(rhg_context_t *ctx, double *ms_out) {
   if (!ctx || !ms_out) return fail(RHG_EINVAL, "null argument");
   for (int s = 0; s < 4; s++) ms_out[s] = ctx->last_ms[s];
+  return 0;
+}
+
+int rhg_rf_timings(rhg_context_t *ctx, double *ms_out) {
+  if (!ctx || !ms_out) return fail(RHG_EINVAL, "null argument");
+  for (int s = 0; s < 5; s++) ms_out[s] = ctx->rf_ms[s];
   return 0;
 }
 

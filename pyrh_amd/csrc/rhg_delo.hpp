@@ -19,21 +19,23 @@ struct Point {
   T chi;     // chi_I
   T S[4];    // eta_n / chi_I
   T q[3];    // chi_Q,U,V / chi_I
+  T r[3];    // magneto-optical chip_Q,U,V / chi_I (MO only)
 };
 
 // Point from the continuum (cchi, ceta) and the line opacity at one depth;
-// F(field) reads the column state there.
-template <typename T, typename Field>
+// F(field) reads the column state there; the Humlicek functions in TV; MO:
+// with the magneto-optical terms.
+template <typename T, typename TV = T, bool MO = false, typename Field>
 __device__ __forceinline__ void form_point_f(const DevModel &M, int ls, int has_line,
                                              const Field &F, T mu, int moving, T cchi, T ceta,
                                              Point<T> &p) {
-  T chi4[4], eta4[4];
+  T chi4[4], eta4[4], chip[3] = {T(0), T(0), T(0)};
   chi4[0] = cchi; eta4[0] = ceta;
   chi4[1] = chi4[2] = chi4[3] = T(0);
   eta4[1] = eta4[2] = eta4[3] = T(0);
   if (has_line) {
     T lI, lE, lq[3], le[3];
-    line_point_f<T>(M, ls, F, mu, lI, lq, lE, le, moving);
+    line_point_f<T, TV, MO>(M, ls, F, mu, lI, lq, lE, le, moving, chip);
     chi4[0] += lI; eta4[0] += lE;
 #pragma unroll
     for (int n = 0; n < 3; n++) { chi4[n + 1] += lq[n]; eta4[n + 1] += le[n]; }
@@ -44,6 +46,18 @@ __device__ __forceinline__ void form_point_f(const DevModel &M, int ls, int has_
   for (int n = 0; n < 4; n++) p.S[n] = eta4[n] * ichi;
 #pragma unroll
   for (int n = 0; n < 3; n++) p.q[n] = chi4[n + 1] * ichi;
+#pragma unroll
+  for (int n = 0; n < 3; n++) p.r[n] = MO ? chip[n] * ichi : T(0);
+}
+
+// FP64 point over the column state at st (stride fs) with the Humlicek
+// functions in TV and the continuum stored as R
+template <typename TV, typename R, bool MO = false>
+__device__ __forceinline__ void form_point_v(const DevModel &M, int ls, int has_line,
+                                             const double *st, int64_t fs, double mu, int moving,
+                                             R cchi, R ceta, Point<double> &p) {
+  auto F = [st, fs](int f) { return st[f * fs]; };
+  form_point_f<double, TV, MO>(M, ls, has_line, F, mu, moving, (double) cchi, (double) ceta, p);
 }
 
 // y = K(q) x for K(q) = [[0, q^T], [q, 0]]
@@ -92,6 +106,7 @@ struct DeloState {
   Point<T> Pu, P0, Pd;      // points at k+1, k, k-1
   T dSu[4], dqu[3];         // derivatives of S and q at the upwind point
   T dtau_uw, dsup, dchi_up, dchi_c;
+  T dru[3];                 // derivatives of the magneto-optical r (MO only)
 };
 
 // Set-up from the three deepest points and the thermalised lower boundary
@@ -117,6 +132,8 @@ __device__ __forceinline__ void delo_prologue(DeloState<T> &s, const Point<T> &P
   for (int n = 0; n < 4; n++) s.dSu[n] = (s.P0.S[n] - s.Pu.S[n]) * idt;
 #pragma unroll
   for (int n = 0; n < 3; n++) s.dqu[n] = (s.P0.q[n] - s.Pu.q[n]) * idt;
+#pragma unroll
+  for (int n = 0; n < 3; n++) s.dru[n] = (s.P0.r[n] - s.Pu.r[n]) * idt;
 }
 
 // The right-hand side V0 of a step and the solve Md Iv = V0 are formed in
@@ -271,6 +288,152 @@ __device__ __forceinline__ void delo_final(const DeloState<T> &s, T dz, T imu, T
   x[0] = rdiv(V0[0] - e * q0V, T(1) - e * e * qq);
 #pragma unroll
   for (int n = 0; n < 3; n++) x[n + 1] = V0[n + 1] - e * s.P0.q[n] * x[0];
+}
+
+// ---------------------------------------------------------------------------
+// The polarized step with the magneto-optical terms (MAGNETO_OPTICAL), FP64:
+// K(q, r) = [[0, q^T], [q, R]] with the antisymmetric
+// R = [[0, r_V, -r_U], [-r_V, 0, r_Q], [r_U, -r_Q, 0]] (rh/stokesopac.c:56-86).
+// Md = I + cM K0 + gM (K0^2 + dK0) is then a general 4x4 matrix, solved by
+// Gaussian elimination with partial pivoting.
+
+// y = K(q, r) x
+template <typename T>
+__device__ __forceinline__ void kmul_mo(const T *q, const T *r, const T *x, T *y) {
+  y[0] = q[0] * x[1] + q[1] * x[2] + q[2] * x[3];
+  y[1] = q[0] * x[0] + r[2] * x[2] - r[1] * x[3];
+  y[2] = q[1] * x[0] - r[2] * x[1] + r[0] * x[3];
+  y[3] = q[2] * x[0] + r[1] * x[1] - r[0] * x[2];
+}
+
+// K(q, r) as a matrix
+template <typename T>
+__device__ __forceinline__ void kmat_mo(const T *q, const T *r, T K[4][4]) {
+  K[0][0] = T(0); K[0][1] = q[0]; K[0][2] = q[1]; K[0][3] = q[2];
+  K[1][0] = q[0]; K[1][1] = T(0); K[1][2] = r[2]; K[1][3] = -r[1];
+  K[2][0] = q[1]; K[2][1] = -r[2]; K[2][2] = T(0); K[2][3] = r[0];
+  K[3][0] = q[2]; K[3][1] = r[1]; K[3][2] = -r[0]; K[3][3] = T(0);
+}
+
+// x = A^-1 b, A 4x4 (overwritten), Gaussian elimination with partial pivoting
+template <typename T>
+__device__ __forceinline__ void solve4_gepp(T A[4][4], const T *b, T *x) {
+  T y[4] = {b[0], b[1], b[2], b[3]};
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    int p = c;
+#pragma unroll
+    for (int r = c + 1; r < 4; r++)
+      if (fabs(A[r][c]) > fabs(A[p][c])) p = r;
+    if (p != c) {
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const T t = A[c][j]; A[c][j] = A[p][j]; A[p][j] = t;
+      }
+      const T t = y[c]; y[c] = y[p]; y[p] = t;
+    }
+    const T inv = rrcp(A[c][c]);
+#pragma unroll
+    for (int r = c + 1; r < 4; r++) {
+      const T f = A[r][c] * inv;
+#pragma unroll
+      for (int j = c + 1; j < 4; j++) A[r][j] = A[r][j] - f * A[c][j];
+      y[r] = y[r] - f * y[c];
+    }
+  }
+#pragma unroll
+  for (int r = 3; r >= 0; r--) {
+    T acc = y[r];
+#pragma unroll
+    for (int j = r + 1; j < 4; j++) acc = acc - A[r][j] * x[j];
+    x[r] = rdiv(acc, A[r][r]);
+  }
+}
+
+// One step at depth k with the magneto-optical terms (see delo_step).
+template <typename T>
+__device__ __forceinline__ void delo_step_mo(DeloState<T> &s, const Point<T> &Pn, bool has_pn,
+                                             T dzk, T dzk1, T imu) {
+  const T dsdn = dzk * imu;
+  T dchi_dn;
+  if (has_pn) {
+    const T dsdn2 = dzk1 * imu;
+    dchi_dn = CentDeriv<T>(dsdn, dsdn2)(s.P0.chi, s.Pd.chi, Pn.chi);
+  } else {
+    dchi_dn = rdiv(s.Pd.chi - s.P0.chi, dsdn);
+  }
+  const T c2 = rmax0(s.P0.chi + (dsdn * T(1.0 / 3.0)) * s.dchi_c);
+  const T c1 = rmax0(s.Pd.chi - (dsdn * T(1.0 / 3.0)) * dchi_dn);
+  const T dtau_dw = T(0.25) * dsdn * (s.P0.chi + s.Pd.chi + c1 + c2);
+  const Bezier3<T> bz = bezier3(s.dtau_uw);
+  const T dt03 = s.dtau_uw * T(1.0 / 3.0);
+  T dS0[4], dq0[3], dr0[3];
+  const CentDeriv<T> cd(s.dtau_uw, dtau_dw);
+#pragma unroll
+  for (int i = 0; i < 4; i++) dS0[i] = cd(s.Pu.S[i], s.P0.S[i], s.Pd.S[i]);
+#pragma unroll
+  for (int i = 0; i < 3; i++) dq0[i] = cd(s.Pu.q[i], s.P0.q[i], s.Pd.q[i]);
+#pragma unroll
+  for (int i = 0; i < 3; i++) dr0[i] = cd(s.Pu.r[i], s.P0.r[i], s.Pd.r[i]);
+  const T al = bz.alpha, be = bz.beta, ga = bz.gamma, th = bz.theta, ep = bz.eps;
+  const T ca = th * (dt03 - T(1)) - be, gu = th * dt03, cb0 = be + th, cc0 = al + ga,
+          g0 = ga * dt03, cM = al + ga * (T(1) + dt03);
+  T KI[4], KKI[4], dKI[4], KSu[4], KS0[4], V0[4];
+  kmul_mo(s.Pu.q, s.Pu.r, s.Iv, KI);
+  kmul_mo(s.Pu.q, s.Pu.r, KI, KKI);
+  kmul_mo(s.dqu, s.dru, s.Iv, dKI);
+  kmul_mo(s.Pu.q, s.Pu.r, s.Pu.S, KSu);
+  kmul_mo(s.P0.q, s.P0.r, s.P0.S, KS0);
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+    V0[i] = ep * s.Iv[i] + ca * KI[i] + gu * (KKI[i] + dKI[i]) + cb0 * s.Pu.S[i] - gu * KSu[i] +
+            cc0 * s.P0.S[i] + g0 * KS0[i] + dt03 * (ga * dS0[i] - th * s.dSu[i]);
+  T K0[4][4], dK0[4][4], Md[4][4];
+  kmat_mo(s.P0.q, s.P0.r, K0);
+  kmat_mo(dq0, dr0, dK0);
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      T kk = T(0);
+#pragma unroll
+      for (int m = 0; m < 4; m++) kk += K0[i][m] * K0[m][j];
+      Md[i][j] = (i == j ? T(1) : T(0)) + cM * K0[i][j] + g0 * (kk + dK0[i][j]);
+    }
+  solve4_gepp(Md, V0, s.Iv);
+  s.Pu = s.P0;
+  s.P0 = s.Pd;
+  if (has_pn) s.Pd = Pn;
+#pragma unroll
+  for (int n = 0; n < 4; n++) s.dSu[n] = dS0[n];
+#pragma unroll
+  for (int n = 0; n < 3; n++) { s.dqu[n] = dq0[n]; s.dru[n] = dr0[n]; }
+  s.dtau_uw = dtau_dw;
+  s.dsup = dsdn;
+  s.dchi_up = s.dchi_c;
+  s.dchi_c = dchi_dn;
+}
+
+// The last interval with the magneto-optical terms (see delo_final):
+// Md = I + (w0 - w1/dtau) K0, V0 = w0 S0 - w1 dSu + ((1 - w0) I - w1/dtau Ku) Iv.
+template <typename T>
+__device__ __forceinline__ void delo_final_mo(const DeloState<T> &s, T dz, T imu, T *x) {
+  T w[3];
+  const T dtl = T(0.5) * imu * (s.P0.chi + s.Pu.chi) * dz;
+  w3_t(dtl, w);
+  const T w0 = w[0], w1 = w[1], w1t = rdiv(w[1], dtl);
+  T KI[4], V0[4], Md[4][4];
+  kmul_mo(s.Pu.q, s.Pu.r, s.Iv, KI);
+#pragma unroll
+  for (int n = 0; n < 4; n++)
+    V0[n] = w0 * s.P0.S[n] - w1 * s.dSu[n] + (T(1) - w0) * s.Iv[n] - w1t * KI[n];
+  const T e = w0 - w1t;
+  kmat_mo(s.P0.q, s.P0.r, Md);
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) Md[i][j] = (i == j ? T(1) : T(0)) + e * Md[i][j];
+  solve4_gepp(Md, V0, x);
 }
 
 }  // namespace rhg

@@ -102,9 +102,29 @@ __device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
 // Reciprocal and quotient: IEEE-exact in FP64; in the FP32 form the
 // hardware reciprocal (v_rcp_f32, 1 ulp), which the form's 1e-4 tolerance
 // allows.
+//
+// A translation unit of the FP32 form defines RHG_FP32_FORM_TU before
+// including this header: its FP64 arithmetic (the recursion and the points,
+// which need FP64 range and precision but not the reference's rounding)
+// takes the reciprocal as the hardware estimate refined by two Newton steps
+// (relative error ~1e-16, not correctly rounded) instead of the IEEE
+// division sequence, and the Bezier weights below from their series.
+#ifdef RHG_FP32_FORM_TU
+__device__ __forceinline__ double rrcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+__device__ __forceinline__ double rdiv(double a, double b) { return a * rrcp(b); }
+constexpr bool kFp32FormTU = true;
+#else
 __device__ __forceinline__ double rrcp(double x) { return 1.0 / x; }
-__device__ __forceinline__ float rrcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double rdiv(double a, double b) { return a / b; }
+constexpr bool kFp32FormTU = false;
+#endif
+__device__ __forceinline__ float rrcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float rdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
 // a / b with one reciprocal of |b|^2
@@ -140,10 +160,12 @@ __device__ __forceinline__ int humlicek_region(R a, R v) {
 // quotient is taken after they rejoin, so a wavefront whose lanes sit in
 // several regions pays for the quotient once.  The operations and their
 // order per lane are those of the per-region formulas.
+// voigt_humlicek_r takes the region from the caller (the FP32 form tests
+// the region on the FP64 argument, so that both forms evaluate the same
+// approximation).
 template <typename R>
-__device__ __forceinline__ R voigt_humlicek(R a, R v, R *F) {
+__device__ __forceinline__ R voigt_humlicek_r(R a, R v, int region, R *F) {
   cpx<R> z = {a, -v}, z1, z2, u, e = {R(0), R(0)};
-  const int region = humlicek_region(a, v);
   if (region == 0) {
     z1 = {R(0.5641896) * z.r, R(0.5641896) * z.i};
     z2 = cmul(z, z);
@@ -190,6 +212,11 @@ __device__ __forceinline__ R voigt_humlicek(R a, R v, R *F) {
   if (region == 3) W = {e.r - W.r, e.i - W.i};
   *F = W.i;
   return W.r;
+}
+
+template <typename R>
+__device__ __forceinline__ R voigt_humlicek(R a, R v, R *F) {
+  return voigt_humlicek_r<R>(a, v, humlicek_region(a, v), F);
 }
 
 // ---------------------------------------------------------------------------
@@ -360,9 +387,31 @@ constexpr double e_k(int k) {
 }
 constexpr int BEZ_NSERIES = 10;
 
+// FP64 weights: the reference's branches, bezier3_coeffs().  In a
+// translation unit of the FP32 form the same branches with the quotients by
+// constants taken as products and 1/dt^3 by rrcp (no IEEE division).
 __device__ __forceinline__ Bezier3<double> bezier3(double dt) {
   Bezier3<double> b;
-  bezier3_coeffs(dt, b.alpha, b.beta, b.gamma, b.theta, b.eps);
+  if constexpr (kFp32FormTU) {
+    const double dt2 = dt * dt, dt3 = dt2 * dt;
+    if (dt >= 5.e-2) {
+      const double e = exp(-dt), idt3 = rrcp(dt3);
+      b.eps = e;
+      b.alpha = (-6.0 + 6.0 * dt - 3.0 * dt2 + dt3 + 6.0 * e) * idt3;
+      b.beta = (6.0 + (-6.0 - dt * (6.0 + dt * (3.0 + dt))) * e) * idt3;
+      b.gamma = 3.0 * (6.0 + (-4.0 + dt) * dt - 2.0 * (3.0 + dt) * e) * idt3;
+      b.theta = 3.0 * (e * (6.0 + dt2 + 4.0 * dt) + 2.0 * dt - 6.0) * idt3;
+    } else {
+      const double dt4 = dt2 * dt2;
+      b.eps = 1.0 - dt + 0.5 * dt2 - dt3 * (1.0 / 6.0) + dt4 * (1.0 / 24.0);
+      b.alpha = 0.25 * dt - 0.05 * dt2 + dt3 * (1.0 / 120.0) - dt4 * (1.0 / 840.0);
+      b.beta = 0.25 * dt - 0.20 * dt2 + dt3 * (1.0 / 12.0) - dt4 * (1.0 / 42.0);
+      b.gamma = 0.25 * dt - 0.10 * dt2 + dt3 * 0.025 - dt4 * (1.0 / 210.0);
+      b.theta = 0.25 * dt - 0.15 * dt2 + dt3 * 0.05 - dt4 * (1.0 / 84.0);
+    }
+  } else {
+    bezier3_coeffs(dt, b.alpha, b.beta, b.gamma, b.theta, b.eps);
+  }
   return b;
 }
 

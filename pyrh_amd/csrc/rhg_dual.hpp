@@ -22,6 +22,9 @@ struct Dual {
   template <typename S, typename = std::enable_if_t<std::is_arithmetic<S>::value>>
   __host__ __device__ __forceinline__ constexpr Dual(S val) : v(R(val)), d(R(0)) {}
   __host__ __device__ __forceinline__ constexpr Dual(R val, R der) : v(val), d(der) {}
+  // between precisions (the FP32 form's Voigt functions inside FP64 points)
+  template <typename S, typename = std::enable_if_t<!std::is_same<S, R>::value>>
+  __host__ __device__ __forceinline__ explicit constexpr Dual(Dual<S> o) : v(R(o.v)), d(R(o.d)) {}
 
   friend __device__ __forceinline__ Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
   friend __device__ __forceinline__ Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
@@ -172,5 +175,152 @@ __device__ __forceinline__ void w3d(double dtau, double *w, double *dw) {
     dw[1] = dtau * e;
   }
 }
+
+
+// ---------------------------------------------------------------------------
+// Multi-tangent dual numbers: a value and NP directional derivatives, for
+// the response functions' chain to the atmosphere parameters (one pass
+// carries every parameter's tangent; the value is computed once).
+template <typename R, int NP>
+struct DualN {
+  R v, d[NP];
+  DualN() = default;
+  template <typename S, typename = std::enable_if_t<std::is_arithmetic<S>::value>>
+  __device__ __forceinline__ DualN(S val) : v(R(val)) {
+#pragma unroll
+    for (int i = 0; i < NP; i++) d[i] = R(0);
+  }
+  template <typename S, typename = std::enable_if_t<!std::is_same<S, R>::value>>
+  __device__ __forceinline__ explicit DualN(const DualN<S, NP> &o) : v(R(o.v)) {
+#pragma unroll
+    for (int i = 0; i < NP; i++) d[i] = R(o.d[i]);
+  }
+  friend __device__ __forceinline__ DualN operator+(const DualN &a, const DualN &b) {
+    DualN r;
+    r.v = a.v + b.v;
+#pragma unroll
+    for (int i = 0; i < NP; i++) r.d[i] = a.d[i] + b.d[i];
+    return r;
+  }
+  friend __device__ __forceinline__ DualN operator-(const DualN &a, const DualN &b) {
+    DualN r;
+    r.v = a.v - b.v;
+#pragma unroll
+    for (int i = 0; i < NP; i++) r.d[i] = a.d[i] - b.d[i];
+    return r;
+  }
+  friend __device__ __forceinline__ DualN operator*(const DualN &a, const DualN &b) {
+    DualN r;
+    r.v = a.v * b.v;
+#pragma unroll
+    for (int i = 0; i < NP; i++) r.d[i] = a.d[i] * b.v + a.v * b.d[i];
+    return r;
+  }
+  friend __device__ __forceinline__ DualN operator/(const DualN &a, const DualN &b) {
+    DualN r;
+    const R ib = R(1) / b.v;
+    r.v = a.v * ib;
+#pragma unroll
+    for (int i = 0; i < NP; i++) r.d[i] = (a.d[i] - r.v * b.d[i]) * ib;
+    return r;
+  }
+  friend __device__ __forceinline__ DualN operator-(const DualN &a) {
+    DualN r;
+    r.v = -a.v;
+#pragma unroll
+    for (int i = 0; i < NP; i++) r.d[i] = -a.d[i];
+    return r;
+  }
+  __device__ __forceinline__ DualN &operator+=(const DualN &b) { *this = *this + b; return *this; }
+  __device__ __forceinline__ DualN &operator-=(const DualN &b) { *this = *this - b; return *this; }
+  __device__ __forceinline__ DualN &operator*=(const DualN &b) { *this = *this * b; return *this; }
+  friend __device__ __forceinline__ bool operator>(const DualN &a, const DualN &b) { return a.v > b.v; }
+  friend __device__ __forceinline__ bool operator<(const DualN &a, const DualN &b) { return a.v < b.v; }
+  friend __device__ __forceinline__ bool operator>=(const DualN &a, const DualN &b) { return a.v >= b.v; }
+  friend __device__ __forceinline__ bool operator<=(const DualN &a, const DualN &b) { return a.v <= b.v; }
+  friend __device__ __forceinline__ bool operator==(const DualN &a, const DualN &b) { return a.v == b.v; }
+  friend __device__ __forceinline__ bool operator!=(const DualN &a, const DualN &b) { return a.v != b.v; }
+};
+
+template <typename R, int NP>
+__device__ __forceinline__ R val(const DualN<R, NP> &x) { return x.v; }
+
+// f(x) with f(x.v) = fv and f'(x.v) = fd
+template <typename R, int NP>
+__device__ __forceinline__ DualN<R, NP> dn_apply(const DualN<R, NP> &x, R fv, R fd) {
+  DualN<R, NP> r;
+  r.v = fv;
+#pragma unroll
+  for (int i = 0; i < NP; i++) r.d[i] = fd * x.d[i];
+  return r;
+}
+template <typename R, int NP>
+__device__ __forceinline__ DualN<R, NP> fabs(const DualN<R, NP> &x) {
+  return dn_apply(x, x.v < R(0) ? -x.v : x.v, x.v < R(0) ? R(-1) : R(1));
+}
+template <typename R, int NP>
+__device__ __forceinline__ DualN<R, NP> rexp(const DualN<R, NP> &x) {
+  const R e = rexp(x.v);
+  return dn_apply(x, e, e);
+}
+template <typename R, int NP>
+__device__ __forceinline__ DualN<R, NP> rrcp(const DualN<R, NP> &x) {
+  const R r = rrcp(x.v);
+  return dn_apply(x, r, -r * r);
+}
+template <typename R, int NP>
+__device__ __forceinline__ DualN<R, NP> rdiv(const DualN<R, NP> &a, const DualN<R, NP> &b) {
+  return a * rrcp(b);
+}
+
+// Armstrong's function with its partials in a and v (as for Dual)
+template <typename R, int NP>
+__device__ __forceinline__ DualN<R, NP> voigt_armstrong_t(const DualN<R, NP> &a,
+                                                          const DualN<R, NP> &v) {
+  const double av = (double) a.v, vv = (double) v.v, h = 1.0e-6;
+  const double H = voigt_armstrong(av, vv);
+  const double da = (voigt_armstrong(av + h, vv) - voigt_armstrong(av > h ? av - h : av, vv)) /
+                    (av > h ? 2.0 * h : h);
+  const double dv = (voigt_armstrong(av, vv + h) - voigt_armstrong(av, vv - h)) / (2.0 * h);
+  DualN<R, NP> r;
+  r.v = (R) H;
+#pragma unroll
+  for (int i = 0; i < NP; i++) r.d[i] = (R) (da * (double) a.d[i] + dv * (double) v.d[i]);
+  return r;
+}
+
+// The Humlicek functions of the line profiles.  A plain or dual argument
+// evaluates voigt_humlicek() in that type.  For NP tangents: w = H + iF is
+// an analytic function of z = a - iv inside each region (rational, region
+// IV with exp(z^2)), so one evaluation on a dual number seeded in a gives
+// w' = dw/dz = (H_a, F_a), and dH = H_a da + F_a dv, dF = F_a da - H_a dv
+// (Cauchy-Riemann) for every tangent.
+// `region` is Humlicek's region of the argument (humlicek_region).
+template <typename T>
+struct VoigtEval {
+  static __device__ __forceinline__ T eval(T a, T v, int region, T *F) {
+    return voigt_humlicek_r<T>(a, v, region, F);
+  }
+};
+template <typename R, int NP>
+struct VoigtEval<DualN<R, NP>> {
+  static __device__ __forceinline__ DualN<R, NP> eval(const DualN<R, NP> &a,
+                                                      const DualN<R, NP> &v, int region,
+                                                      DualN<R, NP> *F) {
+    Dual<R> Fd;
+    const Dual<R> Hd =
+        voigt_humlicek_r<Dual<R>>(Dual<R>(a.v, R(1)), Dual<R>(v.v, R(0)), region, &Fd);
+    const R Ha = Hd.d, Fa = Fd.d;
+    DualN<R, NP> H;
+    H.v = Hd.v;
+    F->v = Fd.v;
+#pragma unroll
+    for (int i = 0; i < NP; i++) {
+      H.d[i] = Ha * a.d[i] + Fa * v.d[i];
+      F->d[i] = Fa * a.d[i] - Ha * v.d[i];
+    }
+    return H;
+  }
+};
 
 }  // namespace rhg

@@ -128,10 +128,15 @@ __device__ __forceinline__ double atom_sum(const DevModel &M, int l0, int nl, do
 // number type T (double, float, or a dual number of either).  F(field)
 // returns a column-state field at that depth as a T; the Doppler offset
 // (lambda/lambda0 - 1) c comes from the model in FP64.  Returns phi;
-// phi_Q,U,V are set for a polarizable line with damping.
-template <typename T, typename Field>
+// phi_Q,U,V are set for a polarizable line with damping, and with MO (the
+// magneto-optical terms) the Faraday-Voigt profiles psi[3] = psi_Q,U,V
+// (rh/kurucz.c:811-819).  The Humlicek functions of the Zeeman components
+// are evaluated in TV (T, or float in the FP32 form), their sums and
+// combinations in T.
+template <typename T, typename TV = T, bool MO = false, typename Field>
 __device__ __forceinline__ T line_profile_f(const DevModel &M, int e, int l, const Field &F, T mu,
-                                            T vz, int moving, T &phi_Q, T &phi_U, T &phi_V) {
+                                            T vz, int moving, T &phi_Q, T &phi_U, T &phi_V,
+                                            T *psi = nullptr) {
   const int lb = line_field_base(M.ncont);
   const T adamp = F(lb + 4 * l + 2), ivb = F(lb + 4 * l + 3);
   T v = T(M.lam_line_dv[e]) * ivb;
@@ -144,14 +149,25 @@ __device__ __forceinline__ T line_profile_f(const DevModel &M, int e, int l, con
       const T sin2_gamma = T(1.0) - cosg * cosg;
       const T vB = T(M.line_larmor[l]) * F(RHG_CF_B) * ivb;
       T phi_sm = T(0), phi_pi = T(0), phi_sp = T(0), H, Fv;
+      T psi_sm = T(0), psi_pi = T(0), psi_sp = T(0);
       const int c0 = M.line_comp_ptr[l], c1 = M.line_comp_ptr[l + 1];
       for (int nz = c0; nz < c1; nz++) {
-        H = voigt_humlicek<T>(adamp, v - T(M.comp_shift[nz]) * vB, &Fv);
+        TV Fw;
+        const T u = v - T(M.comp_shift[nz]) * vB;
+        const int reg = humlicek_region(val(adamp), val(u));
+        H = (T) VoigtEval<TV>::eval((TV) adamp, (TV) u, reg, &Fw);
+        Fv = (T) Fw;
         const int q = M.comp_q[nz];
         const T sH = T(M.comp_strength[nz]) * H;
         if (q == -1) phi_sm += sH;
         else if (q == 0) phi_pi += sH;
         else phi_sp += sH;
+        if constexpr (MO) {
+          const T sF = T(M.comp_strength[nz]) * Fv;
+          if (q == -1) psi_sm += sF;
+          else if (q == 0) psi_pi += sF;
+          else psi_sp += sF;
+        }
       }
       const T phi_sigma = phi_sp + phi_sm;
       const T phi_delta = T(0.5) * phi_pi - T(0.25) * phi_sigma;
@@ -159,6 +175,13 @@ __device__ __forceinline__ T line_profile_f(const DevModel &M, int e, int l, con
       phi_Q = phi_delta * sin2_gamma * F(RHG_CF_COS2X) * sv;
       phi_U = phi_delta * sin2_gamma * F(RHG_CF_SIN2X) * sv;
       phi_V = T(0.5) * (phi_sp - phi_sm) * cosg * sv;
+      if constexpr (MO) {
+        const T psi_sigma = psi_sp + psi_sm;
+        const T psi_delta = T(0.5) * psi_pi - T(0.25) * psi_sigma;
+        psi[0] = psi_delta * sin2_gamma * F(RHG_CF_COS2X) * sv;
+        psi[1] = psi_delta * sin2_gamma * F(RHG_CF_SIN2X) * sv;
+        psi[2] = T(0.5) * (psi_sp - psi_sm) * cosg * sv;
+      }
     } else {
       phi = voigt_armstrong_t(adamp, v) * sv;
     }
@@ -172,21 +195,24 @@ __device__ __forceinline__ T line_profile_f(const DevModel &M, int e, int l, con
 // Kurucz-line opacity and emissivity of the emergent ray (to_obs = 1) at
 // wavelength ls and one depth, rh/kurucz.c:600-715: the sum over the
 // wavelength's line list of line_profile_f times the line's chi_l, eta_l.
-// chiI/etaI and chiQ[3]/etaQ[3] accumulate from zero.
-template <typename T, typename Field>
+// chiI/etaI and chiQ[3]/etaQ[3] accumulate from zero; with MO also the
+// magneto-optical chip[3] (chi_l psi_Q,U,V, rh/kurucz.c:711-715).
+template <typename T, typename TV = T, bool MO = false, typename Field>
 __device__ __forceinline__ void line_point_f(const DevModel &M, int ls, const Field &F, T mu,
-                                             T &chiI, T *chiQ, T &etaI, T *etaQ, int moving) {
+                                             T &chiI, T *chiQ, T &etaI, T *etaQ, int moving,
+                                             T *chip = nullptr) {
   const int p0 = M.lam_line_ptr[ls], p1 = M.lam_line_ptr[ls + 1];
   const int lb = line_field_base(M.ncont);
   chiI = T(0); etaI = T(0);
   chiQ[0] = chiQ[1] = chiQ[2] = T(0);
   etaQ[0] = etaQ[1] = etaQ[2] = T(0);
+  if constexpr (MO) chip[0] = chip[1] = chip[2] = T(0);
   const T vz = F(RHG_CF_VLOS);
   for (int e = p0; e < p1; e++) {
     const int l = M.lam_line_idx[e];
     const T chi_l = F(lb + 4 * l), eta_l = F(lb + 4 * l + 1);
-    T phi_Q = T(0), phi_U = T(0), phi_V = T(0);
-    const T phi = line_profile_f<T>(M, e, l, F, mu, vz, moving, phi_Q, phi_U, phi_V);
+    T phi_Q = T(0), phi_U = T(0), phi_V = T(0), psi[3] = {T(0), T(0), T(0)};
+    const T phi = line_profile_f<T, TV, MO>(M, e, l, F, mu, vz, moving, phi_Q, phi_U, phi_V, psi);
     if (phi != T(0)) {
       chiI += chi_l * phi;
       etaI += eta_l * phi;
@@ -197,6 +223,11 @@ __device__ __forceinline__ void line_point_f(const DevModel &M, int ls, const Fi
         etaQ[0] += eta_l * phi_Q;
         etaQ[1] += eta_l * phi_U;
         etaQ[2] += eta_l * phi_V;
+        if constexpr (MO) {
+          chip[0] += chi_l * psi[0];
+          chip[1] += chi_l * psi[1];
+          chip[2] += chi_l * psi[2];
+        }
       }
     }
   }
@@ -251,7 +282,7 @@ template <typename R>
 __global__ void k_continuum(const DevModel *M, const int32_t *ndep, int ncol, int Nmax,
                             const double *state, int nfield, const int32_t *out_ls, int nout,
                             R *cchi, R *ceta);
-template <typename R>
+template <typename R, bool MO>
 __global__ void k_formal_stokes(const DevModel *M, const int32_t *ndep, const int32_t *moving,
                                 int ncol, int Nmax, const double *state, const int32_t *out_ls,
                                 const int32_t *io_list, int nlist, int nout, const R *cchi,
@@ -266,7 +297,7 @@ __global__ void k_rf_loggf(const DevModel *M, const int32_t *ndep, const int32_t
                            const int32_t *io_list, int nlist, int nout, const double *cchi,
                            const double *ceta, double mu, int npar, const int32_t *par_line,
                            double *Iin, double *rf);
-constexpr int NCARRY = 15;   // carried scalars of the DELO recursion state
+constexpr int RF_NCARRY = 13;  // carried scalars entering a DELO step (StepCarry)
 __global__ void k_rf_perturb(const double *atmos, double *out, int64_t n, int Nmax, int row,
                              double delta);
 template <typename O>
@@ -275,19 +306,24 @@ template <typename R>
 __global__ void k_rf_forward(const DevModel *M, const int32_t *ndep, const int32_t *moving, int ncol,
                              int Nmax, const double *state, const int32_t *out_ls,
                              const int32_t *io_list, int nlist, int nout, const R *cchi,
-                             const R *ceta, double mu, double *stokes, R *pts, R *car);
+                             const R *ceta, double mu, double *stokes, double *car);
 template <typename R>
 __global__ void k_rf_reverse(const DevModel *M, const int32_t *ndep, const int32_t *moving, int ncol,
                              int Nmax, const double *state, const int32_t *out_ls,
-                             const int32_t *io_list, int nlist, double mu, const R *pts,
-                             const R *car, R *aP, R *aZ, R *aB);
-template <typename R>
+                             const int32_t *io_list, int nlist, const R *cchi, const R *ceta,
+                             double mu, const double *car, R *aP, R *aZ, R *aB);
+// the response-function parameters of one call (rhg_rf_param order)
+struct RfParamList {
+  int32_t npar;
+  int32_t p[RHG_RF_NPARAM];
+};
+template <typename R, int NP>
 __global__ void k_rf_chain(const DevModel *M, const int32_t *ndep, const int32_t *moving, int ncol,
                            int Nmax, const double *state, const double *dstate, const double *chiref,
                            const double *dchiref, const R *cchi, const R *ceta, const R *dcchi,
                            const R *dceta, const int32_t *out_ls, const int32_t *io_list, int nlist,
-                           int nout, double mu, int atm_scale, const R *aP, const R *aZ,
-                           const R *aB, R *rf);
+                           int nout, double mu, int atm_scale, RfParamList pl, const R *aP,
+                           const R *aZ, const R *aB, R *rf, int64_t ldr);
 __global__ void k_census(const DevModel *M, const int32_t *ndep, const int32_t *moving, int ncol,
                          int Nmax, const double *state, int nfield, const int32_t *out_ls,
                          int nout, double mu, int32_t *census);

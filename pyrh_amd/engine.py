@@ -25,7 +25,7 @@ RF_PARAMS = ("T", "vlos", "B", "gamma", "chi", "vmic")
 RF_ROWS = (1, 3, 5, 6, 7, 4)        # atmosphere rows of the parameters
 STAGES = ("state", "geometry", "continuum", "formal")
 # the four intervals rhg_last_timings reports after rhg_rf_lte
-RF_STAGES = ("synthesis_inputs", "state_tangents", "tape", "reverse_and_chain")
+RF_STAGES = ("synthesis_inputs", "state_tangents", "forward", "reverse", "chain")
 # rhg_nlte_timings slots (RHG_NLTE_NTIMING)
 NLTE_TIMINGS = ("setup_ms", "scatter_bezier_ms", "iter_bezier_ms", "stateq_ms", "ng_ms",
                 "total_ms", "rt_passes", "rt_pairs", "scatter_feautrier_ms", "iter_feautrier_ms")
@@ -67,9 +67,12 @@ def lib():
         L.rhg_synth_lte.argtypes = [C.c_void_p, C.POINTER(RhgColumns), C.c_int32, C.c_int32, PD]
         L.rhg_synth_lte_diag.argtypes = [C.c_void_p, C.POINTER(RhgColumns), C.c_int32, PD, PD, PD]
         L.rhg_last_timings.argtypes = [C.c_void_p, PD]
+        L.rhg_rf_timings.argtypes = [C.c_void_p, PD]
         L.rhg_count_work.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
         L.rhg_rf_lte.argtypes = [C.c_void_p, C.POINTER(RhgColumns), C.c_int32, C.c_int32, PD,
                                  C.c_void_p]
+        L.rhg_rf_lte_params.argtypes = [C.c_void_p, C.POINTER(RhgColumns), C.c_int32, C.c_int32,
+                                        C.c_int32, C.POINTER(C.c_int32), PD, C.c_void_p]
         L.rhg_rf_loggf.argtypes = [C.c_void_p, C.POINTER(RhgColumns), C.c_int32,
                                    C.POINTER(C.c_int32), C.c_int32, PD, PD]
         L.rhg_nlte_solve.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(RhgColumns), C.c_int32,
@@ -149,18 +152,22 @@ class Engine:
         _check(lib().rhg_synth_lte(self._h, C.byref(s), int(precision), 1,
                                    C.cast(C.c_void_p(d_out), C.POINTER(C.c_double))))
 
-    def rf_lte(self, atmos, mu=1.0, atm_scale=0, ndep=None, precision=0):
+    def rf_lte(self, atmos, mu=1.0, atm_scale=0, ndep=None, precision=0, params=None):
         """Stokes [ncol, 4, nout] and response functions
-        [ncol, RF_NPARAM, ndep_max, 4, nout] (d Stokes / d parameter at each
+        [ncol, nparam, ndep_max, 4, nout] (d Stokes / d parameter at each
         depth; parameters T, v_LOS, B, gamma, chi, v_mic in the atmosphere's
-        units) for host atmospheres [ncol, 9, ndep_max]."""
+        units, or the subset `params` of RF_PARAMS names in that order) for
+        host atmospheres [ncol, 9, ndep_max]."""
         cs = ColumnsStruct(atmos, ndep, atm_scale, mu)
         ncol, nd = cs.struct.ncol, cs.struct.ndep_max
         out = np.empty((ncol, 4, self.nout))
-        rf = np.empty((RF_NPARAM, nd, 4, self.nout, ncol),
+        ids = np.arange(RF_NPARAM, dtype=np.int32) if params is None else \
+            np.array([RF_PARAMS.index(p) for p in params], np.int32)
+        rf = np.empty((ids.size, nd, 4, self.nout, ncol),
                       dtype=np.float32 if precision == 1 else np.float64)
-        _check(lib().rhg_rf_lte(self._h, cs.ptr(), int(precision), 0, _pd(out),
-                                rf.ctypes.data_as(C.c_void_p)))
+        _check(lib().rhg_rf_lte_params(self._h, cs.ptr(), int(precision), 0, int(ids.size),
+                                       ids.ctypes.data_as(C.POINTER(C.c_int32)), _pd(out),
+                                       rf.ctypes.data_as(C.c_void_p)))
         return out, np.ascontiguousarray(np.moveaxis(rf, -1, 0))
 
     def rf_loggf(self, atmos, par_line, mu=1.0, atm_scale=0, ndep=None):
@@ -177,18 +184,22 @@ class Engine:
         return out, rf
 
     def rf_lte_device(self, d_atmos, d_ndep, ncol, ndep_max, d_out, d_rf, mu=1.0, atm_scale=0,
-                      precision=0):
+                      precision=0, params=None):
         """rf_lte with device pointers: Stokes [ncol, 4, nout] float64 and the
-        response functions in the library's layout [RF_NPARAM, ndep_max, 4,
-        nout, ncol] (float32 for precision 1, else float64)."""
+        response functions in the library's layout [nparam, ndep_max, 4,
+        nout, ncol] (float32 for precision 1, else float64) for all
+        parameters or the subset `params` (RF_PARAMS names)."""
         s = RhgColumns()
         s.ncol, s.ndep_max = int(ncol), int(ndep_max)
         s.ndep = C.cast(C.c_void_p(d_ndep), C.POINTER(C.c_int32))
         s.atmos = C.cast(C.c_void_p(d_atmos), C.POINTER(C.c_double))
         s.atm_scale, s.mu = int(atm_scale), float(mu)
-        _check(lib().rhg_rf_lte(self._h, C.byref(s), int(precision), 1,
-                                C.cast(C.c_void_p(d_out), C.POINTER(C.c_double)),
-                                C.c_void_p(d_rf)))
+        ids = np.arange(RF_NPARAM, dtype=np.int32) if params is None else \
+            np.array([RF_PARAMS.index(p) for p in params], np.int32)
+        _check(lib().rhg_rf_lte_params(self._h, C.byref(s), int(precision), 1, int(ids.size),
+                                       ids.ctypes.data_as(C.POINTER(C.c_int32)),
+                                       C.cast(C.c_void_p(d_out), C.POINTER(C.c_double)),
+                                       C.c_void_p(d_rf)))
 
     def nlte_solve(self, nm, atmos, mu=1.0, atm_scale=0, ndep=None, stokes=False, nstar=False):
         """NLTE populations of the active atom of nm (pyrh_amd.nlte.build_nlte;
@@ -308,8 +319,8 @@ class Engine:
 
     def rf_timings(self):
         """Per-interval kernel milliseconds of the last rf_lte call."""
-        t = np.zeros(RHG_NSTAGE)
-        _check(lib().rhg_last_timings(self._h, _pd(t)))
+        t = np.zeros(len(RF_STAGES))
+        _check(lib().rhg_rf_timings(self._h, _pd(t)))
         return dict(zip(RF_STAGES, t))
 
     def count_work(self):

@@ -630,8 +630,6 @@ def _check_supported(setup, nlte=False):
         raise NotImplementedError("ATMOS_ITOP irradiation at the top boundary with an ACTIVE atom")
     if kw.str("STOKES_MODE") not in STOKES_MODES:
         raise RHInputError(f"Unknown STOKES_MODE = {kw.str('STOKES_MODE')}")
-    if kw.bool("MAGNETO_OPTICAL"):
-        raise NotImplementedError("MAGNETO_OPTICAL = TRUE")
     if not nlte and any(a.active for a in setup.atoms):
         raise NotImplementedError("active atoms (NLTE) are not solved by the LTE path; "
                                   "use pyrh_amd.nlte")
@@ -723,8 +721,11 @@ def build_model(setup: RunSetup, wave, nlte=False, check=True, lam_ref=None,
     lc[:, 21] = scatt_f
     # irradiation at the top (ATMOS_ITOP, rh/rhf1d/multiatmos.c:185-222): an
     # XDR file of Nspect x Nrays doubles on the merged grid, read after
-    # rhf1d() has set one ray (rh/rhf1d/pyrh_compute1dray.c:293, 308); only
-    # the Feautrier solver of the emergent ray uses it (feautrier.c:94-96)
+    # rhf1d() has set one ray (rh/rhf1d/pyrh_compute1dray.c:293, 308); the
+    # Feautrier solver of the emergent ray uses it (feautrier.c:94-96), and
+    # so does the inward scalar Bezier ray of the log gf response functions
+    # (bezier_1D.c:370-380).  A relative path is taken from the run
+    # directory; pyrh opens it from the process's working directory.
     itop = kw.str("ATMOS_ITOP")
     if itop != "none":
         path = itop if os.path.isabs(itop) else os.path.join(setup.cwd, itop)
@@ -878,7 +879,11 @@ def build_model(setup: RunSetup, wave, nlte=False, check=True, lam_ref=None,
                             polar = True
         line_ptr.append(len(line_idx))
         if polar:
-            solver[n] = 2 if smode == 3 else 1
+            # outside FULL_STOKES a polarizable line still marks the
+            # wavelength polarized (pyrh sets atmos.Stokes,
+            # pyrh_compute1dray.c:258; kurucz.c:656-659), which keeps it
+            # angle-dependent in static columns too (formal.c:98-101)
+            solver[n] = 2 if smode == 3 else 3
         elif hasline:
             solver[n] = 1   # refined per column by the 'moving' flag
         else:
@@ -1010,7 +1015,8 @@ def build_model(setup: RunSetup, wave, nlte=False, check=True, lam_ref=None,
     for k_, v_ in la_.items():
         arrays[k_] = v_ if v_.size else np.zeros(1, v_.dtype)
     scalars = dict(
-        stokes_mode=smode, magneto_optical=0, vmacro_tresh=vmacro, vmicro_char=vchar,
+        stokes_mode=smode, magneto_optical=int(kw.bool("MAGNETO_OPTICAL")),
+        vmacro_tresh=vmacro, vmicro_char=vchar,
         wght_per_H=el.wght_per_H, totalAbund=el.totalAbund,
         gravity=K.pow10(4.4) * K.CM_TO_M,
         nspect=nspect, ref_index=ref_index, natom=len(atoms), ncont=len(cont_desc),

@@ -82,3 +82,25 @@ def test_compute1d_get_atomic_rfs(fixtures):
     big = np.max(np.abs(ref), axis=0)
     assert np.max(np.abs(rf[:ref.shape[1]].T - ref) / big) < TOL
     np.testing.assert_allclose(I, fixtures[name + "__stokes_o2"][0, 0], rtol=1e-10, atol=0)
+
+
+@pytest.mark.parametrize("tag", ("mu1", "mu07"))
+def test_loggf_rf_matches_reference_itop(tag):
+    """d I / d log gf with ATMOS_ITOP (NO_STOKES, moving columns): the inward
+    ray starts at the irradiating intensity (rh/rhf1d/bezier_1D.c:378-379),
+    GPU against the reference's own output."""
+    from conftest import GOLDEN
+    from pyrh_amd.engine import Engine
+    f = np.load(GOLDEN / "static_fixtures.npz")
+    lg = f["itop_loggf"]
+    kw = dict(loggf_ids=lg[:, 0].astype(np.int32), loggf_values=lg[:, 1])
+    s = model.open_run(str(GOLDEN / "run_lte_itop_nostokes"), **kw)
+    m = model.build_model(s, f["itop_wave"])
+    par = model.atomic_rf_lines(s, kw["loggf_ids"], 0)
+    eng = Engine(m)
+    st, rf = eng.rf_loggf(f["itop_atmos"], par, float(f[f"itop_rf_{tag}__mu"]), 0)
+    ref, sref = f[f"itop_rf_{tag}__rf"], f[f"itop_rf_{tag}__stokes"]
+    err = float(np.max(np.abs(rf - ref)) / np.max(np.abs(ref)))
+    eI = float(np.max(np.abs(st[:, 0] - sref[:, 0]) / sref[:, 0]))
+    print(f"itop rf {tag}: rf {err:.2e}  I {eI:.2e}", flush=True)
+    assert err < TOL and eI < 1e-10

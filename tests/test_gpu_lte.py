@@ -159,10 +159,33 @@ def test_gpu_work_census(engines):
     print(c, flush=True)
 
 
+def test_gpu_fp32_form_matches_reference(fixtures, setups):
+    """The FP32 form directly against the reference's output on every golden
+    case: I within 1e-4 relative, Q/U/V within 1e-5 I_c."""
+    from conftest import fixture_cases, golden_model
+    from pyrh_amd.engine import Engine
+    F = fixtures
+    worst = [0.0, 0.0]
+    for name in fixture_cases(F):
+        m = golden_model(F, setups, name)
+        eng = Engine(m)
+        st = eng.synth_lte(F[name + "__atmos"], float(F[name + "__mu"]),
+                           int(F[name + "__scale"]), precision=1)
+        eng.close()
+        ref = F[name + "__stokes_o2"]
+        Ic = ref[:, 0, :1]
+        eI = float(np.max(np.abs(st[:, 0] - ref[:, 0]) / ref[:, 0]))
+        eP = float(np.max(np.abs(st[:, 1:] - ref[:, 1:]) / Ic[:, None]))
+        print(f"{name} FP32 form: I {eI:.2e}  QUV/Ic {eP:.2e}", flush=True)
+        worst = [max(worst[0], eI), max(worst[1], eP)]
+    assert worst[0] < 1e-4 and worst[1] < 1e-5
+
+
 def test_gpu_fp32_form_accuracy(engines):
-    """FP32 form (float line opacity and DELO-Bezier recursion, FP64
-    equilibria, heights and Bezier weights) against the FP64 path:
-    I within 1e-4 of I_c, Q/U/V within 1e-5 I_c."""
+    """FP32 form (float continuum and Voigt functions, FP64 profile sums,
+    source functions, DELO-Bezier recursion, equilibria and heights) against
+    the FP64 path on 512 map columns: I within 1e-4 of I_c, Q/U/V within
+    1e-5 I_c."""
     wave = workloads.wavelengths_630()
     eng = engines("lte", wave)
     atm = workloads.stokes_map(512, seed=51)
@@ -283,3 +306,64 @@ def test_itop_matches_reference(tag):
     eP = float(np.max(np.abs(st[:, 1:] - ref[:, 1:]) / Ic[:, None]))
     print(f"itop {tag}: I {eI:.2e}  QUV/Ic {eP:.2e}", flush=True)
     assert eI < 1e-10 and eP < 1e-10
+
+
+@pytest.mark.parametrize("mode,tag", [(m, t) for m in ("nostokes", "field_free")
+                                      for t in ("mu1", "mu07")])
+def test_static_polarizable_matches_reference(mode, tag):
+    """NO_STOKES / FIELD_FREE with VMACRO_TRESH = 0.1 km/s, two static columns
+    and a moving one: a polarizable line keeps the static columns on the
+    scalar Bezier (rh/rhf1d/formal.c:98-101), GPU against the reference's
+    rhf1d() within 1e-10."""
+    from conftest import GOLDEN
+    from pyrh_amd import api
+    f = np.load(GOLDEN / "static_fixtures.npz")
+    e = api.engine_for(str(GOLDEN / f"run_lte_{mode}_static"), f["wave"])
+    st = e.synth_lte(f["atmos"], float(f[f"{mode}_{tag}__mu"]))
+    ref = f[f"{mode}_{tag}__stokes"]
+    err = float(np.max(np.abs(st[:, 0] - ref[:, 0]) / ref[:, 0]))
+    print(f"static {mode} {tag}: I {err:.2e}", flush=True)
+    assert err < 1e-10
+    assert np.all(st[:, 1:] == 0.0)
+
+
+@pytest.mark.parametrize("name", ("mo_single", "mo_map", "mo_mu07"))
+def test_magneto_optical_matches_reference(name):
+    """MAGNETO_OPTICAL = TRUE on the GPU against the reference's rhf1d()
+    (tests/golden/make_golden_mo.py): the plain FP64 path within 1e-10, the
+    FP32 form within 1e-4 in I and 1e-5 I_c in Q, U, V."""
+    from conftest import GOLDEN
+    from pyrh_amd import api
+    f = np.load(GOLDEN / "mo_fixtures.npz")
+    e = api.engine_for(str(GOLDEN / "run_lte_mo"), f["wave"])
+    atm, mu = f[name + "__atmos"], float(f[name + "__mu"])
+    ref = f[name + "__stokes"]
+    Ic = ref[:, 0, :1]
+    for prec, tI, tP in ((0, 1e-10, 1e-10), (1, 1e-4, 1e-5)):
+        st = e.synth_lte(atm, mu, precision=prec)
+        eI = float(np.max(np.abs(st[:, 0] - ref[:, 0]) / ref[:, 0]))
+        eP = float(np.max(np.abs(st[:, 1:] - ref[:, 1:]) / Ic[:, None]))
+        print(f"MO {name} precision {prec}: I {eI:.2e}  QUV/Ic {eP:.2e}", flush=True)
+        assert eI < tI and eP < tP
+
+
+def test_magneto_optical_matches_oracle_on_map():
+    """48 seeded map columns with the magneto-optical terms at mu = 0.6
+    against the oracle (FP64 within 1e-10); their response functions are
+    not implemented and say so."""
+    import pyoracle
+    from conftest import GOLDEN
+    from pyrh_amd import api
+    from pyrh_amd.engine import EngineError
+    wave = workloads.wavelengths_630()
+    e = api.engine_for(str(GOLDEN / "run_lte_mo"), wave)
+    atm = workloads.stokes_map(48, seed=workloads.SEED_BASE + 63)
+    st = e.synth_lte(atm, 0.6)
+    ref = pyoracle.synth_lte(e.model, atm, mu=0.6)
+    Ic = ref[:, 0, :1]
+    eI = float(np.max(np.abs(st[:, 0] - ref[:, 0]) / ref[:, 0]))
+    eP = float(np.max(np.abs(st[:, 1:] - ref[:, 1:]) / Ic[:, None]))
+    print(f"MO map vs oracle: I {eI:.2e}  QUV/Ic {eP:.2e}", flush=True)
+    assert eI < 1e-10 and eP < 1e-10
+    with pytest.raises(EngineError):
+        e.rf_lte(atm[:2])

@@ -81,12 +81,21 @@ def test_model_rejects_unsupported(setups, tmp_path):
     import shutil
     d = tmp_path / "run"
     shutil.copytree(ROOT / "tests/golden/run_lte", d)
-    kw = (d / "keyword.input").read_text().replace("MAGNETO_OPTICAL = FALSE",
-                                                   "MAGNETO_OPTICAL = TRUE")
+    kw = (d / "keyword.input").read_text().replace("RLK_SCATTER = FALSE",
+                                                   "RLK_SCATTER = TRUE")
     (d / "keyword.input").write_text(kw)
     s = model.open_run(str(d))
     with pytest.raises(NotImplementedError):
         model.build_model(s, workloads.wavelengths_630())
+
+
+def test_model_magneto_optical_switch():
+    """MAGNETO_OPTICAL = TRUE reaches the model's switch (the polarized
+    solver then carries the Faraday-Voigt terms)."""
+    for run, flag in (("run_lte", 0), ("run_lte_mo", 1)):
+        s = model.open_run(str(ROOT / "tests/golden" / run))
+        m = model.build_model(s, workloads.wavelengths_630())
+        assert m.scalars["magneto_optical"] == flag
 
 
 def test_workloads_are_seeded():

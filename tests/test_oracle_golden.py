@@ -57,13 +57,14 @@ def test_oracle_loggf_rf_matches_reference(fixtures, setups, name):
 
 
 def test_fixture_cases_cover_every_solver(fixtures, setups):
-    """Stokes DELO-Bezier, scalar Bezier and Feautrier all occur."""
+    """Stokes DELO-Bezier, scalar Bezier (with and without a polarizable
+    line) and Feautrier all occur."""
     from pyrh_amd import model
     seen = set()
     for name in fixture_cases(fixtures):
         m = golden_model(fixtures, setups, name)
         seen |= set(np.unique(m.arrays["solver"][m.out_index]).tolist())
-    assert seen == {0, 1, 2}
+    assert seen == {0, 1, 2, 3}
 
 
 def test_depth_scale_and_reference_opacity(fixtures, setups):
@@ -209,3 +210,80 @@ def test_oracle_matches_reference_itop(tag):
     m0 = model.build_model(model.open_run(str(GOLDEN / "run_lte")), f["wave"])
     st0 = pyoracle.synth_lte(m0, f["atmos"][:1], mu=float(f[tag + "__mu"]))
     assert float(np.max(np.abs(st0[0, 0] - ref[0, 0]) / ref[0, 0])) > 1e-4
+
+
+STATIC_CASES = [(m, t) for m in ("nostokes", "field_free") for t in ("mu1", "mu07")]
+
+
+@pytest.mark.parametrize("mode,tag", STATIC_CASES)
+def test_oracle_matches_reference_static_polarizable(mode, tag):
+    """NO_STOKES and FIELD_FREE with VMACRO_TRESH = 0.1 km/s: two static
+    columns and a moving one.  The polarizable Fe I lines keep their
+    wavelengths angle-dependent, so the static columns take the scalar Bezier
+    like the moving one (tests/golden/make_golden_static.py).  Switching
+    them to Feautrier, as for an unpolarizable line, misses by > 1e-5."""
+    import pyoracle
+    from pyrh_amd import model
+    f = np.load(GOLDEN / "static_fixtures.npz")
+    m = model.build_model(model.open_run(str(GOLDEN / f"run_lte_{mode}_static")), f["wave"])
+    sol = m.arrays["solver"][m.out_index]
+    assert np.all(np.isin(sol, (0, 3))) and np.any(sol == 3)
+    mu = float(f[f"{mode}_{tag}__mu"])
+    st = pyoracle.synth_lte(m, f["atmos"], mu=mu)
+    ref = f[f"{mode}_{tag}__stokes"]
+    assert float(np.max(np.abs(st[:, 0] - ref[:, 0]) / ref[:, 0])) < 1e-13
+    assert np.all(st[:, 1:] == 0.0)
+    # the planted mistake: the static columns on Feautrier
+    m.arrays["solver"] = np.where(m.arrays["solver"] == 3, 1, m.arrays["solver"])
+    bad = pyoracle.synth_lte(m, f["atmos"], mu=mu)
+    assert float(np.max(np.abs(bad[:2, 0] - ref[:2, 0]) / ref[:2, 0])) > 1e-5
+    np.testing.assert_array_equal(bad[2], st[2])
+
+
+@pytest.mark.parametrize("tag", ("mu1", "mu07"))
+def test_oracle_loggf_rf_matches_reference_itop(tag):
+    """d I / d log gf with ATMOS_ITOP (NO_STOKES): the inward ray starts at
+    the irradiating intensity (rh/rhf1d/bezier_1D.c:378-379), and the
+    emergent ray's log gf recursion reads it.  Starting it at zero instead
+    misses the reference by > 1e-4 of the largest response."""
+    import pyoracle
+    from pyrh_amd import model
+    f = np.load(GOLDEN / "static_fixtures.npz")
+    lg = f["itop_loggf"]
+    kw = dict(loggf_ids=lg[:, 0].astype(np.int32), loggf_values=lg[:, 1])
+    s = model.open_run(str(GOLDEN / "run_lte_itop_nostokes"), **kw)
+    m = model.build_model(s, f["itop_wave"])
+    par = model.atomic_rf_lines(s, kw["loggf_ids"], 0)
+    mu = float(f[f"itop_rf_{tag}__mu"])
+    st, rf = pyoracle.rf_loggf(m, f["itop_atmos"], par, mu=mu)
+    ref, sref = f[f"itop_rf_{tag}__rf"], f[f"itop_rf_{tag}__stokes"]
+    big = np.max(np.abs(ref))
+    assert float(np.max(np.abs(rf - ref))) / big < 1e-13
+    assert float(np.max(np.abs(st[:, 0] - sref[:, 0]) / sref[:, 0])) < 1e-13
+    # the planted mistake: no irradiation on the inward ray
+    m.arrays["lamconst"] = m.arrays["lamconst"].copy()
+    lc = m.arrays["lamconst"].reshape(m.lam.size, -1)
+    lc[:, 22] = 0.0          # RHG_LC_ITOP
+    rf0 = pyoracle.rf_loggf(m, f["itop_atmos"], par, mu=mu)[1]
+    assert float(np.max(np.abs(rf0 - ref))) / big > 1e-4
+
+
+@pytest.mark.parametrize("name", ("mo_single", "mo_map", "mo_mu07"))
+def test_oracle_matches_reference_magneto_optical(name):
+    """MAGNETO_OPTICAL = TRUE: the Faraday-Voigt terms in the propagation
+    matrix (rh/stokesopac.c:56-86, rh/kurucz.c:711-715, 811-819), against the
+    reference (tests/golden/make_golden_mo.py).  Leaving them out misses by
+    more than 1e-2 of I_c."""
+    import pyoracle
+    from pyrh_amd import model
+    f = np.load(GOLDEN / "mo_fixtures.npz")
+    m = model.build_model(model.open_run(str(GOLDEN / "run_lte_mo")), f["wave"])
+    atm, mu = f[name + "__atmos"], float(f[name + "__mu"])
+    st = pyoracle.synth_lte(m, atm, mu=mu)
+    ref = f[name + "__stokes"]
+    Ic = ref[:, 0, :1]
+    assert float(np.max(np.abs(st[:, 0] - ref[:, 0]) / ref[:, 0])) < 1e-13
+    assert float(np.max(np.abs(st[:, 1:] - ref[:, 1:]) / Ic[:, None])) < 1e-13
+    m.scalars["magneto_optical"] = 0
+    off = pyoracle.synth_lte(m, atm, mu=mu)
+    assert float(np.max(np.abs(off - ref) / Ic[:, None])) > 1e-2
