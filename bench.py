@@ -53,16 +53,13 @@ RF_WORKLOAD = "lte_stokes_rf_1024"
 # central-difference steps of the finite-difference response functions, in
 # the atmosphere's units (K, km/s, G, rad, rad, km/s)
 RF_STEP = {"T": 1.0, "vlos": 0.01, "B": 1.0, "gamma": 0.01, "chi": 0.01, "vmic": 0.01}
-# NLTE leg: the Ca II-like five-level atom with continuum, ACTIVE, in the
+# NLTE leg: the Ca II-like five-level atom with continuum (rh/Atoms/CaII.atom
+# with its lines in complete redistribution, 101 points each), ACTIVE, in the
 # run directory tests/golden/run_nlte (5 Gauss-Legendre rays, Ng 2/10/3,
 # ITER_LIMIT 1e-3), on 4096 map columns
 NLTE_RUN = ROOT / "tests" / "golden" / "run_nlte"
 NLTE_WORKLOAD = "nlte_caii_4096"
 NLTE_WAVE = (393.4776, 396.9591, 854.44)
-# algorithmic FP64 operations per (column, wavelength, ray direction, depth)
-# of an iteration's formal solution with the operator: opacity of the active
-# set, the Bezier step, J and the rate terms (about 150 per point)
-NLTE_FLOP_PER_POINT = 150.0
 
 
 def _traffic_from_profile(ncol):
@@ -73,6 +70,21 @@ def _traffic_from_profile(ncol):
         return None
     m = re.search(r"hbm_bytes_per_launch:\s*([0-9.eE+]+)", f.read_text())
     return float(m.group(1)) if m else None
+
+
+def _host_cpu():
+    """The host CPU model and how many logical CPUs this process may use;
+    the baselines use at most 16 of them, the host share of one GPU."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{model}; {len(os.sched_getaffinity(0))} logical CPUs in the affinity mask"
 
 
 def _cpu_baseline(setup, wave, atm, nworkers):
@@ -91,7 +103,7 @@ def _cpu_baseline(setup, wave, atm, nworkers):
         with ThreadPoolExecutor(nworkers) as ex:
             list(ex.map(lambda a: refrun.run_synth(RUN_DIR, a, wave, variant="o1"), chunks))
         dt = time.perf_counter() - t0
-        return {"value": ns / dt, "unit": "columns/s", "cores": nworkers, "kind": "reference",
+        return {"value": ns / dt, "unit": "columns/s", "cores": nworkers, "cpu": _host_cpu(), "kind": "reference",
                 "sample": f"{ns} columns of the map, rhf1d() per column (input files re-read "
                           f"per call as pyrh does), {nworkers} processes, {dt:.1f} s wall"}
     import pyoracle
@@ -100,7 +112,7 @@ def _cpu_baseline(setup, wave, atm, nworkers):
     t0 = time.perf_counter()
     pyoracle.synth_lte(m, sample, nthreads=nworkers)
     dt = time.perf_counter() - t0
-    return {"value": ns / dt, "unit": "columns/s", "cores": nworkers, "kind": "port",
+    return {"value": ns / dt, "unit": "columns/s", "cores": nworkers, "cpu": _host_cpu(), "kind": "port",
             "sample": f"{ns} columns of the map, C restatement, OpenMP {nworkers} threads, "
                       f"{dt:.1f} s wall"}
 
@@ -253,13 +265,15 @@ def _nlte_leg(torch, dev, ncol, do_cpu, nworkers):
     cp, cn, _ = pyoracle.nlte_solve(nm, atm[:16], nthreads=nworkers)
     err = float(np.max(np.abs(pops - cp) / cp))
     # the dominant kernel, k_nlte_bezier (wavelengths with a line): its
-    # points are (column, wavelength) pairs x 2 directions x rays x depths
+    # operations per column and pass from the census of the active sets,
+    # times the columns of the passes without and with the operator
+    cen = R.nlte_bezier_flops(nm, nd)
+    flop_bez = cen["no_op"] * tm["cols_no_op"] + cen["op"] * tm["cols_op"]
     nset = np.diff(nm.as_ptr)
     nbez = int(np.sum((nset > 0) & (nm.as_bezier != 0)))
-    pairs_bez = tm["rt_pairs"] * nbez / int(np.sum(nset > 0))
-    pts = pairs_bez * 2 * nm.muz.size * nd
+    pts = (tm["cols_no_op"] + tm["cols_op"]) * nbez * 2 * nm.muz.size * nd
     rt_ms = tm["scatter_bezier_ms"] + tm["iter_bezier_ms"]
-    achieved = NLTE_FLOP_PER_POINT * pts / (rt_ms * 1e-3) / 1e12
+    achieved = flop_bez / (rt_ms * 1e-3) / 1e12
     hist = {int(k): int(v) for k, v in zip(*np.unique(niter, return_counts=True))}
     res = {
         "workload": NLTE_WORKLOAD if ncol == 4096 else f"nlte_caii_{ncol}",
@@ -272,7 +286,8 @@ def _nlte_leg(torch, dev, ncol, do_cpu, nworkers):
         "roofline": {"bound": "valu", "kernel": "k_nlte_bezier", "achieved": round(achieved, 3),
                      "peak": R.PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / R.PEAK_FP64_TFLOPS, 4), "traffic": None,
-                     "flop_per_point": NLTE_FLOP_PER_POINT,
+                     "flop_per_point": round(flop_bez / pts, 2),
+                     "flop_per_column_pass": {"no_op": cen["no_op"], "op": cen["op"]},
                      "avg_launch_ms": round(rt_ms / max(tm["rt_passes"], 1), 3)},
         "oracle_check": {"columns": 16, "max_rel_err_pops": float(f"{err:.3g}"),
                          "iterations_equal": bool(np.array_equal(niter[:16], cn))},
@@ -289,7 +304,7 @@ def _nlte_leg(torch, dev, ncol, do_cpu, nworkers):
                 list(ex.map(lambda x: refrun.run_synth(NLTE_RUN, x, wave, variant="o1"), chunks))
             dtc = time.perf_counter() - t0
             res["cpu_baseline"] = {
-                "value": round(ns / dtc, 3), "unit": "columns/s", "cores": nworkers,
+                "value": round(ns / dtc, 3), "unit": "columns/s", "cores": nworkers, "cpu": _host_cpu(),
                 "kind": "reference",
                 "sample": f"{ns} columns of the batch, rhf1d() per column to convergence plus its "
                           f"emergent spectrum, {nworkers} processes, {dtc:.1f} s wall"}
@@ -298,7 +313,7 @@ def _nlte_leg(torch, dev, ncol, do_cpu, nworkers):
             pyoracle.nlte_solve(nm, atm[:ns], nthreads=nworkers)
             dtc = time.perf_counter() - t0
             res["cpu_baseline"] = {
-                "value": round(ns / dtc, 3), "unit": "columns/s", "cores": nworkers, "kind": "port",
+                "value": round(ns / dtc, 3), "unit": "columns/s", "cores": nworkers, "cpu": _host_cpu(), "kind": "port",
                 "sample": f"{ns} columns, C restatement, OpenMP {nworkers} threads, {dtc:.1f} s wall"}
     del d_atm, d_pops
     eng.close()
@@ -386,7 +401,7 @@ def _hse_leg(torch, dev, ncol, do_cpu, nworkers, sample=128):
                             chunks))
             dtc = time.perf_counter() - t0
             res["cpu_baseline"] = {
-                "value": round(ns / dtc, 3), "unit": "columns/s", "cores": nworkers,
+                "value": round(ns / dtc, 3), "unit": "columns/s", "cores": nworkers, "cpu": _host_cpu(),
                 "kind": "reference",
                 "sample": f"{ns} columns of the batch, hse() per column, {nworkers} processes, "
                           f"{dtc:.1f} s wall"}
@@ -395,7 +410,7 @@ def _hse_leg(torch, dev, ncol, do_cpu, nworkers, sample=128):
             pyoracle.hse(hm, sc[:ns], T[:ns], 0.1, nthreads=nworkers)
             dtc = time.perf_counter() - t0
             res["cpu_baseline"] = {
-                "value": round(ns / dtc, 3), "unit": "columns/s", "cores": nworkers, "kind": "port",
+                "value": round(ns / dtc, 3), "unit": "columns/s", "cores": nworkers, "cpu": _host_cpu(), "kind": "port",
                 "sample": f"{ns} columns, C restatement, OpenMP {nworkers} threads, {dtc:.1f} s wall"}
     del outs, d
     eng.close()
@@ -509,7 +524,7 @@ def _mixed_leg(torch, dev, ncol, do_cpu, nworkers, rf_cols=256, oracle_cols=128)
             dtc = time.perf_counter() - t0
             res["cpu_baseline"] = {
                 "value": round(len(sample) / dtc, 3), "unit": "(column, mu) syntheses/s",
-                "cores": nworkers, "kind": "reference",
+                "cores": nworkers, "cpu": _host_cpu(), "kind": "reference",
                 "sample": f"{len(sample)} columns of the batch (depths {sorted(set(nd[:nworkers].tolist()))}) "
                           f"at mu = 1, rhf1d() per column with its input parsing, {nworkers} processes, "
                           f"{dtc:.1f} s wall"}

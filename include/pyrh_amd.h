@@ -227,8 +227,11 @@ typedef struct {
   const double *tr_alpha;   /* continuum cross-section per point [m^2]     */
   int32_t nline;
   const double *ln_lambda0, *ln_Aji, *ln_Bji, *ln_Bij, *ln_Grad;
-  const int32_t *ln_vdw;    /* 0 none, 1 Unsold                            */
-  const double *ln_vdw_cross;
+  const int32_t *ln_vdw;    /* 0 none, 1 Unsold, 2 Barklem                 */
+  const double *ln_vdw_cross;     /* Unsold: H+He cross-section; Barklem: the
+                                     ABO hydrogen cross-section             */
+  const double *ln_vdw_alpha;     /* Barklem: velocity exponent alpha       */
+  const double *ln_vdw_cross_he;  /* Barklem: Unsold helium cross-section   */
   const int32_t *ln_stark;  /* 0 none, 1 |c| ne, 2 quadratic (Traving)     */
   const double *ln_stark_c, *ln_stark_C, *ln_stark_Cm;
   const int32_t *as_ptr;    /* [nspect+1] active sets over the model grid  */
@@ -391,8 +394,10 @@ int rhg_nlte_solve(rhg_context_t *ctx, const rhg_nlte_t *atom, const rhg_columns
  * scattering passes, [2] the same in the iterations, [3] statistical
  * equilibrium, [4] Ng acceleration, [5] total, [6] formal-solution passes,
  * [7] (column, wavelength) pairs solved, [8] and [9] the continuum-only
- * wavelengths (Feautrier) in the scattering passes and the iterations. */
-#define RHG_NLTE_NTIMING 10
+ * wavelengths (Feautrier) in the scattering passes and the iterations,
+ * [10] and [11] the columns summed over the passes without and with the
+ * operator's rate terms (scattering and final J passes; iterations). */
+#define RHG_NLTE_NTIMING 12
 int rhg_nlte_timings(rhg_context_t *ctx, double *out);
 
 /* Gas pressure in hydrostatic equilibrium for a batch of temperature

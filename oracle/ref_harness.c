@@ -332,6 +332,7 @@ void getProfiles(void)
     for (kr = 0; kr < a->Nline; kr++) {
       rec("wphi", kr, 0, 0, a->line[kr].wphi, atmos.Nspace);
       rec("phi0", kr, 0, 0, a->line[kr].phi[0], atmos.Nspace);
+      rec("line_lambda", kr, 0, 0, a->line[kr].lambda, a->line[kr].Nlambda);
     }
   }
 }

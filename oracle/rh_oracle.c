@@ -2227,7 +2227,12 @@ static int nlte_column(const rhg_model_t *m, const rhg_nlte_t *na, const column_
     for (t = 0; t < na->ntrans && na->tr_line[t] != l; t++);
     for (k = 0; k < N; k++) {
       double Qelast = 0.0, G;
-      if (na->ln_vdw[l]) {
+      if (na->ln_vdw[l] == 2) {   /* rh/broad.c:125-136 */
+        G = na->ln_vdw_cross[l] * pow(c->T[k], (1.0 - na->ln_vdw_alpha[l]) / 2.0) +
+            na->ln_vdw_cross_he[l] * pow(c->T[k], 0.3);
+        G *= c->nH0[k];
+        Qelast += G;
+      } else if (na->ln_vdw[l]) { /* rh/broad.c:87-100 */
         G = na->ln_vdw_cross[l] * pow(c->T[k], 0.3);
         G *= c->nH0[k];
         Qelast += G;

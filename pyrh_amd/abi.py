@@ -123,6 +123,7 @@ NLTE_FIELDS = [
     ("nline", C.c_int32, "s"),
     ("ln_lambda0", P_D, "d"), ("ln_Aji", P_D, "d"), ("ln_Bji", P_D, "d"), ("ln_Bij", P_D, "d"),
     ("ln_Grad", P_D, "d"), ("ln_vdw", P_I, "i"), ("ln_vdw_cross", P_D, "d"),
+    ("ln_vdw_alpha", P_D, "d"), ("ln_vdw_cross_he", P_D, "d"),
     ("ln_stark", P_I, "i"), ("ln_stark_c", P_D, "d"), ("ln_stark_C", P_D, "d"),
     ("ln_stark_Cm", P_D, "d"),
     ("as_ptr", P_I, "i"), ("as_tr", P_I, "i"), ("as_bezier", P_I, "i"),

@@ -1168,7 +1168,8 @@ int nlte_upload(rhg_context_t *c, const rhg_nlte_t *a, const NlteHost &h) {
   ADD(tr_type, nt); ADD(tr_i, nt); ADD(tr_j, nt); ADD(tr_nblue, nt); ADD(tr_nlam, nt);
   ADD(tr_ptr, nt + 1); ADD(tr_line, nt); ADD(tr_wlam, h.nentry); ADD(tr_alpha, h.nentry);
   ADD(ln_lambda0, nln); ADD(ln_Aji, nln); ADD(ln_Bji, nln); ADD(ln_Bij, nln); ADD(ln_Grad, nln);
-  ADD(ln_vdw_cross, nln); ADD(ln_stark_c, nln); ADD(ln_stark_C, nln); ADD(ln_stark_Cm, nln);
+  ADD(ln_vdw_cross, nln); ADD(ln_vdw_alpha, nln); ADD(ln_vdw_cross_he, nln);
+  ADD(ln_stark_c, nln); ADD(ln_stark_C, nln); ADD(ln_stark_Cm, nln);
   ADD(ln_vdw, nln); ADD(ln_stark, nln);
   ADD(as_ptr, ns + 1); ADD(as_tr, a->as_ptr[ns]); ADD(as_bezier, ns);
   ADD(coll_type, a->ncoll); ADD(coll_i, a->ncoll); ADD(coll_j, a->ncoll); ADD(coll_ptr, a->ncoll + 1);
@@ -1392,6 +1393,7 @@ int rhg_nlte_solve(rhg_context_t *c, const rhg_nlte_t *a, const rhg_columns_t *c
         }
         c->nlte_ms[6] += 1.0;
         c->nlte_ms[7] += (double) nact * nlist;
+        c->nlte_ms[eval ? 11 : 10] += (double) nact;
         return 0;
       });
       if (r) return r;

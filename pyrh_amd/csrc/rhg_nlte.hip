@@ -126,7 +126,11 @@ __global__ void k_nlte_pre(const DevModel *__restrict__ Mp, const DevNlte *__res
     const double cDop = (NM_TO_M * A.ln_lambda0[l]) / (4.0 * PI);
     double Qelast = 0.0;
     if (A.ln_vdw[l]) {
-      double G = A.ln_vdw_cross[l] * pow(T, 0.3);
+      // rh/broad.c:87-100 (Unsold), 125-136 (Barklem), 145
+      double G = A.ln_vdw[l] == 2
+                     ? A.ln_vdw_cross[l] * pow(T, (1.0 - A.ln_vdw_alpha[l]) / 2.0) +
+                           A.ln_vdw_cross_he[l] * pow(T, 0.3)
+                     : A.ln_vdw_cross[l] * pow(T, 0.3);
       G *= nH0;
       Qelast += G;
     }

@@ -35,8 +35,8 @@ struct DevNlte {
   const int32_t *stage;
   const int32_t *tr_type, *tr_i, *tr_j, *tr_nblue, *tr_nlam, *tr_ptr, *tr_line;
   const double *tr_wlam, *tr_alpha;
-  const double *ln_lambda0, *ln_Aji, *ln_Bji, *ln_Bij, *ln_Grad, *ln_vdw_cross, *ln_stark_c,
-      *ln_stark_C, *ln_stark_Cm;
+  const double *ln_lambda0, *ln_Aji, *ln_Bji, *ln_Bij, *ln_Grad, *ln_vdw_cross, *ln_vdw_alpha,
+      *ln_vdw_cross_he, *ln_stark_c, *ln_stark_C, *ln_stark_Cm;
   const int32_t *ln_vdw, *ln_stark, *ln_tr, *ln_phi;
   const int32_t *as_ptr, *as_tr, *as_bezier;
   const int32_t *coll_type, *coll_i, *coll_j, *coll_ptr;

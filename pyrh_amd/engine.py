@@ -28,7 +28,8 @@ STAGES = ("state", "geometry", "continuum", "formal")
 RF_STAGES = ("synthesis_inputs", "state_tangents", "forward", "reverse", "chain")
 # rhg_nlte_timings slots (RHG_NLTE_NTIMING)
 NLTE_TIMINGS = ("setup_ms", "scatter_bezier_ms", "iter_bezier_ms", "stateq_ms", "ng_ms",
-                "total_ms", "rt_passes", "rt_pairs", "scatter_feautrier_ms", "iter_feautrier_ms")
+                "total_ms", "rt_passes", "rt_pairs", "scatter_feautrier_ms", "iter_feautrier_ms",
+                "cols_no_op", "cols_op")
 
 
 class EngineError(RuntimeError):

@@ -28,10 +28,78 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import constants as K
-from .model import RunSetup, build_model, gaunt_bf, linear, spline_coef, spline_eval
+from .model import (RunSetup, _abo_scale, build_model, cubeconvol, gaunt_bf, linear, locate,
+                    spline_coef, spline_eval)
+from .rhdata import BARKLEM_DELTA_NEFF, read_barklem
 
 TR_LINE, TR_CONT = 0, 1
-VDW_NONE, VDW_UNSOLD = 0, 1
+VDW_NONE, VDW_UNSOLD, VDW_BARKLEM = 0, 1, 2
+
+ORBITALS = "SPDFGHIJKLMNOQRTUV"
+
+
+def _determinate(label, g):
+    """determinate (rh/zeeman.c:37-84): (S, L, J) of a level from its label,
+    or None when the label shows no parity or the level is composite."""
+    p = len(label) - 1
+    while p > 0 and label[p] not in "EO":
+        p -= 1
+    if p <= 0:
+        return None
+    words = label[:p + 1].split()
+    last = words[-1][-3:]
+    if not last[0].isdigit():
+        return None
+    S = (int(last[0]) - 1) / 2.0
+    L = ORBITALS.index(last[1]) if last[1] in ORBITALS else -1
+    J = (g - 1.0) / 2.0
+    if J > L + S:
+        return None
+    return S, L, J
+
+
+def _barklem_active(setup, atom, ln):
+    """getBarklemactivecross (rh/barklem.c:202-313): the ABO cross-section
+    and velocity exponent of an active line from the Barklem tables, or None
+    (ionised lower level, undetermined quantum numbers, no table for the
+    orbital pair, or effective quantum numbers off the table): the
+    reference then warns and uses UNSOLD (rh/readatom.c:311-318)."""
+    i, j = ln.i, ln.j
+    if atom.stage[i] > 0:
+        return None
+    if atom.abo_level[i] != -1 and atom.abo_level[j] != -1:
+        Ll, Lu = atom.abo_level[i], atom.abo_level[j]
+    else:
+        di, dj = _determinate(atom.label[i], atom.g[i]), _determinate(atom.label[j], atom.g[j])
+        if di is None or dj is None:
+            return None
+        Ll, Lu = di[1], dj[1]
+    pair = {(0, 1): "SP", (1, 0): "SP", (1, 2): "PD", (2, 1): "PD", (2, 3): "DF", (3, 2): "DF"}
+    kind = pair.get((Ll, Lu))
+    if kind is None:
+        return None
+    bs = read_barklem(setup.barklem_paths[kind], kind)
+    Z = atom.stage[j] + 1
+    ic = j + 1
+    while atom.stage[ic] < atom.stage[j] + 1:
+        ic += 1
+    E_Ryd = K.E_RYDBERG / (1.0 + K.M_ELECTRON / (atom.weight * K.AMU))
+    neff1 = Z * math.sqrt(E_Ryd / (atom.E[ic] - atom.E[i]))
+    neff2 = Z * math.sqrt(E_Ryd / (atom.E[ic] - atom.E[j]))
+    if Ll > Lu:
+        neff1, neff2 = neff2, neff1
+    if neff1 < bs.neff1[0] or neff1 > bs.neff1[-1]:
+        return None
+    f1 = locate(bs.neff1, neff1)
+    f1 = f1 + (neff1 - bs.neff1[f1]) / BARKLEM_DELTA_NEFF
+    if neff2 < bs.neff2[0] or neff2 > bs.neff2[-1]:
+        return None
+    f2 = locate(bs.neff2, neff2)
+    f2 = f2 + (neff2 - bs.neff2[f2]) / BARKLEM_DELTA_NEFF
+    N1, N2 = bs.cross.shape
+    cross = cubeconvol(N2, N1, bs.cross, f2, f1)
+    alpha = cubeconvol(N2, N1, bs.alpha, f2, f1)
+    return _abo_scale(cross, alpha, atom.weight, setup.atoms[0].weight), alpha
 STARK_NONE, STARK_ABS, STARK_QUAD = 0, 1, 2
 COLL_TYPES = {"OMEGA": 0, "CE": 1, "CI": 2, "CP": 3, "CH": 4, "CH0": 5, "CH+": 6}
 # the active set at one wavelength holds at most this many transitions
@@ -100,6 +168,49 @@ def _hunt(grid, v):
     return i
 
 
+def rh_hunt(arr, value, ilow):
+    """Hunt (rh/hunt.c:17-80) on an ascending array from the guess ilow:
+    the index j with arr[j] <= value < arr[j + 1] as that routine finds it.
+    Where the search hunts down onto a value equal to a grid point, the
+    point becomes the upper bracket and the result is one index short; the
+    reference's transition spans (rh/sortlambda.c:306-311, 380-385) carry
+    that quirk, so the host model keeps it."""
+    n = len(arr)
+    if ilow <= 0 or ilow > n - 1:
+        ilow, ihigh = 0, n
+    else:
+        inc = 1
+        if value >= arr[ilow]:
+            if ilow == n - 1:
+                return ilow
+            ihigh = ilow + inc
+            while value >= arr[ihigh]:
+                ilow = ihigh
+                inc += inc
+                ihigh = ilow + inc
+                if ihigh >= n:
+                    ihigh = n
+                    break
+        else:
+            ihigh = ilow
+            if ilow == 0:
+                return ilow
+            while value <= arr[ilow]:
+                ihigh = ilow
+                inc += inc
+                ilow = ihigh - inc
+                if ilow <= 0:
+                    ilow = 0
+                    break
+    while ihigh - ilow > 1:
+        mid = (ihigh + ilow) >> 1
+        if value >= arr[mid]:
+            ilow = mid
+        else:
+            ihigh = mid
+    return ilow
+
+
 @dataclass
 class NLTEModel:
     lte: object                 # background model on the merged grid
@@ -130,6 +241,8 @@ class NLTEModel:
     ln_Grad: np.ndarray
     ln_vdw: np.ndarray
     ln_vdw_cross: np.ndarray
+    ln_vdw_alpha: np.ndarray
+    ln_vdw_cross_he: np.ndarray
     ln_stark: np.ndarray
     ln_stark_c: np.ndarray      # |cStark| (ABS) or cStark23 (QUAD)
     ln_stark_C: np.ndarray
@@ -253,10 +366,17 @@ def _collisions(atom):
 def _broadening(setup, atom, ln):
     """Constants of rh/broad.c:34-316 for one active line."""
     j, i = ln.j, ln.i
-    vdw, cross = VDW_NONE, 0.0
-    if ln.cvdWaals[0] > 0.0 or ln.cvdWaals[2] > 0.0:
-        if "UNSOLD" not in ln.vdw:
-            raise NotImplementedError(f"van der Waals type {ln.vdw} for an active line")
+    vdw, cross, alpha, cross_he = VDW_NONE, 0.0, 0.0, 0.0
+    cvdw = list(ln.cvdWaals)
+    barklem = None
+    if "BARKLEM" in ln.vdw:
+        barklem = _barklem_active(setup, atom, ln)
+        if barklem is not None:
+            # rh/barklem.c:304-311: He by UNSOLD with cvdWaals[2] = 1
+            cvdw = [barklem[0], barklem[1], 1.0, 0.0]
+    elif "UNSOLD" not in ln.vdw:
+        raise NotImplementedError(f"van der Waals type {ln.vdw} for an active line")
+    if cvdw[0] > 0.0 or cvdw[2] > 0.0:
         He_w, He_ab = float(setup.elements.weight[1]), float(setup.elements.abund[1])
         H_w = setup.atoms[0].weight
         fourPIeps0 = 4.0 * K.PI * K.EPSILON_0
@@ -272,9 +392,16 @@ def _broadening(setup, atom, ln):
                         * 2 * K.PI * ((Z * K.RBOHR) * (Z * K.RBOHR)) / K.HPLANCK * deltaR, 0.4)
         vrel35_H = math.pow(8.0 * K.KBOLTZMANN / (K.PI * K.AMU * atom.weight)
                             * (1.0 + atom.weight / H_w), 0.3)
-        # UNSOLD sets cvdWaals[1] = cvdWaals[3] = 0 (rh/readatom.c:318-320)
-        cross = 8.08 * (ln.cvdWaals[0] * vrel35_H + ln.cvdWaals[2] * He_ab * vrel35_He) * C625
-        vdw = VDW_UNSOLD
+        if barklem is None:
+            # UNSOLD (also the fallback of BARKLEM) sets cvdWaals[1] =
+            # cvdWaals[3] = 0 (rh/readatom.c:311-320); rh/broad.c:87-100
+            cross = 8.08 * (cvdw[0] * vrel35_H + cvdw[2] * He_ab * vrel35_He) * C625
+            vdw = VDW_UNSOLD
+        else:
+            # rh/broad.c:125-136: cross_H T^((1 - alpha)/2) + cross_He T^0.3
+            cross, alpha = cvdw[0], cvdw[1]
+            cross_he = 8.08 * cvdw[2] * He_ab * vrel35_He * C625
+            vdw = VDW_BARKLEM
     stark, c0, C, Cm = STARK_NONE, 0.0, 0.0, 0.0
     if ln.cStark != 0.0:
         if ln.cStark < 0.0:
@@ -299,7 +426,7 @@ def _broadening(setup, atom, ln):
                   * ((neff_u * (5.0 * (neff_u * neff_u) + 1.0)) ** 2
                      - (neff_l * (5.0 * (neff_l * neff_l) + 1.0)) ** 2))
             c0 = 11.37 * math.pow(ln.cStark * C4, 0.66666667)
-    return vdw, cross, stark, c0, C, Cm
+    return vdw, cross, stark, c0, C, Cm, alpha, cross_he
 
 
 def build_nlte(setup: RunSetup, wave, fudge=None) -> NLTEModel:
@@ -359,8 +486,11 @@ def build_nlte(setup: RunSetup, wave, fudge=None) -> NLTEModel:
     # --- transitions (continua first, in file order, then lines)
     tr_type, tr_i, tr_j, tr_nb, tr_nl, tr_line = [], [], [], [], [], []
     wl_all, al_all, ptr = [], [], [0]
+    # spans as SortLambda finds them: Nblue from a zero guess, Nred hunted
+    # from the previous transition's Nred (rh/sortlambda.c:296-311, 376-385)
+    nr = 0
     for c, gcur in zip(atom.conts, cgrids):
-        nb, nr = _hunt(lam, gcur[0]), _hunt(lam, gcur[-1])
+        nb, nr = rh_hunt(lam, gcur[0], 0), rh_hunt(lam, gcur[-1], nr)
         sub = lam[nb:nr + 1]
         if c.hydrogenic:
             Z = int(atom.stage[c.j])
@@ -385,7 +515,7 @@ def build_nlte(setup: RunSetup, wave, fudge=None) -> NLTEModel:
         tr_type.append(TR_CONT); tr_i.append(c.i); tr_j.append(c.j)
         tr_nb.append(nb); tr_nl.append(n); tr_line.append(-1)
     for kr, (ln, gcur) in enumerate(zip(atom.lines, lgrids)):
-        nb, nr = _hunt(lam, gcur[0]), _hunt(lam, gcur[-1])
+        nb, nr = rh_hunt(lam, gcur[0], 0), rh_hunt(lam, gcur[-1], nr)
         sub = lam[nb:nr + 1]
         n = sub.size
         dop = K.CLIGHT / ln.lambda0          # asymmetric profile: no factor 2
@@ -453,6 +583,8 @@ def build_nlte(setup: RunSetup, wave, fudge=None) -> NLTEModel:
         ln_Grad=np.array([ln.Grad for ln in atom.lines]),
         ln_vdw=np.array([b[0] for b in br], np.int32),
         ln_vdw_cross=np.array([b[1] for b in br]),
+        ln_vdw_alpha=np.array([b[6] for b in br]),
+        ln_vdw_cross_he=np.array([b[7] for b in br]),
         ln_stark=np.array([b[2] for b in br], np.int32),
         ln_stark_c=np.array([b[3] for b in br]),
         ln_stark_C=np.array([b[4] for b in br]),

@@ -100,3 +100,58 @@ def hse_flops(hm, layer_iters, newton_per_iter, chem_per_iter):
     per_iter = newton_per_iter * newton + chem_per_iter * chem + layer
     return float(layer_iters) * per_iter, {"newton_step": newton, "chem_iteration": chem,
                                            "layer_fixed": layer, "per_layer_iteration": per_iter}
+
+
+# --- k_nlte_bezier (rh/rhf1d/bezier_1D.c:306-541 with the operator,
+# rh/fillgamma.c:82-332): FP64 add/sub/mul/div per unit, read off the
+# kernel's arithmetic, exp counted apart.  Per ray and depth:
+NB_STEP = 98          # the Bezier step: 2 Fritsch-Butland derivatives (16 each),
+                      # 4 clamped control points (3 each), dtau (5), the weights of
+                      # the dt >= 0.05 branch (37 + exp), I (9), Psi and steps (3)
+NB_OPAC = 5           # chi + chi_c, eta + eta_c + sigma J, S = eta / chi
+NB_OPAC_LINE = 9      # V = cV phi; chi += V (n_i - g n_j); eta += tw g V n_j
+NB_OPAC_CONT = 8      # the same with V = alpha
+NB_DEPTH_LINE = 2     # per member and depth (shared by the rays of a block):
+NB_DEPTH_CONT = 6     #   w_la, and for continua g_ij (one exp)
+NB_EMIT = 3           # J += w_mu I per ray, and dJ once per depth
+NB_RATE = 12          # per ray with the operator: Psi/chi, I_eff, w_la mu, Gamma_ji, Gamma_ij
+NB_RATE_MEMBER = 12   # per member and ray: eta, chi_cc, U, w_la mu and the two rates
+NB_RATE_CROSS = 4     # per coupling term (addtoCoupling)
+
+
+def nlte_bezier_flops(nm, nd, rb=2):
+    """FP64 operations and exps of one pass of k_nlte_bezier over one column
+    (every wavelength with an active line, both directions, every ray, nd
+    depths), without and with the operator's rate terms.  The set
+    composition of each wavelength (members, line or continuum, coupling
+    masks) is taken from the atom's active sets as the kernel builds it."""
+    nrays = int(nm.muz.size)
+    nblocks = -(-nrays // rb)
+    out = {"no_op": 0, "op": 0, "exp_no_op": 0, "exp_op": 0}
+    for ls in range(nm.as_ptr.size - 1):
+        p0, p1 = int(nm.as_ptr[ls]), int(nm.as_ptr[ls + 1])
+        if p1 == p0 or not nm.as_bezier[ls]:
+            continue
+        trs = [int(t) for t in nm.as_tr[p0:p1]]
+        lines = [t for t in trs if nm.tr_type[t] == 0]
+        conts = [t for t in trs if nm.tr_type[t] != 0]
+        m = len(trs)
+        opac = nrays * (NB_OPAC + NB_OPAC_LINE * len(lines) + NB_OPAC_CONT * len(conts))
+        depth = nblocks * (NB_DEPTH_LINE * len(lines) + NB_DEPTH_CONT * len(conts))
+        per_depth = nrays * (NB_STEP + NB_EMIT) + opac + depth
+        exps = nrays + nblocks * len(conts)
+        # coupling sums: members sharing a lower level, upper level, and
+        # upper-lower crossings (set_info's masks)
+        ti = [int(nm.tr_i[t]) for t in trs]
+        tj = [int(nm.tr_j[t]) for t in trs]
+        adds = sum(1 for a in range(m) for b in range(m) if ti[b] == ti[a]) \
+            + 2 * sum(1 for a in range(m) for b in range(m) if tj[b] == tj[a]) \
+            + sum(1 for a in range(m) for b in range(m) if tj[b] == ti[a])
+        cross = sum(1 for a in range(m) for b in range(m) if tj[b] == ti[a])
+        rate = nrays * (NB_RATE + NB_RATE_MEMBER * m + adds + NB_RATE_CROSS * cross) \
+            + depth + 2 * m
+        out["no_op"] += 2 * nd * per_depth
+        out["op"] += 2 * nd * (per_depth + rate)
+        out["exp_no_op"] += 2 * nd * exps
+        out["exp_op"] += 2 * nd * (exps + nblocks * len(conts))
+    return out

@@ -7,9 +7,14 @@ iteration to a maximum relative population change of 1e-3) and stores the
 inputs, the converged populations of the active atom, the iteration count
 and last change, the merged wavelength grid, the iteration history and the
 emergent spectrum that rhf1d() returns afterwards at the case's mu.
+A second set, tests/golden/nlte_mg_fixtures.npz, solves the neutral Mg I
+atom of rh/Atoms/MgI_6level.atom (run directory tests/golden/run_nlte_mg):
+its Mg b lines take Barklem van der Waals broadening from the ABO tables
+(rh/barklem.c:202-313, rh/broad.c:125-136), which the ionised Ca II lines
+cannot (the reference falls back to UNSOLD for them).
 Needs /root/reference; run from the repository root:
 
-    PYRH_PATH=tests/golden/rhdata python3 tests/golden/make_golden_nlte.py
+    PYRH_PATH=tests/golden/rhdata python3 tests/golden/make_golden_nlte.py [caii] [mg]
 """
 from __future__ import annotations
 
@@ -27,6 +32,8 @@ from pyrh_amd import workloads as W  # noqa: E402
 
 RUN = HERE / "run_nlte"
 WAVE = np.array([393.4776, 396.9591, 854.44])
+RUN_MG = HERE / "run_nlte_mg"
+WAVE_MG = np.array([285.3, 517.3, 518.5])
 
 
 def cases():
@@ -42,10 +49,16 @@ def cases():
     yield "mu07", mp[2:4].copy(), 0.7
 
 
-def main():
+def cases_mg():
+    mp = W.stokes_map(3, seed=W.SEED_BASE + 7)
+    yield "map3", mp, 1.0
+    yield "mu07", mp[1:2].copy(), 0.7
+
+
+def solve(run, wave, case_list, fname):
     out = {}
-    for name, atm, mu in cases():
-        res = refrun.run_synth(RUN, atm, WAVE, mu=mu, atm_scale=0, variant="o2",
+    for name, atm, mu in case_list:
+        res = refrun.run_synth(run, atm, wave, mu=mu, atm_scale=0, variant="o2",
                                capture=True, timeout=3600)
         cap = res[-1]
         ncol = atm.shape[0]
@@ -67,6 +80,12 @@ def main():
             dpops[c] = its[-1][1]
             hist[c, :len(its)] = [x[1] for x in its]
         lam = [r[5] for r in cap if r[0] == "lambda"][0]
+        # each line's span on the merged grid as SortLambda left it
+        spans = {}
+        for r in cap:
+            if r[0] == "line_lambda" and r[1] == 0:
+                spans.setdefault(r[2], (int(np.searchsorted(lam, r[5][0])), r[5].size))
+        out[f"{name}__line_span"] = np.array([spans[k] for k in sorted(spans)], np.int32)
         out[f"{name}__atmos"] = atm
         out[f"{name}__pops"] = pops
         out[f"{name}__nstar"] = nstar
@@ -78,8 +97,16 @@ def main():
         out[f"{name}__stokes"] = np.stack(res[1:5], axis=1)     # [ncol, 4, nlw]
         out[f"{name}__lamout"] = res[0]
         print(name, atm.shape, "iterations", niter.tolist())
-    out["wave"] = WAVE
-    np.savez_compressed(HERE / "nlte_fixtures.npz", **out)
+    out["wave"] = wave
+    np.savez_compressed(HERE / fname, **out)
+
+
+def main():
+    which = sys.argv[1:] or ["caii", "mg"]
+    if "caii" in which:
+        solve(RUN, WAVE, cases(), "nlte_fixtures.npz")
+    if "mg" in which:
+        solve(RUN_MG, WAVE_MG, cases_mg(), "nlte_mg_fixtures.npz")
 
 
 if __name__ == "__main__":

@@ -113,3 +113,87 @@ def test_quadrature_weights_sum_to_span(nm):
         if nm.tr_type[t] == 0:
             width *= 2.99792458e8 / nm.ln_lambda0[nm.tr_line[t]]
         assert abs(w.sum() - width) < 1e-12 * width
+
+
+@pytest.fixture(scope="module")
+def mg_fix():
+    return np.load(GOLDEN / "nlte_mg_fixtures.npz")
+
+
+@pytest.fixture(scope="module")
+def nm_mg(mg_fix):
+    from pyrh_amd import model, nlte
+    return nlte.build_nlte(model.open_run(str(GOLDEN / "run_nlte_mg")), mg_fix["wave"])
+
+
+def test_barklem_constants_of_active_lines(nm_mg):
+    """The Mg b lines of the neutral Mg I atom take Barklem broadening (3s3p
+    3P - 3s4s 3S is an S-P pair inside the ABO table); the 457 nm
+    intercombination line keeps UNSOLD.  The velocity exponent alpha of the
+    table lies in (0, 1) and the helium term is the UNSOLD cross-section
+    with cvdWaals[2] = 1 (rh/barklem.c:304-311)."""
+    b = [l for l in range(nm_mg.ln_vdw.size) if nm_mg.ln_vdw[l] == 2]
+    assert len(b) == 3 and nm_mg.ln_vdw[0] == 1
+    assert np.all((nm_mg.ln_vdw_alpha[b] > 0.0) & (nm_mg.ln_vdw_alpha[b] < 1.0))
+    assert np.all(nm_mg.ln_vdw_cross_he[b] > 0.0)
+    assert nm_mg.ln_vdw_alpha[0] == 0.0 and nm_mg.ln_vdw_cross_he[0] == 0.0
+
+
+@pytest.mark.parametrize("name", ("map3", "mu07"))
+def test_oracle_populations_match_reference_barklem(nm_mg, mg_fix, name):
+    """Mg I with Barklem-broadened active lines: the oracle's converged
+    populations and emergent intensity against the reference's."""
+    np.testing.assert_array_equal(nm_mg.lam, mg_fix[name + "__lam"])
+    pops, niter, dpops, st = pyoracle.nlte_solve(nm_mg, mg_fix[name + "__atmos"],
+                                                 mu=float(mg_fix[name + "__mu"]), stokes=True)
+    ref = mg_fix[name + "__pops"]
+    np.testing.assert_array_equal(niter, mg_fix[name + "__niter"])
+    np.testing.assert_allclose(dpops, mg_fix[name + "__dpops"], rtol=1e-6, atol=0)
+    assert np.max(np.abs(pops - ref) / ref) <= 1e-8
+    sref = mg_fix[name + "__stokes"]
+    assert np.max(np.abs(st[:, 0] - sref[:, 0]) / sref[:, 0]) < 1e-8
+
+
+def test_barklem_term_changes_the_populations(nm_mg, mg_fix):
+    """Planted mistake: the Mg b lines with UNSOLD damping (the Barklem
+    term dropped) miss the reference's populations; pins that the Barklem
+    branch is what matches."""
+    import copy
+    bad = copy.copy(nm_mg)
+    bad.ln_vdw = np.where(nm_mg.ln_vdw == 2, 1, nm_mg.ln_vdw).astype(np.int32)
+    pops = pyoracle.nlte_solve(bad, mg_fix["map3__atmos"][:1])[0]
+    ref = mg_fix["map3__pops"][:1]
+    assert np.max(np.abs(pops - ref) / ref) > 1e-6
+
+
+@pytest.mark.parametrize("which", ("caii", "mg"))
+def test_line_spans_match_reference(which, nm, nm_mg, nlte_fix, mg_fix):
+    """Every active line's first grid index and point count on the merged
+    grid as the reference's SortLambda left them (its Hunt, hunting down
+    onto an exact grid value, stops one point short: the Mg I 457 nm line
+    spans 48 of its 49 points)."""
+    m, f, case = (nm, nlte_fix, "map6") if which == "caii" else (nm_mg, mg_fix, "map3")
+    span = f[case + "__line_span"]
+    lines = [t for t in range(m.tr_type.size) if m.tr_line[t] >= 0]
+    got = np.array([(m.tr_nblue[t], m.tr_nlam[t]) for t in lines], np.int32)
+    np.testing.assert_array_equal(got, span)
+
+
+def test_rh_hunt_bracket_and_quirk():
+    """Hunt from a zero guess brackets like a bisection (exact matches give
+    their own index), and so does hunting up; hunting down gives one less
+    only where a doubling step lands on the exact match (rh/hunt.c:44-53:
+    that point becomes the upper bracket)."""
+    from pyrh_amd.nlte import rh_hunt
+    g = np.arange(40, dtype=float) * 0.5
+    for x in (0.0, 0.2, 3.5, 7.25, 19.5):
+        exact = int(np.searchsorted(g, x, side="right")) - 1
+        assert rh_hunt(g, x, 0) == exact
+        if exact >= 1:
+            assert rh_hunt(g, x, 1) == exact
+    # from guess 16 the first step down lands on g[14] = 7.0
+    assert rh_hunt(g, 7.0, 16) == 13
+    # from guess 30 the steps (28, 24, 16, 0) pass it by: exact again
+    assert rh_hunt(g, 7.0, 30) == 14
+    # a value between grid points: the ordinary bracket from any guess
+    assert rh_hunt(g, 7.2, 16) == 14 and rh_hunt(g, 7.2, 30) == 14
