@@ -306,6 +306,23 @@ enum rhg_precision { RHG_PREC_FP64 = 0, RHG_PREC_FP32 = 1 };
 int rhg_synth_lte(rhg_context_t *ctx, const rhg_columns_t *cols,
                   int32_t precision, int32_t device_ptrs, double *stokes_out);
 
+/* rhg_synth_lte without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  cols->atmos,
+ * cols->ndep and stokes_out are device pointers; the work buffers come from
+ * a fixed device array of RHG_NOALLOC_SCRATCH_BYTES per GPU, held from the
+ * library's load until the process ends (its address is looked up by
+ * rhg_open).  A batch whose work space (column state, continuum, flags)
+ * exceeds it is rejected with RHG_EINVAL and a message giving the largest
+ * batch of that shape (about 1000 columns of 82 depths and 112 wavelengths
+ * in the FP32 form).  Columns run in the caller's order (no sorting by depth
+ * count; the results are the same).  Depth counts outside [4, ndep_max]
+ * return RHG_EINVAL.  Synchronous.  Calls that overlap on one GPU (two
+ * contexts, two host threads) share the array and race; the caller
+ * serialises them. */
+#define RHG_NOALLOC_SCRATCH_BYTES (128u << 20)
+int rhg_synth_lte_noalloc(rhg_context_t *ctx, const rhg_columns_t *cols,
+                          int32_t precision, double *stokes_out);
 
 /* Response functions of the emergent Stokes vector: dS(lambda)/dp(k) for
  * the atmosphere parameters below at every depth k, for the wavelengths

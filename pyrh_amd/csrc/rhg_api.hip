@@ -5,7 +5,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <algorithm>
+#include <atomic>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -47,6 +49,36 @@ struct DevBuf {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// The synthesis without allocation (rhg_synth_lte_noalloc).  Its work
+// buffers (sanitised depth counts, an error flag, moving flags, column state,
+// continuum) live in `scratch`, a file-scope device array of
+// RHG_NOALLOC_SCRATCH_BYTES that each GPU holds from the library's load until
+// the process ends.  Host code never names the array in a kernel argument:
+// rhg_open looks up its device address on each device it opens
+// (noalloc_scratch_init) and every call uses that pointer.  Calls that
+// overlap on one GPU share the array and race.
+static __device__ __align__(256) unsigned char scratch[RHG_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::atomic<void *> g_scratch_ptr[kMaxDevices];
+std::mutex g_scratch_mutex;
+
+// with `device` current; contexts opened from several threads look it up once
+int noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return fail(RHG_EUNSUPPORTED, "device index too large");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[device].load()) return 0;
+  void *p = nullptr;
+  HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) return fail(RHG_EDEVICE, "no device address for the no-allocation scratch");
+  g_scratch_ptr[device].store(p);
+  return 0;
+}
+
+}  // namespace
 
 struct rhg_context {
   int device = 0;
@@ -345,6 +377,11 @@ int rhg_open(const rhg_model_t *m, int device, rhg_context_t **out) {
     rhg_close(c);
     return fail(RHG_EDEVICE, "hipMemcpy (wavelength index)");
   }
+  if (int e = noalloc_scratch_init(device)) {
+    std::string msg = g_err;
+    rhg_close(c);
+    return fail(e, msg);
+  }
   *out = c;
   g_err.clear();
   return 0;
@@ -628,6 +665,92 @@ int rhg_synth_lte(rhg_context_t *ctx, const rhg_columns_t *cols, int32_t precisi
 int rhg_synth_lte_diag(rhg_context_t *ctx, const rhg_columns_t *cols, int32_t precision,
                        double *stokes_out, double *height_out, double *chiref_out) {
   return synth(ctx, cols, precision, 0, stokes_out, height_out, chiref_out);
+}
+
+}  // extern "C"
+
+namespace {
+
+size_t align256(size_t n) { return (n + 255) & ~(size_t) 255; }
+
+}  // namespace
+
+namespace rhg {
+
+// flag[0] = 0 (the error flag must start at zero on every call)
+__global__ void k_clear_flag(int32_t *flag) {
+  if (threadIdx.x == 0) flag[0] = 0;
+}
+
+// out[i] = ndep[i] clamped to [4, Nmax]; flag[0] = 1 if any was outside
+__global__ void k_check_ndep(const int32_t *__restrict__ ndep, int ncol, int Nmax,
+                             int32_t *__restrict__ out, int32_t *__restrict__ flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ncol) return;
+  const int n = ndep[i];
+  const bool bad = n < 4 || n > Nmax;
+  out[i] = bad ? (n < 4 ? 4 : Nmax) : n;
+  if (bad) atomicOr(flag, 1);
+}
+
+}  // namespace rhg
+
+extern "C" {
+
+int rhg_synth_lte_noalloc(rhg_context_t *ctx, const rhg_columns_t *cols, int32_t precision,
+                          double *stokes_out) {
+  if (!ctx || !cols || !cols->atmos || !cols->ndep || !stokes_out)
+    return fail(RHG_EINVAL, "null argument");
+  if (ctx->nout < 1) return fail(RHG_EINVAL, "the model has no output wavelength");
+  if (precision != RHG_PREC_FP64 && precision != RHG_PREC_FP32)
+    return fail(RHG_EINVAL, "precision must be RHG_PREC_FP64 or RHG_PREC_FP32");
+  const int ncol = cols->ncol, Nmax = cols->ndep_max, nout = ctx->nout;
+  if (ncol < 1 || Nmax < 4 || Nmax > RHG_MAX_DEPTH) return fail(RHG_EINVAL, "bad batch shape");
+  if (!(cols->mu > 0.0 && cols->mu <= 1.0)) return fail(RHG_EINVAL, "mu must be in (0, 1]");
+  if (cols->atm_scale < 0 || cols->atm_scale > 2) return fail(RHG_EINVAL, "bad atm_scale");
+  const int dev = ctx->device;
+  if (dev < 0 || dev >= kMaxDevices || !g_scratch_ptr[dev].load())
+    return fail(RHG_EINVAL, "no-allocation scratch not set up on this device");
+  const int nfield = rhg::state_nfield(ctx->ncont, ctx->nline);
+  const size_t rsz = precision == RHG_PREC_FP32 ? sizeof(float) : sizeof(double);
+  const size_t npts = (size_t) ncol * Nmax;
+  const size_t b_ndep = align256((size_t) ncol * sizeof(int32_t)), b_flag = 256,
+               b_mov = align256((size_t) ncol * sizeof(int32_t)),
+               b_state = align256(npts * nfield * sizeof(double)),
+               b_cont = align256(npts * nout * rsz);
+  const size_t need = b_ndep + b_flag + b_mov + b_state + 2 * b_cont;
+  if (need > (size_t) RHG_NOALLOC_SCRATCH_BYTES) {
+    const size_t per_col = need / (size_t) ncol + 1;
+    return fail(RHG_EINVAL, "batch needs " + std::to_string(need >> 20) + " MiB of work space, "
+                                "beyond the no-allocation scratch of " +
+                                std::to_string(RHG_NOALLOC_SCRATCH_BYTES >> 20) +
+                                " MiB: at most about " +
+                                std::to_string((size_t) RHG_NOALLOC_SCRATCH_BYTES / per_col) +
+                                " columns of this shape per call");
+  }
+  HIPCHECK(hipSetDevice(dev));
+  unsigned char *base = (unsigned char *) g_scratch_ptr[dev].load();
+  int32_t *sndep = (int32_t *) base;
+  int32_t *flag = (int32_t *) (base + b_ndep);
+  int32_t *moving = (int32_t *) (base + b_ndep + b_flag);
+  double *state = (double *) (base + b_ndep + b_flag + b_mov);
+  void *cchi = base + b_ndep + b_flag + b_mov + b_state;
+  void *ceta = (unsigned char *) cchi + b_cont;
+  hipLaunchKernelGGL(rhg::k_clear_flag, dim3(1), dim3(64), 0, ctx->stream, flag);
+  HIPCHECK(hipGetLastError());
+  hipLaunchKernelGGL(rhg::k_check_ndep, dim3((unsigned) ((ncol + 255) / 256)), dim3(256), 0,
+                     ctx->stream, cols->ndep, ncol, Nmax, sndep, flag);
+  HIPCHECK(hipGetLastError());
+  if (int rc = launch_synth(ctx, (const double *) cols->atmos, sndep, ncol, Nmax, cols->mu,
+                            cols->atm_scale, precision, nfield, state, moving, nullptr, cchi,
+                            ceta, stokes_out))
+    return rc;
+  int32_t hflag = 0;
+  HIPCHECK(hipMemcpyAsync(&hflag, flag, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHECK(hipStreamSynchronize(ctx->stream));
+  if (hflag) return fail(RHG_EINVAL, "ndep out of range");
+  g_err.clear();
+  return 0;
 }
 
 int rhg_count_work(rhg_context_t *c, int64_t *counts_out) {

@@ -66,6 +66,7 @@ def lib():
         L.rhg_close.argtypes = [C.c_void_p]
         PD = C.POINTER(C.c_double)
         L.rhg_synth_lte.argtypes = [C.c_void_p, C.POINTER(RhgColumns), C.c_int32, C.c_int32, PD]
+        L.rhg_synth_lte_noalloc.argtypes = [C.c_void_p, C.POINTER(RhgColumns), C.c_int32, PD]
         L.rhg_synth_lte_diag.argtypes = [C.c_void_p, C.POINTER(RhgColumns), C.c_int32, PD, PD, PD]
         L.rhg_last_timings.argtypes = [C.c_void_p, PD]
         L.rhg_rf_timings.argtypes = [C.c_void_p, PD]
