@@ -72,6 +72,70 @@ def _traffic_from_profile(ncol):
     return float(m.group(1)) if m else None
 
 
+def _profile_kernels():
+    """One launch of each kernel of the bench at its workload's shape, inputs
+    in HBM, then one 1 GiB device-to-device copy (the counter calibration):
+    the FP32-form and plain FP64 syntheses of the 16384-column map, the
+    FP32-form response functions of the 1024-column batch (all six
+    parameters), an NLTE solve of 512 columns of the Ca II-like atom, and
+    hse() on 16384 columns."""
+    import torch
+    from pyrh_amd import hse as HS, model as M, nlte as NL, workloads as W
+    from pyrh_amd.engine import Engine
+    dev = torch.device("cuda:0")
+    setup = M.open_run(str(RUN_DIR))
+    wave = W.wavelengths_630()
+    eng = Engine(M.build_model(setup, wave))
+    ncol = 16384
+    atm = torch.from_numpy(W.stokes_map(ncol, seed=W.SEED_BASE + 1)).to(dev)
+    nd = atm.shape[2]
+    ndep = torch.full((ncol,), nd, dtype=torch.int32, device=dev)
+    out = torch.empty((ncol, 4, eng.nout), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    for prec in (1, 0):
+        eng.synth_lte_device(atm.data_ptr(), ndep.data_ptr(), ncol, nd, out.data_ptr(),
+                             precision=prec)
+    print(f"profile: synthesis FP32 and FP64, {ncol} columns", flush=True)
+    nrf = 1024
+    rf = torch.empty((6, nd, 4, eng.nout, nrf), dtype=torch.float32, device=dev)
+    eng.rf_lte_device(atm.data_ptr(), ndep.data_ptr(), nrf, nd, out.data_ptr(), rf.data_ptr(),
+                      precision=1)
+    print(f"profile: response functions FP32, {nrf} columns", flush=True)
+    del rf
+    eng.close()
+    nm = NL.build_nlte(M.open_run(str(NLTE_RUN)), np.array(NLTE_WAVE))
+    en = Engine(nm.lte)
+    nn = 512
+    natm = torch.from_numpy(W.stokes_map(nn, seed=W.SEED_BASE + 5)).to(dev)
+    nnd = torch.full((nn,), nd, dtype=torch.int32, device=dev)
+    pops = torch.empty((nn, nm.nlevel, nd), dtype=torch.float64, device=dev)
+    nit = torch.empty(nn, dtype=torch.int32, device=dev)
+    dp = torch.empty(nn, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    en.nlte_solve_device(nm, natm.data_ptr(), nnd.data_ptr(), nn, nd, pops.data_ptr(),
+                         nit.data_ptr(), dp.data_ptr())
+    print(f"profile: NLTE, {nn} columns", flush=True)
+    en.close()
+    hm = HS.build_hse(M.open_run(str(RUN_DIR)))
+    eh = Engine(hm.model)
+    T = torch.from_numpy(np.ascontiguousarray(W.stokes_map(ncol, seed=W.SEED_BASE + 44)[:, 1]))
+    sc = torch.from_numpy(np.ascontiguousarray(np.broadcast_to(W.LOGTAU, tuple(T.shape))))
+    T, sc = T.to(dev), sc.to(dev)
+    pg = torch.full((ncol,), 0.1, dtype=torch.float64, device=dev)
+    hs = [torch.empty((ncol, nd), dtype=torch.float64, device=dev) for _ in range(4)]
+    it = torch.empty((ncol, nd), dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+    eh.hse_device(hm, sc.data_ptr(), T.data_ptr(), pg.data_ptr(), ndep.data_ptr(), ncol, nd,
+                  *(h.data_ptr() for h in hs), 0, it.data_ptr())
+    print(f"profile: hse, {ncol} columns", flush=True)
+    eh.close()
+    x = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print("profile: 1 GiB copy done", flush=True)
+
+
 def _host_cpu():
     """The host CPU model and how many logical CPUs this process may use;
     the baselines use at most 16 of them, the host share of one GPU."""
@@ -551,6 +615,9 @@ def main():
     ap.add_argument("--hse-ncol", type=int, default=16384)
     ap.add_argument("--hse-only", action="store_true",
                     help="run only the hydrostatic-equilibrium leg and print its object")
+    ap.add_argument("--profile-kernels", action="store_true",
+                    help="each kernel of the bench once at its workload shape and one 1 GiB "
+                         "device copy, for rocprofv3 --pmc")
     ap.add_argument("--profile-step", action="store_true",
                     help="one FP32-form synthesis and one 1 GiB device copy, for rocprofv3 --pmc")
     a = ap.parse_args()
@@ -561,6 +628,9 @@ def main():
         sys.exit(2)
 
     nw = max(1, min(16, len(os.sched_getaffinity(0))))
+    if a.profile_kernels:
+        _profile_kernels()
+        return
     if a.nlte_only or a.mixed_only or a.hse_only:
         import torch
         leg, n = ((_nlte_leg, a.nlte_ncol) if a.nlte_only else
