@@ -63,6 +63,9 @@ static __device__ __align__(256) unsigned char scratch[RHG_NOALLOC_SCRATCH_BYTES
 namespace {
 
 constexpr int kMaxDevices = 64;
+// LDS for a staged line list, at most: 16 KB keeps 8 one-wavefront
+// workgroups per CU (the register limit of the polarized kernels)
+constexpr size_t kLineTabMaxBytes = 16384;
 std::atomic<void *> g_scratch_ptr[kMaxDevices];
 std::mutex g_scratch_mutex;
 
@@ -91,6 +94,11 @@ struct rhg_context {
   // output-wavelength positions by solver class: polarized (STOKES) and
   // scalar (BEZIER / FEAUTRIER)
   int n_pol = 0, n_scal = 0;
+  // largest line list of a wavelength (lines, Zeeman components of damped
+  // polarizable lines) and the LDS bytes of its staged copy; 0 bytes: the
+  // polarized kernels read the lists from global memory
+  int tab_maxl = 0, tab_maxc = 0;
+  size_t tab_bytes = 0;
   const int32_t *d_io_pol = nullptr, *d_io_scal = nullptr;
   DevBuf atmos, ndep, state, moving, cchi, ceta, stokes, chiref, census, census_sum;
   // response functions: perturbed inputs/states, tangents, tape, adjoints
@@ -347,6 +355,20 @@ int rhg_open(const rhg_model_t *m, int device, rhg_context_t **out) {
   if (rc) { std::string e = g_err; rhg_close(c); return fail(rc, e); }
 
   {
+    for (int ls = 0; ls < ns; ls++) {
+      int nc = 0;
+      for (int e = m->lam_line_ptr[ls]; e < m->lam_line_ptr[ls + 1]; e++) {
+        const int l = m->lam_line_idx[e];
+        if (m->line_Grad[l] != 0.0 && m->line_polar[l] != 0)
+          nc += m->line_comp_ptr[l + 1] - m->line_comp_ptr[l];
+      }
+      c->tab_maxl = std::max(c->tab_maxl, m->lam_line_ptr[ls + 1] - m->lam_line_ptr[ls]);
+      c->tab_maxc = std::max(c->tab_maxc, nc);
+    }
+    const size_t b = rhg::linetab_bytes(std::max(c->tab_maxl, 1), std::max(c->tab_maxc, 1));
+    c->tab_bytes = b <= kLineTabMaxBytes ? b : 0;
+  }
+  {
     std::vector<int32_t> pol, scal;
     for (int io = 0; io < nout; io++)
       (m->solver[ols[io]] == RHG_SOLVER_STOKES ? pol : scal).push_back(io);
@@ -446,28 +468,32 @@ static int launch_synth(rhg_context_t *c, const double *d_atm, const int32_t *d_
   {
     const int ncb = (ncol + 63) / 64;
     dim3 blk(64);
-#define RHG_LAUNCH_FORMAL(K32, K64, LIST, NLIST)                                                 \
+#define RHG_LAUNCH_FORMAL(K32, K64, LIST, NLIST, LDS, ...)                                       \
   if (NLIST > 0) {                                                                               \
     dim3 grd((unsigned) rhg::xcd_grid(ncb, NLIST));                                              \
     if (precision == RHG_PREC_FP32)                                                              \
-      hipLaunchKernelGGL(K32, grd, blk, 0, c->stream, c->d_model, d_ndep, moving, ncol, Nmax,    \
+      hipLaunchKernelGGL(K32, grd, blk, LDS, c->stream, c->d_model, d_ndep, moving, ncol, Nmax,  \
                          state, c->d_out_ls, LIST, NLIST, nout, (const float *) cchi,           \
-                         (const float *) ceta, mu, d_out);                                       \
+                         (const float *) ceta, mu, d_out __VA_ARGS__);                           \
     else                                                                                         \
-      hipLaunchKernelGGL(K64, grd, blk, 0, c->stream, c->d_model, d_ndep, moving, ncol, Nmax,    \
+      hipLaunchKernelGGL(K64, grd, blk, LDS, c->stream, c->d_model, d_ndep, moving, ncol, Nmax,  \
                          state, c->d_out_ls, LIST, NLIST, nout, (const double *) cchi,          \
-                         (const double *) ceta, mu, d_out);                                      \
+                         (const double *) ceta, mu, d_out __VA_ARGS__);                          \
     HIPCHECK(hipGetLastError());                                                                 \
   }
+#define RHG_STOKES(MO, LT)                                                                       \
+  RHG_LAUNCH_FORMAL((rhg::k_formal_stokes<float, MO, LT>), (rhg::k_formal_stokes<double, MO, LT>), \
+                    c->d_io_pol, c->n_pol, LT ? (unsigned) c->tab_bytes : 0u, , c->tab_maxl,     \
+                    c->tab_maxc)
+    const bool lt = c->tab_bytes > 0;
     if (c->dm.magneto_optical) {
-      RHG_LAUNCH_FORMAL((rhg::k_formal_stokes<float, true>), (rhg::k_formal_stokes<double, true>),
-                        c->d_io_pol, c->n_pol)
+      if (lt) { RHG_STOKES(true, true) } else { RHG_STOKES(true, false) }
     } else {
-      RHG_LAUNCH_FORMAL((rhg::k_formal_stokes<float, false>), (rhg::k_formal_stokes<double, false>),
-                        c->d_io_pol, c->n_pol)
+      if (lt) { RHG_STOKES(false, true) } else { RHG_STOKES(false, false) }
     }
     RHG_LAUNCH_FORMAL(rhg::k_formal_scalar<float>, rhg::k_formal_scalar<double>, c->d_io_scal,
-                      c->n_scal)
+                      c->n_scal, 0u, )
+#undef RHG_STOKES
 #undef RHG_LAUNCH_FORMAL
   }
   HIPCHECK(hipEventRecord(c->ev[4], c->stream));

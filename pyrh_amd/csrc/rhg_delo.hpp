@@ -25,8 +25,8 @@ struct Point {
 // Point from the continuum (cchi, ceta) and the line opacity at one depth;
 // F(field) reads the column state there; the Humlicek functions in TV; MO:
 // with the magneto-optical terms.
-template <typename T, typename TV = T, bool MO = false, typename Field>
-__device__ __forceinline__ void form_point_f(const DevModel &M, int ls, int has_line,
+template <typename T, typename TV = T, bool MO = false, typename Field, typename Tab>
+__device__ __forceinline__ void form_point_t(const Tab &tab, int ncont, int has_line,
                                              const Field &F, T mu, int moving, T cchi, T ceta,
                                              Point<T> &p) {
   T chi4[4], eta4[4], chip[3] = {T(0), T(0), T(0)};
@@ -35,7 +35,7 @@ __device__ __forceinline__ void form_point_f(const DevModel &M, int ls, int has_
   eta4[1] = eta4[2] = eta4[3] = T(0);
   if (has_line) {
     T lI, lE, lq[3], le[3];
-    line_point_f<T, TV, MO>(M, ls, F, mu, lI, lq, lE, le, moving, chip);
+    line_point_t<T, TV, MO>(tab, ncont, F, mu, lI, lq, lE, le, moving, chip);
     chi4[0] += lI; eta4[0] += lE;
 #pragma unroll
     for (int n = 0; n < 3; n++) { chi4[n + 1] += lq[n]; eta4[n + 1] += le[n]; }
@@ -50,14 +50,35 @@ __device__ __forceinline__ void form_point_f(const DevModel &M, int ls, int has_
   for (int n = 0; n < 3; n++) p.r[n] = MO ? chip[n] * ichi : T(0);
 }
 
+// the same over wavelength ls's line list in the model
+template <typename T, typename TV = T, bool MO = false, typename Field>
+__device__ __forceinline__ void form_point_f(const DevModel &M, int ls, int has_line,
+                                             const Field &F, T mu, int moving, T cchi, T ceta,
+                                             Point<T> &p) {
+  const int p0 = M.lam_line_ptr[ls];
+  form_point_t<T, TV, MO>(LineTabG(M, p0, M.lam_line_ptr[ls + 1] - p0), M.ncont, has_line, F, mu,
+                          moving, cchi, ceta, p);
+}
+
 // FP64 point over the column state at st (stride fs) with the Humlicek
-// functions in TV and the continuum stored as R
+// functions in TV and the continuum stored as R, from the line table tab
+template <typename TV, typename R, bool MO = false, typename Tab>
+__device__ __forceinline__ void form_point_vt(const Tab &tab, int ncont, int has_line,
+                                              const double *st, int64_t fs, double mu, int moving,
+                                              R cchi, R ceta, Point<double> &p) {
+  auto F = [st, fs](int f) { return st[f * fs]; };
+  form_point_t<double, TV, MO>(tab, ncont, has_line, F, mu, moving, (double) cchi, (double) ceta,
+                               p);
+}
+
+// the same over wavelength ls's line list in the model
 template <typename TV, typename R, bool MO = false>
 __device__ __forceinline__ void form_point_v(const DevModel &M, int ls, int has_line,
                                              const double *st, int64_t fs, double mu, int moving,
                                              R cchi, R ceta, Point<double> &p) {
-  auto F = [st, fs](int f) { return st[f * fs]; };
-  form_point_f<double, TV, MO>(M, ls, has_line, F, mu, moving, (double) cchi, (double) ceta, p);
+  const int p0 = M.lam_line_ptr[ls];
+  form_point_vt<TV, R, MO>(LineTabG(M, p0, M.lam_line_ptr[ls + 1] - p0), M.ncont, has_line, st,
+                           fs, mu, moving, cchi, ceta, p);
 }
 
 // y = K(q) x for K(q) = [[0, q^T], [q, 0]]

@@ -216,13 +216,16 @@ k_rf_loggf(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep,
   template __global__ void KNAME<R>(const DevModel *, const int32_t *, const int32_t *, int, int, \
                                     const double *, const int32_t *, const int32_t *, int, int,  \
                                     const R *, const R *, double, double *);
-#define RHG_KSTOKES(R, MO)                                                                    \
-  template __global__ void k_formal_stokes<R, MO>(const DevModel *, const int32_t *, const int32_t *, \
-                                                  int, int, const double *, const int32_t *,       \
-                                                  const int32_t *, int, int, const R *, const R *, \
-                                                  double, double *);
-RHG_KSTOKES(double, false)
-RHG_KSTOKES(double, true)
+#define RHG_KSTOKES(R, MO, LT)                                                                \
+  template __global__ void k_formal_stokes<R, MO, LT>(const DevModel *, const int32_t *,            \
+                                                      const int32_t *, int, int, const double *,    \
+                                                      const int32_t *, const int32_t *, int, int,   \
+                                                      const R *, const R *, double, double *, int,  \
+                                                      int);
+RHG_KSTOKES(double, false, false)
+RHG_KSTOKES(double, true, false)
+RHG_KSTOKES(double, false, true)
+RHG_KSTOKES(double, true, true)
 #undef RHG_KSTOKES
 RHG_KFORMAL(k_formal_scalar, double)
 #undef RHG_KFORMAL

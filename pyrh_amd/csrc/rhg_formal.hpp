@@ -89,14 +89,32 @@ __device__ __forceinline__ void form_point(const DevModel &M, int ls, int has_li
 // advances the recursion at depth k = j + 2, and j = -1 takes the step at
 // k = 1 whose downwind derivative is one-sided.  MO: with the
 // magneto-optical terms (rh/stokesopac.c:56-86, rh/kurucz.c:711-715).
-template <typename R, bool MO>
+// LT: the wavelength's line list (lines, Doppler offsets, Zeeman
+// components) is staged in LDS once per wavefront, tab_maxl lines and
+// tab_maxc components at most (the maxima over the model's wavelengths), so
+// that the depth loop reads those constants from LDS rather than from
+// global memory at every depth.
+template <typename R, bool MO, bool LT>
 __global__ void __launch_bounds__(64)
 k_formal_stokes(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep,
                 const int32_t *__restrict__ moving_arr, int ncol, int Nmax,
                 const double *__restrict__ state, const int32_t *__restrict__ out_ls,
                 const int32_t *__restrict__ io_list, int nlist, int nout,
                 const R *__restrict__ cchi, const R *__restrict__ ceta, double mu_d,
-                double *__restrict__ stokes) {
+                double *__restrict__ stokes, int tab_maxl, int tab_maxc) {
+  // the line table first: every lane of the wavefront stages it
+  auto tab = [&]() {
+    int cb0, t0;
+    xcd_tile(blockIdx.x, nlist, cb0, t0);
+    const int ls0 = out_ls[io_list[t0]];
+    if constexpr (LT) {
+      extern __shared__ __align__(8) char lds_tab[];
+      return stage_lines(*Mp, ls0, lds_tab, tab_maxl, tab_maxc);
+    } else {
+      const int p0 = Mp->lam_line_ptr[ls0];
+      return LineTabG(*Mp, p0, Mp->lam_line_ptr[ls0 + 1] - p0);
+    }
+  }();
   RAY_SETUP();
   (void) mu;
   const double imu = 1.0 / mu_d;
@@ -104,8 +122,8 @@ k_formal_stokes(const DevModel *__restrict__ Mp, const int32_t *__restrict__ nde
   Point<double> Pn;
   for (int j = N - 1; j >= -1; j--) {
     if (j >= 0)
-      form_point_v<R, R, MO>(M, ls, has_line, st0 + (int64_t) j * ncol, fs, mu_d, moving,
-                             cc[(int64_t) j * ncol], ce[(int64_t) j * ncol], Pn);
+      form_point_vt<R, R, MO>(tab, M.ncont, has_line, st0 + (int64_t) j * ncol, fs, mu_d, moving,
+                              cc[(int64_t) j * ncol], ce[(int64_t) j * ncol], Pn);
     if (j == N - 1) {
       st.Pu = Pn;
     } else if (j == N - 2) {

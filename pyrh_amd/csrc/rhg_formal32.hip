@@ -10,13 +10,16 @@ namespace rhg {
   template __global__ void KNAME<R>(const DevModel *, const int32_t *, const int32_t *, int, int, \
                                     const double *, const int32_t *, const int32_t *, int, int,  \
                                     const R *, const R *, double, double *);
-#define RHG_KSTOKES(R, MO)                                                                    \
-  template __global__ void k_formal_stokes<R, MO>(const DevModel *, const int32_t *, const int32_t *, \
-                                                  int, int, const double *, const int32_t *,       \
-                                                  const int32_t *, int, int, const R *, const R *, \
-                                                  double, double *);
-RHG_KSTOKES(float, false)
-RHG_KSTOKES(float, true)
+#define RHG_KSTOKES(R, MO, LT)                                                                \
+  template __global__ void k_formal_stokes<R, MO, LT>(const DevModel *, const int32_t *,            \
+                                                      const int32_t *, int, int, const double *,    \
+                                                      const int32_t *, const int32_t *, int, int,   \
+                                                      const R *, const R *, double, double *, int,  \
+                                                      int);
+RHG_KSTOKES(float, false, false)
+RHG_KSTOKES(float, true, false)
+RHG_KSTOKES(float, false, true)
+RHG_KSTOKES(float, true, true)
 #undef RHG_KSTOKES
 RHG_KFORMAL(k_formal_scalar, float)
 #undef RHG_KFORMAL

@@ -123,47 +123,157 @@ __device__ __forceinline__ double atom_sum(const DevModel &M, int l0, int nl, do
   return sum;
 }
 
-// Profile of entry e (line l) of wavelength ls's line list at one depth,
+// The line list of one wavelength as the profile code reads it: slot i
+// (0 <= i < n) is a line of the wavelength's list with its Doppler offset
+// (lambda/lambda0 - 1) c, and its Zeeman components are c0(i) .. c1(i)-1.
+// LineTabG reads the model in global memory; LineTabL (below) reads a copy
+// staged in LDS by the kernel.
+struct LineTabG {
+  const DevModel &M;
+  int p0, n;
+  // entries p0 .. p0 + n - 1 of the model's wavelength line lists
+  __device__ __forceinline__ LineTabG(const DevModel &M_, int p0_, int n_) : M(M_), p0(p0_), n(n_) {}
+  __device__ __forceinline__ int line(int i) const { return M.lam_line_idx[p0 + i]; }
+  __device__ __forceinline__ double dv(int i) const { return M.lam_line_dv[p0 + i]; }
+  __device__ __forceinline__ bool damped(int i) const { return M.line_Grad[line(i)] != 0.0; }
+  __device__ __forceinline__ bool polar(int i) const { return M.line_polar[line(i)] != 0; }
+  __device__ __forceinline__ double larmor(int i) const { return M.line_larmor[line(i)]; }
+  __device__ __forceinline__ int c0(int i) const { return M.line_comp_ptr[line(i)]; }
+  __device__ __forceinline__ int c1(int i) const { return M.line_comp_ptr[line(i) + 1]; }
+  __device__ __forceinline__ double shift(int c) const { return M.comp_shift[c]; }
+  __device__ __forceinline__ double strength(int c) const { return M.comp_strength[c]; }
+  __device__ __forceinline__ int q(int c) const { return M.comp_q[c]; }
+};
+
+// A wavelength's line list staged in LDS (one copy per wavefront; every
+// lane reads the same address, so the reads are broadcasts).  Counts and
+// indices are made wave-uniform so that the loops over lines and
+// components run as scalar loops.
+struct LineRecL {
+  double dv, larmor;
+  int line, flags, c0, c1;   // flags: 1 damped, 2 polarizable
+};
+
+struct LineTabL {
+  const LineRecL *rec;
+  const double *shift_, *strength_;
+  const int *q_;
+  int n;
+  __device__ __forceinline__ int line(int i) const {
+    return __builtin_amdgcn_readfirstlane(rec[i].line);
+  }
+  __device__ __forceinline__ double dv(int i) const { return rec[i].dv; }
+  __device__ __forceinline__ bool damped(int i) const {
+    return __builtin_amdgcn_readfirstlane(rec[i].flags) & 1;
+  }
+  __device__ __forceinline__ bool polar(int i) const {
+    return __builtin_amdgcn_readfirstlane(rec[i].flags) & 2;
+  }
+  __device__ __forceinline__ double larmor(int i) const { return rec[i].larmor; }
+  __device__ __forceinline__ int c0(int i) const { return __builtin_amdgcn_readfirstlane(rec[i].c0); }
+  __device__ __forceinline__ int c1(int i) const { return __builtin_amdgcn_readfirstlane(rec[i].c1); }
+  __device__ __forceinline__ double shift(int c) const { return shift_[c]; }
+  __device__ __forceinline__ double strength(int c) const { return strength_[c]; }
+  __device__ __forceinline__ int q(int c) const { return __builtin_amdgcn_readfirstlane(q_[c]); }
+};
+
+// LDS bytes of a staged line list of at most maxl lines and maxc components
+__host__ __device__ __forceinline__ size_t linetab_bytes(int maxl, int maxc) {
+  return (size_t) maxl * sizeof(LineRecL) + (size_t) maxc * (2 * sizeof(double) + sizeof(int));
+}
+
+// Stage wavelength ls's line list into smem (linetab_bytes(maxl, maxc)
+// bytes, 8-byte aligned) with the 64 lanes of a one-wavefront workgroup.
+// The caller guarantees that the list fits (the host takes maxl and maxc
+// as the maxima over the model's wavelengths).  Ends with a barrier.
+__device__ __forceinline__ LineTabL stage_lines(const DevModel &M, int ls, char *smem, int maxl,
+                                                int maxc) {
+  LineRecL *rec = (LineRecL *) smem;
+  double *shift = (double *) (smem + (size_t) maxl * sizeof(LineRecL));
+  double *strength = shift + maxc;
+  int *q = (int *) (strength + maxc);
+  const int p0 = M.lam_line_ptr[ls], n = M.lam_line_ptr[ls + 1] - p0;
+  const int lane = threadIdx.x;
+  for (int i = lane; i < n; i += 64) {
+    const int l = M.lam_line_idx[p0 + i];
+    const bool damped = M.line_Grad[l] != 0.0, polar = M.line_polar[l] != 0;
+    rec[i].dv = M.lam_line_dv[p0 + i];
+    rec[i].larmor = M.line_larmor[l];
+    rec[i].line = l;
+    rec[i].flags = (damped ? 1 : 0) | (polar ? 2 : 0);
+    // only a damped polarizable line uses its components
+    rec[i].c1 = (damped && polar) ? M.line_comp_ptr[l + 1] - M.line_comp_ptr[l] : 0;
+  }
+  __syncthreads();
+  // offsets: c0(i) = sum of the counts before slot i
+  for (int i = lane; i < n; i += 64) {
+    int off = 0;
+    for (int k = 0; k < i; k++) off += rec[k].c1;
+    rec[i].c0 = off;
+  }
+  __syncthreads();
+  for (int i = lane; i < n; i += 64) rec[i].c1 += rec[i].c0;
+  __syncthreads();
+  for (int i = 0; i < n; i++) {
+    const int a = rec[i].c0, cnt = rec[i].c1 - a;
+    const int g0 = M.line_comp_ptr[rec[i].line];
+    for (int c = lane; c < cnt; c += 64) {
+      shift[a + c] = M.comp_shift[g0 + c];
+      strength[a + c] = M.comp_strength[g0 + c];
+      q[a + c] = M.comp_q[g0 + c];
+    }
+  }
+  __syncthreads();
+  LineTabL t;
+  t.rec = rec;
+  t.shift_ = shift;
+  t.strength_ = strength;
+  t.q_ = q;
+  t.n = __builtin_amdgcn_readfirstlane(n);
+  return t;
+}
+
+// Profile of slot i of a wavelength's line list (line l) at one depth,
 // RLKProfile rh/kurucz.c:729-827 for the emergent ray (to_obs = 1), in the
 // number type T (double, float, or a dual number of either).  F(field)
 // returns a column-state field at that depth as a T; the Doppler offset
-// (lambda/lambda0 - 1) c comes from the model in FP64.  Returns phi;
+// (lambda/lambda0 - 1) c comes from the line table in FP64.  Returns phi;
 // phi_Q,U,V are set for a polarizable line with damping, and with MO (the
 // magneto-optical terms) the Faraday-Voigt profiles psi[3] = psi_Q,U,V
 // (rh/kurucz.c:811-819).  The Humlicek functions of the Zeeman components
 // are evaluated in TV (T, or float in the FP32 form), their sums and
 // combinations in T.
-template <typename T, typename TV = T, bool MO = false, typename Field>
-__device__ __forceinline__ T line_profile_f(const DevModel &M, int e, int l, const Field &F, T mu,
+template <typename T, typename TV = T, bool MO = false, typename Field, typename Tab>
+__device__ __forceinline__ T line_profile_t(const Tab &tab, int i, int lb, const Field &F, T mu,
                                             T vz, int moving, T &phi_Q, T &phi_U, T &phi_V,
                                             T *psi = nullptr) {
-  const int lb = line_field_base(M.ncont);
+  const int l = tab.line(i);
   const T adamp = F(lb + 4 * l + 2), ivb = F(lb + 4 * l + 3);
-  T v = T(M.lam_line_dv[e]) * ivb;
+  T v = T(tab.dv(i)) * ivb;
   if (moving) v += (mu * vz) * ivb;
   const T sv = ivb * T(INV_SQRTPI);
   T phi;
-  if (M.line_Grad[l] != 0.0) {
-    if (M.line_polar[l] != 0) {
+  if (tab.damped(i)) {
+    if (tab.polar(i)) {
       const T cosg = F(RHG_CF_COSG);
       const T sin2_gamma = T(1.0) - cosg * cosg;
-      const T vB = T(M.line_larmor[l]) * F(RHG_CF_B) * ivb;
+      const T vB = T(tab.larmor(i)) * F(RHG_CF_B) * ivb;
       T phi_sm = T(0), phi_pi = T(0), phi_sp = T(0), H, Fv;
       T psi_sm = T(0), psi_pi = T(0), psi_sp = T(0);
-      const int c0 = M.line_comp_ptr[l], c1 = M.line_comp_ptr[l + 1];
+      const int c0 = tab.c0(i), c1 = tab.c1(i);
       for (int nz = c0; nz < c1; nz++) {
         TV Fw;
-        const T u = v - T(M.comp_shift[nz]) * vB;
+        const T u = v - T(tab.shift(nz)) * vB;
         const int reg = humlicek_region(val(adamp), val(u));
         H = (T) VoigtEval<TV>::eval((TV) adamp, (TV) u, reg, &Fw);
         Fv = (T) Fw;
-        const int q = M.comp_q[nz];
-        const T sH = T(M.comp_strength[nz]) * H;
+        const int q = tab.q(nz);
+        const T sH = T(tab.strength(nz)) * H;
         if (q == -1) phi_sm += sH;
         else if (q == 0) phi_pi += sH;
         else phi_sp += sH;
         if constexpr (MO) {
-          const T sF = T(M.comp_strength[nz]) * Fv;
+          const T sF = T(tab.strength(nz)) * Fv;
           if (q == -1) psi_sm += sF;
           else if (q == 0) psi_pi += sF;
           else psi_sp += sF;
@@ -192,31 +302,41 @@ __device__ __forceinline__ T line_profile_f(const DevModel &M, int e, int l, con
   return phi;
 }
 
-// Kurucz-line opacity and emissivity of the emergent ray (to_obs = 1) at
-// wavelength ls and one depth, rh/kurucz.c:600-715: the sum over the
-// wavelength's line list of line_profile_f times the line's chi_l, eta_l.
-// chiI/etaI and chiQ[3]/etaQ[3] accumulate from zero; with MO also the
-// magneto-optical chip[3] (chi_l psi_Q,U,V, rh/kurucz.c:711-715).
+// Entry e (line l) of wavelength ls's list, from the model (the census and
+// the log gf response functions address entries directly).
 template <typename T, typename TV = T, bool MO = false, typename Field>
-__device__ __forceinline__ void line_point_f(const DevModel &M, int ls, const Field &F, T mu,
+__device__ __forceinline__ T line_profile_f(const DevModel &M, int e, int l, const Field &F, T mu,
+                                            T vz, int moving, T &phi_Q, T &phi_U, T &phi_V,
+                                            T *psi = nullptr) {
+  (void) l;
+  return line_profile_t<T, TV, MO>(LineTabG(M, e, 1), 0, line_field_base(M.ncont), F, mu, vz,
+                                   moving, phi_Q, phi_U, phi_V, psi);
+}
+
+// Kurucz-line opacity and emissivity of the emergent ray (to_obs = 1) at
+// one wavelength and one depth, rh/kurucz.c:600-715: the sum over the
+// wavelength's line list (tab) of line_profile_t times the line's chi_l,
+// eta_l.  chiI/etaI and chiQ[3]/etaQ[3] accumulate from zero; with MO also
+// the magneto-optical chip[3] (chi_l psi_Q,U,V, rh/kurucz.c:711-715).
+template <typename T, typename TV = T, bool MO = false, typename Field, typename Tab>
+__device__ __forceinline__ void line_point_t(const Tab &tab, int ncont, const Field &F, T mu,
                                              T &chiI, T *chiQ, T &etaI, T *etaQ, int moving,
                                              T *chip = nullptr) {
-  const int p0 = M.lam_line_ptr[ls], p1 = M.lam_line_ptr[ls + 1];
-  const int lb = line_field_base(M.ncont);
+  const int lb = line_field_base(ncont);
   chiI = T(0); etaI = T(0);
   chiQ[0] = chiQ[1] = chiQ[2] = T(0);
   etaQ[0] = etaQ[1] = etaQ[2] = T(0);
   if constexpr (MO) chip[0] = chip[1] = chip[2] = T(0);
   const T vz = F(RHG_CF_VLOS);
-  for (int e = p0; e < p1; e++) {
-    const int l = M.lam_line_idx[e];
+  for (int i = 0; i < tab.n; i++) {
+    const int l = tab.line(i);
     const T chi_l = F(lb + 4 * l), eta_l = F(lb + 4 * l + 1);
     T phi_Q = T(0), phi_U = T(0), phi_V = T(0), psi[3] = {T(0), T(0), T(0)};
-    const T phi = line_profile_f<T, TV, MO>(M, e, l, F, mu, vz, moving, phi_Q, phi_U, phi_V, psi);
+    const T phi = line_profile_t<T, TV, MO>(tab, i, lb, F, mu, vz, moving, phi_Q, phi_U, phi_V, psi);
     if (phi != T(0)) {
       chiI += chi_l * phi;
       etaI += eta_l * phi;
-      if (M.line_polar[l] != 0 && M.line_Grad[l] != 0.0) {
+      if (tab.polar(i) && tab.damped(i)) {
         chiQ[0] += chi_l * phi_Q;
         chiQ[1] += chi_l * phi_U;
         chiQ[2] += chi_l * phi_V;
@@ -231,6 +351,16 @@ __device__ __forceinline__ void line_point_f(const DevModel &M, int ls, const Fi
       }
     }
   }
+}
+
+// the same over wavelength ls's list in the model
+template <typename T, typename TV = T, bool MO = false, typename Field>
+__device__ __forceinline__ void line_point_f(const DevModel &M, int ls, const Field &F, T mu,
+                                             T &chiI, T *chiQ, T &etaI, T *etaQ, int moving,
+                                             T *chip = nullptr) {
+  const int p0 = M.lam_line_ptr[ls];
+  line_point_t<T, TV, MO>(LineTabG(M, p0, M.lam_line_ptr[ls + 1] - p0), M.ncont, F, mu, chiI, chiQ,
+                          etaI, etaQ, moving, chip);
 }
 
 // the same with the fields read from the column state at st (stride fs)
@@ -282,11 +412,12 @@ template <typename R>
 __global__ void k_continuum(const DevModel *M, const int32_t *ndep, int ncol, int Nmax,
                             const double *state, int nfield, const int32_t *out_ls, int nout,
                             R *cchi, R *ceta);
-template <typename R, bool MO>
+template <typename R, bool MO, bool LT>
 __global__ void k_formal_stokes(const DevModel *M, const int32_t *ndep, const int32_t *moving,
                                 int ncol, int Nmax, const double *state, const int32_t *out_ls,
                                 const int32_t *io_list, int nlist, int nout, const R *cchi,
-                                const R *ceta, double mu, double *stokes);
+                                const R *ceta, double mu, double *stokes, int tab_maxl,
+                                int tab_maxc);
 template <typename R>
 __global__ void k_formal_scalar(const DevModel *M, const int32_t *ndep, const int32_t *moving,
                                 int ncol, int Nmax, const double *state, const int32_t *out_ls,
