@@ -55,8 +55,8 @@ template <typename T, typename TV = T, bool MO = false, typename Field>
 __device__ __forceinline__ void form_point_f(const DevModel &M, int ls, int has_line,
                                              const Field &F, T mu, int moving, T cchi, T ceta,
                                              Point<T> &p) {
-  const int p0 = M.lam_line_ptr[ls];
-  form_point_t<T, TV, MO>(LineTabG(M, p0, M.lam_line_ptr[ls + 1] - p0), M.ncont, has_line, F, mu,
+  const int p0 = ldc(M.lam_line_ptr)[ls];
+  form_point_t<T, TV, MO>(LineTabG(M, p0, ldc(M.lam_line_ptr)[ls + 1] - p0), M.ncont, has_line, F, mu,
                           moving, cchi, ceta, p);
 }
 
@@ -76,8 +76,8 @@ template <typename TV, typename R, bool MO = false>
 __device__ __forceinline__ void form_point_v(const DevModel &M, int ls, int has_line,
                                              const double *st, int64_t fs, double mu, int moving,
                                              R cchi, R ceta, Point<double> &p) {
-  const int p0 = M.lam_line_ptr[ls];
-  form_point_vt<TV, R, MO>(LineTabG(M, p0, M.lam_line_ptr[ls + 1] - p0), M.ncont, has_line, st,
+  const int p0 = ldc(M.lam_line_ptr)[ls];
+  form_point_vt<TV, R, MO>(LineTabG(M, p0, ldc(M.lam_line_ptr)[ls + 1] - p0), M.ncont, has_line, st,
                            fs, mu, moving, cchi, ceta, p);
 }
 

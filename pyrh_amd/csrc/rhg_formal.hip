@@ -33,7 +33,7 @@ k_rf_loggf(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep,
            double *__restrict__ rf) {
   using R = double;
   RAY_SETUP_BASE();
-  if (column_solver(M.solver[ls], moving) != RHG_SOLVER_BEZIER) return;
+  if (column_solver(ldc(M.solver)[ls], moving) != RHG_SOLVER_BEZIER) return;
   const double zmu = 1.0 / mu_d;
   const int lb = line_field_base(M.ncont);
   double *Ii = Iin + (int64_t) t * Nmax * ncol + ic;
@@ -95,12 +95,12 @@ k_rf_loggf(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep,
     IIN(N - 1) = (1.0 - w[0]) * I_upw + w[0] * P0.S[0] + w[1] * dS_uw;
   }
 
-  const int p0 = M.lam_line_ptr[ls], p1 = M.lam_line_ptr[ls + 1];
+  const int p0 = ldc(M.lam_line_ptr)[ls], p1 = ldc(M.lam_line_ptr)[ls + 1];
   for (int p = 0; p < npar; p++) {
     const int lp = par_line[p];
     int ep = -1;
     for (int e = p0; e < p1; e++)
-      if (M.lam_line_idx[e] == lp) ep = e;
+      if (ldc(M.lam_line_idx)[e] == lp) ep = e;
     if (lp < 0 || ep < 0) continue;
     // d chi and d eta of the parameter's line at depth kk
     auto dline = [&](int kk, double &a, double &b) {
@@ -245,10 +245,10 @@ k_census(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep, cons
   const double *st0 = state + ic;
   int32_t cnt[RHG_NWORK];
   for (int i = 0; i < RHG_NWORK; i++) cnt[i] = 0;
-  const int solver = column_solver(M.solver[ls], moving);
+  const int solver = column_solver(ldc(M.solver)[ls], moving);
   const int slot = solver == RHG_SOLVER_STOKES ? RHG_WORK_STOKES
                  : solver == RHG_SOLVER_BEZIER ? RHG_WORK_BEZIER : RHG_WORK_FEAUTRIER;
-  const int has_line = M.lam_line_ptr[ls + 1] > M.lam_line_ptr[ls];
+  const int has_line = ldc(M.lam_line_ptr)[ls + 1] > ldc(M.lam_line_ptr)[ls];
   for (int k = 1; k < N; k++) {
     cnt[slot]++;
     if (has_line) line_census(M, ls, st0 + (int64_t) k * ncol, fs, mu, moving, cnt);

@@ -64,7 +64,7 @@ __device__ __forceinline__ void form_point(const DevModel &M, int ls, int has_li
   const double *Tc = st0 + (int64_t) RHG_CF_T * fs;                                      \
   const R *cc = cchi + (int64_t) io * Nmax * ncol + ic;                                  \
   const R *ce = ceta + (int64_t) io * Nmax * ncol + ic;                                  \
-  const int has_line = M.lam_line_ptr[ls + 1] > M.lam_line_ptr[ls];                      \
+  const int has_line = ldc(M.lam_line_ptr)[ls + 1] > ldc(M.lam_line_ptr)[ls];                      \
   const double *lc = M.lamconst + (int64_t) ls * RHG_LC_NCONST;                          \
   const R mu = (R) mu_d;                                                                 \
   double B2[2];                                                                          \
@@ -158,7 +158,7 @@ k_formal_scalar(const DevModel *__restrict__ Mp, const int32_t *__restrict__ nde
                 const R *__restrict__ cchi, const R *__restrict__ ceta, double mu_d,
                 double *__restrict__ stokes) {
   RAY_SETUP();
-  const int solver = column_solver(M.solver[ls], moving);
+  const int solver = column_solver(ldc(M.solver)[ls], moving);
   Point<R> Pn;
   if (solver == RHG_SOLVER_BEZIER) {
     const double zmu = 1.0 / mu_d;

@@ -48,11 +48,11 @@ __device__ void hse_stage_factors(const DevModel &M, const DevHse &H, double T, 
   const int npf = M.npf;
   int jT = 0, edge = 0;
   double fx = 0.0;
-  if (T <= M.Tpf[0]) edge = -1;
-  else if (T >= M.Tpf[npf - 1]) edge = 1;
+  if (T <= ldc(M.Tpf)[0]) edge = -1;
+  else if (T >= ldc(M.Tpf)[npf - 1]) edge = 1;
   else {
     jT = locate_asc(npf, M.Tpf, T);
-    fx = (M.Tpf[jT + 1] - T) / (M.Tpf[jT + 1] - M.Tpf[jT]);
+    fx = (ldc(M.Tpf)[jT + 1] - T) / (ldc(M.Tpf)[jT + 1] - ldc(M.Tpf)[jT]);
   }
   auto pf = [&](const double *yt) -> double {
     if (edge < 0) return yt[0];
@@ -79,16 +79,16 @@ __device__ void hse_stage_factors(const DevModel &M, const DevHse &H, double T, 
 // the first ionised one (the neutral fraction of rh/chemequil.c:222-236).
 // The LTE normalisation is then 1 + sum_d g[d] / (cNe_T)^d for any n_e.
 __device__ void hse_level_groups(const DevModel &M, int ia, double T, double *g, int64_t ss) {
-  const int l0 = M.atom_level_ptr[ia], nl = M.atom_level_ptr[ia + 1] - l0;
+  const int l0 = ldc(M.atom_level_ptr)[ia], nl = ldc(M.atom_level_ptr)[ia + 1] - l0;
   double acc[HSE_NDZ + 1];
 #pragma unroll
   for (int d = 0; d <= HSE_NDZ; d++) acc[d] = 0.0;
-  bool neutral = M.level_stage[l0] == 0;
+  bool neutral = ldc(M.level_stage)[l0] == 0;
   for (int i = 1; i < nl; i++) {
-    const double dE = M.level_E[l0 + i] - M.level_E[l0];
-    const double r = (M.level_g[l0 + i] / M.level_g[l0]) * exp(-(dE / (KBOLTZMANN * T)));
-    const int dZ = M.level_stage[l0 + i] - M.level_stage[l0];
-    if (M.level_stage[l0 + i] > 0) neutral = false;
+    const double dE = ldc(M.level_E)[l0 + i] - ldc(M.level_E)[l0];
+    const double r = (ldc(M.level_g)[l0 + i] / ldc(M.level_g)[l0]) * exp(-(dE / (KBOLTZMANN * T)));
+    const int dZ = ldc(M.level_stage)[l0 + i] - ldc(M.level_stage)[l0];
+    if (ldc(M.level_stage)[l0 + i] > 0) neutral = false;
 #pragma unroll
     for (int d = 0; d < HSE_NDZ; d++)
       if (dZ == d) acc[d] += r;
@@ -130,11 +130,11 @@ __device__ double hse_get_ne(const DevModel &M, const DevHse &H, double T, doubl
   int jT = 0, edge = 0;
   double fx = 0.0;
   if (!CACHED) {
-    if (T <= M.Tpf[0]) edge = -1;
-    else if (T >= M.Tpf[npf - 1]) edge = 1;
+    if (T <= ldc(M.Tpf)[0]) edge = -1;
+    else if (T >= ldc(M.Tpf)[npf - 1]) edge = 1;
     else {
       jT = locate_asc(npf, M.Tpf, T);
-      fx = (M.Tpf[jT + 1] - T) / (M.Tpf[jT + 1] - M.Tpf[jT]);
+      fx = (ldc(M.Tpf)[jT + 1] - T) / (ldc(M.Tpf)[jT + 1] - ldc(M.Tpf)[jT]);
     }
   }
   auto pf = [&](const double *yt) -> double {
@@ -242,11 +242,11 @@ __device__ double hse_opacity(const DevModel &M, const DevHse &H, int ls, double
   total += chi;
   total *= H.hmin_fudge;
   if (lc[RHG_LC_OH_EIDX] >= 0.0) {
-    if (T < M.oh_T[0] || T > M.oh_T[NTOH - 1]) chi = 0.0;
+    if (T < ldc(M.oh_T)[0] || T > ldc(M.oh_T)[NTOH - 1]) chi = 0.0;
     else {
       int i2 = locate_asc(NTOH, M.oh_T, T);
       double ti = (i2 == NTOH - 1) ? (double) i2
-                                   : (double) i2 + (T - M.oh_T[i2]) / (M.oh_T[i2 + 1] - M.oh_T[i2]);
+                                   : (double) i2 + (T - ldc(M.oh_T)[i2]) / (ldc(M.oh_T)[i2 + 1] - ldc(M.oh_T)[i2]);
       double kappa = exp(LG10 * bilinear(NTOH, NEOH, M.oh_cross, ti, lc[RHG_LC_OH_EIDX])) *
                      (CM_TO_M * CM_TO_M);
       stim = exp(-lc[RHG_LC_HC_KLA] / T);
@@ -255,11 +255,11 @@ __device__ double hse_opacity(const DevModel &M, const DevHse &H, int ls, double
     total += chi;
   }
   if (lc[RHG_LC_CH_EIDX] >= 0.0) {
-    if (T < M.ch_T[0] || T > M.ch_T[NTCH - 1]) chi = 0.0;
+    if (T < ldc(M.ch_T)[0] || T > ldc(M.ch_T)[NTCH - 1]) chi = 0.0;
     else {
       int i2 = locate_asc(NTCH, M.ch_T, T);
       double ti = (i2 == NTCH - 1) ? (double) i2
-                                   : (double) i2 + (T - M.ch_T[i2]) / (M.ch_T[i2 + 1] - M.ch_T[i2]);
+                                   : (double) i2 + (T - ldc(M.ch_T)[i2]) / (ldc(M.ch_T)[i2 + 1] - ldc(M.ch_T)[i2]);
       double kappa = exp(LG10 * bilinear(NTCH, NECH, M.ch_cross, ti, lc[RHG_LC_CH_EIDX])) *
                      (CM_TO_M * CM_TO_M);
       stim = exp(-lc[RHG_LC_HC_KLA] / T);
@@ -268,15 +268,15 @@ __device__ double hse_opacity(const DevModel &M, const DevHse &H, int ls, double
     total += chi;
   }
   {  // H bound-free, rh/hydrogen.c:150-208 (active-layer branch)
-    const int cp0 = M.lam_cont_ptr[ls], cp1 = M.lam_cont_ptr[ls + 1];
+    const int cp0 = ldc(M.lam_cont_ptr)[ls], cp1 = ldc(M.lam_cont_ptr)[ls + 1];
     bool any = false;
     chi = 0.0;
     for (int e = cp0; e < cp1; e++) {
-      const int ci = M.lam_cont_idx[e];
-      if (M.cont_atom[ci] != 0) continue;
+      const int ci = ldc(M.lam_cont_idx)[e];
+      if (ldc(M.cont_atom)[ci] != 0) continue;
       any = true;
       const double expla = exp(-lc[RHG_LC_BF_HC_KLA] / T);
-      chi += M.lam_cont_alpha[e] * (1.0 - expla) * hpop(M.cont_i[ci]);
+      chi += ldc(M.lam_cont_alpha)[e] * (1.0 - expla) * hpop(ldc(M.cont_i)[ci]);
     }
     if (any) total += chi;
   }
@@ -328,9 +328,9 @@ __device__ void hse_layer(const DevModel &M, const DevHse &H, double T, double n
   double nstar0n[NN], fn0[NN];
 #pragma unroll
   for (int i = 0; i < NN; i++) {
-    const int ia = M.nucl_atom[i];
+    const int ia = ldc(M.nucl_atom)[i];
     const double *g = lg + (int64_t) i * (HSE_NDZ + 1) * ss;
-    const double ntot = M.atom_abund[ia] * nHtot;
+    const double ntot = ldc(M.atom_abund)[ia] * nHtot;
     nstar0n[i] = ntot / hse_level_sum(g, ss, cNe_T);
     fn0[i] = (nstar0n[i] + g[HSE_NDZ * ss] * nstar0n[i]) / ntot;
   }
@@ -339,8 +339,8 @@ __device__ void hse_layer(const DevModel &M, const DevHse &H, double T, double n
               &tc.p15);
   double fracn[NN];
 #pragma unroll
-  for (int i = 0; i < NN; i++) fracn[i] = nnuc[i] / (M.atom_abund[M.nucl_atom[i]] * nHtot);
-  const int lH = M.atom_level_ptr[0], nlH = M.atom_level_ptr[1] - lH;
+  for (int i = 0; i < NN; i++) fracn[i] = nnuc[i] / (ldc(M.atom_abund)[ldc(M.nucl_atom)[i]] * nHtot);
+  const int lH = ldc(M.atom_level_ptr)[0], nlH = ldc(M.atom_level_ptr)[1] - lH;
   const double nH0 = nstar0n[0] * fracn[0];
   const double np = (level_ratio(M, lH, nlH - 1, T, cNe_T) * nstar0n[0]) * fracn[0];
   const double nHmin = ne * (nnuc[0] * PhiHmin);
@@ -350,11 +350,11 @@ __device__ void hse_layer(const DevModel &M, const DevHse &H, double T, double n
   double nHe0 = 0.0;
   if (H.he_atom >= 0 && H.ray_He > 0.0) {
     const int ia = H.he_atom;
-    nHe0 = (M.atom_abund[ia] * nHtot) / hse_level_sum(lg + (int64_t) NN * (HSE_NDZ + 1) * ss, ss,
+    nHe0 = (ldc(M.atom_abund)[ia] * nHtot) / hse_level_sum(lg + (int64_t) NN * (HSE_NDZ + 1) * ss, ss,
                                                        cNe_T);
 #pragma unroll
     for (int i = 0; i < NN; i++)
-      if (M.nucl_atom[i] == ia) nHe0 = nstar0n[i] * fracn[i];
+      if (ldc(M.nucl_atom)[i] == ia) nHe0 = nstar0n[i] * fracn[i];
   }
   const double n0H = nstar0n[0], frH = fracn[0];
   auto hpop = [&](int i) -> double {
@@ -403,7 +403,7 @@ k_hse(const DevModel *__restrict__ Mp, DevHse H, int lpc, int ncol, int Nmax,
     double *lg = sc_t + (int64_t) H.nslot_stage * ss;
 #pragma unroll
     for (int i = 0; i < NN; i++)
-      hse_level_groups(M, M.nucl_atom[i], Tl, lg + (int64_t) i * (HSE_NDZ + 1) * ss, ss);
+      hse_level_groups(M, ldc(M.nucl_atom)[i], Tl, lg + (int64_t) i * (HSE_NDZ + 1) * ss, ss);
     if (H.he_atom >= 0 && H.ray_He > 0.0)
       hse_level_groups(M, H.he_atom, Tl, lg + (int64_t) NN * (HSE_NDZ + 1) * ss, ss);
   };

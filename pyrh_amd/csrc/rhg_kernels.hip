@@ -64,11 +64,11 @@ __device__ __forceinline__ void continuum_point(const DevModel &M, int ls, const
   chi_ai *= lc[RHG_LC_HMIN_FUDGE];
   eta_ai *= lc[RHG_LC_HMIN_FUDGE];
   if (lc[RHG_LC_OH_EIDX] >= 0.0) {
-    if (T < M.oh_T[0] || T > M.oh_T[NTOH - 1]) { chi = 0.0; eta = 0.0; }
+    if (T < ldc(M.oh_T)[0] || T > ldc(M.oh_T)[NTOH - 1]) { chi = 0.0; eta = 0.0; }
     else {
       int i2 = locate_asc(NTOH, M.oh_T, T);
       double ti = (i2 == NTOH - 1) ? (double) i2
-                                   : (double) i2 + (T - M.oh_T[i2]) / (M.oh_T[i2 + 1] - M.oh_T[i2]);
+                                   : (double) i2 + (T - ldc(M.oh_T)[i2]) / (ldc(M.oh_T)[i2 + 1] - ldc(M.oh_T)[i2]);
       double kappa = exp(LG10 * bilinear(NTOH, NEOH, M.oh_cross, ti, lc[RHG_LC_OH_EIDX])) *
                      (CM_TO_M * CM_TO_M);
       stim = exp(-lc[RHG_LC_HC_KLA] / T);
@@ -80,11 +80,11 @@ __device__ __forceinline__ void continuum_point(const DevModel &M, int ls, const
     eta_ai += eta;
   }
   if (lc[RHG_LC_CH_EIDX] >= 0.0) {
-    if (T < M.ch_T[0] || T > M.ch_T[NTCH - 1]) { chi = 0.0; eta = 0.0; }
+    if (T < ldc(M.ch_T)[0] || T > ldc(M.ch_T)[NTCH - 1]) { chi = 0.0; eta = 0.0; }
     else {
       int i2 = locate_asc(NTCH, M.ch_T, T);
       double ti = (i2 == NTCH - 1) ? (double) i2
-                                   : (double) i2 + (T - M.ch_T[i2]) / (M.ch_T[i2 + 1] - M.ch_T[i2]);
+                                   : (double) i2 + (T - ldc(M.ch_T)[i2]) / (ldc(M.ch_T)[i2 + 1] - ldc(M.ch_T)[i2]);
       double kappa = exp(LG10 * bilinear(NTCH, NECH, M.ch_cross, ti, lc[RHG_LC_CH_EIDX])) *
                      (CM_TO_M * CM_TO_M);
       stim = exp(-lc[RHG_LC_HC_KLA] / T);
@@ -96,16 +96,16 @@ __device__ __forceinline__ void continuum_point(const DevModel &M, int ls, const
     eta_ai += eta;
   }
   // bound-free: H continua first, then metals
-  const int cp0 = M.lam_cont_ptr[ls], cp1 = M.lam_cont_ptr[ls + 1];
+  const int cp0 = ldc(M.lam_cont_ptr)[ls], cp1 = ldc(M.lam_cont_ptr)[ls + 1];
   double expla = exp(-lc[RHG_LC_BF_HC_KLA] / T);
   {
     bool any = false;
     chi = eta = 0.0;
     for (int e = cp0; e < cp1; e++) {
-      int ci = M.lam_cont_idx[e];
-      if (M.cont_atom[ci] != 0) continue;
+      int ci = ldc(M.lam_cont_idx)[e];
+      if (ldc(M.cont_atom)[ci] != 0) continue;
       any = true;
-      double sigma = M.lam_cont_alpha[e];
+      double sigma = ldc(M.lam_cont_alpha)[e];
       double ni = st[cont_field(ci) * fs];
       chi += sigma * (1.0 - expla) * ni;
       eta += lc[RHG_LC_BF_TWOHNU3_C2] * expla * sigma * ni;
@@ -141,9 +141,9 @@ __device__ __forceinline__ void continuum_point(const DevModel &M, int ls, const
   }
   chi = eta = 0.0;
   for (int e = cp0; e < cp1; e++) {
-    int ci = M.lam_cont_idx[e];
-    if (M.cont_atom[ci] == 0) continue;
-    double alpha_la = M.lam_cont_alpha[e];
+    int ci = ldc(M.lam_cont_idx)[e];
+    if (ldc(M.cont_atom)[ci] == 0) continue;
+    double alpha_la = ldc(M.lam_cont_alpha)[e];
     double ni = st[cont_field(ci) * fs];
     chi += alpha_la * (1.0 - expla) * ni;
     eta += lc[RHG_LC_BF_TWOHNU3_C2] * expla * alpha_la * ni;
@@ -206,11 +206,11 @@ k_state(const DevModel *__restrict__ Mp, const double *__restrict__ atmos, const
   double nstar0n[NN], fn0[NN];
 #pragma unroll
   for (int i = 0; i < NN; i++) {
-    const int ia = M.nucl_atom[i], l0 = M.atom_level_ptr[ia], nl = M.atom_level_ptr[ia + 1] - l0;
-    nstar0n[i] = (M.atom_abund[ia] * nHtot) / atom_sum(M, l0, nl, T, cNe_T);
-    double ntot = M.atom_abund[ia] * nHtot, f = 0.0;
+    const int ia = ldc(M.nucl_atom)[i], l0 = ldc(M.atom_level_ptr)[ia], nl = ldc(M.atom_level_ptr)[ia + 1] - l0;
+    nstar0n[i] = (ldc(M.atom_abund)[ia] * nHtot) / atom_sum(M, l0, nl, T, cNe_T);
+    double ntot = ldc(M.atom_abund)[ia] * nHtot, f = 0.0;
     for (int j = 0; j < nl; j++) {
-      if (M.level_stage[l0 + j] > 0) break;
+      if (ldc(M.level_stage)[l0 + j] > 0) break;
       f += (j == 0) ? nstar0n[i] : level_ratio(M, l0, j, T, cNe_T) * nstar0n[i];
     }
     fn0[i] = f / ntot;
@@ -224,14 +224,14 @@ k_state(const DevModel *__restrict__ Mp, const double *__restrict__ atmos, const
   chem_eq<NN>(M, T, ne, nHtot, fn0, nnuc, lds_phi, lds_nmol, lds_saha, lstride, PhiHmin);
   double fracn[NN];
 #pragma unroll
-  for (int i = 0; i < NN; i++) fracn[i] = nnuc[i] / (M.atom_abund[M.nucl_atom[i]] * nHtot);
+  for (int i = 0; i < NN; i++) fracn[i] = nnuc[i] / (ldc(M.atom_abund)[ldc(M.nucl_atom)[i]] * nHtot);
   const double nHmin = ne * (nnuc[0] * PhiHmin);
   st[RHG_CF_NHMIN * fs] = nHmin;
   st[RHG_CF_NH2 * fs] = M.mol_h2 >= 0 ? lds_nmol[M.mol_h2 * lstride] : 0.0;
   st[RHG_CF_NOH * fs] = M.mol_oh >= 0 ? lds_nmol[M.mol_oh * lstride] : 0.0;
   st[RHG_CF_NCH * fs] = M.mol_ch >= 0 ? lds_nmol[M.mol_ch * lstride] : 0.0;
   // hydrogen is atom 0 and nucleus 0
-  const int lH = M.atom_level_ptr[0], nlH = M.atom_level_ptr[1] - lH;
+  const int lH = ldc(M.atom_level_ptr)[0], nlH = ldc(M.atom_level_ptr)[1] - lH;
   const double nH0 = nstar0n[0] * fracn[0];
   const double np = (level_ratio(M, lH, nlH - 1, T, cNe_T) * nstar0n[0]) * fracn[0];
   st[RHG_CF_NH0 * fs] = nH0;
@@ -246,23 +246,23 @@ k_state(const DevModel *__restrict__ Mp, const double *__restrict__ atmos, const
     int cur = -1;
     double nst0 = 0.0, fr = 1.0;
     for (int ci = 0; ci < M.ncont; ci++) {
-      const int ia = M.cont_atom[ci], l0 = M.atom_level_ptr[ia];
+      const int ia = ldc(M.cont_atom)[ci], l0 = ldc(M.atom_level_ptr)[ia];
       if (ia != cur) {
         cur = ia;
         int inuc = -1;
 #pragma unroll
         for (int i = 0; i < NN; i++)
-          if (M.nucl_atom[i] == ia) inuc = i;
+          if (ldc(M.nucl_atom)[i] == ia) inuc = i;
         if (inuc >= 0) {
 #pragma unroll
           for (int i = 0; i < NN; i++)
             if (i == inuc) { nst0 = nstar0n[i]; fr = fracn[i]; }
         } else {
-          nst0 = (M.atom_abund[ia] * nHtot) / atom_sum(M, l0, M.atom_level_ptr[ia + 1] - l0, T, cNe_T);
+          nst0 = (ldc(M.atom_abund)[ia] * nHtot) / atom_sum(M, l0, ldc(M.atom_level_ptr)[ia + 1] - l0, T, cNe_T);
           fr = 1.0;
         }
       }
-      const int i = M.cont_i[ci];
+      const int i = ldc(M.cont_i)[ci];
       st[cont_field(ci) * fs] = ((i == 0) ? nst0 : level_ratio(M, l0, i, T, cNe_T) * nst0) * fr;
     }
   }
@@ -271,12 +271,12 @@ k_state(const DevModel *__restrict__ Mp, const double *__restrict__ atmos, const
   // rh/kurucz.c:662-677 and rh/kurucz.c:743-775
   const int lbase = line_field_base(M.ncont);
   for (int l = 0; l < M.nline; l++) {
-    int ie = M.line_el[l], stg = M.line_stage[l];
+    int ie = ldc(M.line_el)[l], stg = ldc(M.line_stage)[l];
     // stage populations of the element, rh/ltepops.c:116-164
     const double C1 = (HPLANCK / (2.0 * PI * M_ELECTRON)) * (HPLANCK / KBOLTZMANN);
-    const int npf = M.npf, ns = M.el_nstage[ie];
-    const double *lp = M.el_logpf + M.el_pf_ptr[ie];
-    const double *ip = M.el_ionpot + M.el_pf_ptr[ie];
+    const int npf = M.npf, ns = ldc(M.el_nstage)[ie];
+    const double *lp = M.el_logpf + ldc(M.el_pf_ptr)[ie];
+    const double *ip = M.el_ionpot + ldc(M.el_pf_ptr)[ie];
     double CT_ne = 2.0 * pow(C1 / T, -1.5) / ne, sum = 1.0, Uk, Ukp1, nst = 1.0, nstage = 1.0;
     Uk = linear1(npf, M.Tpf, lp, T);
     for (int i = 1; i < ns; i++) {
@@ -286,26 +286,26 @@ k_state(const DevModel *__restrict__ Mp, const double *__restrict__ atmos, const
       if (i == stg) nstage = nst;
       Uk = Ukp1;
     }
-    double n0 = M.el_abund[ie] * nHtot / sum;
+    double n0 = ldc(M.el_abund)[ie] * nHtot / sum;
     nstage = (stg == 0) ? n0 : nstage * n0;
     double pf = linear1(npf, M.Tpf, lp + stg * npf, T);
     double kT = 1.0 / (KBOLTZMANN * T);
-    double ni_gi = nstage * exp(-M.line_Ei[l] * kT - pf);
-    double nj_gj = ni_gi * exp(-M.line_hc_la[l] * kT);
-    double chi_l = M.line_Bijhc4pi[l] * (ni_gi - nj_gj);
-    double eta_l = M.line_Bijhc4pi[l] * M.line_twohnu3_c2[l] * nj_gj;
-    double vtherm = 2.0 * KBOLTZMANN / (AMU * M.el_weight[ie]);
+    double ni_gi = nstage * exp(-ldc(M.line_Ei)[l] * kT - pf);
+    double nj_gj = ni_gi * exp(-ldc(M.line_hc_la)[l] * kT);
+    double chi_l = ldc(M.line_Bijhc4pi)[l] * (ni_gi - nj_gj);
+    double eta_l = ldc(M.line_Bijhc4pi)[l] * ldc(M.line_twohnu3_c2)[l] * nj_gj;
+    double vtherm = 2.0 * KBOLTZMANN / (AMU * ldc(M.el_weight)[ie]);
     double vbroad = sqrt(vtherm * T + vturb * vturb);
     double adamp = 0.0;
-    if (M.line_Grad[l] != 0.0) {
+    if (ldc(M.line_Grad)[l] != 0.0) {
       double GvdW;
-      switch (M.line_vdw[l]) {
-        case RHG_VDW_UNSOLD: GvdW = M.line_cross[l] * pow(T, 0.3); break;
-        case RHG_VDW_BARKLEM: GvdW = M.line_cross[l] * pow(T, (1.0 - M.line_alpha[l]) / 2.0); break;
-        default: GvdW = M.line_GvdW[l]; break;
+      switch (ldc(M.line_vdw)[l]) {
+        case RHG_VDW_UNSOLD: GvdW = ldc(M.line_cross)[l] * pow(T, 0.3); break;
+        case RHG_VDW_BARKLEM: GvdW = ldc(M.line_cross)[l] * pow(T, (1.0 - ldc(M.line_alpha)[l]) / 2.0); break;
+        default: GvdW = ldc(M.line_GvdW)[l]; break;
       }
-      adamp = (M.line_Grad[l] + M.line_GStark[l] * ne + GvdW * (nHtot - np)) *
-              (M.line_lambda0[l] * NM_TO_M) / (4.0 * PI * vbroad);
+      adamp = (ldc(M.line_Grad)[l] + ldc(M.line_GStark)[l] * ne + GvdW * (nHtot - np)) *
+              (ldc(M.line_lambda0)[l] * NM_TO_M) / (4.0 * PI * vbroad);
     }
     st[(lbase + 4 * l) * fs] = chi_l;
     st[(lbase + 4 * l + 1) * fs] = eta_l;
@@ -353,7 +353,7 @@ __global__ void k_geometry(const DevModel *__restrict__ Mp, const int32_t *__res
       break;
     }
   moving_out[ic] = moving;
-  if (M.lam_line_ptr[M.ref_index + 1] > M.lam_line_ptr[M.ref_index]) {
+  if (ldc(M.lam_line_ptr)[M.ref_index + 1] > ldc(M.lam_line_ptr)[M.ref_index]) {
     for (int k = 0; k < N; k++) {
       double lchi, leta, q[3], e[3];
       line_point<double>(M, M.ref_index, st0 + (int64_t) k * ncol, fs, mu, lchi, q, leta, e, moving);
@@ -529,16 +529,16 @@ k_continuum(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep, i
   const double sqrt_theta = sqrt(THETA0 * iT), sff_T = sigma_ff / sqrt(T);
   const double hff_fac = ne * np;
   const double y_fac = T * (1.0 / (HPLANCK * CLIGHT));
-  const bool oh_in = !(T < M.oh_T[0] || T > M.oh_T[NTOH - 1]);
-  const bool ch_in = !(T < M.ch_T[0] || T > M.ch_T[NTCH - 1]);
+  const bool oh_in = !(T < ldc(M.oh_T)[0] || T > ldc(M.oh_T)[NTOH - 1]);
+  const bool ch_in = !(T < ldc(M.ch_T)[0] || T > ldc(M.ch_T)[NTCH - 1]);
   double ti_oh = 0.0, ti_ch = 0.0;
   if (oh_in) {
     int i2 = locate_asc(NTOH, M.oh_T, T);
-    ti_oh = (i2 == NTOH - 1) ? (double) i2 : (double) i2 + (T - M.oh_T[i2]) / (M.oh_T[i2 + 1] - M.oh_T[i2]);
+    ti_oh = (i2 == NTOH - 1) ? (double) i2 : (double) i2 + (T - ldc(M.oh_T)[i2]) / (ldc(M.oh_T)[i2 + 1] - ldc(M.oh_T)[i2]);
   }
   if (ch_in) {
     int i2 = locate_asc(NTCH, M.ch_T, T);
-    ti_ch = (i2 == NTCH - 1) ? (double) i2 : (double) i2 + (T - M.ch_T[i2]) / (M.ch_T[i2 + 1] - M.ch_T[i2]);
+    ti_ch = (i2 == NTCH - 1) ? (double) i2 : (double) i2 + (T - ldc(M.ch_T)[i2]) / (ldc(M.ch_T)[i2 + 1] - ldc(M.ch_T)[i2]);
   }
 
   for (int t = 0; t < M.ntile; t++) {
@@ -547,9 +547,10 @@ k_continuum(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep, i
       double bf[BF_TILE];
 #pragma unroll
       for (int j = 0; j < BF_TILE; j++) bf[j] = 0.0;
-      for (int e = M.tile_ptr[t]; e < M.tile_ptr[t + 1]; e++) {
-        const double ni = st[cont_field(M.tile_cont[e]) * fs];
-        const double *al = M.tile_alpha + (int64_t) e * BF_TILE;
+      const int e0 = ldc(M.tile_ptr)[t], e1 = ldc(M.tile_ptr)[t + 1];
+      for (int e = e0; e < e1; e++) {
+        const double ni = st[cont_field(ldc(M.tile_cont)[e]) * fs];
+        const CDouble *al = ldc(M.tile_alpha) + (int64_t) e * BF_TILE;
 #pragma unroll
         for (int j = 0; j < BF_TILE; j++) bf[j] += al[j] * ni;
       }
@@ -561,7 +562,7 @@ k_continuum(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep, i
     for (int j = 0; j < nj; j++) {
       const int io = io0 + j;
       const double bfj = bf_lds[j][threadIdx.x];
-      const double *lc = M.lamconst + (int64_t) out_ls[io] * RHG_LC_NCONST;
+      const CDouble *lc = ldc(M.lamconst) + (int64_t) ldc(out_ls)[io] * RHG_LC_NCONST;
       const double hc_kla = lc[RHG_LC_HC_KLA], twohnu3 = lc[RHG_LC_TWOHNU3_C2];
       const double x = hc_kla * iT;
       // Planck function, rh/planck.c:38-63, and the stimulated-emission

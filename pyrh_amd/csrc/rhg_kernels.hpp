@@ -12,6 +12,15 @@ namespace rhg {
 
 // Device mirror of rhg_model_t: the same fields with device pointers
 // (filled by the host from the caller's model, see rhg_api.hip).
+// Read-only tables through the constant address space: a load with a
+// wave-uniform index becomes a scalar load (through the scalar cache)
+// instead of a vector memory access per lane.  Only for data no kernel
+// writes while it runs.
+typedef __attribute__((address_space(4))) const double CDouble;
+typedef __attribute__((address_space(4))) const int32_t CInt;
+__device__ __forceinline__ CDouble *ldc(const double *p) { return (CDouble *) p; }
+__device__ __forceinline__ CInt *ldc(const int32_t *p) { return (CInt *) p; }
+
 struct DevModel {
   int stokes_mode, magneto_optical;
   double vmacro_tresh, vmicro_char, wght_per_H, totalAbund, gravity;
@@ -106,9 +115,9 @@ __host__ __device__ __forceinline__ int state_nfield(int ncont, int nline) {
 // rh/ltepops.c:87-100
 __device__ __forceinline__ double level_ratio(const DevModel &M, int l0, int i, double T,
                                               double cNe_T) {
-  double dE = M.level_E[l0 + i] - M.level_E[l0];
-  double gi0 = M.level_g[l0 + i] / M.level_g[l0];
-  int dZ = M.level_stage[l0 + i] - M.level_stage[l0];
+  double dE = ldc(M.level_E)[l0 + i] - ldc(M.level_E)[l0];
+  double gi0 = ldc(M.level_g)[l0 + i] / ldc(M.level_g)[l0];
+  int dZ = ldc(M.level_stage)[l0 + i] - ldc(M.level_stage)[l0];
   double dE_kT = dE / (KBOLTZMANN * T);
   double r = gi0 * exp(-dE_kT);
   for (int m = 1; m <= dZ; m++) r /= cNe_T;
@@ -133,16 +142,16 @@ struct LineTabG {
   int p0, n;
   // entries p0 .. p0 + n - 1 of the model's wavelength line lists
   __device__ __forceinline__ LineTabG(const DevModel &M_, int p0_, int n_) : M(M_), p0(p0_), n(n_) {}
-  __device__ __forceinline__ int line(int i) const { return M.lam_line_idx[p0 + i]; }
-  __device__ __forceinline__ double dv(int i) const { return M.lam_line_dv[p0 + i]; }
-  __device__ __forceinline__ bool damped(int i) const { return M.line_Grad[line(i)] != 0.0; }
-  __device__ __forceinline__ bool polar(int i) const { return M.line_polar[line(i)] != 0; }
-  __device__ __forceinline__ double larmor(int i) const { return M.line_larmor[line(i)]; }
-  __device__ __forceinline__ int c0(int i) const { return M.line_comp_ptr[line(i)]; }
-  __device__ __forceinline__ int c1(int i) const { return M.line_comp_ptr[line(i) + 1]; }
-  __device__ __forceinline__ double shift(int c) const { return M.comp_shift[c]; }
-  __device__ __forceinline__ double strength(int c) const { return M.comp_strength[c]; }
-  __device__ __forceinline__ int q(int c) const { return M.comp_q[c]; }
+  __device__ __forceinline__ int line(int i) const { return ldc(M.lam_line_idx)[p0 + i]; }
+  __device__ __forceinline__ double dv(int i) const { return ldc(M.lam_line_dv)[p0 + i]; }
+  __device__ __forceinline__ bool damped(int i) const { return ldc(M.line_Grad)[line(i)] != 0.0; }
+  __device__ __forceinline__ bool polar(int i) const { return ldc(M.line_polar)[line(i)] != 0; }
+  __device__ __forceinline__ double larmor(int i) const { return ldc(M.line_larmor)[line(i)]; }
+  __device__ __forceinline__ int c0(int i) const { return ldc(M.line_comp_ptr)[line(i)]; }
+  __device__ __forceinline__ int c1(int i) const { return ldc(M.line_comp_ptr)[line(i) + 1]; }
+  __device__ __forceinline__ double shift(int c) const { return ldc(M.comp_shift)[c]; }
+  __device__ __forceinline__ double strength(int c) const { return ldc(M.comp_strength)[c]; }
+  __device__ __forceinline__ int q(int c) const { return ldc(M.comp_q)[c]; }
 };
 
 // A wavelength's line list staged in LDS (one copy per wavefront; every
@@ -192,17 +201,17 @@ __device__ __forceinline__ LineTabL stage_lines(const DevModel &M, int ls, char 
   double *shift = (double *) (smem + (size_t) maxl * sizeof(LineRecL));
   double *strength = shift + maxc;
   int *q = (int *) (strength + maxc);
-  const int p0 = M.lam_line_ptr[ls], n = M.lam_line_ptr[ls + 1] - p0;
+  const int p0 = ldc(M.lam_line_ptr)[ls], n = ldc(M.lam_line_ptr)[ls + 1] - p0;
   const int lane = threadIdx.x;
   for (int i = lane; i < n; i += 64) {
-    const int l = M.lam_line_idx[p0 + i];
-    const bool damped = M.line_Grad[l] != 0.0, polar = M.line_polar[l] != 0;
-    rec[i].dv = M.lam_line_dv[p0 + i];
-    rec[i].larmor = M.line_larmor[l];
+    const int l = ldc(M.lam_line_idx)[p0 + i];
+    const bool damped = ldc(M.line_Grad)[l] != 0.0, polar = ldc(M.line_polar)[l] != 0;
+    rec[i].dv = ldc(M.lam_line_dv)[p0 + i];
+    rec[i].larmor = ldc(M.line_larmor)[l];
     rec[i].line = l;
     rec[i].flags = (damped ? 1 : 0) | (polar ? 2 : 0);
     // only a damped polarizable line uses its components
-    rec[i].c1 = (damped && polar) ? M.line_comp_ptr[l + 1] - M.line_comp_ptr[l] : 0;
+    rec[i].c1 = (damped && polar) ? ldc(M.line_comp_ptr)[l + 1] - ldc(M.line_comp_ptr)[l] : 0;
   }
   __syncthreads();
   // offsets: c0(i) = sum of the counts before slot i
@@ -216,11 +225,11 @@ __device__ __forceinline__ LineTabL stage_lines(const DevModel &M, int ls, char 
   __syncthreads();
   for (int i = 0; i < n; i++) {
     const int a = rec[i].c0, cnt = rec[i].c1 - a;
-    const int g0 = M.line_comp_ptr[rec[i].line];
+    const int g0 = ldc(M.line_comp_ptr)[rec[i].line];
     for (int c = lane; c < cnt; c += 64) {
-      shift[a + c] = M.comp_shift[g0 + c];
-      strength[a + c] = M.comp_strength[g0 + c];
-      q[a + c] = M.comp_q[g0 + c];
+      shift[a + c] = ldc(M.comp_shift)[g0 + c];
+      strength[a + c] = ldc(M.comp_strength)[g0 + c];
+      q[a + c] = ldc(M.comp_q)[g0 + c];
     }
   }
   __syncthreads();
@@ -358,8 +367,8 @@ template <typename T, typename TV = T, bool MO = false, typename Field>
 __device__ __forceinline__ void line_point_f(const DevModel &M, int ls, const Field &F, T mu,
                                              T &chiI, T *chiQ, T &etaI, T *etaQ, int moving,
                                              T *chip = nullptr) {
-  const int p0 = M.lam_line_ptr[ls];
-  line_point_t<T, TV, MO>(LineTabG(M, p0, M.lam_line_ptr[ls + 1] - p0), M.ncont, F, mu, chiI, chiQ,
+  const int p0 = ldc(M.lam_line_ptr)[ls];
+  line_point_t<T, TV, MO>(LineTabG(M, p0, ldc(M.lam_line_ptr)[ls + 1] - p0), M.ncont, F, mu, chiI, chiQ,
                           etaI, etaQ, moving, chip);
 }
 
@@ -376,20 +385,20 @@ __device__ __forceinline__ void line_point(const DevModel &M, int ls, const doub
 // line_point(), with counts in place of arithmetic (slots: RHG_WORK_*).
 __device__ __forceinline__ void line_census(const DevModel &M, int ls, const double *st, int64_t fs,
                                             double mu, int moving, int32_t *cnt) {
-  const int p0 = M.lam_line_ptr[ls], p1 = M.lam_line_ptr[ls + 1];
+  const int p0 = ldc(M.lam_line_ptr)[ls], p1 = ldc(M.lam_line_ptr)[ls + 1];
   const int lb = line_field_base(M.ncont);
   const double vz = st[RHG_CF_VLOS * fs];
   for (int e = p0; e < p1; e++) {
-    const int l = M.lam_line_idx[e];
+    const int l = ldc(M.lam_line_idx)[e];
     const double adamp = st[(lb + 4 * l + 2) * fs], ivb = st[(lb + 4 * l + 3) * fs];
-    double v = M.lam_line_dv[e] * ivb;
+    double v = ldc(M.lam_line_dv)[e] * ivb;
     if (moving) v += (mu * vz) * ivb;
     cnt[RHG_WORK_LINE]++;
-    if (M.line_Grad[l] != 0.0) {
-      if (M.line_polar[l] != 0) {
-        const double vB = M.line_larmor[l] * st[RHG_CF_B * fs] * ivb;
-        for (int nz = M.line_comp_ptr[l]; nz < M.line_comp_ptr[l + 1]; nz++)
-          cnt[RHG_WORK_VOIGT_I + humlicek_region<double>(adamp, v - M.comp_shift[nz] * vB)]++;
+    if (ldc(M.line_Grad)[l] != 0.0) {
+      if (ldc(M.line_polar)[l] != 0) {
+        const double vB = ldc(M.line_larmor)[l] * st[RHG_CF_B * fs] * ivb;
+        for (int nz = ldc(M.line_comp_ptr)[l]; nz < ldc(M.line_comp_ptr)[l + 1]; nz++)
+          cnt[RHG_WORK_VOIGT_I + humlicek_region<double>(adamp, v - ldc(M.comp_shift)[nz] * vB)]++;
       } else {
         cnt[RHG_WORK_ARMSTRONG]++;
       }

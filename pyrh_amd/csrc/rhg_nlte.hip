@@ -95,8 +95,8 @@ __global__ void k_nlte_pre(const DevModel *__restrict__ Mp, const DevNlte *__res
 
   double nHe = 0.0;
   if (A.he_atom >= 0) {
-    const int l0 = M.atom_level_ptr[A.he_atom], nl = M.atom_level_ptr[A.he_atom + 1] - l0;
-    nHe = (M.atom_abund[A.he_atom] * nHtot) / atom_sum(M, l0, nl, T, cNe_T);
+    const int l0 = ldc(M.atom_level_ptr)[A.he_atom], nl = ldc(M.atom_level_ptr)[A.he_atom + 1] - l0;
+    nHe = (ldc(M.atom_abund)[A.he_atom] * nHtot) / atom_sum(M, l0, nl, T, cNe_T);
   }
   B.nHe[pk] = nHe;
   {
@@ -105,12 +105,12 @@ __global__ void k_nlte_pre(const DevModel *__restrict__ Mp, const DevNlte *__res
     sca += A.ray_H[lr] * nH0;
     if (A.he_atom >= 0) sca += A.ray_He[lr] * nHe;
     sca += A.ray_H2[lr] * nH2;
-    sca *= M.lamconst[(int64_t) lr * RHG_LC_NCONST + RHG_LC_SCATT_FUDGE];
+    sca *= ldc(M.lamconst)[(int64_t) lr * RHG_LC_NCONST + RHG_LC_SCATT_FUDGE];
     st[RHG_CF_HEIGHT * fs] += sca;
   }
 
-  const int nl = A.nl, a = A.atom_index, l0 = M.atom_level_ptr[a];
-  const double ntotal = M.atom_abund[a] * nHtot;
+  const int nl = A.nl, a = A.atom_index, l0 = ldc(M.atom_level_ptr)[a];
+  const double ntotal = ldc(M.atom_abund)[a] * nHtot;
   const double n0 = ntotal / atom_sum(M, l0, nl, T, cNe_T);
   for (int i = 0; i < nl; i++) {
     const double ns = (i == 0) ? n0 : level_ratio(M, l0, i, T, cNe_T) * n0;
@@ -213,7 +213,7 @@ __global__ void k_nlte_bg(const DevModel *__restrict__ Mp, const DevNlte *__rest
     if (A.he_atom >= 0) sca += A.ray_He[ls] * nHe;
     sca += A.ray_H2[ls] * nH2;
     // the scattering fudge factor, rh/background.c:453-462
-    sca *= M.lamconst[(int64_t) ls * RHG_LC_NCONST + RHG_LC_SCATT_FUDGE];
+    sca *= ldc(M.lamconst)[(int64_t) ls * RHG_LC_NCONST + RHG_LC_SCATT_FUDGE];
     const int64_t q = (int64_t) io * fs + (int64_t) k * ncol + ic;
     B.csca[q] = sca;
     B.cchi[q] += sca;
@@ -242,7 +242,7 @@ __global__ void k_nlte_profile(const DevModel *__restrict__ Mp, const DevNlte *_
   double wphi = 0.0;
   for (int la = 0; la < A.tr_nlam[t]; la++) {
     const int ls = A.tr_nblue[t] + la;
-    const double lamla = M.lamconst[(int64_t) ls * RHG_LC_NCONST + RHG_LC_LAMBDA];
+    const double lamla = ldc(M.lamconst)[(int64_t) ls * RHG_LC_NCONST + RHG_LC_LAMBDA];
     const double wl = A.tr_wlam[A.tr_ptr[t] + la];
     const double v = (lamla - lambda0 - 0.0) * CLIGHT / (vb * lambda0);
     for (int mu = 0; mu < nrays; mu++) {
@@ -419,7 +419,7 @@ k_nlte_bezier(const DevModel *__restrict__ Mp, const DevNlte *__restrict__ Ap, N
   const double *Jold = B.J + par * jb + wo;
   double *Jnew = B.J + (1 - par) * jb + wo;
   const double *chic = B.cchi + wo, *etac = B.ceta + wo, *scac = B.csca + wo;
-  const double lam = M.lamconst[(int64_t) ls * RHG_LC_NCONST + RHG_LC_LAMBDA];
+  const double lam = ldc(M.lamconst)[(int64_t) ls * RHG_LC_NCONST + RHG_LC_LAMBDA];
   const double hc_k = HC / (KBOLTZMANN * NM_TO_M);
   SetInfo<MS> si;
   set_info<MS>(A, ls, lam, si);
@@ -681,7 +681,7 @@ k_nlte_feautrier(const DevModel *__restrict__ Mp, const DevNlte *__restrict__ Ap
   const double *Jold = B.J + par * jb + wo;
   double *Jnew = B.J + (1 - par) * jb + wo;
   const double *chic = B.cchi + wo, *etac = B.ceta + wo, *scac = B.csca + wo;
-  const double lam = M.lamconst[(int64_t) ls * RHG_LC_NCONST + RHG_LC_LAMBDA];
+  const double lam = ldc(M.lamconst)[(int64_t) ls * RHG_LC_NCONST + RHG_LC_LAMBDA];
   const double hc_k = HC / (KBOLTZMANN * NM_TO_M);
   SetInfo<MS> si;
   set_info<MS>(A, ls, lam, si);
@@ -847,7 +847,7 @@ k_nlte_emergent(const DevModel *__restrict__ Mp, const DevNlte *__restrict__ Ap,
   const int64_t wo = (int64_t) io * fs, jb = (int64_t) B.nout * fs;
   const double *Jl = B.J + B.jpar[ic] * jb + wo;
   const double *chic = B.cchi + wo, *etac = B.ceta + wo, *scac = B.csca + wo;
-  const double lam = M.lamconst[(int64_t) ls * RHG_LC_NCONST + RHG_LC_LAMBDA];
+  const double lam = ldc(M.lamconst)[(int64_t) ls * RHG_LC_NCONST + RHG_LC_LAMBDA];
   const double hc_k = HC / (KBOLTZMANN * NM_TO_M);
   SetInfo<MS> si;
   set_info<MS>(A, ls, lam, si);

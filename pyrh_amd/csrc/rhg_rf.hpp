@@ -102,7 +102,7 @@ k_rf_forward(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep,
   RF_SETUP();
   const R *cc = cchi + (int64_t) io * Nmax * ncol + ic;
   const R *ce = ceta + (int64_t) io * Nmax * ncol + ic;
-  const int has_line = M.lam_line_ptr[ls + 1] > M.lam_line_ptr[ls];
+  const int has_line = ldc(M.lam_line_ptr)[ls + 1] > ldc(M.lam_line_ptr)[ls];
   const double imu = 1.0 / mu_d;
   const double B2a = planck(Tc[(int64_t) (N - 2) * ncol], lc[RHG_LC_HC_KLA], lc[RHG_LC_TWOHNU3_C2]);
   const double B2b = planck(Tc[(int64_t) (N - 1) * ncol], lc[RHG_LC_HC_KLA], lc[RHG_LC_TWOHNU3_C2]);
@@ -194,7 +194,7 @@ k_rf_reverse(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep,
   RF_SETUP();
   const R *cc = cchi + (int64_t) io * Nmax * ncol + ic;
   const R *ce = ceta + (int64_t) io * Nmax * ncol + ic;
-  const int has_line = M.lam_line_ptr[ls + 1] > M.lam_line_ptr[ls];
+  const int has_line = ldc(M.lam_line_ptr)[ls + 1] > ldc(M.lam_line_ptr)[ls];
   const double imu = 1.0 / mu_d;
   PbarRing<R> ring{ring_lds, (int) threadIdx.x, 1};
   for (int d = 0; d < 4; d++)
@@ -320,7 +320,7 @@ k_rf_chain(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep,
   RF_SETUP();
   using D = DualN<double, NP>;
   using DV = DualN<R, NP>;
-  const int has_line = M.lam_line_ptr[ls + 1] > M.lam_line_ptr[ls];
+  const int has_line = ldc(M.lam_line_ptr)[ls + 1] > ldc(M.lam_line_ptr)[ls];
   const D muD = D(mu_d);
   const int64_t npts = (int64_t) Nmax * ncol;
   const int64_t tan_stride = (int64_t) state_nfield(M.ncont, M.nline) * npts;
