@@ -181,7 +181,7 @@ def _cpu_baseline(setup, wave, atm, nworkers):
                       f"{dt:.1f} s wall"}
 
 
-def _err_stats(x, ref, thr):
+def _err_stats(x, ref, thr, names=None):
     """Error of x against ref per parameter, relative to each (parameter,
     depth, column)'s largest |ref| over Stokes and wavelength; arrays in the
     library's layout [param, depth, 4, nout, ncol].  Median and 99th
@@ -189,7 +189,7 @@ def _err_stats(x, ref, thr):
     import torch
     from pyrh_amd.engine import RF_PARAMS
     out = {}
-    for ip, name in enumerate(RF_PARAMS):
+    for ip, name in enumerate(names or RF_PARAMS):
         r = ref[ip]
         scale = r.abs().amax(dim=(1, 2), keepdim=True)
         e = (x[ip].double() - r).abs() / scale.clamp_min(1e-300)
@@ -262,6 +262,20 @@ def _rf_leg(eng, W, torch, dev, reps):
     finite_differences()
     t_fd = time.perf_counter() - t0
 
+    # the conditioning reference: the FP64 adjoint of the same columns with
+    # every input scaled by (1 + 1e-7 N(0, 1)), float's relative precision.
+    # Its departures from the FP64 adjoint (points next to the kinks of the
+    # limiter, clamps and Humlicek boundaries, where a rounding-size change
+    # moves the derivative) are what the FP32 form's departures compare to.
+    rng = np.random.default_rng(11)
+    d_atm.mul_(torch.from_numpy(1.0 + 1e-7 * rng.standard_normal(atm.shape)).to(dev))
+    d_rfp = torch.empty_like(d_rf[0])
+    torch.cuda.synchronize()
+    eng.rf_lte_device(d_atm.data_ptr(), d_ndep.data_ptr(), ncol, nd, d_po[0].data_ptr(),
+                      d_rfp.data_ptr(), precision=0)
+    cond = _err_stats(d_rfp, d_rf[0], 1e-4)
+    del d_rfp
+
     I64 = d_st[0][:, 0]
     i_err = float(((d_st[1][:, 0] - I64).abs() / I64).max().item())
     stokes_err = float(((d_st[1] - d_st[0]).abs() / I64[:, :1, None]).max().item())
@@ -279,6 +293,7 @@ def _rf_leg(eng, W, torch, dev, reps):
         "fp32_I_rel_err_max": float(f"{i_err:.3g}"),
         "fp32_stokes_err_over_Ic_max": float(f"{stokes_err:.3g}"),
         "rf_err_fp32_vs_fp64_adjoint": _err_stats(d_rf[1], d_rf[0], 1e-4),
+        "rf_err_fp64_inputs_perturbed_1e-7_vs_fp64_adjoint": cond,
         "rf_err_fp64_adjoint_vs_fd": _err_stats(d_rf[0], fd, 1e-4),
         "rf_err_fp32_adjoint_vs_fd": _err_stats(d_rf[1], fd, 1e-4),
     }
@@ -491,9 +506,9 @@ def _mixed_leg(torch, dev, ncol, do_cpu, nworkers, rf_cols=256, oracle_cols=128)
     400 synthetic Fe I Zeeman lines on 2000 wavelengths, FP32 Stokes at
     mu = 1, 0.7, 0.4 with the batch resident in HBM (one rhg_synth_lte per
     mu; the library runs the batch sorted by depth count); FP32 adjoint
-    response functions on the first rf_cols columns at mu = 1 (all six
-    parameters; the tape of 2000 wavelengths x 256 depths takes ~0.2 GB per
-    column); the FP64 oracle on oracle_cols columns at mu = 0.7; the
+    response functions to T, v_LOS and B of all columns at mu = 1, in chunks
+    of rf_cols columns (the tape of 2000 wavelengths x 256 depths takes
+    ~0.2 GB per column); the FP64 oracle on oracle_cols columns at mu = 0.7; the
     reference's rhf1d() per (column, mu) on a bounded sample.  The unit is
     (column, mu) syntheses per second."""
     from pyrh_amd import model as M, roofline as R, workloads as W
@@ -554,25 +569,58 @@ def _mixed_leg(torch, dev, ncol, do_cpu, nworkers, rf_cols=256, oracle_cols=128)
     }
     del d_out
     torch.cuda.empty_cache()
-    # FP32 adjoint response functions on a subset
+    # FP32 adjoint response functions to T, v_LOS and B of every column at
+    # mu = 1, in chunks of rf_cols columns ordered by depth count (each
+    # chunk's output, 3 x ndep x 4 x 2000 x rf_cols floats, is consumed and
+    # its buffer reused); the first chunk also in FP64 for the error
+    params = ("T", "vlos", "B")
+    order = np.argsort(-nd, kind="stable")
     nrf = min(rf_cols, ncol)
     d_st = torch.empty((nrf, 4, nout), dtype=torch.float64, device=dev)
-    d_rf = torch.empty((6, nmax, 4, nout, nrf), dtype=torch.float32, device=dev)
+    d_rf = torch.empty((len(params), nmax, 4, nout, nrf), dtype=torch.float32, device=dev)
 
-    def rf():
-        eng.rf_lte_device(d_atm.data_ptr(), d_nd.data_ptr(), nrf, nmax, d_st.data_ptr(),
-                          d_rf.data_ptr(), mu=1.0, precision=1)
-    rf()
+    def written(buf, n, m):
+        # the library's layout for n columns of ndep_max m
+        return buf.view(-1)[:len(params) * m * 4 * nout * n].view(len(params), m, 4, nout, n)
+
+    def chunk(c0, prec=1, out=None):
+        idx = order[c0:c0 + nrf]
+        n, m = len(idx), int(nd[idx].max())
+        a = torch.from_numpy(np.ascontiguousarray(atm[idx][:, :, :m])).to(dev)
+        dn = torch.from_numpy(nd[idx]).to(dev)
+        torch.cuda.synchronize()
+        eng.rf_lte_device(a.data_ptr(), dn.data_ptr(), n, m, d_st.data_ptr(),
+                          (d_rf if out is None else out).data_ptr(), mu=1.0, precision=prec,
+                          params=params)
+        return n, m
+
+    chunk(0)                       # warm-up
     torch.cuda.synchronize()
+    stages = {k: 0.0 for k in eng.rf_timings()}
     t0 = time.perf_counter()
-    rf()
+    for c0 in range(0, ncol, nrf):
+        n, m = chunk(c0)
+        for k, v in eng.rf_timings().items():
+            stages[k] += float(v)
     torch.cuda.synchronize()
     dtr = time.perf_counter() - t0
-    res["adjoint_rf_fp32"] = {"columns": nrf, "mu": 1.0, "params": 6, "ms": round(dtr * 1e3, 2),
-                              "columns_per_s": round(nrf / dtr, 1),
-                              "stage_ms": {k: round(float(v), 3) for k, v in eng.rf_timings().items()},
-                              "finite": bool(torch.isfinite(d_rf).all().item())}
-    del d_rf, d_st
+    # the last chunk (shallowest columns) is still in the buffer
+    finite = bool(torch.isfinite(written(d_rf, n, m)).all().item())
+    n0, m0 = chunk(0)
+    torch.cuda.synchronize()
+    rf32 = written(d_rf, n0, m0).clone()
+    d_rf64 = torch.empty((len(params), m0, 4, nout, n0), dtype=torch.float64, device=dev)
+    chunk(0, prec=0, out=d_rf64)
+    torch.cuda.synchronize()
+    err = _err_stats(rf32, d_rf64, 1e-4, params)
+    finite &= bool(torch.isfinite(rf32).all().item())
+    res["adjoint_rf_fp32"] = {"workload": f"{res['workload']}_rf_T_vlos_B", "columns": ncol,
+                              "mu": 1.0, "params": list(params), "chunk_columns": nrf,
+                              "ms": round(dtr * 1e3, 2), "columns_per_s": round(ncol / dtr, 1),
+                              "stage_ms_total": {k: round(v, 2) for k, v in stages.items()},
+                              "finite": finite,
+                              "err_vs_fp64_adjoint_first_chunk": err}
+    del d_rf, d_st, d_rf64, rf32
     eng.close()
     torch.cuda.empty_cache()
     res["cpu_baseline"] = None
