@@ -529,6 +529,12 @@ def _mixed_leg(torch, dev, ncol, do_cpu, nworkers, rf_cols=256, oracle_cols=128)
             eng.synth_lte_device(d_atm.data_ptr(), d_nd.data_ptr(), ncol, nmax, d_out[i].data_ptr(),
                                  mu=mu, precision=prec)
 
+    # launch configuration (pyrh_amd.autotune) on the first 2048 columns,
+    # FP32 form, mu = 1; it applies to the whole batch and every mu
+    from pyrh_amd import autotune
+    nt = min(ncol, 2048)
+    tuning = autotune.tune(eng, d_atm[:nt], d_nd[:nt], d_out[0, :nt], precision=1, reps=1)
+
     synth(1)                       # warm-up (module load, allocations)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -553,6 +559,7 @@ def _mixed_leg(torch, dev, ncol, do_cpu, nworkers, rf_cols=256, oracle_cols=128)
         "unit": "(column, mu) syntheses/s", "ncol": ncol, "mu": list(W.MIXED_MU),
         "ndep_counts": {int(k): int(v) for k, v in zip(*np.unique(nd, return_counts=True))},
         "nlambda": nout, "nline": int(mdl.scalars["nline"]),
+        "autotune": tuning,
         "fp32": {"value": round(len(W.MIXED_MU) * ncol / dt32, 1), "wall_s": round(dt32, 3)},
         "fp64": {"value": round(len(W.MIXED_MU) * ncol / dt64, 1), "wall_s": round(dt64, 3)},
         "roofline_fp32_mu04": {"kernel": "k_formal_stokes<float>", "achieved": round(achieved, 3),
@@ -718,6 +725,10 @@ def main():
         print(f"profile step done: {a.ncol} columns; 1 GiB copy", flush=True)
         return
 
+    # launch configuration for this batch (pyrh_amd.autotune), FP32 form
+    from pyrh_amd import autotune
+    tuning = autotune.tune(eng, d_atm, d_ndep, d_out, precision=1, reps=3)
+
     def timed(precision):
         for _ in range(a.warmup):
             step(precision)
@@ -771,8 +782,8 @@ def main():
         "config": {"workload": WORKLOAD if a.ncol == 16384 else f"lte_stokes_map_{a.ncol}",
                    "ncol": a.ncol, "ndep": int(atm.shape[2]), "nlambda": nout, "mu": 1.0,
                    "lines": "Fe I 630.15 + 630.25 nm, Zeeman",
-                   "form": "FP32 form: float line opacity and DELO-Bezier recursion; FP64 "
-                           "equilibria, heights, path lengths and Bezier weights",
+                   "form": "FP32 form: float continuum and Voigt/Faraday-Voigt functions; FP64 "
+                           "Zeeman sums, points, DELO-Bezier recursion, equilibria and heights",
                    "response_functions": RF_WORKLOAD + " (object response_functions)"},
         "roofline": {"bound": "valu", "kernel": "k_formal_stokes<float>",
                      "achieved": round(achieved, 3), "peak": R.PEAK_FP32_TFLOPS,
@@ -791,6 +802,7 @@ def main():
         "nlte": nlte,
         "mixed": mixed,
         "hse": hse,
+        "autotune": tuning,
         "pipeline_flops": fl["total"],
         "host_buffers_columns_per_s": round(host_rate, 1),
     }

@@ -486,6 +486,24 @@ int rhg_synth_lte_diag(rhg_context_t *ctx, const rhg_columns_t *cols,
 #define RHG_NSTAGE 4
 int rhg_last_timings(rhg_context_t *ctx, double *ms_out);
 
+/* Launch configuration of a context, chosen by the autotuner
+ * (pyrh_amd.autotune), which times the candidates on the caller's batch.
+ * No counterpart in the reference.  Every setting gives bit-identical
+ * results; they differ in speed only.  Values are 0 or 1:
+ *   RHG_TUNE_LINE_TABLE   stage each wavelength's line list in LDS for the
+ *                         polarized formal solution (default 1; has no
+ *                         effect when the largest list exceeds 16 KB)
+ *   RHG_TUNE_COLUMN_SORT  run a batch of uneven depth counts sorted by depth
+ *                         count (default 1)
+ *   RHG_TUNE_WAVE_ORDER   launch the polarized wavelengths in the order of
+ *                         their Zeeman-component counts, most first, rather
+ *                         than in wavelength order (default 0)
+ * rhg_set_tuning returns RHG_EINVAL for an unknown key or value; setting the
+ * wavelength order copies a list to the device (synchronous). */
+enum rhg_tune { RHG_TUNE_LINE_TABLE = 0, RHG_TUNE_COLUMN_SORT = 1, RHG_TUNE_WAVE_ORDER = 2 };
+int rhg_set_tuning(rhg_context_t *ctx, int32_t key, int32_t value);
+int rhg_get_tuning(rhg_context_t *ctx, int32_t key, int32_t *value);
+
 /* Kernel milliseconds of the last rhg_rf_lte / rhg_rf_lte_params call, 5
  * entries: synthesis inputs (state, geometry, continuum), state tangents,
  * forward sweep, reverse sweep, chain to the parameters (with the scalar

@@ -99,6 +99,10 @@ struct rhg_context {
   // polarized kernels read the lists from global memory
   int tab_maxl = 0, tab_maxc = 0;
   size_t tab_bytes = 0;
+  // launch configuration (rhg_set_tuning): LDS line table, sorting of
+  // batches of uneven depth counts, order of the polarized wavelengths
+  int tune_line_table = 1, tune_column_sort = 1, tune_wave_order = 0;
+  std::vector<int32_t> h_pol, h_pol_heavy;
   const int32_t *d_io_pol = nullptr, *d_io_scal = nullptr;
   DevBuf atmos, ndep, state, moving, cchi, ceta, stokes, chiref, census, census_sum;
   // response functions: perturbed inputs/states, tangents, tape, adjoints
@@ -374,6 +378,18 @@ int rhg_open(const rhg_model_t *m, int device, rhg_context_t **out) {
       (m->solver[ols[io]] == RHG_SOLVER_STOKES ? pol : scal).push_back(io);
     c->n_pol = (int) pol.size();
     c->n_scal = (int) scal.size();
+    // the polarized wavelengths by the Zeeman components of their line lists,
+    // most first (an order the tuning may select)
+    c->h_pol = pol;
+    std::vector<int> ncomp(nout, 0);
+    for (int io : pol)
+      for (int e = m->lam_line_ptr[ols[io]]; e < m->lam_line_ptr[ols[io] + 1]; e++) {
+        const int l = m->lam_line_idx[e];
+        ncomp[io] += 1 + m->line_comp_ptr[l + 1] - m->line_comp_ptr[l];
+      }
+    c->h_pol_heavy = pol;
+    std::stable_sort(c->h_pol_heavy.begin(), c->h_pol_heavy.end(),
+                     [&](int a, int b) { return ncomp[a] > ncomp[b]; });
     rc = upload(c, pol.empty() ? nullptr : pol.data(), pol.size(), &c->d_io_pol);
     if (!rc) rc = upload(c, scal.empty() ? nullptr : scal.data(), scal.size(), &c->d_io_scal);
     if (rc) { std::string e = g_err; rhg_close(c); return fail(rc, e); }
@@ -485,7 +501,7 @@ static int launch_synth(rhg_context_t *c, const double *d_atm, const int32_t *d_
   RHG_LAUNCH_FORMAL((rhg::k_formal_stokes<float, MO, LT>), (rhg::k_formal_stokes<double, MO, LT>), \
                     c->d_io_pol, c->n_pol, LT ? (unsigned) c->tab_bytes : 0u, , c->tab_maxl,     \
                     c->tab_maxc)
-    const bool lt = c->tab_bytes > 0;
+    const bool lt = c->tab_bytes > 0 && c->tune_line_table;
     if (c->dm.magneto_optical) {
       if (lt) { RHG_STOKES(true, true) } else { RHG_STOKES(true, false) }
     } else {
@@ -596,7 +612,7 @@ int sort_order(rhg_context_t *c, const rhg_columns_t *cols, int32_t device_ptrs,
                std::vector<int32_t> &perm, std::vector<int32_t> &sorted_ndep) {
   perm.clear();
   const int ncol = cols->ncol;
-  if (ncol < 128 || !cols->ndep) return 0;
+  if (ncol < 128 || !cols->ndep || !c->tune_column_sort) return 0;
   std::vector<int32_t> hndep(ncol);
   if (device_ptrs) {
     HIPCHECK(hipSetDevice(c->device));
@@ -1978,6 +1994,39 @@ int rhg_hse_timing(rhg_context_t *ctx, double *ms_out, int32_t *lanes_out) {
 int rhg_nlte_timings(rhg_context_t *ctx, double *out) {
   if (!ctx || !out) return fail(RHG_EINVAL, "null argument");
   for (int s = 0; s < RHG_NLTE_NTIMING; s++) out[s] = ctx->nlte_ms[s];
+  return 0;
+}
+
+int rhg_set_tuning(rhg_context_t *ctx, int32_t key, int32_t value) {
+  if (!ctx) return fail(RHG_EINVAL, "null argument");
+  if (value != 0 && value != 1) return fail(RHG_EINVAL, "tuning values are 0 or 1");
+  switch (key) {
+    case RHG_TUNE_LINE_TABLE: ctx->tune_line_table = value; break;
+    case RHG_TUNE_COLUMN_SORT: ctx->tune_column_sort = value; break;
+    case RHG_TUNE_WAVE_ORDER: {
+      const std::vector<int32_t> &v = value ? ctx->h_pol_heavy : ctx->h_pol;
+      if (!v.empty()) {
+        HIPCHECK(hipSetDevice(ctx->device));
+        HIPCHECK(hipMemcpy((void *) ctx->d_io_pol, v.data(), v.size() * sizeof(int32_t),
+                           hipMemcpyHostToDevice));
+      }
+      ctx->tune_wave_order = value;
+      break;
+    }
+    default: return fail(RHG_EINVAL, "unknown tuning key");
+  }
+  g_err.clear();
+  return 0;
+}
+
+int rhg_get_tuning(rhg_context_t *ctx, int32_t key, int32_t *value) {
+  if (!ctx || !value) return fail(RHG_EINVAL, "null argument");
+  switch (key) {
+    case RHG_TUNE_LINE_TABLE: *value = ctx->tab_bytes > 0 && ctx->tune_line_table; break;
+    case RHG_TUNE_COLUMN_SORT: *value = ctx->tune_column_sort; break;
+    case RHG_TUNE_WAVE_ORDER: *value = ctx->tune_wave_order; break;
+    default: return fail(RHG_EINVAL, "unknown tuning key");
+  }
   return 0;
 }
 

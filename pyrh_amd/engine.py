@@ -32,6 +32,10 @@ NLTE_TIMINGS = ("setup_ms", "scatter_bezier_ms", "iter_bezier_ms", "stateq_ms", 
                 "cols_no_op", "cols_op")
 
 
+# rhg_tune keys, in enum order
+TUNE_KEYS = ("line_table", "column_sort", "wave_order")
+
+
 class EngineError(RuntimeError):
     pass
 
@@ -69,6 +73,8 @@ def lib():
         L.rhg_synth_lte_noalloc.argtypes = [C.c_void_p, C.POINTER(RhgColumns), C.c_int32, PD]
         L.rhg_synth_lte_diag.argtypes = [C.c_void_p, C.POINTER(RhgColumns), C.c_int32, PD, PD, PD]
         L.rhg_last_timings.argtypes = [C.c_void_p, PD]
+        L.rhg_set_tuning.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+        L.rhg_get_tuning.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]
         L.rhg_rf_timings.argtypes = [C.c_void_p, PD]
         L.rhg_count_work.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
         L.rhg_rf_lte.argtypes = [C.c_void_p, C.POINTER(RhgColumns), C.c_int32, C.c_int32, PD,
@@ -331,6 +337,19 @@ class Engine:
         cnt = np.zeros(RHG_NWORK, np.int64)
         _check(lib().rhg_count_work(self._h, cnt.ctypes.data_as(C.POINTER(C.c_int64))))
         return cnt
+
+    def set_tuning(self, key, value):
+        """Set one launch-configuration knob (TUNE_KEYS; 0 or 1)."""
+        _check(lib().rhg_set_tuning(self._h, TUNE_KEYS.index(key), int(value)))
+
+    def tuning(self):
+        """The context's launch configuration {knob: value}."""
+        out = {}
+        for i, k in enumerate(TUNE_KEYS):
+            v = C.c_int32()
+            _check(lib().rhg_get_tuning(self._h, i, C.byref(v)))
+            out[k] = int(v.value)
+        return out
 
     def timings(self):
         """Per-stage kernel milliseconds of the last call."""
