@@ -146,13 +146,14 @@ __device__ __forceinline__ void rsincos(float x, float *s, float *c) {
 
 // Humlicek's region (0..3 for I..IV) of the point z = a - i v, the test of
 // rh/voigt.c:396-413; voigt_humlicek() branches on exactly this test.
+// The three comparisons are all evaluated and combined with selects, so
+// that a wavefront does not branch to find each lane's region.
 template <typename R>
 __device__ __forceinline__ int humlicek_region(R a, R v) {
   const R s = fabs(v) + a;
-  if (s >= R(15.0)) return 0;
-  if (s >= R(5.5)) return 1;
-  if (a >= R(0.195) * fabs(v) - R(0.176)) return 2;
-  return 3;
+  const int inner = (a >= R(0.195) * fabs(v) - R(0.176)) ? 2 : 3;
+  const int mid = (s >= R(5.5)) ? 1 : inner;
+  return (s >= R(15.0)) ? 0 : mid;
 }
 
 // Each region ends in one complex quotient num/den (region IV: z num/den,

@@ -138,6 +138,7 @@ __device__ __forceinline__ double atom_sum(const DevModel &M, int l0, int nl, do
 // LineTabG reads the model in global memory; LineTabL (below) reads a copy
 // staged in LDS by the kernel.
 struct LineTabG {
+  static constexpr bool grouped = false;
   const DevModel &M;
   int p0, n;
   // entries p0 .. p0 + n - 1 of the model's wavelength line lists
@@ -157,13 +158,19 @@ struct LineTabG {
 // A wavelength's line list staged in LDS (one copy per wavefront; every
 // lane reads the same address, so the reads are broadcasts).  Counts and
 // indices are made wave-uniform so that the loops over lines and
-// components run as scalar loops.
+// components run as scalar loops.  A line's components are stored grouped
+// by their Delta M (sigma-, pi, sigma+), each group in the model's order:
+// each group feeds its own sum, so the sums see the same terms in the same
+// order as from the ungrouped list, and the component loop needs no
+// per-component branch.
 struct LineRecL {
   double dv, larmor;
   int line, flags, c0, c1;   // flags: 1 damped, 2 polarizable
+  int cpi, csp;              // first pi and first sigma+ component
 };
 
 struct LineTabL {
+  static constexpr bool grouped = true;
   const LineRecL *rec;
   const double *shift_, *strength_;
   const int *q_;
@@ -181,6 +188,8 @@ struct LineTabL {
   __device__ __forceinline__ double larmor(int i) const { return rec[i].larmor; }
   __device__ __forceinline__ int c0(int i) const { return __builtin_amdgcn_readfirstlane(rec[i].c0); }
   __device__ __forceinline__ int c1(int i) const { return __builtin_amdgcn_readfirstlane(rec[i].c1); }
+  __device__ __forceinline__ int cpi(int i) const { return __builtin_amdgcn_readfirstlane(rec[i].cpi); }
+  __device__ __forceinline__ int csp(int i) const { return __builtin_amdgcn_readfirstlane(rec[i].csp); }
   __device__ __forceinline__ double shift(int c) const { return shift_[c]; }
   __device__ __forceinline__ double strength(int c) const { return strength_[c]; }
   __device__ __forceinline__ int q(int c) const { return __builtin_amdgcn_readfirstlane(q_[c]); }
@@ -223,13 +232,28 @@ __device__ __forceinline__ LineTabL stage_lines(const DevModel &M, int ls, char 
   __syncthreads();
   for (int i = lane; i < n; i += 64) rec[i].c1 += rec[i].c0;
   __syncthreads();
+  // components grouped by Delta M = -1, 0, +1, each in the model's order
   for (int i = 0; i < n; i++) {
     const int a = rec[i].c0, cnt = rec[i].c1 - a;
     const int g0 = ldc(M.line_comp_ptr)[rec[i].line];
+    int nsm = 0, npi = 0;
+    for (int c = 0; c < cnt; c++) {
+      const int qc = ldc(M.comp_q)[g0 + c];
+      nsm += qc == -1;
+      npi += qc == 0;
+    }
     for (int c = lane; c < cnt; c += 64) {
-      shift[a + c] = ldc(M.comp_shift)[g0 + c];
-      strength[a + c] = ldc(M.comp_strength)[g0 + c];
-      q[a + c] = ldc(M.comp_q)[g0 + c];
+      const int qc = ldc(M.comp_q)[g0 + c];
+      // position of component c within its group
+      int pos = qc == -1 ? 0 : (qc == 0 ? nsm : nsm + npi);
+      for (int b = 0; b < c; b++) pos += ldc(M.comp_q)[g0 + b] == qc;
+      shift[a + pos] = ldc(M.comp_shift)[g0 + c];
+      strength[a + pos] = ldc(M.comp_strength)[g0 + c];
+      q[a + pos] = qc;
+    }
+    if (lane == 0) {
+      rec[i].cpi = a + nsm;
+      rec[i].csp = a + nsm + npi;
     }
   }
   __syncthreads();
@@ -240,6 +264,20 @@ __device__ __forceinline__ LineTabL stage_lines(const DevModel &M, int ls, char 
   t.q_ = q;
   t.n = __builtin_amdgcn_readfirstlane(n);
   return t;
+}
+
+// acc += a b for the Zeeman component sums.  In the FP32 form's
+// translation units (FMA contraction allowed) an explicit FMA, so that the
+// LDS-table and global-table loops round identically whatever the compiler
+// would contract; elsewhere a product and a sum, in the reference's order.
+template <typename T>
+__device__ __forceinline__ void acc_prod(T &acc, const T &a, const T &b) { acc += a * b; }
+__device__ __forceinline__ void acc_prod(double &acc, const double &a, const double &b) {
+#ifdef RHG_FP32_FORM_TU
+  acc = fma(a, b, acc);
+#else
+  acc += a * b;
+#endif
 }
 
 // Profile of slot i of a wavelength's line list (line l) at one depth,
@@ -270,22 +308,39 @@ __device__ __forceinline__ T line_profile_t(const Tab &tab, int i, int lb, const
       T phi_sm = T(0), phi_pi = T(0), phi_sp = T(0), H, Fv;
       T psi_sm = T(0), psi_pi = T(0), psi_sp = T(0);
       const int c0 = tab.c0(i), c1 = tab.c1(i);
-      for (int nz = c0; nz < c1; nz++) {
-        TV Fw;
-        const T u = v - T(tab.shift(nz)) * vB;
-        const int reg = humlicek_region(val(adamp), val(u));
-        H = (T) VoigtEval<TV>::eval((TV) adamp, (TV) u, reg, &Fw);
-        Fv = (T) Fw;
-        const int q = tab.q(nz);
-        const T sH = T(tab.strength(nz)) * H;
-        if (q == -1) phi_sm += sH;
-        else if (q == 0) phi_pi += sH;
-        else phi_sp += sH;
-        if constexpr (MO) {
-          const T sF = T(tab.strength(nz)) * Fv;
-          if (q == -1) psi_sm += sF;
-          else if (q == 0) psi_pi += sF;
-          else psi_sp += sF;
+      if constexpr (Tab::grouped) {
+        // one Delta M group [a, b) into its sums; the next component's
+        // shift and strength are read while this one's profile is formed
+        auto group = [&](int a, int b, T &phi_g, T &psi_g) {
+          double sh = a < b ? tab.shift(a) : 0.0, sg = a < b ? tab.strength(a) : 0.0;
+          for (int nz = a; nz < b; nz++) {
+            const double shc = sh, sgc = sg;
+            if (nz + 1 < b) {
+              sh = tab.shift(nz + 1);
+              sg = tab.strength(nz + 1);
+            }
+            TV Fw;
+            const T u = v - T(shc) * vB;
+            const int reg = humlicek_region(val(adamp), val(u));
+            const T Hc = (T) VoigtEval<TV>::eval((TV) adamp, (TV) u, reg, &Fw);
+            acc_prod(phi_g, T(sgc), Hc);
+            if constexpr (MO) acc_prod(psi_g, T(sgc), (T) Fw);
+          }
+        };
+        group(c0, tab.cpi(i), phi_sm, psi_sm);
+        group(tab.cpi(i), tab.csp(i), phi_pi, psi_pi);
+        group(tab.csp(i), c1, phi_sp, psi_sp);
+      } else {
+        for (int nz = c0; nz < c1; nz++) {
+          TV Fw;
+          const T u = v - T(tab.shift(nz)) * vB;
+          const int reg = humlicek_region(val(adamp), val(u));
+          H = (T) VoigtEval<TV>::eval((TV) adamp, (TV) u, reg, &Fw);
+          Fv = (T) Fw;
+          const int q = tab.q(nz);
+          const T sg = T(tab.strength(nz));
+          acc_prod(q == -1 ? phi_sm : (q == 0 ? phi_pi : phi_sp), sg, H);
+          if constexpr (MO) acc_prod(q == -1 ? psi_sm : (q == 0 ? psi_pi : psi_sp), sg, Fv);
         }
       }
       const T phi_sigma = phi_sp + phi_sm;
