@@ -302,7 +302,11 @@ enum rhg_precision { RHG_PREC_FP64 = 0, RHG_PREC_FP32 = 1 };
  * emergent Stokes vectors at every wavelength of the model except the
  * reference one.  stokes_out is [ncol][4][nspect-1] float64 (I,Q,U,V),
  * host or device memory (device if `device_ptrs` != 0).  `precision` is an
- * rhg_precision.  Synchronous. */
+ * rhg_precision.  Depth counts outside [4, ndep_max] return RHG_EINVAL;
+ * with device buffers they are checked on the device (no copy to the host
+ * before the launches) and the batch of 128 columns or more runs sorted by
+ * depth count (rhg_set_tuning) through a permutation, with the results in
+ * the caller's column order.  Synchronous. */
 int rhg_synth_lte(rhg_context_t *ctx, const rhg_columns_t *cols,
                   int32_t precision, int32_t device_ptrs, double *stokes_out);
 

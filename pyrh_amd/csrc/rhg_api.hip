@@ -118,6 +118,10 @@ struct rhg_context {
   // batches of uneven depth counts run sorted by depth count: the caller's
   // columns gathered into these, the outputs scattered back
   DevBuf srt_atm, srt_ndep, srt_perm, srt_out, srt_rf;
+  // the synthesis's depth counts as its kernels read them (checked, clamped
+  // and, for uneven batches, sorted), the slot -> column permutation, the
+  // counting sort's histogram and the out-of-range flag
+  DevBuf syn_ndep, syn_perm, syn_hist;
   double nlte_ms[RHG_NLTE_NTIMING]{};
   // hydrostatic equilibrium: element tables, inputs, outputs, timing
   DevBuf hs_nstage, hs_ptr, hs_logpf, hs_ionpot, hs_abund, hs_in, hs_ndep, hs_out, hs_niter,
@@ -134,7 +138,6 @@ struct rhg_context {
   int last_ncol = 0, last_Nmax = 0, last_nfield = 0;
   double last_mu = 1.0;
   const int32_t *last_ndep = nullptr;
-  std::vector<int32_t> last_hndep;
 };
 
 namespace {
@@ -427,16 +430,18 @@ int rhg_open(const rhg_model_t *m, int device, rhg_context_t **out) {
 
 // Stage 1 (k_state) for a batch: one thread per (column, depth), the
 // chemical-equilibrium kernel instantiated for the model's nucleus count.
-static int launch_state(rhg_context_t *c, const double *d_atm, const int32_t *d_ndep, int ncol,
-                        int Nmax, double mu, int atm_scale, double *state, int nfield) {
+// perm (or nullptr): slot i of the batch reads the atmosphere of column perm[i].
+static int launch_state(rhg_context_t *c, const double *d_atm, const int32_t *d_ndep,
+                        const int32_t *perm, int ncol, int Nmax, double mu, int atm_scale,
+                        double *state, int nfield) {
   const int64_t npts = (int64_t) ncol * Nmax;
   dim3 blk(64), grd((unsigned) ((npts + 63) / 64));
   const size_t lds = (size_t) 3 * c->dm.nmol * 64 * sizeof(double);
   switch (c->dm.nnuclei) {
 #define RHG_LAUNCH_STATE(NN)                                                                    \
   case NN:                                                                                      \
-    hipLaunchKernelGGL(rhg::k_state<NN>, grd, blk, lds, c->stream, c->d_model, d_atm, d_ndep, ncol, \
-                       Nmax, mu, atm_scale, state, nfield);                                     \
+    hipLaunchKernelGGL(rhg::k_state<NN>, grd, blk, lds, c->stream, c->d_model, d_atm, d_ndep, perm, \
+                       ncol, Nmax, mu, atm_scale, state, nfield);                               \
     break;
     RHG_LAUNCH_STATE(1) RHG_LAUNCH_STATE(2) RHG_LAUNCH_STATE(3) RHG_LAUNCH_STATE(4)
     RHG_LAUNCH_STATE(5) RHG_LAUNCH_STATE(6) RHG_LAUNCH_STATE(7) RHG_LAUNCH_STATE(8)
@@ -449,17 +454,18 @@ static int launch_state(rhg_context_t *c, const double *d_atm, const int32_t *d_
 
 // Stages 1-4 of a synthesis (k_state, k_geometry, k_continuum, the formal
 // kernels) on device buffers, with the stage events; no allocation, no copy,
-// no synchronisation.
-static int launch_synth(rhg_context_t *c, const double *d_atm, const int32_t *d_ndep, int ncol,
-                        int Nmax, double mu, int atm_scale, int32_t precision, int nfield,
-                        double *state, int32_t *moving, double *chiref, void *cchi, void *ceta,
-                        double *d_out) {
+// no synchronisation.  d_ndep is in slot order; perm (or nullptr) maps slot
+// i to the caller's column perm[i] of d_atm and d_out.
+static int launch_synth(rhg_context_t *c, const double *d_atm, const int32_t *d_ndep,
+                        const int32_t *perm, int ncol, int Nmax, double mu, int atm_scale,
+                        int32_t precision, int nfield, double *state, int32_t *moving,
+                        double *chiref, void *cchi, void *ceta, double *d_out) {
   const int64_t npts = (int64_t) ncol * Nmax;
   const int nout = c->nout;
 
   HIPCHECK(hipEventRecord(c->ev[0], c->stream));
   {
-    const int rc = launch_state(c, d_atm, d_ndep, ncol, Nmax, mu, atm_scale, state, nfield);
+    const int rc = launch_state(c, d_atm, d_ndep, perm, ncol, Nmax, mu, atm_scale, state, nfield);
     if (rc) return rc;
   }
   HIPCHECK(hipEventRecord(c->ev[1], c->stream));
@@ -490,11 +496,11 @@ static int launch_synth(rhg_context_t *c, const double *d_atm, const int32_t *d_
     if (precision == RHG_PREC_FP32)                                                              \
       hipLaunchKernelGGL(K32, grd, blk, LDS, c->stream, c->d_model, d_ndep, moving, ncol, Nmax,  \
                          state, c->d_out_ls, LIST, NLIST, nout, (const float *) cchi,           \
-                         (const float *) ceta, mu, d_out __VA_ARGS__);                           \
+                         (const float *) ceta, mu, d_out, perm __VA_ARGS__);                     \
     else                                                                                         \
       hipLaunchKernelGGL(K64, grd, blk, LDS, c->stream, c->d_model, d_ndep, moving, ncol, Nmax,  \
                          state, c->d_out_ls, LIST, NLIST, nout, (const double *) cchi,          \
-                         (const double *) ceta, mu, d_out __VA_ARGS__);                          \
+                         (const double *) ceta, mu, d_out, perm __VA_ARGS__);                    \
     HIPCHECK(hipGetLastError());                                                                 \
   }
 #define RHG_STOKES(MO, LT)                                                                       \
@@ -516,10 +522,168 @@ static int launch_synth(rhg_context_t *c, const double *d_atm, const int32_t *d_
   return 0;
 }
 
-static int synth(rhg_context_t *c, const rhg_columns_t *cols, int32_t precision,
-                 int32_t device_ptrs, double *stokes_out, double *height_out,
-                 double *chiref_out) {
-  if (!c || !cols || !stokes_out) return fail(RHG_EINVAL, "null argument");
+}  // extern "C"
+
+namespace rhg {
+
+// flag[0] = 0 (the error flag must start at zero on every call)
+__global__ void k_clear_flag(int32_t *flag) {
+  if (threadIdx.x == 0) flag[0] = 0;
+}
+
+// out[i] = ndep[i] clamped to [4, Nmax]; flag[0] = 1 if any was outside
+__global__ void k_check_ndep(const int32_t *__restrict__ ndep, int ncol, int Nmax,
+                             int32_t *__restrict__ out, int32_t *__restrict__ flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ncol) return;
+  const int n = ndep[i];
+  const bool bad = n < 4 || n > Nmax;
+  out[i] = bad ? (n < 4 ? 4 : Nmax) : n;
+  if (bad) atomicOr(flag, 1);
+}
+
+// The depth-count sort of a batch, on the device: a stable counting sort of
+// the columns by depth count, largest first, over the values [4, Nmax]
+// (out-of-range counts are clamped and counted as errors).  SORT_CHUNK
+// columns per workgroup; hist is [nchunk][Nmax + 1] followed by nchunk
+// out-of-range counts.
+constexpr int SORT_CHUNK = 1024;
+
+__device__ inline int clamp_ndep(int n, int Nmax) { return n < 4 ? 4 : (n > Nmax ? Nmax : n); }
+
+// hist[g][b] = columns of chunk g whose (clamped) depth count is b;
+// hist[nchunk * (Nmax + 1) + g] = columns of chunk g out of range
+__global__ void __launch_bounds__(SORT_CHUNK)
+k_sort_count(const int32_t *__restrict__ ndep, int ncol, int Nmax, int32_t *__restrict__ hist) {
+  extern __shared__ int32_t h[];   // Nmax + 2
+  const int g = blockIdx.x, nb = Nmax + 1;
+  for (int b = threadIdx.x; b <= nb; b += blockDim.x) h[b] = 0;
+  __syncthreads();
+  const int i = g * SORT_CHUNK + threadIdx.x;
+  if (i < ncol) {
+    const int n = ndep[i];
+    atomicAdd(&h[clamp_ndep(n, Nmax)], 1);
+    if (n < 4 || n > Nmax) atomicAdd(&h[nb], 1);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) hist[(int64_t) g * nb + b] = h[b];
+  if (threadIdx.x == 0) hist[(int64_t) gridDim.x * nb + g] = h[nb];
+}
+
+// One workgroup: hist[g][b] becomes the first sorted position of chunk g's
+// columns of depth count b (the columns of larger depth count, then those of
+// count b in the chunks before g); flag[0] = the number of columns out of
+// range.
+__global__ void __launch_bounds__(1024)
+k_sort_scan(int32_t *__restrict__ hist, int nchunk, int Nmax, int32_t *__restrict__ flag) {
+  extern __shared__ int32_t tot[];   // Nmax + 2
+  const int nb = Nmax + 1;
+  if (threadIdx.x == 0) tot[nb] = 0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    int s = 0;
+    for (int g = 0; g < nchunk; g++) s += hist[(int64_t) g * nb + b];
+    tot[b] = s;
+  }
+  __syncthreads();
+  int bad = 0;
+  for (int g = threadIdx.x; g < nchunk; g += blockDim.x) bad += hist[(int64_t) nchunk * nb + g];
+  if (bad) atomicAdd(&tot[nb], bad);
+  if (threadIdx.x == 0) {
+    int s = 0;
+    for (int b = Nmax; b >= 0; b--) {
+      const int t = tot[b];
+      tot[b] = s;
+      s += t;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) flag[0] = tot[nb];
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    int off = tot[b];
+    for (int g = 0; g < nchunk; g++) {
+      const int t = hist[(int64_t) g * nb + b];
+      hist[(int64_t) g * nb + b] = off;
+      off += t;
+    }
+  }
+}
+
+// Column i goes to sorted position pos = (its chunk's first position for
+// its depth count) + (columns of the same count before it in the chunk):
+// perm[pos] = i, sndep[pos] = its clamped depth count.
+__global__ void __launch_bounds__(SORT_CHUNK)
+k_sort_place(const int32_t *__restrict__ ndep, int ncol, int Nmax, const int32_t *__restrict__ off,
+             int32_t *__restrict__ perm, int32_t *__restrict__ sndep) {
+  __shared__ int32_t v[SORT_CHUNK];
+  const int g = blockIdx.x, i = g * SORT_CHUNK + threadIdx.x;
+  const int n = i < ncol ? clamp_ndep(ndep[i], Nmax) : -1;
+  v[threadIdx.x] = n;
+  __syncthreads();
+  if (i >= ncol) return;
+  int r = 0;
+  for (int j = 0; j < (int) threadIdx.x; j++) r += v[j] == n;
+  const int pos = off[(int64_t) g * (Nmax + 1) + n] + r;
+  perm[pos] = i;
+  sndep[pos] = n;
+}
+
+}  // namespace rhg
+
+namespace {
+
+// The depth counts the synthesis kernels read, on the device, without a
+// copy to the host: the caller's d_ndep checked and clamped to [4, Nmax]
+// into syn_ndep, and for a batch of 128 columns or more (column sorting on)
+// sorted by depth count, largest first, with syn_perm[slot] = the caller's
+// column (a wavefront of 64 columns waits for its deepest column).  The
+// flag (syn_hist's last word) ends as the number of columns out of range.
+// Sets *sndep_out, *perm_out (nullptr: slot = column) and *flag_out.
+int prepare_ndep(rhg_context_t *c, const int32_t *d_ndep, int ncol, int Nmax, bool sort,
+                 const int32_t **sndep_out, const int32_t **perm_out, int32_t **flag_out) {
+  sort = sort && c->tune_column_sort && ncol >= 128;
+  const int nchunk = (ncol + rhg::SORT_CHUNK - 1) / rhg::SORT_CHUNK;
+  const size_t hist_n = sort ? (size_t) nchunk * (Nmax + 2) : 0;
+  if (c->syn_ndep.ensure((size_t) ncol * sizeof(int32_t)) ||
+      (sort && c->syn_perm.ensure((size_t) ncol * sizeof(int32_t))) ||
+      c->syn_hist.ensure((hist_n + 1) * sizeof(int32_t)))
+    return fail(RHG_ENOMEM, "device allocation for the batch failed");
+  int32_t *sndep = (int32_t *) c->syn_ndep.p, *hist = (int32_t *) c->syn_hist.p;
+  int32_t *flag = hist + hist_n;
+  const size_t lds = (size_t) (Nmax + 2) * sizeof(int32_t);
+  if (sort) {
+    int32_t *perm = (int32_t *) c->syn_perm.p;
+    hipLaunchKernelGGL(rhg::k_sort_count, dim3((unsigned) nchunk), dim3(rhg::SORT_CHUNK), lds,
+                       c->stream, d_ndep, ncol, Nmax, hist);
+    HIPCHECK(hipGetLastError());
+    hipLaunchKernelGGL(rhg::k_sort_scan, dim3(1), dim3(1024), lds, c->stream, hist, nchunk, Nmax, flag);
+    HIPCHECK(hipGetLastError());
+    hipLaunchKernelGGL(rhg::k_sort_place, dim3((unsigned) nchunk), dim3(rhg::SORT_CHUNK), 0,
+                       c->stream, d_ndep, ncol, Nmax, hist, perm, sndep);
+    HIPCHECK(hipGetLastError());
+    *perm_out = perm;
+  } else {
+    hipLaunchKernelGGL(rhg::k_clear_flag, dim3(1), dim3(64), 0, c->stream, flag);
+    HIPCHECK(hipGetLastError());
+    hipLaunchKernelGGL(rhg::k_check_ndep, dim3((unsigned) ((ncol + 255) / 256)), dim3(256), 0,
+                       c->stream, d_ndep, ncol, Nmax, sndep, flag);
+    HIPCHECK(hipGetLastError());
+    *perm_out = nullptr;
+  }
+  *sndep_out = sndep;
+  *flag_out = flag;
+  return 0;
+}
+
+// The synthesis of a batch.  Host buffers (device_ptrs 0) are checked on
+// the host and copied in; device buffers are checked on the device (no copy
+// to the host before the launches).  sort: run a batch of uneven depth
+// counts in depth-count order (the kernels read the caller's atmosphere and
+// write its Stokes vectors through the permutation); the diagnostic outputs
+// and the callers that reuse the context's column state take the batch in
+// the caller's order.
+int synth(rhg_context_t *c, const rhg_columns_t *cols, int32_t precision, int32_t device_ptrs,
+          bool sort, double *stokes_out, double *height_out, double *chiref_out) {
+  if (!c || !cols || !stokes_out || !cols->atmos || !cols->ndep) return fail(RHG_EINVAL, "null argument");
   if (c->nout < 1) return fail(RHG_EINVAL, "the model has no output wavelength");
   if (precision != RHG_PREC_FP64 && precision != RHG_PREC_FP32)
     return fail(RHG_EINVAL, "precision must be RHG_PREC_FP64 or RHG_PREC_FP32");
@@ -528,15 +692,12 @@ static int synth(rhg_context_t *c, const rhg_columns_t *cols, int32_t precision,
   if ((int64_t) ncol * Nmax > (int64_t) INT32_MAX) return fail(RHG_EINVAL, "batch too large");
   if (!(cols->mu > 0.0 && cols->mu <= 1.0)) return fail(RHG_EINVAL, "mu must be in (0, 1]");
   if (cols->atm_scale < 0 || cols->atm_scale > 2) return fail(RHG_EINVAL, "bad atm_scale");
-  std::vector<int32_t> hndep(ncol);
-  if (device_ptrs) {
-    HIPCHECK(hipMemcpy(hndep.data(), cols->ndep, ncol * sizeof(int32_t), hipMemcpyDeviceToHost));
-  } else {
-    memcpy(hndep.data(), cols->ndep, ncol * sizeof(int32_t));
-  }
-  for (int i = 0; i < ncol; i++)
-    if (hndep[i] < 4 || hndep[i] > Nmax) return fail(RHG_EINVAL, "ndep out of range");
+  if (device_ptrs && height_out) return fail(RHG_EINVAL, "height output needs host buffers");
+  if (!device_ptrs)
+    for (int i = 0; i < ncol; i++)
+      if (cols->ndep[i] < 4 || cols->ndep[i] > Nmax) return fail(RHG_EINVAL, "ndep out of range");
   HIPCHECK(hipSetDevice(c->device));
+  if (height_out || chiref_out) sort = false;
 
   const int nfield = rhg::state_nfield(c->ncont, c->nline);
   const size_t rsz = precision == RHG_PREC_FP32 ? sizeof(float) : sizeof(double);
@@ -560,10 +721,13 @@ static int synth(rhg_context_t *c, const rhg_columns_t *cols, int32_t precision,
     d_ndep = (const int32_t *) c->ndep.p;
     d_out = (double *) c->stokes.p;
   }
+  const int32_t *sndep = nullptr, *perm = nullptr;
+  int32_t *flag = nullptr;
+  if (int rc = prepare_ndep(c, d_ndep, ncol, Nmax, sort, &sndep, &perm, &flag)) return rc;
   double *state = (double *) c->state.p;
   int32_t *moving = (int32_t *) c->moving.p;
   double *chiref = chiref_out ? (double *) c->chiref.p : nullptr;
-  if (int rc = launch_synth(c, d_atm, d_ndep, ncol, Nmax, cols->mu, cols->atm_scale, precision,
+  if (int rc = launch_synth(c, d_atm, sndep, perm, ncol, Nmax, cols->mu, cols->atm_scale, precision,
                             nfield, state, moving, chiref, c->cchi.p, c->ceta.p, d_out))
     return rc;
   if (!device_ptrs)
@@ -577,17 +741,20 @@ static int synth(rhg_context_t *c, const rhg_columns_t *cols, int32_t precision,
   if (chiref_out)
     HIPCHECK(hipMemcpyAsync(chiref_out, chiref, (size_t) ncol * Nmax * sizeof(double),
                             hipMemcpyDeviceToHost, c->stream));
+  int32_t nbad = 0;
+  HIPCHECK(hipMemcpyAsync(&nbad, flag, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
+  if (nbad) return fail(RHG_EINVAL, "ndep out of range");
   if (height_out)
     for (int ic = 0; ic < ncol; ic++)
       for (int k = 0; k < Nmax; k++)
-        height_out[(size_t) ic * Nmax + k] = k < hndep[ic] ? hfield[(size_t) k * ncol + ic] : 0.0;
+        height_out[(size_t) ic * Nmax + k] =
+            k < cols->ndep[ic] ? hfield[(size_t) k * ncol + ic] : 0.0;
   c->last_ncol = ncol;
   c->last_Nmax = Nmax;
   c->last_nfield = nfield;
   c->last_mu = cols->mu;
-  c->last_ndep = d_ndep;
-  c->last_hndep.swap(hndep);
+  c->last_ndep = sndep;
   // events of the stages: state, geometry, continuum, formal
   float ms;
   for (int s = 0; s < 4; s++) {
@@ -598,16 +765,19 @@ static int synth(rhg_context_t *c, const rhg_columns_t *cols, int32_t precision,
   return 0;
 }
 
-}  // extern "C"
+}  // namespace
+
+
 
 namespace {
 
-// A batch whose depth counts differ runs with its columns sorted by depth
-// count, longest first: a wavefront holds 64 columns at one wavelength, and
-// each lane's depth recursion is as long as its column, so a wavefront of
-// mixed depths waits for its deepest column.  Returns the permutation
-// (empty when the batch is small or already ordered); perm[i] is the
-// caller's column at sorted position i.
+// A response-function batch whose depth counts differ runs with its columns
+// gathered in depth-count order, longest first (the synthesis sorts on the
+// device instead, prepare_ndep): a wavefront holds 64 columns at one
+// wavelength, and each lane's depth recursion is as long as its column, so a
+// wavefront of mixed depths waits for its deepest column.  Returns the
+// permutation (empty when the batch is small or already ordered); perm[i] is
+// the caller's column at sorted position i.
 int sort_order(rhg_context_t *c, const rhg_columns_t *cols, int32_t device_ptrs,
                std::vector<int32_t> &perm, std::vector<int32_t> &sorted_ndep) {
   perm.clear();
@@ -655,9 +825,6 @@ int gather_sorted(rhg_context_t *c, const rhg_columns_t *cols, int32_t device_pt
                      src, (double *) c->srt_atm.p, (const int32_t *) c->srt_perm.p, ncol,
                      (int64_t) 9 * Nmax);
   HIPCHECK(hipGetLastError());
-  // the synthesis reads the sorted depth counts back with a blocking copy,
-  // which does not order with this (non-blocking) stream
-  HIPCHECK(hipStreamSynchronize(c->stream));
   cols2 = *cols;
   cols2.atmos = (const double *) c->srt_atm.p;
   cols2.ndep = (const int32_t *) c->srt_ndep.p;
@@ -685,28 +852,12 @@ extern "C" {
 
 int rhg_synth_lte(rhg_context_t *ctx, const rhg_columns_t *cols, int32_t precision,
                   int32_t device_ptrs, double *stokes_out) {
-  if (!ctx || !cols || !stokes_out) return fail(RHG_EINVAL, "null argument");
-  std::vector<int32_t> perm, sndep;
-  if (cols->ncol > 0 && cols->ncol <= (1 << 28))
-    if (int rc = sort_order(ctx, cols, device_ptrs, perm, sndep)) return rc;
-  if (perm.empty()) return synth(ctx, cols, precision, device_ptrs, stokes_out, nullptr, nullptr);
-  for (int32_t n : sndep)
-    if (n < 4 || n > cols->ndep_max) return fail(RHG_EINVAL, "ndep out of range");
-  rhg_columns_t cols2;
-  if (int rc = gather_sorted(ctx, cols, device_ptrs, perm, sndep, cols2)) return rc;
-  if (ctx->srt_out.ensure((size_t) cols->ncol * 4 * ctx->nout * sizeof(double)))
-    return fail(RHG_ENOMEM, "device allocation for the sorted batch failed");
-  if (int rc = synth(ctx, &cols2, precision, 1, (double *) ctx->srt_out.p, nullptr, nullptr)) return rc;
-  if (int rc = scatter_stokes(ctx, cols->ncol, device_ptrs, (const double *) ctx->srt_out.p, stokes_out))
-    return rc;
-  HIPCHECK(hipStreamSynchronize(ctx->stream));
-  g_err.clear();
-  return 0;
+  return synth(ctx, cols, precision, device_ptrs, true, stokes_out, nullptr, nullptr);
 }
 
 int rhg_synth_lte_diag(rhg_context_t *ctx, const rhg_columns_t *cols, int32_t precision,
                        double *stokes_out, double *height_out, double *chiref_out) {
-  return synth(ctx, cols, precision, 0, stokes_out, height_out, chiref_out);
+  return synth(ctx, cols, precision, 0, false, stokes_out, height_out, chiref_out);
 }
 
 }  // extern "C"
@@ -716,26 +867,6 @@ namespace {
 size_t align256(size_t n) { return (n + 255) & ~(size_t) 255; }
 
 }  // namespace
-
-namespace rhg {
-
-// flag[0] = 0 (the error flag must start at zero on every call)
-__global__ void k_clear_flag(int32_t *flag) {
-  if (threadIdx.x == 0) flag[0] = 0;
-}
-
-// out[i] = ndep[i] clamped to [4, Nmax]; flag[0] = 1 if any was outside
-__global__ void k_check_ndep(const int32_t *__restrict__ ndep, int ncol, int Nmax,
-                             int32_t *__restrict__ out, int32_t *__restrict__ flag) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ncol) return;
-  const int n = ndep[i];
-  const bool bad = n < 4 || n > Nmax;
-  out[i] = bad ? (n < 4 ? 4 : Nmax) : n;
-  if (bad) atomicOr(flag, 1);
-}
-
-}  // namespace rhg
 
 extern "C" {
 
@@ -783,7 +914,7 @@ int rhg_synth_lte_noalloc(rhg_context_t *ctx, const rhg_columns_t *cols, int32_t
   hipLaunchKernelGGL(rhg::k_check_ndep, dim3((unsigned) ((ncol + 255) / 256)), dim3(256), 0,
                      ctx->stream, cols->ndep, ncol, Nmax, sndep, flag);
   HIPCHECK(hipGetLastError());
-  if (int rc = launch_synth(ctx, (const double *) cols->atmos, sndep, ncol, Nmax, cols->mu,
+  if (int rc = launch_synth(ctx, (const double *) cols->atmos, sndep, nullptr, ncol, Nmax, cols->mu,
                             cols->atm_scale, precision, nfield, state, moving, nullptr, cchi,
                             ceta, stokes_out))
     return rc;
@@ -825,8 +956,10 @@ int rhg_count_work(rhg_context_t *c, int64_t *counts_out) {
   for (int i = 0; i < RHG_NWORK; i++) counts_out[i] = 0;
   for (int ic = 0; ic < ncol; ic++)
     for (int i = 0; i < RHG_NWORK; i++) counts_out[i] += cs[(size_t) ic * RHG_NWORK + i];
+  std::vector<int32_t> hn(ncol);
+  HIPCHECK(hipMemcpy(hn.data(), c->last_ndep, ncol * sizeof(int32_t), hipMemcpyDeviceToHost));
   int64_t cd = 0;
-  for (int ic = 0; ic < ncol; ic++) cd += c->last_hndep[ic];
+  for (int ic = 0; ic < ncol; ic++) cd += hn[ic];
   counts_out[RHG_WORK_COLDEPTH] = cd;
   g_err.clear();
   return 0;
@@ -848,8 +981,8 @@ int launch_base(rhg_context_t *c, const double *d_atm, const int32_t *d_ndep, in
     switch (c->dm.nnuclei) {
 #define RHG_LAUNCH_STATE(NN)                                                                   \
   case NN:                                                                                     \
-    hipLaunchKernelGGL(rhg::k_state<NN>, grd, blk, lds, c->stream, c->d_model, d_atm, d_ndep, ncol, \
-                       Nmax, mu, atm_scale, state, nfield);                                    \
+    hipLaunchKernelGGL(rhg::k_state<NN>, grd, blk, lds, c->stream, c->d_model, d_atm, d_ndep,     \
+                       nullptr, ncol, Nmax, mu, atm_scale, state, nfield);                     \
     break;
       RHG_LAUNCH_STATE(1) RHG_LAUNCH_STATE(2) RHG_LAUNCH_STATE(3) RHG_LAUNCH_STATE(4)
       RHG_LAUNCH_STATE(5) RHG_LAUNCH_STATE(6) RHG_LAUNCH_STATE(7) RHG_LAUNCH_STATE(8)
@@ -984,7 +1117,7 @@ int rf_chunk(rhg_context_t *c, const double *d_atm, const int32_t *d_ndep, int n
     dim3 grd((unsigned) rhg::xcd_grid(ncb, c->n_scal));
     hipLaunchKernelGGL(rhg::k_formal_scalar<R>, grd, blk, 0, c->stream, c->d_model, d_ndep, moving,
                        ncol, Nmax, state, c->d_out_ls, c->d_io_scal, c->n_scal, nout,
-                       (const R *) cchi, (const R *) ceta, mu, d_out);
+                       (const R *) cchi, (const R *) ceta, mu, d_out, nullptr);
     HIPCHECK(hipGetLastError());
   }
   HIPCHECK(hipEventRecord(c->ev[5], c->stream));
@@ -1195,7 +1328,7 @@ int rhg_rf_loggf(rhg_context_t *c, const rhg_columns_t *cols, int32_t npar,
     return fail(RHG_EINVAL, "bad batch shape (at most RHG_RF_MAX_COLUMNS columns per call)");
   // the synthesis leaves the column state, the FP64 continuum and the
   // depth counts of this batch in the context
-  int rc = synth(c, cols, RHG_PREC_FP64, device_ptrs, stokes_out, nullptr, nullptr);
+  int rc = synth(c, cols, RHG_PREC_FP64, device_ptrs, false, stokes_out, nullptr, nullptr);
   if (rc) return rc;
   const size_t rf_b = (size_t) ncol * nout * npar * sizeof(double);
   const int nscal = c->n_scal > 0 ? c->n_scal : 1;
@@ -1505,7 +1638,8 @@ int rhg_nlte_solve(rhg_context_t *c, const rhg_nlte_t *a, const rhg_columns_t *c
     // set-up: column state, scattering in chi_ref, depth scale, background,
     // populations, collisions, profiles
     int rc = timed(0, [&]() -> int {
-      if (int r = launch_state(c, d_atm, d_ndep, ncol, Nmax, cols->mu, cols->atm_scale, B.state, nfield))
+      if (int r = launch_state(c, d_atm, d_ndep, nullptr, ncol, Nmax, cols->mu, cols->atm_scale,
+                               B.state, nfield))
         return r;
       hipLaunchKernelGGL(rhg::k_nlte_pre, gpts, b64, 0, c->stream, c->d_model, d_at, B);
       HIPCHECK(hipGetLastError());
@@ -1969,7 +2103,7 @@ int rhg_scales(rhg_context_t *c, const rhg_columns_t *cols, int32_t device_ptrs,
     d_m = d_h + (size_t) ncol * Nmax;
   }
   double *state = (double *) c->state.p;
-  if (int rc = launch_state(c, d_atm, d_ndep, ncol, Nmax, 1.0, cols->atm_scale, state, nfield))
+  if (int rc = launch_state(c, d_atm, d_ndep, nullptr, ncol, Nmax, 1.0, cols->atm_scale, state, nfield))
     return rc;
   dim3 blk(64), grd((unsigned) ((ncol + 63) / 64));
   hipLaunchKernelGGL(rhg::k_scales, grd, blk, 0, c->stream, c->d_model, d_ndep, ncol, Nmax,

@@ -215,13 +215,13 @@ k_rf_loggf(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep,
 #define RHG_KFORMAL(KNAME, R)                                                                 \
   template __global__ void KNAME<R>(const DevModel *, const int32_t *, const int32_t *, int, int, \
                                     const double *, const int32_t *, const int32_t *, int, int,  \
-                                    const R *, const R *, double, double *);
+                                    const R *, const R *, double, double *, const int32_t *);
 #define RHG_KSTOKES(R, MO, LT)                                                                \
   template __global__ void k_formal_stokes<R, MO, LT>(const DevModel *, const int32_t *,            \
                                                       const int32_t *, int, int, const double *,    \
                                                       const int32_t *, const int32_t *, int, int,   \
-                                                      const R *, const R *, double, double *, int,  \
-                                                      int);
+                                                      const R *, const R *, double, double *,       \
+                                                      const int32_t *, int, int);
 RHG_KSTOKES(double, false, false)
 RHG_KSTOKES(double, true, false)
 RHG_KSTOKES(double, false, true)

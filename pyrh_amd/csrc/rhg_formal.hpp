@@ -71,10 +71,11 @@ __device__ __forceinline__ void form_point(const DevModel &M, int ls, int has_li
   B2[0] = planck(Tc[(int64_t) (N - 2) * ncol], lc[RHG_LC_HC_KLA], lc[RHG_LC_TWOHNU3_C2]); \
   B2[1] = planck(Tc[(int64_t) (N - 1) * ncol], lc[RHG_LC_HC_KLA], lc[RHG_LC_TWOHNU3_C2]);
 
-// the same, with the column's Stokes output for wavelength io
+// the same, with the column's Stokes output for wavelength io (slot ic holds
+// the caller's column perm[ic], or ic without a permutation)
 #define RAY_SETUP()                                                                      \
   RAY_SETUP_BASE();                                                                      \
-  double *out = stokes + (int64_t) ic * 4 * nout + io;
+  double *out = stokes + (int64_t) (perm ? perm[ic] : ic) * 4 * nout + io;
 
 #define Z(kk) zc[(int64_t) (kk) * ncol]
 // |z[kk-1] - z[kk]| (kk >= 1), from the column state
@@ -101,7 +102,8 @@ k_formal_stokes(const DevModel *__restrict__ Mp, const int32_t *__restrict__ nde
                 const double *__restrict__ state, const int32_t *__restrict__ out_ls,
                 const int32_t *__restrict__ io_list, int nlist, int nout,
                 const R *__restrict__ cchi, const R *__restrict__ ceta, double mu_d,
-                double *__restrict__ stokes, int tab_maxl, int tab_maxc) {
+                double *__restrict__ stokes, const int32_t *__restrict__ perm, int tab_maxl,
+                int tab_maxc) {
   // the line table first: every lane of the wavefront stages it
   auto tab = [&]() {
     int cb0, t0;
@@ -156,7 +158,7 @@ k_formal_scalar(const DevModel *__restrict__ Mp, const int32_t *__restrict__ nde
                 const double *__restrict__ state, const int32_t *__restrict__ out_ls,
                 const int32_t *__restrict__ io_list, int nlist, int nout,
                 const R *__restrict__ cchi, const R *__restrict__ ceta, double mu_d,
-                double *__restrict__ stokes) {
+                double *__restrict__ stokes, const int32_t *__restrict__ perm) {
   RAY_SETUP();
   const int solver = column_solver(ldc(M.solver)[ls], moving);
   Point<R> Pn;

@@ -154,14 +154,16 @@ __device__ __forceinline__ void continuum_point(const DevModel &M, int ls, const
 
 template <int NN>
 __global__ void __launch_bounds__(64)
-k_state(const DevModel *__restrict__ Mp, const double *__restrict__ atmos, const int32_t *__restrict__ ndep, int ncol,
-        int Nmax, double mu, int atm_scale, double *__restrict__ state, int nfield) {
+k_state(const DevModel *__restrict__ Mp, const double *__restrict__ atmos, const int32_t *__restrict__ ndep,
+        const int32_t *__restrict__ perm, int ncol, int Nmax, double mu, int atm_scale,
+        double *__restrict__ state, int nfield) {
   const DevModel &M = *Mp;
   int64_t gid = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= (int64_t) ncol * Nmax) return;
   const int k = (int) (gid / ncol), ic = (int) (gid % ncol);
   if (k >= ndep[ic]) return;
-  const double *atm = atmos + (int64_t) ic * 9 * Nmax;
+  // slot ic of the batch holds the caller's column perm[ic] (identity without perm)
+  const double *atm = atmos + (int64_t) (perm ? perm[ic] : ic) * 9 * Nmax;
   double *st = state + (int64_t) k * ncol + ic;
   const int64_t fs = (int64_t) Nmax * ncol;
 
@@ -639,8 +641,8 @@ k_continuum(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep, i
 }
 
 #define RHG_KSTATE(NN)                                                                       \
-  template __global__ void k_state<NN>(const DevModel *, const double *, const int32_t *, int, int,  \
-                                       double, int, double *, int);
+  template __global__ void k_state<NN>(const DevModel *, const double *, const int32_t *,        \
+                                       const int32_t *, int, int, double, int, double *, int);
 RHG_KSTATE(1)
 RHG_KSTATE(2)
 RHG_KSTATE(3)

@@ -464,8 +464,9 @@ __device__ __forceinline__ void line_census(const DevModel &M, int ls, const dou
 }
 
 template <int NN>
-__global__ void k_state(const DevModel *M, const double *atmos, const int32_t *ndep, int ncol, int Nmax,
-                        double mu, int atm_scale, double *state, int nfield);
+__global__ void k_state(const DevModel *M, const double *atmos, const int32_t *ndep,
+                        const int32_t *perm, int ncol, int Nmax, double mu, int atm_scale,
+                        double *state, int nfield);
 __global__ void k_geometry(const DevModel *M, const int32_t *ndep, int ncol, int Nmax, int atm_scale,
                            double mu, double *state, int nfield, int32_t *moving_out,
                            double *chiref_out);
@@ -480,13 +481,13 @@ template <typename R, bool MO, bool LT>
 __global__ void k_formal_stokes(const DevModel *M, const int32_t *ndep, const int32_t *moving,
                                 int ncol, int Nmax, const double *state, const int32_t *out_ls,
                                 const int32_t *io_list, int nlist, int nout, const R *cchi,
-                                const R *ceta, double mu, double *stokes, int tab_maxl,
-                                int tab_maxc);
+                                const R *ceta, double mu, double *stokes, const int32_t *perm,
+                                int tab_maxl, int tab_maxc);
 template <typename R>
 __global__ void k_formal_scalar(const DevModel *M, const int32_t *ndep, const int32_t *moving,
                                 int ncol, int Nmax, const double *state, const int32_t *out_ls,
                                 const int32_t *io_list, int nlist, int nout, const R *cchi,
-                                const R *ceta, double mu, double *stokes);
+                                const R *ceta, double mu, double *stokes, const int32_t *perm);
 __global__ void k_rf_loggf(const DevModel *M, const int32_t *ndep, const int32_t *moving, int ncol,
                            int Nmax, const double *state, const int32_t *out_ls,
                            const int32_t *io_list, int nlist, int nout, const double *cchi,

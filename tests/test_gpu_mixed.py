@@ -106,6 +106,58 @@ def test_mixed_device_pointers(eng):
     np.testing.assert_array_equal(d_out.cpu().numpy(), host)
 
 
+def _synth_device(eng, atm, nd, mu=0.7, precision=1):
+    """rhg_synth_lte on device buffers; the output starts as NaN so that a
+    column the kernels never write shows."""
+    import torch
+    dev = torch.device("cuda:0")
+    d_atm = torch.from_numpy(np.ascontiguousarray(atm)).to(dev)
+    d_nd = torch.from_numpy(np.ascontiguousarray(nd)).to(dev)
+    out = torch.full((atm.shape[0], 4, eng.nout), float("nan"), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    eng.synth_lte_device(d_atm.data_ptr(), d_nd.data_ptr(), atm.shape[0], atm.shape[2],
+                         out.data_ptr(), mu=mu, precision=precision)
+    return out.cpu().numpy()
+
+
+def test_device_sort_many_chunks(eng):
+    """2500 uneven columns on device buffers (three workgroups of the
+    on-device counting sort by depth count): with sorting on, every column
+    bit for bit as with sorting off and as in the reversed batch, and every
+    output written."""
+    atm, nd = W.mixed_batch(2500, seed=W.SEED_BASE + 34)
+    try:
+        s1 = _synth_device(eng, atm, nd)
+        eng.set_tuning("column_sort", 0)
+        s0 = _synth_device(eng, atm, nd)
+    finally:
+        eng.set_tuning("column_sort", 1)
+    sr = _synth_device(eng, atm[::-1], nd[::-1])
+    assert np.all(np.isfinite(s1))
+    np.testing.assert_array_equal(s1, s0)
+    np.testing.assert_array_equal(s1, sr[::-1])
+
+
+@pytest.mark.parametrize("sort", (1, 0))
+def test_device_ndep_out_of_range(eng, sort):
+    """A depth count outside [4, ndep_max] in device memory: the call
+    returns the error (the kernels ran on clamped counts, inside the
+    buffers), and the next good call on the context is unaffected."""
+    from pyrh_amd.engine import EngineError
+    atm, nd = W.mixed_batch(300, seed=W.SEED_BASE + 35)
+    good = _synth_device(eng, atm, nd)
+    try:
+        eng.set_tuning("column_sort", sort)
+        for bad in (3, atm.shape[2] + 1, -7):
+            nd2 = nd.copy()
+            nd2[137] = bad
+            with pytest.raises(EngineError, match="ndep out of range"):
+                _synth_device(eng, atm, nd2)
+    finally:
+        eng.set_tuning("column_sort", 1)
+    np.testing.assert_array_equal(_synth_device(eng, atm, nd), good)
+
+
 def test_mixed_rf_sorted(eng):
     """Response functions of a sorted batch: each column as it comes alone,
     zero past its depth, and the T response against central differences
