@@ -220,7 +220,10 @@ def test_rf_fp32_form_full_batch_at_conditioning(setups):
     depth)'s largest response over Stokes and wavelength: the 99th
     percentile below 1e-4 for every parameter, the share above 1e-4 at most
     twice that of the perturbed FP64 run (plus 2e-4), and for T (no kink
-    near its line cores) at most 1e-5."""
+    near its line cores) at most 1e-5.  On every column, the azimuth
+    responses summed over depth obey the rotation identity of
+    test_rf_azimuth_rotation_sum (1e-8 I_c in FP64, 1e-6 I_c in the FP32
+    form)."""
     import torch
     from pyrh_amd import workloads as W
     from pyrh_amd.engine import Engine
@@ -231,7 +234,7 @@ def test_rf_fp32_form_full_batch_at_conditioning(setups):
     d_atm = torch.from_numpy(atm).to(dev)
     d_nd = torch.full((n,), nd, dtype=torch.int32, device=dev)
     st = torch.empty((n, 4, eng.nout), dtype=torch.float64, device=dev)
-    rf = {}
+    rf, sts = {}, {}
     for key, prec, scale in (("fp64", 0, None), ("fp32", 1, None), ("pert", 0, 1e-7)):
         a = d_atm
         if scale:
@@ -244,7 +247,20 @@ def test_rf_fp32_form_full_batch_at_conditioning(setups):
                           precision=prec)
         torch.cuda.synchronize()
         rf[key] = r
+        sts[key] = st.clone()
     ref = rf["fp64"]
+    # the azimuth-rotation identity (test_rf_azimuth_rotation_sum) on every column
+    Ic = sts["fp64"][:, 0, :1].T                                      # [1, n]
+    ich = RF_PARAMS.index("chi")
+    for key, atol in (("fp64", 1e-8), ("fp32", 1e-6)):
+        tot = rf[key][ich].double().sum(dim=0)                        # [4, nout, n]
+        sv = sts[key].permute(1, 2, 0)                                # [4, nout, n]
+        sense = float(torch.sign((tot[2] * sv[1]).sum()))
+        worst = max(float((tot[0] / Ic).abs().max()), float((tot[3] / Ic).abs().max()),
+                    float(((tot[1] + 2 * sense * sv[2]) / Ic).abs().max()),
+                    float(((tot[2] - 2 * sense * sv[1]) / Ic).abs().max()))
+        print(f"{key}: azimuth-sum identity, worst |residual|/I_c {worst:.2e}", flush=True)
+        assert sense != 0 and worst < atol
     for ip, name in enumerate(RF_PARAMS):
         s = ref[ip].abs().amax(dim=(1, 2), keepdim=True).clamp_min(1e-300)
         e32 = (rf["fp32"][ip].double() - ref[ip]).abs() / s
