@@ -428,14 +428,14 @@ int rhg_open(const rhg_model_t *m, int device, rhg_context_t **out) {
   return 0;
 }
 
-// Stage 1 (k_state) for a batch: one thread per (column, depth), the
+// Stage 1 (k_state) for a batch: one thread per (column, depth), a
+// workgroup per (column block, depth) in the XCD-aware order, the
 // chemical-equilibrium kernel instantiated for the model's nucleus count.
 // perm (or nullptr): slot i of the batch reads the atmosphere of column perm[i].
 static int launch_state(rhg_context_t *c, const double *d_atm, const int32_t *d_ndep,
                         const int32_t *perm, int ncol, int Nmax, double mu, int atm_scale,
                         double *state, int nfield) {
-  const int64_t npts = (int64_t) ncol * Nmax;
-  dim3 blk(64), grd((unsigned) ((npts + 63) / 64));
+  dim3 blk(64), grd((unsigned) rhg::xcd_grid((ncol + 63) / 64, Nmax));
   const size_t lds = (size_t) 3 * c->dm.nmol * 64 * sizeof(double);
   switch (c->dm.nnuclei) {
 #define RHG_LAUNCH_STATE(NN)                                                                    \
@@ -976,7 +976,7 @@ int launch_base(rhg_context_t *c, const double *d_atm, const int32_t *d_ndep, in
                 double *chiref, R *cchi, R *ceta) {
   const int64_t npts = (int64_t) ncol * Nmax;
   {
-    dim3 blk(64), grd((unsigned) ((npts + 63) / 64));
+    dim3 blk(64), grd((unsigned) rhg::xcd_grid((ncol + 63) / 64, Nmax));
     const size_t lds = (size_t) 3 * c->dm.nmol * 64 * sizeof(double);
     switch (c->dm.nnuclei) {
 #define RHG_LAUNCH_STATE(NN)                                                                   \

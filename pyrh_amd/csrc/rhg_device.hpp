@@ -65,6 +65,29 @@ __device__ __forceinline__ double linear1(int n, const double *xt, const double 
   return fx * yt[j] + (1 - fx) * yt[j + 1];
 }
 
+// linear1 at one abscissa x for several tables yt on the same xt: the
+// bracket and weight are found once, with linear1's arithmetic, so each
+// value is bit-identical to linear1(n, xt, yt, x).
+struct Linear1At {
+  int n, j, edge;   // edge -1: x <= xt[0]; 1: x >= xt[n - 1]; 0: inside
+  double fx;
+  __device__ __forceinline__ Linear1At(int n_, const double *xt, double x) : n(n_), j(0), edge(0), fx(0.0) {
+    if (x <= xt[0]) {
+      edge = -1;
+    } else if (x >= xt[n - 1]) {
+      edge = 1;
+    } else {
+      j = locate_asc(n, xt, x);
+      fx = (xt[j + 1] - x) / (xt[j + 1] - xt[j]);
+    }
+  }
+  __device__ __forceinline__ double operator()(const double *yt) const {
+    if (edge < 0) return yt[0];
+    if (edge > 0) return yt[n - 1];
+    return fx * yt[j] + (1 - fx) * yt[j + 1];
+  }
+};
+
 // rh/hydrogen.c:998-1020
 __device__ __forceinline__ double bilinear(int Ncol, int Nrow, const double *f, double x, double y) {
   int i = (int) x;

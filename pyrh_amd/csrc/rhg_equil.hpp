@@ -9,19 +9,19 @@
 namespace rhg {
 
 __device__ inline double equilconstant(const DevModel &M, int im, double T) {
-  int neqc = M.mol_neqc[im];
-  const double *c = M.mol_eqc + 8 * im;
+  int neqc = ldc(M.mol_neqc)[im];
+  CDouble *c = ldc(M.mol_eqc) + 8 * im;
   double kT, t, eqc = 0.0, theta, cgs_to_SI = 1.0;
-  if (T < M.mol_tmin[im] || T > M.mol_tmax[im]) return 0.0;
-  switch (M.mol_fit[im]) {
+  if (T < ldc(M.mol_tmin)[im] || T > ldc(M.mol_tmax)[im]) return 0.0;
+  switch (ldc(M.mol_fit)[im]) {
     case 0:
     case 1: {
-      t = (M.mol_fit[im] == 1) ? T * 1.0E-4 : T;
+      t = (ldc(M.mol_fit)[im] == 1) ? T * 1.0E-4 : T;
       kT = KBOLTZMANN * T;
-      int mk = M.mol_nnuclei[im] - 1 - M.mol_charge[im];
+      int mk = ldc(M.mol_nnuclei)[im] - 1 - ldc(M.mol_charge)[im];
       eqc = c[0];
       for (int i = 1; i < neqc; i++) eqc = eqc * t + c[i];
-      eqc = exp(M.mol_ediss[im] / kT + eqc - 1.5 * mk * log(T));
+      eqc = exp(ldc(M.mol_ediss)[im] / kT + eqc - 1.5 * mk * log(T));
       cgs_to_SI = pow(CM_TO_M * CM_TO_M * CM_TO_M, (double) mk);
       break;
     }
@@ -32,7 +32,7 @@ __device__ inline double equilconstant(const DevModel &M, int im, double T) {
       kT = KBOLTZMANN * T;
       eqc = c[0];
       for (int i = 1; i < neqc; i++) eqc = eqc * t + c[i];
-      eqc = pow10d((M.mol_ediss[im] / EV) * theta - eqc) * kT;
+      eqc = pow10d((ldc(M.mol_ediss)[im] / EV) * theta - eqc) * kT;
       cgs_to_SI = 1.0;
       break;
     case 4:
@@ -41,7 +41,8 @@ __device__ inline double equilconstant(const DevModel &M, int im, double T) {
       eqc = c[0];
       for (int i = 1; i < neqc; i++) eqc = eqc * theta + c[i];
       eqc = kT * kT * pow10d(-eqc);
-      cgs_to_SI = pow(CM_TO_M * CM_TO_M * CM_TO_M / ERG_TO_JOULE, (double) (M.mol_nnuclei[im] - 1));
+      cgs_to_SI = pow(CM_TO_M * CM_TO_M * CM_TO_M / ERG_TO_JOULE,
+                      (double) (ldc(M.mol_nnuclei)[im] - 1));
       break;
   }
   return eqc * cgs_to_SI;
@@ -112,7 +113,7 @@ __device__ void chem_eq(const DevModel &M, double T, double ne, double nHtot,
   const double CI = (HPLANCK / (2.0 * PI * M_ELECTRON)) * (HPLANCK / KBOLTZMANN);
   double a[NN];
 #pragma unroll
-  for (int i = 0; i < NN; i++) a[i] = M.nucl_abund[i] * nHtot;
+  for (int i = 0; i < NN; i++) a[i] = ldc(M.nucl_abund)[i] * nHtot;
   // tcache: pow(CI/T, 1.5) and exp(E_ION_HMIN/kT) formed by a caller that
   // solves this temperature again
   double PhiHmin = tcache ? 0.25 * tcache[0] * tcache[1]
@@ -142,7 +143,7 @@ __device__ void chem_eq(const DevModel &M, double T, double ne, double nHtot,
 #pragma unroll
     for (int i = 0; i < NN; i++) fn[i] = fn0[i] * nnuc[i];
     for (int m = 0; m < Nm; m++) {
-      const double *cnt = M.mol_cnt + m * NNUC_MAX;
+      CDouble *cnt = ldc(M.mol_cnt) + m * NNUC_MAX;
       const double nm = lds_nmol[m * ls_stride];
       double sv = lds_phi[m * ls_stride];
 #pragma unroll
@@ -151,7 +152,7 @@ __device__ void chem_eq(const DevModel &M, double T, double ne, double nHtot,
           sv *= ipow(fn[i], (int) cnt[i]);
           f1[i] += cnt[i] * nm;
         }
-      sv /= ipow(ne, M.mol_charge[m]);
+      sv /= ipow(ne, ldc(M.mol_charge)[m]);
       lds_saha[m * ls_stride] = sv;
     }
     // S = D - B C, rhs = f1 - B f2  (C[m][nu] = -saha_m count_nu / n_nu,
@@ -164,7 +165,7 @@ __device__ void chem_eq(const DevModel &M, double T, double ne, double nHtot,
     }
     S[0][0] += fHmin;
     for (int m = 0; m < Nm; m++) {
-      const double *cnt = M.mol_cnt + m * NNUC_MAX;
+      CDouble *cnt = ldc(M.mol_cnt) + m * NNUC_MAX;
       const double sv = lds_saha[m * ls_stride];
       const double f2 = lds_nmol[m * ls_stride] - sv;
       double C[NN];
@@ -203,7 +204,7 @@ __device__ void chem_eq(const DevModel &M, double T, double ne, double nHtot,
     // solutions (rh/maxchange.c:32-50)
     double dnmax = 0.0;
     for (int m = 0; m < Nm; m++) {
-      const double *cnt = M.mol_cnt + m * NNUC_MAX;
+      CDouble *cnt = ldc(M.mol_cnt) + m * NNUC_MAX;
       const double sv = lds_saha[m * ls_stride];
       const double nold = lds_nmol[m * ls_stride];
       double x2 = nold - sv;

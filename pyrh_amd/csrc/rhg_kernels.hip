@@ -158,9 +158,13 @@ k_state(const DevModel *__restrict__ Mp, const double *__restrict__ atmos, const
         const int32_t *__restrict__ perm, int ncol, int Nmax, double mu, int atm_scale,
         double *__restrict__ state, int nfield) {
   const DevModel &M = *Mp;
-  int64_t gid = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= (int64_t) ncol * Nmax) return;
-  const int k = (int) (gid / ncol), ic = (int) (gid % ncol);
+  // workgroup -> (column block, depth): the depths of a column block run one
+  // after another on one XCD, so the atmosphere's cache lines (16 depths of
+  // one row of one column) are fetched into that XCD's L2 once
+  int cb, k;
+  xcd_tile(blockIdx.x, Nmax, cb, k);
+  const int ic = cb * 64 + (int) threadIdx.x;
+  if (ic >= ncol) return;
   if (k >= ndep[ic]) return;
   // slot ic of the batch holds the caller's column perm[ic] (identity without perm)
   const double *atm = atmos + (int64_t) (perm ? perm[ic] : ic) * 9 * Nmax;
@@ -272,17 +276,21 @@ k_state(const DevModel *__restrict__ Mp, const double *__restrict__ atmos, const
   // Kurucz lines: lambda-independent per-depth quantities,
   // rh/kurucz.c:662-677 and rh/kurucz.c:743-775
   const int lbase = line_field_base(M.ncont);
+  // what depends on T (and n_e) alone, formed once for all lines: the
+  // partition-function bracket and the Saha factor
+  const Linear1At pfT(M.npf, M.Tpf, T);
+  const double C1 = (HPLANCK / (2.0 * PI * M_ELECTRON)) * (HPLANCK / KBOLTZMANN);
+  const double CT_ne = M.nline > 0 ? 2.0 * pow(C1 / T, -1.5) / ne : 0.0;
   for (int l = 0; l < M.nline; l++) {
     int ie = ldc(M.line_el)[l], stg = ldc(M.line_stage)[l];
     // stage populations of the element, rh/ltepops.c:116-164
-    const double C1 = (HPLANCK / (2.0 * PI * M_ELECTRON)) * (HPLANCK / KBOLTZMANN);
     const int npf = M.npf, ns = ldc(M.el_nstage)[ie];
     const double *lp = M.el_logpf + ldc(M.el_pf_ptr)[ie];
-    const double *ip = M.el_ionpot + ldc(M.el_pf_ptr)[ie];
-    double CT_ne = 2.0 * pow(C1 / T, -1.5) / ne, sum = 1.0, Uk, Ukp1, nst = 1.0, nstage = 1.0;
-    Uk = linear1(npf, M.Tpf, lp, T);
+    CDouble *ip = ldc(M.el_ionpot) + ldc(M.el_pf_ptr)[ie];
+    double sum = 1.0, Uk, Ukp1, nst = 1.0, nstage = 1.0;
+    Uk = pfT(lp);
     for (int i = 1; i < ns; i++) {
-      Ukp1 = linear1(npf, M.Tpf, lp + i * npf, T);
+      Ukp1 = pfT(lp + i * npf);
       nst = nst * CT_ne * exp(Ukp1 - Uk - ip[i - 1] / (KBOLTZMANN * T));
       sum += nst;
       if (i == stg) nstage = nst;
@@ -290,7 +298,7 @@ k_state(const DevModel *__restrict__ Mp, const double *__restrict__ atmos, const
     }
     double n0 = ldc(M.el_abund)[ie] * nHtot / sum;
     nstage = (stg == 0) ? n0 : nstage * n0;
-    double pf = linear1(npf, M.Tpf, lp + stg * npf, T);
+    double pf = pfT(lp + stg * npf);
     double kT = 1.0 / (KBOLTZMANN * T);
     double ni_gi = nstage * exp(-ldc(M.line_Ei)[l] * kT - pf);
     double nj_gj = ni_gi * exp(-ldc(M.line_hc_la)[l] * kT);
