@@ -106,8 +106,8 @@ struct rhg_context {
   const int32_t *d_io_pol = nullptr, *d_io_scal = nullptr;
   DevBuf atmos, ndep, state, moving, cchi, ceta, stokes, chiref, census, census_sum;
   // response functions: perturbed inputs/states, tangents, tape, adjoints
-  DevBuf rf_atm[2], rf_state[2], rf_cont[4], rf_chiref[2], rf_moving, rf_dstate, rf_dcont[2],
-      rf_dchiref, rf_car, rf_aP, rf_aZ, rf_aB, rf_out;
+  DevBuf rf_atm[2], rf_state[2], rf_cont[4], rf_chiref[2], rf_moving, rf_ndep, rf_dstate,
+      rf_dcont[2], rf_dchiref, rf_car, rf_aP, rf_aZ, rf_aB, rf_out;
   // log gf response functions: inward intensities, output, parameter lines
   DevBuf at_iin, at_out, at_par;
   // NLTE: the active atom (packed arrays and its DevNlte), the batch arrays
@@ -1035,46 +1035,67 @@ int rf_chunk(rhg_context_t *c, const double *d_atm, const int32_t *d_ndep, int n
   if (rc) return rc;
   HIPCHECK(hipEventRecord(c->ev[1], c->stream));
 
-  // 1. state tangents by central differences (every depth at once)
+  // 1. state tangents by central differences (every depth at once).  The
+  // perturbed copies of the chunk run side by side as one batch per group:
+  // group 0 is T (whose continuum and chi_ref tangents are taken too), group
+  // 1 the other parameters; copy 2j is parameter j of the group shifted up,
+  // 2j + 1 shifted down.  Each column's state does not depend on its
+  // neighbours, so the batch gives the same values as one launch per copy.
   const int64_t natm = (int64_t) ncol * 9 * Nmax;
+  const int64_t nst = npts * nfield;
   double *dstate = (double *) c->rf_dstate.p;
-  for (int ip = 0; ip < pl.npar; ip++) {
-    const int p = pl.p[ip];
-    const RfParam &P = kRfParam[p];
-    for (int sgn = 0; sgn < 2; sgn++) {
-      dim3 blk(256), grd((unsigned) ((natm + 255) / 256));
-      hipLaunchKernelGGL(rhg::k_rf_perturb, grd, blk, 0, c->stream, d_atm, (double *) c->rf_atm[sgn].p,
-                         natm, Nmax, P.row, sgn == 0 ? P.h : -P.h);
-      HIPCHECK(hipGetLastError());
-      const bool pT = p == RHG_RF_T;
-      rc = launch_base<double>(c, (const double *) c->rf_atm[sgn].p, d_ndep, ncol, Nmax, mu, scale,
-                               (double *) c->rf_state[sgn].p, nfield, (int32_t *) c->rf_moving.p,
-                               pT ? (double *) c->rf_chiref[sgn].p : nullptr,
-                               pT ? (double *) c->rf_cont[2 * sgn].p : nullptr,
-                               pT ? (double *) c->rf_cont[2 * sgn + 1].p : nullptr);
-      if (rc) return rc;
+  int32_t *rep_ndep = (int32_t *) c->rf_ndep.p;
+  {
+    dim3 blk(256), grd((unsigned) ((ncol + 255) / 256), (unsigned) (2 * pl.npar));
+    hipLaunchKernelGGL(rhg::k_rf_rep_ndep, grd, blk, 0, c->stream, d_ndep, ncol, rep_ndep);
+    HIPCHECK(hipGetLastError());
+  }
+  for (int grp = 0; grp < 2; grp++) {
+    int gp[RHG_RF_NPARAM], ng = 0;
+    for (int ip = 0; ip < pl.npar; ip++)
+      if ((pl.p[ip] == RHG_RF_T) == (grp == 0)) gp[ng++] = pl.p[ip];
+    if (ng == 0) continue;
+    rhg::RfShiftList shift{};
+    for (int j = 0; j < ng; j++) {
+      shift.row[2 * j] = shift.row[2 * j + 1] = kRfParam[gp[j]].row;
+      shift.h[2 * j] = kRfParam[gp[j]].h;
+      shift.h[2 * j + 1] = -kRfParam[gp[j]].h;
     }
-    const double inv2h = 0.5 / P.h;
-    const int64_t nst = npts * nfield;
+    const int W = 2 * ng * ncol;
+    double *atm_b = (double *) c->rf_atm[grp].p, *st_b = (double *) c->rf_state[grp].p;
     {
-      dim3 blk(256), grd((unsigned) ((nst + 255) / 256));
-      hipLaunchKernelGGL(rhg::k_rf_diff<double>, grd, blk, 0, c->stream,
-                         (const double *) c->rf_state[0].p, (const double *) c->rf_state[1].p, nst,
-                         inv2h, dstate + p * nst);
+      dim3 blk(256), grd((unsigned) ((natm + 255) / 256), (unsigned) (2 * ng));
+      hipLaunchKernelGGL(rhg::k_rf_perturb, grd, blk, 0, c->stream, d_atm, atm_b, natm, Nmax, shift);
       HIPCHECK(hipGetLastError());
     }
-    if (p == RHG_RF_T) {
+    const bool gT = grp == 0;
+    rc = launch_base<double>(c, atm_b, rep_ndep, W, Nmax, mu, scale, st_b, nfield,
+                             (int32_t *) c->rf_moving.p, gT ? (double *) c->rf_chiref[0].p : nullptr,
+                             gT ? (double *) c->rf_cont[0].p : nullptr,
+                             gT ? (double *) c->rf_cont[1].p : nullptr);
+    if (rc) return rc;
+    dim3 blk(256), grd((unsigned) ((nst + 255) / 256));
+    for (int j = 0; j < ng; j++) {
+      hipLaunchKernelGGL(rhg::k_rf_diff<double>, grd, blk, 0, c->stream, (const double *) st_b,
+                         (int64_t) W, (int64_t) 2 * j * ncol, (int64_t) (2 * j + 1) * ncol, ncol,
+                         (int64_t) nfield * Nmax, 0.5 / kRfParam[gp[j]].h, dstate + gp[j] * nst);
+      HIPCHECK(hipGetLastError());
+    }
+    if (gT) {
+      const double inv2h = 0.5 / kRfParam[RHG_RF_T].h;
       const int64_t ncc = npts * nout;
-      dim3 blk(256), grd((unsigned) ((ncc + 255) / 256));
+      dim3 grdc((unsigned) ((ncc + 255) / 256));
       for (int q = 0; q < 2; q++) {
-        hipLaunchKernelGGL(rhg::k_rf_diff<R>, grd, blk, 0, c->stream, (const double *) c->rf_cont[q].p,
-                           (const double *) c->rf_cont[2 + q].p, ncc, inv2h, (R *) c->rf_dcont[q].p);
+        hipLaunchKernelGGL(rhg::k_rf_diff<R>, grdc, blk, 0, c->stream, (const double *) c->rf_cont[q].p,
+                           (int64_t) W, (int64_t) 0, (int64_t) ncol, ncol, (int64_t) nout * Nmax,
+                           inv2h, (R *) c->rf_dcont[q].p);
         HIPCHECK(hipGetLastError());
       }
+      // chi_ref is [column][depth]: the two copies are the halves
       dim3 grd2((unsigned) ((npts + 255) / 256));
       hipLaunchKernelGGL(rhg::k_rf_diff<double>, grd2, blk, 0, c->stream,
-                         (const double *) c->rf_chiref[0].p, (const double *) c->rf_chiref[1].p, npts,
-                         inv2h, (double *) c->rf_dchiref.p);
+                         (const double *) c->rf_chiref[0].p, (int64_t) 0, (int64_t) 0, npts,
+                         (int) npts, (int64_t) 1, inv2h, (double *) c->rf_dchiref.p);
       HIPCHECK(hipGetLastError());
     }
   }
@@ -1141,16 +1162,20 @@ int rf_run(rhg_context_t *c, const rhg_columns_t *cols, int32_t device_ptrs,
   const int nfield = rhg::state_nfield(c->ncont, c->nline);
   const size_t npl = (size_t) (c->n_pol > 0 ? c->n_pol : 1);
   const size_t rsz = sizeof(R);
+  // the perturbed copies of the state tangents: 2 for T, 2 per other parameter
+  int nT = 0;
+  for (int i = 0; i < pl.npar; i++) nT += pl.p[i] == RHG_RF_T;
+  const size_t ncopy = 2 * (size_t) pl.npar, nother = (size_t) (pl.npar - nT);
   const size_t per_col =
-      (size_t) Nmax * ((size_t) nfield * 8 + 2 * nout * rsz + 8 + 2 * 9 * 8 + 2 * (size_t) nfield * 8 +
-                       4 * (size_t) nout * 8 + 2 * 8 + RHG_RF_NPARAM * (size_t) nfield * 8 +
-                       2 * nout * rsz + 8 + npl * rhg::RF_NCARRY * 8 + npl * 4 * 8 * rsz +
-                       npl * 4 * rsz) +
-      npl * 8 * rsz + 2 * sizeof(int32_t);
+      (size_t) Nmax * ((size_t) nfield * 8 + 2 * nout * rsz + 8 + ncopy * 9 * 8 +
+                       ncopy * (size_t) nfield * 8 + 4 * (size_t) nout * 8 + 2 * 8 +
+                       RHG_RF_NPARAM * (size_t) nfield * 8 + 2 * nout * rsz + 8 +
+                       npl * rhg::RF_NCARRY * 8 + npl * 4 * 8 * rsz + npl * 4 * rsz) +
+      npl * 8 * rsz + (1 + 2 * ncopy) * sizeof(int32_t);
   DevBuf *work[] = {&c->state, &c->moving, &c->cchi, &c->ceta, &c->chiref, &c->rf_atm[0],
                     &c->rf_atm[1], &c->rf_state[0], &c->rf_state[1], &c->rf_cont[0],
                     &c->rf_cont[1], &c->rf_cont[2], &c->rf_cont[3], &c->rf_chiref[0],
-                    &c->rf_chiref[1], &c->rf_moving, &c->rf_dstate, &c->rf_dcont[0],
+                    &c->rf_chiref[1], &c->rf_moving, &c->rf_ndep, &c->rf_dstate, &c->rf_dcont[0],
                     &c->rf_dcont[1], &c->rf_dchiref, &c->rf_car, &c->rf_aP, &c->rf_aZ, &c->rf_aB};
   size_t held = 0;
   for (DevBuf *b : work) held += b->bytes;
@@ -1186,12 +1211,12 @@ int rf_run(rhg_context_t *c, const rhg_columns_t *cols, int32_t device_ptrs,
       c->ceta.ensure(cc_b) || c->chiref.ensure((size_t) npts * sizeof(double)) ||
       (!device_ptrs && (c->atmos.ensure(atm_all_b) || c->ndep.ensure(ncol_all * sizeof(int32_t)) ||
                         c->stokes.ensure(out_b) || c->rf_out.ensure(rf_b))) ||
-      c->rf_atm[0].ensure(atm_b) || c->rf_atm[1].ensure(atm_b) || c->rf_state[0].ensure(st_b) ||
-      c->rf_state[1].ensure(st_b) || c->rf_cont[0].ensure(cc64_b) || c->rf_cont[1].ensure(cc64_b) ||
-      c->rf_cont[2].ensure(cc64_b) || c->rf_cont[3].ensure(cc64_b) ||
-      c->rf_chiref[0].ensure((size_t) npts * sizeof(double)) ||
-      c->rf_chiref[1].ensure((size_t) npts * sizeof(double)) ||
-      c->rf_moving.ensure(chunk * sizeof(int32_t)) ||
+      c->rf_atm[0].ensure(2 * nT * atm_b) || c->rf_atm[1].ensure(2 * nother * atm_b) ||
+      c->rf_state[0].ensure(2 * nT * st_b) || c->rf_state[1].ensure(2 * nother * st_b) ||
+      c->rf_cont[0].ensure(2 * nT * cc64_b) || c->rf_cont[1].ensure(2 * nT * cc64_b) ||
+      c->rf_chiref[0].ensure(2 * nT * (size_t) npts * sizeof(double)) ||
+      c->rf_moving.ensure(ncopy * chunk * sizeof(int32_t)) ||
+      c->rf_ndep.ensure(ncopy * chunk * sizeof(int32_t)) ||
       c->rf_dstate.ensure(st_b * RHG_RF_NPARAM) || c->rf_dcont[0].ensure(cc_b) ||
       c->rf_dcont[1].ensure(cc_b) || c->rf_dchiref.ensure((size_t) npts * sizeof(double)) ||
       c->rf_car.ensure(car_b) || c->rf_aP.ensure(aP_b) ||

@@ -494,10 +494,17 @@ __global__ void k_rf_loggf(const DevModel *M, const int32_t *ndep, const int32_t
                            const double *ceta, double mu, int npar, const int32_t *par_line,
                            double *Iin, double *rf);
 constexpr int RF_NCARRY = 13;  // carried scalars entering a DELO step (StepCarry)
-__global__ void k_rf_perturb(const double *atmos, double *out, int64_t n, int Nmax, int row,
-                             double delta);
+// row shifts of the perturbed copies of a batch (two per parameter)
+struct RfShiftList {
+  int32_t row[2 * RHG_RF_NPARAM];
+  double h[2 * RHG_RF_NPARAM];
+};
+__global__ void k_rf_perturb(const double *atmos, double *out, int64_t n, int Nmax,
+                             RfShiftList shift);
+__global__ void k_rf_rep_ndep(const int32_t *ndep, int ncol, int32_t *out);
 template <typename O>
-__global__ void k_rf_diff(const double *a, const double *b, int64_t n, double scale, O *out);
+__global__ void k_rf_diff(const double *S, int64_t W, int64_t off_p, int64_t off_m, int ncol,
+                          int64_t nrows, double scale, O *out);
 template <typename R>
 __global__ void k_rf_forward(const DevModel *M, const int32_t *ndep, const int32_t *moving, int ncol,
                              int Nmax, const double *state, const int32_t *out_ls,
