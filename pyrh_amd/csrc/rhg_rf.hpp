@@ -307,8 +307,9 @@ k_rf_reverse(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep,
 // NP-tangent dual numbers per depth gives d(point)/dp for every requested
 // parameter (the Humlicek functions by their complex derivative).
 // The FP32 form's three-tangent chain is held to two waves per SIMD (it
-// needs 292 registers at one; two cost 36 spilled to scratch and run the
-// C4 slice of 1024 columns 1.6x faster); the others keep one.
+// needs 292 registers at one; two cost 36 spilled to scratch and run 1024
+// columns of 64-256 depths under 400 Zeeman lines 1.6x faster); the others
+// keep one.
 template <typename R, int NP>
 __global__ void __launch_bounds__(64, (NP <= 3 && sizeof(R) == 4 ? 2 : 1))
 k_rf_chain(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep,
