@@ -436,7 +436,7 @@ static int launch_state(rhg_context_t *c, const double *d_atm, const int32_t *d_
                         const int32_t *perm, int ncol, int Nmax, double mu, int atm_scale,
                         double *state, int nfield) {
   dim3 blk(64), grd((unsigned) rhg::xcd_grid((ncol + 63) / 64, Nmax));
-  const size_t lds = (size_t) 3 * c->dm.nmol * 64 * sizeof(double);
+  const size_t lds = (size_t) 2 * c->dm.nmol * 64 * sizeof(double);
   switch (c->dm.nnuclei) {
 #define RHG_LAUNCH_STATE(NN)                                                                    \
   case NN:                                                                                      \
@@ -977,7 +977,7 @@ int launch_base(rhg_context_t *c, const double *d_atm, const int32_t *d_ndep, in
   const int64_t npts = (int64_t) ncol * Nmax;
   {
     dim3 blk(64), grd((unsigned) rhg::xcd_grid((ncol + 63) / 64, Nmax));
-    const size_t lds = (size_t) 3 * c->dm.nmol * 64 * sizeof(double);
+    const size_t lds = (size_t) 2 * c->dm.nmol * 64 * sizeof(double);
     switch (c->dm.nnuclei) {
 #define RHG_LAUNCH_STATE(NN)                                                                   \
   case NN:                                                                                     \

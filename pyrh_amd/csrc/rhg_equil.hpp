@@ -103,7 +103,9 @@ __device__ __forceinline__ void small_solve(double (&a)[NN][NN], double (&b)[NN]
 // its LU solution once (rh/ludcmp.c:55-84).  Molecule stoichiometry is the
 // dense table M.mol_cnt[m][nucleus]; the per-molecule equilibrium constant,
 // density and Saha product live in LDS (lds_phi, lds_nmol, lds_saha, stride
-// blockDim.x), everything per nucleus in registers.
+// blockDim.x), everything per nucleus in registers.  With lds_saha nullptr
+// the Saha products are formed again where they are reused, by the same
+// operations (so the same values), which saves a third of the LDS.
 template <int NN>
 __device__ void chem_eq(const DevModel &M, double T, double ne, double nHtot,
                         const double (&fn0)[NN], double (&nnuc)[NN], double *lds_phi,
@@ -142,18 +144,24 @@ __device__ void chem_eq(const DevModel &M, double T, double ne, double nHtot,
     double fn[NN];
 #pragma unroll
     for (int i = 0; i < NN; i++) fn[i] = fn0[i] * nnuc[i];
-    for (int m = 0; m < Nm; m++) {
+    // Saha product of molecule m at this iteration's nucleus densities
+    auto saha = [&](int m) {
       CDouble *cnt = ldc(M.mol_cnt) + m * NNUC_MAX;
-      const double nm = lds_nmol[m * ls_stride];
       double sv = lds_phi[m * ls_stride];
 #pragma unroll
       for (int i = 0; i < NN; i++)
-        if (cnt[i] != 0.0) {
-          sv *= ipow(fn[i], (int) cnt[i]);
-          f1[i] += cnt[i] * nm;
-        }
+        if (cnt[i] != 0.0) sv *= ipow(fn[i], (int) cnt[i]);
       sv /= ipow(ne, ldc(M.mol_charge)[m]);
-      lds_saha[m * ls_stride] = sv;
+      return sv;
+    };
+    auto saha_of = [&](int m) { return lds_saha ? lds_saha[m * ls_stride] : saha(m); };
+    for (int m = 0; m < Nm; m++) {
+      CDouble *cnt = ldc(M.mol_cnt) + m * NNUC_MAX;
+      const double nm = lds_nmol[m * ls_stride];
+#pragma unroll
+      for (int i = 0; i < NN; i++)
+        if (cnt[i] != 0.0) f1[i] += cnt[i] * nm;
+      if (lds_saha) lds_saha[m * ls_stride] = saha(m);
     }
     // S = D - B C, rhs = f1 - B f2  (C[m][nu] = -saha_m count_nu / n_nu,
     // f2[m] = n_m - saha_m)
@@ -166,7 +174,7 @@ __device__ void chem_eq(const DevModel &M, double T, double ne, double nHtot,
     S[0][0] += fHmin;
     for (int m = 0; m < Nm; m++) {
       CDouble *cnt = ldc(M.mol_cnt) + m * NNUC_MAX;
-      const double sv = lds_saha[m * ls_stride];
+      const double sv = saha_of(m);
       const double f2 = lds_nmol[m * ls_stride] - sv;
       double C[NN];
 #pragma unroll
@@ -205,7 +213,7 @@ __device__ void chem_eq(const DevModel &M, double T, double ne, double nHtot,
     double dnmax = 0.0;
     for (int m = 0; m < Nm; m++) {
       CDouble *cnt = ldc(M.mol_cnt) + m * NNUC_MAX;
-      const double sv = lds_saha[m * ls_stride];
+      const double sv = saha_of(m);
       const double nold = lds_nmol[m * ls_stride];
       double x2 = nold - sv;
 #pragma unroll

@@ -152,8 +152,10 @@ __device__ __forceinline__ void continuum_point(const DevModel &M, int ls, const
   eta_ai += eta * 1.0;
 }
 
+// Three waves per SIMD: 168 VGPRs (4 spilled) and 2 Nmol doubles of LDS
+// per thread.
 template <int NN>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64, 3)
 k_state(const DevModel *__restrict__ Mp, const double *__restrict__ atmos, const int32_t *__restrict__ ndep,
         const int32_t *__restrict__ perm, int ncol, int Nmax, double mu, int atm_scale,
         double *__restrict__ state, int nfield) {
@@ -226,8 +228,7 @@ k_state(const DevModel *__restrict__ Mp, const double *__restrict__ atmos, const
   const int lstride = blockDim.x;
   double *lds_phi = lds_mol + threadIdx.x;
   double *lds_nmol = lds_phi + M.nmol * lstride;
-  double *lds_saha = lds_nmol + M.nmol * lstride;
-  chem_eq<NN>(M, T, ne, nHtot, fn0, nnuc, lds_phi, lds_nmol, lds_saha, lstride, PhiHmin);
+  chem_eq<NN>(M, T, ne, nHtot, fn0, nnuc, lds_phi, lds_nmol, nullptr, lstride, PhiHmin);
   double fracn[NN];
 #pragma unroll
   for (int i = 0; i < NN; i++) fracn[i] = nnuc[i] / (ldc(M.atom_abund)[ldc(M.nucl_atom)[i]] * nHtot);
