@@ -38,12 +38,24 @@ __device__ __forceinline__ double planck_lam(double T, double lambda) {
 
 // Fritsch & Butland centred derivative with the reference's divisions,
 // rh/bezier_aux.c:34-51
+// a / b as a times the hardware reciprocal estimate refined by two Newton
+// steps (relative error ~1e-16, not correctly rounded): the quotients of
+// the scalar Bezier rays, where the IEEE division sequence is a long
+// dependent chain per use
+__device__ __forceinline__ double ndiv(double a, double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  double e = fma(-b, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-b, r, 1.0);
+  return a * fma(r, e, r);
+}
+
 __device__ __forceinline__ double cent_div(double dsup, double dsdn, double up, double c,
                                            double dn) {
-  const double fim1 = (c - up) / dsup, fi = (dn - c) / dsdn;
+  const double fim1 = ndiv(c - up, dsup), fi = ndiv(dn - c, dsdn);
   if (fim1 * fi > 0) {
-    const double alpha = 0.333333333333333333333333 * (1.0 + dsdn / (dsdn + dsup));
-    return (fim1 * fi) / ((1.0 - alpha) * fim1 + alpha * fi);
+    const double alpha = 0.333333333333333333333333 * (1.0 + ndiv(dsdn, dsdn + dsup));
+    return ndiv(fim1 * fi, (1.0 - alpha) * fim1 + alpha * fi);
   }
   return 0.0;
 }
@@ -500,7 +512,7 @@ k_nlte_bezier(const DevModel *__restrict__ Mp, const DevNlte *__restrict__ Ap, N
             C[r] = chi + cc;
             double s = eta + ee;
             s += ss * jd;
-            S[r] = s / C[r];
+            S[r] = ndiv(s, C[r]);
           }
         }
       };
@@ -529,7 +541,7 @@ k_nlte_bezier(const DevModel *__restrict__ Mp, const DevNlte *__restrict__ Ap, N
           if (r < nrc) {
             double V[MS];
             absorb(k, c0 + r, dir, V);
-            rate_terms<MS>(si, sd, V, I[r], Psi[r] / C[r], wmu[r], sIJ, sJI);
+            rate_terms<MS>(si, sd, V, I[r], ndiv(Psi[r], C[r]), wmu[r], sIJ, sJI);
           }
         }
 #pragma unroll
@@ -564,12 +576,12 @@ k_nlte_bezier(const DevModel *__restrict__ Mp, const DevNlte *__restrict__ Ap, N
 #pragma unroll
         for (int r = 0; r < RB; r++) {
           const double dsup = dzu * zmu[r], dsdn = dzd * zmu[r];
-          const double dchi_up = (C0[r] - Cm[r]) / dsup;
+          const double dchi_up = ndiv(C0[r] - Cm[r], dsup);
           dchi_c[r] = cent_div(dsup, dsdn, Cm[r], C0[r], Cp[r]);
           const double c1 = max0(C0[r] - (dsup * (1.0 / 3.0)) * dchi_c[r]);
           const double c2 = max0(Cm[r] + (dsup * (1.0 / 3.0)) * dchi_up);
           dtau_uw[r] = dsup * (C0[r] + Cm[r] + c1 + c2) * 0.25;
-          dS_up[r] = (S0[r] - Sm[r]) / dtau_uw[r];
+          dS_up[r] = ndiv(S0[r] - Sm[r], dtau_uw[r]);
         }
       }
       for (int k = ks + dk; k != ke + dk; k += dk) {
@@ -591,7 +603,7 @@ k_nlte_bezier(const DevModel *__restrict__ Mp, const DevNlte *__restrict__ Ap, N
           for (int r = 0; r < RB; r++) {
             const double dsdn = dzd * zmu[r];
             const double dchi_dn = far ? cent_div(dsdn, dzd2 * zmu[r], C0[r], Cp[r], Cn[r])
-                                       : (Cp[r] - C0[r]) / dsdn;
+                                       : ndiv(Cp[r] - C0[r], dsdn);
             double c1 = max0(C0[r] + (dsdn * (1.0 / 3.0)) * dchi_c[r]);
             double c2 = max0(Cp[r] - (dsdn * (1.0 / 3.0)) * dchi_dn);
             const double dtau_dw = dsdn * (C0[r] + Cp[r] + c1 + c2) * 0.25;
