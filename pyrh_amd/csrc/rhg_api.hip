@@ -1390,7 +1390,7 @@ struct NlteHost {
   // output-wavelength positions with an active set: those with a line
   // (Bezier rays) first, then the continuum-only ones (Feautrier)
   std::vector<int32_t> ln_tr, ln_phi, io_list;
-  int nbez = 0, nentry = 0, nphi = 0, maxset_bez = 0;
+  int nbez = 0, nentry = 0, nphi = 0, maxset_bez = 0, maxset_fe = 0;
 };
 
 int nlte_check(const rhg_context_t *c, const rhg_nlte_t *a, NlteHost &h) {
@@ -1451,8 +1451,10 @@ int nlte_check(const rhg_context_t *c, const rhg_nlte_t *a, NlteHost &h) {
   // every other output wavelength, active or not: J is iterated there too,
   // as the reference solves every wavelength in each pass
   for (int ls = 0; ls < ns; ls++)
-    if (ls != c->dm.ref_index && !(a->as_ptr[ls + 1] > a->as_ptr[ls] && a->as_bezier[ls]))
+    if (ls != c->dm.ref_index && !(a->as_ptr[ls + 1] > a->as_ptr[ls] && a->as_bezier[ls])) {
       h.io_list.push_back(ls < c->dm.ref_index ? ls : ls - 1);
+      h.maxset_fe = std::max(h.maxset_fe, a->as_ptr[ls + 1] - a->as_ptr[ls]);
+    }
   for (int t = 0; t < a->ntrans; t++)
     for (int ls = a->tr_nblue[t]; ls < a->tr_nblue[t] + a->tr_nlam[t]; ls++) {
       bool found = false;
@@ -1723,9 +1725,17 @@ int rhg_nlte_solve(rhg_context_t *c, const rhg_nlte_t *a, const rhg_columns_t *c
       if (r) return r;
       r = timed(8 + (slot == 2), [&]() -> int {
         if (nfe > 0) {
+          // the member loops unrolled to the largest active set (one launch:
+          // split by set size, the launches would not fill the GPU)
           dim3 grd((unsigned) rhg::xcd_grid(ncb, nfe));
-          hipLaunchKernelGGL(rhg::k_nlte_feautrier, grd, b64, 0, c->stream, c->d_model, d_at, B,
-                             d_act, nact, c->d_out_ls, d_iolist + h.nbez, nfe, h.nbez, eval);
+#define RHG_LAUNCH_FE(MS)                                                                        \
+  hipLaunchKernelGGL((rhg::k_nlte_feautrier<MS>), grd, b64, 0, c->stream, c->d_model, d_at, B,   \
+                     d_act, nact, c->d_out_ls, d_iolist + h.nbez, nfe, h.nbez, eval)
+          if (h.maxset_fe <= 2) RHG_LAUNCH_FE(2);
+          else if (h.maxset_fe <= 4) RHG_LAUNCH_FE(4);
+          else if (h.maxset_fe <= 6) RHG_LAUNCH_FE(6);
+          else RHG_LAUNCH_FE(8);
+#undef RHG_LAUNCH_FE
           HIPCHECK(hipGetLastError());
         }
         hipLaunchKernelGGL(rhg::k_nlte_flip, dim3((unsigned) ((nact + 63) / 64)), b64, 0, c->stream,

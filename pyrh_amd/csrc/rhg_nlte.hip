@@ -666,12 +666,13 @@ RHG_INST_BEZ(8)
 // of the two directions, weighted by the full ray weight).  The opacity,
 // source function and the elimination coefficients F and z live in the
 // thread's scratch rows; the rate terms of each depth are added as the back
-// substitution reaches it.
+// substitution reaches it.  MS is the largest active set among the
+// wavelengths of io_list (the member loops are unrolled to it).
+template <int MS>
 __global__ void __launch_bounds__(64)
 k_nlte_feautrier(const DevModel *__restrict__ Mp, const DevNlte *__restrict__ Ap, NlteBufs B,
                  const int32_t *__restrict__ act, int nact, const int32_t *__restrict__ out_ls,
                  const int32_t *__restrict__ io_list, int nlist, int il_off, int eval_op) {
-  constexpr int MS = NLTE_MAXSET;
   const DevModel &M = *Mp;
   const DevNlte &A = *Ap;
   const int ncb = (nact + 63) / 64;
@@ -826,6 +827,15 @@ k_nlte_feautrier(const DevModel *__restrict__ Mp, const DevNlte *__restrict__ Ap
 #undef STF
 #undef PK
 }
+#define RHG_INST_FE(MS)                                                                          \
+  template __global__ void k_nlte_feautrier<MS>(const DevModel *, const DevNlte *, NlteBufs,     \
+                                                const int32_t *, int, const int32_t *,           \
+                                                const int32_t *, int, int, int);
+RHG_INST_FE(2)
+RHG_INST_FE(4)
+RHG_INST_FE(6)
+RHG_INST_FE(8)
+#undef RHG_INST_FE
 
 // ---------------------------------------------------------------------------
 // The emergent spectrum after convergence, pyrh's final ray

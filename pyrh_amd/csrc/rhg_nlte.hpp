@@ -74,6 +74,7 @@ template <int RB, int MS>
 __global__ void k_nlte_bezier(const DevModel *Mp, const DevNlte *Ap, NlteBufs B,
                               const int32_t *act, int nact, const int32_t *out_ls,
                               const int32_t *io_list, int nlist, int il_off, int eval_op);
+template <int MS>
 __global__ void k_nlte_feautrier(const DevModel *Mp, const DevNlte *Ap, NlteBufs B,
                                  const int32_t *act, int nact, const int32_t *out_ls,
                                  const int32_t *io_list, int nlist, int il_off, int eval_op);
