@@ -1787,6 +1787,8 @@ int rhg_nlte_solve(rhg_context_t *c, const rhg_nlte_t *a, const rhg_columns_t *c
         const dim3 g((unsigned) ((n + 63) / 64));
         if (nl <= 4)
           hipLaunchKernelGGL(rhg::k_nlte_stateq<4>, g, b64, 0, c->stream, d_at, B, d_act, nact, a->isum);
+        else if (nl <= 6)
+          hipLaunchKernelGGL(rhg::k_nlte_stateq<6>, g, b64, 0, c->stream, d_at, B, d_act, nact, a->isum);
         else if (nl <= 8)
           hipLaunchKernelGGL(rhg::k_nlte_stateq<8>, g, b64, 0, c->stream, d_at, B, d_act, nact, a->isum);
         else

@@ -1223,6 +1223,7 @@ k_nlte_stateq(const DevNlte *__restrict__ Ap, NlteBufs B, const int32_t *__restr
 }
 
 template __global__ void k_nlte_stateq<4>(const DevNlte *, NlteBufs, const int32_t *, int, int);
+template __global__ void k_nlte_stateq<6>(const DevNlte *, NlteBufs, const int32_t *, int, int);
 template __global__ void k_nlte_stateq<8>(const DevNlte *, NlteBufs, const int32_t *, int, int);
 template __global__ void k_nlte_stateq<16>(const DevNlte *, NlteBufs, const int32_t *, int, int);
 
