@@ -71,7 +71,8 @@ __device__ __forceinline__ double linear1(int n, const double *xt, const double 
 struct Linear1At {
   int n, j, edge;   // edge -1: x <= xt[0]; 1: x >= xt[n - 1]; 0: inside
   double fx;
-  __device__ __forceinline__ Linear1At(int n_, const double *xt, double x) : n(n_), j(0), edge(0), fx(0.0) {
+  __device__ __forceinline__ Linear1At(int n_, const double *xt, double x)
+      : n(n_), j(0), edge(0), fx(0.0) {
     if (x <= xt[0]) {
       edge = -1;
     } else if (x >= xt[n - 1]) {
