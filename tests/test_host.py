@@ -163,3 +163,53 @@ def test_hse_host_tables():
         api.hse(str(GOLDEN / "run_falc"), 1, np.zeros(5), np.full(5, 5000.0))
     with pytest.raises(ValueError):
         hse.build_hse(s, fw, None)
+
+
+class _FakeTuneEngine:
+    """Stands in for Engine in the autotuner's host logic: a synthesis whose
+    time depends on the launch configuration and whose output is the same
+    for every configuration (or differs for one, to exercise the check)."""
+
+    def __init__(self, torch, out, bad=None):
+        self.cfg = {"line_table": 1, "column_sort": 1, "wave_order": 0}
+        self.torch, self.out, self.bad, self.calls = torch, out, bad, []
+
+    def set_tuning(self, key, value):
+        self.cfg[key] = int(value)
+
+    def tuning(self):
+        return dict(self.cfg)
+
+    def timings(self):
+        return {"formal": 1.0}
+
+    def synth_lte_device(self, *args, **kw):
+        import time
+        self.calls.append(dict(self.cfg))
+        # line table on is fastest, column sort second
+        time.sleep(0.002 * (2 - self.cfg["line_table"]) + 0.001 * (1 - self.cfg["column_sort"]))
+        self.out.fill_(1.0)
+        if self.bad is not None and all(self.cfg[k] == v for k, v in self.bad.items()):
+            self.out[0, 0, 0] = 2.0
+
+
+def test_autotune_picks_fastest_and_checks_bit_identity(monkeypatch):
+    """pyrh_amd.autotune.tune (host logic, no GPU): every combination of the
+    launch knobs is timed, the fastest is left on the engine, and a
+    configuration that changes the Stokes vectors is an error."""
+    torch = pytest.importorskip("torch")
+    from pyrh_amd import autotune
+    monkeypatch.setattr(torch.cuda, "synchronize", lambda *a, **k: None)
+    atm = torch.zeros((4, 9, 8), dtype=torch.float64)
+    ndep = torch.full((4,), 8, dtype=torch.int32)
+    out = torch.zeros((4, 4, 3), dtype=torch.float64)
+    eng = _FakeTuneEngine(torch, out)
+    rep = autotune.tune(eng, atm, ndep, out, reps=2)
+    assert len(rep["candidates"]) == 8
+    assert {(c["line_table"], c["column_sort"], c["wave_order"]) for c in rep["candidates"]} == \
+        {(a, b, c) for a in (0, 1) for b in (0, 1) for c in (0, 1)}
+    assert rep["chosen"]["line_table"] == 1 and rep["chosen"]["column_sort"] == 1
+    assert eng.tuning() == rep["chosen"]
+    bad = _FakeTuneEngine(torch, out, bad={"line_table": 0, "column_sort": 1, "wave_order": 0})
+    with pytest.raises(RuntimeError, match="changed the Stokes vectors"):
+        autotune.tune(bad, atm, ndep, out, reps=1)
