@@ -408,12 +408,13 @@ __device__ __forceinline__ void set_info(const DevNlte &A, int ls, double lam, S
 // Workgroups map to (64 columns of the active list, list wavelength) by
 // xcd_tile, so a block of columns' state and populations stay in one
 // XCD's L2 across the wavelengths.
-// With active sets of at most two members the kernel is held to three
-// waves per SIMD (168 VGPRs, 69 spilled to scratch): the latency of its
-// loads is hidden better, a solve of 4096 columns of the Ca II atom 6%
-// faster than at two waves (229 VGPRs).
+// With active sets of at most two members and RB = 3 rays per sweep the
+// kernel is held to two waves per SIMD (256 VGPRs, 22 spilled): measured
+// on a solve of 4096 columns of the Ca II atom, 10% faster than RB = 2 at
+// three waves (fewer sweeps re-read the background and update the rate
+// sums) and than RB = 3 or 5 at one.
 template <int RB, int MS>
-__global__ void __launch_bounds__(64, (MS <= 2 ? 3 : 1))
+__global__ void __launch_bounds__(64, (MS <= 2 ? 2 : 1))
 k_nlte_bezier(const DevModel *__restrict__ Mp, const DevNlte *__restrict__ Ap, NlteBufs B,
               const int32_t *__restrict__ act, int nact, const int32_t *__restrict__ out_ls,
               const int32_t *__restrict__ io_list, int nlist, int il_off, int eval_op) {

@@ -20,7 +20,7 @@ enum { NS_CHI = 0, NS_S, NS_F, NS_Z, NLTE_NSLOT };
 // rays swept together by a k_nlte_bezier thread (more rays are swept in
 // chunks of this many)
 #ifndef RHG_NLTE_RB
-#define RHG_NLTE_RB 2
+#define RHG_NLTE_RB 3
 #endif
 constexpr int NLTE_RB = RHG_NLTE_RB;
 
