@@ -1567,7 +1567,9 @@ int rhg_nlte_solve(rhg_context_t *c, const rhg_nlte_t *a, const rhg_columns_t *c
       (size_t) Nmax * sizeof(double) *
           ((size_t) nfield + 5 * (size_t) nout + 2 * (size_t) nl + 3 + 2 * (size_t) a->nline +
            2 * (size_t) nl * nl + (size_t) h.nphi * nrays * 2 + 2 * (size_t) h.nentry +
-           (size_t) rhg::NLTE_NSLOT * (stokes_out ? nlist : nlist - h.nbez) + (size_t) NS * nl +
+           std::max((size_t) rhg::NLTE_NSLOT * (nlist - h.nbez),
+                    stokes_out ? (size_t) rhg::NLTE_NSLOT_EM * nlist : (size_t) 0) +
+           (size_t) NS * nl +
            (size_t) nout) +
       (size_t) nlist * sizeof(double) + 9 * (size_t) Nmax * sizeof(double) + 64;
   size_t freeb = 0, totb = 0;
@@ -1612,9 +1614,10 @@ int rhg_nlte_solve(rhg_context_t *c, const rhg_nlte_t *a, const rhg_columns_t *c
         c->nl_Gam.ensure(pts * nl * nl) ||
         c->nl_phi.ensure(pts * std::max(h.nphi, 1) * nrays * 2) || c->nl_csca.ensure(pts * nout) ||
         c->nl_J.ensure(pts * nout * 2) || c->nl_acc.ensure(pts * h.nentry * 2) ||
-        c->nl_scr.ensure((size_t) rhg::NLTE_NSLOT * Nmax *
-                         std::max(stokes_out ? nlist : nlist - h.nbez, 1) * ncb_max * 64 *
-                         sizeof(double)) ||
+        c->nl_scr.ensure(std::max({(size_t) rhg::NLTE_NSLOT * (nlist - h.nbez),
+                                   stokes_out ? (size_t) rhg::NLTE_NSLOT_EM * nlist : (size_t) 0,
+                                   (size_t) 1}) *
+                         Nmax * ncb_max * 64 * sizeof(double)) ||
         (stokes_out && !device_ptrs && c->nl_st.ensure((size_t) ncol * 4 * nout * sizeof(double))) ||
         c->nl_jpar.ensure(ncol * sizeof(int32_t)) ||
         c->nl_dJ.ensure((size_t) nlist * ncol * sizeof(double)) || c->nl_hist.ensure(pts * NS * nl) ||

@@ -669,10 +669,11 @@ RHG_INST_BEZ(8)
 // opacity for all rays, and per ray the Feautrier solution with its
 // two-sided diagonal operator (rh/rhf1d/feautrier.c:56-202; P is the mean
 // of the two directions, weighted by the full ray weight).  The opacity,
-// source function and the elimination coefficients F and z live in the
-// thread's scratch rows; the rate terms of each depth are added as the back
-// substitution reaches it.  MS is the largest active set among the
-// wavelengths of io_list (the member loops are unrolled to it).
+// source function and the elimination coefficients F and z of each ray
+// live in the thread's scratch rows; the back substitution runs the rays of
+// a chunk together and adds the rate terms of each depth as it reaches it.
+// MS is the largest active set among the wavelengths of io_list (the member
+// loops are unrolled to it).
 template <int MS>
 __global__ void __launch_bounds__(64)
 k_nlte_feautrier(const DevModel *__restrict__ Mp, const DevNlte *__restrict__ Ap, NlteBufs B,
@@ -751,73 +752,143 @@ k_nlte_feautrier(const DevModel *__restrict__ Mp, const DevNlte *__restrict__ Ap
 #define CHI(k) SCR(NS_CHI, k)
 #define SRC(k) SCR(NS_S, k)
 #define HGT(k) STF(RHG_CF_HEIGHT, k)
-  for (int mu = 0; mu < nrays; mu++) {
-    const double wmu = A.wmu[mu];
-    const double zmu = 0.5 / A.muz[mu];
-    auto emit = [&](int k, double P, double Psi) {
+  // The rays in chunks of up to RF: each ray's forward elimination into its
+  // own F and z rows, then the back substitution of the chunk's rays
+  // together, so that each depth forms the member data once and updates J
+  // and the rate sums once per chunk.  The rays are summed in ray order, as
+  // one ray at a time would sum them.
+  constexpr int RF = NLTE_FE_RAYS;
+  for (int c0 = 0; c0 < nrays; c0 += RF) {
+    const int nrc = nrays - c0 < RF ? nrays - c0 : RF;
+    double wmu[RF], zmu[RF];
+#pragma unroll
+    for (int r = 0; r < RF; r++) {
+      wmu[r] = r < nrc ? A.wmu[c0 + r] : 0.0;
+      zmu[r] = r < nrc ? 0.5 / A.muz[c0 + r] : 1.0;
+    }
+    auto emit = [&](int k, const double (&P)[RF], const double (&Psi)[RF]) {
       double *jp = Jnew + PK(k);
-      *jp = (mu == 0 ? 0.0 : *jp) + P * wmu;
+      double j = c0 == 0 ? 0.0 : *jp;
+#pragma unroll
+      for (int r = 0; r < RF; r++)
+        if (r < nrc) j = (c0 + r == 0 ? 0.0 : j) + P[r] * wmu[r];
+      *jp = j;
       if (!eval_op) return;
       SetDepth<MS> sd;
-      double V[MS], sIJ[MS], sJI[MS];
+      double V[MS], aIJ[MS], aJI[MS];
       depth(k, sd, V);
 #pragma unroll
-      for (int n = 0; n < MS; n++) sIJ[n] = sJI[n] = 0.0;
-      rate_terms<MS>(si, sd, V, P, Psi / CHI(k), wmu, sIJ, sJI);
+      for (int n = 0; n < MS; n++) {
+        aIJ[n] = aJI[n] = 0.0;
+        if (n < si.n && c0 > 0) {
+          const double *aij = B.acc + ((int64_t) si.ent[n] * 2) * fs + PK(k);
+          aIJ[n] = aij[0];
+          aJI[n] = aij[fs];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < RF; r++) {
+        if (r < nrc) {
+          double sIJ[MS], sJI[MS];
+#pragma unroll
+          for (int n = 0; n < MS; n++) sIJ[n] = sJI[n] = 0.0;
+          rate_terms<MS>(si, sd, V, P[r], Psi[r] / CHI(k), wmu[r], sIJ, sJI);
+#pragma unroll
+          for (int n = 0; n < MS; n++) {
+            aIJ[n] = (c0 + r == 0 ? 0.0 : aIJ[n]) + sIJ[n];
+            aJI[n] = (c0 + r == 0 ? 0.0 : aJI[n]) + sJI[n];
+          }
+        }
+      }
 #pragma unroll
       for (int n = 0; n < MS; n++) {
         if (n < si.n) {
           double *aij = B.acc + ((int64_t) si.ent[n] * 2) * fs + PK(k);
-          double *aji = aij + fs;
-          *aij = (mu == 0 ? 0.0 : *aij) + sIJ[n];
-          *aji = (mu == 0 ? 0.0 : *aji) + sJI[n];
+          aij[0] = aIJ[n];
+          aij[fs] = aJI[n];
         }
       }
     };
-    auto dtau = [&](int k) { return zmu * (CHI(k) + CHI(k + 1)) * (HGT(k) - HGT(k + 1)); };
+    auto dtau = [&](int r, int k) {
+      return zmu[r] * (CHI(k) + CHI(k + 1)) * (HGT(k) - HGT(k + 1));
+    };
     // forward elimination, rh/rhf1d/feautrier.c:56-202 (dark top,
-    // thermalised bottom, STANDARD order)
-    const double dt0 = dtau(0);
-    const double abc0 = 1.0 + 2.0 * 1.0 / dt0;
-    const double C1_0 = 2.0 / (dt0 * dt0);
-    const double S0 = SRC(0) + 2.0 * 0.0 / ((1.0 + 0.0) * dt0);
-    double F = abc0 / C1_0, z = S0 / (abc0 + C1_0);
-    SCR(NS_F, 0) = F;
-    SCR(NS_Z, 0) = z;
-    double dtm1 = dt0;
+    // thermalised bottom, STANDARD order), ray r into rows NS_F + 2 r and
+    // NS_Z + 2 r
+    double F[RF], z[RF], dtm1[RF], abc0[RF], C1_0[RF];
+#pragma unroll
+    for (int r = 0; r < RF; r++) {
+      if (r < nrc) {
+        const double dt0 = dtau(r, 0);
+        abc0[r] = 1.0 + 2.0 * 1.0 / dt0;
+        C1_0[r] = 2.0 / (dt0 * dt0);
+        const double S0 = SRC(0) + 2.0 * 0.0 / ((1.0 + 0.0) * dt0);
+        F[r] = abc0[r] / C1_0[r];
+        z[r] = S0 / (abc0[r] + C1_0[r]);
+        SCR(NS_F + 2 * r, 0) = F[r];
+        SCR(NS_Z + 2 * r, 0) = z[r];
+        dtm1[r] = dt0;
+      }
+    }
     for (int k = 1; k < N - 1; k++) {
-      const double dtk = dtau(k);
-      const double dtau_mid = 0.5 * (dtk + dtm1);
-      const double A1 = 1.0 / (dtau_mid * dtm1), C1 = 1.0 / (dtau_mid * dtk);
-      F = (1.0 + A1 * F / (1.0 + F)) / C1;
-      z = (SRC(k) + A1 * z) / (C1 * (1.0 + F));
-      SCR(NS_F, k) = F;
-      SCR(NS_Z, k) = z;
-      dtm1 = dtk;
+#pragma unroll
+      for (int r = 0; r < RF; r++) {
+        if (r < nrc) {
+          const double dtk = dtau(r, k);
+          const double dtau_mid = 0.5 * (dtk + dtm1[r]);
+          const double A1 = 1.0 / (dtau_mid * dtm1[r]), C1 = 1.0 / (dtau_mid * dtk);
+          F[r] = (1.0 + A1 * F[r] / (1.0 + F[r])) / C1;
+          z[r] = (SRC(k) + A1 * z[r]) / (C1 * (1.0 + F[r]));
+          SCR(NS_F + 2 * r, k) = F[r];
+          SCR(NS_Z + 2 * r, k) = z[r];
+          dtm1[r] = dtk;
+        }
+      }
     }
     // back substitution with the two-sided operator
-    const double dtN = dtm1;   // dtau[N-2]
-    const double hN = B2_1 - (B2_0 - B2_1) / dtN;
-    const double abcN = 1.0 + 2.0 * 1.0 / dtN;
-    const double A1N = 2.0 / (dtN * dtN);
-    const double SN = SRC(N - 1) + 2.0 * hN / ((1.0 + 0.0) * dtN);
-    const double FN2 = SCR(NS_F, N - 2);
-    double P = (SN + A1N * SCR(NS_Z, N - 2)) / (abcN + A1N * (FN2 / (1.0 + FN2)));
-    emit(N - 1, P, 1.0 / (abcN + A1N * FN2 / (1.0 + FN2)));
-    double G = abcN / A1N;   // G[N-1]
-    double dtk = dtN;        // dtau[k] for k = N-2
-    for (int k = N - 2; k >= 1; k--) {
-      const double Fk = SCR(NS_F, k), Fm = SCR(NS_F, k - 1);
-      P = P / (1.0 + Fk) + SCR(NS_Z, k);
-      const double dtkm = dtau(k - 1);
-      const double dtau_mid = 0.5 * (dtk + dtkm);
-      const double A1 = 1.0 / (dtau_mid * dtkm), C1 = 1.0 / (dtau_mid * dtk);
-      emit(k, P, 1.0 / (1.0 + A1 * Fm / (1.0 + Fm) + C1 * G / (1.0 + G)));
-      G = (1.0 + C1 * G / (1.0 + G)) / A1;
-      dtk = dtkm;
+    double P[RF], Psi[RF], G[RF], dtk[RF];
+#pragma unroll
+    for (int r = 0; r < RF; r++) {
+      P[r] = Psi[r] = G[r] = 0.0;
+      dtk[r] = 1.0;
+      if (r < nrc) {
+        const double dtN = dtm1[r];   // dtau[N-2]
+        const double hN = B2_1 - (B2_0 - B2_1) / dtN;
+        const double abcN = 1.0 + 2.0 * 1.0 / dtN;
+        const double A1N = 2.0 / (dtN * dtN);
+        const double SN = SRC(N - 1) + 2.0 * hN / ((1.0 + 0.0) * dtN);
+        const double FN2 = SCR(NS_F + 2 * r, N - 2);
+        P[r] = (SN + A1N * SCR(NS_Z + 2 * r, N - 2)) / (abcN + A1N * (FN2 / (1.0 + FN2)));
+        Psi[r] = 1.0 / (abcN + A1N * FN2 / (1.0 + FN2));
+        G[r] = abcN / A1N;   // G[N-1]
+        dtk[r] = dtN;        // dtau[k] for k = N-2
+      }
     }
-    P = P / (1.0 + SCR(NS_F, 0)) + SCR(NS_Z, 0);
-    emit(0, P, 1.0 / (abc0 + C1_0 * G / (1.0 + G)));
+    emit(N - 1, P, Psi);
+    for (int k = N - 2; k >= 1; k--) {
+#pragma unroll
+      for (int r = 0; r < RF; r++) {
+        if (r < nrc) {
+          const double Fk = SCR(NS_F + 2 * r, k), Fm = SCR(NS_F + 2 * r, k - 1);
+          P[r] = P[r] / (1.0 + Fk) + SCR(NS_Z + 2 * r, k);
+          const double dtkm = dtau(r, k - 1);
+          const double dtau_mid = 0.5 * (dtk[r] + dtkm);
+          const double A1 = 1.0 / (dtau_mid * dtkm), C1 = 1.0 / (dtau_mid * dtk[r]);
+          Psi[r] = 1.0 / (1.0 + A1 * Fm / (1.0 + Fm) + C1 * G[r] / (1.0 + G[r]));
+          G[r] = (1.0 + C1 * G[r] / (1.0 + G[r])) / A1;
+          dtk[r] = dtkm;
+        }
+      }
+      emit(k, P, Psi);
+    }
+#pragma unroll
+    for (int r = 0; r < RF; r++) {
+      if (r < nrc) {
+        P[r] = P[r] / (1.0 + SCR(NS_F + 2 * r, 0)) + SCR(NS_Z + 2 * r, 0);
+        Psi[r] = 1.0 / (abc0[r] + C1_0[r] * G[r] / (1.0 + G[r]));
+      }
+    }
+    emit(0, P, Psi);
   }
   double dJmax = 0.0;
   for (int k = 0; k < N; k++) {

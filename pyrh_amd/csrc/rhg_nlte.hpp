@@ -14,9 +14,15 @@ constexpr double E_RYDBERG = 2.1798741E-18;
 constexpr int NLTE_MAXSET = 8;
 constexpr int NLTE_MAXLEV = 16;
 constexpr int NLTE_MAXNG = 4;                  // Ng order
-// per-thread scratch rows of k_nlte_feautrier: total opacity, source
-// function and the elimination coefficients F and z
-enum { NS_CHI = 0, NS_S, NS_F, NS_Z, NLTE_NSLOT };
+// per-thread scratch rows of k_nlte_feautrier and k_nlte_emergent: total
+// opacity, source function and the elimination coefficients F and z (ray r
+// of a Feautrier chunk in rows NS_F + 2 r and NS_Z + 2 r)
+enum { NS_CHI = 0, NS_S, NS_F, NS_Z };
+// rays of k_nlte_feautrier solved together (more rays in chunks of this
+// many), and the scratch rows per thread of the two kernels
+constexpr int NLTE_FE_RAYS = 5;
+constexpr int NLTE_NSLOT = 2 + 2 * NLTE_FE_RAYS;
+constexpr int NLTE_NSLOT_EM = 4;
 // rays swept together by a k_nlte_bezier thread (more rays are swept in
 // chunks of this many)
 #ifndef RHG_NLTE_RB
