@@ -650,6 +650,64 @@ def _mixed_leg(torch, dev, ncol, do_cpu, nworkers, rf_cols=256, oracle_cols=128)
     return res
 
 
+def _grids_leg(torch, dev, setup, wave, atm, reps=10, ngrid=4):
+    """Uneven wavelength grids in one launch: the LTE map's columns, column i
+    asking for grid i mod 4 of the 112 wavelengths cut into four windows of
+    28, as one rhg_synth_lte_grids call on the merged context, against four
+    rhg_synth_lte calls on contexts built on one window each (the columns
+    of each window already gathered on the device).  FP32 form; both sides
+    bit-identical on every column."""
+    from pyrh_amd import api, model as M
+    from pyrh_amd.engine import Engine
+    ncol, nd = atm.shape[0], atm.shape[2]
+    n = wave.size
+    waves = [wave[g * n // ngrid:(g + 1) * n // ngrid] for g in range(ngrid)]
+    e = Engine(M.build_model(setup, np.unique(np.concatenate(waves))))
+    pos = api.grid_positions(e.model, waves)
+    e.set_grids(pos)
+    singles = [Engine(M.build_model(setup, w)) for w in waves]
+    gid = (np.arange(ncol) % ngrid).astype(np.int32)
+    parts = [np.flatnonzero(gid == g) for g in range(ngrid)]
+    d_atm = torch.from_numpy(atm).to(dev)
+    d_nd = torch.full((ncol,), nd, dtype=torch.int32, device=dev)
+    d_g = torch.from_numpy(gid).to(dev)
+    d_out = torch.empty((ncol, 4, e.nout), dtype=torch.float64, device=dev)
+    d_atm_g = [d_atm[torch.from_numpy(p).to(dev)].contiguous() for p in parts]
+    d_out_g = [torch.empty((p.size, 4, s.nout), dtype=torch.float64, device=dev)
+               for p, s in zip(parts, singles)]
+
+    def one():
+        e.synth_lte_grids_device(d_atm.data_ptr(), d_nd.data_ptr(), d_g.data_ptr(), ncol, nd,
+                                 d_out.data_ptr(), precision=1)
+
+    def split():
+        for g, s in enumerate(singles):
+            s.synth_lte_device(d_atm_g[g].data_ptr(), d_nd.data_ptr(), parts[g].size, nd,
+                               d_out_g[g].data_ptr(), precision=1)
+
+    res = {}
+    for name, f in (("one_launch", one), ("per_grid_calls", split)):
+        f()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            f()
+        torch.cuda.synchronize()
+        res[name] = (time.perf_counter() - t0) / reps
+    out = d_out.cpu().numpy()
+    same = all(np.array_equal(out[p][:, :, pos[g]], d_out_g[g].cpu().numpy())
+               for g, p in enumerate(parts))
+    for s in singles:
+        s.close()
+    e.close()
+    return {"workload": f"lte_stokes_map_{ncol}_{ngrid}_grids",
+            "grids": [int(p.size) for p in pos],
+            "one_launch_ms": round(res["one_launch"] * 1e3, 3),
+            "per_grid_calls_ms": round(res["per_grid_calls"] * 1e3, 3),
+            "value": round(ncol / res["one_launch"], 1), "unit": "columns/s",
+            "bit_identical": bool(same)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -667,6 +725,7 @@ def main():
     ap.add_argument("--mixed-only", action="store_true",
                     help="run only the mixed-size batch leg and print its object")
     ap.add_argument("--no-hse", action="store_true", help="skip the hydrostatic-equilibrium leg")
+    ap.add_argument("--no-grids", action="store_true", help="skip the wavelength-grids leg")
     ap.add_argument("--hse-ncol", type=int, default=16384)
     ap.add_argument("--hse-only", action="store_true",
                     help="run only the hydrostatic-equilibrium leg and print its object")
@@ -756,6 +815,7 @@ def main():
     nlte = None if a.no_nlte else _nlte_leg(torch, dev, a.nlte_ncol, not a.no_cpu, nw)
     mixed = None if a.no_mixed else _mixed_leg(torch, dev, a.mixed_ncol, not a.no_cpu, nw)
     hse = None if a.no_hse else _hse_leg(torch, dev, a.hse_ncol, not a.no_cpu, nw)
+    grids = None if a.no_grids else _grids_leg(torch, dev, setup, wave, atm)
 
     # host-buffer rate (PCIe both ways), reported beside the resident rate
     t1 = time.perf_counter()
@@ -802,6 +862,7 @@ def main():
         "nlte": nlte,
         "mixed": mixed,
         "hse": hse,
+        "grids": grids,
         "autotune": tuning,
         "pipeline_flops": fl["total"],
         "host_buffers_columns_per_s": round(host_rate, 1),

@@ -310,6 +310,33 @@ enum rhg_precision { RHG_PREC_FP64 = 0, RHG_PREC_FP32 = 1 };
 int rhg_synth_lte(rhg_context_t *ctx, const rhg_columns_t *cols,
                   int32_t precision, int32_t device_ptrs, double *stokes_out);
 
+/* Wavelength grids of one context, for batches whose columns ask for
+ * different wavelengths (every pyrh compute1d call takes its own `wave`,
+ * pyrh.pyx:537-541, merged with LAMBDA_REF by SortLambda,
+ * rh/sortlambda.c:42-225).  The context's model is built on the union of
+ * the grids; grid g is the list of its output positions (the model's output
+ * wavelengths, ascending, reference wavelength dropped)
+ * grid_out[grid_ptr[g]] .. grid_out[grid_ptr[g + 1] - 1], strictly
+ * increasing.  1 <= ngrid <= RHG_MAX_GRIDS; host arrays, copied.  Replaces
+ * the previous grids of the context. */
+#define RHG_MAX_GRIDS 32
+int rhg_set_grids(rhg_context_t *ctx, int32_t ngrid, const int32_t *grid_ptr,
+                  const int32_t *grid_out);
+
+/* rhg_synth_lte for a batch whose column i asks for grid col_grid[i] of
+ * rhg_set_grids, all in one launch per stage: the columns run sorted by
+ * grid and depth count (every wavefront holds columns of one grid), each
+ * solved at its grid's wavelengths only.  stokes_out is rhg_synth_lte's
+ * [ncol][4][nspect-1] on the union grid; a column's entries at wavelengths
+ * outside its grid are zero.  Each solved entry equals what rhg_synth_lte
+ * gives on a context built on that grid alone.  col_grid is host or device
+ * memory as the other inputs; grid ids outside [0, ngrid) or depth counts
+ * outside [4, ndep_max] return RHG_EINVAL.  Needs ngrid (ndep_max + 1) + 1
+ * <= 16384.  Synchronous.  rhg_count_work does not cover a grid batch. */
+int rhg_synth_lte_grids(rhg_context_t *ctx, const rhg_columns_t *cols,
+                        int32_t precision, int32_t device_ptrs,
+                        const int32_t *col_grid, double *stokes_out);
+
 /* rhg_synth_lte without any device allocation, for callers that cannot
  * allocate (code under HIP graph capture, tight serving loops, harnesses
  * that forbid hipMalloc).  No counterpart in the reference.  cols->atmos,

@@ -236,6 +236,47 @@ def synth_batch(cwd, mu, atm_scale, atmospheres, wave, ndep=None, precision="fp6
     return st[:, 0], st[:, 1], st[:, 2], st[:, 3], lam
 
 
+def grid_positions(model, waves):
+    """Positions of each wavelength grid's outputs on model's output grid:
+    for grid g, the output wavelengths compute1d(..., waves[g]) returns
+    (SortLambda: sorted, duplicates and LAMBDA_REF dropped,
+    rh/sortlambda.c:42-225), located on the merged model."""
+    out_lam = model.lam[model.out_index]
+    lam_ref = model.lam[model.ref_index] if model.ref_index >= 0 else None
+    pos = []
+    for w in waves:
+        u = np.unique(np.asarray(w, np.float64).ravel())
+        if lam_ref is not None:
+            u = u[u != lam_ref]
+        p = np.searchsorted(out_lam, u)
+        if np.any(p >= out_lam.size) or np.any(out_lam[np.minimum(p, out_lam.size - 1)] != u):
+            raise ValueError("a grid wavelength is not on the merged grid")
+        pos.append(p.astype(np.int64))
+    return pos
+
+
+def synth_grids(cwd, mu, atm_scale, atmospheres, waves, grid_of, ndep=None, precision="fp64",
+                atomic_number=None, atomic_abundance=None, device=0):
+    """LTE synthesis of a batch whose columns ask for different wavelength
+    grids (column i for waves[grid_of[i]]), in one launch per stage.
+    Returns I, Q, U, V [ncol, nlw] on the merged grid lam [nlw] (zero
+    outside a column's grid) and pos, the list of each grid's positions on
+    it: column i's spectrum on its own grid is I[i, pos[grid_of[i]]], equal
+    to synth_batch with wave = waves[grid_of[i]]."""
+    if len(waves) < 1:
+        raise ValueError("waves must hold at least one grid")
+    merged = np.unique(np.concatenate([np.asarray(w, np.float64).ravel() for w in waves]))
+    e = engine_for(cwd, merged, atomic_number, atomic_abundance, device=device)
+    pos = grid_positions(e.model, waves)
+    if getattr(e, "grids", None) is None or len(e.grids) != len(pos) or \
+            any(a.size != b.size or np.any(a != b) for a, b in zip(e.grids, pos)):
+        e.set_grids(pos)
+    prec = {"fp64": 0, "fp32": 1}[precision]
+    st = e.synth_lte_grids(np.asarray(atmospheres, np.float64), grid_of, mu, atm_scale, ndep, prec)
+    lam = e.model.lam[e.model.out_index]
+    return st[:, 0], st[:, 1], st[:, 2], st[:, 3], lam, pos
+
+
 def compute1d(cwd, mu, atm_scale, atmosphere, wave, loggf_ids=None, loggf_values=None,
               lam_ids=None, lam_values=None, fudge_wave=None, fudge_value=None,
               atomic_number=None, atomic_abundance=None, get_atomic_rfs=False,

@@ -122,6 +122,13 @@ struct rhg_context {
   // and, for uneven batches, sorted), the slot -> column permutation, the
   // counting sort's histogram and the out-of-range flag
   DevBuf syn_ndep, syn_perm, syn_hist;
+  // wavelength grids (rhg_set_grids): per grid its polarized and its scalar
+  // output positions (the lists of all grids concatenated, with offsets
+  // [ngrid + 1]) and its range of bound-free tiles [2 ngrid]; the longest
+  // list of each kind; for a grid batch, the block -> grid table
+  // (and a grid batch's host grid ids copied in)
+  int ngrid = 0, gr_max_pol = 0, gr_max_scal = 0;
+  DevBuf gr_pol, gr_pol_ptr, gr_scal, gr_scal_ptr, gr_tile, gr_blk, gr_col;
   double nlte_ms[RHG_NLTE_NTIMING]{};
   // hydrostatic equilibrium: element tables, inputs, outputs, timing
   DevBuf hs_nstage, hs_ptr, hs_logpf, hs_ionpot, hs_abund, hs_in, hs_ndep, hs_out, hs_niter,
@@ -452,14 +459,27 @@ static int launch_state(rhg_context_t *c, const double *d_atm, const int32_t *d_
   return 0;
 }
 
+// The wavelength grids of a synthesis launch: the batch's block -> grid
+// table (ncol / 64 entries, -1: a block without columns) and the context's
+// per-grid wavelength lists (rhg_set_grids).
+struct GridLaunch {
+  const int32_t *blk_grid;
+  const int32_t *pol, *pol_ptr, *scal, *scal_ptr, *tile;
+  int max_pol, max_scal;
+};
+
 // Stages 1-4 of a synthesis (k_state, k_geometry, k_continuum, the formal
 // kernels) on device buffers, with the stage events; no allocation, no copy,
 // no synchronisation.  d_ndep is in slot order; perm (or nullptr) maps slot
-// i to the caller's column perm[i] of d_atm and d_out.
+// i to the caller's column perm[i] of d_atm and d_out (-1 - c: a copy of
+// column c that writes nothing).  gl (or nullptr): the batch's columns
+// are grouped by wavelength grid in whole blocks of 64 slots, each block
+// solving its grid's wavelengths only.
 static int launch_synth(rhg_context_t *c, const double *d_atm, const int32_t *d_ndep,
                         const int32_t *perm, int ncol, int Nmax, double mu, int atm_scale,
                         int32_t precision, int nfield, double *state, int32_t *moving,
-                        double *chiref, void *cchi, void *ceta, double *d_out) {
+                        double *chiref, void *cchi, void *ceta, double *d_out,
+                        const GridLaunch *gl = nullptr) {
   const int64_t npts = (int64_t) ncol * Nmax;
   const int nout = c->nout;
 
@@ -478,34 +498,39 @@ static int launch_synth(rhg_context_t *c, const double *d_atm, const int32_t *d_
   HIPCHECK(hipEventRecord(c->ev[2], c->stream));
   {
     dim3 blk(rhg::CONT_BLOCK), grd((unsigned) ((npts + rhg::CONT_BLOCK - 1) / rhg::CONT_BLOCK));
+    const int32_t *bg = gl ? gl->blk_grid : nullptr, *gt = gl ? gl->tile : nullptr;
     if (precision == RHG_PREC_FP32)
       hipLaunchKernelGGL(rhg::k_continuum<float>, grd, blk, 0, c->stream, c->d_model, d_ndep, ncol, Nmax,
-                         state, nfield, c->d_out_ls, nout, (float *) cchi, (float *) ceta);
+                         state, nfield, c->d_out_ls, nout, (float *) cchi, (float *) ceta, bg, gt);
     else
       hipLaunchKernelGGL(rhg::k_continuum<double>, grd, blk, 0, c->stream, c->d_model, d_ndep, ncol, Nmax,
-                         state, nfield, c->d_out_ls, nout, (double *) cchi, (double *) ceta);
+                         state, nfield, c->d_out_ls, nout, (double *) cchi, (double *) ceta, bg, gt);
     HIPCHECK(hipGetLastError());
   }
   HIPCHECK(hipEventRecord(c->ev[3], c->stream));
   {
     const int ncb = (ncol + 63) / 64;
     dim3 blk(64);
-#define RHG_LAUNCH_FORMAL(K32, K64, LIST, NLIST, LDS, ...)                                       \
+    const int32_t *bg = gl ? gl->blk_grid : nullptr;
+    const int32_t *l_pol = gl ? gl->pol : c->d_io_pol, *p_pol = gl ? gl->pol_ptr : nullptr;
+    const int32_t *l_scal = gl ? gl->scal : c->d_io_scal, *p_scal = gl ? gl->scal_ptr : nullptr;
+    const int n_pol = gl ? gl->max_pol : c->n_pol, n_scal = gl ? gl->max_scal : c->n_scal;
+#define RHG_LAUNCH_FORMAL(K32, K64, LIST, PTR, NLIST, LDS, ...)                                  \
   if (NLIST > 0) {                                                                               \
     dim3 grd((unsigned) rhg::xcd_grid(ncb, NLIST));                                              \
     if (precision == RHG_PREC_FP32)                                                              \
       hipLaunchKernelGGL(K32, grd, blk, LDS, c->stream, c->d_model, d_ndep, moving, ncol, Nmax,  \
                          state, c->d_out_ls, LIST, NLIST, nout, (const float *) cchi,           \
-                         (const float *) ceta, mu, d_out, perm __VA_ARGS__);                     \
+                         (const float *) ceta, mu, d_out, perm, bg, PTR __VA_ARGS__);            \
     else                                                                                         \
       hipLaunchKernelGGL(K64, grd, blk, LDS, c->stream, c->d_model, d_ndep, moving, ncol, Nmax,  \
                          state, c->d_out_ls, LIST, NLIST, nout, (const double *) cchi,          \
-                         (const double *) ceta, mu, d_out, perm __VA_ARGS__);                    \
+                         (const double *) ceta, mu, d_out, perm, bg, PTR __VA_ARGS__);           \
     HIPCHECK(hipGetLastError());                                                                 \
   }
 #define RHG_STOKES(MO, LT)                                                                       \
   RHG_LAUNCH_FORMAL((rhg::k_formal_stokes<float, MO, LT>), (rhg::k_formal_stokes<double, MO, LT>), \
-                    c->d_io_pol, c->n_pol, LT ? (unsigned) c->tab_bytes : 0u, , c->tab_maxl,     \
+                    l_pol, p_pol, n_pol, LT ? (unsigned) c->tab_bytes : 0u, , c->tab_maxl,       \
                     c->tab_maxc)
     const bool lt = c->tab_bytes > 0 && c->tune_line_table;
     if (c->dm.magneto_optical) {
@@ -513,8 +538,8 @@ static int launch_synth(rhg_context_t *c, const double *d_atm, const int32_t *d_
     } else {
       if (lt) { RHG_STOKES(false, true) } else { RHG_STOKES(false, false) }
     }
-    RHG_LAUNCH_FORMAL(rhg::k_formal_scalar<float>, rhg::k_formal_scalar<double>, c->d_io_scal,
-                      c->n_scal, 0u, )
+    RHG_LAUNCH_FORMAL(rhg::k_formal_scalar<float>, rhg::k_formal_scalar<double>, l_scal, p_scal,
+                      n_scal, 0u, )
 #undef RHG_STOKES
 #undef RHG_LAUNCH_FORMAL
   }
@@ -627,6 +652,128 @@ k_sort_place(const int32_t *__restrict__ ndep, int ncol, int Nmax, const int32_t
   sndep[pos] = n;
 }
 
+// The sort of a batch over wavelength grids: key (grid g, depth count n),
+// the grids in order, each grid's columns largest depth count first and each
+// grid's run of columns starting at a multiple of 64 slots, so that every
+// block of 64 slots holds one grid.  Bins b = g * (Nmax + 1) + n; the
+// histogram is k_sort_count's with ngrid * (Nmax + 1) bins per chunk.  A
+// column whose grid is outside [0, ngrid) or whose depth count is out of
+// range counts as an error (and is sorted as grid 0).
+__device__ inline int grid_key(const int32_t *ndep, const int32_t *grid, int i, int Nmax, int ngrid,
+                               bool &bad) {
+  const int n = ndep[i], g = grid[i];
+  const bool gok = g >= 0 && g < ngrid;
+  bad = !gok || n < 4 || n > Nmax;
+  return (gok ? g : 0) * (Nmax + 1) + clamp_ndep(n, Nmax);
+}
+
+__global__ void __launch_bounds__(SORT_CHUNK)
+k_gsort_count(const int32_t *__restrict__ ndep, const int32_t *__restrict__ grid, int ncol, int Nmax,
+              int ngrid, int32_t *__restrict__ hist) {
+  extern __shared__ int32_t h[];   // nb + 1
+  const int g = blockIdx.x, nb = ngrid * (Nmax + 1);
+  for (int b = threadIdx.x; b <= nb; b += blockDim.x) h[b] = 0;
+  __syncthreads();
+  const int i = g * SORT_CHUNK + threadIdx.x;
+  if (i < ncol) {
+    bool bad;
+    atomicAdd(&h[grid_key(ndep, grid, i, Nmax, ngrid, bad)], 1);
+    if (bad) atomicAdd(&h[nb], 1);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) hist[(int64_t) g * nb + b] = h[b];
+  if (threadIdx.x == 0) hist[(int64_t) gridDim.x * nb + g] = h[nb];
+}
+
+// One workgroup: hist[chunk][b] becomes the first sorted slot of the chunk's
+// columns of bin b; seg[3 g .. 3 g + 2] = grid g's first slot, the end of its
+// columns and the end of its run padded to a multiple of 64; flag[0] = the
+// number of columns in error.
+__global__ void __launch_bounds__(1024)
+k_gsort_scan(int32_t *__restrict__ hist, int nchunk, int Nmax, int ngrid, int32_t *__restrict__ seg,
+             int32_t *__restrict__ flag) {
+  extern __shared__ int32_t tot[];   // nb + 1
+  const int nb = ngrid * (Nmax + 1);
+  if (threadIdx.x == 0) tot[nb] = 0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    int s = 0;
+    for (int g = 0; g < nchunk; g++) s += hist[(int64_t) g * nb + b];
+    tot[b] = s;
+  }
+  __syncthreads();
+  int bad = 0;
+  for (int g = threadIdx.x; g < nchunk; g += blockDim.x) bad += hist[(int64_t) nchunk * nb + g];
+  if (bad) atomicAdd(&tot[nb], bad);
+  if (threadIdx.x == 0) {
+    int s = 0;
+    for (int q = 0; q < ngrid; q++) {
+      s = (s + 63) & ~63;
+      seg[3 * q] = s;
+      for (int n = Nmax; n >= 0; n--) {
+        const int b = q * (Nmax + 1) + n, t = tot[b];
+        tot[b] = s;
+        s += t;
+      }
+      seg[3 * q + 1] = s;
+      seg[3 * q + 2] = (s + 63) & ~63;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) flag[0] = tot[nb];
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    int off = tot[b];
+    for (int g = 0; g < nchunk; g++) {
+      const int t = hist[(int64_t) g * nb + b];
+      hist[(int64_t) g * nb + b] = off;
+      off += t;
+    }
+  }
+}
+
+// Column i goes to slot (its chunk's first slot for its bin) + (columns of
+// the same bin before it in the chunk): perm[slot] = i, sndep[slot] = its
+// clamped depth count.
+__global__ void __launch_bounds__(SORT_CHUNK)
+k_gsort_place(const int32_t *__restrict__ ndep, const int32_t *__restrict__ grid, int ncol, int Nmax,
+              int ngrid, const int32_t *__restrict__ off, int32_t *__restrict__ perm,
+              int32_t *__restrict__ sndep) {
+  __shared__ int32_t v[SORT_CHUNK];
+  const int g = blockIdx.x, i = g * SORT_CHUNK + threadIdx.x;
+  bool bad = false;
+  const int key = i < ncol ? grid_key(ndep, grid, i, Nmax, ngrid, bad) : -1;
+  v[threadIdx.x] = key;
+  __syncthreads();
+  if (i >= ncol) return;
+  int r = 0;
+  for (int j = 0; j < (int) threadIdx.x; j++) r += v[j] == key;
+  const int pos = off[(int64_t) g * ngrid * (Nmax + 1) + key] + r;
+  perm[pos] = i;
+  sndep[pos] = clamp_ndep(ndep[i], Nmax);
+}
+
+// Every slot of [0, nslot) not holding a column: a slot that pads grid g's
+// run to whole blocks becomes a copy of the grid's last column (perm entry
+// -1 - column: read, never written), a slot past every run a copy of column 0
+// with 4 depths; blk[slot / 64] = the grid of the block (-1: none).
+__global__ void k_gsort_pad(const int32_t *__restrict__ seg, int ngrid, int nslot,
+                            int32_t *__restrict__ perm, int32_t *__restrict__ sndep,
+                            int32_t *__restrict__ blk) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslot) return;
+  int g = -1;
+  for (int q = 0; q < ngrid; q++)
+    if (s >= seg[3 * q] && s < seg[3 * q + 2]) g = q;
+  if ((s & 63) == 0) blk[s >> 6] = g;
+  if (g < 0) {
+    perm[s] = -1;
+    sndep[s] = 4;
+  } else if (s >= seg[3 * g + 1]) {
+    const int src = seg[3 * g + 1] - 1;
+    perm[s] = -1 - perm[src];
+    sndep[s] = sndep[src];
+  }
+}
+
 }  // namespace rhg
 
 namespace {
@@ -674,15 +821,57 @@ int prepare_ndep(rhg_context_t *c, const int32_t *d_ndep, int ncol, int Nmax, bo
   return 0;
 }
 
+// The wavelength-grid form of prepare_ndep: the caller's columns (grid ids
+// d_grid) sorted by grid and depth count into nslot = 64 (ceil(ncol / 64) +
+// ngrid) slots (every grid's run a whole number of blocks; the block ->
+// grid table in gr_blk); the flag ends as the number of columns whose depth
+// count or grid is out of range.
+int prepare_grids(rhg_context_t *c, const int32_t *d_ndep, const int32_t *d_grid, int ncol,
+                  int Nmax, int nslot, const int32_t **sndep_out, const int32_t **perm_out,
+                  int32_t **flag_out) {
+  const int ngrid = c->ngrid, nb = ngrid * (Nmax + 1);
+  const int nchunk = (ncol + rhg::SORT_CHUNK - 1) / rhg::SORT_CHUNK;
+  const size_t lds = (size_t) (nb + 1) * sizeof(int32_t);
+  if (lds > 65536) return fail(RHG_EINVAL, "too many wavelength grids for this ndep_max");
+  const size_t hist_n = (size_t) nchunk * (nb + 1);
+  if (c->syn_ndep.ensure((size_t) nslot * sizeof(int32_t)) ||
+      c->syn_perm.ensure((size_t) nslot * sizeof(int32_t)) ||
+      c->syn_hist.ensure((hist_n + 3 * (size_t) ngrid + 1) * sizeof(int32_t)) ||
+      c->gr_blk.ensure((size_t) (nslot / 64) * sizeof(int32_t)))
+    return fail(RHG_ENOMEM, "device allocation for the batch failed");
+  int32_t *sndep = (int32_t *) c->syn_ndep.p, *perm = (int32_t *) c->syn_perm.p;
+  int32_t *hist = (int32_t *) c->syn_hist.p, *seg = hist + hist_n, *flag = seg + 3 * ngrid;
+  hipLaunchKernelGGL(rhg::k_gsort_count, dim3((unsigned) nchunk), dim3(rhg::SORT_CHUNK), lds,
+                     c->stream, d_ndep, d_grid, ncol, Nmax, ngrid, hist);
+  HIPCHECK(hipGetLastError());
+  hipLaunchKernelGGL(rhg::k_gsort_scan, dim3(1), dim3(1024), lds, c->stream, hist, nchunk, Nmax,
+                     ngrid, seg, flag);
+  HIPCHECK(hipGetLastError());
+  hipLaunchKernelGGL(rhg::k_gsort_place, dim3((unsigned) nchunk), dim3(rhg::SORT_CHUNK), 0,
+                     c->stream, d_ndep, d_grid, ncol, Nmax, ngrid, hist, perm, sndep);
+  HIPCHECK(hipGetLastError());
+  hipLaunchKernelGGL(rhg::k_gsort_pad, dim3((unsigned) ((nslot + 255) / 256)), dim3(256), 0,
+                     c->stream, seg, ngrid, nslot, perm, sndep, (int32_t *) c->gr_blk.p);
+  HIPCHECK(hipGetLastError());
+  *sndep_out = sndep;
+  *perm_out = perm;
+  *flag_out = flag;
+  return 0;
+}
+
 // The synthesis of a batch.  Host buffers (device_ptrs 0) are checked on
 // the host and copied in; device buffers are checked on the device (no copy
 // to the host before the launches).  sort: run a batch of uneven depth
 // counts in depth-count order (the kernels read the caller's atmosphere and
 // write its Stokes vectors through the permutation); the diagnostic outputs
 // and the callers that reuse the context's column state take the batch in
-// the caller's order.
+// the caller's order.  col_grid (or nullptr): the wavelength grid of every
+// column (rhg_set_grids), host or device memory as the other inputs; the
+// batch then runs grouped by grid, each column solved at its grid's
+// wavelengths only, the other entries of its Stokes output set to zero.
 int synth(rhg_context_t *c, const rhg_columns_t *cols, int32_t precision, int32_t device_ptrs,
-          bool sort, double *stokes_out, double *height_out, double *chiref_out) {
+          bool sort, double *stokes_out, double *height_out, double *chiref_out,
+          const int32_t *col_grid = nullptr) {
   if (!c || !cols || !stokes_out || !cols->atmos || !cols->ndep) return fail(RHG_EINVAL, "null argument");
   if (c->nout < 1) return fail(RHG_EINVAL, "the model has no output wavelength");
   if (precision != RHG_PREC_FP64 && precision != RHG_PREC_FP32)
@@ -696,19 +885,31 @@ int synth(rhg_context_t *c, const rhg_columns_t *cols, int32_t precision, int32_
   if (!device_ptrs)
     for (int i = 0; i < ncol; i++)
       if (cols->ndep[i] < 4 || cols->ndep[i] > Nmax) return fail(RHG_EINVAL, "ndep out of range");
+  if (col_grid) {
+    if (c->ngrid < 1) return fail(RHG_EINVAL, "no wavelength grids are set (rhg_set_grids)");
+    if (height_out || chiref_out) return fail(RHG_EINVAL, "grid batches have no diagnostic outputs");
+    if (!device_ptrs)
+      for (int i = 0; i < ncol; i++)
+        if (col_grid[i] < 0 || col_grid[i] >= c->ngrid) return fail(RHG_EINVAL, "grid out of range");
+  }
   HIPCHECK(hipSetDevice(c->device));
   if (height_out || chiref_out) sort = false;
+  // slots of the kernels: the columns, or with grids every grid's columns
+  // padded to whole blocks of 64
+  const int nslot = col_grid ? 64 * ((ncol + 63) / 64 + c->ngrid) : ncol;
+  if ((int64_t) nslot * Nmax > (int64_t) INT32_MAX) return fail(RHG_EINVAL, "batch too large");
 
   const int nfield = rhg::state_nfield(c->ncont, c->nline);
   const size_t rsz = precision == RHG_PREC_FP32 ? sizeof(float) : sizeof(double);
   const size_t atm_b = (size_t) ncol * 9 * Nmax * sizeof(double);
-  const size_t st_b = (size_t) ncol * nfield * Nmax * sizeof(double);
-  const size_t cc_b = (size_t) ncol * Nmax * nout * rsz;
+  const size_t st_b = (size_t) nslot * nfield * Nmax * sizeof(double);
+  const size_t cc_b = (size_t) nslot * Nmax * nout * rsz;
   const size_t out_b = (size_t) ncol * 4 * nout * sizeof(double);
-  if (c->state.ensure(st_b) || c->moving.ensure(ncol * sizeof(int32_t)) ||
+  if (c->state.ensure(st_b) || c->moving.ensure(nslot * sizeof(int32_t)) ||
       c->cchi.ensure(cc_b) || c->ceta.ensure(cc_b) ||
       (!device_ptrs && (c->atmos.ensure(atm_b) || c->ndep.ensure(ncol * sizeof(int32_t)) ||
                         c->stokes.ensure(out_b))) ||
+      (col_grid && !device_ptrs && c->gr_col.ensure(ncol * sizeof(int32_t))) ||
       (chiref_out && c->chiref.ensure((size_t) ncol * Nmax * sizeof(double))))
     return fail(RHG_ENOMEM, "device allocation for the batch failed");
   const double *d_atm = (const double *) cols->atmos;
@@ -723,12 +924,29 @@ int synth(rhg_context_t *c, const rhg_columns_t *cols, int32_t precision, int32_
   }
   const int32_t *sndep = nullptr, *perm = nullptr;
   int32_t *flag = nullptr;
-  if (int rc = prepare_ndep(c, d_ndep, ncol, Nmax, sort, &sndep, &perm, &flag)) return rc;
+  GridLaunch gl{};
+  if (col_grid) {
+    const int32_t *d_grid = col_grid;
+    if (!device_ptrs) {
+      HIPCHECK(hipMemcpyAsync(c->gr_col.p, col_grid, ncol * sizeof(int32_t), hipMemcpyHostToDevice,
+                              c->stream));
+      d_grid = (const int32_t *) c->gr_col.p;
+    }
+    if (int rc = prepare_grids(c, d_ndep, d_grid, ncol, Nmax, nslot, &sndep, &perm, &flag)) return rc;
+    gl = GridLaunch{(const int32_t *) c->gr_blk.p, (const int32_t *) c->gr_pol.p,
+                    (const int32_t *) c->gr_pol_ptr.p, (const int32_t *) c->gr_scal.p,
+                    (const int32_t *) c->gr_scal_ptr.p, (const int32_t *) c->gr_tile.p,
+                    c->gr_max_pol, c->gr_max_scal};
+    HIPCHECK(hipMemsetAsync(d_out, 0, out_b, c->stream));
+  } else if (int rc = prepare_ndep(c, d_ndep, ncol, Nmax, sort, &sndep, &perm, &flag)) {
+    return rc;
+  }
   double *state = (double *) c->state.p;
   int32_t *moving = (int32_t *) c->moving.p;
   double *chiref = chiref_out ? (double *) c->chiref.p : nullptr;
-  if (int rc = launch_synth(c, d_atm, sndep, perm, ncol, Nmax, cols->mu, cols->atm_scale, precision,
-                            nfield, state, moving, chiref, c->cchi.p, c->ceta.p, d_out))
+  if (int rc = launch_synth(c, d_atm, sndep, perm, nslot, Nmax, cols->mu, cols->atm_scale,
+                            precision, nfield, state, moving, chiref, c->cchi.p, c->ceta.p, d_out,
+                            col_grid ? &gl : nullptr))
     return rc;
   if (!device_ptrs)
     HIPCHECK(hipMemcpyAsync(stokes_out, d_out, out_b, hipMemcpyDeviceToHost, c->stream));
@@ -744,13 +962,15 @@ int synth(rhg_context_t *c, const rhg_columns_t *cols, int32_t precision, int32_
   int32_t nbad = 0;
   HIPCHECK(hipMemcpyAsync(&nbad, flag, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
-  if (nbad) return fail(RHG_EINVAL, "ndep out of range");
+  if (nbad) return fail(RHG_EINVAL, col_grid ? "ndep or grid out of range" : "ndep out of range");
   if (height_out)
     for (int ic = 0; ic < ncol; ic++)
       for (int k = 0; k < Nmax; k++)
         height_out[(size_t) ic * Nmax + k] =
             k < cols->ndep[ic] ? hfield[(size_t) k * ncol + ic] : 0.0;
-  c->last_ncol = ncol;
+  // the work census reads the last batch's slots in the caller's order
+  // (or sorted by depth count); a grid batch's slots are not that
+  c->last_ncol = col_grid ? 0 : ncol;
   c->last_Nmax = Nmax;
   c->last_nfield = nfield;
   c->last_mu = cols->mu;
@@ -858,6 +1078,61 @@ int rhg_synth_lte(rhg_context_t *ctx, const rhg_columns_t *cols, int32_t precisi
 int rhg_synth_lte_diag(rhg_context_t *ctx, const rhg_columns_t *cols, int32_t precision,
                        double *stokes_out, double *height_out, double *chiref_out) {
   return synth(ctx, cols, precision, 0, false, stokes_out, height_out, chiref_out);
+}
+
+int rhg_set_grids(rhg_context_t *c, int32_t ngrid, const int32_t *grid_ptr,
+                  const int32_t *grid_out) {
+  if (!c || !grid_ptr || (!grid_out && ngrid > 0)) return fail(RHG_EINVAL, "null argument");
+  if (ngrid < 1 || ngrid > RHG_MAX_GRIDS) return fail(RHG_EINVAL, "ngrid out of range");
+  if (grid_ptr[0] != 0) return fail(RHG_EINVAL, "grid_ptr[0] must be 0");
+  const int nout = c->nout;
+  std::vector<char> is_pol(nout, 0);
+  for (int io : c->h_pol) is_pol[io] = 1;
+  std::vector<int32_t> pol, scal, pol_ptr{0}, scal_ptr{0}, tile;
+  int max_pol = 0, max_scal = 0;
+  for (int g = 0; g < ngrid; g++) {
+    if (grid_ptr[g + 1] < grid_ptr[g]) return fail(RHG_EINVAL, "grid_ptr must not decrease");
+    int lo = nout, hi = -1;
+    for (int e = grid_ptr[g]; e < grid_ptr[g + 1]; e++) {
+      const int io = grid_out[e];
+      if (io < 0 || io >= nout) return fail(RHG_EINVAL, "grid wavelength position out of range");
+      if (e > grid_ptr[g] && io <= grid_out[e - 1])
+        return fail(RHG_EINVAL, "a grid's wavelength positions must increase");
+      (is_pol[io] ? pol : scal).push_back(io);
+      lo = std::min(lo, io);
+      hi = std::max(hi, io);
+    }
+    max_pol = std::max(max_pol, (int) pol.size() - pol_ptr.back());
+    max_scal = std::max(max_scal, (int) scal.size() - scal_ptr.back());
+    pol_ptr.push_back((int32_t) pol.size());
+    scal_ptr.push_back((int32_t) scal.size());
+    tile.push_back(hi < 0 ? 0 : lo / rhg::BF_TILE);
+    tile.push_back(hi < 0 ? 0 : hi / rhg::BF_TILE + 1);
+  }
+  HIPCHECK(hipSetDevice(c->device));
+  auto put = [&](DevBuf &b, const std::vector<int32_t> &v) -> int {
+    const size_t n = std::max<size_t>(v.size(), 1) * sizeof(int32_t);
+    if (b.ensure(n)) return fail(RHG_ENOMEM, "device allocation for the grids failed");
+    if (!v.empty()) HIPCHECK(hipMemcpy(b.p, v.data(), v.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    return 0;
+  };
+  c->ngrid = 0;
+  if (int rc = put(c->gr_pol, pol)) return rc;
+  if (int rc = put(c->gr_pol_ptr, pol_ptr)) return rc;
+  if (int rc = put(c->gr_scal, scal)) return rc;
+  if (int rc = put(c->gr_scal_ptr, scal_ptr)) return rc;
+  if (int rc = put(c->gr_tile, tile)) return rc;
+  c->ngrid = ngrid;
+  c->gr_max_pol = max_pol;
+  c->gr_max_scal = max_scal;
+  g_err.clear();
+  return 0;
+}
+
+int rhg_synth_lte_grids(rhg_context_t *ctx, const rhg_columns_t *cols, int32_t precision,
+                        int32_t device_ptrs, const int32_t *col_grid, double *stokes_out) {
+  if (!col_grid) return fail(RHG_EINVAL, "null argument");
+  return synth(ctx, cols, precision, device_ptrs, true, stokes_out, nullptr, nullptr, col_grid);
 }
 
 }  // extern "C"
@@ -1000,7 +1275,7 @@ int launch_base(rhg_context_t *c, const double *d_atm, const int32_t *d_ndep, in
   if (cchi) {
     dim3 blk(rhg::CONT_BLOCK), grd((unsigned) ((npts + rhg::CONT_BLOCK - 1) / rhg::CONT_BLOCK));
     hipLaunchKernelGGL(rhg::k_continuum<R>, grd, blk, 0, c->stream, c->d_model, d_ndep, ncol, Nmax,
-                       state, nfield, c->d_out_ls, c->nout, cchi, ceta);
+                       state, nfield, c->d_out_ls, c->nout, cchi, ceta, (const int32_t *) nullptr, (const int32_t *) nullptr);
     HIPCHECK(hipGetLastError());
   }
   return 0;
@@ -1138,7 +1413,7 @@ int rf_chunk(rhg_context_t *c, const double *d_atm, const int32_t *d_ndep, int n
     dim3 grd((unsigned) rhg::xcd_grid(ncb, c->n_scal));
     hipLaunchKernelGGL(rhg::k_formal_scalar<R>, grd, blk, 0, c->stream, c->d_model, d_ndep, moving,
                        ncol, Nmax, state, c->d_out_ls, c->d_io_scal, c->n_scal, nout,
-                       (const R *) cchi, (const R *) ceta, mu, d_out, nullptr);
+                       (const R *) cchi, (const R *) ceta, mu, d_out, (const int32_t *) nullptr, (const int32_t *) nullptr, (const int32_t *) nullptr);
     HIPCHECK(hipGetLastError());
   }
   HIPCHECK(hipEventRecord(c->ev[5], c->stream));
@@ -1680,7 +1955,7 @@ int rhg_nlte_solve(rhg_context_t *c, const rhg_nlte_t *a, const rhg_columns_t *c
       hipLaunchKernelGGL(rhg::k_continuum<double>,
                          dim3((unsigned) ((npts + rhg::CONT_BLOCK - 1) / rhg::CONT_BLOCK)),
                          dim3(rhg::CONT_BLOCK), 0, c->stream, c->d_model, d_ndep, ncol, Nmax,
-                         B.state, nfield, c->d_out_ls, nout, B.cchi, B.ceta);
+                         B.state, nfield, c->d_out_ls, nout, B.cchi, B.ceta, (const int32_t *) nullptr, (const int32_t *) nullptr);
       HIPCHECK(hipGetLastError());
       hipLaunchKernelGGL(rhg::k_nlte_bg, gpts, b64, 0, c->stream, c->d_model, d_at, B,
                          c->d_out_ls);

@@ -32,6 +32,7 @@ k_rf_loggf(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep,
            int npar, const int32_t *__restrict__ par_line, double *__restrict__ Iin,
            double *__restrict__ rf) {
   using R = double;
+  RAY_TILE();
   RAY_SETUP_BASE();
   if (column_solver(ldc(M.solver)[ls], moving) != RHG_SOLVER_BEZIER) return;
   const double zmu = 1.0 / mu_d;
@@ -211,16 +212,20 @@ k_rf_loggf(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep,
 #undef DZ
 #undef RAY_SETUP
 #undef RAY_SETUP_BASE
+#undef RAY_TILE
+#undef RAY_TILE_GRID
 
 #define RHG_KFORMAL(KNAME, R)                                                                 \
   template __global__ void KNAME<R>(const DevModel *, const int32_t *, const int32_t *, int, int, \
                                     const double *, const int32_t *, const int32_t *, int, int,  \
-                                    const R *, const R *, double, double *, const int32_t *);
+                                    const R *, const R *, double, double *, const int32_t *, \
+                                    const int32_t *, const int32_t *);
 #define RHG_KSTOKES(R, MO, LT)                                                                \
   template __global__ void k_formal_stokes<R, MO, LT>(const DevModel *, const int32_t *,            \
                                                       const int32_t *, int, int, const double *,    \
                                                       const int32_t *, const int32_t *, int, int,   \
                                                       const R *, const R *, double, double *,       \
+                                                      const int32_t *, const int32_t *,             \
                                                       const int32_t *, int, int);
 RHG_KSTOKES(double, false, false)
 RHG_KSTOKES(double, true, false)

@@ -45,15 +45,40 @@ __device__ __forceinline__ void form_point(const DevModel &M, int ls, int has_li
 }
 
 
-// Common set-up of one ray: lane -> column, workgroup -> (column block,
-// entry t of the wavelength list io_list).
-#define RAY_SETUP_BASE()                                                                 \
-  const DevModel &M = *Mp;                                                               \
+// Workgroup -> (column block cb, output wavelength) of the formal kernels.
+// Without wavelength grids (blk_grid == nullptr): entry t < nlist of
+// io_list.  With grids, the columns of block cb all belong to grid g =
+// blk_grid[cb] (-1: a block without columns), whose wavelengths are entries
+// gptr[g] .. gptr[g + 1] - 1 of io_list, and nlist is the longest grid's
+// count.  Returns -1 for a workgroup with nothing to solve.
+__device__ __forceinline__ int ray_io(const int32_t *io_list, int nlist, const int32_t *blk_grid,
+                                      const int32_t *gptr, int ncol, int &cb) {
+  int t;
+  xcd_tile(blockIdx.x, nlist, cb, t);
+  if (!blk_grid) return io_list[t];
+  if (cb * 64 >= ncol) return -1;
+  const int g = ldc(blk_grid)[cb];
+  if (g < 0) return -1;
+  const int e = ldc(gptr)[g] + t;
+  return e < ldc(gptr)[g + 1] ? ldc(io_list)[e] : -1;
+}
+
+// workgroup -> (column block, entry t of the wavelength list io_list)
+#define RAY_TILE()                                                                       \
   int cb, t;                                                                             \
   xcd_tile(blockIdx.x, nlist, cb, t);                                                    \
+  const int io = io_list[t]
+// the same through ray_io (kernels that take wavelength grids)
+#define RAY_TILE_GRID()                                                                  \
+  int cb;                                                                                \
+  const int io = ray_io(io_list, nlist, blk_grid, gptr, ncol, cb);                       \
+  if (io < 0) return
+
+// Common set-up of one ray (after RAY_TILE or RAY_TILE_GRID): lane -> column.
+#define RAY_SETUP_BASE()                                                                 \
+  const DevModel &M = *Mp;                                                               \
   const int ic = cb * 64 + threadIdx.x;                                                  \
   if (ic >= ncol) return;                                                                \
-  const int io = io_list[t];                                                             \
   const int N = ndep[ic];                                                                \
   const int ls = out_ls[io];                                                             \
   const int moving = moving_arr[ic];                                                     \
@@ -72,10 +97,14 @@ __device__ __forceinline__ void form_point(const DevModel &M, int ls, int has_li
   B2[1] = planck(Tc[(int64_t) (N - 1) * ncol], lc[RHG_LC_HC_KLA], lc[RHG_LC_TWOHNU3_C2]);
 
 // the same, with the column's Stokes output for wavelength io (slot ic holds
-// the caller's column perm[ic], or ic without a permutation)
+// the caller's column perm[ic], or ic without a permutation; a negative
+// entry marks a slot that only pads a grid's columns to whole blocks, which
+// writes nothing)
 #define RAY_SETUP()                                                                      \
   RAY_SETUP_BASE();                                                                      \
-  double *out = stokes + (int64_t) (perm ? perm[ic] : ic) * 4 * nout + io;
+  const int oc = perm ? perm[ic] : ic;                                                   \
+  if (oc < 0) return;                                                                    \
+  double *out = stokes + (int64_t) oc * 4 * nout + io;
 
 #define Z(kk) zc[(int64_t) (kk) * ncol]
 // |z[kk-1] - z[kk]| (kk >= 1), from the column state
@@ -102,13 +131,13 @@ k_formal_stokes(const DevModel *__restrict__ Mp, const int32_t *__restrict__ nde
                 const double *__restrict__ state, const int32_t *__restrict__ out_ls,
                 const int32_t *__restrict__ io_list, int nlist, int nout,
                 const R *__restrict__ cchi, const R *__restrict__ ceta, double mu_d,
-                double *__restrict__ stokes, const int32_t *__restrict__ perm, int tab_maxl,
-                int tab_maxc) {
+                double *__restrict__ stokes, const int32_t *__restrict__ perm,
+                const int32_t *__restrict__ blk_grid, const int32_t *__restrict__ gptr,
+                int tab_maxl, int tab_maxc) {
+  RAY_TILE_GRID();
   // the line table first: every lane of the wavefront stages it
   auto tab = [&]() {
-    int cb0, t0;
-    xcd_tile(blockIdx.x, nlist, cb0, t0);
-    const int ls0 = out_ls[io_list[t0]];
+    const int ls0 = out_ls[io];
     if constexpr (LT) {
       extern __shared__ __align__(8) char lds_tab[];
       return stage_lines(*Mp, ls0, lds_tab, tab_maxl, tab_maxc);
@@ -158,7 +187,9 @@ k_formal_scalar(const DevModel *__restrict__ Mp, const int32_t *__restrict__ nde
                 const double *__restrict__ state, const int32_t *__restrict__ out_ls,
                 const int32_t *__restrict__ io_list, int nlist, int nout,
                 const R *__restrict__ cchi, const R *__restrict__ ceta, double mu_d,
-                double *__restrict__ stokes, const int32_t *__restrict__ perm) {
+                double *__restrict__ stokes, const int32_t *__restrict__ perm,
+                const int32_t *__restrict__ blk_grid, const int32_t *__restrict__ gptr) {
+  RAY_TILE_GRID();
   RAY_SETUP();
   const int solver = column_solver(ldc(M.solver)[ls], moving);
   Point<R> Pn;

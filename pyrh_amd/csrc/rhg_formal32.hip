@@ -9,12 +9,14 @@ namespace rhg {
 #define RHG_KFORMAL(KNAME, R)                                                                 \
   template __global__ void KNAME<R>(const DevModel *, const int32_t *, const int32_t *, int, int, \
                                     const double *, const int32_t *, const int32_t *, int, int,  \
-                                    const R *, const R *, double, double *, const int32_t *);
+                                    const R *, const R *, double, double *, const int32_t *, \
+                                    const int32_t *, const int32_t *);
 #define RHG_KSTOKES(R, MO, LT)                                                                \
   template __global__ void k_formal_stokes<R, MO, LT>(const DevModel *, const int32_t *,            \
                                                       const int32_t *, int, int, const double *,    \
                                                       const int32_t *, const int32_t *, int, int,   \
                                                       const R *, const R *, double, double *,       \
+                                                      const int32_t *, const int32_t *,             \
                                                       const int32_t *, int, int);
 RHG_KSTOKES(float, false, false)
 RHG_KSTOKES(float, true, false)

@@ -168,8 +168,12 @@ k_state(const DevModel *__restrict__ Mp, const double *__restrict__ atmos, const
   const int ic = cb * 64 + (int) threadIdx.x;
   if (ic >= ncol) return;
   if (k >= ndep[ic]) return;
-  // slot ic of the batch holds the caller's column perm[ic] (identity without perm)
-  const double *atm = atmos + (int64_t) (perm ? perm[ic] : ic) * 9 * Nmax;
+  // slot ic of the batch holds the caller's column perm[ic] (identity without
+  // perm); a slot that pads a grid's columns to whole blocks holds -1 - c, a
+  // copy of column c whose results are not written
+  int src = perm ? perm[ic] : ic;
+  if (src < 0) src = -1 - src;
+  const double *atm = atmos + (int64_t) src * 9 * Nmax;
   double *st = state + (int64_t) k * ncol + ic;
   const int64_t fs = (int64_t) Nmax * ncol;
 
@@ -511,18 +515,33 @@ __global__ void k_scales(const DevModel *__restrict__ Mp, const int32_t *__restr
 // wave-uniform) and handed to the per-wavelength pass through LDS.  One
 // exponential per wavelength: exp(-hc/(k lambda T)) is the reciprocal of the
 // Planck function's exp(hc/(k lambda T)).
-// Output layout: cchi[(io * Nmax + k) * ncol + column].
+// Output layout: cchi[(io * Nmax + k) * ncol + column].  With wavelength
+// grids (blk_grid != nullptr, ncol a multiple of 64) the columns of block
+// ic / 64 belong to grid g = blk_grid[ic / 64] (-1: none) and take the
+// bound-free tiles gtile[2 g] .. gtile[2 g + 1] - 1 only, the tiles that hold
+// the grid's wavelengths.
 template <typename R>
 __global__ void __launch_bounds__(CONT_BLOCK)
 k_continuum(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep, int ncol, int Nmax,
             const double *__restrict__ state, int nfield, const int32_t *__restrict__ out_ls,
-            int nout, R *__restrict__ cchi, R *__restrict__ ceta) {
+            int nout, R *__restrict__ cchi, R *__restrict__ ceta,
+            const int32_t *__restrict__ blk_grid, const int32_t *__restrict__ gtile) {
   const DevModel &M = *Mp;
   __shared__ double bf_lds[BF_TILE][CONT_BLOCK];
   const int64_t gid = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= (int64_t) ncol * Nmax) return;
   const int k = (int) (gid / ncol), ic = (int) (gid % ncol);
   if (k >= ndep[ic]) return;
+  int tile0 = 0, tile1 = M.ntile;
+  if (blk_grid) {
+    // a wavefront is 64 consecutive slots of one depth (ncol is a multiple
+    // of 64), one block: its grid and tile range are wave-uniform, and kept
+    // in scalar registers so that the tile tables stay scalar loads
+    const int g = __builtin_amdgcn_readfirstlane(blk_grid[ic >> 6]);
+    if (g < 0) return;
+    tile0 = __builtin_amdgcn_readfirstlane(gtile[2 * g]);
+    tile1 = __builtin_amdgcn_readfirstlane(gtile[2 * g + 1]);
+  }
   const int64_t fs = (int64_t) Nmax * ncol;
   const double *st = state + (int64_t) k * ncol + ic;
   const double T = st[RHG_CF_T * fs], ne = st[RHG_CF_NE * fs];
@@ -552,7 +571,7 @@ k_continuum(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep, i
     ti_ch = (i2 == NTCH - 1) ? (double) i2 : (double) i2 + (T - ldc(M.ch_T)[i2]) / (ldc(M.ch_T)[i2 + 1] - ldc(M.ch_T)[i2]);
   }
 
-  for (int t = 0; t < M.ntile; t++) {
+  for (int t = tile0; t < tile1; t++) {
     const int io0 = t * BF_TILE;
     {
       double bf[BF_TILE];
@@ -663,9 +682,11 @@ RHG_KSTATE(8)
 #undef RHG_KSTATE
 
 template __global__ void k_continuum<double>(const DevModel *, const int32_t *, int, int, const double *,
-                                             int, const int32_t *, int, double *, double *);
+                                             int, const int32_t *, int, double *, double *,
+                                             const int32_t *, const int32_t *);
 template __global__ void k_continuum<float>(const DevModel *, const int32_t *, int, int, const double *,
-                                            int, const int32_t *, int, float *, float *);
+                                            int, const int32_t *, int, float *, float *,
+                                            const int32_t *, const int32_t *);
 
 // ---------------------------------------------------------------------------
 // Column reordering.  perm[i] is the caller's column at sorted position i.

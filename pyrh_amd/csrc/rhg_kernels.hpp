@@ -476,18 +476,20 @@ __global__ void k_scales(const DevModel *Mp, const int32_t *ndep, int ncol, int 
 template <typename R>
 __global__ void k_continuum(const DevModel *M, const int32_t *ndep, int ncol, int Nmax,
                             const double *state, int nfield, const int32_t *out_ls, int nout,
-                            R *cchi, R *ceta);
+                            R *cchi, R *ceta, const int32_t *blk_grid, const int32_t *gtile);
 template <typename R, bool MO, bool LT>
 __global__ void k_formal_stokes(const DevModel *M, const int32_t *ndep, const int32_t *moving,
                                 int ncol, int Nmax, const double *state, const int32_t *out_ls,
                                 const int32_t *io_list, int nlist, int nout, const R *cchi,
                                 const R *ceta, double mu, double *stokes, const int32_t *perm,
-                                int tab_maxl, int tab_maxc);
+                                const int32_t *blk_grid, const int32_t *gptr, int tab_maxl,
+                                int tab_maxc);
 template <typename R>
 __global__ void k_formal_scalar(const DevModel *M, const int32_t *ndep, const int32_t *moving,
                                 int ncol, int Nmax, const double *state, const int32_t *out_ls,
                                 const int32_t *io_list, int nlist, int nout, const R *cchi,
-                                const R *ceta, double mu, double *stokes, const int32_t *perm);
+                                const R *ceta, double mu, double *stokes, const int32_t *perm,
+                                const int32_t *blk_grid, const int32_t *gptr);
 __global__ void k_rf_loggf(const DevModel *M, const int32_t *ndep, const int32_t *moving, int ncol,
                            int Nmax, const double *state, const int32_t *out_ls,
                            const int32_t *io_list, int nlist, int nout, const double *cchi,

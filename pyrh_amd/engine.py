@@ -71,6 +71,10 @@ def lib():
         PD = C.POINTER(C.c_double)
         L.rhg_synth_lte.argtypes = [C.c_void_p, C.POINTER(RhgColumns), C.c_int32, C.c_int32, PD]
         L.rhg_synth_lte_noalloc.argtypes = [C.c_void_p, C.POINTER(RhgColumns), C.c_int32, PD]
+        PI = C.POINTER(C.c_int32)
+        L.rhg_set_grids.argtypes = [C.c_void_p, C.c_int32, PI, PI]
+        L.rhg_synth_lte_grids.argtypes = [C.c_void_p, C.POINTER(RhgColumns), C.c_int32,
+                                          C.c_int32, PI, PD]
         L.rhg_synth_lte_diag.argtypes = [C.c_void_p, C.POINTER(RhgColumns), C.c_int32, PD, PD, PD]
         L.rhg_last_timings.argtypes = [C.c_void_p, PD]
         L.rhg_set_tuning.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
@@ -108,6 +112,20 @@ def _check(rc):
 
 def _pd(a):
     return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _pi(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def grid_table(positions):
+    """(grid_ptr [ngrid + 1], grid_out) int32 of rhg_set_grids from a list of
+    per-grid output positions."""
+    parts = [np.asarray(p, dtype=np.int64).ravel() for p in positions]
+    ptr = np.zeros(len(parts) + 1, np.int32)
+    ptr[1:] = np.cumsum([p.size for p in parts])
+    flat = np.concatenate(parts).astype(np.int32) if parts else np.zeros(0, np.int32)
+    return ptr, np.ascontiguousarray(flat)
 
 
 class Engine:
@@ -159,6 +177,39 @@ class Engine:
         s.atm_scale, s.mu = int(atm_scale), float(mu)
         _check(lib().rhg_synth_lte(self._h, C.byref(s), int(precision), 1,
                                    C.cast(C.c_void_p(d_out), C.POINTER(C.c_double))))
+
+    def set_grids(self, positions):
+        """Wavelength grids of this context (rhg_set_grids): positions[g] are
+        grid g's positions on the model's output wavelengths, increasing."""
+        ptr, flat = grid_table(positions)
+        _check(lib().rhg_set_grids(self._h, len(positions), _pi(ptr), _pi(flat)))
+        self.grids = [np.asarray(p, np.int64).ravel().copy() for p in positions]
+
+    def synth_lte_grids(self, atmos, col_grid, mu=1.0, atm_scale=0, ndep=None, precision=0):
+        """synth_lte for columns on different grids of set_grids (column i on
+        grid col_grid[i]), in one launch per stage: Stokes [ncol, 4, nout] on
+        the model's output wavelengths, zero outside each column's grid."""
+        cs = ColumnsStruct(atmos, ndep, atm_scale, mu)
+        ncol = cs.struct.ncol
+        g = np.ascontiguousarray(col_grid, dtype=np.int32)
+        if g.shape != (ncol,):
+            raise ValueError("col_grid must hold one grid id per column")
+        out = np.empty((ncol, 4, self.nout))
+        _check(lib().rhg_synth_lte_grids(self._h, cs.ptr(), int(precision), 0, _pi(g), _pd(out)))
+        return out
+
+    def synth_lte_grids_device(self, d_atmos, d_ndep, d_grid, ncol, ndep_max, d_out, mu=1.0,
+                               atm_scale=0, precision=0):
+        """Same, with device pointers (ints) for atmosphere, depth counts, grid
+        ids and output; no host copies."""
+        s = RhgColumns()
+        s.ncol, s.ndep_max = int(ncol), int(ndep_max)
+        s.ndep = C.cast(C.c_void_p(d_ndep), C.POINTER(C.c_int32))
+        s.atmos = C.cast(C.c_void_p(d_atmos), C.POINTER(C.c_double))
+        s.atm_scale, s.mu = int(atm_scale), float(mu)
+        _check(lib().rhg_synth_lte_grids(self._h, C.byref(s), int(precision), 1,
+                                         C.cast(C.c_void_p(d_grid), C.POINTER(C.c_int32)),
+                                         C.cast(C.c_void_p(d_out), C.POINTER(C.c_double))))
 
     def rf_lte(self, atmos, mu=1.0, atm_scale=0, ndep=None, precision=0, params=None):
         """Stokes [ncol, 4, nout] and response functions

@@ -213,3 +213,30 @@ def test_autotune_picks_fastest_and_checks_bit_identity(monkeypatch):
     bad = _FakeTuneEngine(torch, out, bad={"line_table": 0, "column_sort": 1, "wave_order": 0})
     with pytest.raises(RuntimeError, match="changed the Stokes vectors"):
         autotune.tune(bad, atm, ndep, out, reps=1)
+
+
+def test_grid_positions_and_table():
+    """The wavelength grids of one merged context: each grid's outputs are
+    located on the merged model's output wavelengths (what compute1d with
+    that wave returns: sorted, duplicates and LAMBDA_REF dropped), and the
+    (grid_ptr, grid_out) table of rhg_set_grids concatenates them."""
+    from pyrh_amd import api
+    from pyrh_amd.engine import grid_table
+    run = model.open_run(str(GOLDEN / "run_lte"))
+    wl = workloads.wavelengths_630()
+    lam_ref = run.kw.float("LAMBDA_REF")
+    waves = [wl[::-1][:40], wl[30:], np.concatenate([wl[::9], wl[::9], [lam_ref]]), wl[:0]]
+    merged = np.unique(np.concatenate(waves))
+    m = model.build_model(run, merged)
+    pos = api.grid_positions(m, waves)
+    out_lam = m.lam[m.out_index]
+    for w, p in zip(waves, pos):
+        single = model.build_model(run, w) if w.size else None
+        want = single.lam[single.out_index] if single is not None else np.zeros(0)
+        np.testing.assert_array_equal(out_lam[p], want)
+        assert np.all(np.diff(p) > 0)
+    ptr, flat = grid_table(pos)
+    assert ptr.tolist() == [0, 40, 40 + 82, 40 + 82 + pos[2].size, flat.size]
+    np.testing.assert_array_equal(flat[ptr[1]:ptr[2]], pos[1])
+    with pytest.raises(ValueError, match="not on the merged grid"):
+        api.grid_positions(m, [np.array([wl[0] + 1e-4])])
