@@ -9,7 +9,9 @@ that grid alone gives for it (the per-wavelength work is the same code on
 the same inputs), in the FP64 path and the FP32 form; entries outside a
 column's grid are zero.  Batches: the mixed-size run (400 Fe I Zeeman
 lines, 2000 wavelengths, 64-256 depths) with overlapping, interleaved,
-disjoint and empty grids, and the 630 nm run through pyrh_amd.api."""
+disjoint and empty grids, and through pyrh_amd.api the 630 nm runs whose
+wavelengths go to every solver (DELO-Bezier with and without MO terms,
+scalar Bezier, Feautrier)."""
 import numpy as np
 import pytest
 
@@ -147,3 +149,34 @@ def test_api_synth_grids_matches_synth_batch():
         np.testing.assert_array_equal(lam[pos[g]], r[4])
         for a, b in zip((I, Q, U, V), r[:4]):
             np.testing.assert_array_equal(a[cols][:, pos[g]], b)
+
+
+@pytest.mark.parametrize("run", ("run_lte", "run_lte_field_free", "run_lte_field_free_static",
+                                 "run_lte_mo"))
+@pytest.mark.parametrize("precision", ("fp64", "fp32"))
+def test_api_grids_every_solver(run, precision):
+    """Grids whose wavelengths go to every solver: in the FULL_STOKES run,
+    one grid mixes the Fe I lines' DELO-Bezier wavelengths with continuum
+    points near 600 nm (Feautrier) and one holds only such points; the
+    field-free runs solve the scalar Bezier (BEZIER_ANGLE wavelengths); the
+    magneto-optical run the DELO-Bezier with MO terms.  Each column on
+    its grid against synth_batch on that grid, bit for bit."""
+    from pyrh_amd import api
+    cwd = str(GOLDEN / run)
+    wl = W.wavelengths_630()
+    extra = np.linspace(600.0, 601.0, 40)
+    if run == "run_lte":
+        waves = [np.concatenate([wl[:56], extra[:20]]), wl[56:], extra]
+    else:
+        waves = [wl[:70], wl[50:], wl[::3]]
+    atm = W.stokes_map(192, seed=W.SEED_BASE + 45)
+    atm[::2, 3] = 0.0    # half the columns static
+    gid = (np.arange(atm.shape[0]) * 5 % len(waves)).astype(np.int32)
+    I, Q, U, V, lam, pos = api.synth_grids(cwd, 0.8, 0, atm, waves, gid, precision=precision)
+    for g, w in enumerate(waves):
+        cols = np.flatnonzero(gid == g)
+        r = api.synth_batch(cwd, 0.8, 0, atm[cols], w, precision=precision)
+        np.testing.assert_array_equal(lam[pos[g]], r[4])
+        for a, b in zip((I, Q, U, V), r[:4]):
+            np.testing.assert_array_equal(a[cols][:, pos[g]], b)
+    print(f"{run} {precision}: grids {[p.size for p in pos]} equal", flush=True)
