@@ -125,10 +125,11 @@ struct rhg_context {
   // wavelength grids (rhg_set_grids): per grid its polarized and its scalar
   // output positions (the lists of all grids concatenated, with offsets
   // [ngrid + 1]) and its range of bound-free tiles [2 ngrid]; the longest
-  // list of each kind; for a grid batch, the block -> grid table
+  // list of each kind; per grid a bit mask of its output positions
+  // (gr_words 32-bit words each); for a grid batch, the block -> grid table
   // (and a grid batch's host grid ids copied in)
-  int ngrid = 0, gr_max_pol = 0, gr_max_scal = 0;
-  DevBuf gr_pol, gr_pol_ptr, gr_scal, gr_scal_ptr, gr_tile, gr_blk, gr_col;
+  int ngrid = 0, gr_max_pol = 0, gr_max_scal = 0, gr_words = 0;
+  DevBuf gr_pol, gr_pol_ptr, gr_scal, gr_scal_ptr, gr_tile, gr_mask, gr_blk, gr_col;
   double nlte_ms[RHG_NLTE_NTIMING]{};
   // hydrostatic equilibrium: element tables, inputs, outputs, timing
   DevBuf hs_nstage, hs_ptr, hs_logpf, hs_ionpot, hs_abund, hs_in, hs_ndep, hs_out, hs_niter,
@@ -464,8 +465,8 @@ static int launch_state(rhg_context_t *c, const double *d_atm, const int32_t *d_
 // per-grid wavelength lists (rhg_set_grids).
 struct GridLaunch {
   const int32_t *blk_grid;
-  const int32_t *pol, *pol_ptr, *scal, *scal_ptr, *tile;
-  int max_pol, max_scal;
+  const int32_t *pol, *pol_ptr, *scal, *scal_ptr, *tile, *mask;
+  int max_pol, max_scal, mask_words;
 };
 
 // Stages 1-4 of a synthesis (k_state, k_geometry, k_continuum, the formal
@@ -499,12 +500,16 @@ static int launch_synth(rhg_context_t *c, const double *d_atm, const int32_t *d_
   {
     dim3 blk(rhg::CONT_BLOCK), grd((unsigned) ((npts + rhg::CONT_BLOCK - 1) / rhg::CONT_BLOCK));
     const int32_t *bg = gl ? gl->blk_grid : nullptr, *gt = gl ? gl->tile : nullptr;
+    const int32_t *gm = gl ? gl->mask : nullptr;
+    const int gw = gl ? gl->mask_words : 0;
     if (precision == RHG_PREC_FP32)
       hipLaunchKernelGGL(rhg::k_continuum<float>, grd, blk, 0, c->stream, c->d_model, d_ndep, ncol, Nmax,
-                         state, nfield, c->d_out_ls, nout, (float *) cchi, (float *) ceta, bg, gt);
+                         state, nfield, c->d_out_ls, nout, (float *) cchi, (float *) ceta, bg, gt,
+                         gm, gw);
     else
       hipLaunchKernelGGL(rhg::k_continuum<double>, grd, blk, 0, c->stream, c->d_model, d_ndep, ncol, Nmax,
-                         state, nfield, c->d_out_ls, nout, (double *) cchi, (double *) ceta, bg, gt);
+                         state, nfield, c->d_out_ls, nout, (double *) cchi, (double *) ceta, bg, gt,
+                         gm, gw);
     HIPCHECK(hipGetLastError());
   }
   HIPCHECK(hipEventRecord(c->ev[3], c->stream));
@@ -936,7 +941,7 @@ int synth(rhg_context_t *c, const rhg_columns_t *cols, int32_t precision, int32_
     gl = GridLaunch{(const int32_t *) c->gr_blk.p, (const int32_t *) c->gr_pol.p,
                     (const int32_t *) c->gr_pol_ptr.p, (const int32_t *) c->gr_scal.p,
                     (const int32_t *) c->gr_scal_ptr.p, (const int32_t *) c->gr_tile.p,
-                    c->gr_max_pol, c->gr_max_scal};
+                    (const int32_t *) c->gr_mask.p, c->gr_max_pol, c->gr_max_scal, c->gr_words};
     HIPCHECK(hipMemsetAsync(d_out, 0, out_b, c->stream));
   } else if (int rc = prepare_ndep(c, d_ndep, ncol, Nmax, sort, &sndep, &perm, &flag)) {
     return rc;
@@ -1089,6 +1094,8 @@ int rhg_set_grids(rhg_context_t *c, int32_t ngrid, const int32_t *grid_ptr,
   std::vector<char> is_pol(nout, 0);
   for (int io : c->h_pol) is_pol[io] = 1;
   std::vector<int32_t> pol, scal, pol_ptr{0}, scal_ptr{0}, tile;
+  const int words = (nout + 31) / 32;
+  std::vector<int32_t> mask((size_t) ngrid * words, 0);
   int max_pol = 0, max_scal = 0;
   for (int g = 0; g < ngrid; g++) {
     if (grid_ptr[g + 1] < grid_ptr[g]) return fail(RHG_EINVAL, "grid_ptr must not decrease");
@@ -1099,6 +1106,7 @@ int rhg_set_grids(rhg_context_t *c, int32_t ngrid, const int32_t *grid_ptr,
       if (e > grid_ptr[g] && io <= grid_out[e - 1])
         return fail(RHG_EINVAL, "a grid's wavelength positions must increase");
       (is_pol[io] ? pol : scal).push_back(io);
+      mask[(size_t) g * words + io / 32] |= (int32_t) (1u << (io % 32));
       lo = std::min(lo, io);
       hi = std::max(hi, io);
     }
@@ -1122,7 +1130,9 @@ int rhg_set_grids(rhg_context_t *c, int32_t ngrid, const int32_t *grid_ptr,
   if (int rc = put(c->gr_scal, scal)) return rc;
   if (int rc = put(c->gr_scal_ptr, scal_ptr)) return rc;
   if (int rc = put(c->gr_tile, tile)) return rc;
+  if (int rc = put(c->gr_mask, mask)) return rc;
   c->ngrid = ngrid;
+  c->gr_words = words;
   c->gr_max_pol = max_pol;
   c->gr_max_scal = max_scal;
   g_err.clear();
@@ -1275,7 +1285,7 @@ int launch_base(rhg_context_t *c, const double *d_atm, const int32_t *d_ndep, in
   if (cchi) {
     dim3 blk(rhg::CONT_BLOCK), grd((unsigned) ((npts + rhg::CONT_BLOCK - 1) / rhg::CONT_BLOCK));
     hipLaunchKernelGGL(rhg::k_continuum<R>, grd, blk, 0, c->stream, c->d_model, d_ndep, ncol, Nmax,
-                       state, nfield, c->d_out_ls, c->nout, cchi, ceta, (const int32_t *) nullptr, (const int32_t *) nullptr);
+                       state, nfield, c->d_out_ls, c->nout, cchi, ceta, (const int32_t *) nullptr, (const int32_t *) nullptr, (const int32_t *) nullptr, 0);
     HIPCHECK(hipGetLastError());
   }
   return 0;
@@ -1955,7 +1965,7 @@ int rhg_nlte_solve(rhg_context_t *c, const rhg_nlte_t *a, const rhg_columns_t *c
       hipLaunchKernelGGL(rhg::k_continuum<double>,
                          dim3((unsigned) ((npts + rhg::CONT_BLOCK - 1) / rhg::CONT_BLOCK)),
                          dim3(rhg::CONT_BLOCK), 0, c->stream, c->d_model, d_ndep, ncol, Nmax,
-                         B.state, nfield, c->d_out_ls, nout, B.cchi, B.ceta, (const int32_t *) nullptr, (const int32_t *) nullptr);
+                         B.state, nfield, c->d_out_ls, nout, B.cchi, B.ceta, (const int32_t *) nullptr, (const int32_t *) nullptr, (const int32_t *) nullptr, 0);
       HIPCHECK(hipGetLastError());
       hipLaunchKernelGGL(rhg::k_nlte_bg, gpts, b64, 0, c->stream, c->d_model, d_at, B,
                          c->d_out_ls);

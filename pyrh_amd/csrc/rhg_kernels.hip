@@ -519,13 +519,15 @@ __global__ void k_scales(const DevModel *__restrict__ Mp, const int32_t *__restr
 // grids (blk_grid != nullptr, ncol a multiple of 64) the columns of block
 // ic / 64 belong to grid g = blk_grid[ic / 64] (-1: none) and take the
 // bound-free tiles gtile[2 g] .. gtile[2 g + 1] - 1 only, the tiles that hold
-// the grid's wavelengths.
+// the grid's wavelengths, and in them the wavelengths io whose bit is set in
+// the grid's mask, bit io % 32 of gmask[g * gwords + io / 32].
 template <typename R>
 __global__ void __launch_bounds__(CONT_BLOCK)
 k_continuum(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep, int ncol, int Nmax,
             const double *__restrict__ state, int nfield, const int32_t *__restrict__ out_ls,
             int nout, R *__restrict__ cchi, R *__restrict__ ceta,
-            const int32_t *__restrict__ blk_grid, const int32_t *__restrict__ gtile) {
+            const int32_t *__restrict__ blk_grid, const int32_t *__restrict__ gtile,
+            const int32_t *__restrict__ gmask, int gwords) {
   const DevModel &M = *Mp;
   __shared__ double bf_lds[BF_TILE][CONT_BLOCK];
   const int64_t gid = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
@@ -533,6 +535,7 @@ k_continuum(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep, i
   const int k = (int) (gid / ncol), ic = (int) (gid % ncol);
   if (k >= ndep[ic]) return;
   int tile0 = 0, tile1 = M.ntile;
+  const int32_t *gm = nullptr;
   if (blk_grid) {
     // a wavefront is 64 consecutive slots of one depth (ncol is a multiple
     // of 64), one block: its grid and tile range are wave-uniform, and kept
@@ -541,6 +544,7 @@ k_continuum(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep, i
     if (g < 0) return;
     tile0 = __builtin_amdgcn_readfirstlane(gtile[2 * g]);
     tile1 = __builtin_amdgcn_readfirstlane(gtile[2 * g + 1]);
+    gm = gmask + (int64_t) g * gwords;
   }
   const int64_t fs = (int64_t) Nmax * ncol;
   const double *st = state + (int64_t) k * ncol + ic;
@@ -591,6 +595,7 @@ k_continuum(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep, i
 #pragma unroll 1
     for (int j = 0; j < nj; j++) {
       const int io = io0 + j;
+      if (gm && !(((uint32_t) ldc(gm)[io >> 5] >> (io & 31)) & 1u)) continue;
       const double bfj = bf_lds[j][threadIdx.x];
       const CDouble *lc = ldc(M.lamconst) + (int64_t) ldc(out_ls)[io] * RHG_LC_NCONST;
       const double hc_kla = lc[RHG_LC_HC_KLA], twohnu3 = lc[RHG_LC_TWOHNU3_C2];
@@ -683,10 +688,10 @@ RHG_KSTATE(8)
 
 template __global__ void k_continuum<double>(const DevModel *, const int32_t *, int, int, const double *,
                                              int, const int32_t *, int, double *, double *,
-                                             const int32_t *, const int32_t *);
+                                             const int32_t *, const int32_t *, const int32_t *, int);
 template __global__ void k_continuum<float>(const DevModel *, const int32_t *, int, int, const double *,
                                             int, const int32_t *, int, float *, float *,
-                                            const int32_t *, const int32_t *);
+                                            const int32_t *, const int32_t *, const int32_t *, int);
 
 // ---------------------------------------------------------------------------
 // Column reordering.  perm[i] is the caller's column at sorted position i.

@@ -476,7 +476,8 @@ __global__ void k_scales(const DevModel *Mp, const int32_t *ndep, int ncol, int 
 template <typename R>
 __global__ void k_continuum(const DevModel *M, const int32_t *ndep, int ncol, int Nmax,
                             const double *state, int nfield, const int32_t *out_ls, int nout,
-                            R *cchi, R *ceta, const int32_t *blk_grid, const int32_t *gtile);
+                            R *cchi, R *ceta, const int32_t *blk_grid, const int32_t *gtile,
+                            const int32_t *gmask, int gwords);
 template <typename R, bool MO, bool LT>
 __global__ void k_formal_stokes(const DevModel *M, const int32_t *ndep, const int32_t *moving,
                                 int ncol, int Nmax, const double *state, const int32_t *out_ls,
