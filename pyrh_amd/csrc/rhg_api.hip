@@ -502,14 +502,15 @@ static int launch_synth(rhg_context_t *c, const double *d_atm, const int32_t *d_
     const int32_t *bg = gl ? gl->blk_grid : nullptr, *gt = gl ? gl->tile : nullptr;
     const int32_t *gm = gl ? gl->mask : nullptr;
     const int gw = gl ? gl->mask_words : 0;
-    if (precision == RHG_PREC_FP32)
-      hipLaunchKernelGGL(rhg::k_continuum<float>, grd, blk, 0, c->stream, c->d_model, d_ndep, ncol, Nmax,
-                         state, nfield, c->d_out_ls, nout, (float *) cchi, (float *) ceta, bg, gt,
-                         gm, gw);
-    else
-      hipLaunchKernelGGL(rhg::k_continuum<double>, grd, blk, 0, c->stream, c->d_model, d_ndep, ncol, Nmax,
-                         state, nfield, c->d_out_ls, nout, (double *) cchi, (double *) ceta, bg, gt,
-                         gm, gw);
+#define RHG_LAUNCH_CONT(R, G)                                                                    \
+  hipLaunchKernelGGL((rhg::k_continuum<R, G>), grd, blk, 0, c->stream, c->d_model, d_ndep, ncol, Nmax, \
+                     state, nfield, c->d_out_ls, nout, (R *) cchi, (R *) ceta, bg, gt, gm, gw)
+    if (precision == RHG_PREC_FP32) {
+      if (gl) RHG_LAUNCH_CONT(float, true); else RHG_LAUNCH_CONT(float, false);
+    } else {
+      if (gl) RHG_LAUNCH_CONT(double, true); else RHG_LAUNCH_CONT(double, false);
+    }
+#undef RHG_LAUNCH_CONT
     HIPCHECK(hipGetLastError());
   }
   HIPCHECK(hipEventRecord(c->ev[3], c->stream));
@@ -1284,7 +1285,7 @@ int launch_base(rhg_context_t *c, const double *d_atm, const int32_t *d_ndep, in
   }
   if (cchi) {
     dim3 blk(rhg::CONT_BLOCK), grd((unsigned) ((npts + rhg::CONT_BLOCK - 1) / rhg::CONT_BLOCK));
-    hipLaunchKernelGGL(rhg::k_continuum<R>, grd, blk, 0, c->stream, c->d_model, d_ndep, ncol, Nmax,
+    hipLaunchKernelGGL((rhg::k_continuum<R, false>), grd, blk, 0, c->stream, c->d_model, d_ndep, ncol, Nmax,
                        state, nfield, c->d_out_ls, c->nout, cchi, ceta, (const int32_t *) nullptr, (const int32_t *) nullptr, (const int32_t *) nullptr, 0);
     HIPCHECK(hipGetLastError());
   }
@@ -1962,7 +1963,7 @@ int rhg_nlte_solve(rhg_context_t *c, const rhg_nlte_t *a, const rhg_columns_t *c
                          c->d_model, d_ndep, ncol, Nmax, cols->atm_scale, cols->mu, B.state, nfield,
                          (int32_t *) c->moving.p, (double *) nullptr);
       HIPCHECK(hipGetLastError());
-      hipLaunchKernelGGL(rhg::k_continuum<double>,
+      hipLaunchKernelGGL((rhg::k_continuum<double, false>),
                          dim3((unsigned) ((npts + rhg::CONT_BLOCK - 1) / rhg::CONT_BLOCK)),
                          dim3(rhg::CONT_BLOCK), 0, c->stream, c->d_model, d_ndep, ncol, Nmax,
                          B.state, nfield, c->d_out_ls, nout, B.cchi, B.ceta, (const int32_t *) nullptr, (const int32_t *) nullptr, (const int32_t *) nullptr, 0);

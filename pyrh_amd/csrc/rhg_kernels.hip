@@ -520,8 +520,9 @@ __global__ void k_scales(const DevModel *__restrict__ Mp, const int32_t *__restr
 // ic / 64 belong to grid g = blk_grid[ic / 64] (-1: none) and take the
 // bound-free tiles gtile[2 g] .. gtile[2 g + 1] - 1 only, the tiles that hold
 // the grid's wavelengths, and in them the wavelengths io whose bit is set in
-// the grid's mask, bit io % 32 of gmask[g * gwords + io / 32].
-template <typename R>
+// the grid's mask, bit io % 32 of gmask[g * gwords + io / 32] (the GRID
+// instantiation; the other ignores the three grid arguments).
+template <typename R, bool GRID>
 __global__ void __launch_bounds__(CONT_BLOCK)
 k_continuum(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep, int ncol, int Nmax,
             const double *__restrict__ state, int nfield, const int32_t *__restrict__ out_ls,
@@ -536,7 +537,7 @@ k_continuum(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep, i
   if (k >= ndep[ic]) return;
   int tile0 = 0, tile1 = M.ntile;
   const int32_t *gm = nullptr;
-  if (blk_grid) {
+  if constexpr (GRID) {
     // a wavefront is 64 consecutive slots of one depth (ncol is a multiple
     // of 64), one block: its grid and tile range are wave-uniform, and kept
     // in scalar registers so that the tile tables stay scalar loads
@@ -595,7 +596,8 @@ k_continuum(const DevModel *__restrict__ Mp, const int32_t *__restrict__ ndep, i
 #pragma unroll 1
     for (int j = 0; j < nj; j++) {
       const int io = io0 + j;
-      if (gm && !(((uint32_t) ldc(gm)[io >> 5] >> (io & 31)) & 1u)) continue;
+      if constexpr (GRID)
+        if (!(((uint32_t) ldc(gm)[io >> 5] >> (io & 31)) & 1u)) continue;
       const double bfj = bf_lds[j][threadIdx.x];
       const CDouble *lc = ldc(M.lamconst) + (int64_t) ldc(out_ls)[io] * RHG_LC_NCONST;
       const double hc_kla = lc[RHG_LC_HC_KLA], twohnu3 = lc[RHG_LC_TWOHNU3_C2];
@@ -686,12 +688,15 @@ RHG_KSTATE(7)
 RHG_KSTATE(8)
 #undef RHG_KSTATE
 
-template __global__ void k_continuum<double>(const DevModel *, const int32_t *, int, int, const double *,
-                                             int, const int32_t *, int, double *, double *,
+#define RHG_KCONT(R, G)                                                                       \
+  template __global__ void k_continuum<R, G>(const DevModel *, const int32_t *, int, int,      \
+                                             const double *, int, const int32_t *, int, R *, R *, \
                                              const int32_t *, const int32_t *, const int32_t *, int);
-template __global__ void k_continuum<float>(const DevModel *, const int32_t *, int, int, const double *,
-                                            int, const int32_t *, int, float *, float *,
-                                            const int32_t *, const int32_t *, const int32_t *, int);
+RHG_KCONT(double, false)
+RHG_KCONT(double, true)
+RHG_KCONT(float, false)
+RHG_KCONT(float, true)
+#undef RHG_KCONT
 
 // ---------------------------------------------------------------------------
 // Column reordering.  perm[i] is the caller's column at sorted position i.

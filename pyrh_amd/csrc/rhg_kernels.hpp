@@ -473,7 +473,7 @@ __global__ void k_geometry(const DevModel *M, const int32_t *ndep, int ncol, int
 __global__ void k_scales(const DevModel *Mp, const int32_t *ndep, int ncol, int Nmax, int atm_scale,
                          const double *state, int nfield, double *tau, double *height,
                          double *cmass);
-template <typename R>
+template <typename R, bool GRID>
 __global__ void k_continuum(const DevModel *M, const int32_t *ndep, int ncol, int Nmax,
                             const double *state, int nfield, const int32_t *out_ls, int nout,
                             R *cchi, R *ceta, const int32_t *blk_grid, const int32_t *gtile,
